@@ -1,0 +1,1786 @@
+// CPU restatement of geomechanics/mpm's explicit MPM step — TEST INFRASTRUCTURE.
+//
+// This file is the oracle: only tests/, __graft_entry__.smoke() and bench.py's
+// cpu_baseline leg load it, and only as the checker. It is never part of the
+// product path (mpm_amd/), which fails loudly without its HIP library.
+//
+// Every routine follows the reference statement by statement, in the
+// reference's evaluation order, and cites the file:line it restates (paths
+// relative to the reference root). Built single-threaded it sums every nodal
+// quantity in particle-vector order, which is the one order that defines a
+// unique answer for the reference (its OpenMP run is thread-order
+// nondeterministic: include/nodes/node.tcc:56-65 spin-locks). Built with
+// -DORC_OMP -fopenmp it parallelises the particle and node sweeps exactly as
+// include/mesh/mesh.tcc:70-103,1156-1161 do, with atomic nodal updates
+// standing in for the reference's SpinMutex; that build is the timed
+// "reference CPU path".
+//
+// Where the reference's floating-point association lives inside Eigen
+// expression templates (products, norms), this file fixes one order (the
+// sequential k-order of a non-vectorised product); those functions are pinned
+// by the reference's known-answer tests only to their 1e-7 tolerance. Their
+// list is kept with the parity notes of this project.
+//
+// Build: see oracle/Makefile. Compile with -ffp-contract=off.
+
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <map>
+#include <set>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#ifdef ORC_OMP
+#include <omp.h>
+#define ORC_PRAGMA(x) _Pragma(#x)
+#else
+#define ORC_PRAGMA(x)
+#endif
+
+namespace orc {
+
+typedef int64_t Index;
+static const double kEps = std::numeric_limits<double>::epsilon();
+static const Index kNoCell = -1;
+
+// ---------------------------------------------------------------------------
+// Materials. Parameter vector layout (shared with the product's C-ABI):
+// [0] type (0 LinearElastic, 1 MohrCoulomb) [1] density [2] youngs_modulus
+// [3] poisson_ratio [4] friction (deg) [5] dilation (deg) [6] cohesion
+// [7] residual_friction (deg) [8] residual_dilation (deg) [9] residual_cohesion
+// [10] peak_pdstrain [11] residual_pdstrain [12] tension_cutoff [13] softening
+// [14] packing_fraction_minimum [15] initial packing fraction (<=0 -> 1)
+// ---------------------------------------------------------------------------
+enum { kLE = 0, kMC = 1 };
+static const int kMatParams = 16;
+// MC state variables, reference order (mohr_coulomb.tcc:116-118)
+enum { SV_YIELD = 0, SV_PHI, SV_PSI, SV_COH, SV_TCUT, SV_EPS, SV_RHO, SV_THETA, SV_PDS };
+static const int kNsv = 9;
+
+struct Material {
+  int type = kLE;
+  double density = 0, E = 0, nu = 0;
+  // LinearElastic (linear_elastic.tcc:20-44)
+  double bulk = 0;
+  double de[6][6];
+  // MohrCoulomb (mohr_coulomb.tcc:9-77)
+  double grain_density = 0, min_packing = 0;
+  bool softening = false;
+  double phi_peak = 0, psi_peak = 0, c_peak = 0;
+  double phi_res = 0, psi_res = 0, c_res = 0;
+  double pd_peak = 0, pd_res = 0, tcut = 0;
+  double shear = 0;
+};
+
+static inline double check_low(double val) {  // mohr_coulomb.h:124-126
+  return (val > 1.0e-15 ? val : 1.0e-15);
+}
+
+static void material_init(Material& m, const double* p) {
+  m.type = (int)p[0];
+  m.density = p[1];
+  m.E = p[2];
+  m.nu = p[3];
+  if (m.type == kLE) {
+    // linear_elastic.tcc:26-27, 53-69
+    m.bulk = m.E / (3.0 * (1. - 2. * m.nu));
+    const double G = m.E / (2.0 * (1. + m.nu));
+    const double a1 = m.bulk + (4.0 / 3.0) * G;
+    const double a2 = m.bulk - (2.0 / 3.0) * G;
+    std::memset(m.de, 0, sizeof(m.de));
+    m.de[0][0] = a1; m.de[0][1] = a2; m.de[0][2] = a2;
+    m.de[1][0] = a2; m.de[1][1] = a1; m.de[1][2] = a2;
+    m.de[2][0] = a2; m.de[2][1] = a2; m.de[2][2] = a1;
+    m.de[3][3] = G; m.de[4][4] = G; m.de[5][5] = G;
+  } else {
+    // mohr_coulomb.tcc:9-77
+    double initial_packing_fraction = p[15] > 0. ? p[15] : 1.;
+    m.grain_density = m.density / initial_packing_fraction;
+    m.min_packing = p[14];
+    m.softening = p[13] != 0.;
+    m.phi_peak = p[4] * M_PI / 180.;
+    m.psi_peak = p[5] * M_PI / 180.;
+    m.c_peak = p[6];
+    m.phi_res = p[7] * M_PI / 180.;
+    m.psi_res = p[8] * M_PI / 180.;
+    m.c_res = p[9];
+    m.pd_peak = p[10];
+    m.pd_res = p[11];
+    m.tcut = p[12];
+    m.bulk = m.E / (3.0 * (1. - 2. * m.nu));
+    m.shear = m.E / (2.0 * (1 + m.nu));
+    // compute_elastic_tensor (mohr_coulomb.tcc:517-533)
+    const double G = m.shear;
+    const double a1 = m.bulk + (4.0 / 3.0) * G;
+    const double a2 = m.bulk - (2.0 / 3.0) * G;
+    std::memset(m.de, 0, sizeof(m.de));
+    m.de[0][0] = a1; m.de[0][1] = a2; m.de[0][2] = a2;
+    m.de[1][0] = a2; m.de[1][1] = a1; m.de[1][2] = a2;
+    m.de[2][0] = a2; m.de[2][1] = a2; m.de[2][2] = a1;
+    m.de[3][3] = G; m.de[4][4] = G; m.de[5][5] = G;
+  }
+}
+
+// mohr_coulomb.tcc:85-111
+static void mc_initialise_state(const Material& m, double* sv) {
+  sv[SV_YIELD] = 0;
+  sv[SV_PHI] = m.phi_peak;
+  sv[SV_PSI] = m.psi_peak;
+  sv[SV_COH] = m.c_peak;
+  sv[SV_TCUT] = check_low((m.tcut < m.c_peak / std::tan(m.phi_peak))
+                              ? m.tcut
+                              : m.c_peak / std::tan(m.phi_peak));
+  sv[SV_EPS] = 0.;
+  sv[SV_RHO] = 0.;
+  sv[SV_THETA] = 0.;
+  sv[SV_PDS] = 0.;
+}
+
+// D * x with the sequential k-order of Eigen's coefficient-based 6x6 product
+// (linear_elastic.tcc:81, infinitesimal_elasto_plastic.tcc:12).
+static inline void de_mul(const double de[6][6], const double* x, double* y) {
+  for (int i = 0; i < 6; ++i) {
+    double s = de[i][0] * x[0];
+    for (int k = 1; k < 6; ++k) s = s + de[i][k] * x[k];
+    y[i] = s;
+  }
+}
+
+// x^T * D (row vector), same k-order.
+static inline void de_tmul(const double de[6][6], const double* x, double* y) {
+  for (int j = 0; j < 6; ++j) {
+    double s = x[0] * de[0][j];
+    for (int k = 1; k < 6; ++k) s = s + x[k] * de[k][j];
+    y[j] = s;
+  }
+}
+
+static inline double dot6(const double* a, const double* b) {
+  double s = a[0] * b[0];
+  for (int k = 1; k < 6; ++k) s = s + a[k] * b[k];
+  return s;
+}
+
+// ----------------------------- material_utility.tcc -----------------------
+static inline double mu_p(const double* s) {  // :2-5
+  return (1. / 3. * (s[0] + s[1] + s[2]));
+}
+static inline void mu_dev(const double* s, double* d) {  // :8-19
+  const double p = mu_p(s);
+  for (int i = 0; i < 6; ++i) d[i] = s[i];
+  for (int i = 0; i < 3; ++i) d[i] -= p;
+}
+static inline double sq(double x) { return x * x; }  // std::pow(x, 2)
+static inline double mu_j2(const double* s) {  // :22-32
+  return (sq(s[0] - s[1]) + sq(s[1] - s[2]) + sq(s[0] - s[2])) / 6.0 +
+         sq(s[3]) + sq(s[4]) + sq(s[5]);
+}
+static inline double mu_j3(const double* s) {  // :35-51
+  double d[6];
+  mu_dev(s, d);
+  return (d[0] * d[1] * d[2]) - (d[2] * sq(d[3])) +
+         ((2 * d[3] * d[4] * d[5]) - (d[0] * sq(d[4])) - (d[1] * sq(d[5])));
+}
+static inline double mu_q(const double* s) {  // :54-61
+  const double j2 = mu_j2(s);
+  return (std::sqrt(3 * j2));
+}
+static inline double mu_lode_angle(const double* s, double tol = kEps) {  // :64-81
+  const double j2 = mu_j2(s);
+  const double j3 = mu_j3(s);
+  double v = 0.0;
+  if (std::abs(j2) > tol) v = (3. * std::sqrt(3.) / 2.) * (j3 / std::pow(j2, 1.5));
+  if (v > 1.0) v = 1.0;
+  if (v < -1.0) v = -1.0;
+  return (1. / 3.) * std::acos(v);
+}
+static inline void mu_dq_dsigma(const double* s, double* out) {  // :95-116
+  const double q = mu_q(s);
+  double d[6];
+  mu_dev(s, d);
+  for (int i = 0; i < 6; ++i) out[i] = 0.;
+  if (std::abs(q) > kEps) {
+    out[0] = 3. / (2. * q) * d[0];
+    out[1] = 3. / (2. * q) * d[1];
+    out[2] = 3. / (2. * q) * d[2];
+    out[3] = 3. / q * d[3];
+    out[4] = 3. / q * d[4];
+    out[5] = 3. / q * d[5];
+  }
+}
+static inline void mu_dtheta_dsigma(const double* s, double tol, double* out) {  // :167-204
+  const double j2 = mu_j2(s);
+  const double j3 = mu_j3(s);
+  double dj2[6];
+  mu_dev(s, dj2);  // dj2_dsigma :119-129
+  dj2[3] *= 2.0;
+  dj2[4] *= 2.0;
+  dj2[5] *= 2.0;
+  double dv[6];
+  mu_dev(s, dv);  // dj3_dsigma :132-164
+  const double d1[3] = {dv[0], dv[3], dv[5]};
+  const double d2[3] = {dv[3], dv[1], dv[4]};
+  const double d3[3] = {dv[5], dv[4], dv[2]};
+  auto dot3 = [](const double* a, const double* b) {
+    return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2];
+  };
+  double dj3[6];
+  dj3[0] = dot3(d1, d1) - (2. / 3.) * j2;
+  dj3[1] = dot3(d2, d2) - (2. / 3.) * j2;
+  dj3[2] = dot3(d3, d3) - (2. / 3.) * j2;
+  dj3[3] = 2.0 * dot3(d1, d2);
+  dj3[4] = 2.0 * dot3(d2, d3);
+  dj3[5] = 2.0 * dot3(d1, d3);
+  double dr_dj2 = -9.0 / 4.0 * std::sqrt(3.0) * j3;
+  double dr_dj3 = 3.0 / 2.0 * std::sqrt(3.0);
+  double dtheta_dr = -1.0 / 3.0;
+  if (std::abs(j2) > tol) {
+    double r = j3 / 2.0 * std::pow(j2 / 3.0, -1.5);
+    dr_dj2 *= std::pow(j2, -2.5);
+    dr_dj3 *= std::pow(j2, -1.5);
+    const double factor = (std::abs(1 - r * r) < tol) ? tol : std::abs(1 - r * r);
+    dtheta_dr = -1.0 / (3.0 * std::sqrt(factor));
+  }
+  for (int i = 0; i < 6; ++i)
+    out[i] = dtheta_dr * ((dr_dj2 * dj2[i]) + (dr_dj3 * dj3[i]));
+}
+
+// ------------------------------ Mohr-Coulomb --------------------------------
+enum { FS_ELASTIC = 0, FS_SHEAR = 1, FS_TENSILE = 2, FS_SEPARATED = 3 };
+
+// mohr_coulomb.tcc:124-134
+static inline void mc_invariants(const double* s, double* sv) {
+  sv[SV_EPS] = mu_p(s) * std::sqrt(3.);
+  sv[SV_THETA] = mu_lode_angle(s);
+  sv[SV_RHO] = std::sqrt(2. * mu_j2(s));
+}
+
+// mohr_coulomb.tcc:139-191
+static int mc_yield_state(double* yf, const double* sv) {
+  const double Tolerance = -1E-1;
+  const double epsilon = sv[SV_EPS];
+  const double rho = sv[SV_RHO];
+  const double theta = sv[SV_THETA];
+  const double phi = sv[SV_PHI];
+  const double cohesion = sv[SV_COH];
+  const double tension_cutoff = sv[SV_TCUT];
+  yf[0] = std::sqrt(2. / 3.) * std::cos(theta) * rho + epsilon / std::sqrt(3.) -
+          tension_cutoff;
+  yf[1] = std::sqrt(1.5) * rho *
+              ((std::sin(theta + M_PI / 3.) / (std::sqrt(3.) * std::cos(phi))) +
+               (std::cos(theta + M_PI / 3.) * std::tan(phi) / 3.)) +
+          (epsilon / std::sqrt(3.)) * std::tan(phi) - cohesion;
+  int yield_type = FS_ELASTIC;
+  if (yf[0] > Tolerance && yf[1] > Tolerance) {
+    const double n_phi = (1. + std::sin(phi)) / (1. - std::sin(phi));
+    const double sigma_p = tension_cutoff * n_phi - 2. * cohesion * std::sqrt(n_phi);
+    const double alpha_p = std::sqrt(1. + n_phi * n_phi) + n_phi;
+    const double h =
+        yf[0] + alpha_p * (std::sqrt(2. / 3.) * std::cos(theta - 4. * M_PI / 3.) * rho +
+                           epsilon / std::sqrt(3.) - sigma_p);
+    if (h > kEps)
+      yield_type = FS_TENSILE;
+    else
+      yield_type = FS_SHEAR;
+  }
+  if (yf[0] < Tolerance && yf[1] > Tolerance) yield_type = FS_SHEAR;
+  if (yf[0] > Tolerance && yf[1] < Tolerance) yield_type = FS_TENSILE;
+  return yield_type;
+}
+
+// mohr_coulomb.tcc:195-344
+static void mc_df_dp(const Material& m, int yield_type, const double* sv, const double* stress,
+                     double* df_dsigma, double* dp_dsigma, double* dp_dq, double* softening) {
+  const double rho = sv[SV_RHO];
+  const double theta = sv[SV_THETA];
+  const double phi = sv[SV_PHI];
+  const double psi = sv[SV_PSI];
+  const double tension_cutoff = sv[SV_TCUT];
+  const double pdstrain = sv[SV_PDS];
+  double df_depsilon, df_drho, df_dtheta;
+  if (yield_type == FS_TENSILE) {
+    df_depsilon = 1. / std::sqrt(3.);
+    df_drho = std::sqrt(2. / 3.) * std::cos(theta);
+    df_dtheta = -std::sqrt(2. / 3.) * rho * std::sin(theta);
+  } else {
+    df_depsilon = std::tan(phi) / std::sqrt(3.);
+    df_drho = std::sqrt(1.5) * ((std::sin(theta + M_PI / 3.) / (std::sqrt(3.) * std::cos(phi))) +
+                                (std::cos(theta + M_PI / 3.) * std::tan(phi) / 3.));
+    df_dtheta = std::sqrt(1.5) * rho *
+                ((std::cos(theta + M_PI / 3.) / (std::sqrt(3.) * std::cos(phi))) -
+                 (std::sin(theta + M_PI / 3.) * std::tan(phi) / 3.));
+  }
+  double depsilon_dsigma[6] = {0, 0, 0, 0, 0, 0};
+  for (int i = 0; i < 3; ++i) depsilon_dsigma[i] = 1. / 3. * std::sqrt(3.);
+  double drho_dsigma[6];
+  mu_dq_dsigma(stress, drho_dsigma);
+  for (int i = 0; i < 6; ++i) drho_dsigma[i] = drho_dsigma[i] * std::sqrt(2. / 3.);
+  double dtheta_dsigma[6];
+  mu_dtheta_dsigma(stress, kEps, dtheta_dsigma);
+  for (int i = 0; i < 6; ++i)
+    df_dsigma[i] = ((df_depsilon * depsilon_dsigma[i]) + (df_drho * drho_dsigma[i])) +
+                   (df_dtheta * dtheta_dsigma[i]);
+  if (yield_type == FS_TENSILE) {
+    const double et_value = 0.6;
+    const double xit = 0.1;
+    double sqpart = 4. * (1 - et_value * et_value) * std::cos(theta) * std::cos(theta) +
+                    5. * et_value * et_value - 4. * et_value;
+    if (sqpart < kEps) sqpart = 1.E-5;
+    double rt_den = 2. * (1 - et_value * et_value) * std::cos(theta) +
+                    (2. * et_value - 1) * std::sqrt(sqpart);
+    const double rt_num = 4. * (1 - et_value * et_value) * std::cos(theta) * std::cos(theta) +
+                          (2. * et_value - 1) * (2. * et_value - 1);
+    if (std::fabs(rt_den) < kEps) rt_den = 1.E-5;
+    const double rt = rt_num / (3. * rt_den);
+    const double dp_drt = 1.5 * rho * rho * rt /
+                          check_low(std::sqrt(xit * xit * tension_cutoff * tension_cutoff +
+                                              1.5 * rt * rt * rho * rho));
+    const double dp_drho = 1.5 * rho * rt * rt /
+                           check_low(std::sqrt(xit * xit * tension_cutoff * tension_cutoff +
+                                               1.5 * rt * rt * rho * rho));
+    const double dp_depsilon = 1. / std::sqrt(3.);
+    const double drtden_dtheta =
+        -2. * (1 - et_value * et_value) * std::sin(theta) -
+        (2. * et_value - 1) * 4. * (1 - et_value * et_value) * std::cos(theta) * std::sin(theta) /
+            std::sqrt(4. * (1 - et_value * et_value) * std::cos(theta) * std::cos(theta) +
+                      5. * et_value * et_value - 4. * et_value);
+    const double drtnum_dtheta = -8. * (1 - et_value * et_value) * std::cos(theta) * std::sin(theta);
+    const double drt_dtheta =
+        (drtnum_dtheta * rt_den - drtden_dtheta * rt_num) / (3. * rt_den * rt_den);
+    for (int i = 0; i < 6; ++i)
+      dp_dsigma[i] = ((dp_depsilon * depsilon_dsigma[i]) + (dp_drho * drho_dsigma[i])) +
+                     (dp_drt * drt_dtheta * dtheta_dsigma[i]);
+    *dp_dq = dp_drho * std::sqrt(2. / 3.);
+  } else {
+    const double r_mc = (3. - std::sin(phi)) / (6 * std::cos(phi));
+    double e_val = (3. - std::sin(phi)) / (3. + std::sin(phi));
+    if (e_val <= 0.5) e_val = 0.5 + 1.E-10;
+    if (e_val > 1.) e_val = 1.;
+    double sqpart = (4. * (1 - e_val * e_val) * sq(std::cos(theta))) + (5 * e_val * e_val) -
+                    (4. * e_val);
+    if (sqpart < kEps) sqpart = 1.E-5;
+    double mm = (2. * (1 - e_val * e_val) * std::cos(theta)) + ((2. * e_val - 1) * std::sqrt(sqpart));
+    if (std::fabs(mm) < kEps) mm = 1.E-5;
+    const double l = (4. * (1. - e_val * e_val) * sq(std::cos(theta))) + sq(2. * e_val - 1.);
+    const double r_mw = (l / mm) * r_mc;
+    const double xi = 0.1;
+    double omega = sq(xi * m.c_peak * std::tan(psi)) + sq(r_mw * std::sqrt(1.5) * rho);
+    if (omega < kEps) omega = 1.E-5;
+    const double dl_dtheta = -8. * (1. - e_val * e_val) * std::cos(theta) * std::sin(theta);
+    const double dm_dtheta = (-2. * (1. - e_val * e_val) * std::sin(theta)) +
+                             (0.5 * (2. * e_val - 1.) * dl_dtheta) / std::sqrt(sqpart);
+    const double drmw_dtheta = ((mm * dl_dtheta) - (l * dm_dtheta)) / (mm * mm);
+    const double dp_depsilon = std::tan(psi) / std::sqrt(3.);
+    const double dp_drho = 3. * rho * r_mw * r_mw / (2. * std::sqrt(omega));
+    const double dp_dtheta = (3. * rho * rho * r_mw * r_mc * drmw_dtheta) / (2. * std::sqrt(omega));
+    for (int i = 0; i < 6; ++i)
+      dp_dsigma[i] = ((dp_depsilon * depsilon_dsigma[i]) + (dp_drho * drho_dsigma[i])) +
+                     (dp_dtheta * dtheta_dsigma[i]);
+    *dp_dq = dp_drho * std::sqrt(2. / 3.);
+  }
+  double dphi_dpstrain = 0.;
+  double dc_dpstrain = 0.;
+  *softening = 0.;
+  if (m.softening && pdstrain > m.pd_peak && pdstrain < m.pd_res) {
+    dphi_dpstrain = (m.phi_res - m.phi_peak) / (m.pd_res - m.pd_peak);
+    dc_dpstrain = (m.c_res - m.c_peak) / (m.pd_res - m.pd_peak);
+    double df_dphi = std::sqrt(1.5) * rho *
+                         ((std::sin(phi) * std::sin(theta + M_PI / 3.) /
+                           (std::sqrt(3.) * std::cos(phi) * std::cos(phi))) +
+                          (std::cos(theta + M_PI / 3.) / (3. * std::cos(phi) * std::cos(phi)))) +
+                     (mu_p(stress) / (std::cos(phi) * std::cos(phi)));
+    double df_dc = -1.;
+    *softening = (-1.) * ((df_dphi * dphi_dpstrain) + (df_dc * dc_dpstrain)) * (*dp_dq);
+  }
+}
+
+// mohr_coulomb.tcc:348-513. `density` is the particle's current mass density.
+static void mc_compute_stress(const Material& m, const double* stress, const double* dstrain,
+                              double density, double* sv, double* out) {
+  const double current_packing_density = density;
+  const double critical_density = m.min_packing * m.grain_density;
+  const double pdstrain = sv[SV_PDS];
+  if (m.softening && pdstrain > m.pd_peak) {
+    if (pdstrain < m.pd_res) {
+      sv[SV_PHI] = m.phi_res + ((m.phi_peak - m.phi_res) * (pdstrain - m.pd_res) /
+                                (m.pd_peak - m.pd_res));
+      sv[SV_PSI] = m.psi_res + ((m.psi_peak - m.psi_res) * (pdstrain - m.pd_res) /
+                                (m.pd_peak - m.pd_res));
+      sv[SV_COH] = m.c_res + ((m.c_peak - m.c_res) * (pdstrain - m.pd_res) /
+                              (m.pd_peak - m.pd_res));
+    } else {
+      sv[SV_PHI] = m.phi_res;
+      sv[SV_PSI] = m.psi_res;
+      sv[SV_COH] = m.c_res;
+    }
+    const double apex = sv[SV_COH] / std::tan(sv[SV_PHI]);
+    if (sv[SV_TCUT] > apex) sv[SV_TCUT] = check_low(apex);
+  }
+  sv[SV_YIELD] = 0;
+  double trial[6];
+  {
+    double ds[6];
+    de_mul(m.de, dstrain, ds);
+    for (int i = 0; i < 6; ++i) trial[i] = stress[i] + ds[i];
+  }
+  if (current_packing_density <= critical_density) {
+    sv[SV_PDS] += mu_q(trial) / 3.0 / m.shear;
+    sv[SV_YIELD] = 3;
+    for (int i = 0; i < 6; ++i) out[i] = 0.;
+    return;
+  }
+  mc_invariants(trial, sv);
+  double yft[2];
+  int ytt = mc_yield_state(yft, sv);
+  if (ytt == FS_ELASTIC) {
+    sv[SV_YIELD] = 0;
+    for (int i = 0; i < 6; ++i) out[i] = trial[i];
+    return;
+  }
+  const double Tolerance = 1E-1;
+  double softening_trial = 0., dp_dq_trial = 0.;
+  double df_t[6] = {0}, dp_t[6] = {0};
+  mc_df_dp(m, ytt, sv, trial, df_t, dp_t, &dp_dq_trial, &softening_trial);
+  double yield_trial = 0.;
+  if (ytt == FS_TENSILE) {
+    sv[SV_YIELD] = 2;
+    yield_trial = yft[0];
+  }
+  if (ytt == FS_SHEAR) {
+    sv[SV_YIELD] = 1;
+    yield_trial = yft[1];
+  }
+  double tmp[6];
+  de_tmul(m.de, df_t, tmp);
+  double lambda_trial = yield_trial / (dot6(tmp, dp_t) + softening_trial);
+  mc_invariants(stress, sv);
+  double yf[2];
+  int yt = mc_yield_state(yf, sv);
+  double yield = std::numeric_limits<double>::max();
+  if (yt == FS_TENSILE) yield = yf[0];
+  if (yt == FS_SHEAR) yield = yf[1];
+  double softening = 0., dp_dq = 0.;
+  double df[6] = {0}, dp[6] = {0};
+  mc_df_dp(m, yt, sv, stress, df, dp, &dp_dq, &softening);
+  de_tmul(m.de, df, tmp);
+  const double lambda = dot6(tmp, dstrain) / (dot6(tmp, dp) + softening);
+  double upd[6];
+  for (int i = 0; i < 6; ++i) upd[i] = trial[i];
+  double dpdstrain = 0.;
+  // -(lambda * de * dp): Eigen folds the scalar into the product's left
+  // operand; each entry is (lambda*D_ik) summed over k in order.
+  auto corr = [&](double lam, const double* dpv) {
+    for (int i = 0; i < 6; ++i) {
+      double s = (lam * m.de[i][0]) * dpv[0];
+      for (int k = 1; k < 6; ++k) s = s + (lam * m.de[i][k]) * dpv[k];
+      upd[i] += -s;
+    }
+  };
+  if (std::fabs(yield) < Tolerance) {
+    corr(lambda, dp);
+    dpdstrain = lambda * dp_dq;
+  } else {
+    corr(lambda_trial, dp_t);
+    dpdstrain = lambda_trial * dp_dq_trial;
+  }
+  const int itr_max = 100;
+  for (int itr = 0; itr < itr_max; ++itr) {
+    mc_invariants(upd, sv);
+    ytt = mc_yield_state(yft, sv);
+    if (yft[0] < Tolerance && yft[1] < Tolerance) break;
+    mc_df_dp(m, ytt, sv, upd, df_t, dp_t, &dp_dq_trial, &softening_trial);
+    if (ytt == FS_TENSILE) yield_trial = yft[0];
+    if (ytt == FS_SHEAR) yield_trial = yft[1];
+    de_tmul(m.de, df_t, tmp);
+    lambda_trial = yield_trial / (dot6(tmp, dp_t) + softening_trial);
+    corr(lambda_trial, dp_t);
+    dpdstrain += lambda_trial * dp_dq_trial;
+  }
+  mc_invariants(upd, sv);
+  sv[SV_PDS] += dpdstrain;
+  for (int i = 0; i < 6; ++i) out[i] = upd[i];
+}
+
+// ------------------------------ Elements -----------------------------------
+// quadrilateral_element.tcc:11-43 ; hexahedron_element.tcc:19-73
+static void shapefn(int dim, const double* xi, double* N) {
+  if (dim == 2) {
+    N[0] = 0.25 * (1 - xi[0]) * (1 - xi[1]);
+    N[1] = 0.25 * (1 + xi[0]) * (1 - xi[1]);
+    N[2] = 0.25 * (1 + xi[0]) * (1 + xi[1]);
+    N[3] = 0.25 * (1 - xi[0]) * (1 + xi[1]);
+  } else {
+    N[0] = 0.125 * (1 - xi[0]) * (1 - xi[1]) * (1 - xi[2]);
+    N[1] = 0.125 * (1 + xi[0]) * (1 - xi[1]) * (1 - xi[2]);
+    N[2] = 0.125 * (1 + xi[0]) * (1 + xi[1]) * (1 - xi[2]);
+    N[3] = 0.125 * (1 - xi[0]) * (1 + xi[1]) * (1 - xi[2]);
+    N[4] = 0.125 * (1 - xi[0]) * (1 - xi[1]) * (1 + xi[2]);
+    N[5] = 0.125 * (1 + xi[0]) * (1 - xi[1]) * (1 + xi[2]);
+    N[6] = 0.125 * (1 + xi[0]) * (1 + xi[1]) * (1 + xi[2]);
+    N[7] = 0.125 * (1 - xi[0]) * (1 + xi[1]) * (1 + xi[2]);
+  }
+}
+
+// grad[i*3 + a] = dN_i / dxi_a
+static void grad_shapefn(int dim, const double* xi, double* g) {
+  if (dim == 2) {
+    g[0 * 3 + 0] = -0.25 * (1 - xi[1]);
+    g[1 * 3 + 0] = 0.25 * (1 - xi[1]);
+    g[2 * 3 + 0] = 0.25 * (1 + xi[1]);
+    g[3 * 3 + 0] = -0.25 * (1 + xi[1]);
+    g[0 * 3 + 1] = -0.25 * (1 - xi[0]);
+    g[1 * 3 + 1] = -0.25 * (1 + xi[0]);
+    g[2 * 3 + 1] = 0.25 * (1 + xi[0]);
+    g[3 * 3 + 1] = 0.25 * (1 - xi[0]);
+  } else {
+    g[0 * 3 + 0] = -0.125 * (1 - xi[1]) * (1 - xi[2]);
+    g[1 * 3 + 0] = 0.125 * (1 - xi[1]) * (1 - xi[2]);
+    g[2 * 3 + 0] = 0.125 * (1 + xi[1]) * (1 - xi[2]);
+    g[3 * 3 + 0] = -0.125 * (1 + xi[1]) * (1 - xi[2]);
+    g[4 * 3 + 0] = -0.125 * (1 - xi[1]) * (1 + xi[2]);
+    g[5 * 3 + 0] = 0.125 * (1 - xi[1]) * (1 + xi[2]);
+    g[6 * 3 + 0] = 0.125 * (1 + xi[1]) * (1 + xi[2]);
+    g[7 * 3 + 0] = -0.125 * (1 + xi[1]) * (1 + xi[2]);
+
+    g[0 * 3 + 1] = -0.125 * (1 - xi[0]) * (1 - xi[2]);
+    g[1 * 3 + 1] = -0.125 * (1 + xi[0]) * (1 - xi[2]);
+    g[2 * 3 + 1] = 0.125 * (1 + xi[0]) * (1 - xi[2]);
+    g[3 * 3 + 1] = 0.125 * (1 - xi[0]) * (1 - xi[2]);
+    g[4 * 3 + 1] = -0.125 * (1 - xi[0]) * (1 + xi[2]);
+    g[5 * 3 + 1] = -0.125 * (1 + xi[0]) * (1 + xi[2]);
+    g[6 * 3 + 1] = 0.125 * (1 + xi[0]) * (1 + xi[2]);
+    g[7 * 3 + 1] = 0.125 * (1 - xi[0]) * (1 + xi[2]);
+
+    g[0 * 3 + 2] = -0.125 * (1 - xi[0]) * (1 - xi[1]);
+    g[1 * 3 + 2] = -0.125 * (1 + xi[0]) * (1 - xi[1]);
+    g[2 * 3 + 2] = -0.125 * (1 + xi[0]) * (1 + xi[1]);
+    g[3 * 3 + 2] = -0.125 * (1 - xi[0]) * (1 + xi[1]);
+    g[4 * 3 + 2] = 0.125 * (1 - xi[0]) * (1 - xi[1]);
+    g[5 * 3 + 2] = 0.125 * (1 + xi[0]) * (1 - xi[1]);
+    g[6 * 3 + 2] = 0.125 * (1 + xi[0]) * (1 + xi[1]);
+    g[7 * 3 + 2] = 0.125 * (1 - xi[0]) * (1 + xi[1]);
+  }
+}
+
+// Inverse of a 2x2 / 3x3 matrix by the adjugate over the determinant
+// (cofactor expansion along the first column).
+static void inverse(int dim, const double J[3][3], double Ji[3][3]) {
+  if (dim == 2) {
+    const double det = J[0][0] * J[1][1] - J[1][0] * J[0][1];
+    const double invdet = 1.0 / det;
+    Ji[0][0] = J[1][1] * invdet;
+    Ji[1][0] = -J[1][0] * invdet;
+    Ji[0][1] = -J[0][1] * invdet;
+    Ji[1][1] = J[0][0] * invdet;
+  } else {
+    // cofactors c_ij of J
+    double c[3][3];
+    c[0][0] = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+    c[1][0] = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+    c[2][0] = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+    c[0][1] = J[2][1] * J[0][2] - J[2][2] * J[0][1];
+    c[1][1] = J[2][2] * J[0][0] - J[2][0] * J[0][2];
+    c[2][1] = J[2][0] * J[0][1] - J[2][1] * J[0][0];
+    c[0][2] = J[0][1] * J[1][2] - J[0][2] * J[1][1];
+    c[1][2] = J[0][2] * J[1][0] - J[0][0] * J[1][2];
+    c[2][2] = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+    // c[i][j] is here the (j,i) cofactor, i.e. the adjugate entry adj(i,j).
+    const double det = (J[0][0] * c[0][0] + J[1][0] * c[0][1]) + J[2][0] * c[0][2];
+    const double invdet = 1.0 / det;
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) Ji[i][j] = c[i][j] * invdet;
+  }
+}
+
+// dn_dx (quadrilateral_element.tcc:295-311, hexahedron_element.tcc:307-321):
+// J = grad^T X (sequential over nodes), dN/dx = grad * (J^-1)^T.
+static void dn_dx(int dim, const double* xi, const double X[8][3], double out[8][3]) {
+  const int nn = dim == 2 ? 4 : 8;
+  double g[24];
+  grad_shapefn(dim, xi, g);
+  double J[3][3] = {{0}};
+  for (int a = 0; a < dim; ++a)
+    for (int b = 0; b < dim; ++b) {
+      double s = g[0 * 3 + a] * X[0][b];
+      for (int i = 1; i < nn; ++i) s = s + g[i * 3 + a] * X[i][b];
+      J[a][b] = s;
+    }
+  double Ji[3][3];
+  inverse(dim, J, Ji);
+  for (int i = 0; i < nn; ++i)
+    for (int a = 0; a < dim; ++a) {
+      double s = g[i * 3 + 0] * Ji[a][0];
+      for (int b = 1; b < dim; ++b) s = s + g[i * 3 + b] * Ji[a][b];
+      out[i][a] = s;
+    }
+}
+
+static inline double norm_diff(int dim, const double* a, const double* b) {
+  double s = (a[0] - b[0]) * (a[0] - b[0]);
+  for (int k = 1; k < dim; ++k) s = s + (a[k] - b[k]) * (a[k] - b[k]);
+  return std::sqrt(s);
+}
+
+// ---------------------------------------------------------------------------
+struct Node {
+  double x[3] = {0, 0, 0};
+  double mass = 0, momentum[3] = {0, 0, 0}, velocity[3] = {0, 0, 0}, acceleration[3] = {0, 0, 0};
+  double fext[3] = {0, 0, 0}, fint[3] = {0, 0, 0};
+  bool status = false;
+  std::map<unsigned, double> vel_c, acc_c;
+  bool friction = false;
+  unsigned f_dir = 0;
+  int f_sign = 0;
+  double f_mu = 0;
+};
+
+struct Cell {
+  Index id = 0;
+  Index nodes[8];
+  double X[8][3];
+  double centroid[3], mean_length = 0, volume = 0;
+  double dndx_c[8][3];
+  std::vector<Index> neighbours;  // ascending (std::set order)
+  Index nparticles = 0;
+};
+
+struct Particle {
+  Index id = 0;
+  double x[3] = {0, 0, 0}, u[3] = {0, 0, 0}, v[3] = {0, 0, 0}, xi[3] = {0, 0, 0};
+  Index cell = kNoCell;
+  Index hint = kNoCell;  // cell id held without a cell pointer (mesh.tcc:1119-1121)
+  double N[8], dndx[8][3];
+  double mass = 0, volume = 0, density = 0;
+  double stress[6] = {0}, strain[6] = {0}, dstrain[6] = {0}, strain_rate[6] = {0};
+  double dvol = 0;
+  double F[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+  double dF[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+  int mat = 0;
+  double sv[kNsv] = {0};
+};
+
+enum { SCHEME_USF = 0, SCHEME_USL = 1, SCHEME_MUSL = 2 };
+enum { VU_FLIP = 0, VU_PIC = 1 };
+
+struct Solver {
+  int dim = 2, nn = 4;
+  int scheme = SCHEME_USF;
+  int vu = VU_FLIP;
+  double blending = 1.0;
+  std::vector<Node> nodes;
+  std::vector<Cell> cells;
+  std::vector<Particle> particles;
+  std::vector<Material> materials;
+  double gravity[3] = {0, 0, 0};
+  double ramp = 0;
+  Index step = 0;
+  std::vector<Index> unlocatable;
+  std::string error;
+
+  // ---------------- mesh -----------------
+  // cell.tcc:33-67 initialise, :254-277 centroid/mean length, elements' volume
+  void init_cell(Cell& c) {
+    const int d = dim;
+    for (int k = 0; k < d; ++k) c.centroid[k] = 0.;
+    for (int i = 0; i < nn; ++i)
+      for (int k = 0; k < d; ++k) c.centroid[k] += c.X[i][k];
+    for (int k = 0; k < d; ++k) c.centroid[k] /= nn;
+    static const int q_sides[4][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}};
+    static const int h_sides[12][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {4, 5}, {5, 6},
+                                       {6, 7}, {7, 4}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
+    const int ns = d == 2 ? 4 : 12;
+    c.mean_length = 0.;
+    for (int s = 0; s < ns; ++s) {
+      const int a = d == 2 ? q_sides[s][0] : h_sides[s][0];
+      const int b = d == 2 ? q_sides[s][1] : h_sides[s][1];
+      c.mean_length += norm_diff(d, c.X[a], c.X[b]);
+    }
+    c.mean_length /= ns;
+    c.volume = element_volume(c.X);
+    const double zero[3] = {0, 0, 0};
+    dn_dx(d, zero, c.X, c.dndx_c);
+  }
+
+  double element_volume(const double X[8][3]) const {
+    if (dim == 2) {  // quadrilateral_element.tcc:473-497
+      const double a = norm_diff(2, X[0], X[3]);
+      const double b = norm_diff(2, X[2], X[3]);
+      const double c = norm_diff(2, X[1], X[2]);
+      const double d = norm_diff(2, X[0], X[1]);
+      const double p = norm_diff(2, X[0], X[2]);
+      const double q = norm_diff(2, X[1], X[3]);
+      return 0.25 * std::sqrt(4 * p * p * q * q - sq((a * a + c * c - b * b - d * d)));
+    }
+    // hexahedron_element.tcc:559-597
+    auto sub = [](const double* u, const double* w, double* r) {
+      for (int k = 0; k < 3; ++k) r[k] = u[k] - w[k];
+    };
+    auto cross = [](const double* u, const double* w, double* r) {
+      r[0] = u[1] * w[2] - u[2] * w[1];
+      r[1] = u[2] * w[0] - u[0] * w[2];
+      r[2] = u[0] * w[1] - u[1] * w[0];
+    };
+    auto dot = [](const double* u, const double* w) {
+      return (u[0] * w[0] + u[1] * w[1]) + u[2] * w[2];
+    };
+    const double *a = X[7], *b = X[6], *c = X[2], *d = X[3], *e = X[4], *f = X[5], *g = X[1],
+                 *h = X[0];
+    double t1[3], t2[3], t3[3], t4[3], cr1[3], cr2[3], cr3[3], cr4[3], cr5[3], cr6[3], s[3];
+    double bd[3], ca[3], eb[3], fa[3], de[3], ha[3], cg[3], cf[3], fg[3], hf[3], hg[3], hc[3];
+    sub(b, d, bd); sub(c, a, ca); sub(e, b, eb); sub(f, a, fa); sub(d, e, de); sub(h, a, ha);
+    sub(c, g, cg); sub(c, f, cf); sub(f, g, fg); sub(h, f, hf); sub(h, g, hg); sub(h, c, hc);
+    cross(bd, ca, cr1);  // (b-d)x(c-a)
+    cross(eb, fa, cr2);  // (e-b)x(f-a)
+    cross(de, ha, cr3);  // (d-e)x(h-a)
+    cross(cg, cf, cr4);  // (c-g)x(c-f)
+    cross(fg, hf, cr5);  // (f-g)x(h-f)
+    cross(hg, hc, cr6);  // (h-g)x(h-c)
+    sub(a, g, t1); sub(b, g, t2); sub(e, g, t3); sub(d, g, t4);
+    for (int k = 0; k < 3; ++k) s[k] = (cr1[k] + cr2[k]) + cr3[k];
+    const double v1 = (1.0 / 12) * dot(t1, s);
+    for (int k = 0; k < 3; ++k) s[k] = cr1[k] + cr4[k];
+    const double v2 = (1.0 / 12) * dot(t2, s);
+    for (int k = 0; k < 3; ++k) s[k] = cr2[k] + cr5[k];
+    const double v3 = (1.0 / 12) * dot(t3, s);
+    for (int k = 0; k < 3; ++k) s[k] = cr3[k] + cr6[k];
+    const double v4 = (1.0 / 12) * dot(t4, s);
+    return ((v1 + v2) + v3) + v4;
+  }
+
+  void set_mesh(Index nnodes, const double* coords, Index ncells, const Index* cn) {
+    nodes.assign(nnodes, Node());
+    for (Index i = 0; i < nnodes; ++i)
+      for (int k = 0; k < dim; ++k) nodes[i].x[k] = coords[i * dim + k];
+    cells.assign(ncells, Cell());
+    for (Index c = 0; c < ncells; ++c) {
+      Cell& cl = cells[c];
+      cl.id = c;
+      for (int i = 0; i < nn; ++i) {
+        const Index n = cn[c * nn + i];
+        if (n < 0 || n >= nnodes) throw std::runtime_error("cell node id out of range");
+        cl.nodes[i] = n;
+        for (int k = 0; k < dim; ++k) cl.X[i][k] = nodes[n].x[k];
+        for (int k = dim; k < 3; ++k) cl.X[i][k] = 0.;
+      }
+      init_cell(cl);
+      if (cl.volume <= 0) throw std::runtime_error("Negative or zero volume cell");
+    }
+    // mesh.tcc:287-307 find_cell_neighbours
+    std::vector<std::vector<Index>> node_cells(nnodes);
+    for (Index c = 0; c < ncells; ++c)
+      for (int i = 0; i < nn; ++i) node_cells[cells[c].nodes[i]].push_back(c);
+    for (Index c = 0; c < ncells; ++c) {
+      std::set<Index> nb;
+      for (int i = 0; i < nn; ++i)
+        for (Index o : node_cells[cells[c].nodes[i]])
+          if (o != c) nb.insert(o);
+      cells[c].neighbours.assign(nb.begin(), nb.end());
+    }
+  }
+
+  // ---------------- locate (cell.tcc:324-380, 423-606) -----------------
+  bool is_point_in_cell(const Cell& c, const double* p, double* xi) const {
+    for (int k = 0; k < dim; ++k) xi[k] = std::numeric_limits<double>::max();
+    // approx_point_in_cell
+    if (!(norm_diff(dim, p, c.centroid) < (c.mean_length * 2.))) return false;
+    // local_coordinates_point (Cartesian)
+    double len[3];
+    len[0] = norm_diff(dim, c.X[0], c.X[1]);
+    len[1] = norm_diff(dim, c.X[1], c.X[2]);
+    if (dim == 3) len[2] = norm_diff(dim, c.X[1], c.X[5]);
+    double centre[3];
+    for (int k = 0; k < dim; ++k) {
+      double s = c.X[0][k] + c.X[1][k];
+      for (int i = 2; i < nn; ++i) s = s + c.X[i][k];
+      centre[k] = s / (dim == 2 ? 4.0 : (double)nn);
+    }
+    for (int k = 0; k < dim; ++k) xi[k] = 2. * (p[k] - centre[k]) / len[k];
+    bool status = true;
+    const double tol = kEps;
+    for (int k = 0; k < dim; ++k) {
+      if (xi[k] < -1. || xi[k] > 1. || std::isnan(xi[k]))
+        status = false;
+      else {
+        if (xi[k] < -1. + tol) xi[k] = -1. + tol;
+        if (xi[k] > 1. - tol) xi[k] = 1. - tol;
+      }
+    }
+    return status;
+  }
+
+  void assign_cell_xi(Particle& p, Index c, const double* xi) {
+    if (p.cell != kNoCell) cells[p.cell].nparticles -= 1;
+    p.cell = c;
+    p.hint = kNoCell;
+    for (int k = 0; k < dim; ++k) p.xi[k] = xi[k];
+    cells[c].nparticles += 1;
+  }
+
+  // mesh.tcc:1116-1153
+  bool locate_particle(Particle& p) {
+    double xi[3];
+    if (p.cell != kNoCell || p.hint != kNoCell) {
+      if (p.cell != kNoCell && is_point_in_cell(cells[p.cell], p.x, xi)) {
+        for (int k = 0; k < dim; ++k) p.xi[k] = xi[k];
+        return true;
+      }
+      const Index cid = p.cell != kNoCell ? p.cell : p.hint;
+      for (Index nb : cells[cid].neighbours) {
+        if (is_point_in_cell(cells[nb], p.x, xi)) {
+          assign_cell_xi(p, nb, xi);
+          return true;
+        }
+      }
+    }
+    for (Index c = 0; c < (Index)cells.size(); ++c) {
+      if (is_point_in_cell(cells[c], p.x, xi)) {
+        assign_cell_xi(p, c, xi);
+        return true;
+      }
+    }
+    return false;
+  }
+
+  // mpm_scheme.tcc:245-261 -> mesh.tcc:1097-1112 (sequential sweep)
+  void locate_particles() {
+    unlocatable.clear();
+    for (auto& p : particles)
+      if (!locate_particle(p)) unlocatable.push_back(p.id);
+    if (!unlocatable.empty()) {
+      std::string msg = "Particle(s) outside the mesh domain: ";
+      for (Index id : unlocatable) msg += std::to_string(id) + " ";
+      throw std::runtime_error(msg);
+    }
+  }
+
+  // ---------------- particles init -----------------
+  void add_particle_state(Particle& p) {
+    if (p.mat < 0 || p.mat >= (int)materials.size()) throw std::runtime_error("bad material id");
+  }
+
+  // mesh.tcc:508-576 generate_material_points + cell.tcc:91-143 generate_points
+  // + particle.tcc:589-626 compute_volume / compute_mass.
+  void generate_gauss(int nppd, Index nset, const Index* cset, int mat) {
+    if (mat < 0 || mat >= (int)materials.size()) throw std::runtime_error("bad material id");
+    const Index before = (Index)particles.size();
+    double qp[8][3];
+    int nq = 0;
+    const double g = 1. / std::sqrt(3.);
+    if (nppd == 1) {
+      nq = 1;
+      qp[0][0] = qp[0][1] = qp[0][2] = 0.;
+    } else if (nppd == 2) {
+      if (dim == 2) {  // quadrilateral_quadrature.tcc:22-38
+        const double s[4][2] = {{-g, -g}, {g, -g}, {g, g}, {-g, g}};
+        nq = 4;
+        for (int i = 0; i < 4; ++i) { qp[i][0] = s[i][0]; qp[i][1] = s[i][1]; qp[i][2] = 0; }
+      } else {  // hexahedron_quadrature.tcc:21-57
+        const double s[8][3] = {{-g, -g, -g}, {g, -g, -g}, {g, g, -g}, {-g, g, -g},
+                                {-g, -g, g},  {g, -g, g},  {g, g, g},  {-g, g, g}};
+        nq = 8;
+        for (int i = 0; i < 8; ++i)
+          for (int k = 0; k < 3; ++k) qp[i][k] = s[i][k];
+      }
+    } else {
+      throw std::runtime_error("gauss generator supports nparticles_per_dir 1 or 2");
+    }
+    const Index nc = nset < 0 ? (Index)cells.size() : nset;
+    for (Index ic = 0; ic < nc; ++ic) {
+      const Index c = nset < 0 ? ic : cset[ic];
+      Cell& cl = cells[c];
+      for (int q = 0; q < nq; ++q) {
+        double N[8];
+        shapefn(dim, qp[q], N);
+        Particle p;
+        p.id = (Index)particles.size();
+        for (int k = 0; k < dim; ++k) {
+          double s = cl.X[0][k] * N[0];
+          for (int i = 1; i < nn; ++i) s = s + cl.X[i][k] * N[i];
+          p.x[k] = s;
+        }
+        double xi[3];
+        if (!is_point_in_cell(cl, p.x, xi)) continue;  // generate_points status
+        p.mat = mat;
+        if (materials[mat].type == kMC) mc_initialise_state(materials[mat], p.sv);
+        particles.push_back(p);
+        assign_cell_xi(particles.back(), c, xi);
+      }
+    }
+    if ((Index)particles.size() == before) throw std::runtime_error("No particles were generated!");
+    // locate_particles_mesh (all particles already hold a cell)
+    locate_particles();
+    // compute_volume then compute_mass
+    for (Index i = before; i < (Index)particles.size(); ++i) {
+      Particle& p = particles[i];
+      const Cell& cl = cells[p.cell];
+      p.volume = cl.volume / cl.nparticles;
+      p.density = materials[p.mat].density;
+      p.mass = p.volume * p.density;
+    }
+  }
+
+  // ---------------- step phases -----------------
+  // particle.tcc:537-554 compute_shapefn
+  void compute_shapefn(Particle& p) {
+    const Cell& c = cells[p.cell];
+    shapefn(dim, p.xi, p.N);
+    dn_dx(dim, p.xi, c.X, p.dndx);
+  }
+
+  // node.tcc:56-65, 149-162 (mass_ * factor + mass with factor 1.0)
+  inline void add(double& dst, double v) {
+    ORC_PRAGMA(omp atomic)
+    dst += v;
+  }
+
+  // particle.tcc:630-658
+  void map_mass_momentum(const Particle& p) {
+    const Cell& c = cells[p.cell];
+    for (int i = 0; i < nn; ++i) {
+      Node& n = nodes[c.nodes[i]];
+      add(n.mass, p.mass * p.N[i]);
+      for (int k = 0; k < dim; ++k) add(n.momentum[k], p.mass * p.N[i] * p.v[k]);
+    }
+  }
+
+  // node.tcc:370-404 (Cartesian branch)
+  void apply_velocity_constraints(Node& n) {
+    for (const auto& c : n.vel_c) {
+      const unsigned direction = c.first % dim;
+      n.velocity[direction] = c.second;
+      n.acceleration[direction] = 0.;
+    }
+  }
+
+  // node.tcc:236-252
+  void compute_velocity(Node& n) {
+    const double tolerance = 1.E-16;
+    if (n.mass > tolerance) {
+      for (int k = 0; k < dim; ++k) n.velocity[k] = n.momentum[k] / n.mass;
+      for (int k = 0; k < dim; ++k)
+        if (std::abs(n.velocity[k]) < 1.E-15) n.velocity[k] = 0.;
+    }
+    apply_velocity_constraints(n);
+  }
+
+  // particle.tcc:1605-1651 (2D block / 3D full), then :1643-1651
+  void update_dF(Particle& p, double dt) {
+    const Cell& c = cells[p.cell];
+    double R[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    for (int i = 0; i < nn; ++i) {
+      const Node& n = nodes[c.nodes[i]];
+      for (int a = 0; a < dim; ++a)
+        for (int b = 0; b < dim; ++b) R[a][b] = R[a][b] + (n.velocity[a] * p.dndx[i][b]) * dt;
+    }
+    for (int a = 0; a < dim; ++a)
+      for (int b = 0; b < dim; ++b) {
+        if (a != b && std::fabs(R[a][b]) < 1.E-15) R[a][b] = 0.;
+        if (a == b && std::fabs(R[a][b] - 1.) < 1.E-15) R[a][b] = 1.;
+      }
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) p.dF[i][j] = R[i][j];
+  }
+
+  // particle.tcc:819-858
+  void strain_rate(const Particle& p, const double dndx[8][3], double* sr) {
+    const Cell& c = cells[p.cell];
+    for (int k = 0; k < 6; ++k) sr[k] = 0.;
+    for (int i = 0; i < nn; ++i) {
+      const double* v = nodes[c.nodes[i]].velocity;
+      if (dim == 2) {
+        sr[0] += dndx[i][0] * v[0];
+        sr[1] += dndx[i][1] * v[1];
+        sr[3] += dndx[i][1] * v[0] + dndx[i][0] * v[1];
+      } else {
+        sr[0] += dndx[i][0] * v[0];
+        sr[1] += dndx[i][1] * v[1];
+        sr[2] += dndx[i][2] * v[2];
+        sr[3] += dndx[i][1] * v[0] + dndx[i][0] * v[1];
+        sr[4] += dndx[i][2] * v[1] + dndx[i][1] * v[2];
+        sr[5] += dndx[i][2] * v[0] + dndx[i][0] * v[2];
+      }
+    }
+    if (dim == 2) {
+      if (std::fabs(sr[0]) < 1.E-15) sr[0] = 0.;
+      if (std::fabs(sr[1]) < 1.E-15) sr[1] = 0.;
+      if (std::fabs(sr[3]) < 1.E-15) sr[3] = 0.;
+    } else {
+      for (int k = 0; k < 6; ++k)
+        if (std::fabs(sr[k]) < 1.E-15) sr[k] = 0.;
+    }
+  }
+
+  // particle.tcc:861-877
+  void compute_strain(Particle& p, double dt) {
+    strain_rate(p, p.dndx, p.strain_rate);
+    for (int k = 0; k < 6; ++k) p.dstrain[k] = p.strain_rate[k] * dt;
+    for (int k = 0; k < 6; ++k) p.strain[k] += p.dstrain[k];
+    double src[6];
+    strain_rate(p, cells[p.cell].dndx_c, src);
+    double s = src[0];
+    for (int k = 1; k < dim; ++k) s = s + src[k];
+    p.dvol = dt * s;
+  }
+
+  // particle.tcc:598-606
+  void update_volume(Particle& p) {
+    p.volume *= (1. + p.dvol);
+    p.density = p.density / (1. + p.dvol);
+  }
+
+  // particle.tcc:880-890 -> material
+  void compute_stress(Particle& p) {
+    const Material& m = materials[p.mat];
+    double out[6];
+    if (m.type == kLE) {
+      double ds[6];
+      de_mul(m.de, p.dstrain, ds);
+      for (int k = 0; k < 6; ++k) out[k] = p.stress[k] + ds[k];
+    } else {
+      mc_compute_stress(m, p.stress, p.dstrain, p.density, p.sv, out);
+    }
+    std::memcpy(p.stress, out, sizeof(out));
+  }
+
+  // particle.tcc:1654-1659 F = dF * F (sequential k)
+  void update_F(Particle& p) {
+    double R[3][3];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        double s = p.dF[i][0] * p.F[0][j];
+        for (int k = 1; k < 3; ++k) s = s + p.dF[i][k] * p.F[k][j];
+        R[i][j] = s;
+      }
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) p.F[i][j] = R[i][j];
+  }
+
+  // mpm_scheme.tcc:75-103
+  void compute_stress_strain(double dt) {
+    const Index np = (Index)particles.size();
+    ORC_PRAGMA(omp parallel for schedule(static))
+    for (Index i = 0; i < np; ++i) update_dF(particles[i], dt);
+    ORC_PRAGMA(omp parallel for schedule(static))
+    for (Index i = 0; i < np; ++i) compute_strain(particles[i], dt);
+    ORC_PRAGMA(omp parallel for schedule(static))
+    for (Index i = 0; i < np; ++i) update_volume(particles[i]);
+    ORC_PRAGMA(omp parallel for schedule(dynamic, 256))
+    for (Index i = 0; i < np; ++i) compute_stress(particles[i]);
+    ORC_PRAGMA(omp parallel for schedule(static))
+    for (Index i = 0; i < np; ++i) update_F(particles[i]);
+  }
+
+  // particle.tcc:893-898
+  void map_body_force(const Particle& p, const double* g) {
+    const Cell& c = cells[p.cell];
+    for (int i = 0; i < nn; ++i) {
+      Node& n = nodes[c.nodes[i]];
+      for (int k = 0; k < dim; ++k) add(n.fext[k], g[k] * p.mass * p.N[i]);
+    }
+  }
+
+  // particle.tcc:914-950
+  void map_internal_force(const Particle& p) {
+    const Cell& c = cells[p.cell];
+    const double* s = p.stress;
+    for (int i = 0; i < nn; ++i) {
+      Node& n = nodes[c.nodes[i]];
+      const double* d = p.dndx[i];
+      double f[3];
+      if (dim == 2) {
+        f[0] = d[0] * s[0] + d[1] * s[3];
+        f[1] = d[1] * s[1] + d[0] * s[3];
+      } else {
+        f[0] = d[0] * s[0] + d[1] * s[3] + d[2] * s[5];
+        f[1] = d[1] * s[1] + d[0] * s[3] + d[2] * s[4];
+        f[2] = d[2] * s[2] + d[1] * s[4] + d[0] * s[5];
+      }
+      const double mv = -1. * p.volume;
+      for (int k = 0; k < dim; ++k) add(n.fint[k], f[k] * mv);
+    }
+  }
+
+  // node.tcc:510-657 (Cartesian branches)
+  void apply_friction(Node& n, double dt) {
+    if (!n.friction) return;
+    auto sign = [](double value) { return (value > 0.) ? 1. : -1.; };
+    const unsigned dir_n = n.f_dir;
+    const double sign_n = sign(n.f_sign);
+    const double mu = n.f_mu;
+    if (dim == 2) {
+      const unsigned dir_t = (dim - 1) - dir_n;
+      double acc_n = n.acceleration[dir_n];
+      double acc_t = n.acceleration[dir_t];
+      double vel_t = n.velocity[dir_t];
+      if ((acc_n * sign_n) > 0.0) {
+        if (vel_t != 0.0) {
+          const double vel_net = dt * acc_t + vel_t;
+          const double vel_frictional = dt * mu * std::abs(acc_n);
+          if (std::abs(vel_net) <= vel_frictional)
+            acc_t = -vel_t / dt;
+          else
+            acc_t -= sign(vel_net) * mu * std::abs(acc_n);
+        } else {
+          if (std::abs(acc_t) <= mu * std::abs(acc_n))
+            acc_t = 0.0;
+          else
+            acc_t -= sign(acc_t) * mu * std::abs(acc_n);
+        }
+        n.acceleration[dir_t] = acc_t;
+      }
+    } else {
+      static const int tdir[3][2] = {{1, 2}, {0, 2}, {0, 1}};
+      const unsigned t0 = tdir[dir_n][0], t1 = tdir[dir_n][1];
+      double acc[3], vel[3];
+      for (int k = 0; k < 3; ++k) { acc[k] = n.acceleration[k]; vel[k] = n.velocity[k]; }
+      const double acc_n = acc[dir_n];
+      double acc_t = std::sqrt(acc[t0] * acc[t0] + acc[t1] * acc[t1]);
+      const double vel_t = std::sqrt(vel[t0] * vel[t0] + vel[t1] * vel[t1]);
+      if (acc_n * sign_n > 0.0) {
+        if (vel_t != 0.0) {
+          double vn0 = vel[t0] + acc[t0] * dt;
+          double vn1 = vel[t1] + acc[t1] * dt;
+          const double vel_net_t = std::sqrt(vn0 * vn0 + vn1 * vn1);
+          const double vel_fricion = mu * std::abs(acc_n) * dt;
+          if (vel_net_t <= vel_fricion) {
+            acc[t0] = -vel[t0] / dt;
+            acc[t1] = -vel[t1] / dt;
+          } else {
+            acc[t0] -= mu * std::abs(acc_n) * (vn0 / vel_net_t);
+            acc[t1] -= mu * std::abs(acc_n) * (vn1 / vel_net_t);
+          }
+        } else {
+          if (acc_t <= mu * std::abs(acc_n)) {
+            acc[t0] = 0;
+            acc[t1] = 0;
+          } else {
+            acc_t -= mu * std::abs(acc_n);
+            acc[t0] -= mu * std::abs(acc_n) * (acc[t0] / acc_t);
+            acc[t1] -= mu * std::abs(acc_n) * (acc[t1] / acc_t);
+          }
+        }
+        for (int k = 0; k < 3; ++k) n.acceleration[k] = acc[k];
+      }
+    }
+  }
+
+  // node.tcc:456-485 (Cartesian)
+  void apply_acceleration_constraints(Node& n) {
+    for (const auto& c : n.acc_c) n.acceleration[c.first % dim] = c.second;
+  }
+
+  // node.tcc:309-346
+  bool compute_acceleration_velocity_cundall(Node& n, double dt, double damping) {
+    const double tolerance = 1.0E-15;
+    if (!(n.mass > tolerance)) return false;
+    double f[3] = {0, 0, 0};
+    for (int k = 0; k < dim; ++k) f[k] = n.fext[k] + n.fint[k];
+    double nrm2 = f[0] * f[0];
+    for (int k = 1; k < dim; ++k) nrm2 = nrm2 + f[k] * f[k];
+    const double dn = damping * std::sqrt(nrm2);
+    for (int k = 0; k < dim; ++k) {
+      const double sg = (n.velocity[k] > 0.) ? 1. : ((n.velocity[k] < 0.) ? -1. : 0.);
+      n.acceleration[k] = (f[k] - dn * sg) / n.mass;
+    }
+    apply_friction(n, dt);
+    apply_acceleration_constraints(n);
+    for (int k = 0; k < dim; ++k) n.velocity[k] += n.acceleration[k] * dt;
+    apply_velocity_constraints(n);
+    for (int k = 0; k < dim; ++k)
+      if (std::abs(n.velocity[k]) < tolerance) n.velocity[k] = 0.;
+    for (int k = 0; k < dim; ++k)
+      if (std::abs(n.acceleration[k]) < tolerance) n.acceleration[k] = 0.;
+    return true;
+  }
+
+  // particle.tcc:994-1047 (FLIP), :1050-1070 (PIC)
+  void compute_updated_position(Particle& p, double dt) {
+    const Cell& c = cells[p.cell];
+    double nv[3] = {0, 0, 0}, na[3] = {0, 0, 0};
+    for (int i = 0; i < nn; ++i) {
+      const Node& n = nodes[c.nodes[i]];
+      for (int k = 0; k < dim; ++k) {
+        nv[k] += p.N[i] * n.velocity[k];
+        na[k] += p.N[i] * n.acceleration[k];
+      }
+    }
+    if (vu == VU_FLIP) {
+      for (int k = 0; k < dim; ++k) p.v[k] += na[k] * dt;
+      for (int k = 0; k < dim; ++k) p.v[k] = blending * p.v[k] + (1.0 - blending) * nv[k];
+    } else {
+      for (int k = 0; k < dim; ++k) p.v[k] = nv[k];
+    }
+    for (int k = 0; k < dim; ++k) p.x[k] += nv[k] * dt;
+    for (int k = 0; k < dim; ++k) p.u[k] += nv[k] * dt;
+  }
+
+  void node_reset(Node& n) {  // node.tcc:26-43
+    n.mass = 0;
+    for (int k = 0; k < 3; ++k) {
+      n.momentum[k] = n.velocity[k] = n.acceleration[k] = n.fext[k] = n.fint[k] = 0;
+    }
+    n.status = false;
+  }
+
+  // mpm_explicit.tcc:112-190 for one step
+  void one_step(double dt, double damping) {
+    const Index np = (Index)particles.size();
+    const Index nnd = (Index)nodes.size();
+    const Index ncl = (Index)cells.size();
+    // MPMScheme::initialise (mpm_scheme.tcc:19-40)
+    ORC_PRAGMA(omp parallel for schedule(static))
+    for (Index i = 0; i < nnd; ++i) node_reset(nodes[i]);
+    for (Index c = 0; c < ncl; ++c)
+      if (cells[c].nparticles > 0)
+        for (int i = 0; i < nn; ++i) nodes[cells[c].nodes[i]].status = true;
+    ORC_PRAGMA(omp parallel for schedule(static))
+    for (Index i = 0; i < np; ++i) compute_shapefn(particles[i]);
+    // compute_nodal_kinematics (mpm_scheme.tcc:44-71)
+    ORC_PRAGMA(omp parallel for schedule(static))
+    for (Index i = 0; i < np; ++i) map_mass_momentum(particles[i]);
+    ORC_PRAGMA(omp parallel for schedule(static))
+    for (Index i = 0; i < nnd; ++i)
+      if (nodes[i].status) compute_velocity(nodes[i]);
+    // precompute_stress_strain (USF)
+    if (scheme == SCHEME_USF) compute_stress_strain(dt);
+    // gravity ramp (mpm_explicit.tcc:143-153)
+    double gm = 1.0;
+    if (ramp > 0.0) gm = std::min(1.0, static_cast<double>(step) * dt / ramp);
+    double g[3];
+    for (int k = 0; k < dim; ++k) g[k] = gm * gravity[k];
+    // compute_forces (mpm_scheme.tcc:131-190)
+    ORC_PRAGMA(omp parallel for schedule(static))
+    for (Index i = 0; i < np; ++i) map_body_force(particles[i], g);
+    ORC_PRAGMA(omp parallel for schedule(static))
+    for (Index i = 0; i < np; ++i) map_internal_force(particles[i]);
+    // compute_particle_kinematics (mpm_scheme.tcc:214-241), always Cundall
+    ORC_PRAGMA(omp parallel for schedule(static))
+    for (Index i = 0; i < nnd; ++i)
+      if (nodes[i].status) compute_acceleration_velocity_cundall(nodes[i], dt, damping);
+    ORC_PRAGMA(omp parallel for schedule(static))
+    for (Index i = 0; i < np; ++i) compute_updated_position(particles[i], dt);
+    // postcompute_nodal_kinematics (MUSL, mpm_scheme_musl.tcc:21-35)
+    if (scheme == SCHEME_MUSL) {
+      for (Index i = 0; i < nnd; ++i)
+        if (nodes[i].status) {
+          nodes[i].mass = 0.;
+          for (int k = 0; k < 3; ++k) nodes[i].momentum[k] = 0.;
+        }
+      ORC_PRAGMA(omp parallel for schedule(static))
+      for (Index i = 0; i < np; ++i) map_mass_momentum(particles[i]);
+      ORC_PRAGMA(omp parallel for schedule(static))
+      for (Index i = 0; i < nnd; ++i)
+        if (nodes[i].status) compute_velocity(nodes[i]);
+    }
+    // postcompute_stress_strain (USL / MUSL)
+    if (scheme != SCHEME_USF) compute_stress_strain(dt);
+    // locate_particles
+    locate_particles();
+    ++step;
+  }
+};
+
+}  // namespace orc
+
+// ============================ C interface ===================================
+using orc::Index;
+static bool is_point_in_cell_ok(orc::Solver& s, Index c, const double* x, double* xi) {
+  return s.is_point_in_cell(s.cells[c], x, xi);
+}
+struct orc_ctx {
+  orc::Solver s;
+};
+
+extern "C" {
+
+const char* orc_last_error(orc_ctx* c) { return c->s.error.c_str(); }
+
+orc_ctx* orc_create(int dim, int scheme, int velocity_update, double blending_ratio) {
+  if (dim != 2 && dim != 3) return nullptr;
+  orc_ctx* c = new orc_ctx();
+  c->s.dim = dim;
+  c->s.nn = dim == 2 ? 4 : 8;
+  c->s.scheme = scheme;
+  c->s.vu = velocity_update;
+  c->s.blending = blending_ratio;
+  return c;
+}
+
+void orc_destroy(orc_ctx* c) { delete c; }
+
+int orc_threads() {
+#ifdef ORC_OMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+#define ORC_TRY(...)                  \
+  try {                               \
+    __VA_ARGS__;                      \
+    return 0;                         \
+  } catch (std::exception & e) {      \
+    c->s.error = e.what();            \
+    return -1;                        \
+  }
+
+int orc_set_mesh(orc_ctx* c, Index nnodes, const double* coords, Index ncells, const Index* cn) {
+  ORC_TRY(c->s.set_mesh(nnodes, coords, ncells, cn))
+}
+
+int orc_set_materials(orc_ctx* c, int n, const double* params) {
+  ORC_TRY({
+    c->s.materials.assign(n, orc::Material());
+    for (int i = 0; i < n; ++i) orc::material_init(c->s.materials[i], params + i * orc::kMatParams);
+  })
+}
+
+// Velocity constraints: first assignment per (node, dir) wins (std::map::insert)
+int orc_set_velocity_constraints(orc_ctx* c, Index n, const Index* node, const int* dir,
+                                 const double* val) {
+  ORC_TRY({
+    for (Index i = 0; i < n; ++i) {
+      if ((unsigned)dir[i] >= (unsigned)c->s.dim) throw std::runtime_error("bad dir");
+      c->s.nodes.at(node[i]).vel_c.insert(std::make_pair((unsigned)dir[i], val[i]));
+    }
+  })
+}
+
+int orc_set_acceleration_constraints(orc_ctx* c, Index n, const Index* node, const int* dir,
+                                     const double* val) {
+  ORC_TRY({
+    for (Index i = 0; i < n; ++i) {
+      if ((unsigned)dir[i] >= (unsigned)c->s.dim) throw std::runtime_error("bad dir");
+      auto& nd = c->s.nodes.at(node[i]);
+      if (nd.vel_c.count((unsigned)dir[i]))
+        throw std::runtime_error("Acceleration and velocity constraints set in the same direction");
+      nd.acc_c.insert(std::make_pair((unsigned)dir[i], val[i]));
+    }
+  })
+}
+
+// Friction: last assignment per node wins (std::tuple assignment)
+int orc_set_friction_constraints(orc_ctx* c, Index n, const Index* node, const int* dir,
+                                 const int* sign_n, const double* mu) {
+  ORC_TRY({
+    for (Index i = 0; i < n; ++i) {
+      if ((unsigned)dir[i] >= (unsigned)c->s.dim) throw std::runtime_error("bad dir");
+      auto& nd = c->s.nodes.at(node[i]);
+      nd.friction = true;
+      nd.f_dir = (unsigned)dir[i];
+      nd.f_sign = sign_n[i];
+      nd.f_mu = mu[i];
+    }
+  })
+}
+
+int orc_set_velocity_update(orc_ctx* c, int velocity_update, double blending_ratio) {
+  c->s.vu = velocity_update;
+  c->s.blending = blending_ratio;
+  return 0;
+}
+
+int orc_set_gravity(orc_ctx* c, const double* g, double ramp) {
+  for (int k = 0; k < c->s.dim; ++k) c->s.gravity[k] = g[k];
+  c->s.ramp = ramp;
+  return 0;
+}
+
+int orc_generate_gauss(orc_ctx* c, int nppd, Index nset, const Index* cset, int mat) {
+  ORC_TRY(c->s.generate_gauss(nppd, nset, cset, mat))
+}
+
+// Particles from full state (HDF5-resume style, particle.tcc:35-177).
+// Arrays are in particle order; null pointers leave the field at its default.
+// cell < 0 means "locate by search". volume/mass/density must be given.
+int orc_set_particles(orc_ctx* c, Index n, const double* x, const double* u, const double* v,
+                      const double* stress, const double* strain, const double* F,
+                      const double* mass, const double* volume, const double* density,
+                      const int* mat, const Index* cell, const double* sv) {
+  ORC_TRY({
+    auto& s = c->s;
+    const int d = s.dim;
+    const Index base = (Index)s.particles.size();
+    for (Index i = 0; i < n; ++i) {
+      orc::Particle p;
+      p.id = base + i;
+      for (int k = 0; k < d; ++k) {
+        p.x[k] = x[i * d + k];
+        p.u[k] = u ? u[i * d + k] : 0.;
+        p.v[k] = v ? v[i * d + k] : 0.;
+      }
+      for (int k = 0; k < 6; ++k) {
+        p.stress[k] = stress ? stress[i * 6 + k] : 0.;
+        p.strain[k] = strain ? strain[i * 6 + k] : 0.;
+      }
+      if (F)
+        for (int a = 0; a < 3; ++a)
+          for (int b = 0; b < 3; ++b) p.F[a][b] = F[i * 9 + a * 3 + b];
+      p.mass = mass[i];
+      p.volume = volume[i];
+      p.density = density[i];
+      p.mat = mat[i];
+      if (p.mat < 0 || p.mat >= (int)s.materials.size()) throw std::runtime_error("bad material id");
+      if (s.materials[p.mat].type == orc::kMC) {
+        if (sv)
+          for (int k = 0; k < orc::kNsv; ++k) p.sv[k] = sv[i * orc::kNsv + k];
+        else
+          orc::mc_initialise_state(s.materials[p.mat], p.sv);
+      }
+      p.cell = orc::kNoCell;
+      s.particles.push_back(p);
+      if (cell && cell[i] >= 0) {
+        double xi[3];
+        orc::Particle& q = s.particles.back();
+        if (cell[i] >= (Index)s.cells.size()) throw std::runtime_error("bad cell id");
+        // particle.tcc:386-416 assign_cell: keeps the cell only if the point is in it
+        if (s.is_point_in_cell(s.cells[cell[i]], q.x, xi)) s.assign_cell_xi(q, cell[i], xi);
+      }
+    }
+    s.locate_particles();
+  })
+}
+
+int orc_nparticles(orc_ctx* c, Index* n) {
+  *n = (Index)c->s.particles.size();
+  return 0;
+}
+
+int orc_get_particles(orc_ctx* c, double* x, double* u, double* v, double* stress, double* strain,
+                      double* F, double* mass, double* volume, double* density, int* mat,
+                      Index* cell, double* xi, double* sv) {
+  auto& s = c->s;
+  const int d = s.dim;
+  for (size_t i = 0; i < s.particles.size(); ++i) {
+    const auto& p = s.particles[i];
+    for (int k = 0; k < d; ++k) {
+      if (x) x[i * d + k] = p.x[k];
+      if (u) u[i * d + k] = p.u[k];
+      if (v) v[i * d + k] = p.v[k];
+      if (xi) xi[i * d + k] = p.xi[k];
+    }
+    for (int k = 0; k < 6; ++k) {
+      if (stress) stress[i * 6 + k] = p.stress[k];
+      if (strain) strain[i * 6 + k] = p.strain[k];
+    }
+    if (F)
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) F[i * 9 + a * 3 + b] = p.F[a][b];
+    if (mass) mass[i] = p.mass;
+    if (volume) volume[i] = p.volume;
+    if (density) density[i] = p.density;
+    if (mat) mat[i] = p.mat;
+    if (cell) cell[i] = p.cell;
+    if (sv)
+      for (int k = 0; k < orc::kNsv; ++k) sv[i * orc::kNsv + k] = p.sv[k];
+  }
+  return 0;
+}
+
+int orc_get_nodes(orc_ctx* c, double* mass, double* momentum, double* velocity,
+                  double* acceleration, double* fext, double* fint, int* status) {
+  auto& s = c->s;
+  const int d = s.dim;
+  for (size_t i = 0; i < s.nodes.size(); ++i) {
+    const auto& n = s.nodes[i];
+    if (mass) mass[i] = n.mass;
+    for (int k = 0; k < d; ++k) {
+      if (momentum) momentum[i * d + k] = n.momentum[k];
+      if (velocity) velocity[i * d + k] = n.velocity[k];
+      if (acceleration) acceleration[i * d + k] = n.acceleration[k];
+      if (fext) fext[i * d + k] = n.fext[k];
+      if (fint) fint[i * d + k] = n.fint[k];
+    }
+    if (status) status[i] = n.status ? 1 : 0;
+  }
+  return 0;
+}
+
+int orc_get_cells(orc_ctx* c, double* volume, double* mean_length, double* centroid) {
+  auto& s = c->s;
+  for (size_t i = 0; i < s.cells.size(); ++i) {
+    if (volume) volume[i] = s.cells[i].volume;
+    if (mean_length) mean_length[i] = s.cells[i].mean_length;
+    if (centroid)
+      for (int k = 0; k < s.dim; ++k) centroid[i * s.dim + k] = s.cells[i].centroid[k];
+  }
+  return 0;
+}
+
+// Runs n steps. Returns 0, or -1 with the error message set (e.g. a particle
+// left the mesh, mpm_scheme.tcc:250-256).
+int orc_step(orc_ctx* c, int n, double dt, double damping) {
+  ORC_TRY({
+    for (int i = 0; i < n; ++i) c->s.one_step(dt, damping);
+  })
+}
+
+// ------------- phase-level entry points (replay of reference unit tests) -----
+// Each phase runs one reference sweep over all particles or nodes, as named.
+enum {
+  PH_RESET_NODES = 0,     // node.tcc:26-43 on all nodes + cell.tcc:171-178
+  PH_SHAPEFN = 1,         // particle.tcc:537-554
+  PH_MAP_MASS_MOM = 2,    // particle.tcc:630-658
+  PH_COMPUTE_VEL = 3,     // node.tcc:236-252 (active nodes)
+  PH_DF = 4,              // particle.tcc:1643-1651
+  PH_STRAIN = 5,          // particle.tcc:861-877
+  PH_VOLUME = 6,          // particle.tcc:598-606
+  PH_STRESS = 7,          // particle.tcc:880-890
+  PH_UPDATE_F = 8,        // particle.tcc:1654-1659
+  PH_BODY_FORCE = 9,      // particle.tcc:893-898 with g = arg
+  PH_INTERNAL_FORCE = 10, // particle.tcc:914-950
+  PH_ACC_VEL = 11,        // node.tcc:309-346 (active nodes), damping = arg[0]
+  PH_POSITION = 12,       // particle.tcc:994-1070
+  PH_LOCATE = 13,         // mesh.tcc:1097-1153
+  PH_ACC_VEL_PLAIN = 14   // node.tcc:272-305 (no damping term)
+};
+
+int orc_phase(orc_ctx* c, int phase, double dt, const double* arg) {
+  ORC_TRY({
+    auto& s = c->s;
+    switch (phase) {
+      case PH_RESET_NODES:
+        for (auto& n : s.nodes) s.node_reset(n);
+        for (auto& cl : s.cells)
+          if (cl.nparticles > 0)
+            for (int i = 0; i < s.nn; ++i) s.nodes[cl.nodes[i]].status = true;
+        break;
+      case PH_SHAPEFN: for (auto& p : s.particles) s.compute_shapefn(p); break;
+      case PH_MAP_MASS_MOM: for (auto& p : s.particles) s.map_mass_momentum(p); break;
+      case PH_COMPUTE_VEL:
+        for (auto& n : s.nodes)
+          if (n.status) s.compute_velocity(n);
+        break;
+      case PH_DF: for (auto& p : s.particles) s.update_dF(p, dt); break;
+      case PH_STRAIN: for (auto& p : s.particles) s.compute_strain(p, dt); break;
+      case PH_VOLUME: for (auto& p : s.particles) s.update_volume(p); break;
+      case PH_STRESS: for (auto& p : s.particles) s.compute_stress(p); break;
+      case PH_UPDATE_F: for (auto& p : s.particles) s.update_F(p); break;
+      case PH_BODY_FORCE: for (auto& p : s.particles) s.map_body_force(p, arg); break;
+      case PH_INTERNAL_FORCE: for (auto& p : s.particles) s.map_internal_force(p); break;
+      case PH_ACC_VEL:
+        for (auto& n : s.nodes)
+          if (n.status) s.compute_acceleration_velocity_cundall(n, dt, arg ? arg[0] : 0.);
+        break;
+      case PH_ACC_VEL_PLAIN:
+        for (auto& n : s.nodes)
+          if (n.status) s.compute_acceleration_velocity_cundall(n, dt, 0.);
+        break;
+      case PH_POSITION: for (auto& p : s.particles) s.compute_updated_position(p, dt); break;
+      case PH_LOCATE: s.locate_particles(); break;
+      default: throw std::runtime_error("unknown phase");
+    }
+  })
+}
+
+// Overwrite one node field: 0 mass, 1 momentum, 2 velocity, 3 acceleration,
+// 4 fext, 5 fint, 6 status.
+int orc_set_node_field(orc_ctx* c, Index node, int field, const double* val) {
+  ORC_TRY({
+    auto& n = c->s.nodes.at(node);
+    const int d = c->s.dim;
+    double* dst = nullptr;
+    switch (field) {
+      case 0: n.mass = val[0]; return 0;
+      case 1: dst = n.momentum; break;
+      case 2: dst = n.velocity; break;
+      case 3: dst = n.acceleration; break;
+      case 4: dst = n.fext; break;
+      case 5: dst = n.fint; break;
+      case 6: n.status = val[0] != 0.; return 0;
+      default: throw std::runtime_error("bad field");
+    }
+    for (int k = 0; k < d; ++k) dst[k] = val[k];
+  })
+}
+
+// Overwrite one particle field: 0 x, 1 v, 2 stress(6), 3 volume, 4 mass,
+// 5 density, 6 u, 7 strain(6), 8 sv(9).
+int orc_set_particle_field(orc_ctx* c, Index pid, int field, const double* val) {
+  ORC_TRY({
+    auto& p = c->s.particles.at(pid);
+    const int d = c->s.dim;
+    switch (field) {
+      case 0: for (int k = 0; k < d; ++k) p.x[k] = val[k]; break;
+      case 1: for (int k = 0; k < d; ++k) p.v[k] = val[k]; break;
+      case 2: for (int k = 0; k < 6; ++k) p.stress[k] = val[k]; break;
+      case 3: p.volume = val[0]; break;
+      case 4: p.mass = val[0]; break;
+      case 5: p.density = val[0]; break;
+      case 6: for (int k = 0; k < d; ++k) p.u[k] = val[k]; break;
+      case 7: for (int k = 0; k < 6; ++k) p.strain[k] = val[k]; break;
+      case 8: for (int k = 0; k < orc::kNsv; ++k) p.sv[k] = val[k]; break;
+      default: throw std::runtime_error("bad field");
+    }
+  })
+}
+
+// Per-particle scratch fields: N (nn), dndx (nn*d), dstrain (6), dF (9), dvol.
+int orc_get_particle_scratch(orc_ctx* c, Index pid, double* N, double* dndx, double* dstrain,
+                             double* dF, double* dvol) {
+  ORC_TRY({
+    const auto& p = c->s.particles.at(pid);
+    const int d = c->s.dim, nn = c->s.nn;
+    for (int i = 0; i < nn; ++i) {
+      if (N) N[i] = p.N[i];
+      for (int k = 0; k < d; ++k)
+        if (dndx) dndx[i * d + k] = p.dndx[i][k];
+    }
+    for (int k = 0; k < 6; ++k)
+      if (dstrain) dstrain[k] = p.dstrain[k];
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b)
+        if (dF) dF[a * 3 + b] = p.dF[a][b];
+    if (dvol) *dvol = p.dvol;
+  })
+}
+
+void orc_kat_jacobian(int dim, const double* xi, const double* X, double* J) {
+  const int nn = dim == 2 ? 4 : 8;
+  double g[24];
+  orc::grad_shapefn(dim, xi, g);
+  for (int a = 0; a < dim; ++a)
+    for (int b = 0; b < dim; ++b) {
+      double s = g[0 * 3 + a] * X[0 * dim + b];
+      for (int i = 1; i < nn; ++i) s = s + g[i * 3 + a] * X[i * dim + b];
+      J[a * dim + b] = s;
+    }
+}
+
+// dF/dsigma and dP/dsigma for a given yield type (mohr_coulomb.tcc:195-344).
+void orc_kat_mc_df_dp(const double* params, int yield_type, const double* sv, const double* stress,
+                      double* df, double* dp, double* dp_dq, double* softening) {
+  orc::Material m;
+  orc::material_init(m, params);
+  orc::mc_df_dp(m, yield_type, sv, stress, df, dp, dp_dq, softening);
+}
+
+// Stress invariants into a state vector (mohr_coulomb.tcc:124-134).
+void orc_kat_mc_invariants(const double* stress, double* sv) { orc::mc_invariants(stress, sv); }
+
+void orc_kat_mc_init_state(const double* params, double* sv) {
+  orc::Material m;
+  orc::material_init(m, params);
+  orc::mc_initialise_state(m, sv);
+}
+
+// ------------------- per-function entry points for the KATs ------------------
+void orc_kat_shapefn(int dim, const double* xi, double* N, double* grad) {
+  orc::shapefn(dim, xi, N);
+  if (grad) {
+    double g[24];
+    orc::grad_shapefn(dim, xi, g);
+    const int nn = dim == 2 ? 4 : 8;
+    for (int i = 0; i < nn; ++i)
+      for (int a = 0; a < dim; ++a) grad[i * dim + a] = g[i * 3 + a];
+  }
+}
+
+void orc_kat_dn_dx(int dim, const double* xi, const double* X, double* out) {
+  const int nn = dim == 2 ? 4 : 8;
+  double Xl[8][3] = {{0}};
+  for (int i = 0; i < nn; ++i)
+    for (int k = 0; k < dim; ++k) Xl[i][k] = X[i * dim + k];
+  double o[8][3];
+  orc::dn_dx(dim, xi, Xl, o);
+  for (int i = 0; i < nn; ++i)
+    for (int k = 0; k < dim; ++k) out[i * dim + k] = o[i][k];
+}
+
+double orc_kat_cell_volume(int dim, const double* X) {
+  orc::Solver s;
+  s.dim = dim;
+  s.nn = dim == 2 ? 4 : 8;
+  double Xl[8][3] = {{0}};
+  for (int i = 0; i < s.nn; ++i)
+    for (int k = 0; k < dim; ++k) Xl[i][k] = X[i * dim + k];
+  return s.element_volume(Xl);
+}
+
+// One constitutive update. params: material parameter vector; sv (9) is read
+// and updated for MohrCoulomb (initialised from the material when init != 0).
+int orc_kat_stress(const double* params, const double* stress, const double* dstrain,
+                   double density, double* sv, int init_sv, double* out) {
+  orc::Material m;
+  orc::material_init(m, params);
+  if (m.type == orc::kLE) {
+    double ds[6];
+    orc::de_mul(m.de, dstrain, ds);
+    for (int k = 0; k < 6; ++k) out[k] = stress[k] + ds[k];
+  } else {
+    if (init_sv) orc::mc_initialise_state(m, sv);
+    orc::mc_compute_stress(m, stress, dstrain, density, sv, out);
+  }
+  return 0;
+}
+
+// Invariants of a Voigt stress: p, q, j2, j3, lode angle; dtheta/dsigma (6).
+void orc_kat_invariants(const double* s, double* out, double* dtheta) {
+  out[0] = orc::mu_p(s);
+  out[1] = orc::mu_q(s);
+  out[2] = orc::mu_j2(s);
+  out[3] = orc::mu_j3(s);
+  out[4] = orc::mu_lode_angle(s);
+  if (dtheta) orc::mu_dtheta_dsigma(s, orc::kEps, dtheta);
+}
+
+// Yield functions and state from MC state variables (epsilon, rho, theta,
+// phi, cohesion, tension_cutoff are read from sv).
+int orc_kat_mc_yield(const double* sv, double* yf) { return orc::mc_yield_state(yf, sv); }
+
+// Node acceleration/velocity update for one node (node.tcc:309-346).
+// vel: in/out, acc: out. Constraint lists are dir-indexed; NaN = none.
+int orc_kat_node_update(int dim, double mass, const double* fext, const double* fint,
+                        double* vel, double* acc, double dt, double damping,
+                        const double* vel_c, const double* acc_c, int f_dir, int f_sign,
+                        double f_mu) {
+  orc::Solver s;
+  s.dim = dim;
+  orc::Node n;
+  n.mass = mass;
+  for (int k = 0; k < dim; ++k) {
+    n.fext[k] = fext[k];
+    n.fint[k] = fint[k];
+    n.velocity[k] = vel[k];
+    n.acceleration[k] = acc[k];
+    if (vel_c && !std::isnan(vel_c[k])) n.vel_c.insert(std::make_pair((unsigned)k, vel_c[k]));
+    if (acc_c && !std::isnan(acc_c[k])) n.acc_c.insert(std::make_pair((unsigned)k, acc_c[k]));
+  }
+  if (f_dir >= 0) {
+    n.friction = true;
+    n.f_dir = f_dir;
+    n.f_sign = f_sign;
+    n.f_mu = f_mu;
+  }
+  bool st = s.compute_acceleration_velocity_cundall(n, dt, damping);
+  for (int k = 0; k < dim; ++k) {
+    vel[k] = n.velocity[k];
+    acc[k] = n.acceleration[k];
+  }
+  return st ? 1 : 0;
+}
+
+// Friction alone on given acceleration/velocity (node.tcc:510-657).
+void orc_kat_friction(int dim, double* acc, const double* vel, double dt, int f_dir, int f_sign,
+                      double f_mu) {
+  orc::Solver s;
+  s.dim = dim;
+  orc::Node n;
+  for (int k = 0; k < dim; ++k) {
+    n.acceleration[k] = acc[k];
+    n.velocity[k] = vel[k];
+  }
+  n.friction = true;
+  n.f_dir = f_dir;
+  n.f_sign = f_sign;
+  n.f_mu = f_mu;
+  s.apply_friction(n, dt);
+  for (int k = 0; k < dim; ++k) acc[k] = n.acceleration[k];
+}
+
+}  // extern "C"
