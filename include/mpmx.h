@@ -1,0 +1,177 @@
+/*
+ * mpmx — C-ABI of the MI355X explicit MPM step.
+ *
+ * The reference (geomechanics/mpm) has no FFI: its explicit step is reached
+ * through factory keys and virtual C++ interfaces. This header is the
+ * drop-in boundary that replaces, for one solver instance, the phase
+ * interface mpm::MPMScheme<Tdim> (include/solvers/mpm_scheme/mpm_scheme.h:18-150)
+ * as driven by mpm::MPMExplicit<Tdim>::solve()'s time loop
+ * (include/solvers/mpm_explicit.tcc:112-190), together with the state that
+ * loop advances (particles, nodes, cells of mpm::Mesh<Tdim>).
+ *
+ * Conventions (mirroring the reference):
+ *  - One opaque context per solver instance; calls on one context are not
+ *    thread-safe (the reference's caller is single-threaded).
+ *  - Every call returns MPMX_OK (0) or a negative error class; the message is
+ *    available from mpmx_last_error(). No C++ exception crosses the ABI.
+ *  - Host arrays are caller-owned and only read/written during the call.
+ *    Device buffers are library-owned. Each context owns one HIP stream.
+ *  - Particle arrays are in the reference's particle order (creation order,
+ *    mpm::Mesh::particles_ vector, include/mesh/mesh.tcc:1156-1161); the
+ *    library keeps that order at its boundary whatever order it uses inside.
+ *  - Floating point is IEEE FP64 throughout.
+ */
+#ifndef MPMX_H_
+#define MPMX_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mpmx_ctx mpmx_ctx;
+
+/* status codes */
+#define MPMX_OK 0
+#define MPMX_E_ARG (-1)         /* invalid argument or size */
+#define MPMX_E_STATE (-2)       /* call out of order (e.g. step before mesh) */
+#define MPMX_E_HIP (-3)         /* HIP runtime error */
+#define MPMX_E_UNLOCATABLE (-4) /* particle left the mesh (mpm_scheme.tcc:250-256) */
+#define MPMX_E_UNSUPPORTED (-5) /* option not available in this build */
+
+/* stress update schemes (mpm_explicit.tcc:8-13) */
+#define MPMX_USF 0
+#define MPMX_USL 1
+#define MPMX_MUSL 2
+
+/* velocity update (mpm::VelocityUpdate, data_types.h:50; particle.tcc:994-1070) */
+#define MPMX_FLIP 0
+#define MPMX_PIC 1
+
+/* creation flags */
+#define MPMX_ORDERED 0x1u /* nodal sums in particle order: bitwise equal to a
+                             single-threaded reference run (slower) */
+#define MPMX_NO_GRAPH 0x2u /* launch kernels directly instead of replaying a
+                              captured hipGraph per step */
+
+/* Number of doubles in one material parameter vector (layout documented in
+ * mpm_amd/materials.py; replaces Factory<Material<Tdim>, unsigned, const Json&>
+ * construction from the JSON "materials" array, src/material.cc:31-53). */
+#define MPMX_MATERIAL_PARAMS 16
+/* MohrCoulomb state variables per particle (mohr_coulomb.tcc:85-111) */
+#define MPMX_STATE_VARS 9
+
+/* Create a solver for dim 2 (quad4, "ED2Q4") or 3 (hex8, "ED3H8").
+ * Replaces Factory<mpm::MPM>::create("MPMExplicit2D"/"MPMExplicit3D")
+ * (src/mpm.cc:26-32) plus the scheme/velocity-update choice read by
+ * MPMBase's constructor (mpm_base.tcc:57-100). */
+int mpmx_create(mpmx_ctx** ctx, int dim, int scheme, int velocity_update,
+                double blending_ratio, unsigned flags);
+int mpmx_destroy(mpmx_ctx* ctx);
+const char* mpmx_last_error(const mpmx_ctx* ctx);
+
+/* Mesh: node coordinates [nnodes*dim] and cell connectivity [ncells*2^dim]
+ * in the element's local node order. Replaces Mesh::create_nodes /
+ * create_cells / find_cell_neighbours (mesh.tcc:17-49, 211-257, 287-307).
+ * Only isoparametric = 0 (Cartesian local coordinates, cell.tcc:423-606) is
+ * supported; 1 returns MPMX_E_UNSUPPORTED. */
+int mpmx_set_mesh(mpmx_ctx* ctx, int64_t nnodes, const double* coords,
+                  int64_t ncells, const int64_t* cell_nodes, int isoparametric);
+
+/* Materials (params: n * MPMX_MATERIAL_PARAMS). Material id = index. */
+int mpmx_set_materials(mpmx_ctx* ctx, int n, const double* params);
+
+/* Nodal constraints, Cartesian form. Velocity/acceleration: per (node, dir)
+ * the first value given wins, as std::map::insert does in
+ * Node::assign_velocity_constraint (node.tcc:348-366, 407-437).
+ * Friction: one (dir, sign_n, mu) per node, last one wins
+ * (Node::assign_friction_constraint, node.tcc:488-507). */
+int mpmx_set_velocity_constraints(mpmx_ctx* ctx, int64_t n, const int64_t* node,
+                                  const int* dir, const double* value);
+int mpmx_set_acceleration_constraints(mpmx_ctx* ctx, int64_t n, const int64_t* node,
+                                      const int* dir, const double* value);
+int mpmx_set_friction_constraints(mpmx_ctx* ctx, int64_t n, const int64_t* node,
+                                  const int* dir, const int* sign_n, const double* mu);
+
+/* Gravity and its ramp (MPMBase::initialise_loads, mpm_base.tcc:788-830;
+ * multiplier min(1, step*dt/T), mpm_explicit.tcc:143-153). */
+int mpmx_set_gravity(mpmx_ctx* ctx, const double* g, double ramping_time);
+
+/* Particles at the Gauss points of cells ("gauss" generator,
+ * Mesh::generate_material_points + Cell::generate_points, mesh.tcc:508-576,
+ * cell.tcc:91-143), then located, volume = cell volume / particles in the
+ * cell and mass = volume * density (particle.tcc:589-626).
+ * nset < 0 means all cells. nparticles_per_dir: 1 or 2. */
+int mpmx_generate_gauss(mpmx_ctx* ctx, int nparticles_per_dir, int64_t nset,
+                        const int64_t* cell_set, int material_id);
+
+/* Append particles from full state, as the HDF5 resume path does
+ * (Particle::initialise_particle(POD), particle.tcc:35-177). Arrays of n
+ * rows; x,u,v: dim per row; stress, strain: 6 (Voigt xx,yy,zz,xy,yz,xz);
+ * F: 9 (row-major); statevars: MPMX_STATE_VARS (MohrCoulomb only).
+ * u, v, stress, strain, F, cell, statevars may be NULL (zero / identity /
+ * locate / material initial state). cell[i] >= 0 is a hint that is kept when
+ * the point lies in that cell (locate_particle_cells, mesh.tcc:1116-1153). */
+int mpmx_set_particles(mpmx_ctx* ctx, int64_t n, const double* x, const double* u,
+                       const double* v, const double* stress, const double* strain,
+                       const double* F, const double* mass, const double* volume,
+                       const double* density, const int* material_id,
+                       const int64_t* cell, const double* statevars);
+
+/* Assign every particle's velocity (n must equal the particle count; rows of
+ * dim, reference particle order). Replaces Particle::assign_velocity
+ * (particle.tcc:771-777) as applied to all particles, e.g. for initial
+ * velocities that the reference receives through its HDF5 resume path. */
+int mpmx_assign_velocity(mpmx_ctx* ctx, int64_t n, const double* v);
+
+/* Assign every particle's stress (n rows of 6 Voigt components, reference
+ * particle order). Replaces Mesh::assign_particles_stresses
+ * (mesh.tcc:1502-1523), the "particles_stresses" input (mpm_base.tcc:1595-1632). */
+int mpmx_assign_stresses(mpmx_ctx* ctx, int64_t n, const double* stress);
+
+int mpmx_nparticles(const mpmx_ctx* ctx, int64_t* n);
+
+/* Copy particle state back in reference particle order; NULL skips a field.
+ * xi: natural coordinates in the current cell (dim per row). */
+int mpmx_get_particles(mpmx_ctx* ctx, double* x, double* u, double* v, double* stress,
+                       double* strain, double* F, double* mass, double* volume,
+                       double* density, int* material_id, int64_t* cell, double* xi,
+                       double* statevars);
+
+/* Nodal state after the last step (for checks): mass, velocity [n*dim],
+ * acceleration [n*dim], active flag. NULL skips a field. */
+int mpmx_get_nodes(mpmx_ctx* ctx, double* mass, double* velocity, double* acceleration,
+                   int* active);
+
+/* Advance nsteps explicit steps (one iteration of the loop in
+ * mpm_explicit.tcc:112-190 each: MPMScheme::initialise,
+ * compute_nodal_kinematics, pre/postcompute_stress_strain, compute_forces,
+ * compute_particle_kinematics with Cundall damping, postcompute_nodal_kinematics,
+ * locate_particles). Returns MPMX_E_UNLOCATABLE if a particle left the mesh;
+ * mpmx_last_error() then names it and the step. */
+int mpmx_step(mpmx_ctx* ctx, int nsteps, double dt, double damping_factor);
+
+/* Step counter (drives the gravity ramp); set on resume (mpm_base.tcc:509-551). */
+int mpmx_set_step(mpmx_ctx* ctx, int64_t step);
+int mpmx_get_step(const mpmx_ctx* ctx, int64_t* step);
+
+/* Block until all work queued on the context's stream has finished. */
+int mpmx_synchronize(mpmx_ctx* ctx);
+
+/* Per-kernel timing with HIP events recorded on the context's stream
+ * (no counterpart in the reference, which only logs the solver duration,
+ * mpm_explicit.tcc:191-196). While enabled, mpmx_step launches kernels
+ * directly with an event pair around each launch. */
+int mpmx_profile(mpmx_ctx* ctx, int enable);
+/* Fills up to cap entries: names (NUL-terminated, <= 47 chars each, 48-byte
+ * stride), total milliseconds and launch counts since profiling was enabled.
+ * *n receives the number of kernels. */
+int mpmx_kernel_times(mpmx_ctx* ctx, int cap, char* names, double* total_ms,
+                      int64_t* launches, int* n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MPMX_H_ */

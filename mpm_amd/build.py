@@ -1,0 +1,72 @@
+"""Build recipe for the HIP library (gfx950) and the C++ host driver.
+
+``build()`` compiles every source under mpm_amd/csrc into
+``mpm_amd/libmpmx.so`` with hipcc, in-tree, so the library travels with the
+repository snapshot. Flags:
+
+* ``-ffp-contract=off``: no FMA contraction, so every FP64 expression rounds
+  exactly as written (the reference's x86-64 build has no FMA either);
+* ``-Xclang -target-feature -Xclang -packed-fp32-ops``: no packed FP32 ops;
+* ``-munsafe-fp-atomics``: hardware FP64 atomic add on coarse-grained memory.
+"""
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libmpmx.so")
+ARCH = os.environ.get("MPMX_OFFLOAD_ARCH", "gfx950")
+
+SOURCES = ["mpmx_host.cpp", "mpmx_kernels.hip", "mpmx_sort.hip"]
+HOST_TOOL = "mpmx_run"  # C++ drop-in driver (mpm::MPMExplicit over the C-ABI)
+
+
+def hipcc():
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if cand and os.path.exists(cand):
+            return cand
+    raise RuntimeError("hipcc not found")
+
+
+COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math",
+          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+          "-munsafe-fp-atomics", "-Wno-unused-command-line-argument"]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout)
+        raise RuntimeError("build failed: %s" % " ".join(cmd))
+    return r.stdout
+
+
+def build(verbose=False, jobs=4):
+    """Compile the library (always recompiles) and return its path."""
+    from concurrent.futures import ThreadPoolExecutor
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    cc = hipcc()
+    objs = []
+    cmds = []
+    for src in SOURCES:
+        obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
+        objs.append(obj)
+        lang = ["-x", "hip"] if src.endswith(".hip") or src.endswith(".cpp") else []
+        cmds.append([cc, "--offload-arch=" + ARCH] + COMMON + lang +
+                    ["-c", os.path.join(CSRC, src), "-o", obj])
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        outs = list(ex.map(_run, cmds))
+    if verbose:
+        for o in outs:
+            sys.stdout.write(o)
+    tmp = LIB + ".tmp"
+    _run([cc, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

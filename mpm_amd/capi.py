@@ -1,0 +1,268 @@
+"""ctypes binding of the mpmx C-ABI (include/mpmx.h).
+
+``Solver`` is the thin host-side object over one ``mpmx_ctx``: the same
+operations, argument meaning and particle order as the C-ABI, with numpy
+arrays. It loads ``mpm_amd/libmpmx.so`` and raises if the library or a HIP
+device is missing; there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmpmx.so")
+
+OK, E_ARG, E_STATE, E_HIP, E_UNLOCATABLE, E_UNSUPPORTED = 0, -1, -2, -3, -4, -5
+SCHEMES = {"usf": 0, "usl": 1, "musl": 2}
+VELOCITY_UPDATES = {"flip": 0, "pic": 1}
+ORDERED = 0x1
+NO_GRAPH = 0x2
+
+# Every entry point declared in include/mpmx.h.
+EXPORTS = (
+    "mpmx_create", "mpmx_destroy", "mpmx_last_error", "mpmx_set_mesh", "mpmx_set_materials",
+    "mpmx_set_velocity_constraints", "mpmx_set_acceleration_constraints",
+    "mpmx_set_friction_constraints", "mpmx_set_gravity", "mpmx_generate_gauss",
+    "mpmx_set_particles", "mpmx_assign_velocity",
+    "mpmx_assign_stresses", "mpmx_nparticles", "mpmx_get_particles", "mpmx_get_nodes",
+    "mpmx_step", "mpmx_set_step", "mpmx_get_step", "mpmx_synchronize", "mpmx_profile",
+    "mpmx_kernel_times",
+)
+
+_D = C.POINTER(C.c_double)
+_I64 = C.POINTER(C.c_int64)
+_I32 = C.POINTER(C.c_int)
+_lib = None
+
+
+class MpmxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("mpmx error %d: %s" % (code, msg))
+        self.code = code
+
+
+class UnlocatableParticle(MpmxError):
+    pass
+
+
+def load(path=LIB_PATH):
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError("mpmx HIP library not built: %s (run __graft_entry__.build())" % path)
+    lib = C.CDLL(path)
+    V = C.c_void_p
+    sig = {
+        "mpmx_create": [C.POINTER(V), C.c_int, C.c_int, C.c_int, C.c_double, C.c_uint],
+        "mpmx_destroy": [V],
+        "mpmx_last_error": [V],
+        "mpmx_set_mesh": [V, C.c_int64, _D, C.c_int64, _I64, C.c_int],
+        "mpmx_set_materials": [V, C.c_int, _D],
+        "mpmx_set_velocity_constraints": [V, C.c_int64, _I64, _I32, _D],
+        "mpmx_set_acceleration_constraints": [V, C.c_int64, _I64, _I32, _D],
+        "mpmx_set_friction_constraints": [V, C.c_int64, _I64, _I32, _I32, _D],
+        "mpmx_set_gravity": [V, _D, C.c_double],
+        "mpmx_generate_gauss": [V, C.c_int, C.c_int64, _I64, C.c_int],
+        "mpmx_set_particles": [V, C.c_int64, _D, _D, _D, _D, _D, _D, _D, _D, _D, _I32, _I64, _D],
+        "mpmx_assign_velocity": [V, C.c_int64, _D],
+        "mpmx_assign_stresses": [V, C.c_int64, _D],
+        "mpmx_nparticles": [V, _I64],
+        "mpmx_get_particles": [V, _D, _D, _D, _D, _D, _D, _D, _D, _D, _I32, _I64, _D, _D],
+        "mpmx_get_nodes": [V, _D, _D, _D, _I32],
+        "mpmx_step": [V, C.c_int, C.c_double, C.c_double],
+        "mpmx_set_step": [V, C.c_int64],
+        "mpmx_get_step": [V, _I64],
+        "mpmx_synchronize": [V],
+        "mpmx_profile": [V, C.c_int],
+        "mpmx_kernel_times": [V, C.c_int, C.c_char_p, _D, _I64, _I32],
+    }
+    for name, args in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = C.c_char_p if name == "mpmx_last_error" else C.c_int
+    _lib = lib
+    return lib
+
+
+def _p(a, t=_D):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+def _f64(a, shape=None):
+    if a is None:
+        return None
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a if shape is None else a.reshape(shape)
+
+
+class Solver:
+    """One explicit-MPM solver instance on the GPU (one mpmx context)."""
+
+    def __init__(self, dim, scheme="usf", velocity_update="flip", blending_ratio=1.0,
+                 ordered=False, graph=True):
+        self.lib = load()
+        self.dim = dim
+        self.nn = 4 if dim == 2 else 8
+        flags = (ORDERED if ordered else 0) | (0 if graph else NO_GRAPH)
+        ctx = C.c_void_p()
+        rc = self.lib.mpmx_create(C.byref(ctx), dim, SCHEMES[scheme],
+                                  VELOCITY_UPDATES[velocity_update], blending_ratio, flags)
+        self.ctx = ctx
+        if rc != OK:
+            msg = self.lib.mpmx_last_error(ctx).decode() if ctx.value else "create failed"
+            if ctx.value:
+                self.lib.mpmx_destroy(ctx)
+            self.ctx = None
+            raise MpmxError(rc, msg)
+        self.nnodes = 0
+        self.ncells = 0
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            self.lib.mpmx_destroy(self.ctx)
+            self.ctx = None
+
+    __del__ = close
+
+    def _check(self, rc):
+        if rc != OK:
+            msg = self.lib.mpmx_last_error(self.ctx).decode()
+            if rc == E_UNLOCATABLE:
+                raise UnlocatableParticle(rc, msg)
+            raise MpmxError(rc, msg)
+
+    # -- setup ---------------------------------------------------------------
+    def set_mesh(self, coords, cells, isoparametric=False):
+        coords = _f64(coords)
+        cells = np.ascontiguousarray(cells, dtype=np.int64)
+        self.nnodes, self.ncells = coords.shape[0], cells.shape[0]
+        self._check(self.lib.mpmx_set_mesh(self.ctx, self.nnodes, _p(coords), self.ncells,
+                                           _p(cells, _I64), int(bool(isoparametric))))
+
+    def set_materials(self, params):
+        params = _f64(params).reshape(-1, 16)
+        self._check(self.lib.mpmx_set_materials(self.ctx, params.shape[0], _p(params)))
+
+    def _cons(self, fn, nodes, dirs, *vals):
+        nodes = np.ascontiguousarray(nodes, dtype=np.int64)
+        dirs = np.ascontiguousarray(np.broadcast_to(dirs, nodes.shape), dtype=np.int32)
+        args = []
+        for v, t in vals:
+            dt = np.int32 if t is _I32 else np.float64
+            a = np.ascontiguousarray(np.broadcast_to(v, nodes.shape), dtype=dt)
+            args.append((a, t))
+        self._check(fn(self.ctx, len(nodes), _p(nodes, _I64), _p(dirs, _I32),
+                       *[_p(a, t) for a, t in args]))
+
+    def set_velocity_constraints(self, nodes, dirs, values):
+        self._cons(self.lib.mpmx_set_velocity_constraints, nodes, dirs, (values, _D))
+
+    def set_acceleration_constraints(self, nodes, dirs, values):
+        self._cons(self.lib.mpmx_set_acceleration_constraints, nodes, dirs, (values, _D))
+
+    def set_friction_constraints(self, nodes, dirs, sign_n, mu):
+        self._cons(self.lib.mpmx_set_friction_constraints, nodes, dirs, (sign_n, _I32), (mu, _D))
+
+    def set_gravity(self, g, ramping_time=0.0):
+        self._check(self.lib.mpmx_set_gravity(self.ctx, _p(_f64(g)), ramping_time))
+
+    def generate_gauss(self, nparticles_per_dir, cell_set=None, material_id=0):
+        if cell_set is None:
+            self._check(self.lib.mpmx_generate_gauss(self.ctx, nparticles_per_dir, -1, None,
+                                                     material_id))
+        else:
+            cs = np.ascontiguousarray(cell_set, dtype=np.int64)
+            self._check(self.lib.mpmx_generate_gauss(self.ctx, nparticles_per_dir, len(cs),
+                                                     _p(cs, _I64), material_id))
+
+    def set_particles(self, x, mass, volume, density, material_id, v=None, u=None, stress=None,
+                      strain=None, F=None, cell=None, statevars=None):
+        d = self.dim
+        x = _f64(x, (-1, d))
+        n = x.shape[0]
+        a = dict(u=_f64(u, (n, d)), v=_f64(v, (n, d)), stress=_f64(stress, (n, 6)),
+                 strain=_f64(strain, (n, 6)), F=_f64(F, (n, 9)), mass=_f64(mass, (n,)),
+                 volume=_f64(volume, (n,)), density=_f64(density, (n,)),
+                 sv=_f64(statevars, (n, 9)))
+        mat = np.ascontiguousarray(np.broadcast_to(material_id, (n,)), dtype=np.int32)
+        cell = None if cell is None else np.ascontiguousarray(cell, dtype=np.int64)
+        self._check(self.lib.mpmx_set_particles(
+            self.ctx, n, _p(x), _p(a["u"]), _p(a["v"]), _p(a["stress"]), _p(a["strain"]),
+            _p(a["F"]), _p(a["mass"]), _p(a["volume"]), _p(a["density"]), _p(mat, _I32),
+            _p(cell, _I64), _p(a["sv"])))
+
+    def assign_velocity(self, v):
+        v = _f64(v, (-1, self.dim))
+        self._check(self.lib.mpmx_assign_velocity(self.ctx, v.shape[0], _p(v)))
+
+    def assign_stresses(self, stress):
+        stress = _f64(stress, (-1, 6))
+        self._check(self.lib.mpmx_assign_stresses(self.ctx, stress.shape[0], _p(stress)))
+
+    # -- state ---------------------------------------------------------------
+    @property
+    def nparticles(self):
+        n = C.c_int64(0)
+        self._check(self.lib.mpmx_nparticles(self.ctx, C.byref(n)))
+        return n.value
+
+    def particles(self, fields=None):
+        n, d = self.nparticles, self.dim
+        shapes = dict(x=(n, d), u=(n, d), v=(n, d), stress=(n, 6), strain=(n, 6), F=(n, 9),
+                      mass=(n,), volume=(n,), density=(n,), xi=(n, d), statevars=(n, 9))
+        want = set(fields) if fields else set(shapes) | {"material_id", "cell"}
+        out = {k: np.zeros(s) for k, s in shapes.items() if k in want}
+        if "material_id" in want:
+            out["material_id"] = np.zeros(n, dtype=np.int32)
+        if "cell" in want:
+            out["cell"] = np.zeros(n, dtype=np.int64)
+        g = out.get
+        self._check(self.lib.mpmx_get_particles(
+            self.ctx, _p(g("x")), _p(g("u")), _p(g("v")), _p(g("stress")), _p(g("strain")),
+            _p(g("F")), _p(g("mass")), _p(g("volume")), _p(g("density")),
+            _p(g("material_id"), _I32), _p(g("cell"), _I64), _p(g("xi")), _p(g("statevars"))))
+        return out
+
+    def nodes(self):
+        n, d = self.nnodes, self.dim
+        out = dict(mass=np.zeros(n), velocity=np.zeros((n, d)), acceleration=np.zeros((n, d)),
+                   active=np.zeros(n, dtype=np.int32))
+        self._check(self.lib.mpmx_get_nodes(self.ctx, _p(out["mass"]), _p(out["velocity"]),
+                                            _p(out["acceleration"]), _p(out["active"], _I32)))
+        return out
+
+    def step(self, nsteps, dt, damping_factor=0.0):
+        self._check(self.lib.mpmx_step(self.ctx, int(nsteps), float(dt), float(damping_factor)))
+
+    @property
+    def step_count(self):
+        s = C.c_int64(0)
+        self._check(self.lib.mpmx_get_step(self.ctx, C.byref(s)))
+        return s.value
+
+    def set_step(self, step):
+        self._check(self.lib.mpmx_set_step(self.ctx, int(step)))
+
+    def synchronize(self):
+        self._check(self.lib.mpmx_synchronize(self.ctx))
+
+    def profile(self, enable=True):
+        self._check(self.lib.mpmx_profile(self.ctx, 1 if enable else 0))
+
+    def kernel_times(self):
+        """{kernel name: (total ms, launches)} recorded while profiling."""
+        cap = 64
+        names = C.create_string_buffer(48 * cap)
+        ms = np.zeros(cap)
+        cnt = np.zeros(cap, dtype=np.int64)
+        n = C.c_int(0)
+        self._check(self.lib.mpmx_kernel_times(self.ctx, cap, names, _p(ms), _p(cnt, _I64),
+                                               C.byref(n)))
+        out = {}
+        for i in range(min(n.value, cap)):
+            nm = names.raw[48 * i:48 * i + 48].split(b"\0", 1)[0].decode()
+            out[nm] = (float(ms[i]), int(cnt[i]))
+        return out

@@ -1,0 +1,646 @@
+// Device-side math of the explicit MPM step: shape functions, Jacobian,
+// constitutive updates, node update, point location. Each function evaluates
+// the reference's expression in the reference's order (cited per function) so
+// that, with -ffp-contract=off, the kernels reproduce the single-threaded
+// reference arithmetic operation for operation. Where the reference's order
+// lives in Eigen products, the sequential k-order of a non-vectorised product
+// is used throughout.
+#pragma once
+
+#include "mpmx_internal.h"
+
+namespace mpmx {
+
+#define MPMX_DEV __device__ __forceinline__
+
+static constexpr double kDblEps = 2.220446049250313e-16;  // DBL_EPSILON
+
+template <int D> struct Elem;
+template <> struct Elem<2> { static constexpr int NN = 4; };
+template <> struct Elem<3> { static constexpr int NN = 8; };
+
+// quadrilateral_element.tcc:11-43, hexahedron_element.tcc:19-73
+template <int D> MPMX_DEV void shapefn(const double* xi, double* N);
+template <> MPMX_DEV void shapefn<2>(const double* xi, double* N) {
+  N[0] = 0.25 * (1 - xi[0]) * (1 - xi[1]);
+  N[1] = 0.25 * (1 + xi[0]) * (1 - xi[1]);
+  N[2] = 0.25 * (1 + xi[0]) * (1 + xi[1]);
+  N[3] = 0.25 * (1 - xi[0]) * (1 + xi[1]);
+}
+template <> MPMX_DEV void shapefn<3>(const double* xi, double* N) {
+  N[0] = 0.125 * (1 - xi[0]) * (1 - xi[1]) * (1 - xi[2]);
+  N[1] = 0.125 * (1 + xi[0]) * (1 - xi[1]) * (1 - xi[2]);
+  N[2] = 0.125 * (1 + xi[0]) * (1 + xi[1]) * (1 - xi[2]);
+  N[3] = 0.125 * (1 - xi[0]) * (1 + xi[1]) * (1 - xi[2]);
+  N[4] = 0.125 * (1 - xi[0]) * (1 - xi[1]) * (1 + xi[2]);
+  N[5] = 0.125 * (1 + xi[0]) * (1 - xi[1]) * (1 + xi[2]);
+  N[6] = 0.125 * (1 + xi[0]) * (1 + xi[1]) * (1 + xi[2]);
+  N[7] = 0.125 * (1 - xi[0]) * (1 + xi[1]) * (1 + xi[2]);
+}
+
+// One shape function N_i(xi), same expression as shapefn<D>.
+template <int D> MPMX_DEV double shapefn_i(const double* xi, int i);
+template <> MPMX_DEV double shapefn_i<2>(const double* xi, int i) {
+  const double a = (i == 0 || i == 3) ? (1 - xi[0]) : (1 + xi[0]);
+  const double b = (i < 2) ? (1 - xi[1]) : (1 + xi[1]);
+  return 0.25 * a * b;
+}
+template <> MPMX_DEV double shapefn_i<3>(const double* xi, int i) {
+  const int l = i & 3;
+  const double a = (l == 0 || l == 3) ? (1 - xi[0]) : (1 + xi[0]);
+  const double b = (l < 2) ? (1 - xi[1]) : (1 + xi[1]);
+  const double c = (i < 4) ? (1 - xi[2]) : (1 + xi[2]);
+  return 0.125 * a * b * c;
+}
+
+// Gradient dN_i/dxi (same expressions as the element files).
+template <int D> MPMX_DEV void grad_i(const double* xi, int i, double* g);
+template <> MPMX_DEV void grad_i<2>(const double* xi, int i, double* g) {
+  switch (i) {
+    case 0: g[0] = -0.25 * (1 - xi[1]); g[1] = -0.25 * (1 - xi[0]); break;
+    case 1: g[0] = 0.25 * (1 - xi[1]);  g[1] = -0.25 * (1 + xi[0]); break;
+    case 2: g[0] = 0.25 * (1 + xi[1]);  g[1] = 0.25 * (1 + xi[0]);  break;
+    default: g[0] = -0.25 * (1 + xi[1]); g[1] = 0.25 * (1 - xi[0]); break;
+  }
+}
+template <> MPMX_DEV void grad_i<3>(const double* xi, int i, double* g) {
+  const double m0 = 1 - xi[0], p0 = 1 + xi[0];
+  const double m1 = 1 - xi[1], p1 = 1 + xi[1];
+  const double m2 = 1 - xi[2], p2 = 1 + xi[2];
+  switch (i) {
+    case 0: g[0] = -0.125 * m1 * m2; g[1] = -0.125 * m0 * m2; g[2] = -0.125 * m0 * m1; break;
+    case 1: g[0] = 0.125 * m1 * m2;  g[1] = -0.125 * p0 * m2; g[2] = -0.125 * p0 * m1; break;
+    case 2: g[0] = 0.125 * p1 * m2;  g[1] = 0.125 * p0 * m2;  g[2] = -0.125 * p0 * p1; break;
+    case 3: g[0] = -0.125 * p1 * m2; g[1] = 0.125 * m0 * m2;  g[2] = -0.125 * m0 * p1; break;
+    case 4: g[0] = -0.125 * m1 * p2; g[1] = -0.125 * m0 * p2; g[2] = 0.125 * m0 * m1;  break;
+    case 5: g[0] = 0.125 * m1 * p2;  g[1] = -0.125 * p0 * p2; g[2] = 0.125 * p0 * m1;  break;
+    case 6: g[0] = 0.125 * p1 * p2;  g[1] = 0.125 * p0 * p2;  g[2] = 0.125 * p0 * p1;  break;
+    default: g[0] = -0.125 * p1 * p2; g[1] = 0.125 * m0 * p2; g[2] = 0.125 * m0 * p1;  break;
+  }
+}
+
+template <int D>
+MPMX_DEV void load_cell_X(const Mesh& M, int c, double (*X)[D]) {
+  constexpr int NN = Elem<D>::NN;
+#pragma unroll
+  for (int i = 0; i < NN; ++i) {
+    const int n = M.cell_nodes[(int64_t)c * NN + i];
+#pragma unroll
+    for (int k = 0; k < D; ++k) X[i][k] = M.node_x[k * M.nnodes + n];
+  }
+}
+
+// J = grad^T X (sequential over nodes), J^-1 by adjugate / determinant.
+// quadrilateral_element.tcc:295-311, hexahedron_element.tcc:307-321
+template <int D>
+MPMX_DEV void jacobian_inverse(const double* xi, const double (*X)[D], double (*Ji)[D]) {
+  constexpr int NN = Elem<D>::NN;
+  double J[D][D];
+  {
+    double g[D];
+    grad_i<D>(xi, 0, g);
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+#pragma unroll
+      for (int b = 0; b < D; ++b) J[a][b] = g[a] * X[0][b];
+#pragma unroll
+    for (int i = 1; i < NN; ++i) {
+      grad_i<D>(xi, i, g);
+#pragma unroll
+      for (int a = 0; a < D; ++a)
+#pragma unroll
+        for (int b = 0; b < D; ++b) J[a][b] = J[a][b] + g[a] * X[i][b];
+    }
+  }
+  if constexpr (D == 2) {
+    const double det = J[0][0] * J[1][1] - J[1][0] * J[0][1];
+    const double invdet = 1.0 / det;
+    Ji[0][0] = J[1][1] * invdet;
+    Ji[1][0] = -J[1][0] * invdet;
+    Ji[0][1] = -J[0][1] * invdet;
+    Ji[1][1] = J[0][0] * invdet;
+  } else {
+    double c[3][3];
+    c[0][0] = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+    c[1][0] = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+    c[2][0] = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+    c[0][1] = J[2][1] * J[0][2] - J[2][2] * J[0][1];
+    c[1][1] = J[2][2] * J[0][0] - J[2][0] * J[0][2];
+    c[2][1] = J[2][0] * J[0][1] - J[2][1] * J[0][0];
+    c[0][2] = J[0][1] * J[1][2] - J[0][2] * J[1][1];
+    c[1][2] = J[0][2] * J[1][0] - J[0][0] * J[1][2];
+    c[2][2] = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+    const double det = (J[0][0] * c[0][0] + J[1][0] * c[0][1]) + J[2][0] * c[0][2];
+    const double invdet = 1.0 / det;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) Ji[i][j] = c[i][j] * invdet;
+  }
+}
+
+// dN_i/dx = grad_i * (J^-1)^T (sequential over b)
+template <int D>
+MPMX_DEV void dndx_i(const double* xi, int i, const double (*Ji)[D], double* out) {
+  double g[D];
+  grad_i<D>(xi, i, g);
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    double s = g[0] * Ji[a][0];
+#pragma unroll
+    for (int b = 1; b < D; ++b) s = s + g[b] * Ji[a][b];
+    out[a] = s;
+  }
+}
+
+// --------------------------- locate (cell.tcc:324-380) ----------------------
+template <int D>
+MPMX_DEV bool point_in_cell(const Mesh& M, int c, const double* p, double* xi) {
+  {
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      const double d = p[k] - M.centroid[k * M.ncells + c];
+      s = (k == 0) ? d * d : s + d * d;
+    }
+    if (!(sqrt(s) < M.two_mean[c])) return false;
+  }
+  bool status = true;
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    double x = 2. * (p[k] - M.centroid[k * M.ncells + c]) / M.len[k * M.ncells + c];
+    if (x < -1. || x > 1. || isnan(x)) {
+      status = false;
+    } else {
+      if (x < -1. + kDblEps) x = -1. + kDblEps;
+      if (x > 1. - kDblEps) x = 1. - kDblEps;
+    }
+    xi[k] = x;
+  }
+  return status;
+}
+
+// ----------------------- material_utility.tcc ------------------------------
+MPMX_DEV double sq(double x) { return x * x; }
+MPMX_DEV double mu_p(const double* s) { return (1. / 3. * (s[0] + s[1] + s[2])); }
+MPMX_DEV double mu_j2(const double* s) {
+  return (sq(s[0] - s[1]) + sq(s[1] - s[2]) + sq(s[0] - s[2])) / 6.0 + sq(s[3]) + sq(s[4]) +
+         sq(s[5]);
+}
+MPMX_DEV void mu_dev(const double* s, double* d) {
+  const double p = mu_p(s);
+#pragma unroll
+  for (int i = 0; i < 6; ++i) d[i] = s[i];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) d[i] -= p;
+}
+MPMX_DEV double mu_j3(const double* s) {
+  double d[6];
+  mu_dev(s, d);
+  return (d[0] * d[1] * d[2]) - (d[2] * sq(d[3])) +
+         ((2 * d[3] * d[4] * d[5]) - (d[0] * sq(d[4])) - (d[1] * sq(d[5])));
+}
+MPMX_DEV double mu_q(const double* s) { return sqrt(3 * mu_j2(s)); }
+MPMX_DEV double mu_lode(const double* s) {
+  const double j2 = mu_j2(s);
+  const double j3 = mu_j3(s);
+  double v = 0.0;
+  if (fabs(j2) > kDblEps) v = (3. * sqrt(3.) / 2.) * (j3 / pow(j2, 1.5));
+  if (v > 1.0) v = 1.0;
+  if (v < -1.0) v = -1.0;
+  return (1. / 3.) * acos(v);
+}
+MPMX_DEV void mu_dq_dsigma(const double* s, double* out) {
+  const double q = mu_q(s);
+  double d[6];
+  mu_dev(s, d);
+#pragma unroll
+  for (int i = 0; i < 6; ++i) out[i] = 0.;
+  if (fabs(q) > kDblEps) {
+    out[0] = 3. / (2. * q) * d[0];
+    out[1] = 3. / (2. * q) * d[1];
+    out[2] = 3. / (2. * q) * d[2];
+    out[3] = 3. / q * d[3];
+    out[4] = 3. / q * d[4];
+    out[5] = 3. / q * d[5];
+  }
+}
+MPMX_DEV double dot3(const double* a, const double* b) {
+  return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2];
+}
+MPMX_DEV void mu_dtheta_dsigma(const double* s, double* out) {
+  const double tol = kDblEps;
+  const double j2 = mu_j2(s);
+  const double j3 = mu_j3(s);
+  double dj2[6];
+  mu_dev(s, dj2);
+  dj2[3] *= 2.0;
+  dj2[4] *= 2.0;
+  dj2[5] *= 2.0;
+  double dv[6];
+  mu_dev(s, dv);
+  const double d1[3] = {dv[0], dv[3], dv[5]};
+  const double d2[3] = {dv[3], dv[1], dv[4]};
+  const double d3[3] = {dv[5], dv[4], dv[2]};
+  double dj3[6];
+  dj3[0] = dot3(d1, d1) - (2. / 3.) * j2;
+  dj3[1] = dot3(d2, d2) - (2. / 3.) * j2;
+  dj3[2] = dot3(d3, d3) - (2. / 3.) * j2;
+  dj3[3] = 2.0 * dot3(d1, d2);
+  dj3[4] = 2.0 * dot3(d2, d3);
+  dj3[5] = 2.0 * dot3(d1, d3);
+  double dr_dj2 = -9.0 / 4.0 * sqrt(3.0) * j3;
+  double dr_dj3 = 3.0 / 2.0 * sqrt(3.0);
+  double dtheta_dr = -1.0 / 3.0;
+  if (fabs(j2) > tol) {
+    double r = j3 / 2.0 * pow(j2 / 3.0, -1.5);
+    dr_dj2 *= pow(j2, -2.5);
+    dr_dj3 *= pow(j2, -1.5);
+    const double factor = (fabs(1 - r * r) < tol) ? tol : fabs(1 - r * r);
+    dtheta_dr = -1.0 / (3.0 * sqrt(factor));
+  }
+#pragma unroll
+  for (int i = 0; i < 6; ++i) out[i] = dtheta_dr * ((dr_dj2 * dj2[i]) + (dr_dj3 * dj3[i]));
+}
+
+// --------------------------- elastic tensor products ------------------------
+// D x with D = [[a1,a2,a2],[a2,a1,a2],[a2,a2,a1]] (+) G I3; the zero entries of
+// the reference's 6x6 product only add exact zeros and are skipped.
+MPMX_DEV void de_mul(const Material& m, const double* x, double* y) {
+  y[0] = (m.a1 * x[0] + m.a2 * x[1]) + m.a2 * x[2];
+  y[1] = (m.a2 * x[0] + m.a1 * x[1]) + m.a2 * x[2];
+  y[2] = (m.a2 * x[0] + m.a2 * x[1]) + m.a1 * x[2];
+  y[3] = m.G * x[3];
+  y[4] = m.G * x[4];
+  y[5] = m.G * x[5];
+}
+// x^T D (D symmetric in structure; column sums in k order)
+MPMX_DEV void de_tmul(const Material& m, const double* x, double* y) {
+  y[0] = (x[0] * m.a1 + x[1] * m.a2) + x[2] * m.a2;
+  y[1] = (x[0] * m.a2 + x[1] * m.a1) + x[2] * m.a2;
+  y[2] = (x[0] * m.a2 + x[1] * m.a2) + x[2] * m.a1;
+  y[3] = x[3] * m.G;
+  y[4] = x[4] * m.G;
+  y[5] = x[5] * m.G;
+}
+MPMX_DEV double dot6(const double* a, const double* b) {
+  double s = a[0] * b[0];
+#pragma unroll
+  for (int k = 1; k < 6; ++k) s = s + a[k] * b[k];
+  return s;
+}
+
+// ------------------------------ Mohr-Coulomb --------------------------------
+enum { FS_ELASTIC = 0, FS_SHEAR = 1, FS_TENSILE = 2 };
+
+MPMX_DEV void mc_invariants(const double* s, double* sv) {  // mohr_coulomb.tcc:124-134
+  sv[SV_EPS] = mu_p(s) * sqrt(3.);
+  sv[SV_THETA] = mu_lode(s);
+  sv[SV_RHO] = sqrt(2. * mu_j2(s));
+}
+
+MPMX_DEV int mc_yield(double* yf, const double* sv) {  // mohr_coulomb.tcc:139-191
+  const double Tolerance = -1E-1;
+  const double epsilon = sv[SV_EPS], rho = sv[SV_RHO], theta = sv[SV_THETA];
+  const double phi = sv[SV_PHI], cohesion = sv[SV_COH], tension_cutoff = sv[SV_TCUT];
+  yf[0] = sqrt(2. / 3.) * cos(theta) * rho + epsilon / sqrt(3.) - tension_cutoff;
+  yf[1] = sqrt(1.5) * rho *
+              ((sin(theta + M_PI / 3.) / (sqrt(3.) * cos(phi))) +
+               (cos(theta + M_PI / 3.) * tan(phi) / 3.)) +
+          (epsilon / sqrt(3.)) * tan(phi) - cohesion;
+  int t = FS_ELASTIC;
+  if (yf[0] > Tolerance && yf[1] > Tolerance) {
+    const double n_phi = (1. + sin(phi)) / (1. - sin(phi));
+    const double sigma_p = tension_cutoff * n_phi - 2. * cohesion * sqrt(n_phi);
+    const double alpha_p = sqrt(1. + n_phi * n_phi) + n_phi;
+    const double h = yf[0] + alpha_p * (sqrt(2. / 3.) * cos(theta - 4. * M_PI / 3.) * rho +
+                                        epsilon / sqrt(3.) - sigma_p);
+    t = (h > kDblEps) ? FS_TENSILE : FS_SHEAR;
+  }
+  if (yf[0] < Tolerance && yf[1] > Tolerance) t = FS_SHEAR;
+  if (yf[0] > Tolerance && yf[1] < Tolerance) t = FS_TENSILE;
+  return t;
+}
+
+// mohr_coulomb.tcc:195-344
+MPMX_DEV void mc_df_dp(const Material& m, int yt, const double* sv, const double* stress,
+                       double* df, double* dp, double* dp_dq, double* softening) {
+  const double rho = sv[SV_RHO], theta = sv[SV_THETA], phi = sv[SV_PHI], psi = sv[SV_PSI];
+  const double tension_cutoff = sv[SV_TCUT], pdstrain = sv[SV_PDS];
+  double df_depsilon, df_drho, df_dtheta;
+  if (yt == FS_TENSILE) {
+    df_depsilon = 1. / sqrt(3.);
+    df_drho = sqrt(2. / 3.) * cos(theta);
+    df_dtheta = -sqrt(2. / 3.) * rho * sin(theta);
+  } else {
+    df_depsilon = tan(phi) / sqrt(3.);
+    df_drho = sqrt(1.5) * ((sin(theta + M_PI / 3.) / (sqrt(3.) * cos(phi))) +
+                           (cos(theta + M_PI / 3.) * tan(phi) / 3.));
+    df_dtheta = sqrt(1.5) * rho *
+                ((cos(theta + M_PI / 3.) / (sqrt(3.) * cos(phi))) -
+                 (sin(theta + M_PI / 3.) * tan(phi) / 3.));
+  }
+  double depsilon_dsigma[6] = {0, 0, 0, 0, 0, 0};
+  for (int i = 0; i < 3; ++i) depsilon_dsigma[i] = 1. / 3. * sqrt(3.);
+  double drho_dsigma[6];
+  mu_dq_dsigma(stress, drho_dsigma);
+  for (int i = 0; i < 6; ++i) drho_dsigma[i] = drho_dsigma[i] * sqrt(2. / 3.);
+  double dtheta_dsigma[6];
+  mu_dtheta_dsigma(stress, dtheta_dsigma);
+  for (int i = 0; i < 6; ++i)
+    df[i] = ((df_depsilon * depsilon_dsigma[i]) + (df_drho * drho_dsigma[i])) +
+            (df_dtheta * dtheta_dsigma[i]);
+  if (yt == FS_TENSILE) {
+    const double et_value = 0.6;
+    const double xit = 0.1;
+    double sqpart = 4. * (1 - et_value * et_value) * cos(theta) * cos(theta) +
+                    5. * et_value * et_value - 4. * et_value;
+    if (sqpart < kDblEps) sqpart = 1.E-5;
+    double rt_den = 2. * (1 - et_value * et_value) * cos(theta) + (2. * et_value - 1) * sqrt(sqpart);
+    const double rt_num = 4. * (1 - et_value * et_value) * cos(theta) * cos(theta) +
+                          (2. * et_value - 1) * (2. * et_value - 1);
+    if (fabs(rt_den) < kDblEps) rt_den = 1.E-5;
+    const double rt = rt_num / (3. * rt_den);
+    const double den = sqrt(xit * xit * tension_cutoff * tension_cutoff + 1.5 * rt * rt * rho * rho);
+    const double cl = den > 1.0e-15 ? den : 1.0e-15;
+    const double dp_drt = 1.5 * rho * rho * rt / cl;
+    const double dp_drho = 1.5 * rho * rt * rt / cl;
+    const double dp_depsilon = 1. / sqrt(3.);
+    const double drtden_dtheta =
+        -2. * (1 - et_value * et_value) * sin(theta) -
+        (2. * et_value - 1) * 4. * (1 - et_value * et_value) * cos(theta) * sin(theta) /
+            sqrt(4. * (1 - et_value * et_value) * cos(theta) * cos(theta) +
+                 5. * et_value * et_value - 4. * et_value);
+    const double drtnum_dtheta = -8. * (1 - et_value * et_value) * cos(theta) * sin(theta);
+    const double drt_dtheta =
+        (drtnum_dtheta * rt_den - drtden_dtheta * rt_num) / (3. * rt_den * rt_den);
+    for (int i = 0; i < 6; ++i)
+      dp[i] = ((dp_depsilon * depsilon_dsigma[i]) + (dp_drho * drho_dsigma[i])) +
+              (dp_drt * drt_dtheta * dtheta_dsigma[i]);
+    *dp_dq = dp_drho * sqrt(2. / 3.);
+  } else {
+    const double r_mc = (3. - sin(phi)) / (6 * cos(phi));
+    double e_val = (3. - sin(phi)) / (3. + sin(phi));
+    if (e_val <= 0.5) e_val = 0.5 + 1.E-10;
+    if (e_val > 1.) e_val = 1.;
+    double sqpart = (4. * (1 - e_val * e_val) * sq(cos(theta))) + (5 * e_val * e_val) - (4. * e_val);
+    if (sqpart < kDblEps) sqpart = 1.E-5;
+    double mm = (2. * (1 - e_val * e_val) * cos(theta)) + ((2. * e_val - 1) * sqrt(sqpart));
+    if (fabs(mm) < kDblEps) mm = 1.E-5;
+    const double l = (4. * (1. - e_val * e_val) * sq(cos(theta))) + sq(2. * e_val - 1.);
+    const double r_mw = (l / mm) * r_mc;
+    const double xi = 0.1;
+    double omega = sq(xi * m.c_peak * tan(psi)) + sq(r_mw * sqrt(1.5) * rho);
+    if (omega < kDblEps) omega = 1.E-5;
+    const double dl_dtheta = -8. * (1. - e_val * e_val) * cos(theta) * sin(theta);
+    const double dm_dtheta = (-2. * (1. - e_val * e_val) * sin(theta)) +
+                             (0.5 * (2. * e_val - 1.) * dl_dtheta) / sqrt(sqpart);
+    const double drmw_dtheta = ((mm * dl_dtheta) - (l * dm_dtheta)) / (mm * mm);
+    const double dp_depsilon = tan(psi) / sqrt(3.);
+    const double dp_drho = 3. * rho * r_mw * r_mw / (2. * sqrt(omega));
+    const double dp_dtheta = (3. * rho * rho * r_mw * r_mc * drmw_dtheta) / (2. * sqrt(omega));
+    for (int i = 0; i < 6; ++i)
+      dp[i] = ((dp_depsilon * depsilon_dsigma[i]) + (dp_drho * drho_dsigma[i])) +
+              (dp_dtheta * dtheta_dsigma[i]);
+    *dp_dq = dp_drho * sqrt(2. / 3.);
+  }
+  *softening = 0.;
+  if (m.softening && pdstrain > m.pd_peak && pdstrain < m.pd_res) {
+    const double dphi_dpstrain = (m.phi_res - m.phi_peak) / (m.pd_res - m.pd_peak);
+    const double dc_dpstrain = (m.c_res - m.c_peak) / (m.pd_res - m.pd_peak);
+    const double df_dphi = sqrt(1.5) * rho *
+                               ((sin(phi) * sin(theta + M_PI / 3.) /
+                                 (sqrt(3.) * cos(phi) * cos(phi))) +
+                                (cos(theta + M_PI / 3.) / (3. * cos(phi) * cos(phi)))) +
+                           (mu_p(stress) / (cos(phi) * cos(phi)));
+    const double df_dc = -1.;
+    *softening = (-1.) * ((df_dphi * dphi_dpstrain) + (df_dc * dc_dpstrain)) * (*dp_dq);
+  }
+}
+
+// mohr_coulomb.tcc:348-513. Updates stress in place, sv holds the 9 state
+// variables; density is the particle's current mass density.
+MPMX_DEV void mc_stress(const Material& m, double* stress, const double* dstrain, double density,
+                        double* sv) {
+  const double pdstrain = sv[SV_PDS];
+  if (m.softening && pdstrain > m.pd_peak) {
+    if (pdstrain < m.pd_res) {
+      sv[SV_PHI] = m.phi_res + ((m.phi_peak - m.phi_res) * (pdstrain - m.pd_res) / (m.pd_peak - m.pd_res));
+      sv[SV_PSI] = m.psi_res + ((m.psi_peak - m.psi_res) * (pdstrain - m.pd_res) / (m.pd_peak - m.pd_res));
+      sv[SV_COH] = m.c_res + ((m.c_peak - m.c_res) * (pdstrain - m.pd_res) / (m.pd_peak - m.pd_res));
+    } else {
+      sv[SV_PHI] = m.phi_res;
+      sv[SV_PSI] = m.psi_res;
+      sv[SV_COH] = m.c_res;
+    }
+    const double apex = sv[SV_COH] / tan(sv[SV_PHI]);
+    if (sv[SV_TCUT] > apex) sv[SV_TCUT] = (apex > 1.0e-15 ? apex : 1.0e-15);
+  }
+  sv[SV_YIELD] = 0;
+  double trial[6];
+  {
+    double ds[6];
+    de_mul(m, dstrain, ds);
+    for (int i = 0; i < 6; ++i) trial[i] = stress[i] + ds[i];
+  }
+  if (density <= m.critical_density) {
+    sv[SV_PDS] += mu_q(trial) / 3.0 / m.shear;
+    sv[SV_YIELD] = 3;
+    for (int i = 0; i < 6; ++i) stress[i] = 0.;
+    return;
+  }
+  mc_invariants(trial, sv);
+  double yft[2];
+  int ytt = mc_yield(yft, sv);
+  if (ytt == FS_ELASTIC) {
+    sv[SV_YIELD] = 0;
+    for (int i = 0; i < 6; ++i) stress[i] = trial[i];
+    return;
+  }
+  const double Tolerance = 1E-1;
+  double softening_trial = 0., dp_dq_trial = 0.;
+  double df_t[6], dp_t[6], tmp[6];
+  mc_df_dp(m, ytt, sv, trial, df_t, dp_t, &dp_dq_trial, &softening_trial);
+  double yield_trial = 0.;
+  if (ytt == FS_TENSILE) {
+    sv[SV_YIELD] = 2;
+    yield_trial = yft[0];
+  }
+  if (ytt == FS_SHEAR) {
+    sv[SV_YIELD] = 1;
+    yield_trial = yft[1];
+  }
+  de_tmul(m, df_t, tmp);
+  double lambda_trial = yield_trial / (dot6(tmp, dp_t) + softening_trial);
+  mc_invariants(stress, sv);
+  double yf[2];
+  const int yt = mc_yield(yf, sv);
+  double yield = 1.7976931348623157e308;
+  if (yt == FS_TENSILE) yield = yf[0];
+  if (yt == FS_SHEAR) yield = yf[1];
+  double softening = 0., dp_dq = 0.;
+  double df[6], dp[6];
+  mc_df_dp(m, yt, sv, stress, df, dp, &dp_dq, &softening);
+  de_tmul(m, df, tmp);
+  const double lambda = dot6(tmp, dstrain) / (dot6(tmp, dp) + softening);
+  double upd[6];
+  for (int i = 0; i < 6; ++i) upd[i] = trial[i];
+  double dpdstrain = 0.;
+  // upd += -(lam * D * dpv): each entry sums (lam*D_ik)*dpv_k over k in order
+  auto corr = [&](double lam, const double* dpv) {
+    const double la1 = lam * m.a1, la2 = lam * m.a2, lg = lam * m.G;
+    upd[0] += -(((la1 * dpv[0] + la2 * dpv[1]) + la2 * dpv[2]));
+    upd[1] += -(((la2 * dpv[0] + la1 * dpv[1]) + la2 * dpv[2]));
+    upd[2] += -(((la2 * dpv[0] + la2 * dpv[1]) + la1 * dpv[2]));
+    upd[3] += -(lg * dpv[3]);
+    upd[4] += -(lg * dpv[4]);
+    upd[5] += -(lg * dpv[5]);
+  };
+  if (fabs(yield) < Tolerance) {
+    corr(lambda, dp);
+    dpdstrain = lambda * dp_dq;
+  } else {
+    corr(lambda_trial, dp_t);
+    dpdstrain = lambda_trial * dp_dq_trial;
+  }
+  for (int itr = 0; itr < 100; ++itr) {
+    mc_invariants(upd, sv);
+    ytt = mc_yield(yft, sv);
+    if (yft[0] < Tolerance && yft[1] < Tolerance) break;
+    mc_df_dp(m, ytt, sv, upd, df_t, dp_t, &dp_dq_trial, &softening_trial);
+    if (ytt == FS_TENSILE) yield_trial = yft[0];
+    if (ytt == FS_SHEAR) yield_trial = yft[1];
+    de_tmul(m, df_t, tmp);
+    lambda_trial = yield_trial / (dot6(tmp, dp_t) + softening_trial);
+    corr(lambda_trial, dp_t);
+    dpdstrain += lambda_trial * dp_dq_trial;
+  }
+  mc_invariants(upd, sv);
+  sv[SV_PDS] += dpdstrain;
+  for (int i = 0; i < 6; ++i) stress[i] = upd[i];
+}
+
+// ------------------------------ node update ---------------------------------
+// node.tcc:370-404 (Cartesian branch)
+template <int D>
+MPMX_DEV void apply_velocity_constraints(const NodeCons& c, double* v, double* a) {
+#pragma unroll
+  for (int k = 0; k < D; ++k)
+    if (c.vmask & (1 << k)) {
+      v[k] = c.vval[k];
+      a[k] = 0.;
+    }
+}
+
+// node.tcc:510-657 (Cartesian branches)
+template <int D>
+MPMX_DEV void apply_friction(const NodeCons& c, double* acc, const double* vel, double dt) {
+  auto sign = [](double value) { return (value > 0.) ? 1. : -1.; };
+  const int dir_n = c.fdir;
+  const double sign_n = c.fsign;
+  const double mu = c.fmu;
+  if constexpr (D == 2) {
+    const int dir_t = (D - 1) - dir_n;
+    const double acc_n = acc[dir_n];
+    double acc_t = acc[dir_t];
+    const double vel_t = vel[dir_t];
+    if ((acc_n * sign_n) > 0.0) {
+      if (vel_t != 0.0) {
+        const double vel_net = dt * acc_t + vel_t;
+        const double vel_frictional = dt * mu * fabs(acc_n);
+        if (fabs(vel_net) <= vel_frictional)
+          acc_t = -vel_t / dt;
+        else
+          acc_t -= sign(vel_net) * mu * fabs(acc_n);
+      } else {
+        if (fabs(acc_t) <= mu * fabs(acc_n))
+          acc_t = 0.0;
+        else
+          acc_t -= sign(acc_t) * mu * fabs(acc_n);
+      }
+      acc[dir_t] = acc_t;
+    }
+  } else {
+    const int t0 = dir_n == 0 ? 1 : 0;
+    const int t1 = dir_n == 2 ? 1 : 2;
+    const double acc_n = acc[dir_n];
+    double acc_t = sqrt(acc[t0] * acc[t0] + acc[t1] * acc[t1]);
+    const double vel_t = sqrt(vel[t0] * vel[t0] + vel[t1] * vel[t1]);
+    if (acc_n * sign_n > 0.0) {
+      if (vel_t != 0.0) {
+        const double vn0 = vel[t0] + acc[t0] * dt;
+        const double vn1 = vel[t1] + acc[t1] * dt;
+        const double vel_net_t = sqrt(vn0 * vn0 + vn1 * vn1);
+        const double vel_fricion = mu * fabs(acc_n) * dt;
+        if (vel_net_t <= vel_fricion) {
+          acc[t0] = -vel[t0] / dt;
+          acc[t1] = -vel[t1] / dt;
+        } else {
+          acc[t0] -= mu * fabs(acc_n) * (vn0 / vel_net_t);
+          acc[t1] -= mu * fabs(acc_n) * (vn1 / vel_net_t);
+        }
+      } else {
+        if (acc_t <= mu * fabs(acc_n)) {
+          acc[t0] = 0;
+          acc[t1] = 0;
+        } else {
+          acc_t -= mu * fabs(acc_n);
+          acc[t0] -= mu * fabs(acc_n) * (acc[t0] / acc_t);
+          acc[t1] -= mu * fabs(acc_n) * (acc[t1] / acc_t);
+        }
+      }
+    }
+  }
+}
+
+// node.tcc:236-252 compute_velocity (vel in/out, acc in/out for constraints)
+template <int D>
+MPMX_DEV void node_compute_velocity(double mass, const double* mom, double* v, double* a,
+                                    const NodeCons* c) {
+  if (mass > 1.E-16) {
+#pragma unroll
+    for (int k = 0; k < D; ++k) v[k] = mom[k] / mass;
+#pragma unroll
+    for (int k = 0; k < D; ++k)
+      if (fabs(v[k]) < 1.E-15) v[k] = 0.;
+  }
+  if (c) apply_velocity_constraints<D>(*c, v, a);
+}
+
+// node.tcc:309-346 compute_acceleration_velocity_cundall
+template <int D>
+MPMX_DEV bool node_acc_vel(double mass, const double* fext, const double* fint, double* v,
+                           double* a, const NodeCons* c, double dt, double damping) {
+  const double tolerance = 1.0E-15;
+  if (!(mass > tolerance)) return false;
+  double f[D];
+#pragma unroll
+  for (int k = 0; k < D; ++k) f[k] = fext[k] + fint[k];
+  double nrm2 = f[0] * f[0];
+#pragma unroll
+  for (int k = 1; k < D; ++k) nrm2 = nrm2 + f[k] * f[k];
+  const double dn = damping * sqrt(nrm2);
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    const double sg = (v[k] > 0.) ? 1. : ((v[k] < 0.) ? -1. : 0.);
+    a[k] = (f[k] - dn * sg) / mass;
+  }
+  if (c) {
+    if (c->fdir >= 0) apply_friction<D>(*c, a, v, dt);
+#pragma unroll
+    for (int k = 0; k < D; ++k)
+      if (c->amask & (1 << k)) a[k] = c->aval[k];
+  }
+#pragma unroll
+  for (int k = 0; k < D; ++k) v[k] += a[k] * dt;
+  if (c) apply_velocity_constraints<D>(*c, v, a);
+#pragma unroll
+  for (int k = 0; k < D; ++k)
+    if (fabs(v[k]) < tolerance) v[k] = 0.;
+#pragma unroll
+  for (int k = 0; k < D; ++k)
+    if (fabs(a[k]) < tolerance) a[k] = 0.;
+  return true;
+}
+
+}  // namespace mpmx

@@ -1,0 +1,1228 @@
+// Host side of the mpmx C-ABI: context, mesh preprocessing, particle
+// initialisation, and the per-step launch sequence (captured once into a
+// hipGraph and replayed per step).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <limits>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/mpmx.h"
+#include "mpmx_internal.h"
+#include "mpmx_kernels.h"
+
+using namespace mpmx;
+
+namespace {
+
+struct ApiError : std::runtime_error {
+  int code;
+  ApiError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHECK(expr)                                                                       \
+  do {                                                                                       \
+    hipError_t e_ = (expr);                                                                  \
+    if (e_ != hipSuccess)                                                                    \
+      throw ApiError(MPMX_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));         \
+  } while (0)
+
+template <class T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  ~DBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void alloc(size_t count) {
+    release();
+    if (count == 0) return;
+    HIPCHECK(hipMalloc((void**)&p, count * sizeof(T)));
+    n = count;
+  }
+  void upload(const T* h, size_t count, hipStream_t st, size_t offset = 0) {
+    if (count) HIPCHECK(hipMemcpyAsync(p + offset, h, count * sizeof(T), hipMemcpyHostToDevice, st));
+  }
+  void download(T* h, size_t count, hipStream_t st, size_t offset = 0) const {
+    if (count) HIPCHECK(hipMemcpyAsync(h, p + offset, count * sizeof(T), hipMemcpyDeviceToHost, st));
+  }
+};
+
+struct HostCons {
+  std::map<int, double> vel, acc;  // dir -> value (first insert wins)
+  int fdir = -1, fsign = 0;
+  double fmu = 0;
+};
+
+struct KernelTime {
+  double ms = 0;
+  int64_t launches = 0;
+};
+
+}  // namespace
+
+struct mpmx_ctx {
+  int dim = 2, nn = 4;
+  int scheme = USF;
+  int vu = FLIP;
+  double blending = 1.0;
+  unsigned flags = 0;
+  hipStream_t stream = nullptr;
+  std::string error;
+
+  // mesh (host copies for init-time geometry)
+  int64_t nnodes = 0, ncells = 0;
+  std::vector<double> node_x;  // [nnodes][dim]
+  std::vector<int> cell_nodes;  // [ncells][nn]
+  std::vector<double> cell_volume;
+  DBuf<int> d_cell_nodes, d_nb_ptr, d_nb_idx, d_node_cells;
+  DBuf<double> d_node_x, d_centroid, d_len, d_two_mean, d_dndx_c;
+
+  // nodes
+  DBuf<double> d_nmass, d_nmom, d_nvel, d_nacc, d_nfext, d_nfint;
+  DBuf<unsigned char> d_nactive;
+  DBuf<int> d_ncons;
+  DBuf<NodeCons> d_table;
+  std::map<int64_t, HostCons> cons;
+
+  // materials
+  std::vector<Material> mats;
+  DBuf<Material> d_mats;
+  bool any_mc = false;
+
+  // particles
+  int64_t np = 0;
+  DBuf<double> d_x, d_u, d_v, d_stress, d_strain, d_F, d_mass, d_vol, d_rho, d_xi, d_sv;
+  DBuf<int> d_mat, d_cell;
+
+  // step machinery
+  DBuf<int> d_skey, d_perm, d_sort_tmp, d_scan_list, d_cstart, d_cend;
+  DBuf<Status> d_status;
+  Status* h_status = nullptr;  // pinned
+  double g[3] = {0, 0, 0};
+  double ramp = 0;
+  int64_t step = 0;
+
+  // graph cache
+  hipGraphExec_t graph_exec = nullptr;
+  double graph_dt = 0, graph_damping = 0;
+  bool graph_valid = false;
+
+  // profiling
+  bool profile = false;
+  std::vector<std::string> kt_names;
+  std::map<std::string, KernelTime> kt;
+
+  ~mpmx_ctx() {
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (h_status) (void)hipHostFree(h_status);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  void invalidate_graph() {
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    graph_exec = nullptr;
+    graph_valid = false;
+  }
+
+  Mesh mesh() const {
+    Mesh M;
+    M.nnodes = nnodes;
+    M.ncells = ncells;
+    M.cell_nodes = d_cell_nodes.p;
+    M.node_x = d_node_x.p;
+    M.centroid = d_centroid.p;
+    M.len = d_len.p;
+    M.two_mean = d_two_mean.p;
+    M.dndx_c = d_dndx_c.p;
+    M.nb_ptr = d_nb_ptr.p;
+    M.nb_idx = d_nb_idx.p;
+    M.node_cells = d_node_cells.p;
+    return M;
+  }
+  Particles parts() const {
+    Particles P;
+    P.np = np;
+    P.x = d_x.p;
+    P.u = d_u.p;
+    P.v = d_v.p;
+    P.stress = d_stress.p;
+    P.strain = d_strain.p;
+    P.F = d_F.p;
+    P.mass = d_mass.p;
+    P.vol = d_vol.p;
+    P.rho = d_rho.p;
+    P.xi = d_xi.p;
+    P.sv = d_sv.p;
+    P.mat = d_mat.p;
+    P.cell = d_cell.p;
+    return P;
+  }
+  Nodes nodes() const {
+    Nodes N;
+    N.mass = d_nmass.p;
+    N.mom = d_nmom.p;
+    N.vel = d_nvel.p;
+    N.acc = d_nacc.p;
+    N.fext = d_nfext.p;
+    N.fint = d_nfint.p;
+    N.active = d_nactive.p;
+    N.cons = d_ncons.p;
+    N.table = d_table.p;
+    return N;
+  }
+};
+
+namespace {
+
+// ----------------------------- cell geometry --------------------------------
+// Same expressions and order as the reference's cell initialisation
+// (cell.tcc:33-67, 254-277), element volume (quadrilateral_element.tcc:473-497,
+// hexahedron_element.tcc:559-597) and dN/dx at the centroid.
+double norm_diff(int dim, const double* a, const double* b) {
+  double s = (a[0] - b[0]) * (a[0] - b[0]);
+  for (int k = 1; k < dim; ++k) s = s + (a[k] - b[k]) * (a[k] - b[k]);
+  return std::sqrt(s);
+}
+
+double quad_volume(const double X[8][3]) {
+  const double a = norm_diff(2, X[0], X[3]);
+  const double b = norm_diff(2, X[2], X[3]);
+  const double c = norm_diff(2, X[1], X[2]);
+  const double d = norm_diff(2, X[0], X[1]);
+  const double p = norm_diff(2, X[0], X[2]);
+  const double q = norm_diff(2, X[1], X[3]);
+  const double t = (a * a + c * c - b * b - d * d);
+  return 0.25 * std::sqrt(4 * p * p * q * q - t * t);
+}
+
+double hex_volume(const double X[8][3]) {
+  auto sub = [](const double* u, const double* w, double* r) {
+    for (int k = 0; k < 3; ++k) r[k] = u[k] - w[k];
+  };
+  auto cross = [](const double* u, const double* w, double* r) {
+    r[0] = u[1] * w[2] - u[2] * w[1];
+    r[1] = u[2] * w[0] - u[0] * w[2];
+    r[2] = u[0] * w[1] - u[1] * w[0];
+  };
+  auto dot = [](const double* u, const double* w) { return (u[0] * w[0] + u[1] * w[1]) + u[2] * w[2]; };
+  const double *a = X[7], *b = X[6], *c = X[2], *d = X[3], *e = X[4], *f = X[5], *g = X[1], *h = X[0];
+  double bd[3], ca[3], eb[3], fa[3], de[3], ha[3], cg[3], cf[3], fg[3], hf[3], hg[3], hc[3];
+  sub(b, d, bd); sub(c, a, ca); sub(e, b, eb); sub(f, a, fa); sub(d, e, de); sub(h, a, ha);
+  sub(c, g, cg); sub(c, f, cf); sub(f, g, fg); sub(h, f, hf); sub(h, g, hg); sub(h, c, hc);
+  double c1[3], c2[3], c3[3], c4[3], c5[3], c6[3], t[3], s[3];
+  cross(bd, ca, c1); cross(eb, fa, c2); cross(de, ha, c3);
+  cross(cg, cf, c4); cross(fg, hf, c5); cross(hg, hc, c6);
+  sub(a, g, t);
+  for (int k = 0; k < 3; ++k) s[k] = (c1[k] + c2[k]) + c3[k];
+  const double v1 = (1.0 / 12) * dot(t, s);
+  sub(b, g, t);
+  for (int k = 0; k < 3; ++k) s[k] = c1[k] + c4[k];
+  const double v2 = (1.0 / 12) * dot(t, s);
+  sub(e, g, t);
+  for (int k = 0; k < 3; ++k) s[k] = c2[k] + c5[k];
+  const double v3 = (1.0 / 12) * dot(t, s);
+  sub(d, g, t);
+  for (int k = 0; k < 3; ++k) s[k] = c3[k] + c6[k];
+  const double v4 = (1.0 / 12) * dot(t, s);
+  return ((v1 + v2) + v3) + v4;
+}
+
+void host_grad(int dim, const double* xi, double* g /* [nn][3] */) {
+  if (dim == 2) {
+    g[0 * 3 + 0] = -0.25 * (1 - xi[1]); g[1 * 3 + 0] = 0.25 * (1 - xi[1]);
+    g[2 * 3 + 0] = 0.25 * (1 + xi[1]);  g[3 * 3 + 0] = -0.25 * (1 + xi[1]);
+    g[0 * 3 + 1] = -0.25 * (1 - xi[0]); g[1 * 3 + 1] = -0.25 * (1 + xi[0]);
+    g[2 * 3 + 1] = 0.25 * (1 + xi[0]);  g[3 * 3 + 1] = 0.25 * (1 - xi[0]);
+    return;
+  }
+  const double m0 = 1 - xi[0], p0 = 1 + xi[0], m1 = 1 - xi[1], p1 = 1 + xi[1], m2 = 1 - xi[2],
+               p2 = 1 + xi[2];
+  const double G[8][3] = {{-0.125 * m1 * m2, -0.125 * m0 * m2, -0.125 * m0 * m1},
+                          {0.125 * m1 * m2, -0.125 * p0 * m2, -0.125 * p0 * m1},
+                          {0.125 * p1 * m2, 0.125 * p0 * m2, -0.125 * p0 * p1},
+                          {-0.125 * p1 * m2, 0.125 * m0 * m2, -0.125 * m0 * p1},
+                          {-0.125 * m1 * p2, -0.125 * m0 * p2, 0.125 * m0 * m1},
+                          {0.125 * m1 * p2, -0.125 * p0 * p2, 0.125 * p0 * m1},
+                          {0.125 * p1 * p2, 0.125 * p0 * p2, 0.125 * p0 * p1},
+                          {-0.125 * p1 * p2, 0.125 * m0 * p2, 0.125 * m0 * p1}};
+  for (int i = 0; i < 8; ++i)
+    for (int k = 0; k < 3; ++k) g[i * 3 + k] = G[i][k];
+}
+
+void host_dndx(int dim, const double* xi, const double X[8][3], double out[8][3]) {
+  const int nn = dim == 2 ? 4 : 8;
+  double g[24];
+  host_grad(dim, xi, g);
+  double J[3][3] = {{0}};
+  for (int a = 0; a < dim; ++a)
+    for (int b = 0; b < dim; ++b) {
+      double s = g[0 * 3 + a] * X[0][b];
+      for (int i = 1; i < nn; ++i) s = s + g[i * 3 + a] * X[i][b];
+      J[a][b] = s;
+    }
+  double Ji[3][3];
+  if (dim == 2) {
+    const double det = J[0][0] * J[1][1] - J[1][0] * J[0][1];
+    const double inv = 1.0 / det;
+    Ji[0][0] = J[1][1] * inv;
+    Ji[1][0] = -J[1][0] * inv;
+    Ji[0][1] = -J[0][1] * inv;
+    Ji[1][1] = J[0][0] * inv;
+  } else {
+    double c[3][3];
+    c[0][0] = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+    c[1][0] = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+    c[2][0] = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+    c[0][1] = J[2][1] * J[0][2] - J[2][2] * J[0][1];
+    c[1][1] = J[2][2] * J[0][0] - J[2][0] * J[0][2];
+    c[2][1] = J[2][0] * J[0][1] - J[2][1] * J[0][0];
+    c[0][2] = J[0][1] * J[1][2] - J[0][2] * J[1][1];
+    c[1][2] = J[0][2] * J[1][0] - J[0][0] * J[1][2];
+    c[2][2] = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+    const double det = (J[0][0] * c[0][0] + J[1][0] * c[0][1]) + J[2][0] * c[0][2];
+    const double inv = 1.0 / det;
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) Ji[i][j] = c[i][j] * inv;
+  }
+  for (int i = 0; i < nn; ++i)
+    for (int a = 0; a < dim; ++a) {
+      double s = g[i * 3 + 0] * Ji[a][0];
+      for (int b = 1; b < dim; ++b) s = s + g[i * 3 + b] * Ji[a][b];
+      out[i][a] = s;
+    }
+}
+
+void host_shapefn(int dim, const double* xi, double* N) {
+  if (dim == 2) {
+    N[0] = 0.25 * (1 - xi[0]) * (1 - xi[1]);
+    N[1] = 0.25 * (1 + xi[0]) * (1 - xi[1]);
+    N[2] = 0.25 * (1 + xi[0]) * (1 + xi[1]);
+    N[3] = 0.25 * (1 - xi[0]) * (1 + xi[1]);
+    return;
+  }
+  N[0] = 0.125 * (1 - xi[0]) * (1 - xi[1]) * (1 - xi[2]);
+  N[1] = 0.125 * (1 + xi[0]) * (1 - xi[1]) * (1 - xi[2]);
+  N[2] = 0.125 * (1 + xi[0]) * (1 + xi[1]) * (1 - xi[2]);
+  N[3] = 0.125 * (1 - xi[0]) * (1 + xi[1]) * (1 - xi[2]);
+  N[4] = 0.125 * (1 - xi[0]) * (1 - xi[1]) * (1 + xi[2]);
+  N[5] = 0.125 * (1 + xi[0]) * (1 - xi[1]) * (1 + xi[2]);
+  N[6] = 0.125 * (1 + xi[0]) * (1 + xi[1]) * (1 + xi[2]);
+  N[7] = 0.125 * (1 - xi[0]) * (1 + xi[1]) * (1 + xi[2]);
+}
+
+// ----------------------------- materials ------------------------------------
+Material make_material(const double* p) {
+  Material m;
+  std::memset(&m, 0, sizeof(m));
+  m.type = (int)p[0];
+  if (m.type != LE && m.type != MC) throw ApiError(MPMX_E_ARG, "unknown material type");
+  m.density = p[1];
+  const double E = p[2], nu = p[3];
+  if (!(m.density > 0) || !(E > 0) || !(nu > -1.0 && nu < 0.5))
+    throw ApiError(MPMX_E_ARG, "material density, youngs_modulus or poisson_ratio out of range");
+  if (m.type == LE) {
+    const double K = E / (3.0 * (1. - 2. * nu));  // linear_elastic.tcc:27
+    const double G = E / (2.0 * (1. + nu));       // linear_elastic.tcc:55
+    m.a1 = K + (4.0 / 3.0) * G;
+    m.a2 = K - (2.0 / 3.0) * G;
+    m.G = G;
+    return m;
+  }
+  const double packing = p[15] > 0. ? p[15] : 1.;     // mohr_coulomb.tcc:12-31
+  const double grain_density = m.density / packing;
+  m.critical_density = p[14] * grain_density;
+  m.softening = p[13] != 0. ? 1 : 0;
+  m.phi_peak = p[4] * M_PI / 180.;
+  m.psi_peak = p[5] * M_PI / 180.;
+  m.c_peak = p[6];
+  m.phi_res = p[7] * M_PI / 180.;
+  m.psi_res = p[8] * M_PI / 180.;
+  m.c_res = p[9];
+  m.pd_peak = p[10];
+  m.pd_res = p[11];
+  const double tcut = p[12];
+  const double K = E / (3.0 * (1. - 2. * nu));  // mohr_coulomb.tcc:75-77
+  m.shear = E / (2.0 * (1 + nu));
+  m.a1 = K + (4.0 / 3.0) * m.shear;
+  m.a2 = K - (2.0 / 3.0) * m.shear;
+  m.G = m.shear;
+  // initialise_state_variables (mohr_coulomb.tcc:85-111)
+  m.sv0[SV_YIELD] = 0;
+  m.sv0[SV_PHI] = m.phi_peak;
+  m.sv0[SV_PSI] = m.psi_peak;
+  m.sv0[SV_COH] = m.c_peak;
+  const double apex = m.c_peak / std::tan(m.phi_peak);
+  const double tc = (tcut < apex) ? tcut : apex;
+  m.sv0[SV_TCUT] = tc > 1.0e-15 ? tc : 1.0e-15;
+  return m;
+}
+
+}  // namespace
+
+// =============================== API ========================================
+#define API_BEGIN(ctx)      \
+  if (!(ctx)) return MPMX_E_ARG; \
+  try {
+#define API_END(ctx)                      \
+  return MPMX_OK;                         \
+  }                                       \
+  catch (ApiError & e) {                  \
+    (ctx)->error = e.what();              \
+    return e.code;                        \
+  }                                       \
+  catch (std::exception & e) {            \
+    (ctx)->error = e.what();              \
+    return MPMX_E_ARG;                    \
+  }
+
+namespace {
+
+void require_mesh(mpmx_ctx* c) {
+  if (c->nnodes == 0) throw ApiError(MPMX_E_STATE, "mesh is not set");
+}
+
+void upload_constraints(mpmx_ctx* c) {
+  std::vector<int> idx(c->nnodes, -1);
+  std::vector<NodeCons> table;
+  for (auto& kv : c->cons) {
+    NodeCons nc;
+    std::memset(&nc, 0, sizeof(nc));
+    nc.fdir = -1;
+    for (auto& v : kv.second.vel) {
+      nc.vmask |= 1 << v.first;
+      nc.vval[v.first] = v.second;
+    }
+    for (auto& a : kv.second.acc) {
+      nc.amask |= 1 << a.first;
+      nc.aval[a.first] = a.second;
+    }
+    if (kv.second.fdir >= 0) {
+      nc.fdir = kv.second.fdir;
+      nc.fsign = kv.second.fsign > 0 ? 1. : -1.;
+      nc.fmu = kv.second.fmu;
+    }
+    idx[kv.first] = (int)table.size();
+    table.push_back(nc);
+  }
+  c->d_ncons.alloc(c->nnodes);
+  c->d_ncons.upload(idx.data(), idx.size(), c->stream);
+  c->d_table.alloc(std::max<size_t>(table.size(), 1));
+  c->d_table.upload(table.data(), table.size(), c->stream);
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  c->invalidate_graph();
+}
+
+// Grow every particle array to hold `n` particles, keeping the first `keep`.
+template <class T>
+void grow(DBuf<T>& b, int64_t keep, int64_t n, int comps, hipStream_t st) {
+  DBuf<T> nb;
+  nb.alloc((size_t)n * comps);
+  for (int k = 0; k < comps; ++k)
+    if (keep > 0)
+      HIPCHECK(hipMemcpyAsync(nb.p + (size_t)k * n, b.p + (size_t)k * keep, keep * sizeof(T),
+                              hipMemcpyDeviceToDevice, st));
+  HIPCHECK(hipStreamSynchronize(st));
+  std::swap(b.p, nb.p);
+  std::swap(b.n, nb.n);
+}
+
+// Upload AoS rows [n][comps] of the new particles into SoA [comps][np] at offset.
+template <class T>
+void put_soa(DBuf<T>& b, int64_t np, int64_t off, int64_t n, int comps, const T* rows,
+             hipStream_t st) {
+  std::vector<T> col(n);
+  for (int k = 0; k < comps; ++k) {
+    for (int64_t i = 0; i < n; ++i) col[i] = rows[i * comps + k];
+    b.upload(col.data(), n, st, (size_t)k * np + off);
+    HIPCHECK(hipStreamSynchronize(st));
+  }
+}
+
+template <class T>
+void get_soa(const DBuf<T>& b, int64_t np, int comps, T* rows, hipStream_t st) {
+  std::vector<T> col(np);
+  for (int k = 0; k < comps; ++k) {
+    b.download(col.data(), np, st, (size_t)k * np);
+    HIPCHECK(hipStreamSynchronize(st));
+    for (int64_t i = 0; i < np; ++i) rows[i * comps + k] = col[i];
+  }
+}
+
+void alloc_step_buffers(mpmx_ctx* c) {
+  const int64_t np = c->np;
+  c->d_skey.alloc(std::max<int64_t>(np, 1));
+  c->d_perm.alloc(std::max<int64_t>(np, 1));
+  c->d_sort_tmp.alloc(radix_sort_temp_ints(std::max<int64_t>(np, 1)));
+  c->d_scan_list.alloc(std::max<int64_t>(np, 1));
+  if (c->flags & MPMX_ORDERED) {
+    c->d_cstart.alloc(std::max<int64_t>(c->ncells, 1));
+    c->d_cend.alloc(std::max<int64_t>(c->ncells, 1));
+  }
+}
+
+void reset_status(mpmx_ctx* c) {
+  Status s;
+  s.step = c->step;
+  s.first_bad_step = std::numeric_limits<int64_t>::max();
+  s.bad_particle = -1;
+  s.nscan = 0;
+  *c->h_status = s;
+  c->d_status.upload(c->h_status, 1, c->stream);
+}
+
+void read_status(mpmx_ctx* c) {
+  c->d_status.download(c->h_status, 1, c->stream);
+  HIPCHECK(hipStreamSynchronize(c->stream));
+}
+
+// Run the initial locate of particles [off, off+n) (hint: cell per new
+// particle or -1) and fail with MPMX_E_UNLOCATABLE if any left the mesh.
+void locate_new(mpmx_ctx* c, int64_t off, int64_t n, const std::vector<int>& hint) {
+  Particles P = c->parts();
+  // operate on the sub-range by offsetting pointers: SoA strides stay np
+  DBuf<int> d_hint;
+  d_hint.alloc(std::max<int64_t>(c->np, 1));
+  std::vector<int> full(c->np, -1);
+  for (int64_t i = 0; i < n; ++i) full[off + i] = hint[i];
+  // particles before `off` keep their cells: give them their own cell as hint
+  std::vector<int> cells(c->np, -1);
+  if (off > 0) {
+    c->d_cell.download(cells.data(), off, c->stream);
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    for (int64_t i = 0; i < off; ++i) full[i] = cells[i];
+  }
+  d_hint.upload(full.data(), c->np, c->stream);
+  reset_status(c);
+  Mesh M = c->mesh();
+  if (c->dim == 2) {
+    launch_locate_init<2>(M, P, c->d_status.p, c->d_scan_list.p, d_hint.p, c->stream);
+    launch_locate_scan<2>(M, P, c->d_status.p, c->d_scan_list.p, c->stream);
+  } else {
+    launch_locate_init<3>(M, P, c->d_status.p, c->d_scan_list.p, d_hint.p, c->stream);
+    launch_locate_scan<3>(M, P, c->d_status.p, c->d_scan_list.p, c->stream);
+  }
+  HIPCHECK(hipGetLastError());
+  read_status(c);
+  if (c->h_status->first_bad_step != std::numeric_limits<int64_t>::max())
+    throw ApiError(MPMX_E_UNLOCATABLE,
+                   "Particle outside the mesh domain: " + std::to_string(c->h_status->bad_particle));
+}
+
+// Append n particles (rows given on the host); returns the offset.
+int64_t append_particles(mpmx_ctx* c, int64_t n, const double* x, const double* u, const double* v,
+                         const double* stress, const double* strain, const double* F,
+                         const double* mass, const double* volume, const double* density,
+                         const int* mat, const double* sv) {
+  const int d = c->dim;
+  const int64_t off = c->np, np = off + n;
+  if (np > (int64_t)std::numeric_limits<int>::max() / 2)
+    throw ApiError(MPMX_E_ARG, "too many particles for 32-bit particle ids");
+  for (int64_t i = 0; i < n; ++i)
+    if (mat[i] < 0 || mat[i] >= (int)c->mats.size()) throw ApiError(MPMX_E_ARG, "bad material id");
+  hipStream_t st = c->stream;
+  grow(c->d_x, off, np, d, st);
+  grow(c->d_u, off, np, d, st);
+  grow(c->d_v, off, np, d, st);
+  grow(c->d_xi, off, np, d, st);
+  grow(c->d_stress, off, np, 6, st);
+  grow(c->d_strain, off, np, 6, st);
+  grow(c->d_F, off, np, 9, st);
+  grow(c->d_mass, off, np, 1, st);
+  grow(c->d_vol, off, np, 1, st);
+  grow(c->d_rho, off, np, 1, st);
+  grow(c->d_mat, off, np, 1, st);
+  grow(c->d_cell, off, np, 1, st);
+  if (c->any_mc) grow(c->d_sv, off, np, NSV, st);
+  c->np = np;
+  std::vector<double> zeros((size_t)n * 9, 0.0), eye((size_t)n * 9, 0.0);
+  for (int64_t i = 0; i < n; ++i) eye[i * 9 + 0] = eye[i * 9 + 4] = eye[i * 9 + 8] = 1.0;
+  put_soa(c->d_x, np, off, n, d, x, st);
+  put_soa(c->d_u, np, off, n, d, u ? u : zeros.data(), st);
+  put_soa(c->d_v, np, off, n, d, v ? v : zeros.data(), st);
+  put_soa(c->d_xi, np, off, n, d, zeros.data(), st);
+  put_soa(c->d_stress, np, off, n, 6, stress ? stress : zeros.data(), st);
+  put_soa(c->d_strain, np, off, n, 6, strain ? strain : zeros.data(), st);
+  put_soa(c->d_F, np, off, n, 9, F ? F : eye.data(), st);
+  put_soa(c->d_mass, np, off, n, 1, mass, st);
+  put_soa(c->d_vol, np, off, n, 1, volume, st);
+  put_soa(c->d_rho, np, off, n, 1, density, st);
+  put_soa(c->d_mat, np, off, n, 1, mat, st);
+  std::vector<int> neg(n, -1);
+  put_soa(c->d_cell, np, off, n, 1, neg.data(), st);
+  if (c->any_mc) {
+    std::vector<double> s((size_t)n * NSV, 0.0);
+    for (int64_t i = 0; i < n; ++i) {
+      const Material& m = c->mats[mat[i]];
+      for (int k = 0; k < NSV; ++k)
+        s[i * NSV + k] = (sv && m.type == MC) ? sv[i * NSV + k] : m.sv0[k];
+    }
+    put_soa(c->d_sv, np, off, n, NSV, s.data(), st);
+  }
+  alloc_step_buffers(c);
+  c->invalidate_graph();
+  return off;
+}
+
+// Remove particles [off, np) again (after a failed initial locate).
+void truncate_particles(mpmx_ctx* c, int64_t keep) {
+  if (keep == c->np) return;
+  hipStream_t st = c->stream;
+  const int d = c->dim;
+  const int64_t old = c->np;
+  auto shrink = [&](auto& b, int comps) {
+    using T = typename std::remove_reference<decltype(*b.p)>::type;
+    DBuf<T> nb;
+    nb.alloc((size_t)std::max<int64_t>(keep, 1) * comps);
+    for (int k = 0; k < comps; ++k)
+      if (keep > 0)
+        HIPCHECK(hipMemcpyAsync(nb.p + (size_t)k * keep, b.p + (size_t)k * old, keep * sizeof(T),
+                                hipMemcpyDeviceToDevice, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    std::swap(b.p, nb.p);
+    std::swap(b.n, nb.n);
+  };
+  shrink(c->d_x, d); shrink(c->d_u, d); shrink(c->d_v, d); shrink(c->d_xi, d);
+  shrink(c->d_stress, 6); shrink(c->d_strain, 6); shrink(c->d_F, 9);
+  shrink(c->d_mass, 1); shrink(c->d_vol, 1); shrink(c->d_rho, 1);
+  shrink(c->d_mat, 1); shrink(c->d_cell, 1);
+  if (c->any_mc) shrink(c->d_sv, NSV);
+  c->np = keep;
+  alloc_step_buffers(c);
+  c->invalidate_graph();
+}
+
+// ------------------------------ step sequence --------------------------------
+struct Launcher {
+  mpmx_ctx* c;
+  std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> evs;
+  template <class F>
+  void operator()(const char* name, F&& f) {
+    if (!c->profile) {
+      f();
+      return;
+    }
+    hipEvent_t a, b;
+    HIPCHECK(hipEventCreate(&a));
+    HIPCHECK(hipEventCreate(&b));
+    HIPCHECK(hipEventRecord(a, c->stream));
+    f();
+    HIPCHECK(hipEventRecord(b, c->stream));
+    evs.push_back({name, {a, b}});
+  }
+  void collect() {
+    if (evs.empty()) return;
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    for (auto& e : evs) {
+      float ms = 0;
+      HIPCHECK(hipEventElapsedTime(&ms, e.second.first, e.second.second));
+      auto it = c->kt.find(e.first);
+      if (it == c->kt.end()) {
+        c->kt_names.push_back(e.first);
+        it = c->kt.emplace(e.first, KernelTime()).first;
+      }
+      it->second.ms += ms;
+      it->second.launches += 1;
+      (void)hipEventDestroy(e.second.first);
+      (void)hipEventDestroy(e.second.second);
+    }
+    evs.clear();
+  }
+};
+
+template <int D>
+void enqueue_step(mpmx_ctx* c, const StepArgs& A, Launcher& L) {
+  const Mesh M = c->mesh();
+  const Particles P = c->parts();
+  const Nodes Nd = c->nodes();
+  hipStream_t st = c->stream;
+  Status* S = c->d_status.p;
+  const Material* mats = c->d_mats.p;
+  int* skey = c->d_skey.p;
+  int* perm = c->d_perm.p;
+  int* scan = c->d_scan_list.p;
+  const bool ordered = (c->flags & MPMX_ORDERED) != 0;
+  L("sort_by_cell", [&] {
+    radix_sort_by_cell(P.cell, P.np, (int)M.ncells, skey, perm, c->d_sort_tmp.p, st);
+  });
+  if (ordered) {
+    L("cell_ranges", [&] {
+      launch_cell_ranges<D>(P.np, M.ncells, skey, c->d_cstart.p, c->d_cend.p, st);
+    });
+    const int* cs = c->d_cstart.p;
+    const int* ce = c->d_cend.p;
+    if (c->scheme == USF) {
+      L("gather_mass_momentum", [&] { launch_gather<D>(GK_MV_VEL, M, P, Nd, A, S, perm, cs, ce, st); });
+      L("stress", [&] { launch_stress<D>(false, M, P, Nd, mats, A, S, scan, st); });
+      L("gather_forces", [&] { launch_gather<D>(GK_F_ACC, M, P, Nd, A, S, perm, cs, ce, st); });
+      L("g2p_locate", [&] { launch_g2p<D>(false, true, M, P, Nd, mats, A, c->vu, S, scan, st); });
+    } else if (c->scheme == USL) {
+      L("gather_all", [&] { launch_gather<D>(GK_ALL_VEL_ACC, M, P, Nd, A, S, perm, cs, ce, st); });
+      L("g2p_stress_locate", [&] { launch_g2p<D>(true, true, M, P, Nd, mats, A, c->vu, S, scan, st); });
+    } else {
+      L("gather_all", [&] { launch_gather<D>(GK_ALL_VEL_ACC, M, P, Nd, A, S, perm, cs, ce, st); });
+      L("g2p", [&] { launch_g2p<D>(false, false, M, P, Nd, mats, A, c->vu, S, scan, st); });
+      L("gather_momentum", [&] { launch_gather<D>(GK_MOM_VEL, M, P, Nd, A, S, perm, cs, ce, st); });
+      L("stress_locate", [&] { launch_stress<D>(true, M, P, Nd, mats, A, S, scan, st); });
+    }
+  } else {
+    if (c->scheme == USF) {
+      L("p2g_mass_momentum", [&] { launch_p2g<D>(P2G_MV, M, P, Nd, A, S, skey, perm, st); });
+      L("node_velocity", [&] { launch_node<D>(ND_INIT | ND_VEL, M, Nd, A, st); });
+      L("stress_force_scatter", [&] { launch_stress_scatter<D>(M, P, Nd, mats, A, S, skey, perm, st); });
+      L("node_acceleration", [&] { launch_node<D>(ND_ACC | ND_ZERO_ALL, M, Nd, A, st); });
+      L("g2p_locate", [&] { launch_g2p<D>(false, true, M, P, Nd, mats, A, c->vu, S, scan, st); });
+    } else if (c->scheme == USL) {
+      L("p2g_all", [&] { launch_p2g<D>(P2G_MV_F, M, P, Nd, A, S, skey, perm, st); });
+      L("node_update", [&] { launch_node<D>(ND_INIT | ND_VEL | ND_ACC | ND_ZERO_ALL, M, Nd, A, st); });
+      L("g2p_stress_locate", [&] { launch_g2p<D>(true, true, M, P, Nd, mats, A, c->vu, S, scan, st); });
+    } else {
+      L("p2g_all", [&] { launch_p2g<D>(P2G_MV_F, M, P, Nd, A, S, skey, perm, st); });
+      L("node_update", [&] { launch_node<D>(ND_INIT | ND_VEL | ND_ACC | ND_ZERO_MOM, M, Nd, A, st); });
+      L("g2p_momentum_scatter", [&] { launch_g2p_scatter<D>(M, P, Nd, A, c->vu, skey, perm, st); });
+      L("node_velocity", [&] { launch_node<D>(ND_VEL | ND_ZERO_ALL, M, Nd, A, st); });
+      L("stress_locate", [&] { launch_stress<D>(true, M, P, Nd, mats, A, S, scan, st); });
+    }
+  }
+  L("locate_scan", [&] { launch_locate_scan<D>(M, P, S, scan, st); });
+  L("step_end", [&] { launch_step_end(S, st); });
+}
+
+void enqueue(mpmx_ctx* c, const StepArgs& A, Launcher& L) {
+  if (c->dim == 2)
+    enqueue_step<2>(c, A, L);
+  else
+    enqueue_step<3>(c, A, L);
+}
+
+}  // namespace
+
+extern "C" {
+
+int mpmx_create(mpmx_ctx** out, int dim, int scheme, int velocity_update, double blending_ratio,
+                unsigned flags) {
+  if (!out) return MPMX_E_ARG;
+  *out = nullptr;
+  if ((dim != 2 && dim != 3) || scheme < 0 || scheme > 2 || velocity_update < 0 ||
+      velocity_update > 1)
+    return MPMX_E_ARG;
+  auto* c = new mpmx_ctx();
+  c->dim = dim;
+  c->nn = dim == 2 ? 4 : 8;
+  c->scheme = scheme;
+  c->vu = velocity_update;
+  c->blending = (blending_ratio < 0. || blending_ratio > 1.) ? 1.0 : blending_ratio;
+  c->flags = flags;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+    c->error = "no HIP device available";
+    *out = c;
+    return MPMX_E_HIP;
+  }
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipHostMalloc((void**)&c->h_status, sizeof(Status)) != hipSuccess) {
+    c->error = "stream / pinned allocation failed";
+    *out = c;
+    return MPMX_E_HIP;
+  }
+  try {
+    c->d_status.alloc(1);
+  } catch (ApiError& e) {
+    c->error = e.what();
+    *out = c;
+    return e.code;
+  }
+  *out = c;
+  return MPMX_OK;
+}
+
+int mpmx_destroy(mpmx_ctx* c) {
+  if (!c) return MPMX_E_ARG;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  delete c;
+  return MPMX_OK;
+}
+
+const char* mpmx_last_error(const mpmx_ctx* c) { return c ? c->error.c_str() : "null context"; }
+
+int mpmx_set_mesh(mpmx_ctx* c, int64_t nnodes, const double* coords, int64_t ncells,
+                  const int64_t* cell_nodes, int isoparametric) {
+  API_BEGIN(c)
+  if (isoparametric)
+    throw ApiError(MPMX_E_UNSUPPORTED,
+                   "isoparametric cells (cell.tcc:617-828) are not supported; use isoparametric: false");
+  if (nnodes <= 0 || ncells <= 0 || !coords || !cell_nodes) throw ApiError(MPMX_E_ARG, "empty mesh");
+  if (nnodes > std::numeric_limits<int>::max() / 8 || ncells > std::numeric_limits<int>::max() / 8)
+    throw ApiError(MPMX_E_ARG, "mesh too large for 32-bit ids");
+  if (c->np > 0) throw ApiError(MPMX_E_STATE, "mesh must be set before particles");
+  const int d = c->dim, nn = c->nn;
+  c->nnodes = nnodes;
+  c->ncells = ncells;
+  c->node_x.assign(coords, coords + nnodes * d);
+  c->cell_nodes.resize(ncells * nn);
+  for (int64_t i = 0; i < ncells * nn; ++i) {
+    if (cell_nodes[i] < 0 || cell_nodes[i] >= nnodes) throw ApiError(MPMX_E_ARG, "cell node id out of range");
+    c->cell_nodes[i] = (int)cell_nodes[i];
+  }
+  // geometry
+  std::vector<double> centroid(d * ncells), len(d * ncells), two_mean(ncells), dndx_c(ncells * nn * d);
+  c->cell_volume.assign(ncells, 0.);
+  static const int q_sides[4][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}};
+  static const int h_sides[12][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {4, 5}, {5, 6},
+                                     {6, 7}, {7, 4}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
+  for (int64_t ci = 0; ci < ncells; ++ci) {
+    double X[8][3] = {{0}};
+    for (int i = 0; i < nn; ++i)
+      for (int k = 0; k < d; ++k) X[i][k] = c->node_x[c->cell_nodes[ci * nn + i] * d + k];
+    for (int k = 0; k < d; ++k) {
+      double s = 0.;
+      for (int i = 0; i < nn; ++i) s += X[i][k];
+      centroid[k * ncells + ci] = s / nn;
+    }
+    const int ns = d == 2 ? 4 : 12;
+    double ml = 0.;
+    for (int sdx = 0; sdx < ns; ++sdx) {
+      const int a = d == 2 ? q_sides[sdx][0] : h_sides[sdx][0];
+      const int b = d == 2 ? q_sides[sdx][1] : h_sides[sdx][1];
+      ml += norm_diff(d, X[a], X[b]);
+    }
+    ml /= ns;
+    two_mean[ci] = ml * 2.;
+    len[0 * ncells + ci] = norm_diff(d, X[0], X[1]);
+    len[1 * ncells + ci] = norm_diff(d, X[1], X[2]);
+    if (d == 3) len[2 * ncells + ci] = norm_diff(d, X[1], X[5]);
+    c->cell_volume[ci] = d == 2 ? quad_volume(X) : hex_volume(X);
+    if (!(c->cell_volume[ci] > 0))
+      throw ApiError(MPMX_E_ARG, "Negative or zero volume cell, misconfigured cell: " + std::to_string(ci));
+    const double zero[3] = {0, 0, 0};
+    double dc[8][3];
+    host_dndx(d, zero, X, dc);
+    for (int i = 0; i < nn; ++i)
+      for (int k = 0; k < d; ++k) dndx_c[(ci * nn + i) * d + k] = dc[i][k];
+  }
+  // neighbours: cells sharing a node (mesh.tcc:287-307), ascending ids
+  std::vector<std::vector<int>> node_cells(nnodes);
+  for (int64_t ci = 0; ci < ncells; ++ci)
+    for (int i = 0; i < nn; ++i) node_cells[c->cell_nodes[ci * nn + i]].push_back((int)ci);
+  std::vector<int> nb_ptr(ncells + 1, 0), nb_idx;
+  std::vector<int> tmp;
+  for (int64_t ci = 0; ci < ncells; ++ci) {
+    tmp.clear();
+    for (int i = 0; i < nn; ++i)
+      for (int o : node_cells[c->cell_nodes[ci * nn + i]])
+        if (o != ci) tmp.push_back(o);
+    std::sort(tmp.begin(), tmp.end());
+    tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
+    nb_idx.insert(nb_idx.end(), tmp.begin(), tmp.end());
+    nb_ptr[ci + 1] = (int)nb_idx.size();
+  }
+  std::vector<int> ncell_tab(nnodes * nn, -1);
+  for (int64_t n = 0; n < nnodes; ++n) {
+    auto& v = node_cells[n];
+    std::sort(v.begin(), v.end());
+    v.erase(std::unique(v.begin(), v.end()), v.end());
+    if ((int)v.size() > nn) throw ApiError(MPMX_E_ARG, "node shared by more than 2^dim cells");
+    for (size_t j = 0; j < v.size(); ++j) ncell_tab[n * nn + j] = v[j];
+  }
+  // node coordinates SoA
+  std::vector<double> nx(d * nnodes);
+  for (int64_t n = 0; n < nnodes; ++n)
+    for (int k = 0; k < d; ++k) nx[k * nnodes + n] = c->node_x[n * d + k];
+  hipStream_t st = c->stream;
+  c->d_cell_nodes.alloc(ncells * nn);
+  c->d_cell_nodes.upload(c->cell_nodes.data(), ncells * nn, st);
+  c->d_node_x.alloc(d * nnodes);
+  c->d_node_x.upload(nx.data(), nx.size(), st);
+  c->d_centroid.alloc(d * ncells);
+  c->d_centroid.upload(centroid.data(), centroid.size(), st);
+  c->d_len.alloc(d * ncells);
+  c->d_len.upload(len.data(), len.size(), st);
+  c->d_two_mean.alloc(ncells);
+  c->d_two_mean.upload(two_mean.data(), two_mean.size(), st);
+  c->d_dndx_c.alloc(ncells * nn * d);
+  c->d_dndx_c.upload(dndx_c.data(), dndx_c.size(), st);
+  c->d_nb_ptr.alloc(ncells + 1);
+  c->d_nb_ptr.upload(nb_ptr.data(), nb_ptr.size(), st);
+  c->d_nb_idx.alloc(std::max<size_t>(nb_idx.size(), 1));
+  c->d_nb_idx.upload(nb_idx.data(), nb_idx.size(), st);
+  c->d_node_cells.alloc(nnodes * nn);
+  c->d_node_cells.upload(ncell_tab.data(), ncell_tab.size(), st);
+  // nodes (zeroed)
+  auto zero_alloc = [&](DBuf<double>& b, int64_t n) {
+    b.alloc(n);
+    HIPCHECK(hipMemsetAsync(b.p, 0, n * sizeof(double), st));
+  };
+  zero_alloc(c->d_nmass, nnodes);
+  zero_alloc(c->d_nmom, d * nnodes);
+  zero_alloc(c->d_nvel, d * nnodes);
+  zero_alloc(c->d_nacc, d * nnodes);
+  zero_alloc(c->d_nfext, d * nnodes);
+  zero_alloc(c->d_nfint, d * nnodes);
+  c->d_nactive.alloc(nnodes);
+  HIPCHECK(hipMemsetAsync(c->d_nactive.p, 0, nnodes, st));
+  HIPCHECK(hipStreamSynchronize(st));
+  c->cons.clear();
+  upload_constraints(c);
+  alloc_step_buffers(c);
+  c->invalidate_graph();
+  API_END(c)
+}
+
+int mpmx_set_materials(mpmx_ctx* c, int n, const double* params) {
+  API_BEGIN(c)
+  if (n <= 0 || !params) throw ApiError(MPMX_E_ARG, "no materials");
+  if (c->np > 0) throw ApiError(MPMX_E_STATE, "materials must be set before particles");
+  c->mats.clear();
+  c->any_mc = false;
+  for (int i = 0; i < n; ++i) {
+    c->mats.push_back(make_material(params + (size_t)i * MPMX_MATERIAL_PARAMS));
+    if (c->mats.back().type == MC) c->any_mc = true;
+  }
+  c->d_mats.alloc(n);
+  c->d_mats.upload(c->mats.data(), n, c->stream);
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  c->invalidate_graph();
+  API_END(c)
+}
+
+int mpmx_set_velocity_constraints(mpmx_ctx* c, int64_t n, const int64_t* node, const int* dir,
+                                  const double* value) {
+  API_BEGIN(c)
+  require_mesh(c);
+  for (int64_t i = 0; i < n; ++i) {
+    if (node[i] < 0 || node[i] >= c->nnodes || dir[i] < 0 || dir[i] >= c->dim)
+      throw ApiError(MPMX_E_ARG, "velocity constraint node/dir out of range");
+    c->cons[node[i]].vel.insert({dir[i], value[i]});
+  }
+  upload_constraints(c);
+  API_END(c)
+}
+
+int mpmx_set_acceleration_constraints(mpmx_ctx* c, int64_t n, const int64_t* node, const int* dir,
+                                      const double* value) {
+  API_BEGIN(c)
+  require_mesh(c);
+  for (int64_t i = 0; i < n; ++i) {
+    if (node[i] < 0 || node[i] >= c->nnodes || dir[i] < 0 || dir[i] >= c->dim)
+      throw ApiError(MPMX_E_ARG, "acceleration constraint node/dir out of range");
+    auto& hc = c->cons[node[i]];
+    if (hc.vel.count(dir[i]))
+      throw ApiError(MPMX_E_ARG,
+                     "Acceleration and velocity constraints set in the same direction for the same node");
+    hc.acc.insert({dir[i], value[i]});
+  }
+  upload_constraints(c);
+  API_END(c)
+}
+
+int mpmx_set_friction_constraints(mpmx_ctx* c, int64_t n, const int64_t* node, const int* dir,
+                                  const int* sign_n, const double* mu) {
+  API_BEGIN(c)
+  require_mesh(c);
+  for (int64_t i = 0; i < n; ++i) {
+    if (node[i] < 0 || node[i] >= c->nnodes || dir[i] < 0 || dir[i] >= c->dim)
+      throw ApiError(MPMX_E_ARG, "friction constraint node/dir out of range");
+    auto& hc = c->cons[node[i]];
+    hc.fdir = dir[i];
+    hc.fsign = sign_n[i];
+    hc.fmu = mu[i];
+  }
+  upload_constraints(c);
+  API_END(c)
+}
+
+int mpmx_set_gravity(mpmx_ctx* c, const double* g, double ramping_time) {
+  API_BEGIN(c)
+  if (!g) throw ApiError(MPMX_E_ARG, "null gravity");
+  if (ramping_time < 0) throw ApiError(MPMX_E_ARG, "Ramping time cannot be negative");
+  for (int k = 0; k < 3; ++k) c->g[k] = k < c->dim ? g[k] : 0.;
+  c->ramp = ramping_time;
+  c->invalidate_graph();
+  API_END(c)
+}
+
+int mpmx_set_particles(mpmx_ctx* c, int64_t n, const double* x, const double* u, const double* v,
+                       const double* stress, const double* strain, const double* F,
+                       const double* mass, const double* volume, const double* density,
+                       const int* material_id, const int64_t* cell, const double* statevars) {
+  API_BEGIN(c)
+  require_mesh(c);
+  if (c->mats.empty()) throw ApiError(MPMX_E_STATE, "materials are not set");
+  if (n <= 0 || !x || !mass || !volume || !density || !material_id)
+    throw ApiError(MPMX_E_ARG, "particles need x, mass, volume, density and material_id");
+  for (int64_t i = 0; i < n; ++i)
+    if (!(volume[i] > 0)) throw ApiError(MPMX_E_ARG, "Particle volume cannot be negative");
+  const int64_t off = append_particles(c, n, x, u, v, stress, strain, F, mass, volume, density,
+                                       material_id, statevars);
+  std::vector<int> hint(n, -1);
+  if (cell)
+    for (int64_t i = 0; i < n; ++i) hint[i] = (cell[i] >= 0 && cell[i] < c->ncells) ? (int)cell[i] : -1;
+  try {
+    locate_new(c, off, n, hint);
+  } catch (...) {
+    truncate_particles(c, off);
+    throw;
+  }
+  API_END(c)
+}
+
+int mpmx_generate_gauss(mpmx_ctx* c, int nppd, int64_t nset, const int64_t* cset, int material_id) {
+  API_BEGIN(c)
+  require_mesh(c);
+  if (material_id < 0 || material_id >= (int)c->mats.size()) throw ApiError(MPMX_E_ARG, "bad material id");
+  const int d = c->dim, nn = c->nn;
+  double qp[8][3] = {{0}};
+  int nq = 0;
+  const double g = 1. / std::sqrt(3.);
+  if (nppd == 1) {
+    nq = 1;
+  } else if (nppd == 2) {
+    if (d == 2) {  // quadrilateral_quadrature.tcc:22-38
+      const double s[4][2] = {{-g, -g}, {g, -g}, {g, g}, {-g, g}};
+      nq = 4;
+      for (int i = 0; i < 4; ++i) { qp[i][0] = s[i][0]; qp[i][1] = s[i][1]; }
+    } else {  // hexahedron_quadrature.tcc:21-57
+      const double s[8][3] = {{-g, -g, -g}, {g, -g, -g}, {g, g, -g}, {-g, g, -g},
+                              {-g, -g, g},  {g, -g, g},  {g, g, g},  {-g, g, g}};
+      nq = 8;
+      for (int i = 0; i < 8; ++i)
+        for (int k = 0; k < 3; ++k) qp[i][k] = s[i][k];
+    }
+  } else {
+    throw ApiError(MPMX_E_UNSUPPORTED, "gauss generator supports nparticles_per_dir 1 or 2");
+  }
+  const int64_t ncs = nset < 0 ? c->ncells : nset;
+  std::vector<double> xs;
+  std::vector<int> hint;
+  xs.reserve(ncs * nq * d);
+  for (int64_t ic = 0; ic < ncs; ++ic) {
+    const int64_t ci = nset < 0 ? ic : cset[ic];
+    if (ci < 0 || ci >= c->ncells) throw ApiError(MPMX_E_ARG, "cell set id out of range");
+    for (int q = 0; q < nq; ++q) {
+      double N[8];
+      host_shapefn(d, qp[q], N);
+      for (int k = 0; k < d; ++k) {
+        double s = c->node_x[c->cell_nodes[ci * nn + 0] * d + k] * N[0];
+        for (int i = 1; i < nn; ++i) s = s + c->node_x[c->cell_nodes[ci * nn + i] * d + k] * N[i];
+        xs.push_back(s);
+      }
+      hint.push_back((int)ci);
+    }
+  }
+  const int64_t n = (int64_t)hint.size();
+  if (n == 0) throw ApiError(MPMX_E_ARG, "No particles were generated!");
+  const double rho = c->mats[material_id].density;
+  std::vector<double> ones(n, 1.0), dens(n, rho);
+  std::vector<int> mat(n, material_id);
+  const int64_t off = append_particles(c, n, xs.data(), nullptr, nullptr, nullptr, nullptr, nullptr,
+                                       ones.data(), ones.data(), dens.data(), mat.data(), nullptr);
+  try {
+    locate_new(c, off, n, hint);
+  } catch (...) {
+    truncate_particles(c, off);
+    throw;
+  }
+  // compute_volume (cell volume / particles in the cell) then compute_mass
+  std::vector<int> cells(c->np);
+  c->d_cell.download(cells.data(), c->np, c->stream);
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  std::vector<int64_t> count(c->ncells, 0);
+  for (int64_t i = 0; i < c->np; ++i) count[cells[i]] += 1;
+  std::vector<double> vol(n), mass(n);
+  for (int64_t i = 0; i < n; ++i) {
+    const int ci = cells[off + i];
+    vol[i] = c->cell_volume[ci] / (double)count[ci];
+    mass[i] = vol[i] * rho;
+  }
+  c->d_vol.upload(vol.data(), n, c->stream, off);
+  c->d_mass.upload(mass.data(), n, c->stream, off);
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  API_END(c)
+}
+
+int mpmx_assign_velocity(mpmx_ctx* c, int64_t n, const double* v) {
+  API_BEGIN(c)
+  if (n != c->np || !v) throw ApiError(MPMX_E_ARG, "assign_velocity needs one row per particle");
+  put_soa(c->d_v, c->np, 0, n, c->dim, v, c->stream);
+  API_END(c)
+}
+
+int mpmx_assign_stresses(mpmx_ctx* c, int64_t n, const double* stress) {
+  API_BEGIN(c)
+  if (n != c->np || !stress)
+    throw ApiError(MPMX_E_ARG, "Number of particles in mesh and initial stresses don't match");
+  put_soa(c->d_stress, c->np, 0, n, 6, stress, c->stream);
+  API_END(c)
+}
+
+int mpmx_nparticles(const mpmx_ctx* c, int64_t* n) {
+  if (!c || !n) return MPMX_E_ARG;
+  *n = c->np;
+  return MPMX_OK;
+}
+
+int mpmx_get_particles(mpmx_ctx* c, double* x, double* u, double* v, double* stress, double* strain,
+                       double* F, double* mass, double* volume, double* density, int* material_id,
+                       int64_t* cell, double* xi, double* statevars) {
+  API_BEGIN(c)
+  const int64_t np = c->np;
+  const int d = c->dim;
+  hipStream_t st = c->stream;
+  HIPCHECK(hipStreamSynchronize(st));
+  if (np == 0) return MPMX_OK;
+  if (x) get_soa(c->d_x, np, d, x, st);
+  if (u) get_soa(c->d_u, np, d, u, st);
+  if (v) get_soa(c->d_v, np, d, v, st);
+  if (stress) get_soa(c->d_stress, np, 6, stress, st);
+  if (strain) get_soa(c->d_strain, np, 6, strain, st);
+  if (F) get_soa(c->d_F, np, 9, F, st);
+  if (mass) get_soa(c->d_mass, np, 1, mass, st);
+  if (volume) get_soa(c->d_vol, np, 1, volume, st);
+  if (density) get_soa(c->d_rho, np, 1, density, st);
+  if (material_id) get_soa(c->d_mat, np, 1, material_id, st);
+  if (xi) get_soa(c->d_xi, np, d, xi, st);
+  if (cell) {
+    std::vector<int> cc(np);
+    get_soa(c->d_cell, np, 1, cc.data(), st);
+    for (int64_t i = 0; i < np; ++i) cell[i] = cc[i];
+  }
+  if (statevars) {
+    if (c->any_mc)
+      get_soa(c->d_sv, np, NSV, statevars, st);
+    else
+      std::fill(statevars, statevars + np * NSV, 0.0);
+  }
+  API_END(c)
+}
+
+int mpmx_get_nodes(mpmx_ctx* c, double* mass, double* velocity, double* acceleration, int* active) {
+  API_BEGIN(c)
+  require_mesh(c);
+  const int64_t nn = c->nnodes;
+  const int d = c->dim;
+  hipStream_t st = c->stream;
+  std::vector<unsigned char> act(nn);
+  c->d_nactive.download(act.data(), nn, st);
+  HIPCHECK(hipStreamSynchronize(st));
+  if (active)
+    for (int64_t i = 0; i < nn; ++i) active[i] = act[i];
+  if (mass) {
+    c->d_nmass.download(mass, nn, st);
+    HIPCHECK(hipStreamSynchronize(st));
+  }
+  if (velocity) get_soa(c->d_nvel, nn, d, velocity, st);
+  if (acceleration) get_soa(c->d_nacc, nn, d, acceleration, st);
+  API_END(c)
+}
+
+int mpmx_step(mpmx_ctx* c, int nsteps, double dt, double damping_factor) {
+  API_BEGIN(c)
+  require_mesh(c);
+  if (c->mats.empty()) throw ApiError(MPMX_E_STATE, "materials are not set");
+  if (nsteps < 0 || !(dt > 0)) throw ApiError(MPMX_E_ARG, "nsteps >= 0 and dt > 0 required");
+  if (nsteps == 0 || c->np == 0) {
+    c->step += nsteps;
+    return MPMX_OK;
+  }
+  StepArgs A;
+  A.dt = dt;
+  A.damping = damping_factor;
+  A.blending = c->blending;
+  for (int k = 0; k < 3; ++k) A.g[k] = c->g[k];
+  A.ramp = c->ramp;
+  reset_status(c);
+  Launcher L{c, {}};
+  const bool use_graph = !(c->flags & MPMX_NO_GRAPH) && !c->profile;
+  if (use_graph) {
+    if (!c->graph_valid || c->graph_dt != dt || c->graph_damping != damping_factor) {
+      c->invalidate_graph();
+      hipGraph_t graph;
+      HIPCHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      enqueue(c, A, L);
+      HIPCHECK(hipStreamEndCapture(c->stream, &graph));
+      hipError_t e = hipGraphInstantiate(&c->graph_exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      HIPCHECK(e);
+      c->graph_valid = true;
+      c->graph_dt = dt;
+      c->graph_damping = damping_factor;
+    }
+    for (int s = 0; s < nsteps; ++s) HIPCHECK(hipGraphLaunch(c->graph_exec, c->stream));
+  } else {
+    for (int s = 0; s < nsteps; ++s) {
+      enqueue(c, A, L);
+      HIPCHECK(hipGetLastError());
+      L.collect();
+    }
+  }
+  read_status(c);
+  c->step = c->h_status->step;
+  if (c->h_status->first_bad_step != std::numeric_limits<int64_t>::max())
+    throw ApiError(MPMX_E_UNLOCATABLE,
+                   "Particle(s) outside the mesh domain: " + std::to_string(c->h_status->bad_particle) +
+                       " (step " + std::to_string(c->h_status->first_bad_step) + ")");
+  API_END(c)
+}
+
+int mpmx_set_step(mpmx_ctx* c, int64_t step) {
+  API_BEGIN(c)
+  if (step < 0) throw ApiError(MPMX_E_ARG, "negative step");
+  c->step = step;
+  API_END(c)
+}
+
+int mpmx_get_step(const mpmx_ctx* c, int64_t* step) {
+  if (!c || !step) return MPMX_E_ARG;
+  *step = c->step;
+  return MPMX_OK;
+}
+
+int mpmx_synchronize(mpmx_ctx* c) {
+  API_BEGIN(c)
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  API_END(c)
+}
+
+int mpmx_profile(mpmx_ctx* c, int enable) {
+  API_BEGIN(c)
+  c->profile = enable != 0;
+  if (c->profile) {
+    c->kt.clear();
+    c->kt_names.clear();
+  }
+  API_END(c)
+}
+
+int mpmx_kernel_times(mpmx_ctx* c, int cap, char* names, double* total_ms, int64_t* launches,
+                      int* n) {
+  API_BEGIN(c)
+  const int cnt = (int)c->kt_names.size();
+  if (n) *n = cnt;
+  for (int i = 0; i < cnt && i < cap; ++i) {
+    const auto& nm = c->kt_names[i];
+    if (names) {
+      std::memset(names + 48 * i, 0, 48);
+      std::strncpy(names + 48 * i, nm.c_str(), 47);
+    }
+    if (total_ms) total_ms[i] = c->kt[nm].ms;
+    if (launches) launches[i] = c->kt[nm].launches;
+  }
+  API_END(c)
+}
+
+}  // extern "C"

@@ -1,0 +1,99 @@
+// Internal device-side data layout of the mpmx library (shared by the host
+// driver and the HIP kernels). Everything is structure-of-arrays in HBM:
+// component k of a per-particle vector lives at base + k * np, so a wave of
+// 64 consecutive particles reads 512 contiguous bytes per component.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mpmx {
+
+enum { LE = 0, MC = 1 };
+enum { SV_YIELD = 0, SV_PHI, SV_PSI, SV_COH, SV_TCUT, SV_EPS, SV_RHO, SV_THETA, SV_PDS, NSV };
+enum { USF = 0, USL = 1, MUSL = 2 };
+enum { FLIP = 0, PIC = 1 };
+
+// Material constants, derived once on the host with the reference's
+// expressions (linear_elastic.tcc:20-69, mohr_coulomb.tcc:9-77, 517-533).
+struct Material {
+  int type;
+  int softening;
+  double density;
+  double a1, a2, G;  // elastic tensor entries: D = [a1 a2 a2; a2 a1 a2; a2 a2 a1] (+) G*I3
+  double shear;      // MohrCoulomb shear_modulus_ (pdstrain of separated state)
+  double phi_peak, psi_peak, c_peak, phi_res, psi_res, c_res, pd_peak, pd_res;
+  double critical_density;  // minimum_packing_fraction * grain_density
+  double sv0[NSV];          // initial state variables
+};
+
+// Nodal constraint record, referenced by Nodes::cons[node] (-1: none).
+struct NodeCons {
+  int vmask;     // bit k: velocity constraint in direction k
+  int amask;     // bit k: acceleration constraint in direction k
+  int fdir;      // friction normal direction, -1: none
+  double fsign;  // sign(sign_n): +1 if sign_n > 0 else -1 (node.tcc:514)
+  double fmu;
+  double vval[3], aval[3];
+};
+
+struct Mesh {
+  int64_t nnodes, ncells;
+  const int* cell_nodes;     // [ncells * NN]
+  const double* node_x;      // SoA [dim][nnodes]
+  const double* centroid;    // SoA [dim][ncells]
+  const double* len;         // SoA [dim][ncells] side lengths used by the local coordinates
+  const double* two_mean;    // [ncells] mean_length * 2.
+  const double* dndx_c;      // [ncells][NN][dim] dN/dx at the cell centroid
+  const int* nb_ptr;         // neighbour CSR (ascending ids)
+  const int* nb_idx;
+  const int* node_cells;     // [nnodes][NN] adjacent cells (-1 pad), ascending
+};
+
+struct Particles {
+  int64_t np;
+  double* x;  // [dim][np]
+  double* u;
+  double* v;
+  double* stress;  // [6][np]
+  double* strain;  // [6][np]
+  double* F;       // [9][np] row-major
+  double* mass;
+  double* vol;
+  double* rho;
+  double* xi;      // [dim][np]
+  double* sv;      // [NSV][np] or null when no MohrCoulomb material
+  int* mat;
+  int* cell;       // current cell (-1: none)
+};
+
+struct Nodes {
+  double* mass;   // [nnodes]
+  double* mom;    // [dim][nnodes]
+  double* vel;
+  double* acc;
+  double* fext;
+  double* fint;
+  unsigned char* active;
+  const int* cons;  // [nnodes] index into constraint table or -1
+  const NodeCons* table;
+};
+
+// Per-step scalars
+struct StepArgs {
+  double dt;
+  double damping;
+  double blending;
+  double g[3];
+  double ramp;  // gravity ramping time (<= 0: none)
+};
+
+// Device-side status written by kernels and read after a run of steps.
+struct Status {
+  int64_t step;           // step counter (drives the gravity ramp)
+  int64_t first_bad_step;  // first step with an unlocatable particle (INT64_MAX: none)
+  int bad_particle;        // one such particle (pid)
+  int nscan;               // particles needing the full-mesh scan this step
+};
+
+}  // namespace mpmx

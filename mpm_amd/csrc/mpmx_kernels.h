@@ -1,0 +1,60 @@
+// Launchers of the step kernels (defined in mpmx_kernels.hip and mpmx_sort.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "mpmx_internal.h"
+
+namespace mpmx {
+
+// node-kernel flags
+enum { ND_VEL = 1, ND_ACC = 2, ND_ZERO_MOM = 4, ND_ZERO_ALL = 8, ND_INIT = 16 };
+// scatter contents
+enum { P2G_MV = 1, P2G_MV_F = 3 };
+// ordered-mode gathers
+enum { GK_MV_VEL = 0, GK_F_ACC = 1, GK_ALL_VEL_ACC = 2, GK_MOM_VEL = 3 };
+
+template <int D>
+void launch_locate_init(const Mesh& M, const Particles& P, Status* S, int* scan_list,
+                        const int* hint, hipStream_t st);
+template <int D>
+void launch_locate_scan(const Mesh& M, const Particles& P, Status* S, const int* scan_list,
+                        hipStream_t st);
+void launch_step_end(Status* S, hipStream_t st);
+template <int D>
+void launch_p2g(int what, const Mesh& M, const Particles& P, const Nodes& Nd, const StepArgs& A,
+                const Status* S, const int* skey, const int* perm, hipStream_t st);
+template <int D>
+void launch_stress_scatter(const Mesh& M, const Particles& P, const Nodes& Nd,
+                           const Material* mats, const StepArgs& A, const Status* S,
+                           const int* skey, const int* perm, hipStream_t st);
+template <int D>
+void launch_stress(bool locate, const Mesh& M, const Particles& P, const Nodes& Nd,
+                   const Material* mats, const StepArgs& A, Status* S, int* scan_list,
+                   hipStream_t st);
+template <int D>
+void launch_g2p(bool stress, bool locate, const Mesh& M, const Particles& P, const Nodes& Nd,
+                const Material* mats, const StepArgs& A, int vu, Status* S, int* scan_list,
+                hipStream_t st);
+template <int D>
+void launch_g2p_scatter(const Mesh& M, const Particles& P, const Nodes& Nd, const StepArgs& A,
+                        int vu, const int* skey, const int* perm, hipStream_t st);
+template <int D>
+void launch_node(int fl, const Mesh& M, const Nodes& Nd, const StepArgs& A, hipStream_t st);
+template <int D>
+void launch_cell_ranges(int64_t np, int64_t ncells, const int* skey, int* cstart, int* cend,
+                        hipStream_t st);
+template <int D>
+void launch_gather(int which, const Mesh& M, const Particles& P, const Nodes& Nd,
+                   const StepArgs& A, const Status* S, const int* perm, const int* cstart,
+                   const int* cend, hipStream_t st);
+
+// Stable LSD radix sort of (cell key, particle id) pairs. keys_in: cell of
+// each particle in particle order (values are the particle ids 0..n-1).
+// Sorted keys/values end in keys_out/vals_out. Temporary buffers are sized by
+// radix_sort_temp_ints(n).
+int64_t radix_sort_temp_ints(int64_t n);
+void radix_sort_by_cell(const int* cell, int64_t n, int max_key, int* keys_out, int* vals_out,
+                        int* tmp, hipStream_t st);
+
+}  // namespace mpmx

@@ -1,0 +1,991 @@
+// HIP kernels of the explicit MPM step for gfx950 (MI355X).
+//
+// Particle state is SoA in particle (reference) order. Every step the
+// particles are radix-sorted by cell (stable, so each cell keeps ascending
+// particle ids); scatter kernels walk that order in 256-particle chunks, stage
+// per-particle data in LDS, reduce each cell segment's contributions per
+// local node in particle order, and add the segment sums to the nodes with
+// FP64 global atomics (one atomic per cell segment, node and component
+// instead of one per particle). In ordered mode the nodes instead gather:
+// each node merges its cells' particle lists by particle id and sums in that
+// order, which is the single-threaded reference's order.
+#include <hip/hip_runtime.h>
+
+#include "mpmx_device.h"
+#include "mpmx_kernels.h"
+
+namespace mpmx {
+
+static constexpr int kBlock = 256;
+
+MPMX_DEV void atomic_add(double* p, double v) { unsafeAtomicAdd(p, v); }
+
+template <int D>
+MPMX_DEV bool valid_cell(const Mesh& M, int c) {
+  return c >= 0 && (int64_t)c < M.ncells;
+}
+
+// ------------------------- per-particle building blocks ----------------------
+// Nodal velocity (and acceleration) of the particle's cell nodes.
+template <int D>
+MPMX_DEV void gather_nodes(const Mesh& M, const Nodes& Nd, int c, double (*nv)[D],
+                           double (*na)[D]) {
+  constexpr int NN = Elem<D>::NN;
+#pragma unroll
+  for (int i = 0; i < NN; ++i) {
+    const int n = M.cell_nodes[(int64_t)c * NN + i];
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      nv[i][k] = Nd.vel[k * M.nnodes + n];
+      if (na) na[i][k] = Nd.acc[k * M.nnodes + n];
+    }
+  }
+}
+
+// compute_stress_strain for one particle (mpm_scheme.tcc:75-103 fused per
+// particle: ΔF, strain, volume, constitutive update, F). dndx: particle's
+// dN/dx (computed at step start from its xi). Returns new stress in `s` and
+// writes the particle's persistent fields.
+template <int D>
+MPMX_DEV void stress_strain(const Mesh& M, const Particles& P, const Material* mats, int p, int c,
+                            const double (*dndx)[D], const double (*nv)[D], double dt,
+                            double* s, double* vol_out) {
+  constexpr int NN = Elem<D>::NN;
+  const int64_t np = P.np;
+  // update_deformation_gradient_increment (particle.tcc:1578-1651)
+  double R[D][D];
+#pragma unroll
+  for (int a = 0; a < D; ++a)
+#pragma unroll
+    for (int b = 0; b < D; ++b) R[a][b] = (a == b) ? 1.0 : 0.0;
+#pragma unroll
+  for (int i = 0; i < NN; ++i)
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+#pragma unroll
+      for (int b = 0; b < D; ++b) R[a][b] = R[a][b] + (nv[i][a] * dndx[i][b]) * dt;
+#pragma unroll
+  for (int a = 0; a < D; ++a)
+#pragma unroll
+    for (int b = 0; b < D; ++b) {
+      if (a != b && fabs(R[a][b]) < 1.E-15) R[a][b] = 0.;
+      if (a == b && fabs(R[a][b] - 1.) < 1.E-15) R[a][b] = 1.;
+    }
+  // compute_strain_rate (particle.tcc:819-858) at the particle and centroid
+  double sr[6] = {0, 0, 0, 0, 0, 0}, sc[6] = {0, 0, 0, 0, 0, 0};
+  const double* dc = M.dndx_c + (int64_t)c * NN * D;
+#pragma unroll
+  for (int i = 0; i < NN; ++i) {
+    const double* v = nv[i];
+    const double* d = dndx[i];
+    const double* e = dc + i * D;
+    if constexpr (D == 2) {
+      sr[0] += d[0] * v[0];
+      sr[1] += d[1] * v[1];
+      sr[3] += d[1] * v[0] + d[0] * v[1];
+      sc[0] += e[0] * v[0];
+      sc[1] += e[1] * v[1];
+    } else {
+      sr[0] += d[0] * v[0];
+      sr[1] += d[1] * v[1];
+      sr[2] += d[2] * v[2];
+      sr[3] += d[1] * v[0] + d[0] * v[1];
+      sr[4] += d[2] * v[1] + d[1] * v[2];
+      sr[5] += d[2] * v[0] + d[0] * v[2];
+      sc[0] += e[0] * v[0];
+      sc[1] += e[1] * v[1];
+      sc[2] += e[2] * v[2];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    if (fabs(sr[k]) < 1.E-15) sr[k] = 0.;
+    if (fabs(sc[k]) < 1.E-15) sc[k] = 0.;
+  }
+  // compute_strain (particle.tcc:861-877)
+  double ds[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) ds[k] = sr[k] * dt;
+  if constexpr (D == 2) {
+    P.strain[0 * np + p] += ds[0];
+    P.strain[1 * np + p] += ds[1];
+    P.strain[3 * np + p] += ds[3];
+  } else {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) P.strain[k * np + p] += ds[k];
+  }
+  double svol = sc[0];
+#pragma unroll
+  for (int k = 1; k < D; ++k) svol = svol + sc[k];
+  const double dvol = dt * svol;
+  // update_volume (particle.tcc:598-606)
+  double vol = P.vol[p];
+  vol *= (1. + dvol);
+  const double rho = P.rho[p] / (1. + dvol);
+  P.vol[p] = vol;
+  P.rho[p] = rho;
+  *vol_out = vol;
+  // compute_stress (particle.tcc:880-890 -> material)
+#pragma unroll
+  for (int k = 0; k < 6; ++k) s[k] = P.stress[k * np + p];
+  const Material& m = mats[P.mat[p]];
+  if (m.type == LE) {
+    double t[6];
+    de_mul(m, ds, t);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) s[k] = s[k] + t[k];
+  } else {
+    double sv[NSV];
+#pragma unroll
+    for (int k = 0; k < NSV; ++k) sv[k] = P.sv[k * np + p];
+    mc_stress(m, s, ds, rho, sv);
+#pragma unroll
+    for (int k = 0; k < NSV; ++k) P.sv[k * np + p] = sv[k];
+  }
+#pragma unroll
+  for (int k = 0; k < 6; ++k) P.stress[k * np + p] = s[k];
+  // update_deformation_gradient (particle.tcc:1654-1659): rows 0..D-1 of
+  // F <- ΔF F; row 2 of a 2D ΔF is (0,0,1) and leaves F's row 2 unchanged.
+  double Fr[D][3];
+#pragma unroll
+  for (int a = 0; a < D; ++a)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) Fr[a][j] = P.F[(a * 3 + j) * np + p];
+#pragma unroll
+  for (int a = 0; a < D; ++a)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      double t = R[a][0] * Fr[0][j];
+#pragma unroll
+      for (int b = 1; b < D; ++b) t = t + R[a][b] * Fr[b][j];
+      P.F[(a * 3 + j) * np + p] = t;
+    }
+}
+
+// Particle-level dN/dx at its xi.
+template <int D>
+MPMX_DEV void particle_dndx(const Mesh& M, int c, const double* xi, double (*dndx)[D],
+                            double (*Ji)[D]) {
+  constexpr int NN = Elem<D>::NN;
+  double X[NN][D];
+  load_cell_X<D>(M, c, X);
+  jacobian_inverse<D>(xi, X, Ji);
+#pragma unroll
+  for (int i = 0; i < NN; ++i) dndx_i<D>(xi, i, Ji, dndx[i]);
+}
+
+// locate_particle_cells (mesh.tcc:1116-1153): current cell, then its
+// neighbours in ascending id; otherwise queue the particle for the full scan.
+template <int D>
+MPMX_DEV void locate_one(const Mesh& M, const Particles& P, Status* S, int* scan_list, int p,
+                         const double* x, int cur, int hint) {
+  const int64_t np = P.np;
+  double xi[D];
+  if (cur >= 0 && point_in_cell<D>(M, cur, x, xi)) {
+#pragma unroll
+    for (int k = 0; k < D; ++k) P.xi[k * np + p] = xi[k];
+    return;
+  }
+  const int base = cur >= 0 ? cur : hint;
+  if (base >= 0) {
+    const int e = M.nb_ptr[base + 1];
+    for (int j = M.nb_ptr[base]; j < e; ++j) {
+      const int nb = M.nb_idx[j];
+      if (point_in_cell<D>(M, nb, x, xi)) {
+        P.cell[p] = nb;
+#pragma unroll
+        for (int k = 0; k < D; ++k) P.xi[k * np + p] = xi[k];
+        return;
+      }
+    }
+  }
+  const int slot = atomicAdd(&S->nscan, 1);
+  scan_list[slot] = p;
+}
+
+// ------------------------------- kernels ------------------------------------
+
+// Locate every particle (initial placement). hint[p] >= 0 is a cell to try
+// first (kept only if the point lies in it), -1: none.
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_locate_init(Mesh M, Particles P, Status* S,
+                                                        int* scan_list, const int* hint) {
+  const int64_t p = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+  if (p >= P.np) return;
+  double x[D];
+#pragma unroll
+  for (int k = 0; k < D; ++k) x[k] = P.x[k * P.np + p];
+  const int h = hint ? hint[p] : -1;
+  double xi[D];
+  if (h >= 0 && point_in_cell<D>(M, h, x, xi)) {
+    P.cell[p] = h;
+#pragma unroll
+    for (int k = 0; k < D; ++k) P.xi[k * P.np + p] = xi[k];
+    return;
+  }
+  P.cell[p] = -1;
+  locate_one<D>(M, P, S, scan_list, (int)p, x, -1, h);
+}
+
+// Full-mesh scan for queued particles: first cell in ascending id order that
+// contains the point (the single-threaded reading of mesh.tcc:1142-1151).
+// Persistent grid; every block exits after the queue is drained.
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_locate_scan(Mesh M, Particles P, Status* S,
+                                                        const int* scan_list) {
+  __shared__ int best;
+  const int n = S->nscan;
+  for (int q = blockIdx.x; q < n; q += gridDim.x) {
+    const int p = scan_list[q];
+    double x[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) x[k] = P.x[k * P.np + p];
+    if (threadIdx.x == 0) best = 0x7fffffff;
+    __syncthreads();
+    for (int64_t c0 = 0; c0 < M.ncells; c0 += kBlock) {
+      const int64_t c = c0 + threadIdx.x;
+      double xi[D];
+      if (c < M.ncells && point_in_cell<D>(M, (int)c, x, xi)) atomicMin(&best, (int)c);
+      __syncthreads();
+      if (best != 0x7fffffff) break;
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      if (best != 0x7fffffff) {
+        double xi[D];
+        point_in_cell<D>(M, best, x, xi);
+        P.cell[p] = best;
+#pragma unroll
+        for (int k = 0; k < D; ++k) P.xi[k * P.np + p] = xi[k];
+      } else {
+        P.cell[p] = -1;
+        const long long step = (long long)S->step;
+        atomicMin((unsigned long long*)&S->first_bad_step, (unsigned long long)step);
+        atomicExch(&S->bad_particle, p);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// End of step: advance the counter, clear the scan queue.
+__global__ void k_step_end(Status* S) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    S->step += 1;
+    S->nscan = 0;
+  }
+}
+
+// ---------------- fast scatter: LDS-staged segmented reduction --------------
+// What the staging carries per particle.
+enum { SC_MV = 1, SC_F = 2, SC_MOM = 4 };
+
+template <int D, int WHAT>
+struct Stage {
+  static constexpr int NN = Elem<D>::NN;
+  int key[kBlock];
+  double xi[D][kBlock];
+  double m[kBlock];
+  double v[D][kBlock];
+  double s[(WHAT & SC_F) ? 6 : 1][kBlock];
+  double vol[(WHAT & SC_F) ? 1 : 1][kBlock];
+  double Ji[(WHAT & SC_F) ? D * D : 1][kBlock];
+  int seg_start[kBlock + 1];
+  int nseg;
+  int wave_heads[kBlock / 64];
+};
+
+// Phase 2 of every scatter kernel: given staged particles [0, cnt) of this
+// chunk, sorted by cell, add each cell segment's per-node sums to the nodes.
+template <int D, int WHAT>
+MPMX_DEV void scatter_segments(Stage<D, WHAT>& st, int cnt, const Mesh& M, const Nodes& Nd,
+                               const double* g) {
+  constexpr int NN = Elem<D>::NN;
+  const int tid = threadIdx.x;
+  // segment heads (valid particles only; invalid ones carry key -1 at the end)
+  const bool valid = tid < cnt && st.key[tid] >= 0;
+  const bool head = valid && (tid == 0 || st.key[tid] != st.key[tid - 1]);
+  const unsigned long long mask = __ballot(head);
+  const int lane = tid & 63, wave = tid >> 6;
+  if (lane == 0) st.wave_heads[wave] = __popcll(mask);
+  __syncthreads();
+  int before = 0;
+#pragma unroll
+  for (int w = 0; w < kBlock / 64; ++w) before += (w < wave) ? st.wave_heads[w] : 0;
+  const int rank = before + __popcll(mask & ((1ull << lane) - 1ull));
+  if (head) st.seg_start[rank] = tid;
+  if (tid == 0) {
+    int tot = 0;
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; ++w) tot += st.wave_heads[w];
+    st.nseg = tot;
+  }
+  __syncthreads();
+  const int nseg = st.nseg;
+  // number of valid staged particles = end of last segment
+  int nvalid = 0;
+  {
+    // valid particles are a prefix of the chunk (invalid keys sort last)
+    nvalid = cnt;
+    while (nvalid > 0 && st.key[nvalid - 1] < 0) --nvalid;
+  }
+  for (int task = tid; task < nseg * NN; task += kBlock) {
+    const int sidx = task / NN;
+    const int i = task - sidx * NN;
+    const int b = st.seg_start[sidx];
+    const int e = (sidx + 1 < nseg) ? st.seg_start[sidx + 1] : nvalid;
+    const int c = st.key[b];
+    double am = 0., amom[D], afe[D], afi[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) amom[k] = afe[k] = afi[k] = 0.;
+    for (int t = b; t < e; ++t) {
+      double xi[D];
+#pragma unroll
+      for (int k = 0; k < D; ++k) xi[k] = st.xi[k][t];
+      const double N = shapefn_i<D>(xi, i);
+      const double m = st.m[t];
+      if constexpr ((WHAT & SC_MV) != 0) {
+        const double mN = m * N;
+        am = am + mN;
+#pragma unroll
+        for (int k = 0; k < D; ++k) amom[k] = amom[k] + mN * st.v[k][t];
+      }
+      if constexpr ((WHAT & SC_MOM) != 0) {
+        const double mN = m * N;
+#pragma unroll
+        for (int k = 0; k < D; ++k) amom[k] = amom[k] + mN * st.v[k][t];
+      }
+      if constexpr ((WHAT & SC_F) != 0) {
+#pragma unroll
+        for (int k = 0; k < D; ++k) afe[k] = afe[k] + g[k] * m * N;
+        double Ji[D][D];
+#pragma unroll
+        for (int a = 0; a < D; ++a)
+#pragma unroll
+          for (int bb = 0; bb < D; ++bb) Ji[a][bb] = st.Ji[a * D + bb][t];
+        double d[D];
+        dndx_i<D>(xi, i, Ji, d);
+        double f[D];
+        if constexpr (D == 2) {
+          f[0] = d[0] * st.s[0][t] + d[1] * st.s[3][t];
+          f[1] = d[1] * st.s[1][t] + d[0] * st.s[3][t];
+        } else {
+          f[0] = d[0] * st.s[0][t] + d[1] * st.s[3][t] + d[2] * st.s[5][t];
+          f[1] = d[1] * st.s[1][t] + d[0] * st.s[3][t] + d[2] * st.s[4][t];
+          f[2] = d[2] * st.s[2][t] + d[1] * st.s[4][t] + d[0] * st.s[5][t];
+        }
+        const double mv = -1. * st.vol[0][t];
+#pragma unroll
+        for (int k = 0; k < D; ++k) afi[k] = afi[k] + f[k] * mv;
+      }
+    }
+    const int n = M.cell_nodes[(int64_t)c * NN + i];
+    if constexpr ((WHAT & SC_MV) != 0) {
+      atomic_add(&Nd.mass[n], am);
+      Nd.active[n] = 1;
+    }
+    if constexpr ((WHAT & (SC_MV | SC_MOM)) != 0) {
+#pragma unroll
+      for (int k = 0; k < D; ++k) atomic_add(&Nd.mom[k * M.nnodes + n], amom[k]);
+    }
+    if constexpr ((WHAT & SC_F) != 0) {
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+        atomic_add(&Nd.fext[k * M.nnodes + n], afe[k]);
+        atomic_add(&Nd.fint[k * M.nnodes + n], afi[k]);
+      }
+    }
+  }
+}
+
+MPMX_DEV void gravity_now(const StepArgs& A, const Status* S, int dim, double* g) {
+  double gm = 1.0;
+  if (A.ramp > 0.0) {
+    const double r = static_cast<double>(S->step) * A.dt / A.ramp;
+    gm = r < 1.0 ? r : 1.0;  // std::min(1.0, r)
+  }
+  for (int k = 0; k < dim; ++k) g[k] = gm * A.g[k];
+}
+
+// Mass/momentum (and, for USL/MUSL, body and internal forces from the
+// stress of the previous step) scattered from sorted particles.
+// mpm_scheme.tcc:44-71 and 131-190; particle.tcc:630-658, 893-950.
+template <int D, int WHAT>
+__global__ __launch_bounds__(kBlock) void k_p2g(Mesh M, Particles P, Nodes Nd, StepArgs A,
+                                                const Status* S, const int* skey,
+                                                const int* perm) {
+  __shared__ Stage<D, WHAT> st;
+  const int64_t base = blockIdx.x * (int64_t)kBlock;
+  const int cnt = (int)((P.np - base) < kBlock ? (P.np - base) : kBlock);
+  const int tid = threadIdx.x;
+  const int64_t np = P.np;
+  if (tid < cnt) {
+    const int key = skey[base + tid];
+    const int p = perm[base + tid];
+    const bool ok = valid_cell<D>(M, key);
+    st.key[tid] = ok ? key : -1;
+    if (ok) {
+      double xi[D];
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+        xi[k] = P.xi[k * np + p];
+        st.xi[k][tid] = xi[k];
+        st.v[k][tid] = P.v[k * np + p];
+      }
+      st.m[tid] = P.mass[p];
+      if constexpr ((WHAT & SC_F) != 0) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) st.s[k][tid] = P.stress[k * np + p];
+        st.vol[0][tid] = P.vol[p];
+        constexpr int NN = Elem<D>::NN;
+        double X[NN][D], Ji[D][D];
+        load_cell_X<D>(M, key, X);
+        jacobian_inverse<D>(xi, X, Ji);
+#pragma unroll
+        for (int a = 0; a < D; ++a)
+#pragma unroll
+          for (int b = 0; b < D; ++b) st.Ji[a * D + b][tid] = Ji[a][b];
+      }
+    }
+  }
+  double g[3] = {0, 0, 0};
+  if constexpr ((WHAT & SC_F) != 0) gravity_now(A, S, D, g);
+  __syncthreads();
+  scatter_segments<D, WHAT>(st, cnt, M, Nd, g);
+}
+
+// USF stress update in sorted order, then the force scatter of the new stress.
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_stress_scatter(Mesh M, Particles P, Nodes Nd,
+                                                           const Material* mats, StepArgs A,
+                                                           const Status* S, const int* skey,
+                                                           const int* perm) {
+  constexpr int NN = Elem<D>::NN;
+  __shared__ Stage<D, SC_F> st;
+  const int64_t base = blockIdx.x * (int64_t)kBlock;
+  const int cnt = (int)((P.np - base) < kBlock ? (P.np - base) : kBlock);
+  const int tid = threadIdx.x;
+  const int64_t np = P.np;
+  if (tid < cnt) {
+    const int key = skey[base + tid];
+    const int p = perm[base + tid];
+    const bool ok = valid_cell<D>(M, key);
+    st.key[tid] = ok ? key : -1;
+    if (ok) {
+      double xi[D];
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+        xi[k] = P.xi[k * np + p];
+        st.xi[k][tid] = xi[k];
+      }
+      double dndx[NN][D], Ji[D][D], nv[NN][D];
+      particle_dndx<D>(M, key, xi, dndx, Ji);
+      gather_nodes<D>(M, Nd, key, nv, nullptr);
+      double s[6], vol;
+      stress_strain<D>(M, P, mats, p, key, dndx, nv, A.dt, s, &vol);
+      st.m[tid] = P.mass[p];
+#pragma unroll
+      for (int k = 0; k < 6; ++k) st.s[k][tid] = s[k];
+      st.vol[0][tid] = vol;
+#pragma unroll
+      for (int a = 0; a < D; ++a)
+#pragma unroll
+        for (int b = 0; b < D; ++b) st.Ji[a * D + b][tid] = Ji[a][b];
+    }
+  }
+  double g[3] = {0, 0, 0};
+  gravity_now(A, S, D, g);
+  __syncthreads();
+  scatter_segments<D, SC_F>(st, cnt, M, Nd, g);
+}
+
+// Stress update in particle order (USL/MUSL postcompute, and USF in ordered
+// mode); optionally locate afterwards.
+template <int D, bool LOCATE>
+__global__ __launch_bounds__(kBlock) void k_stress(Mesh M, Particles P, Nodes Nd,
+                                                   const Material* mats, StepArgs A, Status* S,
+                                                   int* scan_list) {
+  constexpr int NN = Elem<D>::NN;
+  const int64_t p = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+  if (p >= P.np) return;
+  const int64_t np = P.np;
+  const int c = P.cell[p];
+  if (!valid_cell<D>(M, c)) return;
+  double xi[D];
+#pragma unroll
+  for (int k = 0; k < D; ++k) xi[k] = P.xi[k * np + p];
+  double dndx[NN][D], Ji[D][D], nv[NN][D];
+  particle_dndx<D>(M, c, xi, dndx, Ji);
+  gather_nodes<D>(M, Nd, c, nv, nullptr);
+  double s[6], vol;
+  stress_strain<D>(M, P, mats, (int)p, c, dndx, nv, A.dt, s, &vol);
+  if constexpr (LOCATE) {
+    double x[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) x[k] = P.x[k * np + p];
+    locate_one<D>(M, P, S, scan_list, (int)p, x, c, -1);
+  }
+}
+
+// G2P: compute_updated_position (particle.tcc:994-1070) in particle order,
+// optionally followed by the USL stress update and by locate.
+template <int D, bool STRESS, bool LOCATE>
+__global__ __launch_bounds__(kBlock) void k_g2p(Mesh M, Particles P, Nodes Nd,
+                                                const Material* mats, StepArgs A, int vu,
+                                                Status* S, int* scan_list) {
+  constexpr int NN = Elem<D>::NN;
+  const int64_t p = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+  if (p >= P.np) return;
+  const int64_t np = P.np;
+  const int c = P.cell[p];
+  if (!valid_cell<D>(M, c)) return;
+  double xi[D];
+#pragma unroll
+  for (int k = 0; k < D; ++k) xi[k] = P.xi[k * np + p];
+  double N[NN];
+  shapefn<D>(xi, N);
+  double nv[NN][D], na[NN][D];
+  gather_nodes<D>(M, Nd, c, nv, na);
+  double sv[D], sa[D];
+#pragma unroll
+  for (int k = 0; k < D; ++k) sv[k] = sa[k] = 0.;
+#pragma unroll
+  for (int i = 0; i < NN; ++i)
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      sv[k] += N[i] * nv[i][k];
+      sa[k] += N[i] * na[i][k];
+    }
+  double x[D];
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    double v = P.v[k * np + p];
+    if (vu == FLIP) {
+      v += sa[k] * A.dt;
+      v = A.blending * v + (1.0 - A.blending) * sv[k];
+    } else {
+      v = sv[k];
+    }
+    P.v[k * np + p] = v;
+    x[k] = P.x[k * np + p] + sv[k] * A.dt;
+    P.x[k * np + p] = x[k];
+    P.u[k * np + p] += sv[k] * A.dt;
+  }
+  if constexpr (STRESS) {
+    double dndx[NN][D], Ji[D][D];
+    particle_dndx<D>(M, c, xi, dndx, Ji);
+    double s[6], vol;
+    stress_strain<D>(M, P, mats, (int)p, c, dndx, nv, A.dt, s, &vol);
+  }
+  if constexpr (LOCATE) locate_one<D>(M, P, S, scan_list, (int)p, x, c, -1);
+}
+
+// MUSL G2P in sorted order with the momentum of the updated particle
+// velocities scattered back (mpm_scheme_musl.tcc:21-35; mass is unchanged
+// because the shape functions are not recomputed).
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_g2p_scatter(Mesh M, Particles P, Nodes Nd, StepArgs A,
+                                                        int vu, const int* skey,
+                                                        const int* perm) {
+  constexpr int NN = Elem<D>::NN;
+  __shared__ Stage<D, SC_MOM> st;
+  const int64_t base = blockIdx.x * (int64_t)kBlock;
+  const int cnt = (int)((P.np - base) < kBlock ? (P.np - base) : kBlock);
+  const int tid = threadIdx.x;
+  const int64_t np = P.np;
+  if (tid < cnt) {
+    const int key = skey[base + tid];
+    const int p = perm[base + tid];
+    const bool ok = valid_cell<D>(M, key);
+    st.key[tid] = ok ? key : -1;
+    if (ok) {
+      double xi[D];
+#pragma unroll
+      for (int k = 0; k < D; ++k) xi[k] = P.xi[k * np + p];
+      double N[NN];
+      shapefn<D>(xi, N);
+      double nv[NN][D], na[NN][D];
+      gather_nodes<D>(M, Nd, key, nv, na);
+      double sv[D], sa[D];
+#pragma unroll
+      for (int k = 0; k < D; ++k) sv[k] = sa[k] = 0.;
+#pragma unroll
+      for (int i = 0; i < NN; ++i)
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+          sv[k] += N[i] * nv[i][k];
+          sa[k] += N[i] * na[i][k];
+        }
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+        double v = P.v[k * np + p];
+        if (vu == FLIP) {
+          v += sa[k] * A.dt;
+          v = A.blending * v + (1.0 - A.blending) * sv[k];
+        } else {
+          v = sv[k];
+        }
+        P.v[k * np + p] = v;
+        P.x[k * np + p] += sv[k] * A.dt;
+        P.u[k * np + p] += sv[k] * A.dt;
+        st.v[k][tid] = v;
+        st.xi[k][tid] = xi[k];
+      }
+      st.m[tid] = P.mass[p];
+    }
+  }
+  __syncthreads();
+  scatter_segments<D, SC_MOM>(st, cnt, M, Nd, nullptr);
+}
+
+// ------------------------------- node kernels -------------------------------
+// Flags for what a node kernel does.
+// ND_INIT: start from v = a = 0 (Node::initialise, node.tcc:26-43) instead of
+// the stored values.
+// (flag values are declared in mpmx_kernels.h)
+
+// compute_velocity (ND_VEL) and/or compute_acceleration_velocity_cundall
+// (ND_ACC) for active nodes; then clears accumulators for the next pass.
+template <int D, int FL>
+__global__ __launch_bounds__(kBlock) void k_node(Mesh M, Nodes Nd, StepArgs A) {
+  const int64_t n = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+  if (n >= M.nnodes) return;
+  const int64_t nn = M.nnodes;
+  if (!Nd.active[n]) return;
+  const int ci = Nd.cons[n];
+  const NodeCons* c = ci >= 0 ? &Nd.table[ci] : nullptr;
+  const double mass = Nd.mass[n];
+  double v[D], a[D];
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    v[k] = ((FL & ND_INIT) != 0) ? 0. : Nd.vel[k * nn + n];
+    a[k] = ((FL & ND_INIT) != 0) ? 0. : Nd.acc[k * nn + n];
+  }
+  if constexpr ((FL & ND_VEL) != 0) {
+    double mom[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) mom[k] = Nd.mom[k * nn + n];
+    node_compute_velocity<D>(mass, mom, v, a, c);
+  }
+  if constexpr ((FL & ND_ACC) != 0) {
+    double fe[D], fi[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      fe[k] = Nd.fext[k * nn + n];
+      fi[k] = Nd.fint[k * nn + n];
+    }
+    node_acc_vel<D>(mass, fe, fi, v, a, c, A.dt, A.damping);
+  }
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    Nd.vel[k * nn + n] = v[k];
+    Nd.acc[k * nn + n] = a[k];
+  }
+  if constexpr ((FL & (ND_ZERO_MOM | ND_ZERO_ALL)) != 0) {
+#pragma unroll
+    for (int k = 0; k < D; ++k) Nd.mom[k * nn + n] = 0.;
+  }
+  if constexpr ((FL & ND_ZERO_ALL) != 0) {
+    Nd.mass[n] = 0.;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      Nd.fext[k * nn + n] = 0.;
+      Nd.fint[k * nn + n] = 0.;
+    }
+    Nd.active[n] = 0;
+  }
+}
+
+// ---------------------- ordered mode: node-centric gather --------------------
+// cell_start/cell_end: range of each cell in the sorted order (empty: equal).
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_cell_ranges(int64_t np, int64_t ncells,
+                                                        const int* skey, int* cstart,
+                                                        int* cend) {
+  const int64_t k = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+  if (k >= np) return;
+  const int c = skey[k];
+  if (c < 0 || c >= ncells) return;
+  if (k == 0 || skey[k - 1] != c) cstart[c] = (int)k;
+  if (k == np - 1 || skey[k + 1] != c) cend[c] = (int)k + 1;
+}
+
+__global__ void k_fill_int(int* a, int64_t n, int v) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+// Per node: merge the sorted particle lists of its cells by particle id and
+// sum contributions in that order (the single-threaded reference order).
+// WHAT: SC_MV (mass+momentum), SC_F (body+internal force), SC_MOM (momentum
+// only, mass kept). Then the node update flags FL as in k_node.
+template <int D, int WHAT, int FL>
+__global__ __launch_bounds__(kBlock) void k_gather(Mesh M, Particles P, Nodes Nd, StepArgs A,
+                                                   const Status* S, const int* perm,
+                                                   const int* cstart, const int* cend) {
+  constexpr int NN = Elem<D>::NN;
+  const int64_t n = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+  if (n >= M.nnodes) return;
+  const int64_t nnodes = M.nnodes, np = P.np;
+  int cells[NN], loc[NN], h[NN], e[NN];
+  bool any = false;
+#pragma unroll
+  for (int j = 0; j < NN; ++j) {
+    const int c = M.node_cells[n * NN + j];
+    cells[j] = c;
+    h[j] = e[j] = 0;
+    loc[j] = 0;
+    if (c >= 0) {
+      h[j] = cstart[c];
+      e[j] = cend[c];
+      if (h[j] < e[j]) any = true;
+#pragma unroll
+      for (int i = 0; i < NN; ++i)
+        if (M.cell_nodes[(int64_t)c * NN + i] == (int)n) loc[j] = i;
+    }
+  }
+  if ((WHAT & SC_MV) != 0) Nd.active[n] = any ? 1 : 0;
+  if (!any) return;
+  double g[3] = {0, 0, 0};
+  if constexpr ((WHAT & SC_F) != 0) gravity_now(A, S, D, g);
+  double am = 0., amom[D], afe[D], afi[D];
+#pragma unroll
+  for (int k = 0; k < D; ++k) amom[k] = afe[k] = afi[k] = 0.;
+  while (true) {
+    int bj = -1, bp = 0x7fffffff;
+#pragma unroll
+    for (int j = 0; j < NN; ++j)
+      if (h[j] < e[j]) {
+        const int p = perm[h[j]];
+        if (p < bp) {
+          bp = p;
+          bj = j;
+        }
+      }
+    if (bj < 0) break;
+    h[bj] += 1;
+    const int p = bp, c = cells[bj], i = loc[bj];
+    double xi[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) xi[k] = P.xi[k * np + p];
+    const double N = shapefn_i<D>(xi, i);
+    const double m = P.mass[p];
+    if constexpr ((WHAT & (SC_MV | SC_MOM)) != 0) {
+      const double mN = m * N;
+      if constexpr ((WHAT & SC_MV) != 0) am = am + mN;
+#pragma unroll
+      for (int k = 0; k < D; ++k) amom[k] = amom[k] + mN * P.v[k * np + p];
+    }
+    if constexpr ((WHAT & SC_F) != 0) {
+#pragma unroll
+      for (int k = 0; k < D; ++k) afe[k] = afe[k] + g[k] * m * N;
+      double X[NN][D], Ji[D][D], d[D];
+      load_cell_X<D>(M, c, X);
+      jacobian_inverse<D>(xi, X, Ji);
+      dndx_i<D>(xi, i, Ji, d);
+      double s[6];
+#pragma unroll
+      for (int k = 0; k < 6; ++k) s[k] = P.stress[k * np + p];
+      double f[D];
+      if constexpr (D == 2) {
+        f[0] = d[0] * s[0] + d[1] * s[3];
+        f[1] = d[1] * s[1] + d[0] * s[3];
+      } else {
+        f[0] = d[0] * s[0] + d[1] * s[3] + d[2] * s[5];
+        f[1] = d[1] * s[1] + d[0] * s[3] + d[2] * s[4];
+        f[2] = d[2] * s[2] + d[1] * s[4] + d[0] * s[5];
+      }
+      const double mv = -1. * P.vol[p];
+#pragma unroll
+      for (int k = 0; k < D; ++k) afi[k] = afi[k] + f[k] * mv;
+    }
+  }
+  if constexpr ((WHAT & SC_MV) != 0) Nd.mass[n] = am;
+  if constexpr ((WHAT & (SC_MV | SC_MOM)) != 0) {
+#pragma unroll
+    for (int k = 0; k < D; ++k) Nd.mom[k * nnodes + n] = amom[k];
+  }
+  if constexpr ((WHAT & SC_F) != 0) {
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      Nd.fext[k * nnodes + n] = afe[k];
+      Nd.fint[k * nnodes + n] = afi[k];
+    }
+  }
+  // node update
+  const int ci = Nd.cons[n];
+  const NodeCons* cc = ci >= 0 ? &Nd.table[ci] : nullptr;
+  const double mass = Nd.mass[n];
+  double v[D], a[D];
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    v[k] = ((FL & ND_INIT) != 0) ? 0. : Nd.vel[k * nnodes + n];
+    a[k] = ((FL & ND_INIT) != 0) ? 0. : Nd.acc[k * nnodes + n];
+  }
+  if constexpr ((FL & ND_VEL) != 0) {
+    double mom[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) mom[k] = Nd.mom[k * nnodes + n];
+    node_compute_velocity<D>(mass, mom, v, a, cc);
+  }
+  if constexpr ((FL & ND_ACC) != 0) {
+    double fe[D], fi[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      fe[k] = Nd.fext[k * nnodes + n];
+      fi[k] = Nd.fint[k * nnodes + n];
+    }
+    node_acc_vel<D>(mass, fe, fi, v, a, cc, A.dt, A.damping);
+  }
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    Nd.vel[k * nnodes + n] = v[k];
+    Nd.acc[k * nnodes + n] = a[k];
+  }
+}
+
+// ------------------------------- launchers ----------------------------------
+static inline dim3 grid_for(int64_t n) { return dim3((unsigned)((n + kBlock - 1) / kBlock)); }
+
+#define MPMX_LAUNCH(kernel, n, stream, ...)                                       \
+  do {                                                                            \
+    if ((n) > 0) hipLaunchKernelGGL(kernel, grid_for(n), dim3(kBlock), 0, stream, \
+                                    __VA_ARGS__);                                 \
+  } while (0)
+
+template <int D>
+void launch_locate_init(const Mesh& M, const Particles& P, Status* S, int* scan_list,
+                        const int* hint, hipStream_t st) {
+  MPMX_LAUNCH((k_locate_init<D>), P.np, st, M, P, S, scan_list, hint);
+}
+
+template <int D>
+void launch_locate_scan(const Mesh& M, const Particles& P, Status* S, const int* scan_list,
+                        hipStream_t st) {
+  hipLaunchKernelGGL((k_locate_scan<D>), dim3(256), dim3(kBlock), 0, st, M, P, S, scan_list);
+}
+
+void launch_step_end(Status* S, hipStream_t st) {
+  hipLaunchKernelGGL(k_step_end, dim3(1), dim3(64), 0, st, S);
+}
+
+template <int D>
+void launch_p2g(int what, const Mesh& M, const Particles& P, const Nodes& Nd, const StepArgs& A,
+                const Status* S, const int* skey, const int* perm, hipStream_t st) {
+  if (what == SC_MV)
+    MPMX_LAUNCH((k_p2g<D, SC_MV>), P.np, st, M, P, Nd, A, S, skey, perm);
+  else
+    MPMX_LAUNCH((k_p2g<D, SC_MV | SC_F>), P.np, st, M, P, Nd, A, S, skey, perm);
+}
+
+template <int D>
+void launch_stress_scatter(const Mesh& M, const Particles& P, const Nodes& Nd,
+                           const Material* mats, const StepArgs& A, const Status* S,
+                           const int* skey, const int* perm, hipStream_t st) {
+  MPMX_LAUNCH((k_stress_scatter<D>), P.np, st, M, P, Nd, mats, A, S, skey, perm);
+}
+
+template <int D>
+void launch_stress(bool locate, const Mesh& M, const Particles& P, const Nodes& Nd,
+                   const Material* mats, const StepArgs& A, Status* S, int* scan_list,
+                   hipStream_t st) {
+  if (locate)
+    MPMX_LAUNCH((k_stress<D, true>), P.np, st, M, P, Nd, mats, A, S, scan_list);
+  else
+    MPMX_LAUNCH((k_stress<D, false>), P.np, st, M, P, Nd, mats, A, S, scan_list);
+}
+
+template <int D>
+void launch_g2p(bool stress, bool locate, const Mesh& M, const Particles& P, const Nodes& Nd,
+                const Material* mats, const StepArgs& A, int vu, Status* S, int* scan_list,
+                hipStream_t st) {
+  if (stress && locate)
+    MPMX_LAUNCH((k_g2p<D, true, true>), P.np, st, M, P, Nd, mats, A, vu, S, scan_list);
+  else if (locate)
+    MPMX_LAUNCH((k_g2p<D, false, true>), P.np, st, M, P, Nd, mats, A, vu, S, scan_list);
+  else
+    MPMX_LAUNCH((k_g2p<D, false, false>), P.np, st, M, P, Nd, mats, A, vu, S, scan_list);
+}
+
+template <int D>
+void launch_g2p_scatter(const Mesh& M, const Particles& P, const Nodes& Nd, const StepArgs& A,
+                        int vu, const int* skey, const int* perm, hipStream_t st) {
+  MPMX_LAUNCH((k_g2p_scatter<D>), P.np, st, M, P, Nd, A, vu, skey, perm);
+}
+
+template <int D>
+void launch_node(int fl, const Mesh& M, const Nodes& Nd, const StepArgs& A, hipStream_t st) {
+  switch (fl) {
+#define MPMX_NODE_CASE(F) \
+  case (F): MPMX_LAUNCH((k_node<D, (F)>), M.nnodes, st, M, Nd, A); break;
+    MPMX_NODE_CASE(ND_INIT | ND_VEL)
+    MPMX_NODE_CASE(ND_ACC | ND_ZERO_ALL)
+    MPMX_NODE_CASE(ND_INIT | ND_VEL | ND_ACC | ND_ZERO_ALL)
+    MPMX_NODE_CASE(ND_INIT | ND_VEL | ND_ACC | ND_ZERO_MOM)
+    MPMX_NODE_CASE(ND_VEL | ND_ZERO_ALL)
+#undef MPMX_NODE_CASE
+    default: break;
+  }
+}
+
+template <int D>
+void launch_cell_ranges(int64_t np, int64_t ncells, const int* skey, int* cstart, int* cend,
+                        hipStream_t st) {
+  if (ncells > 0) {
+    hipLaunchKernelGGL(k_fill_int, grid_for(ncells), dim3(kBlock), 0, st, cstart, ncells, 0);
+    hipLaunchKernelGGL(k_fill_int, grid_for(ncells), dim3(kBlock), 0, st, cend, ncells, 0);
+  }
+  MPMX_LAUNCH((k_cell_ranges<D>), np, st, np, ncells, skey, cstart, cend);
+}
+
+template <int D>
+void launch_gather(int which, const Mesh& M, const Particles& P, const Nodes& Nd,
+                   const StepArgs& A, const Status* S, const int* perm, const int* cstart,
+                   const int* cend, hipStream_t st) {
+  switch (which) {
+    case GK_MV_VEL:  // USF first pass
+      MPMX_LAUNCH((k_gather<D, SC_MV, ND_INIT | ND_VEL>), M.nnodes, st, M, P, Nd, A, S, perm, cstart, cend);
+      break;
+    case GK_F_ACC:  // USF force pass
+      MPMX_LAUNCH((k_gather<D, SC_F, ND_ACC>), M.nnodes, st, M, P, Nd, A, S, perm, cstart, cend);
+      break;
+    case GK_ALL_VEL_ACC:  // USL / MUSL first pass
+      MPMX_LAUNCH((k_gather<D, SC_MV | SC_F, ND_INIT | ND_VEL | ND_ACC>), M.nnodes, st, M, P, Nd, A, S,
+                  perm, cstart, cend);
+      break;
+    case GK_MOM_VEL:  // MUSL second momentum pass
+      MPMX_LAUNCH((k_gather<D, SC_MOM, ND_VEL>), M.nnodes, st, M, P, Nd, A, S, perm, cstart, cend);
+      break;
+  }
+}
+
+#define MPMX_INSTANTIATE(D)                                                                   \
+  template void launch_locate_init<D>(const Mesh&, const Particles&, Status*, int*,          \
+                                      const int*, hipStream_t);                              \
+  template void launch_locate_scan<D>(const Mesh&, const Particles&, Status*, const int*,    \
+                                      hipStream_t);                                          \
+  template void launch_p2g<D>(int, const Mesh&, const Particles&, const Nodes&,              \
+                              const StepArgs&, const Status*, const int*, const int*,        \
+                              hipStream_t);                                                  \
+  template void launch_stress_scatter<D>(const Mesh&, const Particles&, const Nodes&,        \
+                                         const Material*, const StepArgs&, const Status*,    \
+                                         const int*, const int*, hipStream_t);               \
+  template void launch_stress<D>(bool, const Mesh&, const Particles&, const Nodes&,           \
+                                 const Material*, const StepArgs&, Status*, int*,            \
+                                 hipStream_t);                                               \
+  template void launch_g2p<D>(bool, bool, const Mesh&, const Particles&, const Nodes&,       \
+                              const Material*, const StepArgs&, int, Status*, int*,          \
+                              hipStream_t);                                                  \
+  template void launch_g2p_scatter<D>(const Mesh&, const Particles&, const Nodes&,           \
+                                      const StepArgs&, int, const int*, const int*,          \
+                                      hipStream_t);                                          \
+  template void launch_node<D>(int, const Mesh&, const Nodes&, const StepArgs&, hipStream_t);\
+  template void launch_cell_ranges<D>(int64_t, int64_t, const int*, int*, int*, hipStream_t);\
+  template void launch_gather<D>(int, const Mesh&, const Particles&, const Nodes&,           \
+                                 const StepArgs&, const Status*, const int*, const int*,     \
+                                 const int*, hipStream_t);
+
+MPMX_INSTANTIATE(2)
+MPMX_INSTANTIATE(3)
+
+}  // namespace mpmx

@@ -1,0 +1,159 @@
+"""Synthetic input decks of the benchmark configurations.
+
+Each builder returns a ``Workload``: the mesh, materials, constraints,
+gravity, particle generation and run parameters of one configuration. The
+same object drives the GPU ``Solver`` and any object with the same setup
+methods, so every consumer starts from identical state.
+
+* ``column_2d``: plain 2D elastic column under gravity (quad4 32x64 cells of
+  0.05 m, 4 particles per cell, LinearElastic E=1e6 Pa, nu=0.3, rho=1800,
+  fixed bottom, roller sides, USF, dt=1e-4).
+* ``collapse_3d``: 3D granular column collapse (hex8 128x64x64 cells of
+  0.01 m, 8 particles per cell in a 0.64 m cube block = 2,097,152
+  particles, MohrCoulomb phi=30, psi=0, c=0, E=5e6, nu=0.3, rho=1800,
+  frictional base mu=0.3, roller walls, MUSL, dt=5e-5, initial velocities
+  uniform in +-1e-4 m/s from seed 1). The block starts from a K0 = 1 - sin(phi)
+  geostatic stress: with c = 0 the reference's return mapping divides by
+  dP/dsigma = 0 at zero stress (mohr_coulomb.tcc:453-463) and returns NaN, so
+  a stress-free cohesionless start cannot run in the reference either.
+
+Sizes are parameters so tests run the same decks at small scale.
+"""
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+
+from . import materials as mat
+from .mesh import structured_mesh
+
+_M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+
+def splitmix64(seed, n):
+    """n 64-bit outputs of SplitMix64 (Steele, Lea & Flood 2014) from `seed`."""
+    with np.errstate(over="ignore"):
+        z = (np.uint64(seed) + np.arange(1, n + 1, dtype=np.uint64) *
+             np.uint64(0x9E3779B97F4A7C15))
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return z
+
+
+def uniform(seed, n, lo, hi):
+    """n doubles uniform in [lo, hi) from the top 53 bits of SplitMix64."""
+    u = (splitmix64(seed, n) >> np.uint64(11)).astype(np.float64) * (2.0 ** -53)
+    return lo + (hi - lo) * u
+
+
+@dataclass
+class Workload:
+    name: str
+    dim: int
+    scheme: str
+    coords: np.ndarray
+    cells: np.ndarray
+    materials: np.ndarray
+    gravity: np.ndarray
+    dt: float
+    nsteps: int
+    nppd: int = 2
+    cell_set: Optional[np.ndarray] = None
+    vel_nodes: np.ndarray = field(default_factory=lambda: np.zeros(0, dtype=np.int64))
+    vel_dirs: np.ndarray = field(default_factory=lambda: np.zeros(0, dtype=np.int32))
+    vel_values: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    fric_nodes: np.ndarray = field(default_factory=lambda: np.zeros(0, dtype=np.int64))
+    fric_dirs: np.ndarray = field(default_factory=lambda: np.zeros(0, dtype=np.int32))
+    fric_sign: np.ndarray = field(default_factory=lambda: np.zeros(0, dtype=np.int32))
+    fric_mu: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    velocity_seed: Optional[int] = None
+    velocity_amplitude: float = 0.0
+    # K0 geostatic initial stress: (K0, unit weight rho*|g|, free-surface height)
+    geostatic: Optional[tuple] = None
+    damping: float = 0.0
+    info: dict = field(default_factory=dict)
+
+    def setup(self, solver):
+        """Apply this deck to a solver (mesh, materials, constraints, particles)."""
+        solver.set_mesh(self.coords, self.cells)
+        solver.set_materials(self.materials)
+        if len(self.vel_nodes):
+            solver.set_velocity_constraints(self.vel_nodes, self.vel_dirs, self.vel_values)
+        if len(self.fric_nodes):
+            solver.set_friction_constraints(self.fric_nodes, self.fric_dirs, self.fric_sign,
+                                            self.fric_mu)
+        solver.set_gravity(self.gravity)
+        solver.generate_gauss(self.nppd, self.cell_set, 0)
+        if self.geostatic is not None:
+            k0, gamma, top = self.geostatic
+            z = solver.particles(["x"])["x"][:, self.dim - 1]
+            sv = -gamma * (top - z)
+            stress = np.zeros((len(z), 6))
+            stress[:, self.dim - 1] = sv
+            for k in range(self.dim - 1):
+                stress[:, k] = k0 * sv
+            if self.dim == 2:
+                stress[:, 2] = k0 * sv
+            solver.assign_stresses(stress)
+        if self.velocity_seed is not None:
+            n = solver.nparticles
+            a = self.velocity_amplitude
+            solver.assign_velocity(uniform(self.velocity_seed, n * self.dim, -a, a)
+                                   .reshape(n, self.dim))
+        return solver
+
+    @property
+    def nparticles(self):
+        ncell = len(self.cells) if self.cell_set is None else len(self.cell_set)
+        return ncell * self.nppd ** self.dim
+
+
+def _node_sets(coords, axis, value):
+    return np.nonzero(np.abs(coords[:, axis] - value) <= 1e-9)[0].astype(np.int64)
+
+
+def column_2d(nx=32, ny=64, h=0.05, nsteps=1000, dt=1e-4):
+    coords, cells = structured_mesh((0.0, 0.0), (h, h), (nx, ny))
+    bottom = _node_sets(coords, 1, 0.0)
+    left = _node_sets(coords, 0, 0.0)
+    right = _node_sets(coords, 0, nx * h)
+    vn = np.concatenate([bottom, bottom, left, right])
+    vd = np.concatenate([np.zeros(len(bottom)), np.ones(len(bottom)),
+                         np.zeros(len(left)), np.zeros(len(right))]).astype(np.int32)
+    return Workload(
+        name="column_2d", dim=2, scheme="usf", coords=coords, cells=cells,
+        materials=mat.linear_elastic(1e6, 0.3, 1800.0)[None, :],
+        gravity=np.array([0.0, -9.81]), dt=dt, nsteps=nsteps, nppd=2,
+        vel_nodes=vn, vel_dirs=vd, vel_values=np.zeros(len(vn)),
+        info=dict(mesh="quad4 %dx%d @ %g m" % (nx, ny, h), material="LinearElastic",
+                  particles_per_cell=4))
+
+
+def collapse_3d(nx=128, ny=64, nz=64, h=0.01, block=(64, 64, 64), nsteps=2000, dt=5e-5,
+                seed=1):
+    coords, cells = structured_mesh((0.0, 0.0, 0.0), (h, h, h), (nx, ny, nz))
+    ci = np.arange(len(cells), dtype=np.int64)
+    i = ci % nx
+    j = (ci // nx) % ny
+    k = ci // (nx * ny)
+    cset = ci[(i < block[0]) & (j < block[1]) & (k < block[2])]
+    base = _node_sets(coords, 2, 0.0)
+    x0, x1 = _node_sets(coords, 0, 0.0), _node_sets(coords, 0, nx * h)
+    y0, y1 = _node_sets(coords, 1, 0.0), _node_sets(coords, 1, ny * h)
+    vn = np.concatenate([base, x0, x1, y0, y1])
+    vd = np.concatenate([np.full(len(base), 2), np.zeros(len(x0)), np.zeros(len(x1)),
+                         np.ones(len(y0)), np.ones(len(y1))]).astype(np.int32)
+    return Workload(
+        name="collapse_3d", dim=3, scheme="musl", coords=coords, cells=cells,
+        materials=mat.mohr_coulomb(5e6, 0.3, 1800.0, friction=30.0, dilation=0.0,
+                                   cohesion=0.0)[None, :],
+        gravity=np.array([0.0, 0.0, -9.81]), dt=dt, nsteps=nsteps, nppd=2, cell_set=cset,
+        vel_nodes=vn, vel_dirs=vd, vel_values=np.zeros(len(vn)),
+        fric_nodes=base, fric_dirs=np.full(len(base), 2, dtype=np.int32),
+        fric_sign=np.full(len(base), -1, dtype=np.int32), fric_mu=np.full(len(base), 0.3),
+        velocity_seed=seed, velocity_amplitude=1e-4,
+        geostatic=(1.0 - np.sin(np.radians(30.0)), 1800.0 * 9.81, block[2] * h),
+        info=dict(mesh="hex8 %dx%dx%d @ %g m" % (nx, ny, nz, h),
+                  block_cells=list(block), material="MohrCoulomb phi=30 psi=0 c=0",
+                  particles_per_cell=8))

@@ -59,7 +59,8 @@ def _lib(omp=False):
                    "orc_set_gravity", "orc_generate_gauss", "orc_set_particles",
                    "orc_nparticles", "orc_get_particles", "orc_get_nodes", "orc_get_cells",
                    "orc_step", "orc_phase", "orc_set_node_field", "orc_set_particle_field",
-                   "orc_get_particle_scratch"):
+                   "orc_get_particle_scratch", "orc_assign_velocity",
+                   "orc_assign_stresses"):
             getattr(lib, fn).restype = C.c_int
         lib.orc_kat_cell_volume.restype = C.c_double
         lib.orc_kat_mc_yield.restype = C.c_int
@@ -184,13 +185,21 @@ class Oracle:
             _p(arrs["strain"]), _p(arrs["F"]), _p(arrs["mass"]), _p(arrs["volume"]),
             _p(arrs["density"]), _p(mat, _I32), _p(cell, _I64), _p(arrs["sv"])))
 
+    def assign_velocity(self, v):
+        v = _f64(v, (-1, self.dim))
+        self._check(self.lib.orc_assign_velocity(self.ctx, C.c_int64(v.shape[0]), _p(v)))
+
+    def assign_stresses(self, stress):
+        stress = _f64(stress, (-1, 6))
+        self._check(self.lib.orc_assign_stresses(self.ctx, C.c_int64(stress.shape[0]), _p(stress)))
+
     @property
     def nparticles(self):
         n = C.c_int64(0)
         self.lib.orc_nparticles(self.ctx, C.byref(n))
         return n.value
 
-    def particles(self):
+    def particles(self, fields=None):
         n, d = self.nparticles, self.dim
         out = dict(x=np.zeros((n, d)), u=np.zeros((n, d)), v=np.zeros((n, d)),
                    stress=np.zeros((n, 6)), strain=np.zeros((n, 6)), F=np.zeros((n, 9)),

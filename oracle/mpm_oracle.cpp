@@ -1446,6 +1446,24 @@ int orc_set_particles(orc_ctx* c, Index n, const double* x, const double* u, con
   })
 }
 
+int orc_assign_velocity(orc_ctx* c, Index n, const double* v) {
+  ORC_TRY({
+    auto& s = c->s;
+    if (n != (Index)s.particles.size()) throw std::runtime_error("one velocity per particle");
+    for (Index i = 0; i < n; ++i)
+      for (int k = 0; k < s.dim; ++k) s.particles[i].v[k] = v[i * s.dim + k];
+  })
+}
+
+int orc_assign_stresses(orc_ctx* c, Index n, const double* st) {
+  ORC_TRY({
+    auto& s = c->s;
+    if (n != (Index)s.particles.size()) throw std::runtime_error("one stress per particle");
+    for (Index i = 0; i < n; ++i)
+      for (int k = 0; k < 6; ++k) s.particles[i].stress[k] = st[i * 6 + k];
+  })
+}
+
 int orc_nparticles(orc_ctx* c, Index* n) {
   *n = (Index)c->s.particles.size();
   return 0;

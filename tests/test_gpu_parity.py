@@ -1,0 +1,201 @@
+"""GPU parity: the HIP step (through the C-ABI) against the oracle.
+
+Ordered mode sums nodal quantities in particle order, as the single-threaded
+reference does; with LinearElastic it must reproduce the oracle bit for bit
+(np.array_equal on every particle field). Mohr-Coulomb calls sin/cos/tan/acos/
+pow, whose device and host library implementations may differ in the last
+bit, so its check is normwise relative <= 1e-12 per step. The default fast
+mode adds per-cell partial sums with FP64 atomics in arbitrary order; its
+check is normwise relative <= 1e-12 per step (per field: max|a-b| / max|b|)
+and bit-exact cell ids.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from mpm_amd import workloads
+from mpm_amd import materials as mat
+from mpm_amd.mesh import structured_mesh
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("x", "u", "v", "stress", "strain", "F", "volume", "density", "xi")
+STEP_TOL = 1e-12
+
+
+def _solver(dim, scheme, ordered):
+    from mpm_amd.capi import Solver
+    return Solver(dim, scheme, ordered=ordered)
+
+
+def _pair(w, ordered, scheme=None):
+    scheme = scheme or w.scheme
+    g = w.setup(_solver(w.dim, scheme, ordered))
+    o = w.setup(oracle.Oracle(w.dim, scheme))
+    return g, o
+
+
+def _relerr(a, b):
+    scale = float(np.max(np.abs(b))) if b.size else 0.0
+    if scale == 0.0:
+        return float(np.max(np.abs(a))) if a.size else 0.0
+    return float(np.max(np.abs(a - b))) / scale
+
+
+def _compare(g, o, exact, tol=STEP_TOL, what=""):
+    pg, po = g.particles(), o.particles()
+    assert np.array_equal(pg["cell"], po["cell"]), what + ": cell ids differ"
+    errs = {}
+    for f in FIELDS + ("statevars",):
+        if exact and f != "statevars":
+            bad = np.argwhere(pg[f] != po[f])
+            assert bad.size == 0, "%s: %s differs bitwise at %s (gpu %r oracle %r)" % (
+                what, f, bad[:3].tolist(), pg[f][tuple(bad[0])], po[f][tuple(bad[0])])
+        errs[f] = _relerr(pg[f], po[f])
+        if not exact:
+            assert errs[f] <= tol, "%s: %s rel err %.3e > %.1e" % (what, f, errs[f], tol)
+    return errs
+
+
+def _column(nx=8, ny=16):
+    return workloads.column_2d(nx=nx, ny=ny)
+
+
+def _collapse(le=False):
+    w = workloads.collapse_3d(nx=8, ny=4, nz=4, block=(4, 4, 4))
+    if le:
+        w.materials = mat.linear_elastic(5e6, 0.3, 1800.0)[None, :]
+    return w
+
+
+@pytest.mark.parametrize("scheme", ["usf", "usl", "musl"])
+def test_ordered_2d_linear_elastic_bitwise(scheme):
+    w = _column()
+    g, o = _pair(w, ordered=True, scheme=scheme)
+    _compare(g, o, exact=True, what="init")
+    for s in range(20):
+        g.step(1, w.dt)
+        o.step(1, w.dt)
+        _compare(g, o, exact=True, what="%s step %d" % (scheme, s))
+    g.step(80, w.dt)
+    o.step(80, w.dt)
+    _compare(g, o, exact=True, what="%s after 100 steps" % scheme)
+
+
+@pytest.mark.parametrize("scheme", ["usf", "usl", "musl"])
+def test_ordered_3d_linear_elastic_bitwise(scheme):
+    w = _collapse(le=True)
+    g, o = _pair(w, ordered=True, scheme=scheme)
+    _compare(g, o, exact=True, what="init")
+    for s in range(10):
+        g.step(5, w.dt)
+        o.step(5, w.dt)
+        _compare(g, o, exact=True, what="%s step %d" % (scheme, 5 * (s + 1)))
+
+
+@pytest.mark.parametrize("ordered", [True, False])
+def test_mohr_coulomb_3d_per_step(ordered):
+    """Each step from the oracle's state (1e-12 per step)."""
+    w = _collapse()
+    o = w.setup(oracle.Oracle(3, w.scheme))
+    worst = 0.0
+    for s in range(40):
+        g = _gpu_from(o, w, w.scheme, ordered)
+        g.step(1, w.dt)
+        o.step(1, w.dt)
+        e = _compare(g, o, exact=False, what="MC step %d" % s)
+        worst = max(worst, max(e.values()))
+        g.close()
+    print("MC (ordered=%s) worst per-step field rel err over 40 steps: %.3e" % (ordered, worst))
+
+
+def _gpu_from(o, w, scheme, ordered=False):
+    """A fresh GPU solver holding exactly the oracle's particle state."""
+    from mpm_amd.capi import Solver
+    po = o.particles()
+    n = Solver(w.dim, scheme, ordered=ordered)
+    n.set_mesh(w.coords, w.cells)
+    n.set_materials(w.materials)
+    if len(w.vel_nodes):
+        n.set_velocity_constraints(w.vel_nodes, w.vel_dirs, w.vel_values)
+    if len(w.fric_nodes):
+        n.set_friction_constraints(w.fric_nodes, w.fric_dirs, w.fric_sign, w.fric_mu)
+    n.set_gravity(w.gravity)
+    n.set_particles(x=po["x"], mass=po["mass"], volume=po["volume"], density=po["density"],
+                    material_id=po["material_id"], v=po["v"], u=po["u"], stress=po["stress"],
+                    strain=po["strain"], F=po["F"], cell=po["cell"], statevars=po["statevars"])
+    return n
+
+
+@pytest.mark.parametrize("scheme", ["usf", "usl", "musl"])
+def test_fast_2d_per_step(scheme):
+    """Each step from the oracle's state: one step's atomic-order error only."""
+    w = _column()
+    o = w.setup(oracle.Oracle(2, scheme))
+    o.step(5, w.dt)
+    for s in range(10):
+        g = _gpu_from(o, w, scheme)
+        g.step(1, w.dt)
+        o.step(1, w.dt)
+        _compare(g, o, exact=False, what="fast %s step %d" % (scheme, s))
+        g.close()
+
+
+@pytest.mark.parametrize("scheme", ["usf", "musl"])
+def test_fast_2d_full_run(scheme):
+    """100 steps in fast mode stay within 1e-9 of the oracle."""
+    w = _column()
+    g, o = _pair(w, ordered=False, scheme=scheme)
+    g.step(100, w.dt)
+    o.step(100, w.dt)
+    e = _compare(g, o, exact=False, tol=1e-9, what="fast %s 100 steps" % scheme)
+    print("fast %s 100-step errors: %s" % (scheme, {k: "%.2e" % v for k, v in e.items()}))
+
+
+def test_cell_crossing_gpu(kats):
+    c = kats["cell_crossing_2d"]
+    from mpm_amd.capi import Solver
+    for ordered in (True, False):
+        g = Solver(2, "usf", ordered=ordered)
+        g.set_mesh(np.array(c["nodes"]), np.array(c["cells"]))
+        m = c["material"]
+        g.set_materials([mat.linear_elastic(m["youngs_modulus"], m["poisson_ratio"],
+                                            m["density"])])
+        n = len(c["particles"])
+        vol = 0.25
+        g.set_particles(x=c["particles"], mass=[vol * m["density"]] * n, volume=[vol] * n,
+                        density=[m["density"]] * n, material_id=0)
+        assert list(g.particles()["cell"]) == c["cells_before"]
+        vc = np.array(c["velocity_constraints"])
+        g.set_velocity_constraints(vc[:, 0].astype(int), vc[:, 1].astype(int), vc[:, 2])
+        g.step(1, c["dt"])
+        cells = g.particles()["cell"]
+        assert list(np.bincount(cells, minlength=2)) == c["cell_counts_after"]
+
+
+def test_unlocatable_particle_raises():
+    from mpm_amd.capi import Solver, UnlocatableParticle
+    coords, cells = structured_mesh((0.0, 0.0), (1.0, 1.0), (2, 1))
+    g = Solver(2, "usf", ordered=False)
+    g.set_mesh(coords, cells)
+    g.set_materials([mat.linear_elastic(1e7, 0.3, 1000.0)])
+    g.set_particles(x=[[1.5, 0.5]], mass=[1000.0], volume=[1.0], density=[1000.0],
+                    material_id=0, v=[[3000.0, 0.0]])
+    with pytest.raises(UnlocatableParticle):
+        g.step(1, 0.001)
+    with pytest.raises(UnlocatableParticle):
+        g.set_particles(x=[[5.0, 0.5]], mass=[1.0], volume=[1.0], density=[1.0], material_id=0)
+
+
+def test_graph_replay_matches_direct_launches():
+    w = _column()
+    from mpm_amd.capi import Solver
+    a = w.setup(Solver(2, "usf", ordered=False, graph=True))
+    b = w.setup(Solver(2, "usf", ordered=False, graph=False))
+    for _ in range(5):
+        a.step(10, w.dt)
+        b.step(10, w.dt)
+        pa, pb = a.particles(), b.particles()
+        assert np.array_equal(pa["cell"], pb["cell"])
+        for f in ("x", "v", "stress"):
+            assert _relerr(pa[f], pb[f]) <= 1e-12
