@@ -8,6 +8,7 @@
 #pragma once
 
 #include "mpmx_internal.h"
+#include "mpmx_math.h"
 
 namespace mpmx {
 
@@ -205,10 +206,10 @@ MPMX_DEV double mu_lode(const double* s) {
   const double j2 = mu_j2(s);
   const double j3 = mu_j3(s);
   double v = 0.0;
-  if (fabs(j2) > kDblEps) v = (3. * sqrt(3.) / 2.) * (j3 / pow(j2, 1.5));
+  if (fabs(j2) > kDblEps) v = (3. * sqrt(3.) / 2.) * (j3 / dmath::pow_1p5(j2));
   if (v > 1.0) v = 1.0;
   if (v < -1.0) v = -1.0;
-  return (1. / 3.) * acos(v);
+  return (1. / 3.) * dmath::acos(v);
 }
 MPMX_DEV void mu_dq_dsigma(const double* s, double* out) {
   const double q = mu_q(s);
@@ -253,9 +254,9 @@ MPMX_DEV void mu_dtheta_dsigma(const double* s, double* out) {
   double dr_dj3 = 3.0 / 2.0 * sqrt(3.0);
   double dtheta_dr = -1.0 / 3.0;
   if (fabs(j2) > tol) {
-    double r = j3 / 2.0 * pow(j2 / 3.0, -1.5);
-    dr_dj2 *= pow(j2, -2.5);
-    dr_dj3 *= pow(j2, -1.5);
+    double r = j3 / 2.0 * dmath::pow_m1p5(j2 / 3.0);
+    dr_dj2 *= dmath::pow_m2p5(j2);
+    dr_dj3 *= dmath::pow_m1p5(j2);
     const double factor = (fabs(1 - r * r) < tol) ? tol : fabs(1 - r * r);
     dtheta_dr = -1.0 / (3.0 * sqrt(factor));
   }
@@ -291,30 +292,56 @@ MPMX_DEV double dot6(const double* a, const double* b) {
 }
 
 // ------------------------------ Mohr-Coulomb --------------------------------
+// mohr_coulomb.tcc:85-533. Every sin/cos/tan/acos/pow of the reference is
+// evaluated with the deterministic functions of mpmx_math.h; each distinct
+// argument is evaluated once and reused (the functions are pure, so the bits
+// equal those of the reference's repeated calls).
 enum { FS_ELASTIC = 0, FS_SHEAR = 1, FS_TENSILE = 2 };
 
-MPMX_DEV void mc_invariants(const double* s, double* sv) {  // mohr_coulomb.tcc:124-134
+// trigonometry of the friction and dilation angles
+struct McAngles {
+  double sphi, cphi, tphi, tpsi;
+};
+MPMX_DEV McAngles mc_angles(double phi, double psi) {
+  McAngles a;
+  dmath::sincos(phi, &a.sphi, &a.cphi);
+  a.tphi = a.sphi / a.cphi;  // = dmath::tan(phi)
+  a.tpsi = dmath::tan(psi);
+  return a;
+}
+
+// invariants plus the trigonometry of the Lode angle
+struct McInv {
+  double st, ct;    // sin, cos of theta
+  double s3, c3;    // sin, cos of theta + pi/3
+};
+
+// compute_stress_invariants (mohr_coulomb.tcc:124-134)
+MPMX_DEV void mc_invariants(const double* s, double* sv, McInv& I) {
   sv[SV_EPS] = mu_p(s) * sqrt(3.);
   sv[SV_THETA] = mu_lode(s);
   sv[SV_RHO] = sqrt(2. * mu_j2(s));
+  const double theta = sv[SV_THETA];
+  dmath::sincos(theta, &I.st, &I.ct);
+  dmath::sincos(theta + M_PI / 3., &I.s3, &I.c3);
 }
 
-MPMX_DEV int mc_yield(double* yf, const double* sv) {  // mohr_coulomb.tcc:139-191
+// compute_yield_state (mohr_coulomb.tcc:139-191)
+MPMX_DEV int mc_yield(double* yf, const double* sv, const McInv& I, const McAngles& A) {
   const double Tolerance = -1E-1;
   const double epsilon = sv[SV_EPS], rho = sv[SV_RHO], theta = sv[SV_THETA];
-  const double phi = sv[SV_PHI], cohesion = sv[SV_COH], tension_cutoff = sv[SV_TCUT];
-  yf[0] = sqrt(2. / 3.) * cos(theta) * rho + epsilon / sqrt(3.) - tension_cutoff;
-  yf[1] = sqrt(1.5) * rho *
-              ((sin(theta + M_PI / 3.) / (sqrt(3.) * cos(phi))) +
-               (cos(theta + M_PI / 3.) * tan(phi) / 3.)) +
-          (epsilon / sqrt(3.)) * tan(phi) - cohesion;
+  const double cohesion = sv[SV_COH], tension_cutoff = sv[SV_TCUT];
+  yf[0] = sqrt(2. / 3.) * I.ct * rho + epsilon / sqrt(3.) - tension_cutoff;
+  yf[1] = sqrt(1.5) * rho * ((I.s3 / (sqrt(3.) * A.cphi)) + (I.c3 * A.tphi / 3.)) +
+          (epsilon / sqrt(3.)) * A.tphi - cohesion;
   int t = FS_ELASTIC;
   if (yf[0] > Tolerance && yf[1] > Tolerance) {
-    const double n_phi = (1. + sin(phi)) / (1. - sin(phi));
+    const double n_phi = (1. + A.sphi) / (1. - A.sphi);
     const double sigma_p = tension_cutoff * n_phi - 2. * cohesion * sqrt(n_phi);
     const double alpha_p = sqrt(1. + n_phi * n_phi) + n_phi;
-    const double h = yf[0] + alpha_p * (sqrt(2. / 3.) * cos(theta - 4. * M_PI / 3.) * rho +
-                                        epsilon / sqrt(3.) - sigma_p);
+    const double h =
+        yf[0] + alpha_p * (sqrt(2. / 3.) * dmath::cos(theta - 4. * M_PI / 3.) * rho +
+                           epsilon / sqrt(3.) - sigma_p);
     t = (h > kDblEps) ? FS_TENSILE : FS_SHEAR;
   }
   if (yf[0] < Tolerance && yf[1] > Tolerance) t = FS_SHEAR;
@@ -322,42 +349,40 @@ MPMX_DEV int mc_yield(double* yf, const double* sv) {  // mohr_coulomb.tcc:139-1
   return t;
 }
 
-// mohr_coulomb.tcc:195-344
+// compute_df_dp (mohr_coulomb.tcc:195-344)
 MPMX_DEV void mc_df_dp(const Material& m, int yt, const double* sv, const double* stress,
-                       double* df, double* dp, double* dp_dq, double* softening) {
-  const double rho = sv[SV_RHO], theta = sv[SV_THETA], phi = sv[SV_PHI], psi = sv[SV_PSI];
+                       const McInv& I, const McAngles& A, double* df, double* dp, double* dp_dq,
+                       double* softening) {
+  const double rho = sv[SV_RHO];
   const double tension_cutoff = sv[SV_TCUT], pdstrain = sv[SV_PDS];
+  const double ct = I.ct, st = I.st;
   double df_depsilon, df_drho, df_dtheta;
   if (yt == FS_TENSILE) {
     df_depsilon = 1. / sqrt(3.);
-    df_drho = sqrt(2. / 3.) * cos(theta);
-    df_dtheta = -sqrt(2. / 3.) * rho * sin(theta);
+    df_drho = sqrt(2. / 3.) * ct;
+    df_dtheta = -sqrt(2. / 3.) * rho * st;
   } else {
-    df_depsilon = tan(phi) / sqrt(3.);
-    df_drho = sqrt(1.5) * ((sin(theta + M_PI / 3.) / (sqrt(3.) * cos(phi))) +
-                           (cos(theta + M_PI / 3.) * tan(phi) / 3.));
-    df_dtheta = sqrt(1.5) * rho *
-                ((cos(theta + M_PI / 3.) / (sqrt(3.) * cos(phi))) -
-                 (sin(theta + M_PI / 3.) * tan(phi) / 3.));
+    df_depsilon = A.tphi / sqrt(3.);
+    df_drho = sqrt(1.5) * ((I.s3 / (sqrt(3.) * A.cphi)) + (I.c3 * A.tphi / 3.));
+    df_dtheta = sqrt(1.5) * rho * ((I.c3 / (sqrt(3.) * A.cphi)) - (I.s3 * A.tphi / 3.));
   }
-  double depsilon_dsigma[6] = {0, 0, 0, 0, 0, 0};
-  for (int i = 0; i < 3; ++i) depsilon_dsigma[i] = 1. / 3. * sqrt(3.);
+  const double depsilon_dsigma = 1. / 3. * sqrt(3.);  // entries 0..2 (3..5 are 0)
   double drho_dsigma[6];
   mu_dq_dsigma(stress, drho_dsigma);
   for (int i = 0; i < 6; ++i) drho_dsigma[i] = drho_dsigma[i] * sqrt(2. / 3.);
   double dtheta_dsigma[6];
   mu_dtheta_dsigma(stress, dtheta_dsigma);
   for (int i = 0; i < 6; ++i)
-    df[i] = ((df_depsilon * depsilon_dsigma[i]) + (df_drho * drho_dsigma[i])) +
+    df[i] = ((df_depsilon * (i < 3 ? depsilon_dsigma : 0.0)) + (df_drho * drho_dsigma[i])) +
             (df_dtheta * dtheta_dsigma[i]);
   if (yt == FS_TENSILE) {
     const double et_value = 0.6;
     const double xit = 0.1;
-    double sqpart = 4. * (1 - et_value * et_value) * cos(theta) * cos(theta) +
-                    5. * et_value * et_value - 4. * et_value;
+    double sqpart = 4. * (1 - et_value * et_value) * ct * ct + 5. * et_value * et_value -
+                    4. * et_value;
     if (sqpart < kDblEps) sqpart = 1.E-5;
-    double rt_den = 2. * (1 - et_value * et_value) * cos(theta) + (2. * et_value - 1) * sqrt(sqpart);
-    const double rt_num = 4. * (1 - et_value * et_value) * cos(theta) * cos(theta) +
+    double rt_den = 2. * (1 - et_value * et_value) * ct + (2. * et_value - 1) * sqrt(sqpart);
+    const double rt_num = 4. * (1 - et_value * et_value) * ct * ct +
                           (2. * et_value - 1) * (2. * et_value - 1);
     if (fabs(rt_den) < kDblEps) rt_den = 1.E-5;
     const double rt = rt_num / (3. * rt_den);
@@ -367,40 +392,40 @@ MPMX_DEV void mc_df_dp(const Material& m, int yt, const double* sv, const double
     const double dp_drho = 1.5 * rho * rt * rt / cl;
     const double dp_depsilon = 1. / sqrt(3.);
     const double drtden_dtheta =
-        -2. * (1 - et_value * et_value) * sin(theta) -
-        (2. * et_value - 1) * 4. * (1 - et_value * et_value) * cos(theta) * sin(theta) /
-            sqrt(4. * (1 - et_value * et_value) * cos(theta) * cos(theta) +
-                 5. * et_value * et_value - 4. * et_value);
-    const double drtnum_dtheta = -8. * (1 - et_value * et_value) * cos(theta) * sin(theta);
+        -2. * (1 - et_value * et_value) * st -
+        (2. * et_value - 1) * 4. * (1 - et_value * et_value) * ct * st /
+            sqrt(4. * (1 - et_value * et_value) * ct * ct + 5. * et_value * et_value -
+                 4. * et_value);
+    const double drtnum_dtheta = -8. * (1 - et_value * et_value) * ct * st;
     const double drt_dtheta =
         (drtnum_dtheta * rt_den - drtden_dtheta * rt_num) / (3. * rt_den * rt_den);
     for (int i = 0; i < 6; ++i)
-      dp[i] = ((dp_depsilon * depsilon_dsigma[i]) + (dp_drho * drho_dsigma[i])) +
+      dp[i] = ((dp_depsilon * (i < 3 ? depsilon_dsigma : 0.0)) + (dp_drho * drho_dsigma[i])) +
               (dp_drt * drt_dtheta * dtheta_dsigma[i]);
     *dp_dq = dp_drho * sqrt(2. / 3.);
   } else {
-    const double r_mc = (3. - sin(phi)) / (6 * cos(phi));
-    double e_val = (3. - sin(phi)) / (3. + sin(phi));
+    const double r_mc = (3. - A.sphi) / (6 * A.cphi);
+    double e_val = (3. - A.sphi) / (3. + A.sphi);
     if (e_val <= 0.5) e_val = 0.5 + 1.E-10;
     if (e_val > 1.) e_val = 1.;
-    double sqpart = (4. * (1 - e_val * e_val) * sq(cos(theta))) + (5 * e_val * e_val) - (4. * e_val);
+    double sqpart = (4. * (1 - e_val * e_val) * sq(ct)) + (5 * e_val * e_val) - (4. * e_val);
     if (sqpart < kDblEps) sqpart = 1.E-5;
-    double mm = (2. * (1 - e_val * e_val) * cos(theta)) + ((2. * e_val - 1) * sqrt(sqpart));
+    double mm = (2. * (1 - e_val * e_val) * ct) + ((2. * e_val - 1) * sqrt(sqpart));
     if (fabs(mm) < kDblEps) mm = 1.E-5;
-    const double l = (4. * (1. - e_val * e_val) * sq(cos(theta))) + sq(2. * e_val - 1.);
+    const double l = (4. * (1. - e_val * e_val) * sq(ct)) + sq(2. * e_val - 1.);
     const double r_mw = (l / mm) * r_mc;
     const double xi = 0.1;
-    double omega = sq(xi * m.c_peak * tan(psi)) + sq(r_mw * sqrt(1.5) * rho);
+    double omega = sq(xi * m.c_peak * A.tpsi) + sq(r_mw * sqrt(1.5) * rho);
     if (omega < kDblEps) omega = 1.E-5;
-    const double dl_dtheta = -8. * (1. - e_val * e_val) * cos(theta) * sin(theta);
-    const double dm_dtheta = (-2. * (1. - e_val * e_val) * sin(theta)) +
+    const double dl_dtheta = -8. * (1. - e_val * e_val) * ct * st;
+    const double dm_dtheta = (-2. * (1. - e_val * e_val) * st) +
                              (0.5 * (2. * e_val - 1.) * dl_dtheta) / sqrt(sqpart);
     const double drmw_dtheta = ((mm * dl_dtheta) - (l * dm_dtheta)) / (mm * mm);
-    const double dp_depsilon = tan(psi) / sqrt(3.);
+    const double dp_depsilon = A.tpsi / sqrt(3.);
     const double dp_drho = 3. * rho * r_mw * r_mw / (2. * sqrt(omega));
     const double dp_dtheta = (3. * rho * rho * r_mw * r_mc * drmw_dtheta) / (2. * sqrt(omega));
     for (int i = 0; i < 6; ++i)
-      dp[i] = ((dp_depsilon * depsilon_dsigma[i]) + (dp_drho * drho_dsigma[i])) +
+      dp[i] = ((dp_depsilon * (i < 3 ? depsilon_dsigma : 0.0)) + (dp_drho * drho_dsigma[i])) +
               (dp_dtheta * dtheta_dsigma[i]);
     *dp_dq = dp_drho * sqrt(2. / 3.);
   }
@@ -409,17 +434,16 @@ MPMX_DEV void mc_df_dp(const Material& m, int yt, const double* sv, const double
     const double dphi_dpstrain = (m.phi_res - m.phi_peak) / (m.pd_res - m.pd_peak);
     const double dc_dpstrain = (m.c_res - m.c_peak) / (m.pd_res - m.pd_peak);
     const double df_dphi = sqrt(1.5) * rho *
-                               ((sin(phi) * sin(theta + M_PI / 3.) /
-                                 (sqrt(3.) * cos(phi) * cos(phi))) +
-                                (cos(theta + M_PI / 3.) / (3. * cos(phi) * cos(phi)))) +
-                           (mu_p(stress) / (cos(phi) * cos(phi)));
+                               ((A.sphi * I.s3 / (sqrt(3.) * A.cphi * A.cphi)) +
+                                (I.c3 / (3. * A.cphi * A.cphi))) +
+                           (mu_p(stress) / (A.cphi * A.cphi));
     const double df_dc = -1.;
     *softening = (-1.) * ((df_dphi * dphi_dpstrain) + (df_dc * dc_dpstrain)) * (*dp_dq);
   }
 }
 
-// mohr_coulomb.tcc:348-513. Updates stress in place, sv holds the 9 state
-// variables; density is the particle's current mass density.
+// compute_stress (mohr_coulomb.tcc:348-513). Updates `stress` in place; sv
+// holds the 9 state variables; density is the particle's mass density.
 MPMX_DEV void mc_stress(const Material& m, double* stress, const double* dstrain, double density,
                         double* sv) {
   const double pdstrain = sv[SV_PDS];
@@ -433,9 +457,10 @@ MPMX_DEV void mc_stress(const Material& m, double* stress, const double* dstrain
       sv[SV_PSI] = m.psi_res;
       sv[SV_COH] = m.c_res;
     }
-    const double apex = sv[SV_COH] / tan(sv[SV_PHI]);
+    const double apex = sv[SV_COH] / dmath::tan(sv[SV_PHI]);
     if (sv[SV_TCUT] > apex) sv[SV_TCUT] = (apex > 1.0e-15 ? apex : 1.0e-15);
   }
+  const McAngles A = mc_angles(sv[SV_PHI], sv[SV_PSI]);
   sv[SV_YIELD] = 0;
   double trial[6];
   {
@@ -449,9 +474,10 @@ MPMX_DEV void mc_stress(const Material& m, double* stress, const double* dstrain
     for (int i = 0; i < 6; ++i) stress[i] = 0.;
     return;
   }
-  mc_invariants(trial, sv);
+  McInv I;
+  mc_invariants(trial, sv, I);
   double yft[2];
-  int ytt = mc_yield(yft, sv);
+  int ytt = mc_yield(yft, sv, I, A);
   if (ytt == FS_ELASTIC) {
     sv[SV_YIELD] = 0;
     for (int i = 0; i < 6; ++i) stress[i] = trial[i];
@@ -460,7 +486,7 @@ MPMX_DEV void mc_stress(const Material& m, double* stress, const double* dstrain
   const double Tolerance = 1E-1;
   double softening_trial = 0., dp_dq_trial = 0.;
   double df_t[6], dp_t[6], tmp[6];
-  mc_df_dp(m, ytt, sv, trial, df_t, dp_t, &dp_dq_trial, &softening_trial);
+  mc_df_dp(m, ytt, sv, trial, I, A, df_t, dp_t, &dp_dq_trial, &softening_trial);
   double yield_trial = 0.;
   if (ytt == FS_TENSILE) {
     sv[SV_YIELD] = 2;
@@ -472,15 +498,15 @@ MPMX_DEV void mc_stress(const Material& m, double* stress, const double* dstrain
   }
   de_tmul(m, df_t, tmp);
   double lambda_trial = yield_trial / (dot6(tmp, dp_t) + softening_trial);
-  mc_invariants(stress, sv);
+  mc_invariants(stress, sv, I);
   double yf[2];
-  const int yt = mc_yield(yf, sv);
+  const int yt = mc_yield(yf, sv, I, A);
   double yield = 1.7976931348623157e308;
   if (yt == FS_TENSILE) yield = yf[0];
   if (yt == FS_SHEAR) yield = yf[1];
   double softening = 0., dp_dq = 0.;
   double df[6], dp[6];
-  mc_df_dp(m, yt, sv, stress, df, dp, &dp_dq, &softening);
+  mc_df_dp(m, yt, sv, stress, I, A, df, dp, &dp_dq, &softening);
   de_tmul(m, df, tmp);
   const double lambda = dot6(tmp, dstrain) / (dot6(tmp, dp) + softening);
   double upd[6];
@@ -504,10 +530,10 @@ MPMX_DEV void mc_stress(const Material& m, double* stress, const double* dstrain
     dpdstrain = lambda_trial * dp_dq_trial;
   }
   for (int itr = 0; itr < 100; ++itr) {
-    mc_invariants(upd, sv);
-    ytt = mc_yield(yft, sv);
+    mc_invariants(upd, sv, I);
+    ytt = mc_yield(yft, sv, I, A);
     if (yft[0] < Tolerance && yft[1] < Tolerance) break;
-    mc_df_dp(m, ytt, sv, upd, df_t, dp_t, &dp_dq_trial, &softening_trial);
+    mc_df_dp(m, ytt, sv, upd, I, A, df_t, dp_t, &dp_dq_trial, &softening_trial);
     if (ytt == FS_TENSILE) yield_trial = yft[0];
     if (ytt == FS_SHEAR) yield_trial = yft[1];
     de_tmul(m, df_t, tmp);
@@ -515,7 +541,10 @@ MPMX_DEV void mc_stress(const Material& m, double* stress, const double* dstrain
     corr(lambda_trial, dp_t);
     dpdstrain += lambda_trial * dp_dq_trial;
   }
-  mc_invariants(upd, sv);
+  // final invariants (mohr_coulomb.tcc:507); the Lode trigonometry is unused
+  sv[SV_EPS] = mu_p(upd) * sqrt(3.);
+  sv[SV_THETA] = mu_lode(upd);
+  sv[SV_RHO] = sqrt(2. * mu_j2(upd));
   sv[SV_PDS] += dpdstrain;
   for (int i = 0; i < 6; ++i) stress[i] = upd[i];
 }

@@ -19,6 +19,7 @@
 #include "../../include/mpmx.h"
 #include "mpmx_internal.h"
 #include "mpmx_kernels.h"
+#include "mpmx_math.h"
 
 using namespace mpmx;
 
@@ -367,7 +368,7 @@ Material make_material(const double* p) {
   m.sv0[SV_PHI] = m.phi_peak;
   m.sv0[SV_PSI] = m.psi_peak;
   m.sv0[SV_COH] = m.c_peak;
-  const double apex = m.c_peak / std::tan(m.phi_peak);
+  const double apex = m.c_peak / dmath::tan(m.phi_peak);
   const double tc = (tcut < apex) ? tcut : apex;
   m.sv0[SV_TCUT] = tc > 1.0e-15 ? tc : 1.0e-15;
   return m;

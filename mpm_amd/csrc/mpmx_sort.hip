@@ -19,12 +19,14 @@ static constexpr int kSortBlock = 256;
 static constexpr int kRounds = 16;
 static constexpr int kTile = kSortBlock * kRounds;
 static constexpr int kWaves = kSortBlock / 64;
+static constexpr int kScanTile = 1024;
 
 static inline int64_t tiles_for(int64_t n) { return (n + kTile - 1) / kTile; }
 
 int64_t radix_sort_temp_ints(int64_t n) {
   const int64_t nb = tiles_for(n) > 0 ? tiles_for(n) : 1;
-  return 4 * n + 256 * nb + 64;
+  const int64_t nt = (256 * nb + kScanTile - 1) / kScanTile;
+  return 4 * n + 256 * nb + nt + 64;
 }
 
 // keys: cell in [0, ncells) else max_key; values: particle id
@@ -51,28 +53,79 @@ __global__ __launch_bounds__(kSortBlock) void k_rs_hist(const int* keys, int64_t
   hist[threadIdx.x * nblocks + blockIdx.x] = bins[threadIdx.x];
 }
 
-// Exclusive scan of `total` ints in place with one 1024-thread block.
-__global__ __launch_bounds__(1024) void k_rs_scan(int* a, int64_t total) {
-  __shared__ int part[1024];
-  const int t = threadIdx.x;
-  const int64_t chunk = (total + 1023) / 1024;
-  const int64_t b = t * chunk;
-  const int64_t e = (b + chunk < total) ? b + chunk : total;
-  int s = 0;
-  for (int64_t i = b; i < e; ++i) s += a[i];
-  part[t] = s;
+// Exclusive scan of `total` ints in place: each 1024-element tile is scanned
+// by one block (4 consecutive elements per thread, wave prefix sums by
+// shuffles), tile totals are scanned by one block, then added back.
+
+__device__ __forceinline__ int wave_incl_scan(int v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int u = __shfl_up(v, off, 64);
+    if (lane >= off) v += u;
+  }
+  return v;
+}
+
+// block-wide exclusive scan of one value per thread (256 threads); returns
+// the exclusive prefix and writes the block total to *total
+__device__ __forceinline__ int block_excl_scan(int v, int* total) {
+  __shared__ int wsum[4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int incl = wave_incl_scan(v);
+  if (lane == 63) wsum[wave] = incl;
   __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
-    const int v = (t >= off) ? part[t - off] : 0;
+  int before = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < 4; ++w) {
+    before += (w < wave) ? wsum[w] : 0;
+    tot += wsum[w];
+  }
+  __syncthreads();
+  *total = tot;
+  return before + incl - v;
+}
+
+__global__ __launch_bounds__(256) void k_rs_scan_tiles(int* a, int64_t total, int* tile_sums) {
+  const int64_t base = blockIdx.x * (int64_t)kScanTile + threadIdx.x * 4;
+  int v[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) v[j] = (base + j < total) ? a[base + j] : 0;
+  const int s = (v[0] + v[1]) + (v[2] + v[3]);
+  int tot;
+  int run = block_excl_scan(s, &tot);
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    if (base + j < total) a[base + j] = run;
+    run += v[j];
+  }
+  if (threadIdx.x == 0) tile_sums[blockIdx.x] = tot;
+}
+
+// exclusive scan of the tile sums (any count) by one block
+__global__ __launch_bounds__(256) void k_rs_scan_sums(int* sums, int64_t n) {
+  __shared__ int carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int64_t b = 0; b < n; b += 256) {
+    const int64_t i = b + threadIdx.x;
+    const int v = i < n ? sums[i] : 0;
+    int tot;
+    const int ex = block_excl_scan(v, &tot);
+    if (i < n) sums[i] = carry + ex;
     __syncthreads();
-    part[t] += v;
+    if (threadIdx.x == 0) carry += tot;
     __syncthreads();
   }
-  int run = part[t] - s;  // exclusive prefix of this chunk
-  for (int64_t i = b; i < e; ++i) {
-    const int v = a[i];
-    a[i] = run;
-    run += v;
+}
+
+__global__ __launch_bounds__(256) void k_rs_scan_add(int* a, int64_t total, const int* sums) {
+  const int add = sums[blockIdx.x];
+  const int64_t base = blockIdx.x * (int64_t)kScanTile;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int64_t i = base + j * 256 + threadIdx.x;
+    if (i < total) a[i] += add;
   }
 }
 
@@ -133,6 +186,8 @@ void radix_sort_by_cell(const int* cell, int64_t n, int max_key, int* keys_out, 
   int* k1 = tmp + 2 * n;
   int* v1 = tmp + 3 * n;
   int* hist = tmp + 4 * n;
+  int* tile_sums = hist + 256 * nb;
+  const int64_t ntiles = (256 * nb + kScanTile - 1) / kScanTile;
   hipLaunchKernelGGL(k_rs_init, dim3((unsigned)((n + kSortBlock - 1) / kSortBlock)),
                      dim3(kSortBlock), 0, st, cell, n, max_key, k0, v0);
   int bits = 1;
@@ -156,7 +211,11 @@ void radix_sort_by_cell(const int* cell, int64_t n, int max_key, int* keys_out, 
     const int shift = 8 * pass;
     hipLaunchKernelGGL(k_rs_hist, dim3((unsigned)nb), dim3(kSortBlock), 0, st, sk, n, shift, hist,
                        nb);
-    hipLaunchKernelGGL(k_rs_scan, dim3(1), dim3(1024), 0, st, hist, 256 * nb);
+    hipLaunchKernelGGL(k_rs_scan_tiles, dim3((unsigned)ntiles), dim3(256), 0, st, hist, 256 * nb,
+                       tile_sums);
+    hipLaunchKernelGGL(k_rs_scan_sums, dim3(1), dim3(256), 0, st, tile_sums, ntiles);
+    hipLaunchKernelGGL(k_rs_scan_add, dim3((unsigned)ntiles), dim3(256), 0, st, hist, 256 * nb,
+                       tile_sums);
     hipLaunchKernelGGL(k_rs_scatter, dim3((unsigned)nb), dim3(kSortBlock), 0, st, sk, sv, dk, dv,
                        n, shift, hist, nb);
     sk = dk;

@@ -35,15 +35,19 @@ NODE_FIELDS = dict(mass=0, momentum=1, velocity=2, acceleration=3, fext=4, fint=
 PARTICLE_FIELDS = dict(x=0, v=1, stress=2, volume=3, mass=4, density=5, u=6, strain=7, sv=8)
 
 
+_BUILDS = {"portable": "liboracle.so", "platform": "liboracle_libm.so",
+           "omp": "liboracle_omp.so"}
+
+
 def build(force=False):
-    """Compile both oracle builds with the committed Makefile."""
-    if force or not all(os.path.exists(os.path.join(_HERE, f))
-                        for f in ("liboracle.so", "liboracle_omp.so")):
+    """Compile the oracle builds with the committed Makefile."""
+    if force or not all(os.path.exists(os.path.join(_HERE, f)) for f in _BUILDS.values()):
         subprocess.run(["make", "-s", "-C", _HERE] + (["-B"] if force else []), check=True)
 
 
-def _lib(omp=False):
-    name = "liboracle_omp.so" if omp else "liboracle.so"
+def _lib(omp=False, libm="portable"):
+    """omp: OpenMP build (platform libm); else libm 'portable' or 'platform'."""
+    name = _BUILDS["omp"] if omp else _BUILDS[libm]
     if name not in _LIBS:
         path = os.path.join(_HERE, name)
         if not os.path.exists(path):
@@ -66,6 +70,8 @@ def _lib(omp=False):
         lib.orc_kat_mc_yield.restype = C.c_int
         lib.orc_kat_node_update.restype = C.c_int
         lib.orc_threads.restype = C.c_int
+        lib.orc_kat_math.restype = C.c_double
+        lib.orc_kat_math.argtypes = [C.c_int, C.c_double]
         _LIBS[name] = lib
     return _LIBS[name]
 
@@ -92,8 +98,9 @@ class OracleError(RuntimeError):
 class Oracle:
     """One reference-semantics solver instance (particles, nodes, cells)."""
 
-    def __init__(self, dim, scheme="usf", velocity_update="flip", blending_ratio=1.0, omp=False):
-        self.lib = _lib(omp)
+    def __init__(self, dim, scheme="usf", velocity_update="flip", blending_ratio=1.0, omp=False,
+                 libm="portable"):
+        self.lib = _lib(omp, libm)
         self.dim = dim
         self.nn = 4 if dim == 2 else 8
         ptr = self.lib.orc_create(dim, SCHEMES[scheme], VELOCITY_UPDATES[velocity_update],
@@ -261,6 +268,14 @@ class Oracle:
 
 
 # -- function-level known-answer hooks ----------------------------------------
+def math_fn(which, x, libm="portable"):
+    """Elementary function of one oracle build (see orc_kat_math)."""
+    lib = _lib(libm=libm)
+    names = ("sin", "cos", "tan", "acos", "pow_1p5", "pow_m1p5", "pow_m2p5")
+    w = names.index(which)
+    return np.array([lib.orc_kat_math(w, float(v)) for v in np.atleast_1d(x)])
+
+
 def shapefn(dim, xi):
     lib = _lib()
     nn = 4 if dim == 2 else 8

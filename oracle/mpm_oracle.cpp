@@ -34,6 +34,38 @@
 #include <string>
 #include <vector>
 
+// Elementary functions. The reference calls the platform libm; built with
+// -DORC_PLATFORM_LIBM this file does too. The default build uses the
+// deterministic FP64 functions of mpm_amd/csrc/mpmx_math.h (the same ones the
+// GPU kernels use), so that the oracle and the device agree bit for bit.
+#ifdef ORC_PLATFORM_LIBM
+namespace orc {
+namespace mth {
+inline double sin(double x) { return std::sin(x); }
+inline double cos(double x) { return std::cos(x); }
+inline double tan(double x) { return std::tan(x); }
+inline double acos(double x) { return std::acos(x); }
+inline double pow(double x, double y) { return std::pow(x, y); }
+}  // namespace mth
+}  // namespace orc
+#else
+#include "mpmx_math.h"
+namespace orc {
+namespace mth {
+inline double sin(double x) { return mpmx::dmath::sin(x); }
+inline double cos(double x) { return mpmx::dmath::cos(x); }
+inline double tan(double x) { return mpmx::dmath::tan(x); }
+inline double acos(double x) { return mpmx::dmath::acos(x); }
+inline double pow(double x, double y) {
+  if (y == 1.5) return mpmx::dmath::pow_1p5(x);
+  if (y == -1.5) return mpmx::dmath::pow_m1p5(x);
+  if (y == -2.5) return mpmx::dmath::pow_m2p5(x);
+  return std::pow(x, y);
+}
+}  // namespace mth
+}  // namespace orc
+#endif
+
 #ifdef ORC_OMP
 #include <omp.h>
 #define ORC_PRAGMA(x) _Pragma(#x)
@@ -131,9 +163,9 @@ static void mc_initialise_state(const Material& m, double* sv) {
   sv[SV_PHI] = m.phi_peak;
   sv[SV_PSI] = m.psi_peak;
   sv[SV_COH] = m.c_peak;
-  sv[SV_TCUT] = check_low((m.tcut < m.c_peak / std::tan(m.phi_peak))
+  sv[SV_TCUT] = check_low((m.tcut < m.c_peak / mth::tan(m.phi_peak))
                               ? m.tcut
-                              : m.c_peak / std::tan(m.phi_peak));
+                              : m.c_peak / mth::tan(m.phi_peak));
   sv[SV_EPS] = 0.;
   sv[SV_RHO] = 0.;
   sv[SV_THETA] = 0.;
@@ -174,7 +206,7 @@ static inline void mu_dev(const double* s, double* d) {  // :8-19
   for (int i = 0; i < 6; ++i) d[i] = s[i];
   for (int i = 0; i < 3; ++i) d[i] -= p;
 }
-static inline double sq(double x) { return x * x; }  // std::pow(x, 2)
+static inline double sq(double x) { return x * x; }  // mth::pow(x, 2)
 static inline double mu_j2(const double* s) {  // :22-32
   return (sq(s[0] - s[1]) + sq(s[1] - s[2]) + sq(s[0] - s[2])) / 6.0 +
          sq(s[3]) + sq(s[4]) + sq(s[5]);
@@ -193,10 +225,10 @@ static inline double mu_lode_angle(const double* s, double tol = kEps) {  // :64
   const double j2 = mu_j2(s);
   const double j3 = mu_j3(s);
   double v = 0.0;
-  if (std::abs(j2) > tol) v = (3. * std::sqrt(3.) / 2.) * (j3 / std::pow(j2, 1.5));
+  if (std::abs(j2) > tol) v = (3. * std::sqrt(3.) / 2.) * (j3 / mth::pow(j2, 1.5));
   if (v > 1.0) v = 1.0;
   if (v < -1.0) v = -1.0;
-  return (1. / 3.) * std::acos(v);
+  return (1. / 3.) * mth::acos(v);
 }
 static inline void mu_dq_dsigma(const double* s, double* out) {  // :95-116
   const double q = mu_q(s);
@@ -239,9 +271,9 @@ static inline void mu_dtheta_dsigma(const double* s, double tol, double* out) { 
   double dr_dj3 = 3.0 / 2.0 * std::sqrt(3.0);
   double dtheta_dr = -1.0 / 3.0;
   if (std::abs(j2) > tol) {
-    double r = j3 / 2.0 * std::pow(j2 / 3.0, -1.5);
-    dr_dj2 *= std::pow(j2, -2.5);
-    dr_dj3 *= std::pow(j2, -1.5);
+    double r = j3 / 2.0 * mth::pow(j2 / 3.0, -1.5);
+    dr_dj2 *= mth::pow(j2, -2.5);
+    dr_dj3 *= mth::pow(j2, -1.5);
     const double factor = (std::abs(1 - r * r) < tol) ? tol : std::abs(1 - r * r);
     dtheta_dr = -1.0 / (3.0 * std::sqrt(factor));
   }
@@ -268,19 +300,19 @@ static int mc_yield_state(double* yf, const double* sv) {
   const double phi = sv[SV_PHI];
   const double cohesion = sv[SV_COH];
   const double tension_cutoff = sv[SV_TCUT];
-  yf[0] = std::sqrt(2. / 3.) * std::cos(theta) * rho + epsilon / std::sqrt(3.) -
+  yf[0] = std::sqrt(2. / 3.) * mth::cos(theta) * rho + epsilon / std::sqrt(3.) -
           tension_cutoff;
   yf[1] = std::sqrt(1.5) * rho *
-              ((std::sin(theta + M_PI / 3.) / (std::sqrt(3.) * std::cos(phi))) +
-               (std::cos(theta + M_PI / 3.) * std::tan(phi) / 3.)) +
-          (epsilon / std::sqrt(3.)) * std::tan(phi) - cohesion;
+              ((mth::sin(theta + M_PI / 3.) / (std::sqrt(3.) * mth::cos(phi))) +
+               (mth::cos(theta + M_PI / 3.) * mth::tan(phi) / 3.)) +
+          (epsilon / std::sqrt(3.)) * mth::tan(phi) - cohesion;
   int yield_type = FS_ELASTIC;
   if (yf[0] > Tolerance && yf[1] > Tolerance) {
-    const double n_phi = (1. + std::sin(phi)) / (1. - std::sin(phi));
+    const double n_phi = (1. + mth::sin(phi)) / (1. - mth::sin(phi));
     const double sigma_p = tension_cutoff * n_phi - 2. * cohesion * std::sqrt(n_phi);
     const double alpha_p = std::sqrt(1. + n_phi * n_phi) + n_phi;
     const double h =
-        yf[0] + alpha_p * (std::sqrt(2. / 3.) * std::cos(theta - 4. * M_PI / 3.) * rho +
+        yf[0] + alpha_p * (std::sqrt(2. / 3.) * mth::cos(theta - 4. * M_PI / 3.) * rho +
                            epsilon / std::sqrt(3.) - sigma_p);
     if (h > kEps)
       yield_type = FS_TENSILE;
@@ -304,15 +336,15 @@ static void mc_df_dp(const Material& m, int yield_type, const double* sv, const 
   double df_depsilon, df_drho, df_dtheta;
   if (yield_type == FS_TENSILE) {
     df_depsilon = 1. / std::sqrt(3.);
-    df_drho = std::sqrt(2. / 3.) * std::cos(theta);
-    df_dtheta = -std::sqrt(2. / 3.) * rho * std::sin(theta);
+    df_drho = std::sqrt(2. / 3.) * mth::cos(theta);
+    df_dtheta = -std::sqrt(2. / 3.) * rho * mth::sin(theta);
   } else {
-    df_depsilon = std::tan(phi) / std::sqrt(3.);
-    df_drho = std::sqrt(1.5) * ((std::sin(theta + M_PI / 3.) / (std::sqrt(3.) * std::cos(phi))) +
-                                (std::cos(theta + M_PI / 3.) * std::tan(phi) / 3.));
+    df_depsilon = mth::tan(phi) / std::sqrt(3.);
+    df_drho = std::sqrt(1.5) * ((mth::sin(theta + M_PI / 3.) / (std::sqrt(3.) * mth::cos(phi))) +
+                                (mth::cos(theta + M_PI / 3.) * mth::tan(phi) / 3.));
     df_dtheta = std::sqrt(1.5) * rho *
-                ((std::cos(theta + M_PI / 3.) / (std::sqrt(3.) * std::cos(phi))) -
-                 (std::sin(theta + M_PI / 3.) * std::tan(phi) / 3.));
+                ((mth::cos(theta + M_PI / 3.) / (std::sqrt(3.) * mth::cos(phi))) -
+                 (mth::sin(theta + M_PI / 3.) * mth::tan(phi) / 3.));
   }
   double depsilon_dsigma[6] = {0, 0, 0, 0, 0, 0};
   for (int i = 0; i < 3; ++i) depsilon_dsigma[i] = 1. / 3. * std::sqrt(3.);
@@ -327,12 +359,12 @@ static void mc_df_dp(const Material& m, int yield_type, const double* sv, const 
   if (yield_type == FS_TENSILE) {
     const double et_value = 0.6;
     const double xit = 0.1;
-    double sqpart = 4. * (1 - et_value * et_value) * std::cos(theta) * std::cos(theta) +
+    double sqpart = 4. * (1 - et_value * et_value) * mth::cos(theta) * mth::cos(theta) +
                     5. * et_value * et_value - 4. * et_value;
     if (sqpart < kEps) sqpart = 1.E-5;
-    double rt_den = 2. * (1 - et_value * et_value) * std::cos(theta) +
+    double rt_den = 2. * (1 - et_value * et_value) * mth::cos(theta) +
                     (2. * et_value - 1) * std::sqrt(sqpart);
-    const double rt_num = 4. * (1 - et_value * et_value) * std::cos(theta) * std::cos(theta) +
+    const double rt_num = 4. * (1 - et_value * et_value) * mth::cos(theta) * mth::cos(theta) +
                           (2. * et_value - 1) * (2. * et_value - 1);
     if (std::fabs(rt_den) < kEps) rt_den = 1.E-5;
     const double rt = rt_num / (3. * rt_den);
@@ -344,11 +376,11 @@ static void mc_df_dp(const Material& m, int yield_type, const double* sv, const 
                                                1.5 * rt * rt * rho * rho));
     const double dp_depsilon = 1. / std::sqrt(3.);
     const double drtden_dtheta =
-        -2. * (1 - et_value * et_value) * std::sin(theta) -
-        (2. * et_value - 1) * 4. * (1 - et_value * et_value) * std::cos(theta) * std::sin(theta) /
-            std::sqrt(4. * (1 - et_value * et_value) * std::cos(theta) * std::cos(theta) +
+        -2. * (1 - et_value * et_value) * mth::sin(theta) -
+        (2. * et_value - 1) * 4. * (1 - et_value * et_value) * mth::cos(theta) * mth::sin(theta) /
+            std::sqrt(4. * (1 - et_value * et_value) * mth::cos(theta) * mth::cos(theta) +
                       5. * et_value * et_value - 4. * et_value);
-    const double drtnum_dtheta = -8. * (1 - et_value * et_value) * std::cos(theta) * std::sin(theta);
+    const double drtnum_dtheta = -8. * (1 - et_value * et_value) * mth::cos(theta) * mth::sin(theta);
     const double drt_dtheta =
         (drtnum_dtheta * rt_den - drtden_dtheta * rt_num) / (3. * rt_den * rt_den);
     for (int i = 0; i < 6; ++i)
@@ -356,25 +388,25 @@ static void mc_df_dp(const Material& m, int yield_type, const double* sv, const 
                      (dp_drt * drt_dtheta * dtheta_dsigma[i]);
     *dp_dq = dp_drho * std::sqrt(2. / 3.);
   } else {
-    const double r_mc = (3. - std::sin(phi)) / (6 * std::cos(phi));
-    double e_val = (3. - std::sin(phi)) / (3. + std::sin(phi));
+    const double r_mc = (3. - mth::sin(phi)) / (6 * mth::cos(phi));
+    double e_val = (3. - mth::sin(phi)) / (3. + mth::sin(phi));
     if (e_val <= 0.5) e_val = 0.5 + 1.E-10;
     if (e_val > 1.) e_val = 1.;
-    double sqpart = (4. * (1 - e_val * e_val) * sq(std::cos(theta))) + (5 * e_val * e_val) -
+    double sqpart = (4. * (1 - e_val * e_val) * sq(mth::cos(theta))) + (5 * e_val * e_val) -
                     (4. * e_val);
     if (sqpart < kEps) sqpart = 1.E-5;
-    double mm = (2. * (1 - e_val * e_val) * std::cos(theta)) + ((2. * e_val - 1) * std::sqrt(sqpart));
+    double mm = (2. * (1 - e_val * e_val) * mth::cos(theta)) + ((2. * e_val - 1) * std::sqrt(sqpart));
     if (std::fabs(mm) < kEps) mm = 1.E-5;
-    const double l = (4. * (1. - e_val * e_val) * sq(std::cos(theta))) + sq(2. * e_val - 1.);
+    const double l = (4. * (1. - e_val * e_val) * sq(mth::cos(theta))) + sq(2. * e_val - 1.);
     const double r_mw = (l / mm) * r_mc;
     const double xi = 0.1;
-    double omega = sq(xi * m.c_peak * std::tan(psi)) + sq(r_mw * std::sqrt(1.5) * rho);
+    double omega = sq(xi * m.c_peak * mth::tan(psi)) + sq(r_mw * std::sqrt(1.5) * rho);
     if (omega < kEps) omega = 1.E-5;
-    const double dl_dtheta = -8. * (1. - e_val * e_val) * std::cos(theta) * std::sin(theta);
-    const double dm_dtheta = (-2. * (1. - e_val * e_val) * std::sin(theta)) +
+    const double dl_dtheta = -8. * (1. - e_val * e_val) * mth::cos(theta) * mth::sin(theta);
+    const double dm_dtheta = (-2. * (1. - e_val * e_val) * mth::sin(theta)) +
                              (0.5 * (2. * e_val - 1.) * dl_dtheta) / std::sqrt(sqpart);
     const double drmw_dtheta = ((mm * dl_dtheta) - (l * dm_dtheta)) / (mm * mm);
-    const double dp_depsilon = std::tan(psi) / std::sqrt(3.);
+    const double dp_depsilon = mth::tan(psi) / std::sqrt(3.);
     const double dp_drho = 3. * rho * r_mw * r_mw / (2. * std::sqrt(omega));
     const double dp_dtheta = (3. * rho * rho * r_mw * r_mc * drmw_dtheta) / (2. * std::sqrt(omega));
     for (int i = 0; i < 6; ++i)
@@ -389,10 +421,10 @@ static void mc_df_dp(const Material& m, int yield_type, const double* sv, const 
     dphi_dpstrain = (m.phi_res - m.phi_peak) / (m.pd_res - m.pd_peak);
     dc_dpstrain = (m.c_res - m.c_peak) / (m.pd_res - m.pd_peak);
     double df_dphi = std::sqrt(1.5) * rho *
-                         ((std::sin(phi) * std::sin(theta + M_PI / 3.) /
-                           (std::sqrt(3.) * std::cos(phi) * std::cos(phi))) +
-                          (std::cos(theta + M_PI / 3.) / (3. * std::cos(phi) * std::cos(phi)))) +
-                     (mu_p(stress) / (std::cos(phi) * std::cos(phi)));
+                         ((mth::sin(phi) * mth::sin(theta + M_PI / 3.) /
+                           (std::sqrt(3.) * mth::cos(phi) * mth::cos(phi))) +
+                          (mth::cos(theta + M_PI / 3.) / (3. * mth::cos(phi) * mth::cos(phi)))) +
+                     (mu_p(stress) / (mth::cos(phi) * mth::cos(phi)));
     double df_dc = -1.;
     *softening = (-1.) * ((df_dphi * dphi_dpstrain) + (df_dc * dc_dpstrain)) * (*dp_dq);
   }
@@ -417,7 +449,7 @@ static void mc_compute_stress(const Material& m, const double* stress, const dou
       sv[SV_PSI] = m.psi_res;
       sv[SV_COH] = m.c_res;
     }
-    const double apex = sv[SV_COH] / std::tan(sv[SV_PHI]);
+    const double apex = sv[SV_COH] / mth::tan(sv[SV_PHI]);
     if (sv[SV_TCUT] > apex) sv[SV_TCUT] = check_low(apex);
   }
   sv[SV_YIELD] = 0;
@@ -1784,6 +1816,20 @@ int orc_kat_node_update(int dim, double mass, const double* fext, const double* 
 }
 
 // Friction alone on given acceleration/velocity (node.tcc:510-657).
+// Elementary functions of this build: which 0 sin, 1 cos, 2 tan, 3 acos,
+// 4 x^1.5, 5 x^-1.5, 6 x^-2.5.
+double orc_kat_math(int which, double x) {
+  switch (which) {
+    case 0: return orc::mth::sin(x);
+    case 1: return orc::mth::cos(x);
+    case 2: return orc::mth::tan(x);
+    case 3: return orc::mth::acos(x);
+    case 4: return orc::mth::pow(x, 1.5);
+    case 5: return orc::mth::pow(x, -1.5);
+    default: return orc::mth::pow(x, -2.5);
+  }
+}
+
 void orc_kat_friction(int dim, double* acc, const double* vel, double dt, int f_dir, int f_sign,
                       double f_mu) {
   orc::Solver s;
