@@ -2,8 +2,9 @@
 
 ``Solver`` is the thin host-side object over one ``mpmx_ctx``: the same
 operations, argument meaning and particle order as the C-ABI, with numpy
-arrays. It loads ``mpm_amd/libmpmx.so`` and raises if the library or a HIP
-device is missing; there is no CPU fallback.
+arrays. It loads ``mpm_amd/libmpmx.so`` (or the build named by the
+``MPMX_LIB`` environment variable) and raises if the library or a HIP device
+is missing; there is no CPU fallback.
 """
 import ctypes as C
 import os
@@ -11,7 +12,7 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libmpmx.so")
+LIB_PATH = os.environ.get("MPMX_LIB") or os.path.join(HERE, "libmpmx.so")
 
 OK, E_ARG, E_STATE, E_HIP, E_UNLOCATABLE, E_UNSUPPORTED = 0, -1, -2, -3, -4, -5
 SCHEMES = {"usf": 0, "usl": 1, "musl": 2}

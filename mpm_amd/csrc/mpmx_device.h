@@ -310,27 +310,32 @@ MPMX_DEV McAngles mc_angles(double phi, double psi) {
   return a;
 }
 
-// invariants plus the trigonometry of the Lode angle
+// Haigh-Westergaard invariants (the state variables epsilon, rho, theta)
+// plus the trigonometry of the Lode angle.
 struct McInv {
+  double eps, rho, theta;
   double st, ct;    // sin, cos of theta
   double s3, c3;    // sin, cos of theta + pi/3
 };
 
-// compute_stress_invariants (mohr_coulomb.tcc:124-134)
-MPMX_DEV void mc_invariants(const double* s, double* sv, McInv& I) {
-  sv[SV_EPS] = mu_p(s) * sqrt(3.);
-  sv[SV_THETA] = mu_lode(s);
-  sv[SV_RHO] = sqrt(2. * mu_j2(s));
-  const double theta = sv[SV_THETA];
-  dmath::sincos(theta, &I.st, &I.ct);
-  dmath::sincos(theta + M_PI / 3., &I.s3, &I.c3);
+// compute_stress_invariants (mohr_coulomb.tcc:124-134), without the trigonometry
+MPMX_DEV void mc_invariants_only(const double* s, McInv& I) {
+  I.eps = mu_p(s) * sqrt(3.);
+  I.theta = mu_lode(s);
+  I.rho = sqrt(2. * mu_j2(s));
+}
+
+MPMX_DEV void mc_invariants(const double* s, McInv& I) {
+  mc_invariants_only(s, I);
+  dmath::sincos(I.theta, &I.st, &I.ct);
+  dmath::sincos(I.theta + M_PI / 3., &I.s3, &I.c3);
 }
 
 // compute_yield_state (mohr_coulomb.tcc:139-191)
-MPMX_DEV int mc_yield(double* yf, const double* sv, const McInv& I, const McAngles& A) {
+MPMX_DEV int mc_yield(double* yf, const McInv& I, const McAngles& A, double cohesion,
+                      double tension_cutoff) {
   const double Tolerance = -1E-1;
-  const double epsilon = sv[SV_EPS], rho = sv[SV_RHO], theta = sv[SV_THETA];
-  const double cohesion = sv[SV_COH], tension_cutoff = sv[SV_TCUT];
+  const double epsilon = I.eps, rho = I.rho, theta = I.theta;
   yf[0] = sqrt(2. / 3.) * I.ct * rho + epsilon / sqrt(3.) - tension_cutoff;
   yf[1] = sqrt(1.5) * rho * ((I.s3 / (sqrt(3.) * A.cphi)) + (I.c3 * A.tphi / 3.)) +
           (epsilon / sqrt(3.)) * A.tphi - cohesion;
@@ -349,12 +354,11 @@ MPMX_DEV int mc_yield(double* yf, const double* sv, const McInv& I, const McAngl
   return t;
 }
 
-// compute_df_dp (mohr_coulomb.tcc:195-344)
-MPMX_DEV void mc_df_dp(const Material& m, int yt, const double* sv, const double* stress,
-                       const McInv& I, const McAngles& A, double* df, double* dp, double* dp_dq,
-                       double* softening) {
-  const double rho = sv[SV_RHO];
-  const double tension_cutoff = sv[SV_TCUT], pdstrain = sv[SV_PDS];
+// compute_df_dp (mohr_coulomb.tcc:195-344); I holds the invariants of `stress`
+MPMX_DEV void mc_df_dp(const Material& m, int yt, double tension_cutoff, double pdstrain,
+                       const double* stress, const McInv& I, const McAngles& A, double* df,
+                       double* dp, double* dp_dq, double* softening) {
+  const double rho = I.rho;
   const double ct = I.ct, st = I.st;
   double df_depsilon, df_drho, df_dtheta;
   if (yt == FS_TENSILE) {
@@ -474,77 +478,91 @@ MPMX_DEV void mc_stress(const Material& m, double* stress, const double* dstrain
     for (int i = 0; i < 6; ++i) stress[i] = 0.;
     return;
   }
+  const double Tolerance = 1E-1;
+  const double cohesion = sv[SV_COH], tcut = sv[SV_TCUT];
   McInv I;
-  mc_invariants(trial, sv, I);
-  double yft[2];
-  int ytt = mc_yield(yft, sv, I, A);
-  if (ytt == FS_ELASTIC) {
+  double yf[2];
+  int yt = FS_ELASTIC;
+  {
+    mc_invariants(trial, I);
+    yt = mc_yield(yf, I, A, cohesion, tcut);
+  }
+  if (yt == FS_ELASTIC) {
     sv[SV_YIELD] = 0;
+    sv[SV_EPS] = I.eps;
+    sv[SV_THETA] = I.theta;
+    sv[SV_RHO] = I.rho;
     for (int i = 0; i < 6; ++i) stress[i] = trial[i];
     return;
   }
-  const double Tolerance = 1E-1;
-  double softening_trial = 0., dp_dq_trial = 0.;
-  double df_t[6], dp_t[6], tmp[6];
-  mc_df_dp(m, ytt, sv, trial, I, A, df_t, dp_t, &dp_dq_trial, &softening_trial);
-  double yield_trial = 0.;
-  if (ytt == FS_TENSILE) {
-    sv[SV_YIELD] = 2;
-    yield_trial = yft[0];
+  sv[SV_YIELD] = (yt == FS_TENSILE) ? 2 : 1;
+  double yield_trial = (yt == FS_TENSILE) ? yf[0] : yf[1];
+  // The reference evaluates df/dsigma, dp/dsigma at both the trial and the
+  // current stress and corrects with one pair: the current one if
+  // |yield(current)| < 0.1, else the trial one. Only the used pair is
+  // evaluated (compute_df_dp is pure, so the bits are the reference's). x is
+  // the stress at which the next correction evaluates df/dp.
+  double x[6];
+  bool use_current = false;
+  {
+    McInv Ic;
+    mc_invariants(stress, Ic);
+    double yc[2];
+    const int ytc = mc_yield(yc, Ic, A, cohesion, tcut);
+    double yield = 1.7976931348623157e308;
+    if (ytc == FS_TENSILE) yield = yc[0];
+    if (ytc == FS_SHEAR) yield = yc[1];
+    use_current = fabs(yield) < Tolerance;
+    if (use_current) {
+      I = Ic;
+      yt = ytc;
+    }
+    for (int i = 0; i < 6; ++i) x[i] = use_current ? stress[i] : trial[i];
   }
-  if (ytt == FS_SHEAR) {
-    sv[SV_YIELD] = 1;
-    yield_trial = yft[1];
-  }
-  de_tmul(m, df_t, tmp);
-  double lambda_trial = yield_trial / (dot6(tmp, dp_t) + softening_trial);
-  mc_invariants(stress, sv, I);
-  double yf[2];
-  const int yt = mc_yield(yf, sv, I, A);
-  double yield = 1.7976931348623157e308;
-  if (yt == FS_TENSILE) yield = yf[0];
-  if (yt == FS_SHEAR) yield = yf[1];
-  double softening = 0., dp_dq = 0.;
-  double df[6], dp[6];
-  mc_df_dp(m, yt, sv, stress, I, A, df, dp, &dp_dq, &softening);
-  de_tmul(m, df, tmp);
-  const double lambda = dot6(tmp, dstrain) / (dot6(tmp, dp) + softening);
   double upd[6];
   for (int i = 0; i < 6; ++i) upd[i] = trial[i];
   double dpdstrain = 0.;
-  // upd += -(lam * D * dpv): each entry sums (lam*D_ik)*dpv_k over k in order
-  auto corr = [&](double lam, const double* dpv) {
+  bool converged = false;
+  // it = 0: the first correction; it = 1..100: iterations 0..99 of the
+  // reference's correction loop, each after the yield check of `upd`.
+#pragma nounroll
+  for (int it = 0;; ++it) {
+    double df[6], dp[6], tmp[6], dp_dq = 0., softening = 0.;
+    mc_df_dp(m, yt, tcut, pdstrain, x, I, A, df, dp, &dp_dq, &softening);
+    de_tmul(m, df, tmp);
+    // first, current: lambda = (df^T D).dstrain / ((df^T D).dp + softening)
+    // otherwise: lambda_trial = yield_trial / ((df^T D).dp + softening)
+    const double lam = (use_current ? dot6(tmp, dstrain) : yield_trial) / (dot6(tmp, dp) + softening);
+    // upd += -(lam * D * dp): each entry sums (lam*D_ik)*dp_k over k in order
     const double la1 = lam * m.a1, la2 = lam * m.a2, lg = lam * m.G;
-    upd[0] += -(((la1 * dpv[0] + la2 * dpv[1]) + la2 * dpv[2]));
-    upd[1] += -(((la2 * dpv[0] + la1 * dpv[1]) + la2 * dpv[2]));
-    upd[2] += -(((la2 * dpv[0] + la2 * dpv[1]) + la1 * dpv[2]));
-    upd[3] += -(lg * dpv[3]);
-    upd[4] += -(lg * dpv[4]);
-    upd[5] += -(lg * dpv[5]);
-  };
-  if (fabs(yield) < Tolerance) {
-    corr(lambda, dp);
-    dpdstrain = lambda * dp_dq;
-  } else {
-    corr(lambda_trial, dp_t);
-    dpdstrain = lambda_trial * dp_dq_trial;
+    upd[0] += -(((la1 * dp[0] + la2 * dp[1]) + la2 * dp[2]));
+    upd[1] += -(((la2 * dp[0] + la1 * dp[1]) + la2 * dp[2]));
+    upd[2] += -(((la2 * dp[0] + la2 * dp[1]) + la1 * dp[2]));
+    upd[3] += -(lg * dp[3]);
+    upd[4] += -(lg * dp[4]);
+    upd[5] += -(lg * dp[5]);
+    if (it == 0)
+      dpdstrain = lam * dp_dq;
+    else
+      dpdstrain += lam * dp_dq;
+    use_current = false;
+    if (it == 100) break;
+    mc_invariants(upd, I);
+    yt = mc_yield(yf, I, A, cohesion, tcut);
+    if (yf[0] < Tolerance && yf[1] < Tolerance) {
+      converged = true;
+      break;
+    }
+    if (yt == FS_TENSILE) yield_trial = yf[0];
+    if (yt == FS_SHEAR) yield_trial = yf[1];
+    for (int i = 0; i < 6; ++i) x[i] = upd[i];
   }
-  for (int itr = 0; itr < 100; ++itr) {
-    mc_invariants(upd, sv, I);
-    ytt = mc_yield(yft, sv, I, A);
-    if (yft[0] < Tolerance && yft[1] < Tolerance) break;
-    mc_df_dp(m, ytt, sv, upd, I, A, df_t, dp_t, &dp_dq_trial, &softening_trial);
-    if (ytt == FS_TENSILE) yield_trial = yft[0];
-    if (ytt == FS_SHEAR) yield_trial = yft[1];
-    de_tmul(m, df_t, tmp);
-    lambda_trial = yield_trial / (dot6(tmp, dp_t) + softening_trial);
-    corr(lambda_trial, dp_t);
-    dpdstrain += lambda_trial * dp_dq_trial;
-  }
-  // final invariants (mohr_coulomb.tcc:507); the Lode trigonometry is unused
-  sv[SV_EPS] = mu_p(upd) * sqrt(3.);
-  sv[SV_THETA] = mu_lode(upd);
-  sv[SV_RHO] = sqrt(2. * mu_j2(upd));
+  // final invariants (mohr_coulomb.tcc:507): after a converged loop they are
+  // those just computed from the unchanged `upd`
+  if (!converged) mc_invariants_only(upd, I);
+  sv[SV_EPS] = I.eps;
+  sv[SV_THETA] = I.theta;
+  sv[SV_RHO] = I.rho;
   sv[SV_PDS] += dpdstrain;
   for (int i = 0; i < 6; ++i) stress[i] = upd[i];
 }

@@ -18,6 +18,20 @@ namespace mpmx {
 
 static constexpr int kBlock = 256;
 
+// Occupancy floor (waves per SIMD) for the kernels that run the constitutive
+// update; 0 leaves the register budget to the compiler. Left alone, the
+// compiler schedules the Mohr-Coulomb return mapping for 1 wave per SIMD
+// (>256 registers); 2 fits in 256 registers without spills and halves the
+// kernel time on the 2M-particle collapse, 3 and 4 spill to scratch and lose.
+#ifndef MPMX_MC_WAVES
+#define MPMX_MC_WAVES 2
+#endif
+#if MPMX_MC_WAVES > 0
+#define MPMX_MC_ATTR __attribute__((amdgpu_waves_per_eu(MPMX_MC_WAVES)))
+#else
+#define MPMX_MC_ATTR
+#endif
+
 MPMX_DEV void atomic_add(double* p, double v) { unsafeAtomicAdd(p, v); }
 
 template <int D>
@@ -125,6 +139,25 @@ MPMX_DEV void stress_strain(const Mesh& M, const Particles& P, const Material* m
   P.vol[p] = vol;
   P.rho[p] = rho;
   *vol_out = vol;
+  // update_deformation_gradient (particle.tcc:1654-1659): rows 0..D-1 of
+  // F <- ΔF F; row 2 of a 2D ΔF is (0,0,1) and leaves F's row 2 unchanged.
+  // Independent of the stress, so it is done first and ΔF dies here.
+  {
+    double Fr[D][3];
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) Fr[a][j] = P.F[(a * 3 + j) * np + p];
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        double t = R[a][0] * Fr[0][j];
+#pragma unroll
+        for (int b = 1; b < D; ++b) t = t + R[a][b] * Fr[b][j];
+        P.F[(a * 3 + j) * np + p] = t;
+      }
+  }
   // compute_stress (particle.tcc:880-890 -> material)
 #pragma unroll
   for (int k = 0; k < 6; ++k) s[k] = P.stress[k * np + p];
@@ -144,22 +177,6 @@ MPMX_DEV void stress_strain(const Mesh& M, const Particles& P, const Material* m
   }
 #pragma unroll
   for (int k = 0; k < 6; ++k) P.stress[k * np + p] = s[k];
-  // update_deformation_gradient (particle.tcc:1654-1659): rows 0..D-1 of
-  // F <- ΔF F; row 2 of a 2D ΔF is (0,0,1) and leaves F's row 2 unchanged.
-  double Fr[D][3];
-#pragma unroll
-  for (int a = 0; a < D; ++a)
-#pragma unroll
-    for (int j = 0; j < 3; ++j) Fr[a][j] = P.F[(a * 3 + j) * np + p];
-#pragma unroll
-  for (int a = 0; a < D; ++a)
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-      double t = R[a][0] * Fr[0][j];
-#pragma unroll
-      for (int b = 1; b < D; ++b) t = t + R[a][b] * Fr[b][j];
-      P.F[(a * 3 + j) * np + p] = t;
-    }
 }
 
 // Particle-level dN/dx at its xi.
@@ -456,7 +473,7 @@ __global__ __launch_bounds__(kBlock) void k_p2g(Mesh M, Particles P, Nodes Nd, S
 
 // USF stress update in sorted order, then the force scatter of the new stress.
 template <int D>
-__global__ __launch_bounds__(kBlock) void k_stress_scatter(Mesh M, Particles P, Nodes Nd,
+__global__ __launch_bounds__(kBlock) MPMX_MC_ATTR void k_stress_scatter(Mesh M, Particles P, Nodes Nd,
                                                            const Material* mats, StepArgs A,
                                                            const Status* S, const int* skey,
                                                            const int* perm) {
@@ -502,7 +519,7 @@ __global__ __launch_bounds__(kBlock) void k_stress_scatter(Mesh M, Particles P, 
 // Stress update in particle order (USL/MUSL postcompute, and USF in ordered
 // mode); optionally locate afterwards.
 template <int D, bool LOCATE>
-__global__ __launch_bounds__(kBlock) void k_stress(Mesh M, Particles P, Nodes Nd,
+__global__ __launch_bounds__(kBlock) MPMX_MC_ATTR void k_stress(Mesh M, Particles P, Nodes Nd,
                                                    const Material* mats, StepArgs A, Status* S,
                                                    int* scan_list) {
   constexpr int NN = Elem<D>::NN;
@@ -530,7 +547,7 @@ __global__ __launch_bounds__(kBlock) void k_stress(Mesh M, Particles P, Nodes Nd
 // G2P: compute_updated_position (particle.tcc:994-1070) in particle order,
 // optionally followed by the USL stress update and by locate.
 template <int D, bool STRESS, bool LOCATE>
-__global__ __launch_bounds__(kBlock) void k_g2p(Mesh M, Particles P, Nodes Nd,
+__global__ __launch_bounds__(kBlock) MPMX_MC_ATTR void k_g2p(Mesh M, Particles P, Nodes Nd,
                                                 const Material* mats, StepArgs A, int vu,
                                                 Status* S, int* scan_list) {
   constexpr int NN = Elem<D>::NN;
