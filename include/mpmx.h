@@ -152,6 +152,13 @@ int mpmx_get_nodes(mpmx_ctx* ctx, double* mass, double* velocity, double* accele
  * mpmx_last_error() then names it and the step. */
 int mpmx_step(mpmx_ctx* ctx, int nsteps, double dt, double damping_factor);
 
+/* Fast mode: a full radix sort of the particles by cell every `steps` steps
+ * (default 8; 1 = every step). In between, the last sort's order is reused
+ * with the current cells; particles that changed cell add scatter atomics
+ * but not error. Ordered mode sorts every step regardless. No counterpart in
+ * the reference, whose scatter works in particle order (mpm_scheme.tcc:44-71). */
+int mpmx_set_sort_interval(mpmx_ctx* ctx, int steps);
+
 /* Step counter (drives the gravity ramp); set on resume (mpm_base.tcc:509-551). */
 int mpmx_set_step(mpmx_ctx* ctx, int64_t step);
 int mpmx_get_step(const mpmx_ctx* ctx, int64_t* step);

@@ -27,8 +27,8 @@ EXPORTS = (
     "mpmx_set_friction_constraints", "mpmx_set_gravity", "mpmx_generate_gauss",
     "mpmx_set_particles", "mpmx_assign_velocity",
     "mpmx_assign_stresses", "mpmx_nparticles", "mpmx_get_particles", "mpmx_get_nodes",
-    "mpmx_step", "mpmx_set_step", "mpmx_get_step", "mpmx_synchronize", "mpmx_profile",
-    "mpmx_kernel_times",
+    "mpmx_step", "mpmx_set_sort_interval", "mpmx_set_step", "mpmx_get_step",
+    "mpmx_synchronize", "mpmx_profile", "mpmx_kernel_times",
 )
 
 _D = C.POINTER(C.c_double)
@@ -75,6 +75,7 @@ def load(path=LIB_PATH):
         "mpmx_get_nodes": [V, _D, _D, _D, _I32],
         "mpmx_step": [V, C.c_int, C.c_double, C.c_double],
         "mpmx_set_step": [V, C.c_int64],
+        "mpmx_set_sort_interval": [V, C.c_int],
         "mpmx_get_step": [V, _I64],
         "mpmx_synchronize": [V],
         "mpmx_profile": [V, C.c_int],
@@ -243,6 +244,10 @@ class Solver:
         s = C.c_int64(0)
         self._check(self.lib.mpmx_get_step(self.ctx, C.byref(s)))
         return s.value
+
+    def set_sort_interval(self, steps):
+        """Fast mode: full particle sort by cell every `steps` steps."""
+        self._check(self.lib.mpmx_set_sort_interval(self.ctx, int(steps)))
 
     def set_step(self, step):
         self._check(self.lib.mpmx_set_step(self.ctx, int(step)))

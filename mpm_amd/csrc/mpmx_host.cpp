@@ -119,10 +119,16 @@ struct mpmx_ctx {
   double ramp = 0;
   int64_t step = 0;
 
-  // graph cache
-  hipGraphExec_t graph_exec = nullptr;
+  // particle order by cell: a full radix sort every `sort_every` steps (and
+  // whenever the particle set changed); in between, fast mode reuses the last
+  // permutation with refreshed keys
+  int sort_every = 8;
+  bool need_sort = true;
+  int64_t since_sort = 0;
+
+  // graph cache: [0] step with key refresh, [1] step with a full sort
+  hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
   double graph_dt = 0, graph_damping = 0;
-  bool graph_valid = false;
 
   // profiling
   bool profile = false;
@@ -130,15 +136,17 @@ struct mpmx_ctx {
   std::map<std::string, KernelTime> kt;
 
   ~mpmx_ctx() {
-    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    for (auto& g : graph_exec)
+      if (g) (void)hipGraphExecDestroy(g);
     if (h_status) (void)hipHostFree(h_status);
     if (stream) (void)hipStreamDestroy(stream);
   }
 
   void invalidate_graph() {
-    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
-    graph_exec = nullptr;
-    graph_valid = false;
+    for (auto& g : graph_exec) {
+      if (g) (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
   }
 
   Mesh mesh() const {
@@ -471,6 +479,7 @@ void alloc_step_buffers(mpmx_ctx* c) {
   c->d_perm.alloc(std::max<int64_t>(np, 1));
   c->d_sort_tmp.alloc(radix_sort_temp_ints(std::max<int64_t>(np, 1)));
   c->d_scan_list.alloc(std::max<int64_t>(np, 1));
+  c->need_sort = true;
   if (c->flags & MPMX_ORDERED) {
     c->d_cstart.alloc(std::max<int64_t>(c->ncells, 1));
     c->d_cend.alloc(std::max<int64_t>(c->ncells, 1));
@@ -647,7 +656,7 @@ struct Launcher {
 };
 
 template <int D>
-void enqueue_step(mpmx_ctx* c, const StepArgs& A, Launcher& L) {
+void enqueue_step(mpmx_ctx* c, const StepArgs& A, Launcher& L, bool full_sort) {
   const Mesh M = c->mesh();
   const Particles P = c->parts();
   const Nodes Nd = c->nodes();
@@ -658,9 +667,12 @@ void enqueue_step(mpmx_ctx* c, const StepArgs& A, Launcher& L) {
   int* perm = c->d_perm.p;
   int* scan = c->d_scan_list.p;
   const bool ordered = (c->flags & MPMX_ORDERED) != 0;
-  L("sort_by_cell", [&] {
-    radix_sort_by_cell(P.cell, P.np, (int)M.ncells, skey, perm, c->d_sort_tmp.p, st);
-  });
+  if (full_sort || ordered)
+    L("sort_by_cell", [&] {
+      radix_sort_by_cell(P.cell, P.np, (int)M.ncells, skey, perm, c->d_sort_tmp.p, st);
+    });
+  else
+    L("refresh_keys", [&] { refresh_sorted_keys(P.cell, perm, P.np, (int)M.ncells, skey, st); });
   if (ordered) {
     L("cell_ranges", [&] {
       launch_cell_ranges<D>(P.np, M.ncells, skey, c->d_cstart.p, c->d_cend.p, st);
@@ -704,11 +716,20 @@ void enqueue_step(mpmx_ctx* c, const StepArgs& A, Launcher& L) {
   L("step_end", [&] { launch_step_end(S, st); });
 }
 
-void enqueue(mpmx_ctx* c, const StepArgs& A, Launcher& L) {
+void enqueue(mpmx_ctx* c, const StepArgs& A, Launcher& L, bool full_sort) {
   if (c->dim == 2)
-    enqueue_step<2>(c, A, L);
+    enqueue_step<2>(c, A, L, full_sort);
   else
-    enqueue_step<3>(c, A, L);
+    enqueue_step<3>(c, A, L, full_sort);
+}
+
+// Whether the next step sorts fully (ordered mode needs exact cell order
+// every step); advances the counter.
+bool next_step_sorts(mpmx_ctx* c) {
+  const bool full = (c->flags & MPMX_ORDERED) || c->need_sort || c->since_sort >= c->sort_every;
+  c->need_sort = false;
+  c->since_sort = full ? 1 : c->since_sort + 1;
+  return full;
 }
 
 }  // namespace
@@ -1150,23 +1171,27 @@ int mpmx_step(mpmx_ctx* c, int nsteps, double dt, double damping_factor) {
   Launcher L{c, {}};
   const bool use_graph = !(c->flags & MPMX_NO_GRAPH) && !c->profile;
   if (use_graph) {
-    if (!c->graph_valid || c->graph_dt != dt || c->graph_damping != damping_factor) {
+    if (c->graph_dt != dt || c->graph_damping != damping_factor) {
       c->invalidate_graph();
-      hipGraph_t graph;
-      HIPCHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-      enqueue(c, A, L);
-      HIPCHECK(hipStreamEndCapture(c->stream, &graph));
-      hipError_t e = hipGraphInstantiate(&c->graph_exec, graph, nullptr, nullptr, 0);
-      (void)hipGraphDestroy(graph);
-      HIPCHECK(e);
-      c->graph_valid = true;
       c->graph_dt = dt;
       c->graph_damping = damping_factor;
     }
-    for (int s = 0; s < nsteps; ++s) HIPCHECK(hipGraphLaunch(c->graph_exec, c->stream));
+    for (int s = 0; s < nsteps; ++s) {
+      const int kind = next_step_sorts(c) ? 1 : 0;
+      if (!c->graph_exec[kind]) {
+        hipGraph_t graph;
+        HIPCHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        enqueue(c, A, L, kind == 1);
+        HIPCHECK(hipStreamEndCapture(c->stream, &graph));
+        hipError_t e = hipGraphInstantiate(&c->graph_exec[kind], graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        HIPCHECK(e);
+      }
+      HIPCHECK(hipGraphLaunch(c->graph_exec[kind], c->stream));
+    }
   } else {
     for (int s = 0; s < nsteps; ++s) {
-      enqueue(c, A, L);
+      enqueue(c, A, L, next_step_sorts(c));
       HIPCHECK(hipGetLastError());
       L.collect();
     }
@@ -1177,6 +1202,13 @@ int mpmx_step(mpmx_ctx* c, int nsteps, double dt, double damping_factor) {
     throw ApiError(MPMX_E_UNLOCATABLE,
                    "Particle(s) outside the mesh domain: " + std::to_string(c->h_status->bad_particle) +
                        " (step " + std::to_string(c->h_status->first_bad_step) + ")");
+  API_END(c)
+}
+
+int mpmx_set_sort_interval(mpmx_ctx* c, int steps) {
+  API_BEGIN(c)
+  if (steps < 1) throw ApiError(MPMX_E_ARG, "sort interval must be >= 1");
+  c->sort_every = steps;
   API_END(c)
 }
 

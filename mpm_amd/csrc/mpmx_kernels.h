@@ -56,5 +56,9 @@ void launch_gather(int which, const Mesh& M, const Particles& P, const Nodes& Nd
 int64_t radix_sort_temp_ints(int64_t n);
 void radix_sort_by_cell(const int* cell, int64_t n, int max_key, int* keys_out, int* vals_out,
                         int* tmp, hipStream_t st);
+// keys[i] = cell[perm[i]] for an earlier sort's permutation (cells changed
+// since then leave the keys only nearly sorted).
+void refresh_sorted_keys(const int* cell, const int* perm, int64_t n, int max_key, int* keys,
+                         hipStream_t st);
 
 }  // namespace mpmx

@@ -10,11 +10,13 @@
 // ranges the return mapping uses (|x| < 10 for sin/cos, [-1, 1] for acos).
 //
 // Methods: Cody & Waite argument reduction by pi/2 with a three-part split;
-// Maclaurin series of sin/cos on [-pi/4, pi/4] (Horner, exact rational
-// coefficients rounded to double); acos through the half-angle identities
-// acos(v) = 2 asin(sqrt((1-v)/2)) for v > 1/2 and pi - 2 asin(sqrt((1+v)/2))
-// for v < -1/2 (as used by fdlibm, Sun Microsystems 1993) with the Maclaurin
-// series of asin on [-1/2, 1/2]. Constants: gen_mathconst.py.
+// on [-pi/4, pi/4], sin r = r + r z S(z) and cos r = 1 - z/2 + z^2 C(z)
+// (z = r^2); acos through the half-angle identities acos(v) =
+// 2 asin(sqrt((1-v)/2)) for v > 1/2 and pi - 2 asin(sqrt((1+v)/2)) for
+// v < -1/2 (as used by fdlibm, Sun Microsystems 1993), with asin y =
+// y + y z A(z) on [-1/2, 1/2]. S, C and A are Chebyshev interpolants
+// (degrees 5, 5, 12), evaluated by Estrin's scheme (short dependency chains
+// for the GPU's in-order lanes). Constants: gen_mathconst.py.
 #pragma once
 
 #include <cmath>
@@ -36,32 +38,21 @@ static constexpr double kPio2Hi = 1.5707963267948966;
 static constexpr double kPio2Lo = 6.123233995736766e-17;
 static constexpr double kPiHi = 3.141592653589793;
 static constexpr double kPiLo = 1.2246467991473532e-16;
+static constexpr double kSin[6] = {-0.16666666666666666, 0.008333333333330948, -0.00019841269836758574, 2.755731610255244e-06, -2.5051131845003624e-08, 1.5918129294866608e-10};
+static constexpr double kCos[6] = {0.041666666666666664, -0.0013888888888887398, 2.480158729876569e-05, -2.7557317271729793e-07, 2.08761462684032e-09, -1.1382632425521717e-11};
+static constexpr double kAsin[13] = {0.16666666666666669, 0.07499999999998433, 0.04464285714635543, 0.030381944138531247, 0.02237217294214989, 0.017352392720869973, 0.013971212973552933, 0.011479177415184906, 0.01032281435018578, 0.005457506718640358, 0.01740087944269402, -0.014851887071247204, 0.028757851367421566};
 
-// sin r = r + r^3 * sum_n kSin[n] r^(2n), kSin[n] = (-1)^(n+1)/(2n+3)!
-MPMX_MATH_FN double sin_poly(double r, double z) {
-  double p = 2.8114572543455206e-15;
-  p = -7.647163731819816e-13 + z * p;
-  p = 1.6059043836821613e-10 + z * p;
-  p = -2.505210838544172e-08 + z * p;
-  p = 2.7557319223985893e-06 + z * p;
-  p = -0.0001984126984126984 + z * p;
-  p = 0.008333333333333333 + z * p;
-  p = -0.16666666666666666 + z * p;
-  return r + (r * z) * p;
+// c[0] + c[1] z + ... + c[5] z^5 by Estrin's scheme
+MPMX_MATH_FN double estrin6(const double* c, double z) {
+  const double z2 = z * z, z4 = z2 * z2;
+  return ((c[0] + c[1] * z) + (c[2] + c[3] * z) * z2) + (c[4] + c[5] * z) * z4;
 }
 
-// cos r = 1 - z/2 + z^2 * sum_n kCos[n] z^n
-MPMX_MATH_FN double cos_poly(double z) {
-  double p = -1.5619206968586225e-16;
-  p = 4.779477332387385e-14 + z * p;
-  p = -1.1470745597729725e-11 + z * p;
-  p = 2.08767569878681e-09 + z * p;
-  p = -2.755731922398589e-07 + z * p;
-  p = 2.48015873015873e-05 + z * p;
-  p = -0.001388888888888889 + z * p;
-  p = 0.041666666666666664 + z * p;
-  return 1.0 - (0.5 * z - (z * z) * p);
-}
+// sin r = r + r z S(z), z = r^2
+MPMX_MATH_FN double sin_poly(double r, double z) { return r + (r * z) * estrin6(kSin, z); }
+
+// cos r = 1 - z/2 + z^2 C(z), z = r^2
+MPMX_MATH_FN double cos_poly(double z) { return 1.0 - (0.5 * z - (z * z) * estrin6(kCos, z)); }
 
 // sin and cos of x (|x| < 2^19).
 MPMX_MATH_FN void sincos(double x, double* s, double* c) {
@@ -95,44 +86,29 @@ MPMX_MATH_FN double tan(double x) {
   return s / c;
 }
 
-// asin y for |y| <= 1/2: y + y^3 * sum_n kAsin[n] y^(2n)
+// asin y = y + y z A(z) for |y| <= 1/2, z = y^2 (A by Estrin's scheme)
 MPMX_MATH_FN double asin_small(double y) {
-  const double z = y * y;
-  double p = 0.0019650336162772837;
-  p = 0.0020776610325181676 + z * p;
-  p = 0.0022014739737101384 + z * p;
-  p = 0.002338091892111975 + z * p;
-  p = 0.0024894486782468836 + z * p;
-  p = 0.00265787063820729 + z * p;
-  p = 0.002846178401108942 + z * p;
-  p = 0.0030578216492580306 + z * p;
-  p = 0.003297059503473485 + z * p;
-  p = 0.0035692053938259347 + z * p;
-  p = 0.003880964558837669 + z * p;
-  p = 0.004240907093679363 + z * p;
-  p = 0.004660143486915096 + z * p;
-  p = 0.005153309682319905 + z * p;
-  p = 0.005740037670841924 + z * p;
-  p = 0.006447210311889649 + z * p;
-  p = 0.0073125258735988454 + z * p;
-  p = 0.008390335809616815 + z * p;
-  p = 0.009761609529194078 + z * p;
-  p = 0.011551800896139705 + z * p;
-  p = 0.01396484375 + z * p;
-  p = 0.017352764423076924 + z * p;
-  p = 0.022372159090909092 + z * p;
-  p = 0.030381944444444444 + z * p;
-  p = 0.044642857142857144 + z * p;
-  p = 0.075 + z * p;
-  p = 0.16666666666666666 + z * p;
-  return y + (y * z) * p;
+  const double* a = kAsin;
+  const double z = y * y, z2 = z * z, z4 = z2 * z2, z8 = z4 * z4;
+  const double q0 = (a[0] + a[1] * z) + (a[2] + a[3] * z) * z2;
+  const double q1 = (a[4] + a[5] * z) + (a[6] + a[7] * z) * z2;
+  const double q2 = (a[8] + a[9] * z) + (a[10] + a[11] * z) * z2;
+  const double P = (q0 + q1 * z4) + (q2 + a[12] * z4) * z8;
+  return y + (y * z) * P;
 }
 
-// acos v for v in [-1, 1] (NaN outside)
+// acos v for v in [-1, 1] (NaN outside):
+//   v >  1/2: 2 asin(sqrt((1-v)/2)),
+//   v < -1/2: pi - 2 asin(sqrt((1+v)/2)),
+//   else    : pi/2 - asin(v),
+// with one asin evaluation (branch-free for the GPU's lanes).
 MPMX_MATH_FN double acos(double v) {
-  if (v > 0.5) return 2.0 * asin_small(sqrt((1.0 - v) * 0.5));
-  if (v < -0.5) return kPiHi - (2.0 * asin_small(sqrt((1.0 + v) * 0.5)) - kPiLo);
-  return kPio2Hi - (asin_small(v) - kPio2Lo);
+  const bool hi = v > 0.5, lo = v < -0.5;
+  const double y = hi ? sqrt((1.0 - v) * 0.5) : (lo ? sqrt((1.0 + v) * 0.5) : v);
+  const double a = asin_small(y);
+  if (hi) return 2.0 * a;
+  if (lo) return kPiHi - (2.0 * a - kPiLo);
+  return kPio2Hi - (a - kPio2Lo);
 }
 
 // x^(n/2) for the odd half-integer exponents the Mohr-Coulomb update uses

@@ -177,6 +177,24 @@ __global__ __launch_bounds__(kSortBlock) void k_rs_scatter(const int* kin, const
   }
 }
 
+// Keys of an earlier sort's permutation with the current cells: keys[i] =
+// cell[perm[i]] (max_key for an invalid cell). Particles that changed cell
+// since that sort sit out of key order and form short runs of their own.
+__global__ __launch_bounds__(kSortBlock) void k_rs_refresh(const int* cell, const int* perm,
+                                                           int64_t n, int max_key, int* keys) {
+  const int64_t i = blockIdx.x * (int64_t)kSortBlock + threadIdx.x;
+  if (i >= n) return;
+  const int c = cell[perm[i]];
+  keys[i] = (c >= 0 && c < max_key) ? c : max_key;
+}
+
+void refresh_sorted_keys(const int* cell, const int* perm, int64_t n, int max_key, int* keys,
+                         hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_rs_refresh, dim3((unsigned)((n + kSortBlock - 1) / kSortBlock)),
+                     dim3(kSortBlock), 0, st, cell, perm, n, max_key, keys);
+}
+
 void radix_sort_by_cell(const int* cell, int64_t n, int max_key, int* keys_out, int* vals_out,
                         int* tmp, hipStream_t st) {
   if (n <= 0) return;
