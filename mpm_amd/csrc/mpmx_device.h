@@ -446,10 +446,36 @@ MPMX_DEV void mc_df_dp(const Material& m, int yt, double tension_cutoff, double 
   }
 }
 
-// compute_stress (mohr_coulomb.tcc:348-513). Updates `stress` in place; sv
-// holds the 9 state variables; density is the particle's mass density.
-MPMX_DEV void mc_stress(const Material& m, double* stress, const double* dstrain, double density,
-                        double* sv) {
+// compute_stress (mohr_coulomb.tcc:348-513) in three phases, so that a
+// caller can run them in separate kernels over compacted particle lists:
+//   mc_begin   : softening, elastic predictor, separated state, invariants and
+//                yield state of the trial stress (complete if elastic);
+//   mc_first   : the first plastic correction;
+//   mc_iterate : the correction loop, resumable.
+// mc_stress composes them. sv holds the 9 state variables; density is the
+// particle's mass density.
+
+// What the correction needs from the trial state.
+struct McTrial {
+  McInv I;             // invariants of the trial stress
+  double yield_trial;  // its yield function value (tensile or shear)
+  int yt;              // its yield type (FS_TENSILE or FS_SHEAR)
+};
+
+// Loop state of the plastic correction.
+struct McIter {
+  double upd[6];
+  double dpdstrain;
+  double yield_trial;
+  int itr;  // corrections done by the reference's loop (0..100)
+};
+
+// Phase 1 (mohr_coulomb.tcc:356-411). Returns true when the update is
+// complete (elastic or separated; stress and sv final). Otherwise `trial` and
+// T describe the plastic trial state, and sv holds the updated strength
+// parameters and yield_state.
+MPMX_DEV bool mc_begin(const Material& m, double* stress, const double* dstrain, double density,
+                       double* sv, double* trial, McTrial& T) {
   const double pdstrain = sv[SV_PDS];
   if (m.softening && pdstrain > m.pd_peak) {
     if (pdstrain < m.pd_res) {
@@ -464,9 +490,7 @@ MPMX_DEV void mc_stress(const Material& m, double* stress, const double* dstrain
     const double apex = sv[SV_COH] / dmath::tan(sv[SV_PHI]);
     if (sv[SV_TCUT] > apex) sv[SV_TCUT] = (apex > 1.0e-15 ? apex : 1.0e-15);
   }
-  const McAngles A = mc_angles(sv[SV_PHI], sv[SV_PSI]);
   sv[SV_YIELD] = 0;
-  double trial[6];
   {
     double ds[6];
     de_mul(m, dstrain, ds);
@@ -476,95 +500,122 @@ MPMX_DEV void mc_stress(const Material& m, double* stress, const double* dstrain
     sv[SV_PDS] += mu_q(trial) / 3.0 / m.shear;
     sv[SV_YIELD] = 3;
     for (int i = 0; i < 6; ++i) stress[i] = 0.;
-    return;
+    return true;
   }
-  const double Tolerance = 1E-1;
-  const double cohesion = sv[SV_COH], tcut = sv[SV_TCUT];
-  McInv I;
+  const McAngles A = mc_angles(sv[SV_PHI], sv[SV_PSI]);
+  mc_invariants(trial, T.I);
   double yf[2];
-  int yt = FS_ELASTIC;
-  {
-    mc_invariants(trial, I);
-    yt = mc_yield(yf, I, A, cohesion, tcut);
-  }
+  const int yt = mc_yield(yf, T.I, A, sv[SV_COH], sv[SV_TCUT]);
   if (yt == FS_ELASTIC) {
-    sv[SV_YIELD] = 0;
-    sv[SV_EPS] = I.eps;
-    sv[SV_THETA] = I.theta;
-    sv[SV_RHO] = I.rho;
+    sv[SV_EPS] = T.I.eps;
+    sv[SV_THETA] = T.I.theta;
+    sv[SV_RHO] = T.I.rho;
     for (int i = 0; i < 6; ++i) stress[i] = trial[i];
-    return;
+    return true;
   }
   sv[SV_YIELD] = (yt == FS_TENSILE) ? 2 : 1;
-  double yield_trial = (yt == FS_TENSILE) ? yf[0] : yf[1];
-  // The reference evaluates df/dsigma, dp/dsigma at both the trial and the
-  // current stress and corrects with one pair: the current one if
-  // |yield(current)| < 0.1, else the trial one. Only the used pair is
-  // evaluated (compute_df_dp is pure, so the bits are the reference's). x is
-  // the stress at which the next correction evaluates df/dp.
+  T.yt = yt;
+  T.yield_trial = (yt == FS_TENSILE) ? yf[0] : yf[1];
+  return false;
+}
+
+// upd += -(lam * D * dp): each entry sums (lam*D_ik)*dp_k over k in order
+MPMX_DEV void mc_correct(const Material& m, double lam, const double* dp, double* upd) {
+  const double la1 = lam * m.a1, la2 = lam * m.a2, lg = lam * m.G;
+  upd[0] += -(((la1 * dp[0] + la2 * dp[1]) + la2 * dp[2]));
+  upd[1] += -(((la2 * dp[0] + la1 * dp[1]) + la2 * dp[2]));
+  upd[2] += -(((la2 * dp[0] + la2 * dp[1]) + la1 * dp[2]));
+  upd[3] += -(lg * dp[3]);
+  upd[4] += -(lg * dp[4]);
+  upd[5] += -(lg * dp[5]);
+}
+
+// Phase 2 (mohr_coulomb.tcc:412-470). The reference evaluates df/dsigma and
+// dp/dsigma at both the trial and the current stress and corrects with one
+// pair: the current one if |yield(current)| < 0.1, else the trial one. Only
+// the used pair is evaluated (compute_df_dp is pure, so the bits are the
+// reference's). `stress` is the start-of-step stress.
+MPMX_DEV void mc_first(const Material& m, const double* stress, const double* trial,
+                       const double* dstrain, const double* sv, const McTrial& T, McIter& R) {
+  const double Tolerance = 1E-1;
+  const McAngles A = mc_angles(sv[SV_PHI], sv[SV_PSI]);
+  McInv Ic;
+  mc_invariants(stress, Ic);
+  double yc[2];
+  const int ytc = mc_yield(yc, Ic, A, sv[SV_COH], sv[SV_TCUT]);
+  double yield = 1.7976931348623157e308;
+  if (ytc == FS_TENSILE) yield = yc[0];
+  if (ytc == FS_SHEAR) yield = yc[1];
+  const bool use_current = fabs(yield) < Tolerance;
   double x[6];
-  bool use_current = false;
-  {
-    McInv Ic;
-    mc_invariants(stress, Ic);
-    double yc[2];
-    const int ytc = mc_yield(yc, Ic, A, cohesion, tcut);
-    double yield = 1.7976931348623157e308;
-    if (ytc == FS_TENSILE) yield = yc[0];
-    if (ytc == FS_SHEAR) yield = yc[1];
-    use_current = fabs(yield) < Tolerance;
-    if (use_current) {
-      I = Ic;
-      yt = ytc;
-    }
-    for (int i = 0; i < 6; ++i) x[i] = use_current ? stress[i] : trial[i];
-  }
-  double upd[6];
-  for (int i = 0; i < 6; ++i) upd[i] = trial[i];
-  double dpdstrain = 0.;
+  for (int i = 0; i < 6; ++i) x[i] = use_current ? stress[i] : trial[i];
+  double df[6], dp[6], tmp[6], dp_dq = 0., softening = 0.;
+  mc_df_dp(m, use_current ? ytc : T.yt, sv[SV_TCUT], sv[SV_PDS], x, use_current ? Ic : T.I, A,
+           df, dp, &dp_dq, &softening);
+  de_tmul(m, df, tmp);
+  // current: lambda = (df^T D).dstrain / ((df^T D).dp + softening)
+  // trial:   lambda_trial = yield_trial / ((df^T D).dp + softening)
+  const double lam =
+      (use_current ? dot6(tmp, dstrain) : T.yield_trial) / (dot6(tmp, dp) + softening);
+  for (int i = 0; i < 6; ++i) R.upd[i] = trial[i];
+  mc_correct(m, lam, dp, R.upd);
+  R.dpdstrain = lam * dp_dq;
+  R.yield_trial = T.yield_trial;
+  R.itr = 0;
+}
+
+// Phase 3 (mohr_coulomb.tcc:472-512): the correction loop from R.itr, at most
+// max_corr corrections in this call. Returns true when the update is complete
+// (converged, or 100 corrections; stress and sv final), false with R advanced
+// to a yield check that a later call repeats (the check is pure).
+MPMX_DEV bool mc_iterate(const Material& m, double* stress, double* sv, McIter& R, int max_corr) {
+  const double Tolerance = 1E-1;
+  const McAngles A = mc_angles(sv[SV_PHI], sv[SV_PSI]);
+  const double cohesion = sv[SV_COH], tcut = sv[SV_TCUT], pdstrain = sv[SV_PDS];
+  McInv I;
   bool converged = false;
-  // it = 0: the first correction; it = 1..100: iterations 0..99 of the
-  // reference's correction loop, each after the yield check of `upd`.
+  int n = 0;
 #pragma nounroll
-  for (int it = 0;; ++it) {
-    double df[6], dp[6], tmp[6], dp_dq = 0., softening = 0.;
-    mc_df_dp(m, yt, tcut, pdstrain, x, I, A, df, dp, &dp_dq, &softening);
-    de_tmul(m, df, tmp);
-    // first, current: lambda = (df^T D).dstrain / ((df^T D).dp + softening)
-    // otherwise: lambda_trial = yield_trial / ((df^T D).dp + softening)
-    const double lam = (use_current ? dot6(tmp, dstrain) : yield_trial) / (dot6(tmp, dp) + softening);
-    // upd += -(lam * D * dp): each entry sums (lam*D_ik)*dp_k over k in order
-    const double la1 = lam * m.a1, la2 = lam * m.a2, lg = lam * m.G;
-    upd[0] += -(((la1 * dp[0] + la2 * dp[1]) + la2 * dp[2]));
-    upd[1] += -(((la2 * dp[0] + la1 * dp[1]) + la2 * dp[2]));
-    upd[2] += -(((la2 * dp[0] + la2 * dp[1]) + la1 * dp[2]));
-    upd[3] += -(lg * dp[3]);
-    upd[4] += -(lg * dp[4]);
-    upd[5] += -(lg * dp[5]);
-    if (it == 0)
-      dpdstrain = lam * dp_dq;
-    else
-      dpdstrain += lam * dp_dq;
-    use_current = false;
-    if (it == 100) break;
-    mc_invariants(upd, I);
-    yt = mc_yield(yf, I, A, cohesion, tcut);
+  while (R.itr < 100) {
+    mc_invariants(R.upd, I);
+    double yf[2];
+    const int yt = mc_yield(yf, I, A, cohesion, tcut);
     if (yf[0] < Tolerance && yf[1] < Tolerance) {
       converged = true;
       break;
     }
-    if (yt == FS_TENSILE) yield_trial = yf[0];
-    if (yt == FS_SHEAR) yield_trial = yf[1];
-    for (int i = 0; i < 6; ++i) x[i] = upd[i];
+    if (n == max_corr) return false;
+    if (yt == FS_TENSILE) R.yield_trial = yf[0];
+    if (yt == FS_SHEAR) R.yield_trial = yf[1];
+    double df[6], dp[6], tmp[6], dp_dq = 0., softening = 0.;
+    mc_df_dp(m, yt, tcut, pdstrain, R.upd, I, A, df, dp, &dp_dq, &softening);
+    de_tmul(m, df, tmp);
+    const double lambda_trial = R.yield_trial / (dot6(tmp, dp) + softening);
+    mc_correct(m, lambda_trial, dp, R.upd);
+    R.dpdstrain += lambda_trial * dp_dq;
+    ++R.itr;
+    ++n;
   }
   // final invariants (mohr_coulomb.tcc:507): after a converged loop they are
   // those just computed from the unchanged `upd`
-  if (!converged) mc_invariants_only(upd, I);
+  if (!converged) mc_invariants_only(R.upd, I);
   sv[SV_EPS] = I.eps;
   sv[SV_THETA] = I.theta;
   sv[SV_RHO] = I.rho;
-  sv[SV_PDS] += dpdstrain;
-  for (int i = 0; i < 6; ++i) stress[i] = upd[i];
+  sv[SV_PDS] += R.dpdstrain;
+  for (int i = 0; i < 6; ++i) stress[i] = R.upd[i];
+  return true;
+}
+
+// The whole update in one call. Updates `stress` in place.
+MPMX_DEV void mc_stress(const Material& m, double* stress, const double* dstrain, double density,
+                        double* sv) {
+  double trial[6];
+  McTrial T;
+  if (mc_begin(m, stress, dstrain, density, sv, trial, T)) return;
+  McIter R;
+  mc_first(m, stress, trial, dstrain, sv, T, R);
+  mc_iterate(m, stress, sv, R, 100);
 }
 
 // ------------------------------ node update ---------------------------------

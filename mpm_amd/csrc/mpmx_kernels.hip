@@ -56,16 +56,29 @@ MPMX_DEV void gather_nodes(const Mesh& M, const Nodes& Nd, int c, double (*nv)[D
   }
 }
 
-// compute_stress_strain for one particle (mpm_scheme.tcc:75-103 fused per
-// particle: ΔF, strain, volume, constitutive update, F). dndx: particle's
-// dN/dx (computed at step start from its xi). Returns new stress in `s` and
-// writes the particle's persistent fields.
+// The kinematic half of compute_stress_strain for one particle
+// (mpm_scheme.tcc:75-103: ΔF, strain rate, strain, volume, density, F).
+// dndx: the particle's dN/dx at its xi. Writes the particle's strain, volume,
+// density and F; returns the strain increment ds and the new density.
 template <int D>
-MPMX_DEV void stress_strain(const Mesh& M, const Particles& P, const Material* mats, int p, int c,
-                            const double (*dndx)[D], const double (*nv)[D], double dt,
-                            double* s, double* vol_out) {
+MPMX_DEV void kinematics(const Mesh& M, const Particles& P, int p, int c, const double (*dndx)[D],
+                         const double (*nv)[D], double dt, double* ds, double* rho_out,
+                         double* vol_out) {
   constexpr int NN = Elem<D>::NN;
   const int64_t np = P.np;
+  // The particle fields updated here are all loaded before the first store:
+  // the compiler must assume that a store may alias any array, so a load
+  // placed after a store waits for the round trip of the loads before it.
+  double eps[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) eps[k] = (D == 2 && (k == 2 || k > 3)) ? 0. : P.strain[k * np + p];
+  double vol = P.vol[p];
+  const double rho0 = P.rho[p];
+  double Fr[D][3];
+#pragma unroll
+  for (int a = 0; a < D; ++a)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) Fr[a][j] = P.F[(a * 3 + j) * np + p];
   // update_deformation_gradient_increment (particle.tcc:1578-1651)
   double R[D][D];
 #pragma unroll
@@ -117,37 +130,30 @@ MPMX_DEV void stress_strain(const Mesh& M, const Particles& P, const Material* m
     if (fabs(sc[k]) < 1.E-15) sc[k] = 0.;
   }
   // compute_strain (particle.tcc:861-877)
-  double ds[6];
 #pragma unroll
   for (int k = 0; k < 6; ++k) ds[k] = sr[k] * dt;
   if constexpr (D == 2) {
-    P.strain[0 * np + p] += ds[0];
-    P.strain[1 * np + p] += ds[1];
-    P.strain[3 * np + p] += ds[3];
+    P.strain[0 * np + p] = eps[0] + ds[0];
+    P.strain[1 * np + p] = eps[1] + ds[1];
+    P.strain[3 * np + p] = eps[3] + ds[3];
   } else {
 #pragma unroll
-    for (int k = 0; k < 6; ++k) P.strain[k * np + p] += ds[k];
+    for (int k = 0; k < 6; ++k) P.strain[k * np + p] = eps[k] + ds[k];
   }
   double svol = sc[0];
 #pragma unroll
   for (int k = 1; k < D; ++k) svol = svol + sc[k];
   const double dvol = dt * svol;
   // update_volume (particle.tcc:598-606)
-  double vol = P.vol[p];
   vol *= (1. + dvol);
-  const double rho = P.rho[p] / (1. + dvol);
+  const double rho = rho0 / (1. + dvol);
   P.vol[p] = vol;
   P.rho[p] = rho;
   *vol_out = vol;
+  *rho_out = rho;
   // update_deformation_gradient (particle.tcc:1654-1659): rows 0..D-1 of
   // F <- ΔF F; row 2 of a 2D ΔF is (0,0,1) and leaves F's row 2 unchanged.
-  // Independent of the stress, so it is done first and ΔF dies here.
   {
-    double Fr[D][3];
-#pragma unroll
-    for (int a = 0; a < D; ++a)
-#pragma unroll
-      for (int j = 0; j < 3; ++j) Fr[a][j] = P.F[(a * 3 + j) * np + p];
 #pragma unroll
     for (int a = 0; a < D; ++a)
 #pragma unroll
@@ -158,19 +164,32 @@ MPMX_DEV void stress_strain(const Mesh& M, const Particles& P, const Material* m
         P.F[(a * 3 + j) * np + p] = t;
       }
   }
-  // compute_stress (particle.tcc:880-890 -> material)
+}
+
+// compute_stress_strain for one particle: kinematics, then the constitutive
+// update (particle.tcc:880-890 -> material). Returns the new stress in `s`.
+template <int D>
+MPMX_DEV void stress_strain(const Mesh& M, const Particles& P, const Material* mats, int p, int c,
+                            const double (*dndx)[D], const double (*nv)[D], double dt,
+                            double* s, double* vol_out) {
+  const int64_t np = P.np;
+  double ds[6], rho;
+  kinematics<D>(M, P, p, c, dndx, nv, dt, ds, &rho, vol_out);
+  // one grouped round trip after the kinematic stores
 #pragma unroll
   for (int k = 0; k < 6; ++k) s[k] = P.stress[k * np + p];
   const Material& m = mats[P.mat[p]];
+  double sv[NSV];
+  if (m.type != LE) {
+#pragma unroll
+    for (int k = 0; k < NSV; ++k) sv[k] = P.sv[k * np + p];
+  }
   if (m.type == LE) {
     double t[6];
     de_mul(m, ds, t);
 #pragma unroll
     for (int k = 0; k < 6; ++k) s[k] = s[k] + t[k];
   } else {
-    double sv[NSV];
-#pragma unroll
-    for (int k = 0; k < NSV; ++k) sv[k] = P.sv[k * np + p];
     mc_stress(m, s, ds, rho, sv);
 #pragma unroll
     for (int k = 0; k < NSV; ++k) P.sv[k * np + p] = sv[k];
@@ -573,10 +592,16 @@ __global__ __launch_bounds__(kBlock) MPMX_MC_ATTR void k_g2p(Mesh M, Particles P
       sv[k] += N[i] * nv[i][k];
       sa[k] += N[i] * na[i][k];
     }
-  double x[D];
+  double x[D], v0[D], u0[D];
+#pragma unroll
+  for (int k = 0; k < D; ++k) {  // loads before stores (see kinematics)
+    v0[k] = P.v[k * np + p];
+    x[k] = P.x[k * np + p];
+    u0[k] = P.u[k * np + p];
+  }
 #pragma unroll
   for (int k = 0; k < D; ++k) {
-    double v = P.v[k * np + p];
+    double v = v0[k];
     if (vu == FLIP) {
       v += sa[k] * A.dt;
       v = A.blending * v + (1.0 - A.blending) * sv[k];
@@ -584,9 +609,9 @@ __global__ __launch_bounds__(kBlock) MPMX_MC_ATTR void k_g2p(Mesh M, Particles P
       v = sv[k];
     }
     P.v[k * np + p] = v;
-    x[k] = P.x[k * np + p] + sv[k] * A.dt;
+    x[k] = x[k] + sv[k] * A.dt;
     P.x[k * np + p] = x[k];
-    P.u[k * np + p] += sv[k] * A.dt;
+    P.u[k * np + p] = u0[k] + sv[k] * A.dt;
   }
   if constexpr (STRESS) {
     double dndx[NN][D], Ji[D][D];
@@ -633,9 +658,17 @@ __global__ __launch_bounds__(kBlock) void k_g2p_scatter(Mesh M, Particles P, Nod
           sv[k] += N[i] * nv[i][k];
           sa[k] += N[i] * na[i][k];
         }
+      double v0[D], x0[D], u0[D];
+#pragma unroll
+      for (int k = 0; k < D; ++k) {  // loads before stores (see kinematics)
+        v0[k] = P.v[k * np + p];
+        x0[k] = P.x[k * np + p];
+        u0[k] = P.u[k * np + p];
+      }
+      st.m[tid] = P.mass[p];
 #pragma unroll
       for (int k = 0; k < D; ++k) {
-        double v = P.v[k * np + p];
+        double v = v0[k];
         if (vu == FLIP) {
           v += sa[k] * A.dt;
           v = A.blending * v + (1.0 - A.blending) * sv[k];
@@ -643,12 +676,11 @@ __global__ __launch_bounds__(kBlock) void k_g2p_scatter(Mesh M, Particles P, Nod
           v = sv[k];
         }
         P.v[k * np + p] = v;
-        P.x[k * np + p] += sv[k] * A.dt;
-        P.u[k * np + p] += sv[k] * A.dt;
+        P.x[k * np + p] = x0[k] + sv[k] * A.dt;
+        P.u[k * np + p] = u0[k] + sv[k] * A.dt;
         st.v[k][tid] = v;
         st.xi[k][tid] = xi[k];
       }
-      st.m[tid] = P.mass[p];
     }
   }
   __syncthreads();
