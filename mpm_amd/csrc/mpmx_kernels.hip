@@ -323,13 +323,41 @@ struct Stage {
   double xi[D][kBlock];
   double m[kBlock];
   double v[D][kBlock];
-  double s[(WHAT & SC_F) ? 6 : 1][kBlock];
-  double vol[(WHAT & SC_F) ? 1 : 1][kBlock];
-  double Ji[(WHAT & SC_F) ? D * D : 1][kBlock];
+  double B[(WHAT & SC_F) ? D * D : 1][kBlock];  // force matrix, see force_matrix()
   int seg_start[kBlock + 1];
   int nseg;
   int wave_heads[kBlock / 64];
 };
+
+// Internal force of a particle on node i (particle.tcc:914-950) is
+// f_i = -V (sigma . dN_i/dx) with dN_i/dx = J^-1 grad_i(xi), i.e.
+// f_i = B grad_i(xi) with B = -V sigma J^-1 (a DxD matrix per particle).
+// Staging B instead of sigma, V and J^-1 (9 doubles instead of 16) lets three
+// scatter workgroups share a CU's LDS; the reassociation changes only the
+// rounding of each contribution (fast mode; ordered mode keeps the
+// reference's order).
+template <int D>
+MPMX_DEV void force_matrix(const double* s, double vol, const double (*Ji)[D], double* B) {
+  double sig[D][D];
+  if constexpr (D == 2) {
+    sig[0][0] = s[0]; sig[0][1] = s[3];
+    sig[1][0] = s[3]; sig[1][1] = s[1];
+  } else {
+    sig[0][0] = s[0]; sig[0][1] = s[3]; sig[0][2] = s[5];
+    sig[1][0] = s[3]; sig[1][1] = s[1]; sig[1][2] = s[4];
+    sig[2][0] = s[5]; sig[2][1] = s[4]; sig[2][2] = s[2];
+  }
+  const double mv = -1. * vol;
+#pragma unroll
+  for (int a = 0; a < D; ++a)
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+      double t = sig[a][0] * Ji[0][c];
+#pragma unroll
+      for (int b = 1; b < D; ++b) t = t + sig[a][b] * Ji[b][c];
+      B[a * D + c] = t * mv;
+    }
+}
 
 // Phase 2 of every scatter kernel: given staged particles [0, cnt) of this
 // chunk, sorted by cell, add each cell segment's per-node sums to the nodes.
@@ -394,25 +422,15 @@ MPMX_DEV void scatter_segments(Stage<D, WHAT>& st, int cnt, const Mesh& M, const
       if constexpr ((WHAT & SC_F) != 0) {
 #pragma unroll
         for (int k = 0; k < D; ++k) afe[k] = afe[k] + g[k] * m * N;
-        double Ji[D][D];
+        double gr[D];
+        grad_i<D>(xi, i, gr);
 #pragma unroll
-        for (int a = 0; a < D; ++a)
+        for (int a = 0; a < D; ++a) {
+          double f = st.B[a * D][t] * gr[0];
 #pragma unroll
-          for (int bb = 0; bb < D; ++bb) Ji[a][bb] = st.Ji[a * D + bb][t];
-        double d[D];
-        dndx_i<D>(xi, i, Ji, d);
-        double f[D];
-        if constexpr (D == 2) {
-          f[0] = d[0] * st.s[0][t] + d[1] * st.s[3][t];
-          f[1] = d[1] * st.s[1][t] + d[0] * st.s[3][t];
-        } else {
-          f[0] = d[0] * st.s[0][t] + d[1] * st.s[3][t] + d[2] * st.s[5][t];
-          f[1] = d[1] * st.s[1][t] + d[0] * st.s[3][t] + d[2] * st.s[4][t];
-          f[2] = d[2] * st.s[2][t] + d[1] * st.s[4][t] + d[0] * st.s[5][t];
+          for (int c = 1; c < D; ++c) f = f + st.B[a * D + c][t] * gr[c];
+          afi[a] = afi[a] + f;
         }
-        const double mv = -1. * st.vol[0][t];
-#pragma unroll
-        for (int k = 0; k < D; ++k) afi[k] = afi[k] + f[k] * mv;
       }
     }
     const int n = M.cell_nodes[(int64_t)c * NN + i];
@@ -470,17 +488,17 @@ __global__ __launch_bounds__(kBlock) void k_p2g(Mesh M, Particles P, Nodes Nd, S
       }
       st.m[tid] = P.mass[p];
       if constexpr ((WHAT & SC_F) != 0) {
+        double sg[6];
 #pragma unroll
-        for (int k = 0; k < 6; ++k) st.s[k][tid] = P.stress[k * np + p];
-        st.vol[0][tid] = P.vol[p];
+        for (int k = 0; k < 6; ++k) sg[k] = P.stress[k * np + p];
+        const double vol = P.vol[p];
         constexpr int NN = Elem<D>::NN;
-        double X[NN][D], Ji[D][D];
+        double X[NN][D], Ji[D][D], Bm[D * D];
         load_cell_X<D>(M, key, X);
         jacobian_inverse<D>(xi, X, Ji);
+        force_matrix<D>(sg, vol, Ji, Bm);
 #pragma unroll
-        for (int a = 0; a < D; ++a)
-#pragma unroll
-          for (int b = 0; b < D; ++b) st.Ji[a * D + b][tid] = Ji[a][b];
+        for (int k = 0; k < D * D; ++k) st.B[k][tid] = Bm[k];
       }
     }
   }
@@ -520,13 +538,10 @@ __global__ __launch_bounds__(kBlock) MPMX_MC_ATTR void k_stress_scatter(Mesh M, 
       double s[6], vol;
       stress_strain<D>(M, P, mats, p, key, dndx, nv, A.dt, s, &vol);
       st.m[tid] = P.mass[p];
+      double Bm[D * D];
+      force_matrix<D>(s, vol, Ji, Bm);
 #pragma unroll
-      for (int k = 0; k < 6; ++k) st.s[k][tid] = s[k];
-      st.vol[0][tid] = vol;
-#pragma unroll
-      for (int a = 0; a < D; ++a)
-#pragma unroll
-        for (int b = 0; b < D; ++b) st.Ji[a * D + b][tid] = Ji[a][b];
+      for (int k = 0; k < D * D; ++k) st.B[k][tid] = Bm[k];
     }
   }
   double g[3] = {0, 0, 0};
