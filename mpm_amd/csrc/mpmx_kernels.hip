@@ -359,6 +359,50 @@ MPMX_DEV void force_matrix(const double* s, double vol, const double (*Ji)[D], d
     }
 }
 
+// Nodal components a scatter adds: mass and momentum (SC_MV), body and
+// internal force (SC_F), momentum only (SC_MOM).
+template <int D, int WHAT>
+struct Comp {
+  static constexpr int MOM = (WHAT & SC_MV) ? 1 : 0;
+  static constexpr int FE = (WHAT & SC_MV) ? 1 + D : 0;
+  static constexpr int FI = FE + D;
+  static constexpr int NC = ((WHAT & SC_MV) ? 1 + D : 0) + ((WHAT & SC_F) ? 2 * D : 0) +
+                            ((WHAT & SC_MOM) ? D : 0);
+};
+
+// Tasks of one round: lane l of a wave holds (cell segment s, local node i)
+// with l = 2^D s + i. Cells adjacent along x in the sorted order share a
+// face: node 1 (5) of a cell is node 0 (4) of the next, node 2 (6) is node 3
+// (7) of the next (quad: 1 -> 0, 2 -> 3), i.e. the lanes 2^D - 1 and 2^D + 1
+// further on. Where the node ids agree, the lower lane adds its neighbour's
+// sums and the neighbour skips its atomics: about 1.8x fewer FP64 atomics
+// for cell-sorted particles, exact for any mesh (only equal ids merge).
+template <int D, int WHAT>
+MPMX_DEV void merge_face_neighbours(int i, int& n, double* vals) {
+  constexpr int NN = Elem<D>::NN;
+  constexpr int NC = Comp<D, WHAT>::NC;
+  const int lane = threadIdx.x & 63;
+  bool absorbed = false;
+#pragma unroll
+  for (int side = 0; side < 2; ++side) {
+    const int off = side == 0 ? NN - 1 : NN + 1;
+    const int pn = __shfl_down(n, off, 64);
+    const bool give = D == 3 ? (side == 0 ? (i == 1 || i == 5) : (i == 2 || i == 6))
+                             : (side == 0 ? i == 1 : i == 2);
+    const bool take = n >= 0 && give && lane + off < 64 && pn == n;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      const double pv = __shfl_down(vals[c], off, 64);
+      if (take) vals[c] = vals[c] + pv;
+    }
+    const int qn = __shfl_up(n, off, 64);
+    const bool recv = D == 3 ? (side == 0 ? (i == 0 || i == 4) : (i == 3 || i == 7))
+                             : (side == 0 ? i == 0 : i == 3);
+    if (n >= 0 && recv && lane >= off && qn == n) absorbed = true;
+  }
+  if (absorbed) n = -1;
+}
+
 // Phase 2 of every scatter kernel: given staged particles [0, cnt) of this
 // chunk, sorted by cell, add each cell segment's per-node sums to the nodes.
 template <int D, int WHAT>
@@ -393,9 +437,17 @@ MPMX_DEV void scatter_segments(Stage<D, WHAT>& st, int cnt, const Mesh& M, const
     nvalid = cnt;
     while (nvalid > 0 && st.key[nvalid - 1] < 0) --nvalid;
   }
-  for (int task = tid; task < nseg * NN; task += kBlock) {
+  using CP = Comp<D, WHAT>;
+  const int ntask = nseg * NN;
+  for (int t0 = 0; t0 < ntask; t0 += kBlock) {  // same trip count for every lane
+    const int task = t0 + tid;
     const int sidx = task / NN;
     const int i = task - sidx * NN;
+    int n = -1;
+    double vals[CP::NC];
+#pragma unroll
+    for (int k = 0; k < CP::NC; ++k) vals[k] = 0.;
+    if (task < ntask) {
     const int b = st.seg_start[sidx];
     const int e = (sidx + 1 < nseg) ? st.seg_start[sidx + 1] : nvalid;
     const int c = st.key[b];
@@ -433,20 +485,35 @@ MPMX_DEV void scatter_segments(Stage<D, WHAT>& st, int cnt, const Mesh& M, const
         }
       }
     }
-    const int n = M.cell_nodes[(int64_t)c * NN + i];
-    if constexpr ((WHAT & SC_MV) != 0) {
-      atomic_add(&Nd.mass[n], am);
-      Nd.active[n] = 1;
-    }
+    n = M.cell_nodes[(int64_t)c * NN + i];
+    if constexpr ((WHAT & SC_MV) != 0) vals[0] = am;
     if constexpr ((WHAT & (SC_MV | SC_MOM)) != 0) {
 #pragma unroll
-      for (int k = 0; k < D; ++k) atomic_add(&Nd.mom[k * M.nnodes + n], amom[k]);
+      for (int k = 0; k < D; ++k) vals[CP::MOM + k] = amom[k];
     }
     if constexpr ((WHAT & SC_F) != 0) {
 #pragma unroll
       for (int k = 0; k < D; ++k) {
-        atomic_add(&Nd.fext[k * M.nnodes + n], afe[k]);
-        atomic_add(&Nd.fint[k * M.nnodes + n], afi[k]);
+        vals[CP::FE + k] = afe[k];
+        vals[CP::FI + k] = afi[k];
+      }
+    }
+    }  // task < ntask
+    merge_face_neighbours<D, WHAT>(i, n, vals);
+    if (n < 0) continue;
+    if constexpr ((WHAT & SC_MV) != 0) {
+      atomic_add(&Nd.mass[n], vals[0]);
+      Nd.active[n] = 1;
+    }
+    if constexpr ((WHAT & (SC_MV | SC_MOM)) != 0) {
+#pragma unroll
+      for (int k = 0; k < D; ++k) atomic_add(&Nd.mom[k * M.nnodes + n], vals[CP::MOM + k]);
+    }
+    if constexpr ((WHAT & SC_F) != 0) {
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+        atomic_add(&Nd.fext[k * M.nnodes + n], vals[CP::FE + k]);
+        atomic_add(&Nd.fint[k * M.nnodes + n], vals[CP::FI + k]);
       }
     }
   }
