@@ -29,16 +29,25 @@ METRIC = "particle-updates/s (FP64 explicit step) + batched & adjoint steps/s at
 
 # Algorithmic bytes (compulsory HBM traffic of a faithful step, FP64, int32
 # cell id): per particle and per active node, split by the kernel that owns
-# each field. 3D MohrCoulomb, MUSL, fast mode.
-PER_PARTICLE_3D_MC = {
-    "p2g_all": 8,                 # mass read
-    "g2p_momentum_scatter": 144,  # x, u, v read + write
-    "stress_locate": 424,         # stress, strain, F, volume, density r/w, MC state, cell r/w
+# each field; MohrCoulomb, fast mode.
+PER_PARTICLE_MC = {
+    (3, "musl"): {
+        "p2g_all": 8,                 # mass read
+        "g2p_momentum_scatter": 144,  # x, u, v read + write
+        "stress_locate": 424,         # stress, strain, F, volume, density r/w, MC state, cell r/w
+    },
+    (2, "usl"): {
+        "p2g_all": 8,
+        "g2p_stress_locate": 360,     # x, u, v, stress(4), strain(3), V, rho, F(4) r/w, MC state, cell
+    },
 }
-PER_NODE_3D = {
-    "p2g_all": 80,        # mass, momentum, f_ext, f_int written
-    "node_update": 128,   # those read, velocity + acceleration written
-    "g2p_momentum_scatter": 48,  # velocity + acceleration read
+PER_NODE = {
+    (3, "musl"): {
+        "p2g_all": 80,        # mass, momentum, f_ext, f_int written
+        "node_update": 128,   # those read, velocity + acceleration written
+        "g2p_momentum_scatter": 48,  # velocity + acceleration read
+    },
+    (2, "usl"): {"p2g_all": 56, "node_update": 88, "g2p_stress_locate": 32},
 }
 STEP_BYTES_PER_PARTICLE = {(3, "mc"): 576, (3, "le"): 528, (2, "mc"): 368, (2, "le"): 320}
 STEP_BYTES_PER_NODE = {3: 256, 2: 176}
@@ -66,12 +75,54 @@ def cpu_baseline(w, budget_s):
                        "oracle OpenMP build, %d threads, %.1f s" % (n, o.nparticles, o.threads, el))
 
 
+def cpu_baseline_batch(b, budget_s):
+    """Oracle OpenMP build, instances one after another (the reference runs one
+    mesh per analysis): particle-updates/s over a bounded window."""
+    import oracle
+    done, el, threads, ninst = 0, 0.0, 1, 0
+    for w in b.instances:
+        o = w.setup(oracle.Oracle(w.dim, w.scheme, omp=True))
+        o.step(1, w.dt)
+        t0 = time.perf_counter()
+        o.step(10, w.dt)
+        el += time.perf_counter() - t0
+        done += o.nparticles * 10
+        threads = o.threads
+        ninst += 1
+        if el > budget_s:
+            break
+    return dict(value=done / el, unit="particle-updates/s", cores=threads, kind="port",
+                sample="10 steps (after 1 warm-up step) of each of the first %d instances, "
+                       "one at a time, oracle OpenMP build, %d threads, %.1f s"
+                       % (ninst, threads, el))
+
+
+def sequential_gpu(b, steps, warmup=5):
+    """Each instance alone on the GPU (its own solver), one after another:
+    the sum over instances of the per-step time."""
+    from mpm_amd.capi import Solver
+    total = 0.0
+    for w in b.instances:
+        s = w.setup(Solver(w.dim, w.scheme))
+        s.step(warmup, w.dt)
+        s.synchronize()
+        t0 = time.perf_counter()
+        s.step(steps, w.dt)
+        s.synchronize()
+        total += (time.perf_counter() - t0) / steps
+        s.close()
+    return total
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="collapse_3d", choices=["collapse_3d", "column_2d"])
+    ap.add_argument("--workload", default="collapse_3d",
+                    choices=["collapse_3d", "column_2d", "batch_2d"])
+    ap.add_argument("--sequential-steps", type=int, default=20,
+                    help="batch_2d: steps per instance of the one-instance-at-a-time comparison")
     ap.add_argument("--ordered", action="store_true", help="particle-order nodal sums")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of timed CPU work")
     ap.add_argument("--no-cpu", action="store_true")
@@ -116,8 +167,9 @@ def main():
     kernels = {k: dict(avg_ms=v[0] / v[1], launches=v[1]) for k, v in kt.items()}
     dom = max(kt, key=lambda k: kt[k][0])
     mc = int(w.materials[0][0]) == 1
-    per_p = PER_PARTICLE_3D_MC if (w.dim == 3 and mc) else {}
-    alg = per_p.get(dom, 0) * npart + PER_NODE_3D.get(dom, 0) * nact if w.dim == 3 else 0
+    per_p = PER_PARTICLE_MC.get((w.dim, w.scheme), {}) if mc else {}
+    per_n = PER_NODE.get((w.dim, w.scheme), {}) if mc else {}
+    alg = per_p.get(dom, 0) * npart + per_n.get(dom, 0) * nact
     dom_ms = kernels[dom]["avg_ms"]
     achieved = alg / (dom_ms * 1e-3) / 1e9 if alg else None
     traffic = None
@@ -154,8 +206,17 @@ def main():
         "kernels_sum_ms_per_step": profiled_ms_per_step,
         "setup_s": setup_s,
     }
+    if hasattr(w, "instances"):
+        seq = sequential_gpu(w, args.sequential_steps)
+        out["sequential"] = {
+            "what": "the same %d instances, each alone in its own solver, one after another"
+                    % len(w.instances),
+            "ms_per_step": 1e3 * seq, "value": npart / seq,
+            "batched_speedup": seq / (ms_per_step * 1e-3),
+            "steps_per_instance": args.sequential_steps}
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(w, args.cpu_budget)
+        out["cpu_baseline"] = (cpu_baseline_batch(w, args.cpu_budget) if hasattr(w, "instances")
+                               else cpu_baseline(w, args.cpu_budget))
     print(json.dumps(out))
 
 

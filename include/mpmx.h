@@ -79,6 +79,20 @@ const char* mpmx_last_error(const mpmx_ctx* ctx);
 int mpmx_set_mesh(mpmx_ctx* ctx, int64_t nnodes, const double* coords,
                   int64_t ncells, const int64_t* cell_nodes, int isoparametric);
 
+/* Batched runs: the mesh given to mpmx_set_mesh is the union of n independent
+ * instances; instance i owns cells [cell_begin[i], cell_begin[i+1])
+ * (cell_begin[0] = 0, cell_begin[n] = ncells) and no node is shared between
+ * instances (checked). One mpmx_step then advances every instance, each as
+ * the reference's MPMExplicit<Tdim>::solve would advance it alone
+ * (mpm_explicit.tcc:112-190): nodal sums never mix instances, and the
+ * full-scan fallback of locate (mesh.tcc:1142-1151) searches only the
+ * particle's own instance (that of its previous cell, or of its cell hint in
+ * mpmx_set_particles). Instances share dt, gravity and scheme; materials,
+ * constraints and particles are per instance through the usual calls with
+ * union-mesh ids. No counterpart in the reference, which runs one mesh per
+ * analysis. Call after mpmx_set_mesh and before adding particles. */
+int mpmx_set_instances(mpmx_ctx* ctx, int64_t n, const int64_t* cell_begin);
+
 /* Materials (params: n * MPMX_MATERIAL_PARAMS). Material id = index. */
 int mpmx_set_materials(mpmx_ctx* ctx, int n, const double* params);
 

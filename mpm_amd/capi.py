@@ -22,7 +22,8 @@ NO_GRAPH = 0x2
 
 # Every entry point declared in include/mpmx.h.
 EXPORTS = (
-    "mpmx_create", "mpmx_destroy", "mpmx_last_error", "mpmx_set_mesh", "mpmx_set_materials",
+    "mpmx_create", "mpmx_destroy", "mpmx_last_error", "mpmx_set_mesh", "mpmx_set_instances",
+    "mpmx_set_materials",
     "mpmx_set_velocity_constraints", "mpmx_set_acceleration_constraints",
     "mpmx_set_friction_constraints", "mpmx_set_gravity", "mpmx_generate_gauss",
     "mpmx_set_particles", "mpmx_assign_velocity",
@@ -61,6 +62,7 @@ def load(path=LIB_PATH):
         "mpmx_destroy": [V],
         "mpmx_last_error": [V],
         "mpmx_set_mesh": [V, C.c_int64, _D, C.c_int64, _I64, C.c_int],
+        "mpmx_set_instances": [V, C.c_int64, _I64],
         "mpmx_set_materials": [V, C.c_int, _D],
         "mpmx_set_velocity_constraints": [V, C.c_int64, _I64, _I32, _D],
         "mpmx_set_acceleration_constraints": [V, C.c_int64, _I64, _I32, _D],
@@ -143,6 +145,11 @@ class Solver:
         self.nnodes, self.ncells = coords.shape[0], cells.shape[0]
         self._check(self.lib.mpmx_set_mesh(self.ctx, self.nnodes, _p(coords), self.ncells,
                                            _p(cells, _I64), int(bool(isoparametric))))
+
+    def set_instances(self, cell_begin):
+        """Batched run: instance i owns cells [cell_begin[i], cell_begin[i+1])."""
+        cb = np.ascontiguousarray(cell_begin, dtype=np.int64)
+        self._check(self.lib.mpmx_set_instances(self.ctx, len(cb) - 1, _p(cb, _I64)))
 
     def set_materials(self, params):
         params = _f64(params).reshape(-1, 16)

@@ -93,6 +93,10 @@ struct mpmx_ctx {
   std::vector<double> cell_volume;
   DBuf<int> d_cell_nodes, d_nb_ptr, d_nb_idx, d_node_cells;
   DBuf<double> d_node_x, d_centroid, d_len, d_two_mean, d_dndx_c;
+  // instances (batched runs): cell ranges, cell -> instance
+  int64_t ninst = 0;
+  DBuf<int> d_cell_inst;
+  DBuf<int64_t> d_inst_begin;
 
   // nodes
   DBuf<double> d_nmass, d_nmom, d_nvel, d_nacc, d_nfext, d_nfint;
@@ -162,6 +166,8 @@ struct mpmx_ctx {
     M.nb_ptr = d_nb_ptr.p;
     M.nb_idx = d_nb_idx.p;
     M.node_cells = d_node_cells.p;
+    M.cell_inst = ninst > 0 ? d_cell_inst.p : nullptr;
+    M.inst_begin = ninst > 0 ? d_inst_begin.p : nullptr;
     return M;
   }
   Particles parts() const {
@@ -522,10 +528,10 @@ void locate_new(mpmx_ctx* c, int64_t off, int64_t n, const std::vector<int>& hin
   Mesh M = c->mesh();
   if (c->dim == 2) {
     launch_locate_init<2>(M, P, c->d_status.p, c->d_scan_list.p, d_hint.p, c->stream);
-    launch_locate_scan<2>(M, P, c->d_status.p, c->d_scan_list.p, c->stream);
+    launch_locate_scan<2>(M, P, c->d_status.p, c->d_scan_list.p, d_hint.p, c->stream);
   } else {
     launch_locate_init<3>(M, P, c->d_status.p, c->d_scan_list.p, d_hint.p, c->stream);
-    launch_locate_scan<3>(M, P, c->d_status.p, c->d_scan_list.p, c->stream);
+    launch_locate_scan<3>(M, P, c->d_status.p, c->d_scan_list.p, d_hint.p, c->stream);
   }
   HIPCHECK(hipGetLastError());
   read_status(c);
@@ -712,7 +718,7 @@ void enqueue_step(mpmx_ctx* c, const StepArgs& A, Launcher& L, bool full_sort) {
       L("stress_locate", [&] { launch_stress<D>(true, M, P, Nd, mats, A, S, scan, st); });
     }
   }
-  L("locate_scan", [&] { launch_locate_scan<D>(M, P, S, scan, st); });
+  L("locate_scan", [&] { launch_locate_scan<D>(M, P, S, scan, nullptr, st); });
   L("step_end", [&] { launch_step_end(S, st); });
 }
 
@@ -795,6 +801,7 @@ int mpmx_set_mesh(mpmx_ctx* c, int64_t nnodes, const double* coords, int64_t nce
   const int d = c->dim, nn = c->nn;
   c->nnodes = nnodes;
   c->ncells = ncells;
+  c->ninst = 0;
   c->node_x.assign(coords, coords + nnodes * d);
   c->cell_nodes.resize(ncells * nn);
   for (int64_t i = 0; i < ncells * nn; ++i) {
@@ -901,6 +908,40 @@ int mpmx_set_mesh(mpmx_ctx* c, int64_t nnodes, const double* coords, int64_t nce
   c->cons.clear();
   upload_constraints(c);
   alloc_step_buffers(c);
+  c->invalidate_graph();
+  API_END(c)
+}
+
+int mpmx_set_instances(mpmx_ctx* c, int64_t n, const int64_t* cell_begin) {
+  API_BEGIN(c)
+  require_mesh(c);
+  if (n < 1 || !cell_begin) throw ApiError(MPMX_E_ARG, "at least one instance required");
+  if (c->np > 0) throw ApiError(MPMX_E_STATE, "instances must be set before particles");
+  if (cell_begin[0] != 0 || cell_begin[n] != c->ncells)
+    throw ApiError(MPMX_E_ARG, "instance cell ranges must start at 0 and end at ncells");
+  std::vector<int> cell_inst(c->ncells);
+  std::vector<int> node_inst(c->nnodes, -1);
+  for (int64_t i = 0; i < n; ++i) {
+    if (cell_begin[i + 1] <= cell_begin[i])
+      throw ApiError(MPMX_E_ARG, "instance " + std::to_string(i) + " has no cells");
+    for (int64_t ci = cell_begin[i]; ci < cell_begin[i + 1]; ++ci) {
+      cell_inst[ci] = (int)i;
+      for (int q = 0; q < c->nn; ++q) {
+        const int nd = c->cell_nodes[ci * c->nn + q];
+        if (node_inst[nd] >= 0 && node_inst[nd] != (int)i)
+          throw ApiError(MPMX_E_ARG, "node " + std::to_string(nd) + " is shared by instances " +
+                                         std::to_string(node_inst[nd]) + " and " +
+                                         std::to_string(i));
+        node_inst[nd] = (int)i;
+      }
+    }
+  }
+  c->ninst = n;
+  c->d_cell_inst.alloc(c->ncells);
+  c->d_cell_inst.upload(cell_inst.data(), c->ncells, c->stream);
+  c->d_inst_begin.alloc(n + 1);
+  c->d_inst_begin.upload(cell_begin, n + 1, c->stream);
+  HIPCHECK(hipStreamSynchronize(c->stream));
   c->invalidate_graph();
   API_END(c)
 }

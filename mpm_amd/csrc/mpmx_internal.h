@@ -48,6 +48,11 @@ struct Mesh {
   const int* nb_ptr;         // neighbour CSR (ascending ids)
   const int* nb_idx;
   const int* node_cells;     // [nnodes][NN] adjacent cells (-1 pad), ascending
+  // Independent instances sharing one context (batched runs): cells
+  // [inst_begin[i], inst_begin[i+1]) form instance i, cell_inst[c] its index.
+  // Null for a single mesh.
+  const int* cell_inst;
+  const int64_t* inst_begin;
 };
 
 struct Particles {

@@ -19,7 +19,7 @@ void launch_locate_init(const Mesh& M, const Particles& P, Status* S, int* scan_
                         const int* hint, hipStream_t st);
 template <int D>
 void launch_locate_scan(const Mesh& M, const Particles& P, Status* S, const int* scan_list,
-                        hipStream_t st);
+                        const int* hint, hipStream_t st);
 void launch_step_end(Status* S, hipStream_t st);
 template <int D>
 void launch_p2g(int what, const Mesh& M, const Particles& P, const Nodes& Nd, const StepArgs& A,

@@ -265,10 +265,12 @@ __global__ __launch_bounds__(kBlock) void k_locate_init(Mesh M, Particles P, Sta
 
 // Full-mesh scan for queued particles: first cell in ascending id order that
 // contains the point (the single-threaded reading of mesh.tcc:1142-1151).
-// Persistent grid; every block exits after the queue is drained.
+// With instances, the scan covers the cells of the particle's instance (that
+// of its previous cell, or of its hint at initial placement). Persistent
+// grid; every block exits after the queue is drained.
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_locate_scan(Mesh M, Particles P, Status* S,
-                                                        const int* scan_list) {
+                                                        const int* scan_list, const int* hint) {
   __shared__ int best;
   const int n = S->nscan;
   for (int q = blockIdx.x; q < n; q += gridDim.x) {
@@ -276,12 +278,22 @@ __global__ __launch_bounds__(kBlock) void k_locate_scan(Mesh M, Particles P, Sta
     double x[D];
 #pragma unroll
     for (int k = 0; k < D; ++k) x[k] = P.x[k * P.np + p];
+    int64_t lo = 0, hi = M.ncells;
+    if (M.cell_inst) {
+      int prev = P.cell[p];
+      if (prev < 0 && hint) prev = hint[p];
+      if (prev >= 0 && prev < M.ncells) {
+        const int inst = M.cell_inst[prev];
+        lo = M.inst_begin[inst];
+        hi = M.inst_begin[inst + 1];
+      }
+    }
     if (threadIdx.x == 0) best = 0x7fffffff;
     __syncthreads();
-    for (int64_t c0 = 0; c0 < M.ncells; c0 += kBlock) {
+    for (int64_t c0 = lo; c0 < hi; c0 += kBlock) {
       const int64_t c = c0 + threadIdx.x;
       double xi[D];
-      if (c < M.ncells && point_in_cell<D>(M, (int)c, x, xi)) atomicMin(&best, (int)c);
+      if (c < hi && point_in_cell<D>(M, (int)c, x, xi)) atomicMin(&best, (int)c);
       __syncthreads();
       if (best != 0x7fffffff) break;
       __syncthreads();
@@ -992,8 +1004,9 @@ void launch_locate_init(const Mesh& M, const Particles& P, Status* S, int* scan_
 
 template <int D>
 void launch_locate_scan(const Mesh& M, const Particles& P, Status* S, const int* scan_list,
-                        hipStream_t st) {
-  hipLaunchKernelGGL((k_locate_scan<D>), dim3(256), dim3(kBlock), 0, st, M, P, S, scan_list);
+                        const int* hint, hipStream_t st) {
+  hipLaunchKernelGGL((k_locate_scan<D>), dim3(256), dim3(kBlock), 0, st, M, P, S, scan_list,
+                     hint);
 }
 
 void launch_step_end(Status* S, hipStream_t st) {
@@ -1094,7 +1107,7 @@ void launch_gather(int which, const Mesh& M, const Particles& P, const Nodes& Nd
   template void launch_locate_init<D>(const Mesh&, const Particles&, Status*, int*,          \
                                       const int*, hipStream_t);                              \
   template void launch_locate_scan<D>(const Mesh&, const Particles&, Status*, const int*,    \
-                                      hipStream_t);                                          \
+                                      const int*, hipStream_t);                              \
   template void launch_p2g<D>(int, const Mesh&, const Particles&, const Nodes&,              \
                               const StepArgs&, const Status*, const int*, const int*,        \
                               hipStream_t);                                                  \

@@ -157,3 +157,107 @@ def collapse_3d(nx=128, ny=64, nz=64, h=0.01, block=(64, 64, 64), nsteps=2000, d
         info=dict(mesh="hex8 %dx%dx%d @ %g m" % (nx, ny, nz, h),
                   block_cells=list(block), material="MohrCoulomb phi=30 psi=0 c=0",
                   particles_per_cell=8))
+
+
+def collapse_2d(width, height, h, youngs, friction, density, dt=1e-5, nsteps=1500,
+                name="collapse_2d"):
+    """2D granular column collapse: a block of width x height quad4 cells of
+    size h (4 particles per cell) on a mesh 2*width+2 cells wide and height+2
+    high, MohrCoulomb (nu=0.3, psi=0, c=0) from a K0 = 1 - sin(phi) start,
+    fixed base, roller side walls, USL."""
+    nx, ny = 2 * width + 2, height + 2
+    coords, cells = structured_mesh((0.0, 0.0), (h, h), (nx, ny))
+    ci = np.arange(len(cells), dtype=np.int64)
+    cset = ci[(ci % nx < width) & (ci // nx < height)]
+    base = _node_sets(coords, 1, 0.0)
+    x0, x1 = _node_sets(coords, 0, 0.0), _node_sets(coords, 0, nx * h)
+    vn = np.concatenate([base, base, x0, x1])
+    vd = np.concatenate([np.zeros(len(base)), np.ones(len(base)), np.zeros(len(x0)),
+                         np.zeros(len(x1))]).astype(np.int32)
+    return Workload(
+        name=name, dim=2, scheme="usl", coords=coords, cells=cells,
+        materials=mat.mohr_coulomb(youngs, 0.3, density, friction=friction, dilation=0.0,
+                                   cohesion=0.0)[None, :],
+        gravity=np.array([0.0, -9.81]), dt=dt, nsteps=nsteps, nppd=2, cell_set=cset,
+        vel_nodes=vn, vel_dirs=vd, vel_values=np.zeros(len(vn)),
+        geostatic=(1.0 - np.sin(np.radians(friction)), density * 9.81, height * h),
+        info=dict(mesh="quad4 %dx%d @ %g m" % (nx, ny, h), block_cells=[width, height]))
+
+
+class Batch:
+    """Independent instances run as one batched solver (mpmx_set_instances):
+    the union of their meshes with node and cell ids offset per instance,
+    material i for instance i, each instance's constraints and K0 start."""
+
+    def __init__(self, name, instances):
+        self.name = name
+        self.instances = instances
+        w0 = instances[0]
+        self.dim, self.scheme, self.dt = w0.dim, w0.scheme, w0.dt
+        self.gravity = w0.gravity
+        for w in instances:
+            assert (w.dim, w.scheme, w.dt) == (self.dim, self.scheme, self.dt)
+        node_off = np.cumsum([0] + [len(w.coords) for w in instances])
+        cell_off = np.cumsum([0] + [len(w.cells) for w in instances])
+        self.node_begin, self.cell_begin = node_off, cell_off
+        self.coords = np.concatenate([w.coords for w in instances])
+        self.cells = np.concatenate([w.cells + node_off[i] for i, w in enumerate(instances)])
+        self.materials = np.concatenate([w.materials for w in instances])
+        self.info = dict(instances=len(instances), mesh="union of quad4 meshes @ %g m" %
+                         (instances[0].coords[1, 0] - instances[0].coords[0, 0]),
+                         material="MohrCoulomb per instance", particles_per_cell=4)
+
+    @property
+    def nparticles(self):
+        return sum(w.nparticles for w in self.instances)
+
+    def setup(self, solver):
+        solver.set_mesh(self.coords, self.cells)
+        solver.set_instances(self.cell_begin)
+        solver.set_materials(self.materials)
+        nodes, dirs, vals = [], [], []
+        for i, w in enumerate(self.instances):
+            nodes.append(w.vel_nodes + self.node_begin[i])
+            dirs.append(w.vel_dirs)
+            vals.append(w.vel_values)
+        solver.set_velocity_constraints(np.concatenate(nodes), np.concatenate(dirs),
+                                        np.concatenate(vals))
+        solver.set_gravity(self.gravity)
+        for i, w in enumerate(self.instances):
+            solver.generate_gauss(w.nppd, w.cell_set + self.cell_begin[i], i)
+        x = solver.particles(["x"])["x"]
+        stress = np.zeros((len(x), 6))
+        off = 0
+        for w in self.instances:
+            n = w.nparticles
+            k0, gamma, top = w.geostatic
+            sv = -gamma * (top - x[off:off + n, self.dim - 1])
+            stress[off:off + n, self.dim - 1] = sv
+            for k in range(self.dim - 1):
+                stress[off:off + n, k] = k0 * sv
+            if self.dim == 2:
+                stress[off:off + n, 2] = k0 * sv
+            off += n
+        solver.assign_stresses(stress)
+        return solver
+
+
+def batch_2d(ninst=256, seed=4, h=0.01, nmin=2000, nmax=64000, dt=1e-5, nsteps=1500):
+    """Batched 2D collapses: per instance a particle count log-uniform in
+    [nmin, nmax] (4 per cell), block aspect ratio (height/width) uniform in
+    [0.5, 4], E log-uniform in [1e5, 1e7] Pa, phi uniform in [25, 40] deg, rho
+    uniform in [1500, 2200] kg/m^3; all draws from SplitMix64(seed)."""
+    u = uniform(seed, 5 * ninst, 0.0, 1.0).reshape(ninst, 5)
+    inst = []
+    for i in range(ninst):
+        npart = np.exp(np.log(nmin) + u[i, 0] * (np.log(nmax) - np.log(nmin)))
+        aspect = 0.5 + 3.5 * u[i, 1]
+        ncell = max(1, int(round(npart / 4)))
+        width = max(1, int(round(np.sqrt(ncell / aspect))))
+        height = max(1, int(round(ncell / width)))
+        youngs = np.exp(np.log(1e5) + u[i, 2] * (np.log(1e7) - np.log(1e5)))
+        phi = 25.0 + 15.0 * u[i, 3]
+        rho = 1500.0 + 700.0 * u[i, 4]
+        inst.append(collapse_2d(width, height, h, youngs, phi, rho, dt=dt, nsteps=nsteps,
+                                name="batch_2d[%d]" % i))
+    return Batch("batch_2d", inst)
