@@ -133,6 +133,14 @@ int mpmx_set_particles(mpmx_ctx* ctx, int64_t n, const double* x, const double* 
                        const double* density, const int* material_id,
                        const int64_t* cell, const double* statevars);
 
+/* Every particle's volume = volume of its cell / number of particles in the
+ * cell, then mass = volume * density of its material (the passes
+ * MPMBase::initialise_particles runs over all particles after generation and
+ * locate, mpm_base.tcc:413-415 with Particle::compute_volume,
+ * particle.tcc:589-595, and MPMExplicit::solve's compute_mass,
+ * mpm_explicit.tcc:93-95 with particle.tcc:618-626). */
+int mpmx_compute_volume_mass(mpmx_ctx* ctx);
+
 /* Assign every particle's velocity (n must equal the particle count; rows of
  * dim, reference particle order). Replaces Particle::assign_velocity
  * (particle.tcc:771-777) as applied to all particles, e.g. for initial

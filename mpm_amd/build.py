@@ -65,7 +65,21 @@ def build(verbose=False, jobs=4):
     tmp = LIB + ".tmp"
     _run([cc, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs)
     os.replace(tmp, LIB)
+    build_driver()
     return LIB
+
+
+def build_driver():
+    """The C++ analysis driver (mpm_amd/mpmx_run) over the C ABI."""
+    out = os.path.join(HERE, HOST_TOOL)
+    rocm = os.path.dirname(os.path.dirname(os.path.realpath(hipcc())))
+    cxx = os.environ.get("CXX", "g++")
+    _run([cxx, "-O2", "-std=c++17", "-Wall", "-I", os.path.join(HERE, "..", "include"),
+          os.path.join(HERE, "host", "mpmx_run.cpp"), "-o", out + ".tmp",
+          "-L", HERE, "-lmpmx", "-Wl,-rpath,$ORIGIN",
+          "-Wl,-rpath-link," + os.path.join(rocm, "lib")])
+    os.replace(out + ".tmp", out)
+    return out
 
 
 if __name__ == "__main__":

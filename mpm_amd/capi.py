@@ -26,7 +26,7 @@ EXPORTS = (
     "mpmx_set_materials",
     "mpmx_set_velocity_constraints", "mpmx_set_acceleration_constraints",
     "mpmx_set_friction_constraints", "mpmx_set_gravity", "mpmx_generate_gauss",
-    "mpmx_set_particles", "mpmx_assign_velocity",
+    "mpmx_set_particles", "mpmx_compute_volume_mass", "mpmx_assign_velocity",
     "mpmx_assign_stresses", "mpmx_nparticles", "mpmx_get_particles", "mpmx_get_nodes",
     "mpmx_step", "mpmx_set_sort_interval", "mpmx_set_step", "mpmx_get_step",
     "mpmx_synchronize", "mpmx_profile", "mpmx_kernel_times",
@@ -70,6 +70,7 @@ def load(path=LIB_PATH):
         "mpmx_set_gravity": [V, _D, C.c_double],
         "mpmx_generate_gauss": [V, C.c_int, C.c_int64, _I64, C.c_int],
         "mpmx_set_particles": [V, C.c_int64, _D, _D, _D, _D, _D, _D, _D, _D, _D, _I32, _I64, _D],
+        "mpmx_compute_volume_mass": [V],
         "mpmx_assign_velocity": [V, C.c_int64, _D],
         "mpmx_assign_stresses": [V, C.c_int64, _D],
         "mpmx_nparticles": [V, _I64],
@@ -202,6 +203,10 @@ class Solver:
             self.ctx, n, _p(x), _p(a["u"]), _p(a["v"]), _p(a["stress"]), _p(a["strain"]),
             _p(a["F"]), _p(a["mass"]), _p(a["volume"]), _p(a["density"]), _p(mat, _I32),
             _p(cell, _I64), _p(a["sv"])))
+
+    def compute_volume_mass(self):
+        """volume = cell volume / particles in the cell, mass = volume * density."""
+        self._check(self.lib.mpmx_compute_volume_mass(self.ctx))
 
     def assign_velocity(self, v):
         v = _f64(v, (-1, self.dim))

@@ -1125,6 +1125,36 @@ int mpmx_assign_velocity(mpmx_ctx* c, int64_t n, const double* v) {
   API_END(c)
 }
 
+int mpmx_compute_volume_mass(mpmx_ctx* c) {
+  API_BEGIN(c)
+  require_mesh(c);
+  const int64_t n = c->np;
+  if (n == 0) return MPMX_OK;
+  std::vector<int> cell(n), mid(n);
+  c->d_cell.download(cell.data(), n, c->stream);
+  c->d_mat.download(mid.data(), n, c->stream);
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  std::vector<int64_t> count(c->ncells, 0);
+  for (int64_t i = 0; i < n; ++i) {
+    if (cell[i] < 0 || cell[i] >= c->ncells)
+      throw ApiError(MPMX_E_STATE, "particle " + std::to_string(i) + " has no cell");
+    ++count[cell[i]];
+  }
+  std::vector<double> vol(n), mass(n), rho(n);
+  for (int64_t i = 0; i < n; ++i) {
+    if (mid[i] < 0 || mid[i] >= (int)c->mats.size())
+      throw ApiError(MPMX_E_STATE, "particle " + std::to_string(i) + " has no material");
+    vol[i] = c->cell_volume[cell[i]] / (double)count[cell[i]];
+    rho[i] = c->mats[mid[i]].density;
+    mass[i] = vol[i] * rho[i];
+  }
+  c->d_vol.upload(vol.data(), n, c->stream);
+  c->d_rho.upload(rho.data(), n, c->stream);
+  c->d_mass.upload(mass.data(), n, c->stream);
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  API_END(c)
+}
+
 int mpmx_assign_stresses(mpmx_ctx* c, int64_t n, const double* stress) {
   API_BEGIN(c)
   if (n != c->np || !stress)

@@ -1,0 +1,61 @@
+"""The C++ analysis driver (mpm_amd/mpmx_run) reading the reference's input
+deck. On the CPU: --dry-run reads and checks the deck and prints what it
+found, compared here with the deck's own content."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from decks import DRIVER, run_driver, write_deck
+from mpm_amd import materials as mat
+from mpm_amd import workloads
+
+pytestmark = pytest.mark.skipif(not os.path.exists(DRIVER), reason="driver not built")
+
+LE = {"id": 3, "type": "LinearElastic2D", "density": 1800.0, "youngs_modulus": 1e6,
+      "poisson_ratio": 0.3}
+MC3 = {"id": 3, "type": "MohrCoulomb3D", "density": 1800.0, "youngs_modulus": 5e6,
+       "poisson_ratio": 0.3, "friction": 30.0, "dilation": 0.0, "cohesion": 0.0,
+       "residual_friction": 30.0, "residual_dilation": 0.0, "residual_cohesion": 0.0,
+       "peak_pdstrain": 0.0, "residual_pdstrain": 0.0, "tension_cutoff": 0.0,
+       "softening": False}
+
+
+@pytest.mark.parametrize("constraints_as", ["sets", "file"])
+def test_dry_run_reads_the_deck(tmp_path, constraints_as):
+    w = workloads.column_2d(nx=6, ny=10)
+    write_deck(w, str(tmp_path), nsteps=20, output_steps=10, constraints_as=constraints_as,
+               material_json=LE)
+    s = json.loads(run_driver(str(tmp_path), "--dry-run").stdout)
+    assert s["analysis"] == "MPMExplicit2D" and s["scheme"] == "usf"
+    assert s["velocity_update"] == "flip"
+    assert (s["nnodes"], s["ncells"]) == (len(w.coords), len(w.cells))
+    assert s["velocity_constraints"] == len(w.vel_nodes)
+    assert s["particles"] == len(w.cells) * 4
+    assert s["dt"] == w.dt and s["nsteps"] == 20 and s["output_steps"] == 10
+    assert s["gravity"][:2] == list(w.gravity)
+    np.testing.assert_array_equal(s["materials"][0], mat.from_json(LE, 2))
+
+
+def test_dry_run_mohr_coulomb_3d_with_friction(tmp_path):
+    w = workloads.collapse_3d(nx=8, ny=4, nz=4, block=(4, 4, 4))
+    write_deck(w, str(tmp_path), nsteps=5, output_steps=5, material_json=MC3)
+    s = json.loads(run_driver(str(tmp_path), "--dry-run").stdout)
+    assert s["analysis"] == "MPMExplicit3D" and s["scheme"] == "musl"
+    assert s["friction_constraints"] == len(w.fric_nodes)
+    assert s["particles"] == len(w.cell_set) * 8
+    np.testing.assert_array_equal(s["materials"][0], mat.from_json(MC3, 3))
+
+
+def test_deck_errors_are_reported(tmp_path):
+    w = workloads.column_2d(nx=2, ny=2)
+    bad = dict(LE)
+    del bad["youngs_modulus"]
+    write_deck(w, str(tmp_path), nsteps=1, output_steps=1, material_json=bad)
+    r = run_driver(str(tmp_path), "--dry-run", check=False)
+    assert r.returncode == 1 and "youngs_modulus" in r.stderr
+    with open(os.path.join(str(tmp_path), "mpm.json"), "w") as f:
+        f.write('{"analysis": {"type": "MPMExplicit2D", "dt": 1e-3,}')
+    r = run_driver(str(tmp_path), "--dry-run", check=False)
+    assert r.returncode == 1 and "JSON parse error" in r.stderr
