@@ -1,0 +1,111 @@
+"""Step options beyond the benchmark decks, GPU (ordered mode) against the
+oracle bit for bit: Cundall damping, PIC and blended FLIP velocity updates,
+acceleration constraints, prescribed non-zero velocities, gravity ramping,
+one particle per cell, 2D Mohr-Coulomb under every scheme; and fast mode on
+the same decks within the per-step tolerance."""
+import numpy as np
+import pytest
+
+import oracle
+from mpm_amd import materials as mat
+from mpm_amd import workloads
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("x", "u", "v", "stress", "strain", "F", "volume", "density", "xi", "statevars")
+
+
+def _solver(w, ordered, **kw):
+    from mpm_amd.capi import Solver
+    return Solver(w.dim, kw.get("scheme", w.scheme), velocity_update=kw.get("vu", "flip"),
+                  blending_ratio=kw.get("blend", 1.0), ordered=ordered)
+
+
+def _oracle(w, **kw):
+    return oracle.Oracle(w.dim, kw.get("scheme", w.scheme), velocity_update=kw.get("vu", "flip"),
+                         blending_ratio=kw.get("blend", 1.0))
+
+
+def _extra(obj, acc=None, ramp=None):
+    if acc is not None:
+        obj.set_acceleration_constraints(*acc)
+    if ramp is not None:
+        obj.set_gravity(ramp[0], ramp[1])
+    return obj
+
+
+def _run(w, steps, damping=0.0, acc=None, ramp=None, ordered=True, **kw):
+    g = _extra(w.setup(_solver(w, ordered, **kw)), acc, ramp)
+    o = _extra(w.setup(_oracle(w, **kw)), acc, ramp)
+    for _ in range(steps // 10):
+        g.step(10, w.dt, damping)
+        o.step(10, w.dt, damping)
+    return g.particles(), o.particles()
+
+
+def _check(pg, po, exact, tol=1e-9):
+    assert np.array_equal(pg["cell"], po["cell"])
+    for f in FIELDS:
+        if exact:
+            assert np.array_equal(pg[f], po[f]), f
+        else:
+            scale = max(float(np.max(np.abs(po[f]))), 1e-300)
+            assert float(np.max(np.abs(pg[f] - po[f]))) / scale <= tol, f
+
+
+def _column_mc(nx=6, ny=10):
+    w = workloads.column_2d(nx=nx, ny=ny)
+    w.materials = mat.mohr_coulomb(1e6, 0.3, 1800.0, friction=30.0, dilation=5.0,
+                                   cohesion=2000.0)[None, :]
+    return w
+
+
+@pytest.mark.parametrize("ordered", [True, False])
+def test_cundall_damping(ordered):
+    w = workloads.column_2d(nx=6, ny=10)
+    pg, po = _run(w, 50, damping=0.05, ordered=ordered)
+    _check(pg, po, exact=ordered)
+
+
+@pytest.mark.parametrize("vu,blend", [("pic", 1.0), ("flip", 0.7)])
+def test_pic_and_blended_flip(vu, blend):
+    w = workloads.column_2d(nx=6, ny=10)
+    pg, po = _run(w, 40, vu=vu, blend=blend)
+    _check(pg, po, exact=True)
+
+
+def test_acceleration_constraints_and_prescribed_velocity():
+    w = workloads.column_2d(nx=6, ny=10)
+    top = np.nonzero(np.abs(w.coords[:, 1] - w.coords[:, 1].max()) < 1e-9)[0]
+    w.vel_nodes = np.concatenate([w.vel_nodes, top])
+    w.vel_dirs = np.concatenate([w.vel_dirs, np.ones(len(top), dtype=np.int32)])
+    w.vel_values = np.concatenate([w.vel_values, np.full(len(top), -0.01)])
+    xmax = w.coords[:, 0].max()
+    mid = np.nonzero((np.abs(w.coords[:, 1] - 0.25) < 1e-9) & (w.coords[:, 0] > 1e-9)
+                     & (w.coords[:, 0] < xmax - 1e-9))[0]
+    assert len(mid) > 0
+    acc = (mid, np.zeros(len(mid), dtype=np.int32), np.full(len(mid), 0.05))
+    pg, po = _run(w, 40, acc=acc)
+    _check(pg, po, exact=True)
+
+
+def test_gravity_ramp():
+    w = workloads.column_2d(nx=6, ny=10)
+    pg, po = _run(w, 40, ramp=(w.gravity, 20 * w.dt))
+    _check(pg, po, exact=True)
+
+
+def test_one_particle_per_cell():
+    w = workloads.column_2d(nx=6, ny=10)
+    w.nppd = 1
+    pg, po = _run(w, 30)
+    assert len(pg["x"]) == 60
+    _check(pg, po, exact=True)
+
+
+@pytest.mark.parametrize("scheme", ["usf", "usl", "musl"])
+def test_mohr_coulomb_2d(scheme):
+    w = _column_mc()
+    pg, po = _run(w, 60, scheme=scheme)
+    _check(pg, po, exact=True)
+    assert np.any(po["statevars"][:, 0] > 0), "no plastic state reached"
