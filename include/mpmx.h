@@ -199,6 +199,38 @@ int mpmx_profile(mpmx_ctx* ctx, int enable);
 int mpmx_kernel_times(mpmx_ctx* ctx, int cap, char* names, double* total_ms,
                       int64_t* launches, int* n);
 
+/* ------------------------------ adjoint -------------------------------------
+ * Reverse-mode derivatives of a run of explicit steps, for material-parameter
+ * inversion. No counterpart in the reference, which has no differentiation.
+ *
+ * mpmx_adjoint_forward advances nsteps steps exactly as mpmx_step does and
+ * records a checkpoint of the full particle state every checkpoint_every
+ * steps (device memory: about (nsteps / checkpoint_every + 1) * np * 40
+ * doubles). mpmx_adjoint_backward then takes the gradient of a loss L with
+ * respect to the final positions and velocities (dL_dx, dL_dv: rows of dim,
+ * NULL = zero) back through the recorded steps, recomputing each segment
+ * from its checkpoint (memory: checkpoint_every + 1 step records of about
+ * (2 dim + 9) * np values), and returns per particle:
+ *   dL_dE   : Young's modulus of the particle's constitutive update;
+ *   dL_drho : material density, through the mass m = V rho (m/rho fixed);
+ *   dL_dphi : Mohr-Coulomb friction angle in degrees (0 for LinearElastic),
+ *             including the tension cutoff where it is the apex c / tan(phi);
+ *   dL_dx0, dL_dv0 : initial positions and velocities.
+ * Any output may be NULL. The context is left at the state after the
+ * recorded run. Derivatives follow the branches the run took (yield states,
+ * return-mapping iterations, cell membership, values snapped to zero).
+ * Supported: USF, USL, MUSL; FLIP, PIC; Cundall damping; velocity and
+ * acceleration constraints; gravity ramp; LinearElastic, MohrCoulomb without
+ * softening; axis-aligned rectangular/box cells. Friction constraints and
+ * softening return MPMX_E_UNSUPPORTED. */
+int mpmx_adjoint_forward(mpmx_ctx* ctx, int nsteps, double dt, double damping_factor,
+                         int checkpoint_every);
+int mpmx_adjoint_backward(mpmx_ctx* ctx, const double* dL_dx, const double* dL_dv,
+                          double* dL_dE, double* dL_drho, double* dL_dphi, double* dL_dx0,
+                          double* dL_dv0);
+/* Free the adjoint's checkpoints and buffers. */
+int mpmx_adjoint_release(mpmx_ctx* ctx);
+
 #ifdef __cplusplus
 }
 #endif

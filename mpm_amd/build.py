@@ -19,7 +19,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libmpmx.so")
 ARCH = os.environ.get("MPMX_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["mpmx_host.cpp", "mpmx_kernels.hip", "mpmx_sort.hip"]
+SOURCES = ["mpmx_host.cpp", "mpmx_kernels.hip", "mpmx_sort.hip", "mpmx_adjoint.hip"]
 HOST_TOOL = "mpmx_run"  # C++ drop-in driver (mpm::MPMExplicit over the C-ABI)
 
 

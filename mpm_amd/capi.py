@@ -30,6 +30,7 @@ EXPORTS = (
     "mpmx_assign_stresses", "mpmx_nparticles", "mpmx_get_particles", "mpmx_get_nodes",
     "mpmx_step", "mpmx_set_sort_interval", "mpmx_set_step", "mpmx_get_step",
     "mpmx_synchronize", "mpmx_profile", "mpmx_kernel_times",
+    "mpmx_adjoint_forward", "mpmx_adjoint_backward", "mpmx_adjoint_release",
 )
 
 _D = C.POINTER(C.c_double)
@@ -83,6 +84,9 @@ def load(path=LIB_PATH):
         "mpmx_synchronize": [V],
         "mpmx_profile": [V, C.c_int],
         "mpmx_kernel_times": [V, C.c_int, C.c_char_p, _D, _I64, _I32],
+        "mpmx_adjoint_forward": [V, C.c_int, C.c_double, C.c_double, C.c_int],
+        "mpmx_adjoint_backward": [V, _D, _D, _D, _D, _D, _D, _D],
+        "mpmx_adjoint_release": [V],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)
@@ -284,3 +288,27 @@ class Solver:
             nm = names.raw[48 * i:48 * i + 48].split(b"\0", 1)[0].decode()
             out[nm] = (float(ms[i]), int(cnt[i]))
         return out
+
+    # ------------------------------- adjoint ---------------------------------
+    def adjoint_forward(self, nsteps, dt, damping_factor=0.0, checkpoint_every=50):
+        """Advance `nsteps` steps (as step()) recording checkpoints for
+        adjoint_backward."""
+        self._check(self.lib.mpmx_adjoint_forward(self.ctx, int(nsteps), float(dt),
+                                                  float(damping_factor), int(checkpoint_every)))
+
+    def adjoint_backward(self, dL_dx=None, dL_dv=None):
+        """Gradients of a loss of the final positions/velocities of the
+        recorded run: dict with E, rho, phi (degrees) per particle and x0, v0
+        (rows of dim)."""
+        n, d = self.nparticles, self.dim
+        gx = _f64(dL_dx, (n, d)) if dL_dx is not None else None
+        gv = _f64(dL_dv, (n, d)) if dL_dv is not None else None
+        out = dict(E=np.zeros(n), rho=np.zeros(n), phi=np.zeros(n), x0=np.zeros((n, d)),
+                   v0=np.zeros((n, d)))
+        self._check(self.lib.mpmx_adjoint_backward(
+            self.ctx, _p(gx), _p(gv), _p(out["E"]), _p(out["rho"]), _p(out["phi"]),
+            _p(out["x0"]), _p(out["v0"])))
+        return out
+
+    def adjoint_release(self):
+        self._check(self.lib.mpmx_adjoint_release(self.ctx))

@@ -1,0 +1,83 @@
+// Device-side data of the adjoint (reverse-mode) step and its launchers
+// (mpmx_adjoint.hip). Layouts are SoA with the particle or node count as
+// stride, like the forward state.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "mpmx_internal.h"
+
+namespace mpmx {
+
+// The recorded state of one step that the backward pass reads.
+struct Slot {
+  const double* v;       // [dim][np]
+  const double* stress;  // [6][np]
+  const double* vol;     // [np]
+  const double* rho;     // [np]
+  const double* xi;      // [dim][np]
+  const int* cell;       // [np]
+};
+struct SlotW {
+  double* v;
+  double* stress;
+  double* vol;
+  double* rho;
+  double* xi;
+  int* cell;
+};
+
+// Node quantities of the step being transposed (recomputed) and their
+// adjoints.
+struct AdjNodes {
+  double* m;    // mass [nn]
+  double* P;    // momentum [dim][nn]
+  double* f;    // external + internal force
+  double* v;    // velocity from the momentum (before the update)
+  double* a;    // acceleration
+  double* vp;   // updated velocity
+  double* P2;   // MUSL: momentum of the updated particle velocities
+  double* vpp;  // MUSL: velocity from P2
+  double* lvp;  // adjoints of the above
+  double* la;
+  double* lv;
+  double* lvpp;
+  double* lf;
+  double* lm;
+  double* lP;
+  double* lm2;
+  double* lP2;
+};
+
+// Per-particle adjoints of the state (x, v, stress, volume), the shape
+// function adjoint of the current step (lxq), and the accumulated parameter
+// gradients (Young's modulus, friction angle in rad, mass).
+struct AdjParticles {
+  double* lx;    // [dim][np]
+  double* lv;    // [dim][np]
+  double* ls;    // [6][np]
+  double* lvol;  // [np]
+  double* lxq;   // [dim][np]
+  double* gE;
+  double* gphi;
+  double* gm;
+};
+
+struct AdjArgs {
+  double dt, damping, blending;
+  double g[3];  // gravity of this step (ramp applied)
+  int vu;
+};
+
+template <int D>
+void launch_ad_save(const Particles& P, const SlotW& s, hipStream_t st);
+void launch_ad_fill(double* a, int64_t n, double v, hipStream_t st);
+// One backward step: recompute the nodal quantities of step t from the
+// recorded states s0 (step t) and s1 (step t+1), then transpose the step.
+template <int D>
+void launch_ad_backward_step(int scheme, const Mesh& M, const Particles& P, const Nodes& Nd,
+                             const Material* mats, const Slot& s0, const Slot& s1,
+                             const AdjNodes& A, const AdjParticles& L, const AdjArgs& a,
+                             hipStream_t st);
+
+}  // namespace mpmx

@@ -18,6 +18,7 @@
 
 #include "../../include/mpmx.h"
 #include "mpmx_internal.h"
+#include "mpmx_adjoint.h"
 #include "mpmx_kernels.h"
 #include "mpmx_math.h"
 
@@ -133,6 +134,17 @@ struct mpmx_ctx {
   // graph cache: [0] step with key refresh, [1] step with a full sort
   hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
   double graph_dt = 0, graph_damping = 0;
+
+  // adjoint: checkpoints of a recorded run (full particle state every
+  // ad_every steps and at its end), the per-step records of one segment, node
+  // and particle adjoint buffers
+  bool ad_recorded = false;
+  int ad_every = 0, ad_nsteps = 0;
+  double ad_dt = 0, ad_damping = 0;
+  int64_t ad_step0 = 0, ad_np = 0;
+  int64_t ad_ck_stride = 0;  // doubles per checkpoint
+  DBuf<double> d_ad_ck, d_ad_slots, d_ad_nodes, d_ad_parts;
+  DBuf<int> d_ad_ck_cell, d_ad_slot_cell;
 
   // profiling
   bool profile = false;
@@ -349,6 +361,7 @@ Material make_material(const double* p) {
   if (m.type != LE && m.type != MC) throw ApiError(MPMX_E_ARG, "unknown material type");
   m.density = p[1];
   const double E = p[2], nu = p[3];
+  m.E = E;
   if (!(m.density > 0) || !(E > 0) || !(nu > -1.0 && nu < 0.5))
     throw ApiError(MPMX_E_ARG, "material density, youngs_modulus or poisson_ratio out of range");
   if (m.type == LE) {
@@ -385,6 +398,11 @@ Material make_material(const double* p) {
   const double apex = m.c_peak / dmath::tan(m.phi_peak);
   const double tc = (tcut < apex) ? tcut : apex;
   m.sv0[SV_TCUT] = tc > 1.0e-15 ? tc : 1.0e-15;
+  // the cutoff follows the friction angle when it is the apex c / tan(phi)
+  if (!(tcut < apex) && apex > 1.0e-15) {
+    const double sphi = dmath::sin(m.phi_peak);
+    m.dtcut_dphi = -m.c_peak / (sphi * sphi);
+  }
   return m;
 }
 
@@ -566,6 +584,7 @@ int64_t append_particles(mpmx_ctx* c, int64_t n, const double* x, const double* 
   grow(c->d_cell, off, np, 1, st);
   if (c->any_mc) grow(c->d_sv, off, np, NSV, st);
   c->np = np;
+  c->ad_recorded = false;
   std::vector<double> zeros((size_t)n * 9, 0.0), eye((size_t)n * 9, 0.0);
   for (int64_t i = 0; i < n; ++i) eye[i * 9 + 0] = eye[i * 9 + 4] = eye[i * 9 + 8] = 1.0;
   put_soa(c->d_x, np, off, n, d, x, st);
@@ -619,6 +638,7 @@ void truncate_particles(mpmx_ctx* c, int64_t keep) {
   shrink(c->d_mat, 1); shrink(c->d_cell, 1);
   if (c->any_mc) shrink(c->d_sv, NSV);
   c->np = keep;
+  c->ad_recorded = false;
   alloc_step_buffers(c);
   c->invalidate_graph();
 }
@@ -736,6 +756,208 @@ bool next_step_sorts(mpmx_ctx* c) {
   c->need_sort = false;
   c->since_sort = full ? 1 : c->since_sort + 1;
   return full;
+}
+
+
+// Queue nsteps steps on the context's stream (one graph replay each).
+void launch_steps(mpmx_ctx* c, int nsteps, double dt, double damping) {
+  StepArgs A;
+  A.dt = dt;
+  A.damping = damping;
+  A.blending = c->blending;
+  for (int k = 0; k < 3; ++k) A.g[k] = c->g[k];
+  A.ramp = c->ramp;
+  Launcher L{c, {}};
+  const bool use_graph = !(c->flags & MPMX_NO_GRAPH) && !c->profile;
+  if (use_graph) {
+    if (c->graph_dt != dt || c->graph_damping != damping) {
+      c->invalidate_graph();
+      c->graph_dt = dt;
+      c->graph_damping = damping;
+    }
+    for (int s = 0; s < nsteps; ++s) {
+      const int kind = next_step_sorts(c) ? 1 : 0;
+      if (!c->graph_exec[kind]) {
+        hipGraph_t graph;
+        HIPCHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        enqueue(c, A, L, kind == 1);
+        HIPCHECK(hipStreamEndCapture(c->stream, &graph));
+        hipError_t e = hipGraphInstantiate(&c->graph_exec[kind], graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        HIPCHECK(e);
+      }
+      HIPCHECK(hipGraphLaunch(c->graph_exec[kind], c->stream));
+    }
+  } else {
+    for (int s = 0; s < nsteps; ++s) {
+      enqueue(c, A, L, next_step_sorts(c));
+      HIPCHECK(hipGetLastError());
+      L.collect();
+    }
+  }
+}
+
+// Wait for the queued steps; take the step counter; fail if a particle left
+// the mesh (locate_particles, mpm_scheme.tcc:250-256).
+void check_status(mpmx_ctx* c) {
+  read_status(c);
+  c->step = c->h_status->step;
+  if (c->h_status->first_bad_step != std::numeric_limits<int64_t>::max())
+    throw ApiError(MPMX_E_UNLOCATABLE,
+                   "Particle(s) outside the mesh domain: " + std::to_string(c->h_status->bad_particle) +
+                       " (step " + std::to_string(c->h_status->first_bad_step) + ")");
+}
+
+// ------------------------------ adjoint --------------------------------------
+// The particle state a checkpoint holds: every per-particle array the step
+// reads or writes (mass and material are constant and not copied).
+struct StateArray {
+  double* p;
+  int64_t comps;
+};
+std::vector<StateArray> ad_state(mpmx_ctx* c) {
+  const int d = c->dim;
+  std::vector<StateArray> v = {{c->d_x.p, d},      {c->d_u.p, d},      {c->d_v.p, d},
+                               {c->d_stress.p, 6}, {c->d_strain.p, 6}, {c->d_F.p, 9},
+                               {c->d_vol.p, 1},    {c->d_rho.p, 1},    {c->d_xi.p, d}};
+  if (c->any_mc) v.push_back({c->d_sv.p, NSV});
+  return v;
+}
+
+void ad_checkpoint(mpmx_ctx* c, int idx, bool save) {
+  double* base = c->d_ad_ck.p + (int64_t)idx * c->ad_ck_stride;
+  int64_t off = 0;
+  for (const auto& a : ad_state(c)) {
+    const size_t bytes = (size_t)a.comps * c->np * sizeof(double);
+    if (save)
+      HIPCHECK(hipMemcpyAsync(base + off, a.p, bytes, hipMemcpyDeviceToDevice, c->stream));
+    else
+      HIPCHECK(hipMemcpyAsync(a.p, base + off, bytes, hipMemcpyDeviceToDevice, c->stream));
+    off += a.comps * c->np;
+  }
+  int* cb = c->d_ad_ck_cell.p + (int64_t)idx * c->np;
+  if (save)
+    HIPCHECK(hipMemcpyAsync(cb, c->d_cell.p, c->np * sizeof(int), hipMemcpyDeviceToDevice, c->stream));
+  else
+    HIPCHECK(hipMemcpyAsync(c->d_cell.p, cb, c->np * sizeof(int), hipMemcpyDeviceToDevice, c->stream));
+}
+
+int64_t ad_slot_doubles(const mpmx_ctx* c) { return (2 * c->dim + 8) * c->np; }
+
+Slot ad_slot(mpmx_ctx* c, int j) {
+  const int64_t np = c->np, d = c->dim;
+  double* b = c->d_ad_slots.p + (int64_t)j * ad_slot_doubles(c);
+  Slot s;
+  s.v = b;
+  s.stress = b + d * np;
+  s.vol = b + (d + 6) * np;
+  s.rho = b + (d + 7) * np;
+  s.xi = b + (d + 8) * np;
+  s.cell = c->d_ad_slot_cell.p + (int64_t)j * np;
+  return s;
+}
+
+void ad_save_slot(mpmx_ctx* c, int j) {
+  const Slot r = ad_slot(c, j);
+  SlotW w{const_cast<double*>(r.v),   const_cast<double*>(r.stress), const_cast<double*>(r.vol),
+          const_cast<double*>(r.rho), const_cast<double*>(r.xi),     const_cast<int*>(r.cell)};
+  if (c->dim == 2)
+    launch_ad_save<2>(c->parts(), w, c->stream);
+  else
+    launch_ad_save<3>(c->parts(), w, c->stream);
+  HIPCHECK(hipGetLastError());
+}
+
+AdjNodes ad_nodes(mpmx_ctx* c) {
+  const int64_t nn = c->nnodes, d = c->dim;
+  double* b = c->d_ad_nodes.p;
+  AdjNodes A;
+  double** vec[] = {&A.P, &A.f, &A.v, &A.a, &A.vp, &A.P2, &A.vpp,
+                    &A.lvp, &A.la, &A.lv, &A.lvpp, &A.lf, &A.lP, &A.lP2};
+  A.m = b;
+  A.lm = b + nn;
+  A.lm2 = b + 2 * nn;
+  int64_t off = 3 * nn;
+  for (auto* f : vec) {
+    *f = b + off;
+    off += d * nn;
+  }
+  return A;
+}
+int64_t ad_node_doubles(const mpmx_ctx* c) { return (3 + 14 * c->dim) * c->nnodes; }
+
+AdjParticles ad_parts(mpmx_ctx* c) {
+  const int64_t np = c->np, d = c->dim;
+  double* b = c->d_ad_parts.p;
+  AdjParticles L;
+  L.lx = b;
+  L.lv = b + d * np;
+  L.ls = b + 2 * d * np;
+  L.lvol = b + (2 * d + 6) * np;
+  L.lxq = b + (2 * d + 7) * np;
+  L.gE = b + (3 * d + 7) * np;
+  L.gphi = b + (3 * d + 8) * np;
+  L.gm = b + (3 * d + 9) * np;
+  return L;
+}
+int64_t ad_part_doubles(const mpmx_ctx* c) { return (3 * c->dim + 10) * c->np; }
+
+// What the adjoint differentiates: Cartesian cells with the reference's local
+// node order, no friction constraints, no strain softening.
+void ad_check_supported(mpmx_ctx* c) {
+  for (const auto& m : c->mats)
+    if (m.type == MC && m.softening)
+      throw ApiError(MPMX_E_UNSUPPORTED, "adjoint: Mohr-Coulomb strain softening is not supported");
+  for (const auto& kv : c->cons)
+    if (kv.second.fdir >= 0)
+      throw ApiError(MPMX_E_UNSUPPORTED, "adjoint: friction constraints are not supported");
+  const int d = c->dim, nn = c->nn;
+  for (int64_t e = 0; e < c->ncells; ++e) {
+    double lo[3], hi[3];
+    for (int k = 0; k < d; ++k) {
+      lo[k] = hi[k] = c->node_x[(int64_t)c->cell_nodes[e * nn] * d + k];
+      for (int i = 1; i < nn; ++i) {
+        const double x = c->node_x[(int64_t)c->cell_nodes[e * nn + i] * d + k];
+        lo[k] = std::min(lo[k], x);
+        hi[k] = std::max(hi[k], x);
+      }
+    }
+    for (int i = 0; i < nn; ++i) {
+      const int l = i & 3;
+      const int sg[3] = {(l == 0 || l == 3) ? -1 : 1, l < 2 ? -1 : 1, i < 4 ? -1 : 1};
+      for (int k = 0; k < d; ++k) {
+        const double x = c->node_x[(int64_t)c->cell_nodes[e * nn + i] * d + k];
+        const double want = sg[k] < 0 ? lo[k] : hi[k];
+        if (std::fabs(x - want) > 1e-9 * (hi[k] - lo[k]))
+          throw ApiError(MPMX_E_UNSUPPORTED,
+                         "adjoint: cell " + std::to_string(e) +
+                             " is not an axis-aligned rectangle/box in the element node order");
+      }
+    }
+  }
+}
+
+void ad_backward_step(mpmx_ctx* c, int j, int64_t t) {
+  AdjArgs a;
+  a.dt = c->ad_dt;
+  a.damping = c->ad_damping;
+  a.blending = c->blending;
+  a.vu = c->vu;
+  double gm = 1.0;
+  if (c->ramp > 0.0) {
+    const double r = static_cast<double>(t) * c->ad_dt / c->ramp;
+    gm = r < 1.0 ? r : 1.0;
+  }
+  for (int k = 0; k < 3; ++k) a.g[k] = gm * c->g[k];
+  launch_ad_fill(c->d_ad_nodes.p, ad_node_doubles(c), 0., c->stream);
+  const Slot s0 = ad_slot(c, j), s1 = ad_slot(c, j + 1);
+  if (c->dim == 2)
+    launch_ad_backward_step<2>(c->scheme, c->mesh(), c->parts(), c->nodes(), c->d_mats.p, s0, s1,
+                               ad_nodes(c), ad_parts(c), a, c->stream);
+  else
+    launch_ad_backward_step<3>(c->scheme, c->mesh(), c->parts(), c->nodes(), c->d_mats.p, s0, s1,
+                               ad_nodes(c), ad_parts(c), a, c->stream);
+  HIPCHECK(hipGetLastError());
 }
 
 }  // namespace
@@ -1232,47 +1454,9 @@ int mpmx_step(mpmx_ctx* c, int nsteps, double dt, double damping_factor) {
     c->step += nsteps;
     return MPMX_OK;
   }
-  StepArgs A;
-  A.dt = dt;
-  A.damping = damping_factor;
-  A.blending = c->blending;
-  for (int k = 0; k < 3; ++k) A.g[k] = c->g[k];
-  A.ramp = c->ramp;
   reset_status(c);
-  Launcher L{c, {}};
-  const bool use_graph = !(c->flags & MPMX_NO_GRAPH) && !c->profile;
-  if (use_graph) {
-    if (c->graph_dt != dt || c->graph_damping != damping_factor) {
-      c->invalidate_graph();
-      c->graph_dt = dt;
-      c->graph_damping = damping_factor;
-    }
-    for (int s = 0; s < nsteps; ++s) {
-      const int kind = next_step_sorts(c) ? 1 : 0;
-      if (!c->graph_exec[kind]) {
-        hipGraph_t graph;
-        HIPCHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        enqueue(c, A, L, kind == 1);
-        HIPCHECK(hipStreamEndCapture(c->stream, &graph));
-        hipError_t e = hipGraphInstantiate(&c->graph_exec[kind], graph, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(graph);
-        HIPCHECK(e);
-      }
-      HIPCHECK(hipGraphLaunch(c->graph_exec[kind], c->stream));
-    }
-  } else {
-    for (int s = 0; s < nsteps; ++s) {
-      enqueue(c, A, L, next_step_sorts(c));
-      HIPCHECK(hipGetLastError());
-      L.collect();
-    }
-  }
-  read_status(c);
-  c->step = c->h_status->step;
-  if (c->h_status->first_bad_step != std::numeric_limits<int64_t>::max())
-    throw ApiError(MPMX_E_UNLOCATABLE,
-                   "Particle(s) outside the mesh domain: " + std::to_string(c->h_status->bad_particle) +
-                       " (step " + std::to_string(c->h_status->first_bad_step) + ")");
+  launch_steps(c, nsteps, dt, damping_factor);
+  check_status(c);
   API_END(c)
 }
 
@@ -1326,6 +1510,131 @@ int mpmx_kernel_times(mpmx_ctx* c, int cap, char* names, double* total_ms, int64
     if (total_ms) total_ms[i] = c->kt[nm].ms;
     if (launches) launches[i] = c->kt[nm].launches;
   }
+  API_END(c)
+}
+
+int mpmx_adjoint_forward(mpmx_ctx* c, int nsteps, double dt, double damping_factor,
+                         int checkpoint_every) {
+  API_BEGIN(c)
+  require_mesh(c);
+  if (c->mats.empty()) throw ApiError(MPMX_E_STATE, "materials are not set");
+  if (c->np == 0) throw ApiError(MPMX_E_STATE, "no particles");
+  if (nsteps < 1 || !(dt > 0) || checkpoint_every < 1)
+    throw ApiError(MPMX_E_ARG, "nsteps >= 1, dt > 0 and checkpoint_every >= 1 required");
+  ad_check_supported(c);
+  c->ad_recorded = false;
+  const int K = checkpoint_every;
+  const int nseg = (nsteps + K - 1) / K;
+  int64_t per = 0;
+  for (const auto& a : ad_state(c)) per += a.comps;
+  c->ad_ck_stride = per * c->np;
+  c->d_ad_ck.alloc((size_t)(nseg + 1) * c->ad_ck_stride);
+  c->d_ad_ck_cell.alloc((size_t)(nseg + 1) * c->np);
+  c->ad_step0 = c->step;
+  reset_status(c);
+  for (int s = 0; s < nseg; ++s) {
+    ad_checkpoint(c, s, true);
+    launch_steps(c, std::min(K, nsteps - s * K), dt, damping_factor);
+  }
+  ad_checkpoint(c, nseg, true);
+  check_status(c);
+  c->ad_every = K;
+  c->ad_nsteps = nsteps;
+  c->ad_dt = dt;
+  c->ad_damping = damping_factor;
+  c->ad_np = c->np;
+  c->ad_recorded = true;
+  API_END(c)
+}
+
+int mpmx_adjoint_backward(mpmx_ctx* c, const double* dL_dx, const double* dL_dv, double* dL_dE,
+                          double* dL_drho, double* dL_dphi, double* dL_dx0, double* dL_dv0) {
+  API_BEGIN(c)
+  if (!c->ad_recorded || c->ad_np != c->np)
+    throw ApiError(MPMX_E_STATE, "no recorded run (mpmx_adjoint_forward) for these particles");
+  const int K = c->ad_every;
+  const int nseg = (c->ad_nsteps + K - 1) / K;
+  const int64_t np = c->np;
+  const int d = c->dim;
+  hipStream_t st = c->stream;
+  c->d_ad_slots.alloc((size_t)(K + 1) * ad_slot_doubles(c));
+  c->d_ad_slot_cell.alloc((size_t)(K + 1) * np);
+  c->d_ad_nodes.alloc(ad_node_doubles(c));
+  c->d_ad_parts.alloc(ad_part_doubles(c));
+  launch_ad_fill(c->d_ad_parts.p, ad_part_doubles(c), 0., st);
+  const AdjParticles L = ad_parts(c);
+  auto put_rows = [&](double* dev, int comps, const double* rows) {
+    std::vector<double> col(np);
+    for (int k = 0; k < comps; ++k) {
+      for (int64_t i = 0; i < np; ++i) col[i] = rows[i * comps + k];
+      HIPCHECK(hipMemcpyAsync(dev + k * np, col.data(), np * sizeof(double),
+                              hipMemcpyHostToDevice, st));
+      HIPCHECK(hipStreamSynchronize(st));
+    }
+  };
+  auto get_rows = [&](const double* dev, int comps, double* rows, double scale,
+                      const std::vector<double>* per) {
+    std::vector<double> col(np);
+    for (int k = 0; k < comps; ++k) {
+      HIPCHECK(hipMemcpyAsync(col.data(), dev + k * np, np * sizeof(double),
+                              hipMemcpyDeviceToHost, st));
+      HIPCHECK(hipStreamSynchronize(st));
+      for (int64_t i = 0; i < np; ++i)
+        rows[i * comps + k] = col[i] * scale * (per ? (*per)[i] : 1.0);
+    }
+  };
+  if (dL_dx) put_rows(L.lx, d, dL_dx);
+  if (dL_dv) put_rows(L.lv, d, dL_dv);
+  // segments from the last: replay from the checkpoint recording each step,
+  // then transpose the steps in reverse
+  for (int s = nseg - 1; s >= 0; --s) {
+    const int ns = std::min(K, c->ad_nsteps - s * K);
+    ad_checkpoint(c, s, false);
+    c->step = c->ad_step0 + (int64_t)s * K;
+    c->need_sort = true;
+    reset_status(c);
+    for (int j = 0; j < ns; ++j) {
+      ad_save_slot(c, j);
+      launch_steps(c, 1, c->ad_dt, c->ad_damping);
+    }
+    ad_save_slot(c, ns);
+    check_status(c);
+    for (int j = ns - 1; j >= 0; --j) ad_backward_step(c, j, c->ad_step0 + (int64_t)s * K + j);
+  }
+  HIPCHECK(hipStreamSynchronize(st));
+  // parameter gradients: E per particle; phi in degrees (the parameter
+  // vector's unit); density through the mass m = V rho of the material
+  std::vector<double> m_over_rho(np);
+  {
+    std::vector<double> mass(np);
+    std::vector<int> mat(np);
+    HIPCHECK(hipMemcpyAsync(mass.data(), c->d_mass.p, np * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipMemcpyAsync(mat.data(), c->d_mat.p, np * sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    for (int64_t i = 0; i < np; ++i) m_over_rho[i] = mass[i] / c->mats[mat[i]].density;
+  }
+  if (dL_dE) get_rows(L.gE, 1, dL_dE, 1.0, nullptr);
+  if (dL_dphi) get_rows(L.gphi, 1, dL_dphi, M_PI / 180., nullptr);
+  if (dL_drho) get_rows(L.gm, 1, dL_drho, 1.0, &m_over_rho);
+  if (dL_dx0) get_rows(L.lx, d, dL_dx0, 1.0, nullptr);
+  if (dL_dv0) get_rows(L.lv, d, dL_dv0, 1.0, nullptr);
+  // leave the context at the end of the recorded run
+  ad_checkpoint(c, nseg, false);
+  c->step = c->ad_step0 + c->ad_nsteps;
+  c->need_sort = true;
+  HIPCHECK(hipStreamSynchronize(st));
+  API_END(c)
+}
+
+int mpmx_adjoint_release(mpmx_ctx* c) {
+  API_BEGIN(c)
+  c->ad_recorded = false;
+  c->d_ad_ck.release();
+  c->d_ad_ck_cell.release();
+  c->d_ad_slots.release();
+  c->d_ad_slot_cell.release();
+  c->d_ad_nodes.release();
+  c->d_ad_parts.release();
   API_END(c)
 }
 

@@ -25,6 +25,9 @@ struct Material {
   double phi_peak, psi_peak, c_peak, phi_res, psi_res, c_res, pd_peak, pd_res;
   double critical_density;  // minimum_packing_fraction * grain_density
   double sv0[NSV];          // initial state variables
+  double E;                 // Young's modulus (the elastic constants are proportional to it)
+  double dtcut_dphi;        // d(initial tension cutoff)/d(friction angle): -c/sin^2(phi)
+                            // when the cutoff is the apex c/tan(phi), else 0
 };
 
 // Nodal constraint record, referenced by Nodes::cons[node] (-1: none).
