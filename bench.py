@@ -114,13 +114,117 @@ def sequential_gpu(b, steps, warmup=5):
     return total
 
 
+def bench_adjoint(b, args):
+    """Adjoint inversion batch: the plain forward run, the recorded forward
+    run (checkpoints) and the backward pass of the squared distance to a
+    target trajectory, each timed whole; then central differences of that
+    loss (GPU forward runs, h relative per instance parameter) along random
+    per-instance directions of E, rho and phi against the adjoint's
+    directional derivatives."""
+    from mpm_amd.capi import Solver
+    nsteps = args.steps or b.nsteps
+    every = b.checkpoint_every
+
+    def fresh(materials):
+        saved = b.materials
+        b.materials = materials
+        s = b.setup(Solver(b.dim, b.scheme, ordered=args.ordered))
+        b.materials = saved
+        s.synchronize()
+        return s
+
+    def final_x(materials):
+        s = fresh(materials)
+        s.step(nsteps, b.dt)
+        x = s.particles(["x"])["x"]
+        s.close()
+        return x
+
+    xt = final_x(b.target_materials)
+    s = fresh(b.materials)
+    npart = s.nparticles
+    t0 = time.perf_counter()
+    s.step(nsteps, b.dt)
+    s.synchronize()
+    fwd = time.perf_counter() - t0
+    s.close()
+    s = fresh(b.materials)
+    t0 = time.perf_counter()
+    s.adjoint_forward(nsteps, b.dt, 0.0, every)
+    s.synchronize()
+    rec = time.perf_counter() - t0
+    x = s.particles(["x"])["x"]
+    loss = float(np.sum((x - xt) ** 2))
+    t0 = time.perf_counter()
+    g = s.adjoint_backward(2.0 * (x - xt))
+    bwd = time.perf_counter() - t0
+    s.adjoint_release()
+    s.close()
+    # directional checks: central differences of the loss (GPU forward runs)
+    # along random per-instance directions; LinearElastic instances (smooth
+    # map) at h = 1e-6, Mohr-Coulomb instances at h = 1e-3 and 1e-6 (their
+    # run crosses yield-state and iteration-count kinks, so the quotient
+    # does not converge as h shrinks)
+    nin = len(b.instances)
+    bounds = np.cumsum([0] + [w.nparticles for w in b.instances])
+    checks = {}
+    for label, col, sel, hs in (("E_le", 2, slice(0, None, 2), (1e-6,)),
+                                ("rho_le", 1, slice(0, None, 2), (1e-6,)),
+                                ("E_mc", 2, slice(1, None, 2), (1e-3, 1e-6)),
+                                ("rho_mc", 1, slice(1, None, 2), (1e-3, 1e-6)),
+                                ("phi_mc", 4, slice(1, None, 2), (1e-3, 1e-6))):
+        rng = np.random.default_rng(12)
+        d = np.zeros(nin)
+        d[sel] = rng.normal(size=len(range(nin)[sel]))
+        name = label.split("_")[0]
+        per_inst = np.array([g[name][bounds[i]:bounds[i + 1]].sum() for i in range(nin)])
+        ad = float(np.sum(per_inst * b.materials[:, col] * d))
+        entry = {"adjoint": ad}
+        for h in hs:
+            ls = []
+            for sg in (1.0, -1.0):
+                m = b.materials.copy()
+                m[:, col] *= 1.0 + sg * h * d
+                ls.append(float(np.sum((final_x(m) - xt) ** 2)))
+            fd = (ls[0] - ls[1]) / (2 * h)
+            entry["fd_h%g" % h] = fd
+            entry["rel_h%g" % h] = abs(ad - fd) / max(abs(fd), 1e-300)
+        checks[label] = entry
+    return {
+        "metric": METRIC,
+        "value": npart * nsteps / bwd,
+        "unit": "particle-updates/s (adjoint backward)",
+        "n_gpus": 1,
+        "steps": nsteps,
+        "warmup": 0,
+        "ms_per_step": 1e3 * bwd / nsteps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": b.name, "particles": npart, "scheme": b.scheme.upper(),
+                   "mode": "ordered" if args.ordered else "fast", **b.info, "dt": b.dt},
+        "adjoint": {
+            "forward_ms_per_step": 1e3 * fwd / nsteps,
+            "recorded_forward_ms_per_step": 1e3 * rec / nsteps,
+            "backward_ms_per_step": 1e3 * bwd / nsteps,
+            "backward_over_forward": bwd / fwd,
+            "loss": loss,
+            "fd_check": dict(directions="per-instance parameter scalings, normal(seed 12)",
+                             **checks),
+        },
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default 200; adjoint_2d: its 500-step run)")
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="collapse_3d",
-                    choices=["collapse_3d", "column_2d", "batch_2d"])
+                    choices=["collapse_3d", "column_2d", "batch_2d", "adjoint_2d"])
     ap.add_argument("--sequential-steps", type=int, default=20,
                     help="batch_2d: steps per instance of the one-instance-at-a-time comparison")
     ap.add_argument("--ordered", action="store_true", help="particle-order nodal sums")
@@ -137,6 +241,11 @@ def main():
     from mpm_amd.capi import Solver
 
     w = getattr(workloads, args.workload)()
+    if args.workload == "adjoint_2d":
+        print(json.dumps(bench_adjoint(w, args)))
+        return
+    if args.steps is None:
+        args.steps = 200
     t0 = time.perf_counter()
     s = w.setup(Solver(w.dim, w.scheme, ordered=args.ordered))
     setup_s = time.perf_counter() - t0

@@ -228,6 +228,17 @@ int mpmx_adjoint_forward(mpmx_ctx* ctx, int nsteps, double dt, double damping_fa
 int mpmx_adjoint_backward(mpmx_ctx* ctx, const double* dL_dx, const double* dL_dv,
                           double* dL_dE, double* dL_drho, double* dL_dphi, double* dL_dx0,
                           double* dL_dv0);
+/* Mohr-Coulomb returns near the apex of the yield surface (cohesionless
+ * material at low stress, e.g. at a free surface): the directions of the
+ * yield and plastic-potential gradients there depend on a Lode angle that is
+ * ill-conditioned, the exact derivative of one update can exceed 1e3, and
+ * over hundreds of steps the backward pass overflows. For returns whose
+ * plastic correction exceeds `ratio` times the returned stress's deviatoric
+ * norm sqrt(2 J2), the backward pass takes those directions as constants
+ * (keeping the derivatives through the yield-function values, the plastic
+ * multipliers and the elastic predictor). Default 0.3; ratio <= 0
+ * differentiates every return exactly. */
+int mpmx_adjoint_set_apex_ratio(mpmx_ctx* ctx, double ratio);
 /* Free the adjoint's checkpoints and buffers. */
 int mpmx_adjoint_release(mpmx_ctx* ctx);
 

@@ -30,7 +30,8 @@ EXPORTS = (
     "mpmx_assign_stresses", "mpmx_nparticles", "mpmx_get_particles", "mpmx_get_nodes",
     "mpmx_step", "mpmx_set_sort_interval", "mpmx_set_step", "mpmx_get_step",
     "mpmx_synchronize", "mpmx_profile", "mpmx_kernel_times",
-    "mpmx_adjoint_forward", "mpmx_adjoint_backward", "mpmx_adjoint_release",
+    "mpmx_adjoint_forward", "mpmx_adjoint_backward", "mpmx_adjoint_set_apex_ratio",
+    "mpmx_adjoint_release",
 )
 
 _D = C.POINTER(C.c_double)
@@ -86,6 +87,7 @@ def load(path=LIB_PATH):
         "mpmx_kernel_times": [V, C.c_int, C.c_char_p, _D, _I64, _I32],
         "mpmx_adjoint_forward": [V, C.c_int, C.c_double, C.c_double, C.c_int],
         "mpmx_adjoint_backward": [V, _D, _D, _D, _D, _D, _D, _D],
+        "mpmx_adjoint_set_apex_ratio": [V, C.c_double],
         "mpmx_adjoint_release": [V],
     }
     for name, args in sig.items():
@@ -309,6 +311,12 @@ class Solver:
             self.ctx, _p(gx), _p(gv), _p(out["E"]), _p(out["rho"]), _p(out["phi"]),
             _p(out["x0"]), _p(out["v0"])))
         return out
+
+    def adjoint_set_apex_ratio(self, ratio):
+        """Mohr-Coulomb returns near the apex take constant flow directions in
+        the backward pass when the plastic correction exceeds ratio * sqrt(2
+        J2) of the returned stress (default 0.3); ratio <= 0: exact."""
+        self._check(self.lib.mpmx_adjoint_set_apex_ratio(self.ctx, float(ratio)))
 
     def adjoint_release(self):
         self._check(self.lib.mpmx_adjoint_release(self.ctx))

@@ -464,11 +464,32 @@ MPMX_AD void correct(const McParams<T>& m, const T& lam, const T* dp, T* upd) {
   upd[5] += -(lg * dp[5]);
 }
 
+// compute_df_dp at the values of its arguments, as constants (no tangents)
+template <class T>
+MPMX_AD void df_dp_frozen(const McParams<T>& m, int yt, const T* stress, T* df, T* dp, T* dp_dq) {
+  const McParams<double> md{val(m.a1), val(m.a2), val(m.G), val(m.phi),
+                            m.psi,      m.cohesion, val(m.tcut), m.c_peak};
+  double sd[6], a[6], b[6], q;
+  for (int i = 0; i < 6; ++i) sd[i] = val(stress[i]);
+  const Angles<double> Ad = angles(md);
+  Inv<double> Id;
+  invariants(sd, Id);
+  df_dp(md, yt, sd, Id, Ad, a, b, &q);
+  for (int i = 0; i < 6; ++i) {
+    df[i] = cst<T>(a[i]);
+    dp[i] = cst<T>(b[i]);
+  }
+  *dp_dq = cst<T>(q);
+}
+
 // The return mapping (mohr_coulomb.tcc:388-512) for a particle that is not in
 // the separated state: out = updated stress. Returns the yield type of the
-// trial stress (AD_ELASTIC: out = trial).
+// trial stress (AD_ELASTIC: out = trial). With `frozen_flow`, the yield and
+// plastic-potential gradients of every correction carry no tangents (values
+// unchanged).
 template <class T>
-MPMX_AD int mc_update(const McParams<T>& m, const T* stress, const T* dstrain, T* out) {
+MPMX_AD int mc_update(const McParams<T>& m, const T* stress, const T* dstrain, T* out,
+                      bool frozen_flow = false) {
   const double Tolerance = 1E-1;
   T trial[6];
   {
@@ -498,11 +519,17 @@ MPMX_AD int mc_update(const McParams<T>& m, const T* stress, const T* dstrain, T
   T df[6], dp[6], tmp[6], dp_dq;
   T lam;
   if (use_current) {
-    df_dp(m, ytc, stress, Ic, A, df, dp, &dp_dq);
+    if (frozen_flow)
+      df_dp_frozen(m, ytc, stress, df, dp, &dp_dq);
+    else
+      df_dp(m, ytc, stress, Ic, A, df, dp, &dp_dq);
     de_tmul(m, df, tmp);
     lam = dot6(tmp, dstrain) / (dot6(tmp, dp) + 0.);
   } else {
-    df_dp(m, ytt, trial, It, A, df, dp, &dp_dq);
+    if (frozen_flow)
+      df_dp_frozen(m, ytt, trial, df, dp, &dp_dq);
+    else
+      df_dp(m, ytt, trial, It, A, df, dp, &dp_dq);
     de_tmul(m, df, tmp);
     lam = yield_trial / (dot6(tmp, dp) + 0.);
   }
@@ -515,7 +542,10 @@ MPMX_AD int mc_update(const McParams<T>& m, const T* stress, const T* dstrain, T
     if (val(yf[0]) < Tolerance && val(yf[1]) < Tolerance) break;
     if (yt == AD_TENSILE) yield_trial = yf[0];
     if (yt == AD_SHEAR) yield_trial = yf[1];
-    df_dp(m, yt, out, I, A, df, dp, &dp_dq);
+    if (frozen_flow)
+      df_dp_frozen(m, yt, out, df, dp, &dp_dq);
+    else
+      df_dp(m, yt, out, I, A, df, dp, &dp_dq);
     de_tmul(m, df, tmp);
     const T lt = yield_trial / (dot6(tmp, dp) + 0.);
     correct(m, lt, dp, out);

@@ -65,7 +65,8 @@ struct AdjParticles {
 
 struct AdjArgs {
   double dt, damping, blending;
-  double g[3];  // gravity of this step (ramp applied)
+  double g[3];        // gravity of this step (ramp applied)
+  double apex_ratio;  // Mohr-Coulomb returns near the apex: see mpmx_adjoint_set_apex_ratio
   int vu;
 };
 

@@ -26,9 +26,15 @@
 // adjoints are folded into lxq and added to dL/dx at the end of the step.
 // Cell membership is piecewise constant and has no derivative.
 //
-// Non-smooth points follow the forward branch taken: a value the reference
-// snaps to zero (|v| < 1e-15 and the like) passes no derivative; constrained
-// components pass none either.
+// Non-smooth points follow the forward branch taken (yield type, iteration
+// count, constraints: a constrained component passes no derivative), with one
+// exception: the reference's snapping of tiny values to zero (|v|, |a|,
+// strain rate < 1e-15, node.tcc:245-248, 340-343, particle.tcc:829-857) only
+// removes round-off, and its derivative is taken as that of the unsnapped
+// value. Otherwise a body at rest in discrete equilibrium (nodal
+// acceleration at round-off level) would have no derivative with respect to
+// its mass or loads, though any finite perturbation leaves the snapped
+// branch.
 #include <hip/hip_runtime.h>
 
 #include "mpmx_ad.h"
@@ -147,7 +153,7 @@ template <> struct Seeds<3> {
 template <int D, int KC>
 MPMX_DEV void mc_vjp(const Material& m, double phi, double psi, double coh, double tcut,
                      const double* s_in, const double* de, const double* ls_out, double* ls_in,
-                     double* lde, double& gE, double& gphi) {
+                     double* lde, double& gE, double& gphi, bool frozen_flow) {
   using DT = ad::Dual<KC>;
   using Sd = Seeds<D>;
   constexpr int NQ = Sd::NS + Sd::NE + 2;
@@ -183,7 +189,7 @@ MPMX_DEV void mc_vjp(const Material& m, double phi, double psi, double coh, doub
       }
     }
     DT O[6];
-    ad::mc_update(P, S, E6, O);
+    ad::mc_update(P, S, E6, O, frozen_flow);
 #pragma unroll
     for (int j = 0; j < KC; ++j) {
       const int q = base + j;
@@ -208,11 +214,24 @@ MPMX_DEV void mc_vjp(const Material& m, double phi, double psi, double coh, doub
   }
 }
 
+// Near the apex of the Mohr-Coulomb surface (cohesionless material at low
+// stress, e.g. at a free surface) the directions of the yield and plastic
+// potential gradients are set by the Lode angle of a stress whose deviatoric
+// part is small against the step's plastic correction; their derivatives
+// scale with 1 / rho and the step's Jacobian gains up to 1e4, which the
+// backward pass multiplies over hundreds of steps. For returns whose plastic
+// correction exceeds apex_ratio times the returned deviatoric norm
+// sqrt(2 J2), the adjoint takes the gradient directions as constants (the
+// derivatives through the yield-function values, the multipliers and the
+// elastic predictor are kept); apex_ratio <= 0 differentiates every return
+// exactly.
+
 // One constitutive update's adjoint (particle.tcc:880-890 -> material).
 template <int D>
 MPMX_DEV void material_vjp(const Material& m, const Particles& P, int64_t p, const double* s_in,
                            const double* de, double density_new, const double* ls_out,
-                           double* ls_in, double* lde, double& gE, double& gphi) {
+                           double* ls_in, double* lde, double& gE, double& gphi,
+                           double apex_ratio) {
 #pragma unroll
   for (int k = 0; k < 6; ++k) ls_in[k] = lde[k] = 0.;
   bool elastic = m.type == LE;
@@ -224,7 +243,13 @@ MPMX_DEV void material_vjp(const Material& m, const Particles& P, int64_t p, con
     ad::McParams<double> mp{m.a1, m.a2, m.G, phi, psi, coh, tcut, m.c_peak};
     double out[6];
     if (ad::mc_update(mp, s_in, de, out) != ad::AD_ELASTIC) {
-      mc_vjp<D, MPMX_AD_TANGENTS>(m, phi, psi, coh, tcut, s_in, de, ls_out, ls_in, lde, gE, gphi);
+      double dde[6], corr = 0.;
+      de_mul(m, de, dde);
+#pragma unroll
+      for (int k = 0; k < 6; ++k) corr = fmax(corr, fabs((s_in[k] + dde[k]) - out[k]));
+      const bool apex = apex_ratio > 0. && corr > apex_ratio * sqrt(2. * ad::mu_j2(out));
+      mc_vjp<D, MPMX_AD_TANGENTS>(m, phi, psi, coh, tcut, s_in, de, ls_out, ls_in, lde, gE, gphi,
+                                  apex);
       return;
     }
     elastic = true;
@@ -251,10 +276,10 @@ MPMX_DEV void material_vjp(const Material& m, const Particles& P, int64_t p, con
 // into lnv and folds the shape-gradient adjoint into lxi.
 template <int D>
 MPMX_DEV void stress_vjp(const Mesh& M, const Particles& P, const Material& mat, int64_t p,
-                         const Shape<D>& S, const double* nv, double dt, const double* s_in,
-                         double V_in, double rho_in, const double* ls_out, double lV_out,
-                         double* ls_in, double& lV_in, double& gE, double& gphi, double* lnv,
-                         double* lxi) {
+                         const Shape<D>& S, const double* nv, double dt, double apex_ratio,
+                         const double* s_in, double V_in, double rho_in, const double* ls_out,
+                         double lV_out, double* ls_in, double& lV_in, double& gE, double& gphi,
+                         double* lnv, double* lxi) {
   constexpr int NN = Elem<D>::NN;
   const int64_t nn = M.nnodes;
   double v[NN][D], G[NN][D];
@@ -290,17 +315,12 @@ MPMX_DEV void stress_vjp(const Mesh& M, const Particles& P, const Material& mat,
       sc[2] += gc[2] * v[i][2];
     }
   }
-  bool live[6], livec[3];
 #pragma unroll
-  for (int k = 0; k < 6; ++k) {
-    live[k] = !(fabs(sr[k]) < 1.E-15);
-    if (!live[k]) sr[k] = 0.;
-  }
+  for (int k = 0; k < 6; ++k)
+    if (fabs(sr[k]) < 1.E-15) sr[k] = 0.;
 #pragma unroll
-  for (int k = 0; k < D; ++k) {
-    livec[k] = !(fabs(sc[k]) < 1.E-15);
-    if (!livec[k]) sc[k] = 0.;
-  }
+  for (int k = 0; k < D; ++k)
+    if (fabs(sc[k]) < 1.E-15) sc[k] = 0.;
   double de[6];
 #pragma unroll
   for (int k = 0; k < 6; ++k) de[k] = sr[k] * dt;
@@ -311,17 +331,17 @@ MPMX_DEV void stress_vjp(const Mesh& M, const Particles& P, const Material& mat,
   const double rho_new = rho_in / (1. + dvol);
   // constitutive update
   double lde[6];
-  material_vjp<D>(mat, P, p, s_in, de, rho_new, ls_out, ls_in, lde, gE, gphi);
+  material_vjp<D>(mat, P, p, s_in, de, rho_new, ls_out, ls_in, lde, gE, gphi, apex_ratio);
   // volume: V' = V (1 + dvol)
   lV_in = lV_out * (1. + dvol);
   const double ldvol = lV_out * V_in;
   double lsc[3];
 #pragma unroll
-  for (int k = 0; k < D; ++k) lsc[k] = livec[k] ? ldvol * dt : 0.;
+  for (int k = 0; k < D; ++k) lsc[k] = ldvol * dt;
   // strain increment -> velocity gradient L[a][b] = sum_i v_i[a] G_i[b]
   double lsr[6];
 #pragma unroll
-  for (int k = 0; k < 6; ++k) lsr[k] = live[k] ? lde[k] * dt : 0.;
+  for (int k = 0; k < 6; ++k) lsr[k] = lde[k] * dt;
   double lL[D][D];
   if constexpr (D == 2) {
     lL[0][0] = lsr[0];
@@ -598,7 +618,6 @@ MPMX_DEV void velocity_vjp(double mass, const double* mom, const NodeCons* c, co
   for (int k = 0; k < D; ++k) {
     if (c && (c->vmask & (1 << k))) continue;
     const double vk = mom[k] / mass;
-    if (fabs(vk) < 1.E-15) continue;
     lP[k] = lv[k] / mass;
     lm += -vk * lv[k] / mass;
   }
@@ -661,10 +680,8 @@ __global__ __launch_bounds__(kAdBlock) void k_ad_node(Mesh M, Nodes Nd, AdjNodes
       for (int k = 0; k < D; ++k) {
         if (c && (c->vmask & (1 << k))) continue;  // v'_k and a_k fixed
         const bool acon = c && (c->amask & (1 << k));
-        const double a_k = acon ? c->aval[k] : ap[k];
-        const double vnew = v[k] + a_k * a.dt;
-        const double lvp = fabs(vnew) < tol ? 0. : A.lvp[k * nn + n];
-        const double la = fabs(a_k) < tol ? 0. : A.la[k * nn + n];
+        const double lvp = A.lvp[k * nn + n];
+        const double la = A.la[k * nn + n];
         lv[k] += lvp;
         if (acon) continue;
         const double lap = la + a.dt * lvp;
@@ -721,8 +738,9 @@ __global__ __launch_bounds__(kAdBlock) void k_ad_b1(Mesh M, Particles P,
     load_vec<D>(s0.stress, np, p, 6, s_in);
     load_vec<D>(L.ls, np, p, 6, ls_out);
     double lV_in = 0., gE = L.gE[p], gphi = L.gphi[p];
-    stress_vjp<D>(M, P, mat, p, S, SCH == USL ? A.vp : A.vpp, a.dt, s_in, s0.vol[p], s0.rho[p],
-                  ls_out, L.lvol[p], ls_in, lV_in, gE, gphi, SCH == USL ? A.lvp : A.lvpp, lxi);
+    stress_vjp<D>(M, P, mat, p, S, SCH == USL ? A.vp : A.vpp, a.dt, a.apex_ratio, s_in,
+                  s0.vol[p], s0.rho[p], ls_out, L.lvol[p], ls_in, lV_in, gE, gphi,
+                  SCH == USL ? A.lvp : A.lvpp, lxi);
     store_vec<D>(L.ls, np, p, 6, ls_in);
     L.lvol[p] = lV_in;
     L.gE[p] = gE;
@@ -782,8 +800,8 @@ __global__ __launch_bounds__(kAdBlock) void k_ad_b3(Mesh M, Particles P,
     double s_in[6], ls_in[6];
     load_vec<D>(s0.stress, np, p, 6, s_in);
     double lV_in = 0., gE = L.gE[p], gphi = L.gphi[p];
-    stress_vjp<D>(M, P, mat, p, S, A.v, a.dt, s_in, s0.vol[p], s0.rho[p], ls1, lV1, ls_in,
-                  lV_in, gE, gphi, A.lv, lxi);
+    stress_vjp<D>(M, P, mat, p, S, A.v, a.dt, a.apex_ratio, s_in, s0.vol[p], s0.rho[p], ls1,
+                  lV1, ls_in, lV_in, gE, gphi, A.lv, lxi);
     store_vec<D>(L.ls, np, p, 6, ls_in);
     L.lvol[p] = lV_in;
     L.gE[p] = gE;

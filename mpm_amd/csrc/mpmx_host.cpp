@@ -141,6 +141,7 @@ struct mpmx_ctx {
   bool ad_recorded = false;
   int ad_every = 0, ad_nsteps = 0;
   double ad_dt = 0, ad_damping = 0;
+  double ad_apex_ratio = 0.3;
   int64_t ad_step0 = 0, ad_np = 0;
   int64_t ad_ck_stride = 0;  // doubles per checkpoint
   DBuf<double> d_ad_ck, d_ad_slots, d_ad_nodes, d_ad_parts;
@@ -943,6 +944,7 @@ void ad_backward_step(mpmx_ctx* c, int j, int64_t t) {
   a.damping = c->ad_damping;
   a.blending = c->blending;
   a.vu = c->vu;
+  a.apex_ratio = c->ad_apex_ratio;
   double gm = 1.0;
   if (c->ramp > 0.0) {
     const double r = static_cast<double>(t) * c->ad_dt / c->ramp;
@@ -1623,6 +1625,13 @@ int mpmx_adjoint_backward(mpmx_ctx* c, const double* dL_dx, const double* dL_dv,
   c->step = c->ad_step0 + c->ad_nsteps;
   c->need_sort = true;
   HIPCHECK(hipStreamSynchronize(st));
+  API_END(c)
+}
+
+int mpmx_adjoint_set_apex_ratio(mpmx_ctx* c, double ratio) {
+  API_BEGIN(c)
+  if (!(ratio == ratio)) throw ApiError(MPMX_E_ARG, "apex ratio is NaN");
+  c->ad_apex_ratio = ratio;
   API_END(c)
 }
 

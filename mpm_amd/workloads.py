@@ -261,3 +261,42 @@ def batch_2d(ninst=256, seed=4, h=0.01, nmin=2000, nmax=64000, dt=1e-5, nsteps=1
         inst.append(collapse_2d(width, height, h, youngs, phi, rho, dt=dt, nsteps=nsteps,
                                 name="batch_2d[%d]" % i))
     return Batch("batch_2d", inst)
+
+
+def adjoint_2d(ninst=64, seed=5, h=0.01, columns=64, dt=1e-5, nsteps=500, checkpoint_every=50,
+               perturb=0.1):
+    """Adjoint inversion batch: `ninst` 2D column collapses, each a column of
+    columns x columns quad4 cells (4 particles per cell: 16384 particles for
+    64) on a mesh twice as wide (runout space) plus walls, USL, dt=1e-5,
+    500 steps recorded with a checkpoint every 50. Instances alternate
+    LinearElastic (even) and MohrCoulomb (odd, phi uniform in [28, 38] deg,
+    c=0, psi=0); E log-uniform in [5e5, 5e6] Pa and rho uniform in
+    [1600, 2000] kg/m^3 per instance, from SplitMix64(seed). The target
+    trajectory comes from the same batch with every instance's E, rho and
+    phi scaled by factors uniform in 1 +- `perturb` (SplitMix64(seed + 1))."""
+    u = uniform(seed, 3 * ninst, 0.0, 1.0).reshape(ninst, 3)
+    inst = []
+    for i in range(ninst):
+        youngs = np.exp(np.log(5e5) + u[i, 0] * (np.log(5e6) - np.log(5e5)))
+        rho = 1600.0 + 400.0 * u[i, 1]
+        phi = 28.0 + 10.0 * u[i, 2]
+        w = collapse_2d(columns, columns, h, youngs, phi, rho, dt=dt, nsteps=nsteps,
+                        name="adjoint_2d[%d]" % i)
+        if i % 2 == 0:
+            nu = 0.3
+            w.materials = mat.linear_elastic(youngs, nu, rho)[None, :]
+            w.geostatic = (nu / (1.0 - nu), rho * 9.81, columns * h)
+        inst.append(w)
+    b = Batch("adjoint_2d", inst)
+    b.nsteps, b.checkpoint_every = nsteps, checkpoint_every
+    f = uniform(seed + 1, 3 * ninst, 1.0 - perturb, 1.0 + perturb).reshape(ninst, 3)
+    target = b.materials.copy()
+    target[:, 2] *= f[:, 0]          # E
+    target[:, 1] *= f[:, 1]          # rho
+    target[1::2, 4] *= f[1::2, 2]    # phi of the MohrCoulomb instances
+    b.target_materials = target
+    b.info = dict(instances=ninst, mesh="union of quad4 %dx%d meshes @ %g m" %
+                  (2 * columns + 2, columns + 2, h), particles_per_cell=4,
+                  material="LinearElastic (even) / MohrCoulomb (odd) per instance",
+                  nsteps=nsteps, checkpoint_every=checkpoint_every)
+    return b
