@@ -16,6 +16,7 @@ struct Slot {
   const double* vol;     // [np]
   const double* rho;     // [np]
   const double* xi;      // [dim][np]
+  const double* yield;   // [np] Mohr-Coulomb yield state (null: no Mohr-Coulomb)
   const int* cell;       // [np]
 };
 struct SlotW {
@@ -24,11 +25,12 @@ struct SlotW {
   double* vol;
   double* rho;
   double* xi;
+  double* yield;
   int* cell;
 };
 
-// Node quantities of the step being transposed (recomputed) and their
-// adjoints.
+// Node quantities of the step being transposed (recorded by the replayed
+// forward step; v computed from mass and momentum) and their adjoints.
 struct AdjNodes {
   double* m;    // mass [nn]
   double* P;    // momentum [dim][nn]
@@ -63,6 +65,18 @@ struct AdjParticles {
   double* gm;
 };
 
+// Plastic Mohr-Coulomb updates of a backward step, whose constitutive
+// adjoint runs in a separate pass over this compacted list: count at
+// count[0]; per entry the particle id, the apex flag and (SoA, stride cap)
+// the strain increment, replaced by its adjoint.
+struct AdjList {
+  int* count;
+  int* ids;
+  int* apex;
+  double* buf;  // [6][cap]
+  int64_t cap;
+};
+
 struct AdjArgs {
   double dt, damping, blending;
   double g[3];        // gravity of this step (ramp applied)
@@ -73,12 +87,12 @@ struct AdjArgs {
 template <int D>
 void launch_ad_save(const Particles& P, const SlotW& s, hipStream_t st);
 void launch_ad_fill(double* a, int64_t n, double v, hipStream_t st);
-// One backward step: recompute the nodal quantities of step t from the
-// recorded states s0 (step t) and s1 (step t+1), then transpose the step.
+// One backward step: transpose step t from the recorded particle states s0
+// (step t) and s1 (step t+1) and the step's recorded nodal quantities.
 template <int D>
 void launch_ad_backward_step(int scheme, const Mesh& M, const Particles& P, const Nodes& Nd,
                              const Material* mats, const Slot& s0, const Slot& s1,
-                             const AdjNodes& A, const AdjParticles& L, const AdjArgs& a,
-                             hipStream_t st);
+                             const AdjNodes& A, const AdjParticles& L, const AdjList& Q,
+                             const AdjArgs& a, hipStream_t st);
 
 }  // namespace mpmx

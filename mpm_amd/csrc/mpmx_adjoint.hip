@@ -7,11 +7,11 @@
 // back. The adjoint of one step takes lambda_{t+1} = dL/dz_{t+1} to
 // lambda_t and accumulates dL/dE, dL/dphi and dL/dm per particle.
 //
-// Inputs of a backward step: the recorded state of step t (and of t+1 for
-// USF and MUSL), and the nodal quantities of step t, recomputed here by
-// k_ad_p2g (mass, momentum, force; MUSL: second momentum) and k_ad_node_fwd
-// (nodal velocity, acceleration, updated velocity). The kernels then
-// transpose the step phase by phase, per scheme:
+// Inputs of a backward step: the recorded particle state of step t (and of
+// t+1 for USF and MUSL), and the nodal quantities of step t that the replayed
+// forward step recorded (mass, momentum, force, acceleration, updated
+// velocity; MUSL: second momentum and its velocity; StepArgs::ad_nodes).
+// The kernels then transpose the step phase by phase, per scheme:
 //
 //   USL : b1 [stress (v'), G2P] -> node [update, velocity] -> b3 [forces(t), P2G]
 //   USF : b1 [G2P] -> node [update] -> b3 [forces(t+1), stress (v)]
@@ -47,7 +47,10 @@ namespace {
 
 constexpr int kAdBlock = 256;
 #ifndef MPMX_AD_TANGENTS
-#define MPMX_AD_TANGENTS 1
+#define MPMX_AD_TANGENTS 3
+#endif
+#ifndef MPMX_AD_ATTR
+#define MPMX_AD_ATTR
 #endif
 
 MPMX_DEV void ad_atomic(double* p, double v) {
@@ -227,69 +230,124 @@ MPMX_DEV void mc_vjp(const Material& m, double phi, double psi, double coh, doub
 // exactly.
 
 // One constitutive update's adjoint (particle.tcc:880-890 -> material).
+// yield_new and s_out are the update's result as the recorded run computed
+// it (Mohr-Coulomb yield state and stress after the update). A plastic
+// Mohr-Coulomb update is not differentiated here: it is appended to Q
+// (strain increment, apex flag) for k_ad_mc, and the function returns true
+// with ls_in and lde untouched.
 template <int D>
-MPMX_DEV void material_vjp(const Material& m, const Particles& P, int64_t p, const double* s_in,
-                           const double* de, double density_new, const double* ls_out,
-                           double* ls_in, double* lde, double& gE, double& gphi,
-                           double apex_ratio) {
+MPMX_DEV bool material_vjp(const Material& m, const double* s_in, const double* de,
+                           double yield_new, const double* s_out, const double* ls_out,
+                           double* ls_in, double* lde, double& gE, double apex_ratio,
+                           int64_t p, const AdjList& Q) {
 #pragma unroll
   for (int k = 0; k < 6; ++k) ls_in[k] = lde[k] = 0.;
-  bool elastic = m.type == LE;
+  double dde[6];
+  de_mul(m, de, dde);
   if (m.type == MC) {
-    if (density_new <= m.critical_density) return;  // separated: stress = 0
-    const int64_t np = P.np;
-    const double phi = P.sv[SV_PHI * np + p], psi = P.sv[SV_PSI * np + p];
-    const double coh = P.sv[SV_COH * np + p], tcut = P.sv[SV_TCUT * np + p];
-    ad::McParams<double> mp{m.a1, m.a2, m.G, phi, psi, coh, tcut, m.c_peak};
-    double out[6];
-    if (ad::mc_update(mp, s_in, de, out) != ad::AD_ELASTIC) {
-      double dde[6], corr = 0.;
-      de_mul(m, de, dde);
+    if (yield_new == 3.) return false;  // separated: stress = 0
+    if (yield_new != 0.) {
+      double corr = 0.;
 #pragma unroll
-      for (int k = 0; k < 6; ++k) corr = fmax(corr, fabs((s_in[k] + dde[k]) - out[k]));
-      const bool apex = apex_ratio > 0. && corr > apex_ratio * sqrt(2. * ad::mu_j2(out));
-      mc_vjp<D, MPMX_AD_TANGENTS>(m, phi, psi, coh, tcut, s_in, de, ls_out, ls_in, lde, gE, gphi,
-                                  apex);
-      return;
+      for (int k = 0; k < 6; ++k) corr = fmax(corr, fabs((s_in[k] + dde[k]) - s_out[k]));
+      const bool apex = apex_ratio > 0. && corr > apex_ratio * sqrt(2. * ad::mu_j2(s_out));
+      const int slot = atomicAdd(Q.count, 1);
+      Q.ids[slot] = (int)p;
+      Q.apex[slot] = apex ? 1 : 0;
+#pragma unroll
+      for (int k = 0; k < 6; ++k) Q.buf[k * Q.cap + slot] = de[k];
+      return true;
     }
-    elastic = true;
   }
-  if (elastic) {
-    // s' = s + D de: ls_in = ls_out, lde = D ls_out, dE: ls_out . (D de) / E
-    double dde[6];
-    de_mul(m, de, dde);
-    de_mul(m, ls_out, lde);
-    double t = 0.;
+  // elastic: s' = s + D de: ls_in = ls_out, lde = D ls_out, dE: ls_out . (D de) / E
+  de_mul(m, ls_out, lde);
+  double t = 0.;
 #pragma unroll
-    for (int k = 0; k < 6; ++k) {
-      ls_in[k] = ls_out[k];
-      t += ls_out[k] * dde[k];
+  for (int k = 0; k < 6; ++k) {
+    ls_in[k] = ls_out[k];
+    t += ls_out[k] * dde[k];
+  }
+  gE += t / m.E;
+  return false;
+}
+
+// Velocity-gradient adjoint of one particle's strain increment: adds
+// dL/d(nodal velocity) = sum_b lL[a][b] dN_i/dx_b (+ the centroid term of the
+// volume change, lsc) to acc and folds dL/d(dN_i/dx) into lxi.
+template <int D>
+MPMX_DEV void strain_rate_vjp(const Mesh& M, const Shape<D>& S, const double (*v)[D],
+                              const double (*G)[D], const double* lde, const double* lsc,
+                              double dt, double (*acc)[D], double* lxi) {
+  constexpr int NN = Elem<D>::NN;
+  const double* dc = M.dndx_c + (int64_t)S.c * NN * D;
+  double lsr[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) lsr[k] = lde[k] * dt;
+  double lL[D][D];
+  if constexpr (D == 2) {
+    lL[0][0] = lsr[0];
+    lL[1][1] = lsr[1];
+    lL[0][1] = lL[1][0] = lsr[3];
+  } else {
+    lL[0][0] = lsr[0];
+    lL[1][1] = lsr[1];
+    lL[2][2] = lsr[2];
+    lL[0][1] = lL[1][0] = lsr[3];
+    lL[1][2] = lL[2][1] = lsr[4];
+    lL[0][2] = lL[2][0] = lsr[5];
+  }
+#pragma unroll
+  for (int i = 0; i < NN; ++i) {
+    const double* gc = dc + i * D;
+    double lG[D];
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+      double t = lsc ? lsc[a] * gc[a] : 0.;
+#pragma unroll
+      for (int b = 0; b < D; ++b) t += lL[a][b] * G[i][b];
+      acc[i][a] += t;
     }
-    gE += t / m.E;
+#pragma unroll
+    for (int b = 0; b < D; ++b) {
+      double t = 0.;
+#pragma unroll
+      for (int a = 0; a < D; ++a) t += lL[a][b] * v[i][a];
+      lG[b] = t;
+    }
+    fold_xi<D>(S, i, 0., lG, lxi);
   }
 }
 
-// Adjoint of compute_stress_strain for one particle (mpm_scheme.tcc:75-103:
-// strain rate from the nodal velocity nv, strain increment, volume and
-// density update, constitutive update). Returns the adjoints of the input
-// stress and volume, adds dL/dE and dL/dphi, scatters dL/d(nodal velocity)
-// into lnv and folds the shape-gradient adjoint into lxi.
 template <int D>
-MPMX_DEV void stress_vjp(const Mesh& M, const Particles& P, const Material& mat, int64_t p,
-                         const Shape<D>& S, const double* nv, double dt, double apex_ratio,
-                         const double* s_in, double V_in, double rho_in, const double* ls_out,
-                         double lV_out, double* ls_in, double& lV_in, double& gE, double& gphi,
-                         double* lnv, double* lxi) {
+MPMX_DEV void load_kinematics(const Mesh& M, const Shape<D>& S, const double* nv,
+                              double (*v)[D], double (*G)[D]) {
   constexpr int NN = Elem<D>::NN;
   const int64_t nn = M.nnodes;
-  double v[NN][D], G[NN][D];
-  const double* dc = M.dndx_c + (int64_t)S.c * NN * D;
 #pragma unroll
   for (int i = 0; i < NN; ++i) {
 #pragma unroll
     for (int k = 0; k < D; ++k) v[i][k] = nv[k * nn + S.node[i]];
     dndx_i<D>(S.xi, i, S.Ji, G[i]);
   }
+}
+
+// Adjoint of compute_stress_strain for one particle (mpm_scheme.tcc:75-103:
+// strain rate from the nodal velocity nv, strain increment, volume and
+// density update, constitutive update). Returns the adjoints of the input
+// stress and volume, adds dL/dE, adds dL/d(nodal velocity) to acc and
+// folds the shape-gradient adjoint into lxi. Returns true when the
+// constitutive adjoint was deferred to k_ad_mc (ls_in not set; the volume
+// part is done).
+template <int D>
+MPMX_DEV bool stress_vjp(const Mesh& M, const Material& mat, int64_t p, const Shape<D>& S,
+                         const double* nv, double dt, double apex_ratio, const double* s_in,
+                         double V_in, double yield_new, const double* s_out,
+                         const double* ls_out, double lV_out, double* ls_in, double& lV_in,
+                         double& gE, double (*acc)[D], double* lxi, const AdjList& Q) {
+  constexpr int NN = Elem<D>::NN;
+  double v[NN][D], G[NN][D];
+  load_kinematics<D>(M, S, nv, v, G);
+  const double* dc = M.dndx_c + (int64_t)S.c * NN * D;
   // strain rate at the particle and at the centroid (particle.tcc:819-858)
   double sr[6], sc[6];
 #pragma unroll
@@ -328,60 +386,26 @@ MPMX_DEV void stress_vjp(const Mesh& M, const Particles& P, const Material& mat,
 #pragma unroll
   for (int k = 1; k < D; ++k) ssum = ssum + sc[k];
   const double dvol = dt * ssum;
-  const double rho_new = rho_in / (1. + dvol);
   // constitutive update
   double lde[6];
-  material_vjp<D>(mat, P, p, s_in, de, rho_new, ls_out, ls_in, lde, gE, gphi, apex_ratio);
+  const bool deferred = material_vjp<D>(mat, s_in, de, yield_new, s_out, ls_out, ls_in, lde, gE,
+                                        apex_ratio, p, Q);
   // volume: V' = V (1 + dvol)
   lV_in = lV_out * (1. + dvol);
-  const double ldvol = lV_out * V_in;
   double lsc[3];
 #pragma unroll
-  for (int k = 0; k < D; ++k) lsc[k] = ldvol * dt;
-  // strain increment -> velocity gradient L[a][b] = sum_i v_i[a] G_i[b]
-  double lsr[6];
-#pragma unroll
-  for (int k = 0; k < 6; ++k) lsr[k] = lde[k] * dt;
-  double lL[D][D];
-  if constexpr (D == 2) {
-    lL[0][0] = lsr[0];
-    lL[1][1] = lsr[1];
-    lL[0][1] = lL[1][0] = lsr[3];
-  } else {
-    lL[0][0] = lsr[0];
-    lL[1][1] = lsr[1];
-    lL[2][2] = lsr[2];
-    lL[0][1] = lL[1][0] = lsr[3];
-    lL[1][2] = lL[2][1] = lsr[4];
-    lL[0][2] = lL[2][0] = lsr[5];
-  }
-#pragma unroll
-  for (int i = 0; i < NN; ++i) {
-    const double* gc = dc + i * D;
-    double lG[D];
-#pragma unroll
-    for (int a = 0; a < D; ++a) {
-      double t = lsc[a] * gc[a];
-#pragma unroll
-      for (int b = 0; b < D; ++b) t += lL[a][b] * G[i][b];
-      ad_atomic(&lnv[a * nn + S.node[i]], t);
-    }
-#pragma unroll
-    for (int b = 0; b < D; ++b) {
-      double t = 0.;
-#pragma unroll
-      for (int a = 0; a < D; ++a) t += lL[a][b] * v[i][a];
-      lG[b] = t;
-    }
-    fold_xi<D>(S, i, 0., lG, lxi);
-  }
+  for (int k = 0; k < D; ++k) lsc[k] = lV_out * V_in * dt;
+  strain_rate_vjp<D>(M, S, v, G, lde, lsc, dt, acc, lxi);
+  return deferred;
 }
 
-// Adjoint of compute_updated_position (particle.tcc:994-1070): scatters
-// dL/dv' and dL/da to the nodes, returns dL/dv_p (into lv), folds lxi.
+// Adjoint of compute_updated_position (particle.tcc:994-1070): adds dL/dv'
+// and dL/da of the cell's nodes to accvp and acca, returns dL/dv_p (in lv),
+// folds lxi.
 template <int D>
 MPMX_DEV void g2p_vjp(const Mesh& M, const Shape<D>& S, const AdjNodes& A, const AdjArgs& a,
-                      const double* lx, double* lv, double* lxi) {
+                      const double* lx, double* lv, double (*accvp)[D], double (*acca)[D],
+                      double* lxi) {
   constexpr int NN = Elem<D>::NN;
   const int64_t nn = M.nnodes;
   const bool flip = a.vu == FLIP;
@@ -400,12 +424,55 @@ MPMX_DEV void g2p_vjp(const Mesh& M, const Shape<D>& S, const AdjNodes& A, const
     double lN = 0.;
 #pragma unroll
     for (int k = 0; k < D; ++k) {
-      ad_atomic(&A.lvp[k * nn + n], N * lnv[k]);
-      if (flip) ad_atomic(&A.la[k * nn + n], N * lna[k]);
+      accvp[i][k] += N * lnv[k];
+      acca[i][k] += N * lna[k];
       lN += A.vp[k * nn + n] * lnv[k] + (flip ? A.a[k * nn + n] * lna[k] : 0.);
     }
     fold_xi<D>(S, i, lN, nullptr, lxi);
   }
+}
+
+// Sum per-node contributions over the lanes of a wave that hold particles of
+// the same cell (consecutive lanes with equal key: the cells' particles are
+// mostly consecutive in particle order) and add each sum to target with one
+// FP64 atomic per node and component. Every lane of the wave must call it;
+// lanes without a particle pass key < 0 and zero contributions.
+template <int D>
+MPMX_DEV void wave_flush(int key, const int* node, double (*acc)[D], double* target,
+                         int64_t nn) {
+  constexpr int NN = Elem<D>::NN;
+  const int lane = __lane_id();
+  const int prev = __shfl_up(key, 1);
+  const bool head = lane == 0 || prev != key;
+  const unsigned long long heads = __ballot(head);
+  const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+  const int start = 63 - __clzll(heads & upto);
+  const bool tail = lane == 63 || ((heads >> (lane + 1)) & 1ull);
+  for (int off = 1; off < 64; off <<= 1) {
+    const bool take = lane - start >= off;
+    if (!__ballot(take)) break;
+#pragma unroll
+    for (int i = 0; i < NN; ++i)
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+        const double y = __shfl_up(acc[i][k], off);
+        if (take) acc[i][k] += y;
+      }
+  }
+  if (tail && key >= 0) {
+#pragma unroll
+    for (int i = 0; i < NN; ++i)
+#pragma unroll
+      for (int k = 0; k < D; ++k) ad_atomic(&target[k * nn + node[i]], acc[i][k]);
+  }
+}
+
+template <int D>
+MPMX_DEV void zero_acc(double (*acc)[D]) {
+#pragma unroll
+  for (int i = 0; i < Elem<D>::NN; ++i)
+#pragma unroll
+    for (int k = 0; k < D; ++k) acc[i][k] = 0.;
 }
 
 // Adjoint of the nodal sums a particle contributes to (particle.tcc:630-658,
@@ -507,6 +574,7 @@ __global__ __launch_bounds__(kAdBlock) void k_ad_save(Particles P, SlotW s) {
   for (int k = 0; k < 6; ++k) s.stress[k * np + p] = P.stress[k * np + p];
   s.vol[p] = P.vol[p];
   s.rho[p] = P.rho[p];
+  if (s.yield) s.yield[p] = P.sv[SV_YIELD * np + p];
   s.cell[p] = P.cell[p];
 }
 
@@ -515,59 +583,10 @@ __global__ void k_ad_fill(double* a, int64_t n, double v) {
   if (i < n) a[i] = v;
 }
 
-// Nodal sums of step t from the recorded state: mass, momentum, force
-// (USL/MUSL: stress and volume of step t; USF: of step t+1, i.e. after the
-// stress update), MUSL: momentum of the updated particle velocities.
-template <int D, int SCH>
-__global__ __launch_bounds__(kAdBlock) void k_ad_p2g(Mesh M, Particles P, Slot s0, Slot s1,
-                                                     AdjNodes A, AdjArgs a) {
-  constexpr int NN = Elem<D>::NN;
-  const int64_t p = blockIdx.x * (int64_t)kAdBlock + threadIdx.x;
-  const int64_t np = P.np, nn = M.nnodes;
-  if (p >= np) return;
-  Shape<D> S;
-  if (!load_shape<D>(M, s0, np, p, S)) return;
-  const Slot& sf = SCH == USF ? s1 : s0;
-  const double m = P.mass[p];
-  double v[D], v1[D], st[6];
-#pragma unroll
-  for (int k = 0; k < D; ++k) {
-    v[k] = s0.v[k * np + p];
-    v1[k] = s1.v[k * np + p];
-  }
-#pragma unroll
-  for (int k = 0; k < 6; ++k) st[k] = sf.stress[k * np + p];
-  const double V = sf.vol[p];
-#pragma unroll
-  for (int i = 0; i < NN; ++i) {
-    const int n = S.node[i];
-    const double N = shapefn_i<D>(S.xi, i);
-    const double mN = m * N;
-    ad_atomic(&A.m[n], mN);
-    double Gi[D], f[D];
-    dndx_i<D>(S.xi, i, S.Ji, Gi);
-    if constexpr (D == 2) {
-      f[0] = Gi[0] * st[0] + Gi[1] * st[3];
-      f[1] = Gi[1] * st[1] + Gi[0] * st[3];
-    } else {
-      f[0] = Gi[0] * st[0] + Gi[1] * st[3] + Gi[2] * st[5];
-      f[1] = Gi[1] * st[1] + Gi[0] * st[3] + Gi[2] * st[4];
-      f[2] = Gi[2] * st[2] + Gi[1] * st[4] + Gi[0] * st[5];
-    }
-#pragma unroll
-    for (int k = 0; k < D; ++k) {
-      ad_atomic(&A.P[k * nn + n], mN * v[k]);
-      ad_atomic(&A.f[k * nn + n], a.g[k] * m * N + f[k] * (-1. * V));
-      if (SCH == MUSL) ad_atomic(&A.P2[k * nn + n], mN * v1[k]);
-    }
-  }
-}
-
-// Nodal velocity, acceleration and updated velocity of step t (node.tcc:
-// 236-252, 309-346); MUSL: the velocity of the second momentum.
-template <int D, int SCH>
-__global__ __launch_bounds__(kAdBlock) void k_ad_node_fwd(Mesh M, Nodes Nd, AdjNodes A,
-                                                          AdjArgs a) {
+// Pre-update nodal velocity of step t from its recorded mass and momentum
+// (node.tcc:236-252).
+template <int D>
+__global__ __launch_bounds__(kAdBlock) void k_ad_node_fwd(Mesh M, Nodes Nd, AdjNodes A) {
   const int64_t n = blockIdx.x * (int64_t)kAdBlock + threadIdx.x;
   const int64_t nn = M.nnodes;
   if (n >= nn) return;
@@ -575,29 +594,15 @@ __global__ __launch_bounds__(kAdBlock) void k_ad_node_fwd(Mesh M, Nodes Nd, AdjN
   if (!(mass > 0.)) return;
   const int ci = Nd.cons[n];
   const NodeCons* c = ci >= 0 ? &Nd.table[ci] : nullptr;
-  double v[D], acc[D], mom[D], f[D], zero[D];
+  double v[D], acc[D], mom[D];
 #pragma unroll
   for (int k = 0; k < D; ++k) {
-    v[k] = acc[k] = zero[k] = 0.;
+    v[k] = acc[k] = 0.;
     mom[k] = A.P[k * nn + n];
-    f[k] = A.f[k * nn + n];
   }
   node_compute_velocity<D>(mass, mom, v, acc, c);
 #pragma unroll
   for (int k = 0; k < D; ++k) A.v[k * nn + n] = v[k];
-  node_acc_vel<D>(mass, f, zero, v, acc, c, a.dt, a.damping);
-#pragma unroll
-  for (int k = 0; k < D; ++k) {
-    A.vp[k * nn + n] = v[k];
-    A.a[k * nn + n] = acc[k];
-  }
-  if (SCH == MUSL) {
-#pragma unroll
-    for (int k = 0; k < D; ++k) mom[k] = A.P2[k * nn + n];
-    node_compute_velocity<D>(mass, mom, v, acc, c);
-#pragma unroll
-    for (int k = 0; k < D; ++k) A.vpp[k * nn + n] = v[k];
-  }
 }
 
 // Node-side adjoints.
@@ -716,116 +721,138 @@ __global__ __launch_bounds__(kAdBlock) void k_ad_node(Mesh M, Nodes Nd, AdjNodes
 }
 
 // First particle pass of a backward step.
-//   USL : stress update with v' (scatter -> lvp), then G2P (-> lvp, la)
+//   USL : stress update with v' (-> lvp), then G2P (-> lvp, la)
 //   USF : G2P (-> lvp, la)
 //   MUSL: stress update with v'' (-> lvpp)
 template <int D, int SCH>
-__global__ __launch_bounds__(kAdBlock) void k_ad_b1(Mesh M, Particles P,
+__global__ __launch_bounds__(kAdBlock) MPMX_AD_ATTR void k_ad_b1(Mesh M, Particles P,
                                                                  const Material* mats, Slot s0,
-                                                                 AdjNodes A, AdjParticles L,
+                                                                 Slot s1, AdjNodes A,
+                                                                 AdjParticles L, AdjList Q,
                                                                  AdjArgs a) {
+  constexpr int NN = Elem<D>::NN;
   const int64_t p = blockIdx.x * (int64_t)kAdBlock + threadIdx.x;
   const int64_t np = P.np;
-  if (p >= np) return;
   Shape<D> S;
-  if (!load_shape<D>(M, s0, np, p, S)) return;
-  double lxi[D];
+  const bool valid = p < np && load_shape<D>(M, s0, np, p, S);
+  double acc0[NN][D], acc1[NN][D];
+  zero_acc<D>(acc0);
+  zero_acc<D>(acc1);
+  if (valid) {
+    double lxi[D];
 #pragma unroll
-  for (int k = 0; k < D; ++k) lxi[k] = 0.;
-  if (SCH == USL || SCH == MUSL) {
-    const Material mat = mats[P.mat[p]];
-    double s_in[6], ls_out[6], ls_in[6];
-    load_vec<D>(s0.stress, np, p, 6, s_in);
-    load_vec<D>(L.ls, np, p, 6, ls_out);
-    double lV_in = 0., gE = L.gE[p], gphi = L.gphi[p];
-    stress_vjp<D>(M, P, mat, p, S, SCH == USL ? A.vp : A.vpp, a.dt, a.apex_ratio, s_in,
-                  s0.vol[p], s0.rho[p], ls_out, L.lvol[p], ls_in, lV_in, gE, gphi,
-                  SCH == USL ? A.lvp : A.lvpp, lxi);
-    store_vec<D>(L.ls, np, p, 6, ls_in);
-    L.lvol[p] = lV_in;
-    L.gE[p] = gE;
-    L.gphi[p] = gphi;
+    for (int k = 0; k < D; ++k) lxi[k] = 0.;
+    if (SCH == USL || SCH == MUSL) {
+      const Material mat = mats[P.mat[p]];
+      double s_in[6], s_out[6], ls_out[6], ls_in[6];
+      load_vec<D>(s0.stress, np, p, 6, s_in);
+      load_vec<D>(s1.stress, np, p, 6, s_out);
+      load_vec<D>(L.ls, np, p, 6, ls_out);
+      double lV_in = 0., gE = L.gE[p];
+      const double ynew = s1.yield ? s1.yield[p] : 0.;
+      const bool deferred = stress_vjp<D>(M, mat, p, S, SCH == USL ? A.vp : A.vpp, a.dt,
+                                          a.apex_ratio, s_in, s0.vol[p], ynew, s_out, ls_out,
+                                          L.lvol[p], ls_in, lV_in, gE, acc0, lxi, Q);
+      if (!deferred) store_vec<D>(L.ls, np, p, 6, ls_in);
+      L.lvol[p] = lV_in;
+      L.gE[p] = gE;
+    }
+    if (SCH == USL || SCH == USF) {
+      double lx[D], lv[D];
+      load_vec<D>(L.lx, np, p, D, lx);
+      load_vec<D>(L.lv, np, p, D, lv);
+      g2p_vjp<D>(M, S, A, a, lx, lv, acc0, acc1, lxi);
+      store_vec<D>(L.lv, np, p, D, lv);
+    }
+    add_lxq<D>(L, np, p, S, lxi);
   }
-  if (SCH == USL || SCH == USF) {
-    double lx[D], lv[D];
-    load_vec<D>(L.lx, np, p, D, lx);
-    load_vec<D>(L.lv, np, p, D, lv);
-    g2p_vjp<D>(M, S, A, a, lx, lv, lxi);
-    store_vec<D>(L.lv, np, p, D, lv);
-  }
-  add_lxq<D>(L, np, p, S, lxi);
+  const int key = valid ? S.c : -1;
+  wave_flush<D>(key, S.node, acc0, SCH == MUSL ? A.lvpp : A.lvp, M.nnodes);
+  if (SCH != MUSL && a.vu == FLIP) wave_flush<D>(key, S.node, acc1, A.la, M.nnodes);
 }
 
 // Second particle pass.
 //   USL : forces of step t and P2G (gather lf, lm, lP); end of step
 //   USF : forces of step t+1 (gather lf), then the stress update with v
-//         (scatter -> lv)
-//   MUSL: second P2G (gather lm2, lP2), then G2P (scatter -> lvp, la)
+//         (-> lv)
+//   MUSL: second P2G (gather lm2, lP2), then G2P (-> lvp, la)
 template <int D, int SCH>
-__global__ __launch_bounds__(kAdBlock) void k_ad_b3(Mesh M, Particles P,
+__global__ __launch_bounds__(kAdBlock) MPMX_AD_ATTR void k_ad_b3(Mesh M, Particles P,
                                                                  const Material* mats, Slot s0,
                                                                  Slot s1, AdjNodes A,
-                                                                 AdjParticles L, AdjArgs a) {
+                                                                 AdjParticles L, AdjList Q,
+                                                                 AdjArgs a) {
+  constexpr int NN = Elem<D>::NN;
   const int64_t p = blockIdx.x * (int64_t)kAdBlock + threadIdx.x;
   const int64_t np = P.np;
-  if (p >= np) return;
   Shape<D> S;
-  if (!load_shape<D>(M, s0, np, p, S)) return;
-  double lxi[D];
+  const bool valid = p < np && load_shape<D>(M, s0, np, p, S);
+  double acc0[NN][D], acc1[NN][D];
+  zero_acc<D>(acc0);
+  zero_acc<D>(acc1);
+  if (valid) {
+    double lxi[D];
 #pragma unroll
-  for (int k = 0; k < D; ++k) lxi[k] = 0.;
-  const double mp = P.mass[p];
-  double lm_p = 0.;
-  double lv[D];
-  load_vec<D>(L.lv, np, p, D, lv);
-  if (SCH == USL) {
-    double v0[D], st[6], ls[6];
-    load_vec<D>(s0.v, np, p, D, v0);
-    load_vec<D>(s0.stress, np, p, 6, st);
-    load_vec<D>(L.ls, np, p, 6, ls);
-    double lV = L.lvol[p];
-    gather_vjp<D>(M, S, A.lm, A.lP, A.lf, mp, v0, a.g, st, s0.vol[p], lm_p, lv, ls, lV, lxi);
-    store_vec<D>(L.ls, np, p, 6, ls);
-    L.lvol[p] = lV;
-  } else if (SCH == USF) {
-    // forces of the updated stress and volume
-    double st1[6], ls1[6];
-    load_vec<D>(s1.stress, np, p, 6, st1);
-    load_vec<D>(L.ls, np, p, 6, ls1);
-    double lV1 = L.lvol[p];
-    gather_vjp<D>(M, S, nullptr, nullptr, A.lf, mp, nullptr, a.g, st1, s1.vol[p], lm_p, lv, ls1,
-                  lV1, lxi);
-    // stress update from the pre-update nodal velocity
-    const Material mat = mats[P.mat[p]];
-    double s_in[6], ls_in[6];
-    load_vec<D>(s0.stress, np, p, 6, s_in);
-    double lV_in = 0., gE = L.gE[p], gphi = L.gphi[p];
-    stress_vjp<D>(M, P, mat, p, S, A.v, a.dt, a.apex_ratio, s_in, s0.vol[p], s0.rho[p], ls1,
-                  lV1, ls_in, lV_in, gE, gphi, A.lv, lxi);
-    store_vec<D>(L.ls, np, p, 6, ls_in);
-    L.lvol[p] = lV_in;
-    L.gE[p] = gE;
-    L.gphi[p] = gphi;
-  } else {
-    // second P2G with the updated particle velocity, then G2P
-    double v1[D], dummy_s[6] = {0, 0, 0, 0, 0, 0}, dummy_ls[6] = {0, 0, 0, 0, 0, 0};
-    double dummy_lV = 0.;
-    load_vec<D>(s1.v, np, p, D, v1);
-    gather_vjp<D>(M, S, A.lm2, A.lP2, nullptr, mp, v1, a.g, dummy_s, 0., lm_p, lv, dummy_ls,
-                  dummy_lV, lxi);
-    double lx[D];
-    load_vec<D>(L.lx, np, p, D, lx);
-    g2p_vjp<D>(M, S, A, a, lx, lv, lxi);
-  }
-  store_vec<D>(L.lv, np, p, D, lv);
-  L.gm[p] += lm_p;
-  add_lxq<D>(L, np, p, S, lxi);
-  if (SCH == USL) {
-#pragma unroll
-    for (int k = 0; k < D; ++k) {
-      L.lx[k * np + p] += L.lxq[k * np + p];
-      L.lxq[k * np + p] = 0.;
+    for (int k = 0; k < D; ++k) lxi[k] = 0.;
+    const double mp = P.mass[p];
+    double lm_p = 0.;
+    double lv[D];
+    load_vec<D>(L.lv, np, p, D, lv);
+    if (SCH == USL) {
+      double v0[D], st[6], ls[6];
+      load_vec<D>(s0.v, np, p, D, v0);
+      load_vec<D>(s0.stress, np, p, 6, st);
+      load_vec<D>(L.ls, np, p, 6, ls);
+      double lV = L.lvol[p];
+      gather_vjp<D>(M, S, A.lm, A.lP, A.lf, mp, v0, a.g, st, s0.vol[p], lm_p, lv, ls, lV, lxi);
+      store_vec<D>(L.ls, np, p, 6, ls);
+      L.lvol[p] = lV;
+    } else if (SCH == USF) {
+      // forces of the updated stress and volume
+      double st1[6], ls1[6];
+      load_vec<D>(s1.stress, np, p, 6, st1);
+      load_vec<D>(L.ls, np, p, 6, ls1);
+      double lV1 = L.lvol[p];
+      gather_vjp<D>(M, S, nullptr, nullptr, A.lf, mp, nullptr, a.g, st1, s1.vol[p], lm_p, lv,
+                    ls1, lV1, lxi);
+      // stress update from the pre-update nodal velocity
+      const Material mat = mats[P.mat[p]];
+      double s_in[6], ls_in[6];
+      load_vec<D>(s0.stress, np, p, 6, s_in);
+      double lV_in = 0., gE = L.gE[p];
+      const double ynew = s1.yield ? s1.yield[p] : 0.;
+      const bool deferred = stress_vjp<D>(M, mat, p, S, A.v, a.dt, a.apex_ratio, s_in,
+                                          s0.vol[p], ynew, st1, ls1, lV1, ls_in, lV_in, gE, acc0,
+                                          lxi, Q);
+      store_vec<D>(L.ls, np, p, 6, deferred ? ls1 : ls_in);
+      L.lvol[p] = lV_in;
+      L.gE[p] = gE;
+    } else {
+      // second P2G with the updated particle velocity, then G2P
+      double v1[D], dummy_s[6] = {0, 0, 0, 0, 0, 0}, dummy_ls[6] = {0, 0, 0, 0, 0, 0};
+      double dummy_lV = 0.;
+      load_vec<D>(s1.v, np, p, D, v1);
+      gather_vjp<D>(M, S, A.lm2, A.lP2, nullptr, mp, v1, a.g, dummy_s, 0., lm_p, lv, dummy_ls,
+                    dummy_lV, lxi);
+      double lx[D];
+      load_vec<D>(L.lx, np, p, D, lx);
+      g2p_vjp<D>(M, S, A, a, lx, lv, acc0, acc1, lxi);
     }
+    store_vec<D>(L.lv, np, p, D, lv);
+    L.gm[p] += lm_p;
+    add_lxq<D>(L, np, p, S, lxi);
+    if (SCH == USL) {
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+        L.lx[k * np + p] += L.lxq[k * np + p];
+        L.lxq[k * np + p] = 0.;
+      }
+    }
+  }
+  if (SCH == USF || SCH == MUSL) {
+    const int key = valid ? S.c : -1;
+    wave_flush<D>(key, S.node, acc0, SCH == USF ? A.lv : A.lvp, M.nnodes);
+    if (SCH == MUSL && a.vu == FLIP) wave_flush<D>(key, S.node, acc1, A.la, M.nnodes);
   }
 }
 
@@ -864,6 +891,72 @@ __global__ __launch_bounds__(kAdBlock) void k_ad_b5(Mesh M, Particles P, Slot s0
   }
 }
 
+// The deferred plastic Mohr-Coulomb updates: constitutive adjoint by
+// forward-mode passes (stress adjoint and parameter gradients written; the
+// strain-increment adjoint left in Q.buf).
+template <int D>
+__global__ __launch_bounds__(kAdBlock) MPMX_AD_ATTR void k_ad_mc(Particles P,
+                                                                 const Material* mats, Slot s0,
+                                                                 AdjParticles L, AdjList Q) {
+  const int64_t slot = blockIdx.x * (int64_t)kAdBlock + threadIdx.x;
+  if (slot >= *Q.count) return;
+  const int64_t np = P.np;
+  const int p = Q.ids[slot];
+  const Material m = mats[P.mat[p]];
+  double s_in[6], de[6], ls_out[6], ls_in[6], lde[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    s_in[k] = s0.stress[k * np + p];
+    de[k] = Q.buf[k * Q.cap + slot];
+    ls_out[k] = L.ls[k * np + p];
+    ls_in[k] = lde[k] = 0.;
+  }
+  const double phi = P.sv[SV_PHI * np + p], psi = P.sv[SV_PSI * np + p];
+  const double coh = P.sv[SV_COH * np + p], tcut = P.sv[SV_TCUT * np + p];
+  double gE = 0., gphi = 0.;
+  mc_vjp<D, MPMX_AD_TANGENTS>(m, phi, psi, coh, tcut, s_in, de, ls_out, ls_in, lde, gE, gphi,
+                              Q.apex[slot] != 0);
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    L.ls[k * np + p] = ls_in[k];
+    Q.buf[k * Q.cap + slot] = lde[k];
+  }
+  L.gE[p] += gE;
+  L.gphi[p] += gphi;
+}
+
+// Their strain-increment adjoints through the velocity gradient: scatter
+// into lnv, shape adjoint into lxq.
+template <int D>
+__global__ __launch_bounds__(kAdBlock) void k_ad_mc_tail(Mesh M, Particles P, Slot s0,
+                                                         const double* nv, double* lnv,
+                                                         AdjParticles L, AdjList Q,
+                                                         AdjArgs a) {
+  constexpr int NN = Elem<D>::NN;
+  const int64_t slot = blockIdx.x * (int64_t)kAdBlock + threadIdx.x;
+  if (slot >= *Q.count) return;
+  const int64_t np = P.np;
+  const int p = Q.ids[slot];
+  Shape<D> S;
+  if (!load_shape<D>(M, s0, np, p, S)) return;
+  double v[NN][D], G[NN][D], lde[6], lxi[D];
+  load_kinematics<D>(M, S, nv, v, G);
+#pragma unroll
+  for (int k = 0; k < 6; ++k) lde[k] = Q.buf[k * Q.cap + slot];
+#pragma unroll
+  for (int k = 0; k < D; ++k) lxi[k] = 0.;
+  double acc[NN][D];
+  zero_acc<D>(acc);
+  strain_rate_vjp<D>(M, S, v, G, lde, nullptr, a.dt, acc, lxi);
+  add_lxq<D>(L, np, p, S, lxi);
+#pragma unroll
+  for (int i = 0; i < NN; ++i)
+#pragma unroll
+    for (int k = 0; k < D; ++k) ad_atomic(&lnv[k * M.nnodes + S.node[i]], acc[i][k]);
+}
+
+__global__ void k_ad_zero_int(int* a) { *a = 0; }
+
 inline dim3 ad_grid(int64_t n) { return dim3((unsigned)((n + kAdBlock - 1) / kAdBlock)); }
 
 }  // namespace
@@ -880,23 +973,31 @@ void launch_ad_fill(double* a, int64_t n, double v, hipStream_t st) {
 template <int D, int SCH>
 static void backward_scheme(const Mesh& M, const Particles& P, const Nodes& Nd,
                             const Material* mats, const Slot& s0, const Slot& s1,
-                            const AdjNodes& A, const AdjParticles& L, const AdjArgs& a,
-                            hipStream_t st) {
+                            const AdjNodes& A, const AdjParticles& L, const AdjList& Q,
+                            const AdjArgs& a, hipStream_t st) {
   const dim3 gp = ad_grid(P.np), gn = ad_grid(M.nnodes), b(kAdBlock);
-  hipLaunchKernelGGL((k_ad_p2g<D, SCH>), gp, b, 0, st, M, P, s0, s1, A, a);
-  hipLaunchKernelGGL((k_ad_node_fwd<D, SCH>), gn, b, 0, st, M, Nd, A, a);
-  hipLaunchKernelGGL((k_ad_b1<D, SCH>), gp, b, 0, st, M, P, mats, s0, A, L, a);
+  // the plastic Mohr-Coulomb updates queued by the stress pass
+  auto plastic = [&](const double* nv, double* lnv) {
+    hipLaunchKernelGGL((k_ad_mc<D>), gp, b, 0, st, P, mats, s0, L, Q);
+    hipLaunchKernelGGL((k_ad_mc_tail<D>), gp, b, 0, st, M, P, s0, nv, lnv, L, Q, a);
+  };
+  hipLaunchKernelGGL(k_ad_zero_int, dim3(1), dim3(1), 0, st, Q.count);
+  hipLaunchKernelGGL((k_ad_node_fwd<D>), gn, b, 0, st, M, Nd, A);
+  hipLaunchKernelGGL((k_ad_b1<D, SCH>), gp, b, 0, st, M, P, mats, s0, s1, A, L, Q, a);
   if (SCH == USL) {
+    plastic(A.vp, A.lvp);
     hipLaunchKernelGGL((k_ad_node<D, ND_AD_UPD | ND_AD_VEL>), gn, b, 0, st, M, Nd, A, a);
-    hipLaunchKernelGGL((k_ad_b3<D, SCH>), gp, b, 0, st, M, P, mats, s0, s1, A, L, a);
+    hipLaunchKernelGGL((k_ad_b3<D, SCH>), gp, b, 0, st, M, P, mats, s0, s1, A, L, Q, a);
   } else if (SCH == USF) {
     hipLaunchKernelGGL((k_ad_node<D, ND_AD_UPD>), gn, b, 0, st, M, Nd, A, a);
-    hipLaunchKernelGGL((k_ad_b3<D, SCH>), gp, b, 0, st, M, P, mats, s0, s1, A, L, a);
+    hipLaunchKernelGGL((k_ad_b3<D, SCH>), gp, b, 0, st, M, P, mats, s0, s1, A, L, Q, a);
+    plastic(A.v, A.lv);
     hipLaunchKernelGGL((k_ad_node<D, ND_AD_VEL>), gn, b, 0, st, M, Nd, A, a);
     hipLaunchKernelGGL((k_ad_b5<D, SCH>), gp, b, 0, st, M, P, s0, A, L, a);
   } else {
+    plastic(A.vpp, A.lvpp);
     hipLaunchKernelGGL((k_ad_node<D, ND_AD_VEL2>), gn, b, 0, st, M, Nd, A, a);
-    hipLaunchKernelGGL((k_ad_b3<D, SCH>), gp, b, 0, st, M, P, mats, s0, s1, A, L, a);
+    hipLaunchKernelGGL((k_ad_b3<D, SCH>), gp, b, 0, st, M, P, mats, s0, s1, A, L, Q, a);
     hipLaunchKernelGGL((k_ad_node<D, ND_AD_UPD | ND_AD_VEL>), gn, b, 0, st, M, Nd, A, a);
     hipLaunchKernelGGL((k_ad_b5<D, SCH>), gp, b, 0, st, M, P, s0, A, L, a);
   }
@@ -905,26 +1006,26 @@ static void backward_scheme(const Mesh& M, const Particles& P, const Nodes& Nd,
 template <int D>
 void launch_ad_backward_step(int scheme, const Mesh& M, const Particles& P, const Nodes& Nd,
                              const Material* mats, const Slot& s0, const Slot& s1,
-                             const AdjNodes& A, const AdjParticles& L, const AdjArgs& a,
-                             hipStream_t st) {
+                             const AdjNodes& A, const AdjParticles& L, const AdjList& Q,
+                             const AdjArgs& a, hipStream_t st) {
   if (P.np == 0) return;
   if (scheme == USL)
-    backward_scheme<D, USL>(M, P, Nd, mats, s0, s1, A, L, a, st);
+    backward_scheme<D, USL>(M, P, Nd, mats, s0, s1, A, L, Q, a, st);
   else if (scheme == USF)
-    backward_scheme<D, USF>(M, P, Nd, mats, s0, s1, A, L, a, st);
+    backward_scheme<D, USF>(M, P, Nd, mats, s0, s1, A, L, Q, a, st);
   else
-    backward_scheme<D, MUSL>(M, P, Nd, mats, s0, s1, A, L, a, st);
+    backward_scheme<D, MUSL>(M, P, Nd, mats, s0, s1, A, L, Q, a, st);
 }
 
 template void launch_ad_save<2>(const Particles&, const SlotW&, hipStream_t);
 template void launch_ad_save<3>(const Particles&, const SlotW&, hipStream_t);
 template void launch_ad_backward_step<2>(int, const Mesh&, const Particles&, const Nodes&,
                                          const Material*, const Slot&, const Slot&,
-                                         const AdjNodes&, const AdjParticles&, const AdjArgs&,
-                                         hipStream_t);
+                                         const AdjNodes&, const AdjParticles&, const AdjList&,
+                                         const AdjArgs&, hipStream_t);
 template void launch_ad_backward_step<3>(int, const Mesh&, const Particles&, const Nodes&,
                                          const Material*, const Slot&, const Slot&,
-                                         const AdjNodes&, const AdjParticles&, const AdjArgs&,
-                                         hipStream_t);
+                                         const AdjNodes&, const AdjParticles&, const AdjList&,
+                                         const AdjArgs&, hipStream_t);
 
 }  // namespace mpmx

@@ -132,7 +132,8 @@ struct mpmx_ctx {
   int64_t since_sort = 0;
 
   // graph cache: [0] step with key refresh, [1] step with a full sort
-  hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
+  // [2], [3]: the same steps recording nodal quantities into d_ad_rec
+  hipGraphExec_t graph_exec[4] = {nullptr, nullptr, nullptr, nullptr};
   double graph_dt = 0, graph_damping = 0;
 
   // adjoint: checkpoints of a recorded run (full particle state every
@@ -144,8 +145,9 @@ struct mpmx_ctx {
   double ad_apex_ratio = 0.3;
   int64_t ad_step0 = 0, ad_np = 0;
   int64_t ad_ck_stride = 0;  // doubles per checkpoint
-  DBuf<double> d_ad_ck, d_ad_slots, d_ad_nodes, d_ad_parts;
-  DBuf<int> d_ad_ck_cell, d_ad_slot_cell;
+  DBuf<double> d_ad_ck, d_ad_slots, d_ad_node_slots, d_ad_rec, d_ad_nodes, d_ad_parts;
+  DBuf<int> d_ad_ck_cell, d_ad_slot_cell, d_ad_list;
+  DBuf<double> d_ad_list_buf;
 
   // profiling
   bool profile = false;
@@ -761,15 +763,19 @@ bool next_step_sorts(mpmx_ctx* c) {
 
 
 // Queue nsteps steps on the context's stream (one graph replay each).
-void launch_steps(mpmx_ctx* c, int nsteps, double dt, double damping) {
+// ad_nodes: record each step's nodal quantities there (direct launches).
+void launch_steps(mpmx_ctx* c, int nsteps, double dt, double damping,
+                  double* ad_nodes = nullptr) {
   StepArgs A;
   A.dt = dt;
   A.damping = damping;
   A.blending = c->blending;
   for (int k = 0; k < 3; ++k) A.g[k] = c->g[k];
   A.ramp = c->ramp;
+  A.ad_nodes = ad_nodes;
   Launcher L{c, {}};
   const bool use_graph = !(c->flags & MPMX_NO_GRAPH) && !c->profile;
+  const int rec = ad_nodes ? 2 : 0;
   if (use_graph) {
     if (c->graph_dt != dt || c->graph_damping != damping) {
       c->invalidate_graph();
@@ -777,11 +783,11 @@ void launch_steps(mpmx_ctx* c, int nsteps, double dt, double damping) {
       c->graph_damping = damping;
     }
     for (int s = 0; s < nsteps; ++s) {
-      const int kind = next_step_sorts(c) ? 1 : 0;
+      const int kind = (next_step_sorts(c) ? 1 : 0) + rec;
       if (!c->graph_exec[kind]) {
         hipGraph_t graph;
         HIPCHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        enqueue(c, A, L, kind == 1);
+        enqueue(c, A, L, (kind & 1) == 1);
         HIPCHECK(hipStreamEndCapture(c->stream, &graph));
         hipError_t e = hipGraphInstantiate(&c->graph_exec[kind], graph, nullptr, nullptr, 0);
         (void)hipGraphDestroy(graph);
@@ -843,7 +849,7 @@ void ad_checkpoint(mpmx_ctx* c, int idx, bool save) {
     HIPCHECK(hipMemcpyAsync(c->d_cell.p, cb, c->np * sizeof(int), hipMemcpyDeviceToDevice, c->stream));
 }
 
-int64_t ad_slot_doubles(const mpmx_ctx* c) { return (2 * c->dim + 8) * c->np; }
+int64_t ad_slot_doubles(const mpmx_ctx* c) { return (2 * c->dim + 9) * c->np; }
 
 Slot ad_slot(mpmx_ctx* c, int j) {
   const int64_t np = c->np, d = c->dim;
@@ -854,14 +860,17 @@ Slot ad_slot(mpmx_ctx* c, int j) {
   s.vol = b + (d + 6) * np;
   s.rho = b + (d + 7) * np;
   s.xi = b + (d + 8) * np;
+  s.yield = c->any_mc ? b + (2 * d + 8) * np : nullptr;
   s.cell = c->d_ad_slot_cell.p + (int64_t)j * np;
   return s;
 }
 
 void ad_save_slot(mpmx_ctx* c, int j) {
   const Slot r = ad_slot(c, j);
-  SlotW w{const_cast<double*>(r.v),   const_cast<double*>(r.stress), const_cast<double*>(r.vol),
-          const_cast<double*>(r.rho), const_cast<double*>(r.xi),     const_cast<int*>(r.cell)};
+  SlotW w{const_cast<double*>(r.v),   const_cast<double*>(r.stress),
+          const_cast<double*>(r.vol), const_cast<double*>(r.rho),
+          const_cast<double*>(r.xi),  const_cast<double*>(r.yield),
+          const_cast<int*>(r.cell)};
   if (c->dim == 2)
     launch_ad_save<2>(c->parts(), w, c->stream);
   else
@@ -869,23 +878,38 @@ void ad_save_slot(mpmx_ctx* c, int j) {
   HIPCHECK(hipGetLastError());
 }
 
-AdjNodes ad_nodes(mpmx_ctx* c) {
+// Nodal quantities recorded by the replayed step j (StepArgs::ad_nodes
+// layout): mass, then momentum, force, updated velocity, acceleration, MUSL
+// second momentum and its velocity.
+int64_t ad_node_slot_doubles(const mpmx_ctx* c) { return (1 + 6 * c->dim) * c->nnodes; }
+double* ad_node_slot(mpmx_ctx* c, int j) {
+  return c->d_ad_node_slots.p + (int64_t)j * ad_node_slot_doubles(c);
+}
+
+AdjNodes ad_nodes(mpmx_ctx* c, int j) {
   const int64_t nn = c->nnodes, d = c->dim;
-  double* b = c->d_ad_nodes.p;
+  double* r = ad_node_slot(c, j);
   AdjNodes A;
-  double** vec[] = {&A.P, &A.f, &A.v, &A.a, &A.vp, &A.P2, &A.vpp,
-                    &A.lvp, &A.la, &A.lv, &A.lvpp, &A.lf, &A.lP, &A.lP2};
-  A.m = b;
-  A.lm = b + nn;
-  A.lm2 = b + 2 * nn;
-  int64_t off = 3 * nn;
+  A.m = r;
+  A.P = r + nn;
+  A.f = r + (1 + d) * nn;
+  A.vp = r + (1 + 2 * d) * nn;
+  A.a = r + (1 + 3 * d) * nn;
+  A.P2 = r + (1 + 4 * d) * nn;
+  A.vpp = r + (1 + 5 * d) * nn;
+  // this step's pre-update velocity and the adjoints
+  double* b = c->d_ad_nodes.p;
+  double** vec[] = {&A.v, &A.lvp, &A.la, &A.lv, &A.lvpp, &A.lf, &A.lP, &A.lP2};
+  A.lm = b;
+  A.lm2 = b + nn;
+  int64_t off = 2 * nn;
   for (auto* f : vec) {
     *f = b + off;
     off += d * nn;
   }
   return A;
 }
-int64_t ad_node_doubles(const mpmx_ctx* c) { return (3 + 14 * c->dim) * c->nnodes; }
+int64_t ad_node_doubles(const mpmx_ctx* c) { return (2 + 8 * c->dim) * c->nnodes; }
 
 AdjParticles ad_parts(mpmx_ctx* c) {
   const int64_t np = c->np, d = c->dim;
@@ -953,12 +977,18 @@ void ad_backward_step(mpmx_ctx* c, int j, int64_t t) {
   for (int k = 0; k < 3; ++k) a.g[k] = gm * c->g[k];
   launch_ad_fill(c->d_ad_nodes.p, ad_node_doubles(c), 0., c->stream);
   const Slot s0 = ad_slot(c, j), s1 = ad_slot(c, j + 1);
+  AdjList Q;
+  Q.cap = c->np;
+  Q.count = c->d_ad_list.p;
+  Q.ids = c->d_ad_list.p + 1;
+  Q.apex = c->d_ad_list.p + 1 + c->np;
+  Q.buf = c->d_ad_list_buf.p;
   if (c->dim == 2)
     launch_ad_backward_step<2>(c->scheme, c->mesh(), c->parts(), c->nodes(), c->d_mats.p, s0, s1,
-                               ad_nodes(c), ad_parts(c), a, c->stream);
+                               ad_nodes(c, j), ad_parts(c), Q, a, c->stream);
   else
     launch_ad_backward_step<3>(c->scheme, c->mesh(), c->parts(), c->nodes(), c->d_mats.p, s0, s1,
-                               ad_nodes(c), ad_parts(c), a, c->stream);
+                               ad_nodes(c, j), ad_parts(c), Q, a, c->stream);
   HIPCHECK(hipGetLastError());
 }
 
@@ -1561,8 +1591,19 @@ int mpmx_adjoint_backward(mpmx_ctx* c, const double* dL_dx, const double* dL_dv,
   hipStream_t st = c->stream;
   c->d_ad_slots.alloc((size_t)(K + 1) * ad_slot_doubles(c));
   c->d_ad_slot_cell.alloc((size_t)(K + 1) * np);
+  c->d_ad_node_slots.alloc((size_t)K * ad_node_slot_doubles(c));
+  if (c->d_ad_rec.n != (size_t)ad_node_slot_doubles(c)) {
+    c->d_ad_rec.alloc(ad_node_slot_doubles(c));
+    for (int k = 2; k < 4; ++k)
+      if (c->graph_exec[k]) {
+        (void)hipGraphExecDestroy(c->graph_exec[k]);
+        c->graph_exec[k] = nullptr;
+      }
+  }
   c->d_ad_nodes.alloc(ad_node_doubles(c));
   c->d_ad_parts.alloc(ad_part_doubles(c));
+  c->d_ad_list.alloc(1 + 2 * (size_t)np);
+  c->d_ad_list_buf.alloc(6 * (size_t)np);
   launch_ad_fill(c->d_ad_parts.p, ad_part_doubles(c), 0., st);
   const AdjParticles L = ad_parts(c);
   auto put_rows = [&](double* dev, int comps, const double* rows) {
@@ -1597,7 +1638,10 @@ int mpmx_adjoint_backward(mpmx_ctx* c, const double* dL_dx, const double* dL_dv,
     reset_status(c);
     for (int j = 0; j < ns; ++j) {
       ad_save_slot(c, j);
-      launch_steps(c, 1, c->ad_dt, c->ad_damping);
+      launch_steps(c, 1, c->ad_dt, c->ad_damping, c->d_ad_rec.p);
+      HIPCHECK(hipMemcpyAsync(ad_node_slot(c, j), c->d_ad_rec.p,
+                              ad_node_slot_doubles(c) * sizeof(double), hipMemcpyDeviceToDevice,
+                              st));
     }
     ad_save_slot(c, ns);
     check_status(c);
@@ -1642,8 +1686,17 @@ int mpmx_adjoint_release(mpmx_ctx* c) {
   c->d_ad_ck_cell.release();
   c->d_ad_slots.release();
   c->d_ad_slot_cell.release();
+  c->d_ad_node_slots.release();
+  c->d_ad_rec.release();
+  for (int k = 2; k < 4; ++k)
+    if (c->graph_exec[k]) {
+      (void)hipGraphExecDestroy(c->graph_exec[k]);
+      c->graph_exec[k] = nullptr;
+    }
   c->d_ad_nodes.release();
   c->d_ad_parts.release();
+  c->d_ad_list.release();
+  c->d_ad_list_buf.release();
   API_END(c)
 }
 

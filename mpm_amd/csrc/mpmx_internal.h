@@ -94,6 +94,11 @@ struct StepArgs {
   double blending;
   double g[3];
   double ramp;  // gravity ramping time (<= 0: none)
+  // Adjoint recording: when set, the node update kernels also store the
+  // step's nodal quantities here (layout: mass [nn], then momentum, force,
+  // updated velocity, acceleration, MUSL second momentum and its velocity,
+  // [dim][nn] each). Null in ordinary steps.
+  double* ad_nodes;
 };
 
 // Device-side status written by kernels and read after a run of steps.

@@ -782,6 +782,35 @@ __global__ __launch_bounds__(kBlock) void k_g2p_scatter(Mesh M, Particles P, Nod
 }
 
 // ------------------------------- node kernels -------------------------------
+// Adjoint recording (StepArgs::ad_nodes): the nodal quantities of this step.
+// FIRST: after the update (mass, momentum, external + internal force,
+// updated velocity, acceleration); SECOND: MUSL's second momentum and its
+// velocity.
+template <int D>
+MPMX_DEV void ad_record_first(const StepArgs& A, int64_t n, int64_t nn, double mass,
+                              const double* mom, const double* fe, const double* fi,
+                              const double* v, const double* a) {
+  double* b = A.ad_nodes;
+  b[n] = mass;
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    b[(1 + 0 * D + k) * nn + n] = mom[k];
+    b[(1 + 1 * D + k) * nn + n] = fe[k] + fi[k];
+    b[(1 + 2 * D + k) * nn + n] = v[k];
+    b[(1 + 3 * D + k) * nn + n] = a[k];
+  }
+}
+template <int D>
+MPMX_DEV void ad_record_second(const StepArgs& A, int64_t n, int64_t nn, const double* mom,
+                               const double* v) {
+  double* b = A.ad_nodes;
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    b[(1 + 4 * D + k) * nn + n] = mom[k];
+    b[(1 + 5 * D + k) * nn + n] = v[k];
+  }
+}
+
 // Flags for what a node kernel does.
 // ND_INIT: start from v = a = 0 (Node::initialise, node.tcc:26-43) instead of
 // the stored values.
@@ -794,7 +823,10 @@ __global__ __launch_bounds__(kBlock) void k_node(Mesh M, Nodes Nd, StepArgs A) {
   const int64_t n = blockIdx.x * (int64_t)kBlock + threadIdx.x;
   if (n >= M.nnodes) return;
   const int64_t nn = M.nnodes;
-  if (!Nd.active[n]) return;
+  if (!Nd.active[n]) {
+    if ((FL & ND_ACC) != 0 && A.ad_nodes) A.ad_nodes[n] = 0.;
+    return;
+  }
   const int ci = Nd.cons[n];
   const NodeCons* c = ci >= 0 ? &Nd.table[ci] : nullptr;
   const double mass = Nd.mass[n];
@@ -818,6 +850,20 @@ __global__ __launch_bounds__(kBlock) void k_node(Mesh M, Nodes Nd, StepArgs A) {
       fi[k] = Nd.fint[k * nn + n];
     }
     node_acc_vel<D>(mass, fe, fi, v, a, c, A.dt, A.damping);
+    if (A.ad_nodes) {
+      double mom[D];
+#pragma unroll
+      for (int k = 0; k < D; ++k) mom[k] = Nd.mom[k * nn + n];
+      ad_record_first<D>(A, n, nn, mass, mom, fe, fi, v, a);
+    }
+  }
+  if constexpr ((FL & (ND_VEL | ND_ACC | ND_INIT)) == ND_VEL) {
+    if (A.ad_nodes) {
+      double mom[D];
+#pragma unroll
+      for (int k = 0; k < D; ++k) mom[k] = Nd.mom[k * nn + n];
+      ad_record_second<D>(A, n, nn, mom, v);
+    }
   }
 #pragma unroll
   for (int k = 0; k < D; ++k) {
@@ -888,7 +934,10 @@ __global__ __launch_bounds__(kBlock) void k_gather(Mesh M, Particles P, Nodes Nd
     }
   }
   if ((WHAT & SC_MV) != 0) Nd.active[n] = any ? 1 : 0;
-  if (!any) return;
+  if (!any) {
+    if ((FL & ND_ACC) != 0 && A.ad_nodes) A.ad_nodes[n] = 0.;
+    return;
+  }
   double g[3] = {0, 0, 0};
   if constexpr ((WHAT & SC_F) != 0) gravity_now(A, S, D, g);
   double am = 0., amom[D], afe[D], afi[D];
@@ -979,6 +1028,20 @@ __global__ __launch_bounds__(kBlock) void k_gather(Mesh M, Particles P, Nodes Nd
       fi[k] = Nd.fint[k * nnodes + n];
     }
     node_acc_vel<D>(mass, fe, fi, v, a, cc, A.dt, A.damping);
+    if (A.ad_nodes) {
+      double mom[D];
+#pragma unroll
+      for (int k = 0; k < D; ++k) mom[k] = Nd.mom[k * nnodes + n];
+      ad_record_first<D>(A, n, nnodes, mass, mom, fe, fi, v, a);
+    }
+  }
+  if constexpr ((FL & (ND_VEL | ND_ACC | ND_INIT)) == ND_VEL) {
+    if (A.ad_nodes) {
+      double mom[D];
+#pragma unroll
+      for (int k = 0; k < D; ++k) mom[k] = Nd.mom[k * nnodes + n];
+      ad_record_second<D>(A, n, nnodes, mom, v);
+    }
   }
 #pragma unroll
   for (int k = 0; k < D; ++k) {
