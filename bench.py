@@ -237,7 +237,7 @@ def main():
                     help="timed steps (default 200; adjoint_2d: its 500-step run)")
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="collapse_3d",
-                    choices=["collapse_3d", "column_2d", "batch_2d", "adjoint_2d"])
+                    choices=["collapse_3d", "wedge_3d", "column_2d", "batch_2d", "adjoint_2d"])
     ap.add_argument("--sequential-steps", type=int, default=20,
                     help="batch_2d: steps per instance of the one-instance-at-a-time comparison")
     ap.add_argument("--ordered", action="store_true", help="particle-order nodal sums")

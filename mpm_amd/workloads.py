@@ -69,7 +69,8 @@ class Workload:
     fric_mu: np.ndarray = field(default_factory=lambda: np.zeros(0))
     velocity_seed: Optional[int] = None
     velocity_amplitude: float = 0.0
-    # K0 geostatic initial stress: (K0, unit weight rho*|g|, free-surface height)
+    # K0 geostatic initial stress: (K0, unit weight rho*|g|, free-surface
+    # height: a number, or a function of the particle positions)
     geostatic: Optional[tuple] = None
     damping: float = 0.0
     info: dict = field(default_factory=dict)
@@ -87,7 +88,10 @@ class Workload:
         solver.generate_gauss(self.nppd, self.cell_set, 0)
         if self.geostatic is not None:
             k0, gamma, top = self.geostatic
-            z = solver.particles(["x"])["x"][:, self.dim - 1]
+            x = solver.particles(["x"])["x"]
+            z = x[:, self.dim - 1]
+            if callable(top):  # free-surface height as a function of the positions
+                top = top(x)
             sv = -gamma * (top - z)
             stress = np.zeros((len(z), 6))
             stress[:, self.dim - 1] = sv
@@ -300,3 +304,41 @@ def adjoint_2d(ninst=64, seed=5, h=0.01, columns=64, dt=1e-5, nsteps=500, checkp
                   material="LinearElastic (even) / MohrCoulomb (odd) per instance",
                   nsteps=nsteps, checkpoint_every=checkpoint_every)
     return b
+
+
+def wedge_3d(nx=256, ny=128, nz=128, h=0.02, crest=1.28, nsteps=1000, dt=2e-5):
+    """Large 3D slope runout: hex8 nx x ny x nz cells of h (5.12 x 2.56 x 2.56
+    m for the defaults), 8 particles per cell in the cells whose centroid lies
+    under a surface that is flat at the mesh top up to x = crest and falls at
+    45 degrees beyond (half of the cells: 2,097,152 cells, 16,777,216
+    particles), MohrCoulomb phi=35 psi=0 c=2 kPa E=2e7 nu=0.3 rho=2000, MUSL,
+    dt=2e-5; fixed base, roller side walls, K0 = 1 - sin(phi) start under the
+    local surface height."""
+    coords, cells = structured_mesh((0.0, 0.0, 0.0), (h, h, h), (nx, ny, nz))
+    H = nz * h
+
+    def surface(x):
+        return np.minimum(H, H - (x[:, 0] - crest))
+
+    ci = np.arange(len(cells), dtype=np.int64)
+    xc = np.stack([(ci % nx + 0.5) * h, ((ci // nx) % ny + 0.5) * h, (ci // (nx * ny) + 0.5) * h],
+                  axis=1)
+    cset = ci[xc[:, 2] <= surface(xc)]
+    base = _node_sets(coords, 2, 0.0)
+    x0, x1 = _node_sets(coords, 0, 0.0), _node_sets(coords, 0, nx * h)
+    y0, y1 = _node_sets(coords, 1, 0.0), _node_sets(coords, 1, ny * h)
+    vn = np.concatenate([base, base, base, x0, x1, y0, y1])
+    vd = np.concatenate([np.zeros(len(base)), np.ones(len(base)), np.full(len(base), 2),
+                         np.zeros(len(x0)), np.zeros(len(x1)), np.ones(len(y0)),
+                         np.ones(len(y1))]).astype(np.int32)
+    phi = 35.0
+    return Workload(
+        name="wedge_3d", dim=3, scheme="musl", coords=coords, cells=cells,
+        materials=mat.mohr_coulomb(2e7, 0.3, 2000.0, friction=phi, dilation=0.0,
+                                   cohesion=2000.0)[None, :],
+        gravity=np.array([0.0, 0.0, -9.81]), dt=dt, nsteps=nsteps, nppd=2, cell_set=cset,
+        vel_nodes=vn, vel_dirs=vd, vel_values=np.zeros(len(vn)),
+        geostatic=(1.0 - np.sin(np.radians(phi)), 2000.0 * 9.81, surface),
+        info=dict(mesh="hex8 %dx%dx%d @ %g m" % (nx, ny, nz, h),
+                  material="MohrCoulomb phi=35 psi=0 c=2 kPa E=2e7",
+                  particles_per_cell=8, slope="45 deg from x = %g m" % crest))
