@@ -243,6 +243,8 @@ def main():
     ap.add_argument("--ordered", action="store_true", help="particle-order nodal sums")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of timed CPU work")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default scatter chunk and sort interval")
     ap.add_argument("--no-check", action="store_true",
                     help="adjoint_2d: skip the finite-difference checks")
     args = ap.parse_args()
@@ -265,6 +267,8 @@ def main():
     s = w.setup(Solver(w.dim, w.scheme, ordered=args.ordered))
     setup_s = time.perf_counter() - t0
     npart = s.nparticles
+    # scatter chunk and sort interval from timed trial steps (state restored)
+    tuned = None if args.no_autotune else s.autotune(w.dt, trial_steps=10)
     s.step(args.warmup, w.dt)
     s.synchronize()
     t0 = time.perf_counter()
@@ -282,6 +286,9 @@ def main():
     nact = active_nodes(w, cells)
     s.close()
     s = w.setup(Solver(w.dim, w.scheme, ordered=args.ordered))
+    if tuned:
+        s.set_chunk(tuned["chunk"])
+        s.set_sort_interval(tuned["sort_every"])
     s.step(args.warmup, w.dt)
     s.profile(True)
     s.step(args.steps, w.dt)
@@ -328,6 +335,7 @@ def main():
                           "frac": step_gbs / HBM_PEAK_GBS, "unit": "GB/s"},
         "kernels": kernels,
         "kernels_sum_ms_per_step": profiled_ms_per_step,
+        "autotune": tuned,
         "setup_s": setup_s,
     }
     if hasattr(w, "instances"):

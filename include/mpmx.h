@@ -181,6 +181,21 @@ int mpmx_step(mpmx_ctx* ctx, int nsteps, double dt, double damping_factor);
  * the reference, whose scatter works in particle order (mpm_scheme.tcc:44-71). */
 int mpmx_set_sort_interval(mpmx_ctx* ctx, int steps);
 
+/* Fast mode: particles per workgroup of the LDS-staged scatter kernels (the
+ * chunk of cell-sorted particles whose per-node sums are formed in LDS
+ * before the FP64 atomics): 128, 256 (default) or 512. No counterpart in the
+ * reference. */
+int mpmx_set_chunk(mpmx_ctx* ctx, int particles_per_workgroup);
+
+/* Fast mode: time trial_steps steps (after one warm-up step) for every pair
+ * of scatter chunk {128, 256, 512} and full-sort interval {4, 8, 16} from the
+ * current state, keep the fastest pair and restore the state (particles and
+ * step counter are as before the call). Writes the chosen pair. Ordered mode
+ * has nothing to tune and returns the current settings. No counterpart in
+ * the reference. */
+int mpmx_autotune(mpmx_ctx* ctx, double dt, double damping_factor, int trial_steps,
+                  int* chunk, int* sort_every);
+
 /* Step counter (drives the gravity ramp); set on resume (mpm_base.tcc:509-551). */
 int mpmx_set_step(mpmx_ctx* ctx, int64_t step);
 int mpmx_get_step(const mpmx_ctx* ctx, int64_t* step);

@@ -28,7 +28,8 @@ EXPORTS = (
     "mpmx_set_friction_constraints", "mpmx_set_gravity", "mpmx_generate_gauss",
     "mpmx_set_particles", "mpmx_compute_volume_mass", "mpmx_assign_velocity",
     "mpmx_assign_stresses", "mpmx_nparticles", "mpmx_get_particles", "mpmx_get_nodes",
-    "mpmx_step", "mpmx_set_sort_interval", "mpmx_set_step", "mpmx_get_step",
+    "mpmx_step", "mpmx_set_sort_interval", "mpmx_set_chunk", "mpmx_autotune", "mpmx_set_step",
+    "mpmx_get_step",
     "mpmx_synchronize", "mpmx_profile", "mpmx_kernel_times",
     "mpmx_adjoint_forward", "mpmx_adjoint_backward", "mpmx_adjoint_set_apex_ratio",
     "mpmx_adjoint_release",
@@ -81,6 +82,8 @@ def load(path=LIB_PATH):
         "mpmx_step": [V, C.c_int, C.c_double, C.c_double],
         "mpmx_set_step": [V, C.c_int64],
         "mpmx_set_sort_interval": [V, C.c_int],
+        "mpmx_set_chunk": [V, C.c_int],
+        "mpmx_autotune": [V, C.c_double, C.c_double, C.c_int, _I32, _I32],
         "mpmx_get_step": [V, _I64],
         "mpmx_synchronize": [V],
         "mpmx_profile": [V, C.c_int],
@@ -266,6 +269,18 @@ class Solver:
     def set_sort_interval(self, steps):
         """Fast mode: full particle sort by cell every `steps` steps."""
         self._check(self.lib.mpmx_set_sort_interval(self.ctx, int(steps)))
+
+    def set_chunk(self, particles_per_workgroup):
+        """Fast mode: particles per workgroup of the scatter kernels (128, 256, 512)."""
+        self._check(self.lib.mpmx_set_chunk(self.ctx, int(particles_per_workgroup)))
+
+    def autotune(self, dt, damping_factor=0.0, trial_steps=10):
+        """Pick the fastest (scatter chunk, sort interval) by timing trial steps
+        from the current state, which is restored. Returns the pair."""
+        ch, se = C.c_int(0), C.c_int(0)
+        self._check(self.lib.mpmx_autotune(self.ctx, float(dt), float(damping_factor),
+                                           int(trial_steps), C.byref(ch), C.byref(se)))
+        return dict(chunk=ch.value, sort_every=se.value)
 
     def set_step(self, step):
         self._check(self.lib.mpmx_set_step(self.ctx, int(step)))

@@ -128,6 +128,7 @@ struct mpmx_ctx {
   // whenever the particle set changed); in between, fast mode reuses the last
   // permutation with refreshed keys
   int sort_every = 8;
+  int chunk = 256;  // particles per workgroup of the LDS-staged scatters
   bool need_sort = true;
   int64_t since_sort = 0;
 
@@ -724,19 +725,19 @@ void enqueue_step(mpmx_ctx* c, const StepArgs& A, Launcher& L, bool full_sort) {
     }
   } else {
     if (c->scheme == USF) {
-      L("p2g_mass_momentum", [&] { launch_p2g<D>(P2G_MV, M, P, Nd, A, S, skey, perm, st); });
+      L("p2g_mass_momentum", [&] { launch_p2g<D>(P2G_MV, c->chunk, M, P, Nd, A, S, skey, perm, st); });
       L("node_velocity", [&] { launch_node<D>(ND_INIT | ND_VEL, M, Nd, A, st); });
-      L("stress_force_scatter", [&] { launch_stress_scatter<D>(M, P, Nd, mats, A, S, skey, perm, st); });
+      L("stress_force_scatter", [&] { launch_stress_scatter<D>(c->chunk, M, P, Nd, mats, A, S, skey, perm, st); });
       L("node_acceleration", [&] { launch_node<D>(ND_ACC | ND_ZERO_ALL, M, Nd, A, st); });
       L("g2p_locate", [&] { launch_g2p<D>(false, true, M, P, Nd, mats, A, c->vu, S, scan, st); });
     } else if (c->scheme == USL) {
-      L("p2g_all", [&] { launch_p2g<D>(P2G_MV_F, M, P, Nd, A, S, skey, perm, st); });
+      L("p2g_all", [&] { launch_p2g<D>(P2G_MV_F, c->chunk, M, P, Nd, A, S, skey, perm, st); });
       L("node_update", [&] { launch_node<D>(ND_INIT | ND_VEL | ND_ACC | ND_ZERO_ALL, M, Nd, A, st); });
       L("g2p_stress_locate", [&] { launch_g2p<D>(true, true, M, P, Nd, mats, A, c->vu, S, scan, st); });
     } else {
-      L("p2g_all", [&] { launch_p2g<D>(P2G_MV_F, M, P, Nd, A, S, skey, perm, st); });
+      L("p2g_all", [&] { launch_p2g<D>(P2G_MV_F, c->chunk, M, P, Nd, A, S, skey, perm, st); });
       L("node_update", [&] { launch_node<D>(ND_INIT | ND_VEL | ND_ACC | ND_ZERO_MOM, M, Nd, A, st); });
-      L("g2p_momentum_scatter", [&] { launch_g2p_scatter<D>(M, P, Nd, A, c->vu, skey, perm, st); });
+      L("g2p_momentum_scatter", [&] { launch_g2p_scatter<D>(c->chunk, M, P, Nd, A, c->vu, skey, perm, st); });
       L("node_velocity", [&] { launch_node<D>(ND_VEL | ND_ZERO_ALL, M, Nd, A, st); });
       L("stress_locate", [&] { launch_stress<D>(true, M, P, Nd, mats, A, S, scan, st); });
     }
@@ -1496,6 +1497,106 @@ int mpmx_set_sort_interval(mpmx_ctx* c, int steps) {
   API_BEGIN(c)
   if (steps < 1) throw ApiError(MPMX_E_ARG, "sort interval must be >= 1");
   c->sort_every = steps;
+  API_END(c)
+}
+
+int mpmx_set_chunk(mpmx_ctx* c, int particles_per_workgroup) {
+  API_BEGIN(c)
+  const int ch = particles_per_workgroup;
+  if (ch != 128 && ch != 256 && ch != 512)
+    throw ApiError(MPMX_E_ARG, "particles per workgroup must be 128, 256 or 512");
+  if (ch != c->chunk) {
+    c->chunk = ch;
+    c->invalidate_graph();
+  }
+  API_END(c)
+}
+
+int mpmx_autotune(mpmx_ctx* c, double dt, double damping_factor, int trial_steps,
+                  int* chunk_out, int* sort_every_out) {
+  API_BEGIN(c)
+  require_mesh(c);
+  if (c->mats.empty()) throw ApiError(MPMX_E_STATE, "materials are not set");
+  if (!(dt > 0) || trial_steps < 1) throw ApiError(MPMX_E_ARG, "dt > 0 and trial_steps >= 1");
+  if (c->np > 0 && !(c->flags & MPMX_ORDERED)) {
+    // snapshot of the particle state (the node accumulators are zero between
+    // steps), restored before every trial and at the end
+    int64_t per = 0;
+    for (const auto& a : ad_state(c)) per += a.comps;
+    DBuf<double> snap;
+    DBuf<int> snap_cell;
+    snap.alloc((size_t)per * c->np);
+    snap_cell.alloc(c->np);
+    auto copy_state = [&](bool save) {
+      int64_t off = 0;
+      for (const auto& a : ad_state(c)) {
+        const size_t bytes = (size_t)a.comps * c->np * sizeof(double);
+        if (save)
+          HIPCHECK(hipMemcpyAsync(snap.p + off, a.p, bytes, hipMemcpyDeviceToDevice, c->stream));
+        else
+          HIPCHECK(hipMemcpyAsync(a.p, snap.p + off, bytes, hipMemcpyDeviceToDevice, c->stream));
+        off += a.comps * c->np;
+      }
+      const size_t cb = c->np * sizeof(int);
+      if (save)
+        HIPCHECK(hipMemcpyAsync(snap_cell.p, c->d_cell.p, cb, hipMemcpyDeviceToDevice, c->stream));
+      else
+        HIPCHECK(hipMemcpyAsync(c->d_cell.p, snap_cell.p, cb, hipMemcpyDeviceToDevice, c->stream));
+    };
+    const int64_t step0 = c->step;
+    copy_state(true);
+    hipEvent_t e0, e1;
+    HIPCHECK(hipEventCreate(&e0));
+    HIPCHECK(hipEventCreate(&e1));
+    int best_chunk = c->chunk, best_sort = c->sort_every;
+    float best_ms = 0.f;
+    bool first = true;
+    try {
+      for (const int ch : {128, 256, 512})
+        for (const int se : {4, 8, 16}) {
+          copy_state(false);
+          c->step = step0;
+          c->chunk = ch;
+          c->sort_every = se;
+          c->need_sort = true;
+          c->invalidate_graph();
+          reset_status(c);
+          launch_steps(c, 1, dt, damping_factor);  // capture, first sort
+          HIPCHECK(hipEventRecord(e0, c->stream));
+          launch_steps(c, trial_steps, dt, damping_factor);
+          HIPCHECK(hipEventRecord(e1, c->stream));
+          check_status(c);
+          float ms = 0.f;
+          HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
+          if (first || ms < best_ms) {
+            best_ms = ms;
+            best_chunk = ch;
+            best_sort = se;
+            first = false;
+          }
+        }
+    } catch (...) {
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      copy_state(false);
+      c->step = step0;
+      c->need_sort = true;
+      c->invalidate_graph();
+      (void)hipStreamSynchronize(c->stream);
+      throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    copy_state(false);
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    c->step = step0;
+    c->chunk = best_chunk;
+    c->sort_every = best_sort;
+    c->need_sort = true;
+    c->invalidate_graph();
+  }
+  if (chunk_out) *chunk_out = c->chunk;
+  if (sort_every_out) *sort_every_out = c->sort_every;
   API_END(c)
 }
 

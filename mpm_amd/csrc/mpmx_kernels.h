@@ -21,11 +21,13 @@ template <int D>
 void launch_locate_scan(const Mesh& M, const Particles& P, Status* S, const int* scan_list,
                         const int* hint, hipStream_t st);
 void launch_step_end(Status* S, hipStream_t st);
+// chunk: particles per workgroup of the LDS-staged scatters (128, 256, 512)
 template <int D>
-void launch_p2g(int what, const Mesh& M, const Particles& P, const Nodes& Nd, const StepArgs& A,
-                const Status* S, const int* skey, const int* perm, hipStream_t st);
+void launch_p2g(int what, int chunk, const Mesh& M, const Particles& P, const Nodes& Nd,
+                const StepArgs& A, const Status* S, const int* skey, const int* perm,
+                hipStream_t st);
 template <int D>
-void launch_stress_scatter(const Mesh& M, const Particles& P, const Nodes& Nd,
+void launch_stress_scatter(int chunk, const Mesh& M, const Particles& P, const Nodes& Nd,
                            const Material* mats, const StepArgs& A, const Status* S,
                            const int* skey, const int* perm, hipStream_t st);
 template <int D>
@@ -37,8 +39,9 @@ void launch_g2p(bool stress, bool locate, const Mesh& M, const Particles& P, con
                 const Material* mats, const StepArgs& A, int vu, Status* S, int* scan_list,
                 hipStream_t st);
 template <int D>
-void launch_g2p_scatter(const Mesh& M, const Particles& P, const Nodes& Nd, const StepArgs& A,
-                        int vu, const int* skey, const int* perm, hipStream_t st);
+void launch_g2p_scatter(int chunk, const Mesh& M, const Particles& P, const Nodes& Nd,
+                        const StepArgs& A, int vu, const int* skey, const int* perm,
+                        hipStream_t st);
 template <int D>
 void launch_node(int fl, const Mesh& M, const Nodes& Nd, const StepArgs& A, hipStream_t st);
 template <int D>

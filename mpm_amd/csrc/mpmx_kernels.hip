@@ -328,17 +328,18 @@ __global__ void k_step_end(Status* S) {
 // What the staging carries per particle.
 enum { SC_MV = 1, SC_F = 2, SC_MOM = 4 };
 
-template <int D, int WHAT>
+// CH: particles per workgroup (one per thread), a multiple of 64.
+template <int D, int WHAT, int CH>
 struct Stage {
   static constexpr int NN = Elem<D>::NN;
-  int key[kBlock];
-  double xi[D][kBlock];
-  double m[kBlock];
-  double v[D][kBlock];
-  double B[(WHAT & SC_F) ? D * D : 1][kBlock];  // force matrix, see force_matrix()
-  int seg_start[kBlock + 1];
+  int key[CH];
+  double xi[D][CH];
+  double m[CH];
+  double v[D][CH];
+  double B[(WHAT & SC_F) ? D * D : 1][CH];  // force matrix, see force_matrix()
+  int seg_start[CH + 1];
   int nseg;
-  int wave_heads[kBlock / 64];
+  int wave_heads[CH / 64];
 };
 
 // Internal force of a particle on node i (particle.tcc:914-950) is
@@ -417,8 +418,8 @@ MPMX_DEV void merge_face_neighbours(int i, int& n, double* vals) {
 
 // Phase 2 of every scatter kernel: given staged particles [0, cnt) of this
 // chunk, sorted by cell, add each cell segment's per-node sums to the nodes.
-template <int D, int WHAT>
-MPMX_DEV void scatter_segments(Stage<D, WHAT>& st, int cnt, const Mesh& M, const Nodes& Nd,
+template <int D, int WHAT, int CH>
+MPMX_DEV void scatter_segments(Stage<D, WHAT, CH>& st, int cnt, const Mesh& M, const Nodes& Nd,
                                const double* g) {
   constexpr int NN = Elem<D>::NN;
   const int tid = threadIdx.x;
@@ -431,13 +432,13 @@ MPMX_DEV void scatter_segments(Stage<D, WHAT>& st, int cnt, const Mesh& M, const
   __syncthreads();
   int before = 0;
 #pragma unroll
-  for (int w = 0; w < kBlock / 64; ++w) before += (w < wave) ? st.wave_heads[w] : 0;
+  for (int w = 0; w < CH / 64; ++w) before += (w < wave) ? st.wave_heads[w] : 0;
   const int rank = before + __popcll(mask & ((1ull << lane) - 1ull));
   if (head) st.seg_start[rank] = tid;
   if (tid == 0) {
     int tot = 0;
 #pragma unroll
-    for (int w = 0; w < kBlock / 64; ++w) tot += st.wave_heads[w];
+    for (int w = 0; w < CH / 64; ++w) tot += st.wave_heads[w];
     st.nseg = tot;
   }
   __syncthreads();
@@ -451,7 +452,7 @@ MPMX_DEV void scatter_segments(Stage<D, WHAT>& st, int cnt, const Mesh& M, const
   }
   using CP = Comp<D, WHAT>;
   const int ntask = nseg * NN;
-  for (int t0 = 0; t0 < ntask; t0 += kBlock) {  // same trip count for every lane
+  for (int t0 = 0; t0 < ntask; t0 += CH) {  // same trip count for every lane
     const int task = t0 + tid;
     const int sidx = task / NN;
     const int i = task - sidx * NN;
@@ -543,13 +544,13 @@ MPMX_DEV void gravity_now(const StepArgs& A, const Status* S, int dim, double* g
 // Mass/momentum (and, for USL/MUSL, body and internal forces from the
 // stress of the previous step) scattered from sorted particles.
 // mpm_scheme.tcc:44-71 and 131-190; particle.tcc:630-658, 893-950.
-template <int D, int WHAT>
-__global__ __launch_bounds__(kBlock) void k_p2g(Mesh M, Particles P, Nodes Nd, StepArgs A,
+template <int D, int WHAT, int CH>
+__global__ __launch_bounds__(CH) void k_p2g(Mesh M, Particles P, Nodes Nd, StepArgs A,
                                                 const Status* S, const int* skey,
                                                 const int* perm) {
-  __shared__ Stage<D, WHAT> st;
-  const int64_t base = blockIdx.x * (int64_t)kBlock;
-  const int cnt = (int)((P.np - base) < kBlock ? (P.np - base) : kBlock);
+  __shared__ Stage<D, WHAT, CH> st;
+  const int64_t base = blockIdx.x * (int64_t)CH;
+  const int cnt = (int)((P.np - base) < CH ? (P.np - base) : CH);
   const int tid = threadIdx.x;
   const int64_t np = P.np;
   if (tid < cnt) {
@@ -584,19 +585,19 @@ __global__ __launch_bounds__(kBlock) void k_p2g(Mesh M, Particles P, Nodes Nd, S
   double g[3] = {0, 0, 0};
   if constexpr ((WHAT & SC_F) != 0) gravity_now(A, S, D, g);
   __syncthreads();
-  scatter_segments<D, WHAT>(st, cnt, M, Nd, g);
+  scatter_segments<D, WHAT, CH>(st, cnt, M, Nd, g);
 }
 
 // USF stress update in sorted order, then the force scatter of the new stress.
-template <int D>
-__global__ __launch_bounds__(kBlock) MPMX_MC_ATTR void k_stress_scatter(Mesh M, Particles P, Nodes Nd,
+template <int D, int CH>
+__global__ __launch_bounds__(CH) MPMX_MC_ATTR void k_stress_scatter(Mesh M, Particles P, Nodes Nd,
                                                            const Material* mats, StepArgs A,
                                                            const Status* S, const int* skey,
                                                            const int* perm) {
   constexpr int NN = Elem<D>::NN;
-  __shared__ Stage<D, SC_F> st;
-  const int64_t base = blockIdx.x * (int64_t)kBlock;
-  const int cnt = (int)((P.np - base) < kBlock ? (P.np - base) : kBlock);
+  __shared__ Stage<D, SC_F, CH> st;
+  const int64_t base = blockIdx.x * (int64_t)CH;
+  const int cnt = (int)((P.np - base) < CH ? (P.np - base) : CH);
   const int tid = threadIdx.x;
   const int64_t np = P.np;
   if (tid < cnt) {
@@ -626,7 +627,7 @@ __global__ __launch_bounds__(kBlock) MPMX_MC_ATTR void k_stress_scatter(Mesh M, 
   double g[3] = {0, 0, 0};
   gravity_now(A, S, D, g);
   __syncthreads();
-  scatter_segments<D, SC_F>(st, cnt, M, Nd, g);
+  scatter_segments<D, SC_F, CH>(st, cnt, M, Nd, g);
 }
 
 // Stress update in particle order (USL/MUSL postcompute, and USF in ordered
@@ -719,14 +720,14 @@ __global__ __launch_bounds__(kBlock) MPMX_MC_ATTR void k_g2p(Mesh M, Particles P
 // MUSL G2P in sorted order with the momentum of the updated particle
 // velocities scattered back (mpm_scheme_musl.tcc:21-35; mass is unchanged
 // because the shape functions are not recomputed).
-template <int D>
-__global__ __launch_bounds__(kBlock) void k_g2p_scatter(Mesh M, Particles P, Nodes Nd, StepArgs A,
+template <int D, int CH>
+__global__ __launch_bounds__(CH) void k_g2p_scatter(Mesh M, Particles P, Nodes Nd, StepArgs A,
                                                         int vu, const int* skey,
                                                         const int* perm) {
   constexpr int NN = Elem<D>::NN;
-  __shared__ Stage<D, SC_MOM> st;
-  const int64_t base = blockIdx.x * (int64_t)kBlock;
-  const int cnt = (int)((P.np - base) < kBlock ? (P.np - base) : kBlock);
+  __shared__ Stage<D, SC_MOM, CH> st;
+  const int64_t base = blockIdx.x * (int64_t)CH;
+  const int cnt = (int)((P.np - base) < CH ? (P.np - base) : CH);
   const int tid = threadIdx.x;
   const int64_t np = P.np;
   if (tid < cnt) {
@@ -778,7 +779,7 @@ __global__ __launch_bounds__(kBlock) void k_g2p_scatter(Mesh M, Particles P, Nod
     }
   }
   __syncthreads();
-  scatter_segments<D, SC_MOM>(st, cnt, M, Nd, nullptr);
+  scatter_segments<D, SC_MOM, CH>(st, cnt, M, Nd, nullptr);
 }
 
 // ------------------------------- node kernels -------------------------------
@@ -1053,6 +1054,24 @@ __global__ __launch_bounds__(kBlock) void k_gather(Mesh M, Particles P, Nodes Nd
 // ------------------------------- launchers ----------------------------------
 static inline dim3 grid_for(int64_t n) { return dim3((unsigned)((n + kBlock - 1) / kBlock)); }
 
+// Scatter kernels: one workgroup of CH threads per chunk of CH particles.
+#define MPMX_LAUNCH_CH(CH, kernel, n, stream, ...)                                    \
+  do {                                                                                \
+    if ((n) > 0)                                                                      \
+      hipLaunchKernelGGL(kernel, dim3((unsigned)(((n) + (CH)-1) / (CH))), dim3(CH), 0, \
+                         stream, __VA_ARGS__);                                        \
+  } while (0)
+#define MPMX_CHUNK_DISPATCH(chunk, LAUNCH) \
+  do {                                     \
+    if ((chunk) == 128) {                  \
+      LAUNCH(128);                         \
+    } else if ((chunk) == 512) {           \
+      LAUNCH(512);                         \
+    } else {                               \
+      LAUNCH(256);                         \
+    }                                      \
+  } while (0)
+
 #define MPMX_LAUNCH(kernel, n, stream, ...)                                       \
   do {                                                                            \
     if ((n) > 0) hipLaunchKernelGGL(kernel, grid_for(n), dim3(kBlock), 0, stream, \
@@ -1077,19 +1096,29 @@ void launch_step_end(Status* S, hipStream_t st) {
 }
 
 template <int D>
-void launch_p2g(int what, const Mesh& M, const Particles& P, const Nodes& Nd, const StepArgs& A,
-                const Status* S, const int* skey, const int* perm, hipStream_t st) {
-  if (what == SC_MV)
-    MPMX_LAUNCH((k_p2g<D, SC_MV>), P.np, st, M, P, Nd, A, S, skey, perm);
-  else
-    MPMX_LAUNCH((k_p2g<D, SC_MV | SC_F>), P.np, st, M, P, Nd, A, S, skey, perm);
+void launch_p2g(int what, int chunk, const Mesh& M, const Particles& P, const Nodes& Nd,
+                const StepArgs& A, const Status* S, const int* skey, const int* perm,
+                hipStream_t st) {
+  if (what == SC_MV) {
+#define L_(CH) MPMX_LAUNCH_CH(CH, (k_p2g<D, SC_MV, CH>), P.np, st, M, P, Nd, A, S, skey, perm)
+    MPMX_CHUNK_DISPATCH(chunk, L_);
+#undef L_
+  } else {
+#define L_(CH) \
+  MPMX_LAUNCH_CH(CH, (k_p2g<D, SC_MV | SC_F, CH>), P.np, st, M, P, Nd, A, S, skey, perm)
+    MPMX_CHUNK_DISPATCH(chunk, L_);
+#undef L_
+  }
 }
 
 template <int D>
-void launch_stress_scatter(const Mesh& M, const Particles& P, const Nodes& Nd,
+void launch_stress_scatter(int chunk, const Mesh& M, const Particles& P, const Nodes& Nd,
                            const Material* mats, const StepArgs& A, const Status* S,
                            const int* skey, const int* perm, hipStream_t st) {
-  MPMX_LAUNCH((k_stress_scatter<D>), P.np, st, M, P, Nd, mats, A, S, skey, perm);
+#define L_(CH) \
+  MPMX_LAUNCH_CH(CH, (k_stress_scatter<D, CH>), P.np, st, M, P, Nd, mats, A, S, skey, perm)
+  MPMX_CHUNK_DISPATCH(chunk, L_);
+#undef L_
 }
 
 template <int D>
@@ -1115,9 +1144,12 @@ void launch_g2p(bool stress, bool locate, const Mesh& M, const Particles& P, con
 }
 
 template <int D>
-void launch_g2p_scatter(const Mesh& M, const Particles& P, const Nodes& Nd, const StepArgs& A,
-                        int vu, const int* skey, const int* perm, hipStream_t st) {
-  MPMX_LAUNCH((k_g2p_scatter<D>), P.np, st, M, P, Nd, A, vu, skey, perm);
+void launch_g2p_scatter(int chunk, const Mesh& M, const Particles& P, const Nodes& Nd,
+                        const StepArgs& A, int vu, const int* skey, const int* perm,
+                        hipStream_t st) {
+#define L_(CH) MPMX_LAUNCH_CH(CH, (k_g2p_scatter<D, CH>), P.np, st, M, P, Nd, A, vu, skey, perm)
+  MPMX_CHUNK_DISPATCH(chunk, L_);
+#undef L_
 }
 
 template <int D>
@@ -1171,10 +1203,10 @@ void launch_gather(int which, const Mesh& M, const Particles& P, const Nodes& Nd
                                       const int*, hipStream_t);                              \
   template void launch_locate_scan<D>(const Mesh&, const Particles&, Status*, const int*,    \
                                       const int*, hipStream_t);                              \
-  template void launch_p2g<D>(int, const Mesh&, const Particles&, const Nodes&,              \
+  template void launch_p2g<D>(int, int, const Mesh&, const Particles&, const Nodes&,         \
                               const StepArgs&, const Status*, const int*, const int*,        \
                               hipStream_t);                                                  \
-  template void launch_stress_scatter<D>(const Mesh&, const Particles&, const Nodes&,        \
+  template void launch_stress_scatter<D>(int, const Mesh&, const Particles&, const Nodes&,   \
                                          const Material*, const StepArgs&, const Status*,    \
                                          const int*, const int*, hipStream_t);               \
   template void launch_stress<D>(bool, const Mesh&, const Particles&, const Nodes&,           \
@@ -1183,7 +1215,7 @@ void launch_gather(int which, const Mesh& M, const Particles& P, const Nodes& Nd
   template void launch_g2p<D>(bool, bool, const Mesh&, const Particles&, const Nodes&,       \
                               const Material*, const StepArgs&, int, Status*, int*,          \
                               hipStream_t);                                                  \
-  template void launch_g2p_scatter<D>(const Mesh&, const Particles&, const Nodes&,           \
+  template void launch_g2p_scatter<D>(int, const Mesh&, const Particles&, const Nodes&,      \
                                       const StepArgs&, int, const int*, const int*,          \
                                       hipStream_t);                                          \
   template void launch_node<D>(int, const Mesh&, const Nodes&, const StepArgs&, hipStream_t);\
