@@ -108,6 +108,18 @@ int mpmx_set_acceleration_constraints(mpmx_ctx* ctx, int64_t n, const int64_t* n
 int mpmx_set_friction_constraints(mpmx_ctx* ctx, int64_t n, const int64_t* node,
                                   const int* dir, const int* sign_n, const double* mu);
 
+/* Particle velocity constraints: after the velocity update of every step,
+ * velocity component dir[i] of particle particle[i] (-1: every particle)
+ * is set to value[i], constraints applied in the order given (the reference's
+ * particles_velocity_constraints with particle sets expanded to ids,
+ * MPMBase::particle_velocity_constraints mpm_base.tcc:1549-1585,
+ * Mesh::apply_particle_velocity_constraints mesh.tcc:1454-1468,
+ * Particle::apply_particle_velocity_constraints particle.tcc:1190-1195).
+ * Replaces any earlier set; n = 0 removes them. Particle ids in reference
+ * order; particles added later are unconstrained unless the set uses -1. */
+int mpmx_set_particle_velocity_constraints(mpmx_ctx* ctx, int64_t n, const int64_t* particle,
+                                           const int* dir, const double* value);
+
 /* Gravity and its ramp (MPMBase::initialise_loads, mpm_base.tcc:788-830;
  * multiplier min(1, step*dt/T), mpm_explicit.tcc:143-153). */
 int mpmx_set_gravity(mpmx_ctx* ctx, const double* g, double ramping_time);

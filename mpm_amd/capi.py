@@ -25,7 +25,8 @@ EXPORTS = (
     "mpmx_create", "mpmx_destroy", "mpmx_last_error", "mpmx_set_mesh", "mpmx_set_instances",
     "mpmx_set_materials",
     "mpmx_set_velocity_constraints", "mpmx_set_acceleration_constraints",
-    "mpmx_set_friction_constraints", "mpmx_set_gravity", "mpmx_generate_gauss",
+    "mpmx_set_friction_constraints", "mpmx_set_particle_velocity_constraints",
+    "mpmx_set_gravity", "mpmx_generate_gauss",
     "mpmx_set_particles", "mpmx_compute_volume_mass", "mpmx_assign_velocity",
     "mpmx_assign_stresses", "mpmx_nparticles", "mpmx_get_particles", "mpmx_get_nodes",
     "mpmx_step", "mpmx_set_sort_interval", "mpmx_set_chunk", "mpmx_autotune", "mpmx_set_step",
@@ -70,6 +71,7 @@ def load(path=LIB_PATH):
         "mpmx_set_velocity_constraints": [V, C.c_int64, _I64, _I32, _D],
         "mpmx_set_acceleration_constraints": [V, C.c_int64, _I64, _I32, _D],
         "mpmx_set_friction_constraints": [V, C.c_int64, _I64, _I32, _I32, _D],
+        "mpmx_set_particle_velocity_constraints": [V, C.c_int64, _I64, _I32, _D],
         "mpmx_set_gravity": [V, _D, C.c_double],
         "mpmx_generate_gauss": [V, C.c_int, C.c_int64, _I64, C.c_int],
         "mpmx_set_particles": [V, C.c_int64, _D, _D, _D, _D, _D, _D, _D, _D, _D, _I32, _I64, _D],
@@ -184,6 +186,15 @@ class Solver:
 
     def set_friction_constraints(self, nodes, dirs, sign_n, mu):
         self._cons(self.lib.mpmx_set_friction_constraints, nodes, dirs, (sign_n, _I32), (mu, _D))
+
+    def set_particle_velocity_constraints(self, particles, dirs, values):
+        """Velocity component dirs[i] of particle particles[i] (-1: all) is set
+        to values[i] after every velocity update, in order."""
+        pa = np.ascontiguousarray(particles, dtype=np.int64)
+        di = np.ascontiguousarray(dirs, dtype=np.int32)
+        va = np.ascontiguousarray(values, dtype=np.float64)
+        self._check(self.lib.mpmx_set_particle_velocity_constraints(
+            self.ctx, len(pa), _p(pa, _I64), _p(di, _I32), _p(va)))
 
     def set_gravity(self, g, ramping_time=0.0):
         self._check(self.lib.mpmx_set_gravity(self.ctx, _p(_f64(g)), ramping_time))

@@ -399,13 +399,14 @@ MPMX_DEV bool stress_vjp(const Mesh& M, const Material& mat, int64_t p, const Sh
   return deferred;
 }
 
-// Adjoint of compute_updated_position (particle.tcc:994-1070): adds dL/dv'
-// and dL/da of the cell's nodes to accvp and acca, returns dL/dv_p (in lv),
-// folds lxi.
+// Adjoint of compute_updated_position (particle.tcc:994-1070) and of the
+// particle velocity constraints (pvc: constrained components, mesh.tcc:
+// 1454-1468): adds dL/dv' and dL/da of the cell's nodes to accvp and acca,
+// returns dL/dv_p (in lv), folds lxi.
 template <int D>
 MPMX_DEV void g2p_vjp(const Mesh& M, const Shape<D>& S, const AdjNodes& A, const AdjArgs& a,
-                      const double* lx, double* lv, double (*accvp)[D], double (*acca)[D],
-                      double* lxi) {
+                      int pvc, const double* lx, double* lv, double (*accvp)[D],
+                      double (*acca)[D], double* lxi) {
   constexpr int NN = Elem<D>::NN;
   const int64_t nn = M.nnodes;
   const bool flip = a.vu == FLIP;
@@ -413,6 +414,7 @@ MPMX_DEV void g2p_vjp(const Mesh& M, const Shape<D>& S, const AdjNodes& A, const
   double lnv[D], lna[D];
 #pragma unroll
   for (int k = 0; k < D; ++k) {
+    if (pvc & (1 << k)) lv[k] = 0.;  // velocity constraint: v'_k is fixed
     lnv[k] = a.dt * lx[k] + (flip ? (1.0 - beta) * lv[k] : lv[k]);
     lna[k] = flip ? beta * a.dt * lv[k] : 0.;
     lv[k] = flip ? beta * lv[k] : 0.;
@@ -761,7 +763,7 @@ __global__ __launch_bounds__(kAdBlock) MPMX_AD_ATTR void k_ad_b1(Mesh M, Particl
       double lx[D], lv[D];
       load_vec<D>(L.lx, np, p, D, lx);
       load_vec<D>(L.lv, np, p, D, lv);
-      g2p_vjp<D>(M, S, A, a, lx, lv, acc0, acc1, lxi);
+      g2p_vjp<D>(M, S, A, a, P.pvc_mask ? P.pvc_mask[p] : 0, lx, lv, acc0, acc1, lxi);
       store_vec<D>(L.lv, np, p, D, lv);
     }
     add_lxq<D>(L, np, p, S, lxi);
@@ -836,7 +838,7 @@ __global__ __launch_bounds__(kAdBlock) MPMX_AD_ATTR void k_ad_b3(Mesh M, Particl
                     dummy_lV, lxi);
       double lx[D];
       load_vec<D>(L.lx, np, p, D, lx);
-      g2p_vjp<D>(M, S, A, a, lx, lv, acc0, acc1, lxi);
+      g2p_vjp<D>(M, S, A, a, P.pvc_mask ? P.pvc_mask[p] : 0, lx, lv, acc0, acc1, lxi);
     }
     store_vec<D>(L.lv, np, p, D, lv);
     L.gm[p] += lm_p;

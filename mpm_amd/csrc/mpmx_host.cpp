@@ -111,6 +111,17 @@ struct mpmx_ctx {
   DBuf<Material> d_mats;
   bool any_mc = false;
 
+  // particle velocity constraints in the order given: (particle or -1 for
+  // all, direction, value); applied per particle as a mask and values
+  struct Pvc {
+    int64_t particle;
+    int dir;
+    double value;
+  };
+  std::vector<Pvc> pvc;
+  DBuf<unsigned char> d_pvc_mask;
+  DBuf<double> d_pvc_val;
+
   // particles
   int64_t np = 0;
   DBuf<double> d_x, d_u, d_v, d_stress, d_strain, d_F, d_mass, d_vol, d_rho, d_xi, d_sv;
@@ -202,6 +213,8 @@ struct mpmx_ctx {
     P.sv = d_sv.p;
     P.mat = d_mat.p;
     P.cell = d_cell.p;
+    P.pvc_mask = pvc.empty() ? nullptr : d_pvc_mask.p;
+    P.pvc_val = pvc.empty() ? nullptr : d_pvc_val.p;
     return P;
   }
   Nodes nodes() const {
@@ -501,6 +514,30 @@ void get_soa(const DBuf<T>& b, int64_t np, int comps, T* rows, hipStream_t st) {
   }
 }
 
+// Per-particle mask and values of the particle velocity constraints, in the
+// order given (a later constraint on the same particle and direction wins, as
+// Mesh::apply_particle_velocity_constraints applies them in sequence,
+// mesh.tcc:1454-1468).
+void build_pvc(mpmx_ctx* c) {
+  if (c->pvc.empty() || c->np == 0) return;
+  const int64_t np = c->np;
+  std::vector<unsigned char> mask(np, 0);
+  std::vector<double> val((size_t)c->dim * np, 0.0);
+  for (const auto& q : c->pvc) {
+    const int64_t b = q.particle < 0 ? 0 : q.particle;
+    const int64_t e = q.particle < 0 ? np : std::min<int64_t>(q.particle + 1, np);
+    for (int64_t p = b; p < e; ++p) {
+      mask[p] |= (unsigned char)(1u << q.dir);
+      val[(size_t)q.dir * np + p] = q.value;
+    }
+  }
+  c->d_pvc_mask.alloc(np);
+  c->d_pvc_val.alloc((size_t)c->dim * np);
+  c->d_pvc_mask.upload(mask.data(), np, c->stream);
+  c->d_pvc_val.upload(val.data(), (size_t)c->dim * np, c->stream);
+  HIPCHECK(hipStreamSynchronize(c->stream));
+}
+
 void alloc_step_buffers(mpmx_ctx* c) {
   const int64_t np = c->np;
   c->d_skey.alloc(std::max<int64_t>(np, 1));
@@ -589,6 +626,7 @@ int64_t append_particles(mpmx_ctx* c, int64_t n, const double* x, const double* 
   if (c->any_mc) grow(c->d_sv, off, np, NSV, st);
   c->np = np;
   c->ad_recorded = false;
+  build_pvc(c);
   std::vector<double> zeros((size_t)n * 9, 0.0), eye((size_t)n * 9, 0.0);
   for (int64_t i = 0; i < n; ++i) eye[i * 9 + 0] = eye[i * 9 + 4] = eye[i * 9 + 8] = 1.0;
   put_soa(c->d_x, np, off, n, d, x, st);
@@ -643,6 +681,7 @@ void truncate_particles(mpmx_ctx* c, int64_t keep) {
   if (c->any_mc) shrink(c->d_sv, NSV);
   c->np = keep;
   c->ad_recorded = false;
+  build_pvc(c);
   alloc_step_buffers(c);
   c->invalidate_graph();
 }
@@ -1261,6 +1300,28 @@ int mpmx_set_friction_constraints(mpmx_ctx* c, int64_t n, const int64_t* node, c
     hc.fmu = mu[i];
   }
   upload_constraints(c);
+  API_END(c)
+}
+
+int mpmx_set_particle_velocity_constraints(mpmx_ctx* c, int64_t n, const int64_t* particle,
+                                           const int* dir, const double* value) {
+  API_BEGIN(c)
+  if (n < 0 || (n > 0 && (!particle || !dir || !value)))
+    throw ApiError(MPMX_E_ARG, "particle velocity constraints: n rows of particle, dir, value");
+  std::vector<mpmx_ctx::Pvc> list;
+  for (int64_t i = 0; i < n; ++i) {
+    if (dir[i] < 0 || dir[i] >= c->dim)
+      throw ApiError(MPMX_E_ARG, "Invalid direction of velocity constraint");
+    if (particle[i] < -1 || particle[i] >= c->np)
+      throw ApiError(MPMX_E_ARG, "No particle set found to assign velocity constraint");
+    list.push_back({particle[i], dir[i], value[i]});
+  }
+  c->pvc = list;
+  c->d_pvc_mask.release();
+  c->d_pvc_val.release();
+  build_pvc(c);
+  c->invalidate_graph();
+  c->ad_recorded = false;
   API_END(c)
 }
 

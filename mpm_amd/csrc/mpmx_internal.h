@@ -73,6 +73,10 @@ struct Particles {
   double* sv;      // [NSV][np] or null when no MohrCoulomb material
   int* mat;
   int* cell;       // current cell (-1: none)
+  // Particle velocity constraints (mesh.tcc:1454-1468): bit k of pvc_mask[p]
+  // sets v_k = pvc_val[k][p] after the velocity update. Null when none.
+  const unsigned char* pvc_mask;
+  const double* pvc_val;
 };
 
 struct Nodes {

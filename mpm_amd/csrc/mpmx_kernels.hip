@@ -694,6 +694,7 @@ __global__ __launch_bounds__(kBlock) MPMX_MC_ATTR void k_g2p(Mesh M, Particles P
     x[k] = P.x[k * np + p];
     u0[k] = P.u[k * np + p];
   }
+  const int pvc = P.pvc_mask ? P.pvc_mask[p] : 0;
 #pragma unroll
   for (int k = 0; k < D; ++k) {
     double v = v0[k];
@@ -703,6 +704,7 @@ __global__ __launch_bounds__(kBlock) MPMX_MC_ATTR void k_g2p(Mesh M, Particles P
     } else {
       v = sv[k];
     }
+    if (pvc & (1 << k)) v = P.pvc_val[k * np + p];  // particle.tcc:1190-1195
     P.v[k * np + p] = v;
     x[k] = x[k] + sv[k] * A.dt;
     P.x[k * np + p] = x[k];
@@ -761,6 +763,7 @@ __global__ __launch_bounds__(CH) void k_g2p_scatter(Mesh M, Particles P, Nodes N
         u0[k] = P.u[k * np + p];
       }
       st.m[tid] = P.mass[p];
+      const int pvc = P.pvc_mask ? P.pvc_mask[p] : 0;
 #pragma unroll
       for (int k = 0; k < D; ++k) {
         double v = v0[k];
@@ -770,6 +773,7 @@ __global__ __launch_bounds__(CH) void k_g2p_scatter(Mesh M, Particles P, Nodes N
         } else {
           v = sv[k];
         }
+        if (pvc & (1 << k)) v = P.pvc_val[k * np + p];  // particle.tcc:1190-1195
         P.v[k * np + p] = v;
         P.x[k * np + p] = x0[k] + sv[k] * A.dt;
         P.u[k * np + p] = u0[k] + sv[k] * A.dt;

@@ -64,7 +64,7 @@ def _lib(omp=False, libm="portable"):
                    "orc_nparticles", "orc_get_particles", "orc_get_nodes", "orc_get_cells",
                    "orc_step", "orc_phase", "orc_set_node_field", "orc_set_particle_field",
                    "orc_get_particle_scratch", "orc_assign_velocity",
-                   "orc_assign_stresses"):
+                   "orc_assign_stresses", "orc_set_particle_velocity_constraints"):
             getattr(lib, fn).restype = C.c_int
         lib.orc_kat_cell_volume.restype = C.c_double
         lib.orc_kat_mc_yield.restype = C.c_int
@@ -157,6 +157,13 @@ class Oracle:
     def set_friction_constraints(self, nodes, dirs, sign_n, mu):
         self._constraints(self.lib.orc_set_friction_constraints, nodes, dirs, (sign_n, mu),
                           ints=True)
+
+    def set_particle_velocity_constraints(self, particles, dirs, values):
+        pa = np.ascontiguousarray(particles, dtype=np.int64)
+        di = np.ascontiguousarray(dirs, dtype=np.int32)
+        va = np.ascontiguousarray(values, dtype=np.float64)
+        self._check(self.lib.orc_set_particle_velocity_constraints(
+            self.ctx, C.c_int64(len(pa)), _p(pa, _I64), _p(di, _I32), _p(va)))
 
     def set_velocity_update(self, velocity_update, blending_ratio=1.0):
         self.lib.orc_set_velocity_update(self.ctx, VELOCITY_UPDATES[velocity_update],

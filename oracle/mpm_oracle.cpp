@@ -1250,6 +1250,24 @@ struct Solver {
     for (int k = 0; k < dim; ++k) p.u[k] += nv[k] * dt;
   }
 
+  // particle velocity constraints (particle or -1 for all, dir, value), in
+  // the order given: mesh.tcc:1454-1468, particle.tcc:1190-1195
+  struct PVel {
+    Index particle;
+    int dir;
+    double value;
+  };
+  std::vector<PVel> pvel;
+  void apply_particle_velocity_constraints() {
+    for (const auto& c : pvel) {
+      if (c.particle < 0) {
+        for (auto& p : particles) p.v[c.dir] = c.value;
+      } else if (c.particle < (Index)particles.size()) {
+        particles[c.particle].v[c.dir] = c.value;
+      }
+    }
+  }
+
   void node_reset(Node& n) {  // node.tcc:26-43
     n.mass = 0;
     for (int k = 0; k < 3; ++k) {
@@ -1295,6 +1313,7 @@ struct Solver {
       if (nodes[i].status) compute_acceleration_velocity_cundall(nodes[i], dt, damping);
     ORC_PRAGMA(omp parallel for schedule(static))
     for (Index i = 0; i < np; ++i) compute_updated_position(particles[i], dt);
+    apply_particle_velocity_constraints();  // mpm_scheme.tcc:239-240
     // postcompute_nodal_kinematics (MUSL, mpm_scheme_musl.tcc:21-35)
     if (scheme == SCHEME_MUSL) {
       for (Index i = 0; i < nnd; ++i)
@@ -1407,6 +1426,18 @@ int orc_set_friction_constraints(orc_ctx* c, Index n, const Index* node, const i
       nd.f_dir = (unsigned)dir[i];
       nd.f_sign = sign_n[i];
       nd.f_mu = mu[i];
+    }
+  })
+}
+
+int orc_set_particle_velocity_constraints(orc_ctx* c, Index n, const Index* particle,
+                                          const int* dir, const double* value) {
+  ORC_TRY({
+    c->s.pvel.clear();
+    for (Index i = 0; i < n; ++i) {
+      if (dir[i] < 0 || dir[i] >= c->s.dim)
+        throw std::runtime_error("Invalid direction of velocity constraint");
+      c->s.pvel.push_back({particle[i], dir[i], value[i]});
     }
   })
 }

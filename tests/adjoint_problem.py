@@ -15,7 +15,7 @@ class Problem:
     def __init__(self, dim=2, scheme="usl", material="le", ncells=(6, 10), block=(3, 6),
                  h=0.05, nsteps=40, dt=1e-4, seed=3, damping=0.0, velocity_update="flip",
                  blending=1.0, ramp=0.0, acc_constraint=False, vel_amplitude=0.05,
-                 geostatic=False, tension_cutoff=0.0):
+                 geostatic=False, tension_cutoff=0.0, particle_constraints=False):
         rng = np.random.default_rng(seed)
         self.dim, self.scheme, self.nsteps, self.dt = dim, scheme, nsteps, dt
         self.damping, self.vu, self.blending, self.ramp = damping, velocity_update, blending, ramp
@@ -55,6 +55,14 @@ class Problem:
         self.v0 = vel_amplitude * rng.uniform(-1, 1, (self.np, dim))
         self.material = material
         self.tension_cutoff = tension_cutoff
+        # particle velocity constraints: x velocity of every fifth particle,
+        # y velocity of particle 1
+        self.pvc = None
+        if particle_constraints:
+            ids = np.arange(0, self.np, 5)
+            self.pvc = (np.concatenate([ids, [1]]),
+                        np.concatenate([np.zeros(len(ids), dtype=np.int32), [1]]),
+                        np.concatenate([np.full(len(ids), 0.01), [-0.02]]))
         if material == "le":
             kind = np.zeros(self.np, dtype=int)
         elif material == "mc":
@@ -101,6 +109,8 @@ class Problem:
         obj.set_particles(prm["x0"], mass=self.volume * prm["rho"], volume=self.volume,
                           density=prm["rho"], material_id=np.arange(self.np), v=prm["v0"],
                           stress=self.stress0, cell=self.cell)
+        if self.pvc is not None:
+            obj.set_particle_velocity_constraints(*self.pvc)
         return obj
 
     def loss_of(self, p):

@@ -31,6 +31,9 @@ CASES = {
     "usl_le_options": dict(scheme="usl", material="le", blending=0.8, damping=0.05,
                            ramp=0.002, acc_constraint=True),
     "usl_mc": dict(scheme="usl", material="mc", geostatic=True),
+    "musl_le_particle_constraints": dict(scheme="musl", material="le", particle_constraints=True),
+    "usf_mc_particle_constraints": dict(scheme="usf", material="mc", geostatic=True,
+                                        particle_constraints=True),
     "usf_mc": dict(scheme="usf", material="mc", geostatic=True),
     "usl_mc_apex_cutoff": dict(scheme="usl", material="mc", geostatic=True,
                                tension_cutoff=1e5),

@@ -1,8 +1,9 @@
 """Step options beyond the benchmark decks, GPU (ordered mode) against the
 oracle bit for bit: Cundall damping, PIC and blended FLIP velocity updates,
-acceleration constraints, prescribed non-zero velocities, gravity ramping,
-one particle per cell, 2D Mohr-Coulomb under every scheme; and fast mode on
-the same decks within the per-step tolerance."""
+acceleration constraints, prescribed non-zero velocities, particle velocity
+constraints, gravity ramping, one particle per cell, 2D Mohr-Coulomb under
+every scheme; and fast mode on the same decks within the per-step
+tolerance."""
 import numpy as np
 import pytest
 
@@ -26,17 +27,19 @@ def _oracle(w, **kw):
                          blending_ratio=kw.get("blend", 1.0))
 
 
-def _extra(obj, acc=None, ramp=None):
+def _extra(obj, acc=None, ramp=None, pvc=None):
     if acc is not None:
         obj.set_acceleration_constraints(*acc)
     if ramp is not None:
         obj.set_gravity(ramp[0], ramp[1])
+    if pvc is not None:
+        obj.set_particle_velocity_constraints(*pvc)
     return obj
 
 
-def _run(w, steps, damping=0.0, acc=None, ramp=None, ordered=True, **kw):
-    g = _extra(w.setup(_solver(w, ordered, **kw)), acc, ramp)
-    o = _extra(w.setup(_oracle(w, **kw)), acc, ramp)
+def _run(w, steps, damping=0.0, acc=None, ramp=None, pvc=None, ordered=True, **kw):
+    g = _extra(w.setup(_solver(w, ordered, **kw)), acc, ramp, pvc)
+    o = _extra(w.setup(_oracle(w, **kw)), acc, ramp, pvc)
     for _ in range(steps // 10):
         g.step(10, w.dt, damping)
         o.step(10, w.dt, damping)
@@ -109,3 +112,15 @@ def test_mohr_coulomb_2d(scheme):
     pg, po = _run(w, 60, scheme=scheme)
     _check(pg, po, exact=True)
     assert np.any(po["statevars"][:, 0] > 0), "no plastic state reached"
+
+
+@pytest.mark.parametrize("scheme", ["usf", "usl", "musl"])
+def test_particle_velocity_constraints(scheme):
+    """Every particle's x velocity fixed, particle 3's y velocity set twice
+    (the later constraint wins), particle 7's y velocity fixed."""
+    w = workloads.column_2d(nx=6, ny=10)
+    pvc = ([-1, 3, 7, 3], [0, 1, 1, 1], [0.001, -0.02, 0.005, -0.01])
+    pg, po = _run(w, 40, pvc=pvc, scheme=scheme)
+    _check(pg, po, exact=True)
+    assert np.all(pg["v"][:, 0] == 0.001)
+    assert pg["v"][3, 1] == -0.01 and pg["v"][7, 1] == 0.005
