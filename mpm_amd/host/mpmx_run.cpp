@@ -218,10 +218,13 @@ class MPMExplicit {
   double gravity_[3] = {0, 0, 0};
   std::vector<double> coords_;
   std::vector<int64_t> cells_;
-  std::map<int64_t, std::vector<int64_t>> node_sets_, cell_sets_;
+  std::map<int64_t, std::vector<int64_t>> node_sets_, cell_sets_, particle_sets_;
   std::vector<double> mat_params_;
   std::map<int64_t, int> mat_index_;
   std::vector<int64_t> vnode_, fnode_;
+  std::vector<int64_t> pvc_particle_;
+  std::vector<int> pvc_dir_;
+  std::vector<double> pvc_val_;
   std::vector<int> vdir_, fdir_, fsign_;
   std::vector<double> vval_, fmu_;
   int64_t expected_particles_ = 0;
@@ -300,6 +303,7 @@ void MPMExplicit<Tdim>::initialise_mesh() {
     const std::string es = io_->file_name(mp.at("entity_sets").str());
     node_sets_ = read_sets(es, "node_sets");
     cell_sets_ = read_sets(es, "cell_sets");
+    particle_sets_ = read_sets(es, "particle_sets");
   }
   if (!mp.contains("boundary_conditions") || !mp.at("boundary_conditions").is_object()) return;
   const Json& bc = mp.at("boundary_conditions");
@@ -399,8 +403,38 @@ void MPMExplicit<Tdim>::initialise_particles() {
       throw std::runtime_error("particle generator " + type + " is not supported (gauss, file)");
     }
   }
+  // particles_velocity_constraints (mpm_base.tcc:1549-1585): a particle set
+  // id (-1: all particles), a direction and a velocity; applied in order
+  if (mp.contains("boundary_conditions") && mp.at("boundary_conditions").is_object() &&
+      mp.at("boundary_conditions").contains("particles_velocity_constraints")) {
+    for (const Json& c : mp.at("boundary_conditions").at("particles_velocity_constraints").elements()) {
+      const int64_t pset = c.at("pset_id").integer();
+      const int dir = (int)c.at("dir").integer();
+      const double vel = c.at("velocity").number();
+      if (dir < 0 || dir >= (int)Tdim) throw std::runtime_error("Invalid direction of velocity constraint");
+      if (pset == -1) {
+        pvc_particle_.push_back(-1);
+        pvc_dir_.push_back(dir);
+        pvc_val_.push_back(vel);
+        continue;
+      }
+      auto it = particle_sets_.find(pset);
+      if (it == particle_sets_.end())
+        throw std::runtime_error("No particle set found to assign velocity constraint");
+      for (int64_t p : it->second) {
+        pvc_particle_.push_back(p);
+        pvc_dir_.push_back(dir);
+        pvc_val_.push_back(vel);
+      }
+    }
+  }
   if (!ctx_) return;
   check(ctx_, mpmx_compute_volume_mass(ctx_), "particle volume and mass");
+  if (!pvc_particle_.empty())
+    check(ctx_, mpmx_set_particle_velocity_constraints(ctx_, (int64_t)pvc_particle_.size(),
+                                                       pvc_particle_.data(), pvc_dir_.data(),
+                                                       pvc_val_.data()),
+          "particle velocity constraints");
   if (mp.contains("particles_stresses")) {
     const Json& ps = mp.at("particles_stresses");
     const std::string path = io_->file_name(ps.at("location").str());
@@ -451,12 +485,14 @@ void MPMExplicit<Tdim>::summary() const {
   std::printf("{\"analysis\": \"MPMExplicit%uD\", \"scheme\": \"%s\", \"velocity_update\": \"%s\", "
               "\"dt\": %.17g, \"nsteps\": %lld, \"output_steps\": %lld, \"damping\": %.17g, "
               "\"nnodes\": %lld, \"ncells\": %lld, \"node_sets\": %zu, \"cell_sets\": %zu, "
-              "\"velocity_constraints\": %zu, \"friction_constraints\": %zu, \"particles\": %lld, "
+              "\"velocity_constraints\": %zu, \"friction_constraints\": %zu, "
+              "\"particle_velocity_constraints\": %zu, \"particles\": %lld, "
               "\"gravity\": [%.17g, %.17g, %.17g], \"ramping_time\": %.17g, \"materials\": [",
               Tdim, scheme_.c_str(), vu_ == MPMX_PIC ? "pic" : "flip", dt_, (long long)nsteps_,
               (long long)output_steps_, damping_, (long long)(coords_.size() / Tdim),
               (long long)(cells_.size() / NN), node_sets_.size(), cell_sets_.size(),
-              vnode_.size(), fnode_.size(), (long long)expected_particles_, gravity_[0],
+              vnode_.size(), fnode_.size(), pvc_particle_.size(), (long long)expected_particles_,
+              gravity_[0],
               gravity_[1], gravity_[2], ramp_);
   for (size_t i = 0; i < mat_params_.size(); ++i)
     std::printf("%s%s%.17g%s", i % MPMX_MATERIAL_PARAMS == 0 ? (i ? ", [" : "[") : "",

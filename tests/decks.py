@@ -12,9 +12,11 @@ DRIVER = os.path.join(ROOT, "mpm_amd", "mpmx_run")
 
 
 def write_deck(w, d, nsteps, output_steps, constraints_as="sets", uuid="deck",
-               material_json=None, stresses=None):
+               material_json=None, stresses=None, particle_constraints=None):
     """w: a Workload; d: directory. Velocity constraints go to node sets
-    (``constraints_as="sets"``) or to a constraints file (``"file"``)."""
+    (``constraints_as="sets"``) or to a constraints file (``"file"``).
+    particle_constraints: [(particle ids or -1, dir, velocity)], written as
+    particle sets and particles_velocity_constraints."""
     os.makedirs(d, exist_ok=True)
     dim = w.dim
     with open(os.path.join(d, "mesh.txt"), "w") as f:
@@ -49,6 +51,17 @@ def write_deck(w, d, nsteps, output_steps, constraints_as="sets", uuid="deck",
         fc.append({"nset_id": sid, "dir": int(w.fric_dirs[0]), "sign_n": int(w.fric_sign[0]),
                    "friction": float(w.fric_mu[0])})
         bc["friction_constraints"] = fc
+    if particle_constraints:
+        sets["particle_sets"] = []
+        pc = []
+        for ids, k, val in particle_constraints:
+            if isinstance(ids, int) and ids == -1:
+                pc.append({"pset_id": -1, "dir": int(k), "velocity": float(val)})
+                continue
+            sid = 200 + len(sets["particle_sets"])
+            sets["particle_sets"].append({"id": sid, "set": [int(i) for i in ids]})
+            pc.append({"pset_id": sid, "dir": int(k), "velocity": float(val)})
+        bc["particles_velocity_constraints"] = pc
     with open(os.path.join(d, "entity_sets.json"), "w") as f:
         json.dump(sets, f)
     mesh = {"mesh": "mesh.txt", "entity_sets": "entity_sets.json", "io_type": "Ascii%dD" % dim,

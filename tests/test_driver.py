@@ -36,6 +36,15 @@ def test_dry_run_reads_the_deck(tmp_path, constraints_as):
     assert s["dt"] == w.dt and s["nsteps"] == 20 and s["output_steps"] == 10
     assert s["gravity"][:2] == list(w.gravity)
     np.testing.assert_array_equal(s["materials"][0], mat.from_json(LE, 2))
+    assert s["particle_velocity_constraints"] == 0
+
+
+def test_dry_run_particle_velocity_constraints(tmp_path):
+    w = workloads.column_2d(nx=6, ny=10)
+    write_deck(w, str(tmp_path), nsteps=5, output_steps=5, material_json=LE,
+               particle_constraints=[(-1, 0, 0.0), ([1, 2, 3], 1, 0.5)])
+    s = json.loads(run_driver(str(tmp_path), "--dry-run").stdout)
+    assert s["particle_velocity_constraints"] == 4
 
 
 def test_dry_run_mohr_coulomb_3d_with_friction(tmp_path):

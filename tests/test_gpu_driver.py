@@ -52,3 +52,21 @@ def test_driver_3d_mohr_coulomb_with_stresses_matches_binding(tmp_path):
     p = s0.particles()
     _compare(out, p, 3)
     assert np.array_equal(out["pdstrain"], p["statevars"][:, 8])
+
+
+def test_driver_particle_velocity_constraints_match_binding(tmp_path):
+    """particles_velocity_constraints with particle sets and pset_id -1."""
+    w = workloads.column_2d(nx=6, ny=10)
+    pcs = [(-1, 0, 0.001), ([3, 7, 11], 1, -0.01), ([7], 1, 0.002)]
+    d = str(tmp_path)
+    write_deck(w, d, nsteps=20, output_steps=20, material_json=LE, particle_constraints=pcs)
+    run_driver(d, "--ordered")
+    out = read_output(os.path.join(d, "results", "deck", "particles20.txt"), 2)
+    from mpm_amd.capi import Solver
+    s = w.setup(Solver(2, w.scheme, ordered=True))
+    s.set_particle_velocity_constraints([-1, 3, 7, 11, 7], [0, 1, 1, 1, 1],
+                                        [0.001, -0.01, -0.01, -0.01, 0.002])
+    s.step(20, w.dt)
+    p = s.particles()
+    _compare(out, p, 2)
+    assert p["v"][7, 1] == 0.002 and p["v"][3, 1] == -0.01
