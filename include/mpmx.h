@@ -128,7 +128,9 @@ int mpmx_set_gravity(mpmx_ctx* ctx, const double* g, double ramping_time);
  * Mesh::generate_material_points + Cell::generate_points, mesh.tcc:508-576,
  * cell.tcc:91-143), then located, volume = cell volume / particles in the
  * cell and mass = volume * density (particle.tcc:589-626).
- * nset < 0 means all cells. nparticles_per_dir: 1 or 2. */
+ * nset < 0 means all cells. nparticles_per_dir: 1 to 4, the reference's
+ * quadratures QQUAD1-4 / QHEX1-4 in their point order
+ * (quadrilateral_quadrature.tcc, hexahedron_quadrature.tcc). */
 int mpmx_generate_gauss(mpmx_ctx* ctx, int nparticles_per_dir, int64_t nset,
                         const int64_t* cell_set, int material_id);
 

@@ -370,6 +370,66 @@ void host_shapefn(int dim, const double* xi, double* N) {
   N[7] = 0.125 * (1 - xi[0]) * (1 + xi[1]) * (1 + xi[2]);
 }
 
+// Gauss points of the reference's quadratures, in their order
+// (quadrilateral_quadrature.tcc, hexahedron_quadrature.tcc): 1 per direction:
+// the centre; 2: the element's node order; 3: quadrilateral corners, edge
+// midpoints (-y, +x, +y, -x) and centre, hexahedron xi0 slowest and xi2
+// fastest; 4: quadrilateral xi0 slowest, hexahedron xi2 slowest, then xi0,
+// then xi1. Returns the number of points.
+int gauss_points(int d, int nppd, double (*qp)[3]) {
+  for (int i = 0; i < 64; ++i) qp[i][0] = qp[i][1] = qp[i][2] = 0.;
+  switch (nppd) {
+    case 1:
+      return 1;
+    case 2: {
+      const double g = 1. / std::sqrt(3.);
+      const int n = d == 2 ? 4 : 8;
+      for (int i = 0; i < n; ++i) {
+        const int l = i & 3;
+        qp[i][0] = (l == 0 || l == 3) ? -g : g;
+        qp[i][1] = l < 2 ? -g : g;
+        if (d == 3) qp[i][2] = i < 4 ? -g : g;
+      }
+      return n;
+    }
+    case 3: {
+      const double a = std::sqrt(3. / 5.);
+      const double c3[3] = {-a, 0., a};
+      if (d == 2) {
+        // corners in node order, then the edge midpoints, then the centre
+        const int ix[9] = {0, 2, 2, 0, 1, 2, 1, 0, 1};
+        const int iy[9] = {0, 0, 2, 2, 0, 1, 2, 1, 1};
+        for (int q = 0; q < 9; ++q) {
+          qp[q][0] = c3[ix[q]];
+          qp[q][1] = c3[iy[q]];
+        }
+        return 9;
+      }
+      for (int q = 0; q < 27; ++q) {
+        qp[q][0] = c3[q / 9];
+        qp[q][1] = c3[(q / 3) % 3];
+        qp[q][2] = c3[q % 3];
+      }
+      return 27;
+    }
+    case 4: {
+      const double s65 = std::sqrt(6. / 5.);
+      const double a = std::sqrt(3. / 7. - 2. / 7. * s65);
+      const double b = std::sqrt(3. / 7. + 2. / 7. * s65);
+      const double g4[4] = {-b, -a, a, b};
+      const int n = d == 2 ? 16 : 64;
+      for (int q = 0; q < n; ++q) {
+        qp[q][0] = g4[(q / 4) % 4];
+        qp[q][1] = g4[q % 4];
+        if (d == 3) qp[q][2] = g4[q / 16];
+      }
+      return n;
+    }
+    default:
+      throw ApiError(MPMX_E_UNSUPPORTED, "gauss generator supports nparticles_per_dir 1 to 4");
+  }
+}
+
 // ----------------------------- materials ------------------------------------
 Material make_material(const double* p) {
   Material m;
@@ -1365,26 +1425,8 @@ int mpmx_generate_gauss(mpmx_ctx* c, int nppd, int64_t nset, const int64_t* cset
   require_mesh(c);
   if (material_id < 0 || material_id >= (int)c->mats.size()) throw ApiError(MPMX_E_ARG, "bad material id");
   const int d = c->dim, nn = c->nn;
-  double qp[8][3] = {{0}};
-  int nq = 0;
-  const double g = 1. / std::sqrt(3.);
-  if (nppd == 1) {
-    nq = 1;
-  } else if (nppd == 2) {
-    if (d == 2) {  // quadrilateral_quadrature.tcc:22-38
-      const double s[4][2] = {{-g, -g}, {g, -g}, {g, g}, {-g, g}};
-      nq = 4;
-      for (int i = 0; i < 4; ++i) { qp[i][0] = s[i][0]; qp[i][1] = s[i][1]; }
-    } else {  // hexahedron_quadrature.tcc:21-57
-      const double s[8][3] = {{-g, -g, -g}, {g, -g, -g}, {g, g, -g}, {-g, g, -g},
-                              {-g, -g, g},  {g, -g, g},  {g, g, g},  {-g, g, g}};
-      nq = 8;
-      for (int i = 0; i < 8; ++i)
-        for (int k = 0; k < 3; ++k) qp[i][k] = s[i][k];
-    }
-  } else {
-    throw ApiError(MPMX_E_UNSUPPORTED, "gauss generator supports nparticles_per_dir 1 or 2");
-  }
+  double qp[64][3];
+  const int nq = gauss_points(d, nppd, qp);
   const int64_t ncs = nset < 0 ? c->ncells : nset;
   std::vector<double> xs;
   std::vector<int> hint;

@@ -896,30 +896,76 @@ struct Solver {
 
   // mesh.tcc:508-576 generate_material_points + cell.tcc:91-143 generate_points
   // + particle.tcc:589-626 compute_volume / compute_mass.
+  // Gauss points of the reference's quadratures in their order
+  // (quadrilateral_quadrature.tcc, hexahedron_quadrature.tcc):
+  //  1: centre; 2: the element's node order; 3 (quad): corners, edge
+  //  midpoints (-y, +x, +y, -x), centre; 3 (hex): xi0 slowest, xi2 fastest;
+  //  4 (quad): xi0 slowest; 4 (hex): xi2 slowest, then xi0, then xi1.
+  int gauss_points(int nppd, double (*qp)[3]) const {
+    for (int i = 0; i < 64; ++i) qp[i][0] = qp[i][1] = qp[i][2] = 0.;
+    if (nppd == 1) return 1;
+    if (nppd == 2) {
+      const double g = 1. / std::sqrt(3.);
+      const int sg[8][3] = {{-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1},
+                            {-1, -1, 1},  {1, -1, 1},  {1, 1, 1},  {-1, 1, 1}};
+      const int n = dim == 2 ? 4 : 8;
+      for (int i = 0; i < n; ++i)
+        for (int k = 0; k < dim; ++k) qp[i][k] = sg[i][k] * g;
+      return n;
+    }
+    if (nppd == 3) {
+      const double a = std::sqrt(3. / 5.);
+      const double c[3] = {-a, 0., a};
+      if (dim == 2) {
+        const int idx[9][2] = {{0, 0}, {2, 0}, {2, 2}, {0, 2}, {1, 0},
+                               {2, 1}, {1, 2}, {0, 1}, {1, 1}};
+        for (int i = 0; i < 9; ++i) {
+          qp[i][0] = c[idx[i][0]];
+          qp[i][1] = c[idx[i][1]];
+        }
+        return 9;
+      }
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+          for (int k = 0; k < 3; ++k) {
+            const int q = 9 * i + 3 * j + k;
+            qp[q][0] = c[i];
+            qp[q][1] = c[j];
+            qp[q][2] = c[k];
+          }
+      return 27;
+    }
+    if (nppd == 4) {
+      const double s65 = std::sqrt(6. / 5.);
+      const double a = std::sqrt(3. / 7. - 2. / 7. * s65);
+      const double b = std::sqrt(3. / 7. + 2. / 7. * s65);
+      const double g[4] = {-b, -a, a, b};
+      if (dim == 2) {
+        for (int i = 0; i < 4; ++i)
+          for (int j = 0; j < 4; ++j) {
+            qp[4 * i + j][0] = g[i];
+            qp[4 * i + j][1] = g[j];
+          }
+        return 16;
+      }
+      for (int k = 0; k < 4; ++k)
+        for (int i = 0; i < 4; ++i)
+          for (int j = 0; j < 4; ++j) {
+            const int q = 16 * k + 4 * i + j;
+            qp[q][0] = g[i];
+            qp[q][1] = g[j];
+            qp[q][2] = g[k];
+          }
+      return 64;
+    }
+    throw std::runtime_error("gauss generator supports nparticles_per_dir 1 to 4");
+  }
+
   void generate_gauss(int nppd, Index nset, const Index* cset, int mat) {
     if (mat < 0 || mat >= (int)materials.size()) throw std::runtime_error("bad material id");
     const Index before = (Index)particles.size();
-    double qp[8][3];
-    int nq = 0;
-    const double g = 1. / std::sqrt(3.);
-    if (nppd == 1) {
-      nq = 1;
-      qp[0][0] = qp[0][1] = qp[0][2] = 0.;
-    } else if (nppd == 2) {
-      if (dim == 2) {  // quadrilateral_quadrature.tcc:22-38
-        const double s[4][2] = {{-g, -g}, {g, -g}, {g, g}, {-g, g}};
-        nq = 4;
-        for (int i = 0; i < 4; ++i) { qp[i][0] = s[i][0]; qp[i][1] = s[i][1]; qp[i][2] = 0; }
-      } else {  // hexahedron_quadrature.tcc:21-57
-        const double s[8][3] = {{-g, -g, -g}, {g, -g, -g}, {g, g, -g}, {-g, g, -g},
-                                {-g, -g, g},  {g, -g, g},  {g, g, g},  {-g, g, g}};
-        nq = 8;
-        for (int i = 0; i < 8; ++i)
-          for (int k = 0; k < 3; ++k) qp[i][k] = s[i][k];
-      }
-    } else {
-      throw std::runtime_error("gauss generator supports nparticles_per_dir 1 or 2");
-    }
+    double qp[64][3];
+    const int nq = gauss_points(nppd, qp);
     const Index nc = nset < 0 ? (Index)cells.size() : nset;
     for (Index ic = 0; ic < nc; ++ic) {
       const Index c = nset < 0 ? ic : cset[ic];

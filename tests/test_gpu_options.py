@@ -124,3 +124,19 @@ def test_particle_velocity_constraints(scheme):
     _check(pg, po, exact=True)
     assert np.all(pg["v"][:, 0] == 0.001)
     assert pg["v"][3, 1] == -0.01 and pg["v"][7, 1] == 0.005
+
+
+@pytest.mark.parametrize("dim,nppd", [(2, 1), (2, 3), (2, 4), (3, 3), (3, 4)])
+def test_gauss_generator_counts(dim, nppd):
+    """1 to 4 particles per direction: positions, volumes, masses and cells
+    as the oracle generates them, and 10 ordered steps bit for bit."""
+    if dim == 2:
+        w = workloads.column_2d(nx=4, ny=6)
+    else:
+        w = workloads.collapse_3d(nx=6, ny=3, nz=4, block=(3, 3, 3))
+        w.velocity_seed = None
+        w.materials = mat.linear_elastic(5e6, 0.3, 1800.0)[None, :]
+    w.nppd = nppd
+    pg, po = _run(w, 10, scheme=w.scheme)
+    assert len(pg["x"]) == len(w.cell_set if w.cell_set is not None else w.cells) * nppd ** dim
+    _check(pg, po, exact=True)

@@ -234,3 +234,51 @@ def test_cell_crossing_2d(kats):
     cells = o.particles()["cell"]
     assert list(np.bincount(cells, minlength=2)) == c["cell_counts_after"]
     assert list(cells) == [0, 1, 1, 0]
+
+
+def _gauss_order(dim, nppd):
+    """Expected Gauss points (local coordinates) in the reference's order,
+    from numpy's Gauss-Legendre nodes."""
+    import itertools
+    g = np.polynomial.legendre.leggauss(nppd)[0]
+    if nppd == 1:
+        return np.zeros((1, dim))
+    if nppd == 2:
+        s = [(-1, -1, -1), (1, -1, -1), (1, 1, -1), (-1, 1, -1),
+             (-1, -1, 1), (1, -1, 1), (1, 1, 1), (-1, 1, 1)][:2 ** dim]
+        return np.array([[v * g[1] for v in t[:dim]] for t in s])
+    if nppd == 3 and dim == 2:
+        idx = [(0, 0), (2, 0), (2, 2), (0, 2), (1, 0), (2, 1), (1, 2), (0, 1), (1, 1)]
+        return np.array([[g[i], g[j]] for i, j in idx])
+    if nppd == 3:
+        return np.array([[g[i], g[j], g[k]] for i, j, k in itertools.product(range(3), repeat=3)])
+    if dim == 2:
+        return np.array([[g[i], g[j]] for i, j in itertools.product(range(4), repeat=2)])
+    return np.array([[g[i], g[j], g[k]] for k, i, j in itertools.product(range(4), repeat=3)])
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("nppd", [1, 2, 3, 4])
+def test_gauss_generator_points_and_order(dim, nppd):
+    """Particles of the "gauss" generator on a 2 x ... cell mesh: per cell the
+    reference's quadrature points (QQUAD1-4 / QHEX1-4) in their order, each
+    volume = cell volume / points per cell (quadrilateral_quadrature.tcc,
+    hexahedron_quadrature.tcc, cell.tcc:91-143, particle.tcc:589-595)."""
+    from mpm_amd.mesh import structured_mesh
+    h = np.array([0.5, 0.25, 0.125][:dim])
+    n = (2,) + (1,) * (dim - 1)
+    coords, cells = structured_mesh((0.0,) * dim, tuple(h), n)
+    o = oracle.Oracle(dim, "usf")
+    o.set_mesh(coords, cells)
+    o.set_materials(materials.linear_elastic(1e6, 0.3, 1000.0)[None, :])
+    o.generate_gauss(nppd, None, 0)
+    p = o.particles(["x", "volume", "cell"])
+    per = nppd ** dim
+    assert len(p["x"]) == 2 * per
+    xi = _gauss_order(dim, nppd)
+    for c in range(2):
+        lo = np.array([c * h[0]] + [0.0] * (dim - 1))
+        want = lo + (xi + 1.0) * 0.5 * h
+        np.testing.assert_allclose(p["x"][c * per:(c + 1) * per], want, rtol=0, atol=1e-15)
+        assert np.all(p["cell"][c * per:(c + 1) * per] == c)
+    np.testing.assert_allclose(p["volume"], np.prod(h) / per, rtol=1e-15)
