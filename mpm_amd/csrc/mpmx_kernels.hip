@@ -191,8 +191,13 @@ MPMX_DEV void stress_strain(const Mesh& M, const Particles& P, const Material* m
     for (int k = 0; k < 6; ++k) s[k] = s[k] + t[k];
   } else {
     mc_stress(m, s, ds, rho, sv);
+    // friction, dilation, cohesion and tension cutoff change only with
+    // softening (mohr_coulomb.tcc:360-384); the others are rewritten
 #pragma unroll
-    for (int k = 0; k < NSV; ++k) P.sv[k * np + p] = sv[k];
+    for (int k = 0; k < NSV; ++k) {
+      const bool strength = k == SV_PHI || k == SV_PSI || k == SV_COH || k == SV_TCUT;
+      if (!strength || m.softening) P.sv[k * np + p] = sv[k];
+    }
   }
 #pragma unroll
   for (int k = 0; k < 6; ++k) P.stress[k * np + p] = s[k];
