@@ -310,6 +310,21 @@ MPMX_DEV McAngles mc_angles(double phi, double psi) {
   return a;
 }
 
+// The angles' trigonometry for state-variable angles phi, psi: the
+// material's precomputed values when they are its peak angles (always
+// without softening), the same bits as mc_angles(phi, psi).
+MPMX_DEV McAngles mc_angles_of(const Material& m, double phi, double psi) {
+  if (phi == m.phi_peak && psi == m.psi_peak) {
+    McAngles a;
+    a.sphi = m.sphi;
+    a.cphi = m.cphi;
+    a.tphi = m.tphi;
+    a.tpsi = m.tpsi;
+    return a;
+  }
+  return mc_angles(phi, psi);
+}
+
 // Haigh-Westergaard invariants (the state variables epsilon, rho, theta)
 // plus the trigonometry of the Lode angle.
 struct McInv {
@@ -502,7 +517,7 @@ MPMX_DEV bool mc_begin(const Material& m, double* stress, const double* dstrain,
     for (int i = 0; i < 6; ++i) stress[i] = 0.;
     return true;
   }
-  const McAngles A = mc_angles(sv[SV_PHI], sv[SV_PSI]);
+  const McAngles A = mc_angles_of(m, sv[SV_PHI], sv[SV_PSI]);
   mc_invariants(trial, T.I);
   double yf[2];
   const int yt = mc_yield(yf, T.I, A, sv[SV_COH], sv[SV_TCUT]);
@@ -538,7 +553,7 @@ MPMX_DEV void mc_correct(const Material& m, double lam, const double* dp, double
 MPMX_DEV void mc_first(const Material& m, const double* stress, const double* trial,
                        const double* dstrain, const double* sv, const McTrial& T, McIter& R) {
   const double Tolerance = 1E-1;
-  const McAngles A = mc_angles(sv[SV_PHI], sv[SV_PSI]);
+  const McAngles A = mc_angles_of(m, sv[SV_PHI], sv[SV_PSI]);
   McInv Ic;
   mc_invariants(stress, Ic);
   double yc[2];
@@ -570,7 +585,7 @@ MPMX_DEV void mc_first(const Material& m, const double* stress, const double* tr
 // to a yield check that a later call repeats (the check is pure).
 MPMX_DEV bool mc_iterate(const Material& m, double* stress, double* sv, McIter& R, int max_corr) {
   const double Tolerance = 1E-1;
-  const McAngles A = mc_angles(sv[SV_PHI], sv[SV_PSI]);
+  const McAngles A = mc_angles_of(m, sv[SV_PHI], sv[SV_PSI]);
   const double cohesion = sv[SV_COH], tcut = sv[SV_TCUT], pdstrain = sv[SV_PDS];
   McInv I;
   bool converged = false;

@@ -129,6 +129,8 @@ struct mpmx_ctx {
 
   // step machinery
   DBuf<int> d_skey, d_perm, d_sort_tmp, d_scan_list, d_cstart, d_cend;
+  DBuf<int> d_plastic_list;     // deferred Mohr-Coulomb return mappings
+  DBuf<double> d_plastic_rec;
   DBuf<Status> d_status;
   Status* h_status = nullptr;  // pinned
   double g[3] = {0, 0, 0};
@@ -215,6 +217,8 @@ struct mpmx_ctx {
     P.cell = d_cell.p;
     P.pvc_mask = pvc.empty() ? nullptr : d_pvc_mask.p;
     P.pvc_val = pvc.empty() ? nullptr : d_pvc_val.p;
+    P.plastic_list = any_mc ? d_plastic_list.p : nullptr;
+    P.plastic_rec = any_mc ? d_plastic_rec.p : nullptr;
     return P;
   }
   Nodes nodes() const {
@@ -467,6 +471,9 @@ Material make_material(const double* p) {
   m.a1 = K + (4.0 / 3.0) * m.shear;
   m.a2 = K - (2.0 / 3.0) * m.shear;
   m.G = m.shear;
+  dmath::sincos(m.phi_peak, &m.sphi, &m.cphi);
+  m.tphi = m.sphi / m.cphi;
+  m.tpsi = dmath::tan(m.psi_peak);
   // initialise_state_variables (mohr_coulomb.tcc:85-111)
   m.sv0[SV_YIELD] = 0;
   m.sv0[SV_PHI] = m.phi_peak;
@@ -604,6 +611,10 @@ void alloc_step_buffers(mpmx_ctx* c) {
   c->d_perm.alloc(std::max<int64_t>(np, 1));
   c->d_sort_tmp.alloc(radix_sort_temp_ints(std::max<int64_t>(np, 1)));
   c->d_scan_list.alloc(std::max<int64_t>(np, 1));
+  if (c->any_mc) {
+    c->d_plastic_list.alloc(std::max<int64_t>(np, 1));
+    c->d_plastic_rec.alloc(PR_N * std::max<int64_t>(np, 1));
+  }
   c->need_sort = true;
   if (c->flags & MPMX_ORDERED) {
     c->d_cstart.alloc(std::max<int64_t>(c->ncells, 1));
@@ -617,6 +628,8 @@ void reset_status(mpmx_ctx* c) {
   s.first_bad_step = std::numeric_limits<int64_t>::max();
   s.bad_particle = -1;
   s.nscan = 0;
+  s.nplastic = 0;
+  s.nplastic_t = 0;
   *c->h_status = s;
   c->d_status.upload(c->h_status, 1, c->stream);
 }
@@ -796,6 +809,10 @@ void enqueue_step(mpmx_ctx* c, const StepArgs& A, Launcher& L, bool full_sort) {
   int* perm = c->d_perm.p;
   int* scan = c->d_scan_list.p;
   const bool ordered = (c->flags & MPMX_ORDERED) != 0;
+  // the Mohr-Coulomb return mappings deferred by the stress kernels
+  auto plastic = [&] {
+    if (c->any_mc) L("mc_plastic", [&] { launch_plastic(P, mats, S, st); });
+  };
   if (full_sort || ordered)
     L("sort_by_cell", [&] {
       radix_sort_by_cell(P.cell, P.np, (int)M.ncells, skey, perm, c->d_sort_tmp.p, st);
@@ -811,16 +828,19 @@ void enqueue_step(mpmx_ctx* c, const StepArgs& A, Launcher& L, bool full_sort) {
     if (c->scheme == USF) {
       L("gather_mass_momentum", [&] { launch_gather<D>(GK_MV_VEL, M, P, Nd, A, S, perm, cs, ce, st); });
       L("stress", [&] { launch_stress<D>(false, M, P, Nd, mats, A, S, scan, st); });
+      plastic();
       L("gather_forces", [&] { launch_gather<D>(GK_F_ACC, M, P, Nd, A, S, perm, cs, ce, st); });
       L("g2p_locate", [&] { launch_g2p<D>(false, true, M, P, Nd, mats, A, c->vu, S, scan, st); });
     } else if (c->scheme == USL) {
       L("gather_all", [&] { launch_gather<D>(GK_ALL_VEL_ACC, M, P, Nd, A, S, perm, cs, ce, st); });
       L("g2p_stress_locate", [&] { launch_g2p<D>(true, true, M, P, Nd, mats, A, c->vu, S, scan, st); });
+      plastic();
     } else {
       L("gather_all", [&] { launch_gather<D>(GK_ALL_VEL_ACC, M, P, Nd, A, S, perm, cs, ce, st); });
       L("g2p", [&] { launch_g2p<D>(false, false, M, P, Nd, mats, A, c->vu, S, scan, st); });
       L("gather_momentum", [&] { launch_gather<D>(GK_MOM_VEL, M, P, Nd, A, S, perm, cs, ce, st); });
       L("stress_locate", [&] { launch_stress<D>(true, M, P, Nd, mats, A, S, scan, st); });
+      plastic();
     }
   } else {
     if (c->scheme == USF) {
@@ -833,12 +853,14 @@ void enqueue_step(mpmx_ctx* c, const StepArgs& A, Launcher& L, bool full_sort) {
       L("p2g_all", [&] { launch_p2g<D>(P2G_MV_F, c->chunk, M, P, Nd, A, S, skey, perm, st); });
       L("node_update", [&] { launch_node<D>(ND_INIT | ND_VEL | ND_ACC | ND_ZERO_ALL, M, Nd, A, st); });
       L("g2p_stress_locate", [&] { launch_g2p<D>(true, true, M, P, Nd, mats, A, c->vu, S, scan, st); });
+      plastic();
     } else {
       L("p2g_all", [&] { launch_p2g<D>(P2G_MV_F, c->chunk, M, P, Nd, A, S, skey, perm, st); });
       L("node_update", [&] { launch_node<D>(ND_INIT | ND_VEL | ND_ACC | ND_ZERO_MOM, M, Nd, A, st); });
       L("g2p_momentum_scatter", [&] { launch_g2p_scatter<D>(c->chunk, M, P, Nd, A, c->vu, skey, perm, st); });
       L("node_velocity", [&] { launch_node<D>(ND_VEL | ND_ZERO_ALL, M, Nd, A, st); });
       L("stress_locate", [&] { launch_stress<D>(true, M, P, Nd, mats, A, S, scan, st); });
+      plastic();
     }
   }
   L("locate_scan", [&] { launch_locate_scan<D>(M, P, S, scan, nullptr, st); });

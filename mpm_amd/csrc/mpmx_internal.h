@@ -28,6 +28,9 @@ struct Material {
   double E;                 // Young's modulus (the elastic constants are proportional to it)
   double dtcut_dphi;        // d(initial tension cutoff)/d(friction angle): -c/sin^2(phi)
                             // when the cutoff is the apex c/tan(phi), else 0
+  // sin, cos, tan of the peak friction angle and tan of the peak dilation
+  // angle, by the same functions as the per-particle evaluation
+  double sphi, cphi, tphi, tpsi;
 };
 
 // Nodal constraint record, referenced by Nodes::cons[node] (-1: none).
@@ -77,7 +80,18 @@ struct Particles {
   // sets v_k = pvc_val[k][p] after the velocity update. Null when none.
   const unsigned char* pvc_mask;
   const double* pvc_val;
+  // Deferred Mohr-Coulomb return mapping: the particles whose trial stress
+  // yields this step (plastic_list[0, Status::nplastic) in shear,
+  // plastic_list[np - Status::nplastic_t, np) in tension) and their records
+  // (PR_* below), by list slot. Null when no MohrCoulomb material.
+  int* plastic_list;
+  double* plastic_rec;  // [PR_N][np] by list slot
 };
+
+// Record of a deferred return mapping: the strain increment and what the
+// correction needs from the trial state (its invariants with the Lode-angle
+// trigonometry and its yield function value).
+enum { PR_DS = 0, PR_EPS = 6, PR_RHO, PR_THETA, PR_ST, PR_CT, PR_S3, PR_C3, PR_YTRIAL, PR_N };
 
 struct Nodes {
   double* mass;   // [nnodes]
@@ -111,6 +125,8 @@ struct Status {
   int64_t first_bad_step;  // first step with an unlocatable particle (INT64_MAX: none)
   int bad_particle;        // one such particle (pid)
   int nscan;               // particles needing the full-mesh scan this step
+  int nplastic;            // shear entries of the deferred plastic list (its front)
+  int nplastic_t;          // tensile entries (its back)
 };
 
 }  // namespace mpmx

@@ -34,6 +34,10 @@ template <int D>
 void launch_stress(bool locate, const Mesh& M, const Particles& P, const Nodes& Nd,
                    const Material* mats, const StepArgs& A, Status* S, int* scan_list,
                    hipStream_t st);
+// The deferred Mohr-Coulomb return mappings queued by launch_stress and
+// launch_g2p (stress) this step; runs after them, before anything reads the
+// stress or state variables.
+void launch_plastic(const Particles& P, const Material* mats, const Status* S, hipStream_t st);
 template <int D>
 void launch_g2p(bool stress, bool locate, const Mesh& M, const Particles& P, const Nodes& Nd,
                 const Material* mats, const StepArgs& A, int vu, Status* S, int* scan_list,

@@ -32,6 +32,18 @@ static constexpr int kBlock = 256;
 #define MPMX_MC_ATTR
 #endif
 
+// Occupancy floor of the kernels that run the kinematics and the elastic
+// predictor and defer plastic return mappings (k_stress, k_g2p).
+#ifndef MPMX_PRED_WAVES
+#define MPMX_PRED_WAVES 2
+#endif
+#define MPMX_PRED_ATTR __attribute__((amdgpu_waves_per_eu(MPMX_PRED_WAVES)))
+// ... and of the deferred return mappings (k_plastic)
+#ifndef MPMX_PLASTIC_WAVES
+#define MPMX_PLASTIC_WAVES 2
+#endif
+#define MPMX_PLASTIC_ATTR __attribute__((amdgpu_waves_per_eu(MPMX_PLASTIC_WAVES)))
+
 MPMX_DEV void atomic_add(double* p, double v) { unsafeAtomicAdd(p, v); }
 
 template <int D>
@@ -166,12 +178,59 @@ MPMX_DEV void kinematics(const Mesh& M, const Particles& P, int p, int c, const 
   }
 }
 
+// Mohr-Coulomb state variables written back after an update: friction,
+// dilation, cohesion and tension cutoff change only with softening
+// (mohr_coulomb.tcc:360-384); the others are rewritten.
+MPMX_DEV void store_sv(const Particles& P, const Material& m, int64_t p, const double* sv) {
+  const int64_t np = P.np;
+#pragma unroll
+  for (int k = 0; k < NSV; ++k) {
+    const bool strength = k == SV_PHI || k == SV_PSI || k == SV_COH || k == SV_TCUT;
+    if (!strength || m.softening) P.sv[k * np + p] = sv[k];
+  }
+}
+
+// Appends particle p to the deferred plastic list with its record: strain
+// increment and trial state. Shear returns fill the list from the front,
+// tensile ones from the back, so that k_plastic's waves mostly run one
+// branch of the flow rule (one atomic per wave and kind on the counts).
+MPMX_DEV void defer_plastic(const Particles& P, Status* S, int p, const double* ds,
+                            const McTrial& T) {
+  const bool tensile = T.yt == FS_TENSILE;
+  const unsigned long long mask = __ballot(tensile);
+  const unsigned long long all = __ballot(1);
+  const unsigned long long mine = tensile ? mask : (all & ~mask);
+  const int lane = threadIdx.x & 63;
+  const int leader = __ffsll((long long)mine) - 1;
+  int base = 0;
+  if (lane == leader) base = atomicAdd(tensile ? &S->nplastic_t : &S->nplastic, __popcll(mine));
+  base = __shfl(base, leader, 64);
+  const int k = base + __popcll(mine & ((1ull << lane) - 1ull));
+  const int64_t np = P.np;
+  const int slot = tensile ? (int)(np - 1 - k) : k;
+  P.plastic_list[slot] = p;
+  double* r = P.plastic_rec + slot;
+#pragma unroll
+  for (int c = 0; c < 6; ++c) r[(PR_DS + c) * np] = ds[c];
+  r[PR_EPS * np] = T.I.eps;
+  r[PR_RHO * np] = T.I.rho;
+  r[PR_THETA * np] = T.I.theta;
+  r[PR_ST * np] = T.I.st;
+  r[PR_CT * np] = T.I.ct;
+  r[PR_S3 * np] = T.I.s3;
+  r[PR_C3 * np] = T.I.c3;
+  r[PR_YTRIAL * np] = T.yield_trial;
+}
+
 // compute_stress_strain for one particle: kinematics, then the constitutive
 // update (particle.tcc:880-890 -> material). Returns the new stress in `s`.
-template <int D>
+// DEFER: a Mohr-Coulomb particle whose trial stress yields is appended to
+// the plastic list instead (stress and state variables left as they were,
+// `s` not the new stress); k_plastic completes it with the same inputs.
+template <int D, bool DEFER = false>
 MPMX_DEV void stress_strain(const Mesh& M, const Particles& P, const Material* mats, int p, int c,
                             const double (*dndx)[D], const double (*nv)[D], double dt,
-                            double* s, double* vol_out) {
+                            double* s, double* vol_out, Status* S = nullptr) {
   const int64_t np = P.np;
   double ds[6], rho;
   kinematics<D>(M, P, p, c, dndx, nv, dt, ds, &rho, vol_out);
@@ -189,15 +248,26 @@ MPMX_DEV void stress_strain(const Mesh& M, const Particles& P, const Material* m
     de_mul(m, ds, t);
 #pragma unroll
     for (int k = 0; k < 6; ++k) s[k] = s[k] + t[k];
+  } else if constexpr (DEFER) {
+    double trial[6];
+    McTrial T;
+    if (!mc_begin(m, s, ds, rho, sv, trial, T)) {
+      // the yield state (and softened strength parameters) now, the rest of
+      // the state variables and the stress in k_plastic
+      P.sv[SV_YIELD * np + p] = sv[SV_YIELD];
+      if (m.softening) {
+        P.sv[SV_PHI * np + p] = sv[SV_PHI];
+        P.sv[SV_PSI * np + p] = sv[SV_PSI];
+        P.sv[SV_COH * np + p] = sv[SV_COH];
+        P.sv[SV_TCUT * np + p] = sv[SV_TCUT];
+      }
+      defer_plastic(P, S, p, ds, T);
+      return;
+    }
+    store_sv(P, m, p, sv);
   } else {
     mc_stress(m, s, ds, rho, sv);
-    // friction, dilation, cohesion and tension cutoff change only with
-    // softening (mohr_coulomb.tcc:360-384); the others are rewritten
-#pragma unroll
-    for (int k = 0; k < NSV; ++k) {
-      const bool strength = k == SV_PHI || k == SV_PSI || k == SV_COH || k == SV_TCUT;
-      if (!strength || m.softening) P.sv[k * np + p] = sv[k];
-    }
+    store_sv(P, m, p, sv);
   }
 #pragma unroll
   for (int k = 0; k < 6; ++k) P.stress[k * np + p] = s[k];
@@ -326,6 +396,8 @@ __global__ void k_step_end(Status* S) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     S->step += 1;
     S->nscan = 0;
+    S->nplastic = 0;
+    S->nplastic_t = 0;
   }
 }
 
@@ -638,7 +710,7 @@ __global__ __launch_bounds__(CH) MPMX_MC_ATTR void k_stress_scatter(Mesh M, Part
 // Stress update in particle order (USL/MUSL postcompute, and USF in ordered
 // mode); optionally locate afterwards.
 template <int D, bool LOCATE>
-__global__ __launch_bounds__(kBlock) MPMX_MC_ATTR void k_stress(Mesh M, Particles P, Nodes Nd,
+__global__ __launch_bounds__(kBlock) MPMX_PRED_ATTR void k_stress(Mesh M, Particles P, Nodes Nd,
                                                    const Material* mats, StepArgs A, Status* S,
                                                    int* scan_list) {
   constexpr int NN = Elem<D>::NN;
@@ -654,7 +726,7 @@ __global__ __launch_bounds__(kBlock) MPMX_MC_ATTR void k_stress(Mesh M, Particle
   particle_dndx<D>(M, c, xi, dndx, Ji);
   gather_nodes<D>(M, Nd, c, nv, nullptr);
   double s[6], vol;
-  stress_strain<D>(M, P, mats, (int)p, c, dndx, nv, A.dt, s, &vol);
+  stress_strain<D, true>(M, P, mats, (int)p, c, dndx, nv, A.dt, s, &vol, S);
   if constexpr (LOCATE) {
     double x[D];
 #pragma unroll
@@ -666,7 +738,7 @@ __global__ __launch_bounds__(kBlock) MPMX_MC_ATTR void k_stress(Mesh M, Particle
 // G2P: compute_updated_position (particle.tcc:994-1070) in particle order,
 // optionally followed by the USL stress update and by locate.
 template <int D, bool STRESS, bool LOCATE>
-__global__ __launch_bounds__(kBlock) MPMX_MC_ATTR void k_g2p(Mesh M, Particles P, Nodes Nd,
+__global__ __launch_bounds__(kBlock) MPMX_PRED_ATTR void k_g2p(Mesh M, Particles P, Nodes Nd,
                                                 const Material* mats, StepArgs A, int vu,
                                                 Status* S, int* scan_list) {
   constexpr int NN = Elem<D>::NN;
@@ -719,9 +791,64 @@ __global__ __launch_bounds__(kBlock) MPMX_MC_ATTR void k_g2p(Mesh M, Particles P
     double dndx[NN][D], Ji[D][D];
     particle_dndx<D>(M, c, xi, dndx, Ji);
     double s[6], vol;
-    stress_strain<D>(M, P, mats, (int)p, c, dndx, nv, A.dt, s, &vol);
+    stress_strain<D, true>(M, P, mats, (int)p, c, dndx, nv, A.dt, s, &vol, S);
   }
   if constexpr (LOCATE) locate_one<D>(M, P, S, scan_list, (int)p, x, c, -1);
+}
+
+// The deferred Mohr-Coulomb return mappings of this step: the rest of
+// compute_stress (mohr_coulomb.tcc:412-512) after the elastic predictor of
+// stress_strain, i.e. mc_first and mc_iterate on the same start-of-step
+// stress, state variables and strain increment, with the trial state from
+// the list record. Grid-stride loop over the shear entries (front of the
+// list), then the tensile ones (back).
+__global__ __launch_bounds__(kBlock) MPMX_PLASTIC_ATTR void k_plastic(Particles P,
+                                                                      const Material* mats,
+                                                                      const Status* S) {
+  const int ns = S->nplastic, nt = S->nplastic_t;
+  const int64_t np = P.np;
+  for (int c = blockIdx.x * kBlock + threadIdx.x; c < ns + nt; c += gridDim.x * kBlock) {
+    const int slot = c < ns ? c : (int)(np - 1 - (c - ns));
+    const int p = P.plastic_list[slot];
+    const double* r = P.plastic_rec + slot;
+    double ds[6], s[6], sv[NSV];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) ds[k] = r[(PR_DS + k) * np];
+    McTrial T;
+    T.I.eps = r[PR_EPS * np];
+    T.I.rho = r[PR_RHO * np];
+    T.I.theta = r[PR_THETA * np];
+    T.I.st = r[PR_ST * np];
+    T.I.ct = r[PR_CT * np];
+    T.I.s3 = r[PR_S3 * np];
+    T.I.c3 = r[PR_C3 * np];
+    T.yield_trial = r[PR_YTRIAL * np];
+    T.yt = c < ns ? FS_SHEAR : FS_TENSILE;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) s[k] = P.stress[k * np + p];
+    sv[SV_PHI] = P.sv[SV_PHI * np + p];
+    sv[SV_PSI] = P.sv[SV_PSI * np + p];
+    sv[SV_COH] = P.sv[SV_COH * np + p];
+    sv[SV_TCUT] = P.sv[SV_TCUT * np + p];
+    sv[SV_PDS] = P.sv[SV_PDS * np + p];
+    const Material& m = mats[P.mat[p]];
+    double trial[6];
+    {
+      double t[6];
+      de_mul(m, ds, t);
+#pragma unroll
+      for (int k = 0; k < 6; ++k) trial[k] = s[k] + t[k];
+    }
+    McIter R;
+    mc_first(m, s, trial, ds, sv, T, R);
+    mc_iterate(m, s, sv, R, 100);
+    P.sv[SV_EPS * np + p] = sv[SV_EPS];
+    P.sv[SV_THETA * np + p] = sv[SV_THETA];
+    P.sv[SV_RHO * np + p] = sv[SV_RHO];
+    P.sv[SV_PDS * np + p] = sv[SV_PDS];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) P.stress[k * np + p] = s[k];
+  }
 }
 
 // MUSL G2P in sorted order with the momentum of the updated particle
@@ -1138,6 +1265,13 @@ void launch_stress(bool locate, const Mesh& M, const Particles& P, const Nodes& 
     MPMX_LAUNCH((k_stress<D, true>), P.np, st, M, P, Nd, mats, A, S, scan_list);
   else
     MPMX_LAUNCH((k_stress<D, false>), P.np, st, M, P, Nd, mats, A, S, scan_list);
+}
+
+void launch_plastic(const Particles& P, const Material* mats, const Status* S, hipStream_t st) {
+  // enough workgroups to fill the device twice over; each loops over the list
+  const int64_t blocks = std::min<int64_t>((P.np + kBlock - 1) / kBlock, 2048);
+  if (blocks < 1) return;
+  hipLaunchKernelGGL(k_plastic, dim3((unsigned)blocks), dim3(kBlock), 0, st, P, mats, S);
 }
 
 template <int D>
