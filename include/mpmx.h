@@ -108,6 +108,31 @@ int mpmx_set_acceleration_constraints(mpmx_ctx* ctx, int64_t n, const int64_t* n
 int mpmx_set_friction_constraints(mpmx_ctx* ctx, int64_t n, const int64_t* node,
                                   const int* dir, const int* sign_n, const double* mu);
 
+/* Math functions of time-dependent boundary conditions, indexed 0..nf-1
+ * (the reference's "math_functions", MPMBase::initialise_math_functions
+ * mpm_base.tcc:958-1010, functions stored by id). type[f] MPMX_FN_LINEAR:
+ * npts[f] points, params x_0..x_{n-1} then f(x_0)..f(x_{n-1})
+ * (LinearFunction, src/functions/linear_function.cc:3-41: f of the first
+ * point before the table, of the last after it, linear in between);
+ * MPMX_FN_SIN: params x0, a, xmin, xmax (SinFunction,
+ * src/functions/sin_function.cc:3-20: sin(a (x - x0)) on [xmin, xmax], 0
+ * elsewhere). params holds the functions one after another. Replaces any
+ * earlier set (at most 256 functions). */
+#define MPMX_FN_LINEAR 0
+#define MPMX_FN_SIN 1
+int mpmx_set_math_functions(mpmx_ctx* ctx, int nf, const int* type, const int* npts,
+                            const double* params);
+
+/* Acceleration constraints with a math function each (fn[i]: index into the
+ * math functions, -1 none): in the step at time t = step * dt the
+ * constraint's acceleration is value[i] * f(t) (AccelerationConstraint::
+ * acceleration, acceleration_constraint.h:37-43, updated by
+ * Mesh::update_nodal_acceleration_constraints, mesh.tcc:1414-1428, at
+ * mpm_scheme.tcc:219). mpmx_set_acceleration_constraints is this call with
+ * no functions. */
+int mpmx_set_acceleration_constraints_fn(mpmx_ctx* ctx, int64_t n, const int64_t* node,
+                                         const int* dir, const double* value, const int* fn);
+
 /* Particle velocity constraints: after the velocity update of every step,
  * velocity component dir[i] of particle particle[i] (-1: every particle)
  * is set to value[i], constraints applied in the order given (the reference's

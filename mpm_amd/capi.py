@@ -25,6 +25,7 @@ EXPORTS = (
     "mpmx_create", "mpmx_destroy", "mpmx_last_error", "mpmx_set_mesh", "mpmx_set_instances",
     "mpmx_set_materials",
     "mpmx_set_velocity_constraints", "mpmx_set_acceleration_constraints",
+    "mpmx_set_acceleration_constraints_fn", "mpmx_set_math_functions",
     "mpmx_set_friction_constraints", "mpmx_set_particle_velocity_constraints",
     "mpmx_set_gravity", "mpmx_generate_gauss",
     "mpmx_set_particles", "mpmx_compute_volume_mass", "mpmx_assign_velocity",
@@ -70,6 +71,8 @@ def load(path=LIB_PATH):
         "mpmx_set_materials": [V, C.c_int, _D],
         "mpmx_set_velocity_constraints": [V, C.c_int64, _I64, _I32, _D],
         "mpmx_set_acceleration_constraints": [V, C.c_int64, _I64, _I32, _D],
+        "mpmx_set_acceleration_constraints_fn": [V, C.c_int64, _I64, _I32, _D, _I32],
+        "mpmx_set_math_functions": [V, C.c_int, _I32, _I32, _D],
         "mpmx_set_friction_constraints": [V, C.c_int64, _I64, _I32, _I32, _D],
         "mpmx_set_particle_velocity_constraints": [V, C.c_int64, _I64, _I32, _D],
         "mpmx_set_gravity": [V, _D, C.c_double],
@@ -112,6 +115,29 @@ def _f64(a, shape=None):
         return None
     a = np.ascontiguousarray(a, dtype=np.float64)
     return a if shape is None else a.reshape(shape)
+
+
+def pack_math_functions(functions):
+    """(type, npts, params) arrays of mpmx_set_math_functions from a list of
+    ("linear", xvalues, fxvalues) or ("sin", x0, a, (xmin, xmax))."""
+    types, npts, params = [], [], []
+    for f in functions:
+        if f[0] == "linear":
+            x = np.asarray(f[1], dtype=np.float64).ravel()
+            fx = np.asarray(f[2], dtype=np.float64).ravel()
+            if x.shape != fx.shape or len(x) == 0:
+                raise ValueError("a linear function needs xvalues and fxvalues of one length")
+            types.append(0)
+            npts.append(len(x))
+            params += list(x) + list(fx)
+        elif f[0] == "sin":
+            types.append(1)
+            npts.append(4)
+            params += [float(f[1]), float(f[2]), float(f[3][0]), float(f[3][1])]
+        else:
+            raise ValueError("unknown math function %r" % (f[0],))
+    return (np.asarray(types, dtype=np.int32), np.asarray(npts, dtype=np.int32),
+            np.asarray(params if params else [0.0], dtype=np.float64))
 
 
 class Solver:
@@ -181,8 +207,21 @@ class Solver:
     def set_velocity_constraints(self, nodes, dirs, values):
         self._cons(self.lib.mpmx_set_velocity_constraints, nodes, dirs, (values, _D))
 
-    def set_acceleration_constraints(self, nodes, dirs, values):
-        self._cons(self.lib.mpmx_set_acceleration_constraints, nodes, dirs, (values, _D))
+    def set_acceleration_constraints(self, nodes, dirs, values, functions=None):
+        """functions: per constraint the index of the math function scaling
+        its value (-1: none)."""
+        if functions is None:
+            self._cons(self.lib.mpmx_set_acceleration_constraints, nodes, dirs, (values, _D))
+        else:
+            self._cons(self.lib.mpmx_set_acceleration_constraints_fn, nodes, dirs, (values, _D),
+                       (functions, _I32))
+
+    def set_math_functions(self, functions):
+        """Math functions of time-dependent constraints, indexed in list order:
+        ("linear", xvalues, fxvalues) or ("sin", x0, a, (xmin, xmax))."""
+        types, npts, params = pack_math_functions(functions)
+        self._check(self.lib.mpmx_set_math_functions(self.ctx, len(types), _p(types, _I32),
+                                                     _p(npts, _I32), _p(params)))
 
     def set_friction_constraints(self, nodes, dirs, sign_n, mu):
         self._cons(self.lib.mpmx_set_friction_constraints, nodes, dirs, (sign_n, _I32), (mu, _D))

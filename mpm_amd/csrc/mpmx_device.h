@@ -633,6 +633,26 @@ MPMX_DEV void mc_stress(const Material& m, double* stress, const double* dstrain
   mc_iterate(m, stress, sv, R, 100);
 }
 
+// Value of a math function at x: LinearFunction::value
+// (src/functions/linear_function.cc:19-41), SinFunction::value
+// (src/functions/sin_function.cc:17-20).
+MPMX_DEV double math_fn_value(const MathFn& f, const double* data, double x) {
+  const double* d = data + f.off;
+  if (f.type == FN_SIN) {
+    if ((x < d[2]) || (x > d[3])) return 0.0;
+    return dmath::sin(d[1] * (x - d[0]));
+  }
+  const int n = f.n;
+  const double* fx = d + n;
+  if (x < d[0]) return fx[0];
+  if (x > d[n - 1]) return fx[n - 1];
+  for (int i = 0; i + 1 < n; ++i) {
+    const double t = (x - d[i]) / (d[i + 1] - d[i]);
+    if (t > -kDblEps && t < (1.0 + kDblEps)) return (fx[i] + t * (fx[i + 1] - fx[i]));
+  }
+  return __builtin_nan("");
+}
+
 // ------------------------------ node update ---------------------------------
 // node.tcc:370-404 (Cartesian branch)
 template <int D>
@@ -723,7 +743,8 @@ MPMX_DEV void node_compute_velocity(double mass, const double* mom, double* v, d
 // node.tcc:309-346 compute_acceleration_velocity_cundall
 template <int D>
 MPMX_DEV bool node_acc_vel(double mass, const double* fext, const double* fint, double* v,
-                           double* a, const NodeCons* c, double dt, double damping) {
+                           double* a, const NodeCons* c, double dt, double damping,
+                           const double* fnval) {
   const double tolerance = 1.0E-15;
   if (!(mass > tolerance)) return false;
   double f[D];
@@ -742,7 +763,8 @@ MPMX_DEV bool node_acc_vel(double mass, const double* fext, const double* fint, 
     if (c->fdir >= 0) apply_friction<D>(*c, a, v, dt);
 #pragma unroll
     for (int k = 0; k < D; ++k)
-      if (c->amask & (1 << k)) a[k] = c->aval[k];
+      if (c->amask & (1 << k))  // acceleration_constraint.h:37-43
+        a[k] = c->afn[k] >= 0 ? c->aval[k] * fnval[c->afn[k]] : c->aval[k];
   }
 #pragma unroll
   for (int k = 0; k < D; ++k) v[k] += a[k] * dt;

@@ -67,6 +67,7 @@ struct DBuf {
 
 struct HostCons {
   std::map<int, double> vel, acc;  // dir -> value (first insert wins)
+  std::map<int, int> accfn;        // dir -> math function of the acceleration constraint
   int fdir = -1, fsign = 0;
   double fmu = 0;
 };
@@ -105,6 +106,10 @@ struct mpmx_ctx {
   DBuf<int> d_ncons;
   DBuf<NodeCons> d_table;
   std::map<int64_t, HostCons> cons;
+  // math functions of time-dependent constraints and their values this step
+  std::vector<MathFn> fns;
+  DBuf<MathFn> d_fns;
+  DBuf<double> d_fn_data, d_fnval;
 
   // materials
   std::vector<Material> mats;
@@ -232,6 +237,7 @@ struct mpmx_ctx {
     N.active = d_nactive.p;
     N.cons = d_ncons.p;
     N.table = d_table.p;
+    N.fnval = fns.empty() ? nullptr : d_fnval.p;
     return N;
   }
 };
@@ -521,6 +527,7 @@ void upload_constraints(mpmx_ctx* c) {
     NodeCons nc;
     std::memset(&nc, 0, sizeof(nc));
     nc.fdir = -1;
+    nc.afn[0] = nc.afn[1] = nc.afn[2] = -1;
     for (auto& v : kv.second.vel) {
       nc.vmask |= 1 << v.first;
       nc.vval[v.first] = v.second;
@@ -528,6 +535,8 @@ void upload_constraints(mpmx_ctx* c) {
     for (auto& a : kv.second.acc) {
       nc.amask |= 1 << a.first;
       nc.aval[a.first] = a.second;
+      const auto f = kv.second.accfn.find(a.first);
+      if (f != kv.second.accfn.end()) nc.afn[a.first] = f->second;
     }
     if (kv.second.fdir >= 0) {
       nc.fdir = kv.second.fdir;
@@ -809,6 +818,10 @@ void enqueue_step(mpmx_ctx* c, const StepArgs& A, Launcher& L, bool full_sort) {
   int* perm = c->d_perm.p;
   int* scan = c->d_scan_list.p;
   const bool ordered = (c->flags & MPMX_ORDERED) != 0;
+  if (!c->fns.empty())
+    L("time_functions", [&] {
+      launch_fn_eval(c->d_fns.p, c->d_fn_data.p, (int)c->fns.size(), S, A.dt, c->d_fnval.p, st);
+    });
   // the Mohr-Coulomb return mappings deferred by the stress kernels
   auto plastic = [&] {
     if (c->any_mc) L("mc_plastic", [&] { launch_plastic(P, mats, S, st); });
@@ -1352,20 +1365,73 @@ int mpmx_set_velocity_constraints(mpmx_ctx* c, int64_t n, const int64_t* node, c
   API_END(c)
 }
 
-int mpmx_set_acceleration_constraints(mpmx_ctx* c, int64_t n, const int64_t* node, const int* dir,
-                                      const double* value) {
+int mpmx_set_acceleration_constraints_fn(mpmx_ctx* c, int64_t n, const int64_t* node,
+                                         const int* dir, const double* value, const int* fn) {
   API_BEGIN(c)
   require_mesh(c);
   for (int64_t i = 0; i < n; ++i) {
     if (node[i] < 0 || node[i] >= c->nnodes || dir[i] < 0 || dir[i] >= c->dim)
       throw ApiError(MPMX_E_ARG, "acceleration constraint node/dir out of range");
+    if (fn && (fn[i] < -1 || fn[i] >= (int)c->fns.size()))
+      throw ApiError(MPMX_E_ARG, "acceleration constraint math function index out of range");
+  }
+  for (int64_t i = 0; i < n; ++i) {
     auto& hc = c->cons[node[i]];
     if (hc.vel.count(dir[i]))
       throw ApiError(MPMX_E_ARG,
                      "Acceleration and velocity constraints set in the same direction for the same node");
-    hc.acc.insert({dir[i], value[i]});
+    if (hc.acc.insert({dir[i], value[i]}).second && fn && fn[i] >= 0) hc.accfn[dir[i]] = fn[i];
   }
   upload_constraints(c);
+  API_END(c)
+}
+
+int mpmx_set_acceleration_constraints(mpmx_ctx* c, int64_t n, const int64_t* node, const int* dir,
+                                      const double* value) {
+  return mpmx_set_acceleration_constraints_fn(c, n, node, dir, value, nullptr);
+}
+
+static_assert(FN_LINEAR == MPMX_FN_LINEAR && FN_SIN == MPMX_FN_SIN, "math function types");
+
+int mpmx_set_math_functions(mpmx_ctx* c, int nf, const int* type, const int* npts,
+                            const double* params) {
+  API_BEGIN(c)
+  if (nf < 0 || nf > kMaxMathFns) throw ApiError(MPMX_E_ARG, "number of math functions out of range");
+  for (auto& kv : c->cons)
+    for (auto& f : kv.second.accfn)
+      if (f.second >= nf)
+        throw ApiError(MPMX_E_STATE, "a constraint refers to a math function being removed");
+  std::vector<MathFn> fs;
+  std::vector<double> data;
+  size_t off = 0;
+  for (int f = 0; f < nf; ++f) {
+    MathFn m;
+    m.type = type[f];
+    m.off = (int)data.size();
+    if (m.type == FN_LINEAR) {
+      if (npts[f] < 1) throw ApiError(MPMX_E_ARG, "a linear math function needs points");
+      m.n = npts[f];
+      data.insert(data.end(), params + off, params + off + 2 * (size_t)npts[f]);
+      off += 2 * (size_t)npts[f];
+    } else if (m.type == FN_SIN) {
+      m.n = 4;
+      data.insert(data.end(), params + off, params + off + 4);
+      off += 4;
+    } else {
+      throw ApiError(MPMX_E_ARG, "unknown math function type");
+    }
+    fs.push_back(m);
+  }
+  c->fns = fs;
+  if (!fs.empty()) {
+    c->d_fns.alloc(fs.size());
+    c->d_fns.upload(fs.data(), fs.size(), c->stream);
+    c->d_fn_data.alloc(data.size());
+    c->d_fn_data.upload(data.data(), data.size(), c->stream);
+    c->d_fnval.alloc(fs.size());
+    HIPCHECK(hipStreamSynchronize(c->stream));
+  }
+  c->invalidate_graph();
   API_END(c)
 }
 

@@ -41,6 +41,17 @@ struct NodeCons {
   double fsign;  // sign(sign_n): +1 if sign_n > 0 else -1 (node.tcc:514)
   double fmu;
   double vval[3], aval[3];
+  int afn[3];    // math function scaling aval[k] (index into Nodes::fnval), -1: none
+};
+
+// A math function of time-dependent boundary conditions: Linear (a table of
+// n points (x, f(x)) at data[off, off + 2n): x then f) or Sin (data[off..off+3]
+// = x0, a, xmin, xmax).
+enum { FN_LINEAR = 0, FN_SIN = 1 };
+struct MathFn {
+  int type;
+  int n;
+  int off;
 };
 
 struct Mesh {
@@ -103,6 +114,7 @@ struct Nodes {
   unsigned char* active;
   const int* cons;  // [nnodes] index into constraint table or -1
   const NodeCons* table;
+  const double* fnval;  // the math functions' values at the current step's time
 };
 
 // Per-step scalars

@@ -21,6 +21,10 @@ template <int D>
 void launch_locate_scan(const Mesh& M, const Particles& P, Status* S, const int* scan_list,
                         const int* hint, hipStream_t st);
 void launch_step_end(Status* S, hipStream_t st);
+// values of the nf (<= kMaxMathFns) math functions at the current step's time
+constexpr int kMaxMathFns = 256;
+void launch_fn_eval(const MathFn* fns, const double* data, int nf, const Status* S, double dt,
+                    double* out, hipStream_t st);
 // chunk: particles per workgroup of the LDS-staged scatters (128, 256, 512)
 template <int D>
 void launch_p2g(int what, int chunk, const Mesh& M, const Particles& P, const Nodes& Nd,

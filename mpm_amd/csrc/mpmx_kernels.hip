@@ -392,6 +392,14 @@ __global__ __launch_bounds__(kBlock) void k_locate_scan(Mesh M, Particles P, Sta
 }
 
 // End of step: advance the counter, clear the scan queue.
+// Mesh::update_nodal_acceleration_constraints (mesh.tcc:1414-1428): the math
+// functions' values at current_time = step * dt (mpm_scheme.tcc:219).
+__global__ void k_fn_eval(const MathFn* fns, const double* data, int nf, const Status* S,
+                          double dt, double* out) {
+  const int f = threadIdx.x;
+  if (f < nf) out[f] = math_fn_value(fns[f], data, static_cast<double>(S->step) * dt);
+}
+
 __global__ void k_step_end(Status* S) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     S->step += 1;
@@ -986,7 +994,7 @@ __global__ __launch_bounds__(kBlock) void k_node(Mesh M, Nodes Nd, StepArgs A) {
       fe[k] = Nd.fext[k * nn + n];
       fi[k] = Nd.fint[k * nn + n];
     }
-    node_acc_vel<D>(mass, fe, fi, v, a, c, A.dt, A.damping);
+    node_acc_vel<D>(mass, fe, fi, v, a, c, A.dt, A.damping, Nd.fnval);
     if (A.ad_nodes) {
       double mom[D];
 #pragma unroll
@@ -1164,7 +1172,7 @@ __global__ __launch_bounds__(kBlock) void k_gather(Mesh M, Particles P, Nodes Nd
       fe[k] = Nd.fext[k * nnodes + n];
       fi[k] = Nd.fint[k * nnodes + n];
     }
-    node_acc_vel<D>(mass, fe, fi, v, a, cc, A.dt, A.damping);
+    node_acc_vel<D>(mass, fe, fi, v, a, cc, A.dt, A.damping, Nd.fnval);
     if (A.ad_nodes) {
       double mom[D];
 #pragma unroll
@@ -1225,6 +1233,12 @@ void launch_locate_scan(const Mesh& M, const Particles& P, Status* S, const int*
                         const int* hint, hipStream_t st) {
   hipLaunchKernelGGL((k_locate_scan<D>), dim3(256), dim3(kBlock), 0, st, M, P, S, scan_list,
                      hint);
+}
+
+void launch_fn_eval(const MathFn* fns, const double* data, int nf, const Status* S, double dt,
+                    double* out, hipStream_t st) {
+  if (nf <= 0) return;
+  hipLaunchKernelGGL(k_fn_eval, dim3(1), dim3(nf), 0, st, fns, data, nf, S, dt, out);
 }
 
 void launch_step_end(Status* S, hipStream_t st) {

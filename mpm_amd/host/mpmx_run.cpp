@@ -19,6 +19,7 @@
 #include <fstream>
 #include <functional>
 #include <iostream>
+#include <limits>
 #include <map>
 #include <memory>
 #include <sstream>
@@ -227,10 +228,18 @@ class MPMExplicit {
   std::vector<double> pvc_val_;
   std::vector<int> vdir_, fdir_, fsign_;
   std::vector<double> vval_, fmu_;
+  // acceleration constraints and the math functions they may refer to
+  std::vector<int64_t> anode_;
+  std::vector<int> adir_, afn_;
+  std::vector<double> aval_;
+  std::map<int64_t, int> fn_index_;  // math function id -> index
+  std::vector<int> fn_type_, fn_npts_;
+  std::vector<double> fn_params_;
   int64_t expected_particles_ = 0;
 
   void read_analysis();
   void initialise_materials();
+  void initialise_math_functions();
   void initialise_mesh();
   void initialise_particles();
   void initialise_loads();
@@ -329,6 +338,34 @@ void MPMExplicit<Tdim>::initialise_mesh() {
       }
     }
   }
+  // nodal_acceleration_constraints (mpm_base.tcc:1067-1124): from a file
+  // (rows node dir acceleration, io_mesh_ascii.tcc:550-595) or per node set,
+  // with an optional math_function_id
+  if (bc.contains("acceleration_constraints")) {
+    for (const Json& c : bc.at("acceleration_constraints").elements()) {
+      if (c.contains("file")) {
+        for (const auto& r : read_rows(io_->file_name(c.at("file").str()))) {
+          anode_.push_back((int64_t)r.at(0));
+          adir_.push_back((int)r.at(1));
+          aval_.push_back(r.at(2));
+          afn_.push_back(-1);
+        }
+      } else {
+        int fn = -1;
+        if (c.contains("math_function_id")) {
+          const auto it = fn_index_.find(c.at("math_function_id").integer());
+          if (it == fn_index_.end()) throw std::runtime_error("math function not defined");
+          fn = it->second;
+        }
+        for (int64_t n : set_of(c.at("nset_id").integer())) {
+          anode_.push_back(n);
+          adir_.push_back((int)c.at("dir").integer());
+          aval_.push_back(c.at("acceleration").number());
+          afn_.push_back(fn);
+        }
+      }
+    }
+  }
   if (bc.contains("friction_constraints")) {
     for (const Json& c : bc.at("friction_constraints").elements()) {
       if (c.contains("file")) {
@@ -347,6 +384,58 @@ void MPMExplicit<Tdim>::initialise_mesh() {
         }
       }
     }
+  }
+}
+
+// math_functions (MPMBase::initialise_math_functions, mpm_base.tcc:958-1010):
+// Linear with xvalues/fxvalues or a file of "x, f(x)" rows
+// (io_mesh_ascii.tcc:798-816), Sin with x0, a and an optional xrange.
+template <unsigned Tdim>
+void MPMExplicit<Tdim>::initialise_math_functions() {
+  if (!io_->json().contains("math_functions")) return;
+  for (const Json& f : io_->json().at("math_functions").elements()) {
+    const int64_t id = f.at("id").integer();
+    const std::string type = f.at("type").str();
+    if (fn_index_.count(id)) throw std::runtime_error("math function id defined twice");
+    if (type == "Linear") {
+      std::vector<double> x, fx;
+      if (f.contains("file")) {
+        std::ifstream in(io_->file_name(f.at("file").str()));
+        if (!in) throw std::runtime_error("cannot open math function file");
+        std::string line;
+        while (std::getline(in, line)) {
+          for (char& ch : line)
+            if (ch == ',') ch = ' ';
+          std::istringstream is(line);
+          double a, b;
+          if (is >> a >> b) {
+            x.push_back(a);
+            fx.push_back(b);
+          }
+        }
+      } else {
+        for (const Json& v : f.at("xvalues").elements()) x.push_back(v.number());
+        for (const Json& v : f.at("fxvalues").elements()) fx.push_back(v.number());
+      }
+      if (x.empty() || x.size() != fx.size())
+        throw std::runtime_error("Cannot initialise linear function; x and f(x) are invalid");
+      fn_type_.push_back(MPMX_FN_LINEAR);
+      fn_npts_.push_back((int)x.size());
+      fn_params_.insert(fn_params_.end(), x.begin(), x.end());
+      fn_params_.insert(fn_params_.end(), fx.begin(), fx.end());
+    } else if (type == "Sin") {
+      double lo = std::numeric_limits<double>::min(), hi = std::numeric_limits<double>::max();
+      if (f.contains("xrange") && f.at("xrange").is_array() && f.at("xrange").elements().size() == 2) {
+        lo = f.at("xrange").at(0).number();
+        hi = f.at("xrange").at(1).number();
+      }
+      fn_type_.push_back(MPMX_FN_SIN);
+      fn_npts_.push_back(4);
+      fn_params_.insert(fn_params_.end(), {f.at("x0").number(), f.at("a").number(), lo, hi});
+    } else {
+      throw std::runtime_error("math function type " + type + " is not supported");
+    }
+    fn_index_[id] = (int)fn_type_.size() - 1;
   }
 }
 
@@ -485,13 +574,15 @@ void MPMExplicit<Tdim>::summary() const {
   std::printf("{\"analysis\": \"MPMExplicit%uD\", \"scheme\": \"%s\", \"velocity_update\": \"%s\", "
               "\"dt\": %.17g, \"nsteps\": %lld, \"output_steps\": %lld, \"damping\": %.17g, "
               "\"nnodes\": %lld, \"ncells\": %lld, \"node_sets\": %zu, \"cell_sets\": %zu, "
-              "\"velocity_constraints\": %zu, \"friction_constraints\": %zu, "
+              "\"velocity_constraints\": %zu, \"acceleration_constraints\": %zu, "
+              "\"math_functions\": %zu, \"friction_constraints\": %zu, "
               "\"particle_velocity_constraints\": %zu, \"particles\": %lld, "
               "\"gravity\": [%.17g, %.17g, %.17g], \"ramping_time\": %.17g, \"materials\": [",
               Tdim, scheme_.c_str(), vu_ == MPMX_PIC ? "pic" : "flip", dt_, (long long)nsteps_,
               (long long)output_steps_, damping_, (long long)(coords_.size() / Tdim),
               (long long)(cells_.size() / NN), node_sets_.size(), cell_sets_.size(),
-              vnode_.size(), fnode_.size(), pvc_particle_.size(), (long long)expected_particles_,
+              vnode_.size(), anode_.size(), fn_type_.size(), fnode_.size(), pvc_particle_.size(),
+              (long long)expected_particles_,
               gravity_[0],
               gravity_[1], gravity_[2], ramp_);
   for (size_t i = 0; i < mat_params_.size(); ++i)
@@ -505,6 +596,7 @@ template <unsigned Tdim>
 bool MPMExplicit<Tdim>::solve() {
   read_analysis();
   initialise_materials();
+  initialise_math_functions();
   initialise_mesh();
   initialise_loads();
   if (io_->dry_run()) {
@@ -526,6 +618,14 @@ bool MPMExplicit<Tdim>::solve() {
     check(ctx_, mpmx_set_velocity_constraints(ctx_, (int64_t)vnode_.size(), vnode_.data(),
                                               vdir_.data(), vval_.data()),
           "velocity constraints");
+  if (!fn_type_.empty())
+    check(ctx_, mpmx_set_math_functions(ctx_, (int)fn_type_.size(), fn_type_.data(), fn_npts_.data(),
+                                        fn_params_.data()),
+          "math functions");
+  if (!anode_.empty())
+    check(ctx_, mpmx_set_acceleration_constraints_fn(ctx_, (int64_t)anode_.size(), anode_.data(),
+                                                     adir_.data(), aval_.data(), afn_.data()),
+          "acceleration constraints");
   if (!fnode_.empty())
     check(ctx_, mpmx_set_friction_constraints(ctx_, (int64_t)fnode_.size(), fnode_.data(),
                                               fdir_.data(), fsign_.data(), fmu_.data()),

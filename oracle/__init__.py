@@ -64,7 +64,8 @@ def _lib(omp=False, libm="portable"):
                    "orc_nparticles", "orc_get_particles", "orc_get_nodes", "orc_get_cells",
                    "orc_step", "orc_phase", "orc_set_node_field", "orc_set_particle_field",
                    "orc_get_particle_scratch", "orc_assign_velocity",
-                   "orc_assign_stresses", "orc_set_particle_velocity_constraints"):
+                   "orc_assign_stresses", "orc_set_particle_velocity_constraints",
+                   "orc_set_acceleration_constraints_fn", "orc_set_math_functions"):
             getattr(lib, fn).restype = C.c_int
         lib.orc_kat_cell_volume.restype = C.c_double
         lib.orc_kat_mc_yield.restype = C.c_int
@@ -72,8 +73,40 @@ def _lib(omp=False, libm="portable"):
         lib.orc_threads.restype = C.c_int
         lib.orc_kat_math.restype = C.c_double
         lib.orc_kat_math.argtypes = [C.c_int, C.c_double]
+        lib.orc_kat_math_function.restype = C.c_double
+        lib.orc_kat_math_function.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_double]
         _LIBS[name] = lib
     return _LIBS[name]
+
+
+def pack_math_functions(functions):
+    """(type, npts, params) arrays of math functions given as ("linear",
+    xvalues, fxvalues) or ("sin", x0, a, (xmin, xmax))."""
+    types, npts, params = [], [], []
+    for f in functions:
+        if f[0] == "linear":
+            x = np.asarray(f[1], dtype=np.float64)
+            fx = np.asarray(f[2], dtype=np.float64)
+            if x.shape != fx.shape:
+                raise ValueError("xvalues and fxvalues differ in length")
+            types.append(0)
+            npts.append(len(x))
+            params += list(x) + list(fx)
+        elif f[0] == "sin":
+            types.append(1)
+            npts.append(4)
+            params += [float(f[1]), float(f[2]), float(f[3][0]), float(f[3][1])]
+        else:
+            raise ValueError("unknown math function %r" % (f[0],))
+    return (np.asarray(types, dtype=np.int32), np.asarray(npts, dtype=np.int32),
+            np.asarray(params if params else [0.0], dtype=np.float64))
+
+
+def math_function_value(function, x):
+    """The oracle's value of one math function (as pack_math_functions) at x."""
+    lib = _lib()
+    t, n, prm = pack_math_functions([function])
+    return lib.orc_kat_math_function(int(t[0]), int(n[0]), prm.ctypes.data, float(x))
 
 
 def _p(a, ctype=_D):
@@ -151,8 +184,25 @@ class Oracle:
     def set_velocity_constraints(self, nodes, dirs, values):
         self._constraints(self.lib.orc_set_velocity_constraints, nodes, dirs, values)
 
-    def set_acceleration_constraints(self, nodes, dirs, values):
-        self._constraints(self.lib.orc_set_acceleration_constraints, nodes, dirs, values)
+    def set_acceleration_constraints(self, nodes, dirs, values, functions=None):
+        """functions: per constraint the index of its math function (-1: none)."""
+        if functions is None:
+            self._constraints(self.lib.orc_set_acceleration_constraints, nodes, dirs, values)
+            return
+        nodes = np.ascontiguousarray(nodes, dtype=np.int64)
+        dirs = np.ascontiguousarray(np.broadcast_to(dirs, nodes.shape), dtype=np.int32)
+        vals = np.ascontiguousarray(np.broadcast_to(values, nodes.shape), dtype=np.float64)
+        fns = np.ascontiguousarray(np.broadcast_to(functions, nodes.shape), dtype=np.int32)
+        self._check(self.lib.orc_set_acceleration_constraints_fn(
+            self.ctx, C.c_int64(len(nodes)), _p(nodes, _I64), _p(dirs, _I32), _p(vals),
+            _p(fns, _I32)))
+
+    def set_math_functions(self, functions):
+        """functions: list of ("linear", xvalues, fxvalues) or
+        ("sin", x0, a, (xmin, xmax))."""
+        types, npts, params = pack_math_functions(functions)
+        self._check(self.lib.orc_set_math_functions(
+            self.ctx, C.c_int(len(types)), _p(types, _I32), _p(npts, _I32), _p(params)))
 
     def set_friction_constraints(self, nodes, dirs, sign_n, mu):
         self._constraints(self.lib.orc_set_friction_constraints, nodes, dirs, (sign_n, mu),

@@ -658,12 +658,42 @@ static inline double norm_diff(int dim, const double* a, const double* b) {
 }
 
 // ---------------------------------------------------------------------------
+// Math functions of time-dependent boundary conditions.
+// LinearFunction::value (src/functions/linear_function.cc:19-41) and
+// SinFunction::value (src/functions/sin_function.cc:17-20).
+enum { FN_LINEAR = 0, FN_SIN = 1 };
+struct MathFn {
+  int type = FN_LINEAR;
+  std::vector<double> x, fx;  // Linear: the (x, f(x)) table in its given order
+  double x0 = 0., a = 0.;     // Sin: sin(a (x - x0)) on [xmin, xmax], else 0
+  double xmin = std::numeric_limits<double>::min(), xmax = std::numeric_limits<double>::max();
+  double value(double xin) const {
+    if (type == FN_SIN) {
+      if ((xin < xmin) || (xin > xmax)) return 0.0;
+      return mth::sin(a * (xin - x0));
+    }
+    double v = std::numeric_limits<double>::quiet_NaN();
+    if (x.empty()) throw std::runtime_error("Cannot find the f(x); no linear function is defined");
+    if (xin < x.front()) return fx.front();
+    if (xin > x.back()) return fx.back();
+    for (size_t i = 0; i + 1 < x.size(); ++i) {
+      const double f = (xin - x[i]) / (x[i + 1] - x[i]);
+      if (f > (-std::numeric_limits<double>::epsilon()) &&
+          f < (1.0 + std::numeric_limits<double>::epsilon()))
+        return (fx[i] + f * (fx[i + 1] - fx[i]));
+    }
+    return v;
+  }
+};
+
 struct Node {
   double x[3] = {0, 0, 0};
   double mass = 0, momentum[3] = {0, 0, 0}, velocity[3] = {0, 0, 0}, acceleration[3] = {0, 0, 0};
   double fext[3] = {0, 0, 0}, fint[3] = {0, 0, 0};
   bool status = false;
   std::map<unsigned, double> vel_c, acc_c;
+  std::map<unsigned, int> acc_fn;  // math function of an acceleration constraint
+  double acc_now[3] = {0, 0, 0};   // the constraint values at the current time
   bool friction = false;
   unsigned f_dir = 0;
   int f_sign = 0;
@@ -710,6 +740,7 @@ struct Solver {
   double gravity[3] = {0, 0, 0};
   double ramp = 0;
   Index step = 0;
+  std::vector<MathFn> functions;
   std::vector<Index> unlocatable;
   std::string error;
 
@@ -1246,9 +1277,24 @@ struct Solver {
     }
   }
 
-  // node.tcc:456-485 (Cartesian)
+  // node.tcc:456-485 (Cartesian), with the values of the current time
   void apply_acceleration_constraints(Node& n) {
-    for (const auto& c : n.acc_c) n.acceleration[c.first % dim] = c.second;
+    for (const auto& c : n.acc_c) n.acceleration[c.first % dim] = n.acc_now[c.first % dim];
+  }
+
+  // Mesh::update_nodal_acceleration_constraints (mesh.tcc:1414-1428) at
+  // current_time = step * dt (mpm_scheme.tcc:219): each constraint's value is
+  // acceleration * f(t) with f its math function, or * 1.0 without one
+  // (acceleration_constraint.h:37-43)
+  void update_acceleration_constraints(double current_time) {
+    std::vector<double> fv(functions.size());
+    for (size_t f = 0; f < functions.size(); ++f) fv[f] = functions[f].value(current_time);
+    for (auto& n : nodes)
+      for (const auto& c : n.acc_c) {
+        const auto it = n.acc_fn.find(c.first);
+        const double scalar = (it != n.acc_fn.end() && it->second >= 0) ? fv[it->second] : 1.0;
+        n.acc_now[c.first % dim] = c.second * scalar;
+      }
   }
 
   // node.tcc:309-346
@@ -1354,6 +1400,7 @@ struct Solver {
     ORC_PRAGMA(omp parallel for schedule(static))
     for (Index i = 0; i < np; ++i) map_internal_force(particles[i]);
     // compute_particle_kinematics (mpm_scheme.tcc:214-241), always Cundall
+    update_acceleration_constraints(static_cast<double>(step) * dt);
     ORC_PRAGMA(omp parallel for schedule(static))
     for (Index i = 0; i < nnd; ++i)
       if (nodes[i].status) compute_acceleration_velocity_cundall(nodes[i], dt, damping);
@@ -1448,16 +1495,55 @@ int orc_set_velocity_constraints(orc_ctx* c, Index n, const Index* node, const i
   })
 }
 
-int orc_set_acceleration_constraints(orc_ctx* c, Index n, const Index* node, const int* dir,
-                                     const double* val) {
+// fn: per constraint the index of its math function, -1 or a null array: none
+int orc_set_acceleration_constraints_fn(orc_ctx* c, Index n, const Index* node, const int* dir,
+                                        const double* val, const int* fn) {
   ORC_TRY({
     for (Index i = 0; i < n; ++i) {
       if ((unsigned)dir[i] >= (unsigned)c->s.dim) throw std::runtime_error("bad dir");
+      if (fn && (fn[i] < -1 || fn[i] >= (int)c->s.functions.size()))
+        throw std::runtime_error("bad math function index");
       auto& nd = c->s.nodes.at(node[i]);
       if (nd.vel_c.count((unsigned)dir[i]))
         throw std::runtime_error("Acceleration and velocity constraints set in the same direction");
-      nd.acc_c.insert(std::make_pair((unsigned)dir[i], val[i]));
+      if (nd.acc_c.insert(std::make_pair((unsigned)dir[i], val[i])).second)
+        nd.acc_fn[(unsigned)dir[i]] = fn ? fn[i] : -1;
     }
+  })
+}
+
+int orc_set_acceleration_constraints(orc_ctx* c, Index n, const Index* node, const int* dir,
+                                     const double* val) {
+  return orc_set_acceleration_constraints_fn(c, n, node, dir, val, nullptr);
+}
+
+// Math functions, in index order: type[f] FN_LINEAR with npts[f] points
+// (params: x_0..x_{n-1}, f_0..f_{n-1}) or FN_SIN (params: x0, a, xmin, xmax).
+int orc_set_math_functions(orc_ctx* c, int nf, const int* type, const int* npts,
+                           const double* params) {
+  ORC_TRY({
+    std::vector<orc::MathFn> fs;
+    size_t off = 0;
+    for (int f = 0; f < nf; ++f) {
+      orc::MathFn m;
+      m.type = type[f];
+      if (m.type == orc::FN_LINEAR) {
+        if (npts[f] < 1) throw std::runtime_error("linear function needs points");
+        m.x.assign(params + off, params + off + npts[f]);
+        m.fx.assign(params + off + npts[f], params + off + 2 * npts[f]);
+        off += 2 * (size_t)npts[f];
+      } else if (m.type == orc::FN_SIN) {
+        m.x0 = params[off];
+        m.a = params[off + 1];
+        m.xmin = params[off + 2];
+        m.xmax = params[off + 3];
+        off += 4;
+      } else {
+        throw std::runtime_error("unknown math function type");
+      }
+      fs.push_back(m);
+    }
+    c->s.functions = fs;
   })
 }
 
@@ -1691,10 +1777,12 @@ int orc_phase(orc_ctx* c, int phase, double dt, const double* arg) {
       case PH_BODY_FORCE: for (auto& p : s.particles) s.map_body_force(p, arg); break;
       case PH_INTERNAL_FORCE: for (auto& p : s.particles) s.map_internal_force(p); break;
       case PH_ACC_VEL:
+        s.update_acceleration_constraints(static_cast<double>(s.step) * dt);
         for (auto& n : s.nodes)
           if (n.status) s.compute_acceleration_velocity_cundall(n, dt, arg ? arg[0] : 0.);
         break;
       case PH_ACC_VEL_PLAIN:
+        s.update_acceleration_constraints(static_cast<double>(s.step) * dt);
         for (auto& n : s.nodes)
           if (n.status) s.compute_acceleration_velocity_cundall(n, dt, 0.);
         break;
@@ -1876,7 +1964,10 @@ int orc_kat_node_update(int dim, double mass, const double* fext, const double* 
     n.velocity[k] = vel[k];
     n.acceleration[k] = acc[k];
     if (vel_c && !std::isnan(vel_c[k])) n.vel_c.insert(std::make_pair((unsigned)k, vel_c[k]));
-    if (acc_c && !std::isnan(acc_c[k])) n.acc_c.insert(std::make_pair((unsigned)k, acc_c[k]));
+    if (acc_c && !std::isnan(acc_c[k])) {
+      n.acc_c.insert(std::make_pair((unsigned)k, acc_c[k]));
+      n.acc_now[k] = acc_c[k];
+    }
   }
   if (f_dir >= 0) {
     n.friction = true;
@@ -1890,6 +1981,23 @@ int orc_kat_node_update(int dim, double mass, const double* fext, const double* 
     acc[k] = n.acceleration[k];
   }
   return st ? 1 : 0;
+}
+
+// Value of one math function (type, npts, params as orc_set_math_functions)
+// at x; NaN where the Linear table has no segment for x.
+double orc_kat_math_function(int type, int npts, const double* params, double x) {
+  orc::MathFn m;
+  m.type = type;
+  if (type == orc::FN_LINEAR) {
+    m.x.assign(params, params + npts);
+    m.fx.assign(params + npts, params + 2 * npts);
+  } else {
+    m.x0 = params[0];
+    m.a = params[1];
+    m.xmin = params[2];
+    m.xmax = params[3];
+  }
+  return m.value(x);
 }
 
 // Friction alone on given acceleration/velocity (node.tcc:510-657).

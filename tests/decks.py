@@ -12,11 +12,16 @@ DRIVER = os.path.join(ROOT, "mpm_amd", "mpmx_run")
 
 
 def write_deck(w, d, nsteps, output_steps, constraints_as="sets", uuid="deck",
-               material_json=None, stresses=None, particle_constraints=None):
+               material_json=None, stresses=None, particle_constraints=None,
+               acceleration_constraints=None, math_functions=None):
     """w: a Workload; d: directory. Velocity constraints go to node sets
     (``constraints_as="sets"``) or to a constraints file (``"file"``).
     particle_constraints: [(particle ids or -1, dir, velocity)], written as
-    particle sets and particles_velocity_constraints."""
+    particle sets and particles_velocity_constraints.
+    acceleration_constraints: [(node ids, dir, acceleration, math function id
+    or None)], written as node sets; math_functions: the deck's
+    "math_functions" entries (a Linear one with "file" gets its table
+    written to that file from "xvalues"/"fxvalues")."""
     os.makedirs(d, exist_ok=True)
     dim = w.dim
     with open(os.path.join(d, "mesh.txt"), "w") as f:
@@ -51,6 +56,24 @@ def write_deck(w, d, nsteps, output_steps, constraints_as="sets", uuid="deck",
         fc.append({"nset_id": sid, "dir": int(w.fric_dirs[0]), "sign_n": int(w.fric_sign[0]),
                    "friction": float(w.fric_mu[0])})
         bc["friction_constraints"] = fc
+    if acceleration_constraints:
+        ac = []
+        for ids, k, val, fid in acceleration_constraints:
+            sid = 300 + len(sets["node_sets"])
+            sets["node_sets"].append({"id": sid, "set": [int(n) for n in ids]})
+            c = {"nset_id": sid, "dir": int(k), "acceleration": float(val)}
+            if fid is not None:
+                c["math_function_id"] = int(fid)
+            ac.append(c)
+        bc["acceleration_constraints"] = ac
+    fns = []
+    for f in math_functions or []:
+        f = dict(f)
+        if f.get("type") == "Linear" and "file" in f:
+            with open(os.path.join(d, f["file"]), "w") as fh:
+                for x, fx in zip(f.pop("xvalues"), f.pop("fxvalues")):
+                    fh.write("%.17g,%.17g\n" % (x, fx))
+        fns.append(f)
     if particle_constraints:
         sets["particle_sets"] = []
         pc = []
@@ -89,6 +112,8 @@ def write_deck(w, d, nsteps, output_steps, constraints_as="sets", uuid="deck",
                      "resume": {"resume": False}},
         "post_processing": {"path": "results/", "output_steps": output_steps},
     }
+    if fns:
+        deck["math_functions"] = fns
     with open(os.path.join(d, "mpm.json"), "w") as f:
         json.dump(deck, f, indent=1)
     return os.path.join(d, "mpm.json")

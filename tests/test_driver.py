@@ -47,6 +47,25 @@ def test_dry_run_particle_velocity_constraints(tmp_path):
     assert s["particle_velocity_constraints"] == 4
 
 
+def test_dry_run_acceleration_constraints_and_math_functions(tmp_path):
+    w = workloads.column_2d(nx=6, ny=10)
+    fns = [{"id": 5, "type": "Linear", "xvalues": [0.0, 1e-3], "fxvalues": [0.0, 1.0]},
+           {"id": 7, "type": "Linear", "file": "ramp.csv", "xvalues": [0.0, 2e-3, 4e-3],
+            "fxvalues": [1.0, 0.5, 0.0]},
+           {"id": 9, "type": "Sin", "x0": 0.0, "a": 100.0, "xrange": [0.0, 1.0]}]
+    write_deck(w, str(tmp_path), nsteps=5, output_steps=5, material_json=LE,
+               math_functions=fns,
+               acceleration_constraints=[([40, 41], 0, 0.1, 5), ([42], 0, 0.2, None),
+                                         ([43, 44, 45], 0, -0.1, 9)])
+    s = json.loads(run_driver(str(tmp_path), "--dry-run").stdout)
+    assert s["acceleration_constraints"] == 6 and s["math_functions"] == 3
+    deck = json.load(open(os.path.join(str(tmp_path), "mpm.json")))
+    deck["mesh"]["boundary_conditions"]["acceleration_constraints"][0]["math_function_id"] = 6
+    json.dump(deck, open(os.path.join(str(tmp_path), "mpm.json"), "w"))
+    r = run_driver(str(tmp_path), "--dry-run", check=False)
+    assert r.returncode != 0 and "math function" in r.stderr
+
+
 def test_dry_run_mohr_coulomb_3d_with_friction(tmp_path):
     w = workloads.collapse_3d(nx=8, ny=4, nz=4, block=(4, 4, 4))
     write_deck(w, str(tmp_path), nsteps=5, output_steps=5, material_json=MC3)

@@ -70,3 +70,37 @@ def test_driver_particle_velocity_constraints_match_binding(tmp_path):
     p = s.particles()
     _compare(out, p, 2)
     assert p["v"][7, 1] == 0.002 and p["v"][3, 1] == -0.01
+
+
+def test_driver_time_function_acceleration_constraints_match_binding(tmp_path):
+    """acceleration_constraints with Linear (inline and from a file) and Sin
+    math functions."""
+    w = workloads.column_2d(nx=6, ny=10)
+    xmax = w.coords[:, 0].max()
+    row = np.nonzero((np.abs(w.coords[:, 1] - 0.25) < 1e-9) & (w.coords[:, 0] > 1e-9)
+                     & (w.coords[:, 0] < xmax - 1e-9))[0]
+    dt = w.dt
+    fns = [{"id": 2, "type": "Linear", "xvalues": [0.0, 10 * dt], "fxvalues": [0.0, 1.0]},
+           {"id": 4, "type": "Linear", "file": "table.csv", "xvalues": [0.0, 5 * dt, 15 * dt],
+            "fxvalues": [1.0, -0.5, 0.25]},
+           {"id": 8, "type": "Sin", "x0": 2 * dt, "a": 2 * np.pi / (12 * dt),
+            "xrange": [3 * dt, 1.0]}]
+    groups = [row[0::4], row[1::4], row[2::4], row[3::4]]
+    acs = [(groups[0], 0, 0.05, 2), (groups[1], 0, -0.03, 4), (groups[2], 0, 0.02, 8),
+           (groups[3], 0, 0.01, None)]
+    d = str(tmp_path)
+    write_deck(w, d, nsteps=20, output_steps=20, material_json=LE, math_functions=fns,
+               acceleration_constraints=acs)
+    run_driver(d, "--ordered")
+    out = read_output(os.path.join(d, "results", "deck", "particles20.txt"), 2)
+    from mpm_amd.capi import Solver
+    s = w.setup(Solver(2, w.scheme, ordered=True))
+    s.set_math_functions([("linear", [0.0, 10 * dt], [0.0, 1.0]),
+                          ("linear", [0.0, 5 * dt, 15 * dt], [1.0, -0.5, 0.25]),
+                          ("sin", 2 * dt, 2 * np.pi / (12 * dt), (3 * dt, 1.0))])
+    nodes = np.concatenate(groups)
+    vals = np.concatenate([np.full(len(g), v) for g, (_, _, v, _) in zip(groups, acs)])
+    fn = np.concatenate([np.full(len(g), i if i < 3 else -1) for i, g in enumerate(groups)])
+    s.set_acceleration_constraints(nodes, 0, vals, fn)
+    s.step(20, w.dt)
+    _compare(out, s.particles(), 2)

@@ -1,7 +1,7 @@
 """Step options beyond the benchmark decks, GPU (ordered mode) against the
 oracle bit for bit: Cundall damping, PIC and blended FLIP velocity updates,
-acceleration constraints, prescribed non-zero velocities, particle velocity
-constraints, gravity ramping, one particle per cell, 2D Mohr-Coulomb under
+acceleration constraints (constant and scaled by math functions of time),
+prescribed non-zero velocities, particle velocity constraints, gravity ramping, one particle per cell, 2D Mohr-Coulomb under
 every scheme; and fast mode on the same decks within the per-step
 tolerance."""
 import numpy as np
@@ -27,7 +27,9 @@ def _oracle(w, **kw):
                          blending_ratio=kw.get("blend", 1.0))
 
 
-def _extra(obj, acc=None, ramp=None, pvc=None):
+def _extra(obj, acc=None, ramp=None, pvc=None, fns=None):
+    if fns is not None:
+        obj.set_math_functions(fns)
     if acc is not None:
         obj.set_acceleration_constraints(*acc)
     if ramp is not None:
@@ -37,9 +39,9 @@ def _extra(obj, acc=None, ramp=None, pvc=None):
     return obj
 
 
-def _run(w, steps, damping=0.0, acc=None, ramp=None, pvc=None, ordered=True, **kw):
-    g = _extra(w.setup(_solver(w, ordered, **kw)), acc, ramp, pvc)
-    o = _extra(w.setup(_oracle(w, **kw)), acc, ramp, pvc)
+def _run(w, steps, damping=0.0, acc=None, ramp=None, pvc=None, ordered=True, fns=None, **kw):
+    g = _extra(w.setup(_solver(w, ordered, **kw)), acc, ramp, pvc, fns)
+    o = _extra(w.setup(_oracle(w, **kw)), acc, ramp, pvc, fns)
     for _ in range(steps // 10):
         g.step(10, w.dt, damping)
         o.step(10, w.dt, damping)
@@ -90,6 +92,40 @@ def test_acceleration_constraints_and_prescribed_velocity():
     acc = (mid, np.zeros(len(mid), dtype=np.int32), np.full(len(mid), 0.05))
     pg, po = _run(w, 40, acc=acc)
     _check(pg, po, exact=True)
+
+
+@pytest.mark.parametrize("scheme,ordered", [("usl", True), ("musl", True), ("usf", True),
+                                            ("musl", False)])
+def test_time_function_acceleration_constraints(scheme, ordered):
+    """Acceleration constraints scaled by math functions of time: a Linear
+    ramp that starts, rises and holds within the run, a Sin that switches
+    on part-way, and a constant constraint beside them; the constrained
+    nodes' accelerations follow value * f(step * dt)."""
+    w = workloads.column_2d(nx=6, ny=10)
+    xmax = w.coords[:, 0].max()
+    row = np.nonzero((np.abs(w.coords[:, 1] - 0.25) < 1e-9) & (w.coords[:, 0] > 1e-9)
+                     & (w.coords[:, 0] < xmax - 1e-9))[0]
+    assert len(row) >= 3
+    fns = [("linear", [0.0, 10 * w.dt, 30 * w.dt], [0.0, 1.0, 0.25]),
+           ("sin", 5 * w.dt, 2 * np.pi / (20 * w.dt), (8 * w.dt, 1.0))]
+    fn = np.arange(len(row)) % 3 - 1  # -1 (none), 0 (linear), 1 (sin)
+    acc = (row, np.zeros(len(row), dtype=np.int32), np.full(len(row), 0.05), fn)
+    g = _extra(w.setup(_solver(w, ordered, scheme=scheme)), acc=acc, fns=fns)
+    o = _extra(w.setup(_oracle(w, scheme=scheme)), acc=acc, fns=fns)
+    for _ in range(4):
+        g.step(10, w.dt)
+        o.step(10, w.dt)
+    _check(g.particles(), o.particles(), exact=ordered, tol=1e-9)
+    ng, no = g.nodes(), o.nodes()
+    active = no["status"][row] != 0  # nodes of the last step (the GPU clears its flags)
+    assert np.all(active)
+    t = 39 * w.dt  # time of the last step
+    f = np.where(fn == 0, np.interp(t, fns[0][1], fns[0][2]),
+                 np.where(fn == 1, np.sin(fns[1][2] * (t - fns[1][1])), 1.0))
+    a = ng["acceleration"][row, 0]
+    np.testing.assert_allclose(a, 0.05 * f, rtol=1e-13, atol=1e-15)
+    if ordered:
+        assert np.array_equal(a, no["acceleration"][row, 0])
 
 
 def test_gravity_ramp():
