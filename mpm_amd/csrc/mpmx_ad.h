@@ -516,39 +516,34 @@ MPMX_AD int mc_update(const McParams<T>& m, const T* stress, const T* dstrain, T
   if (ytc == AD_TENSILE) ycur = val(yc[0]);
   if (ytc == AD_SHEAR) ycur = val(yc[1]);
   const bool use_current = fabs(ycur) < Tolerance;
-  T df[6], dp[6], tmp[6], dp_dq;
-  T lam;
-  if (use_current) {
-    if (frozen_flow)
-      df_dp_frozen(m, ytc, stress, df, dp, &dp_dq);
-    else
-      df_dp(m, ytc, stress, Ic, A, df, dp, &dp_dq);
-    de_tmul(m, df, tmp);
-    lam = dot6(tmp, dstrain) / (dot6(tmp, dp) + 0.);
-  } else {
-    if (frozen_flow)
-      df_dp_frozen(m, ytt, trial, df, dp, &dp_dq);
-    else
-      df_dp(m, ytt, trial, It, A, df, dp, &dp_dq);
-    de_tmul(m, df, tmp);
-    lam = yield_trial / (dot6(tmp, dp) + 0.);
+  // The first correction (from the trial stress, with the gradients at the
+  // current or the trial stress) and the loop's corrections share one
+  // evaluation of the flow rule: pass -1 is the first correction.
+  T x[6];
+  for (int i = 0; i < 6; ++i) {
+    x[i] = use_current ? stress[i] : trial[i];
+    out[i] = trial[i];
   }
-  for (int i = 0; i < 6; ++i) out[i] = trial[i];
-  correct(m, lam, dp, out);
-  for (int itr = 0; itr < 100; ++itr) {
-    Inv<T> I;
-    invariants(out, I);
-    const int yt = yield(yf, I, A, m.cohesion, m.tcut);
-    if (val(yf[0]) < Tolerance && val(yf[1]) < Tolerance) break;
-    if (yt == AD_TENSILE) yield_trial = yf[0];
-    if (yt == AD_SHEAR) yield_trial = yf[1];
+  Inv<T> I = use_current ? Ic : It;
+  int yt = use_current ? ytc : ytt;
+  for (int itr = -1; itr < 100; ++itr) {
+    if (itr >= 0) {
+      invariants(out, I);
+      yt = yield(yf, I, A, m.cohesion, m.tcut);
+      if (val(yf[0]) < Tolerance && val(yf[1]) < Tolerance) break;
+      if (yt == AD_TENSILE) yield_trial = yf[0];
+      if (yt == AD_SHEAR) yield_trial = yf[1];
+      for (int i = 0; i < 6; ++i) x[i] = out[i];
+    }
+    T df[6], dp[6], tmp[6], dp_dq;
     if (frozen_flow)
-      df_dp_frozen(m, yt, out, df, dp, &dp_dq);
+      df_dp_frozen(m, yt, x, df, dp, &dp_dq);
     else
-      df_dp(m, yt, out, I, A, df, dp, &dp_dq);
+      df_dp(m, yt, x, I, A, df, dp, &dp_dq);
     de_tmul(m, df, tmp);
-    const T lt = yield_trial / (dot6(tmp, dp) + 0.);
-    correct(m, lt, dp, out);
+    const T lam = ((itr < 0 && use_current) ? dot6(tmp, dstrain) : yield_trial) /
+                  (dot6(tmp, dp) + 0.);
+    correct(m, lam, dp, out);
   }
   return ytt;
 }
