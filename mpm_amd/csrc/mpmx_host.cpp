@@ -151,7 +151,8 @@ struct mpmx_ctx {
   int64_t since_sort = 0;
 
   // graph cache: [0] step with key refresh, [1] step with a full sort
-  // [2], [3]: the same steps recording nodal quantities into d_ad_rec
+  // [2], [3]: the same steps recording nodal quantities into the slots of
+  // d_ad_node_slots (the slot index is Status::ad_slot)
   hipGraphExec_t graph_exec[4] = {nullptr, nullptr, nullptr, nullptr};
   double graph_dt = 0, graph_damping = 0;
 
@@ -164,7 +165,7 @@ struct mpmx_ctx {
   double ad_apex_ratio = 0.3;
   int64_t ad_step0 = 0, ad_np = 0;
   int64_t ad_ck_stride = 0;  // doubles per checkpoint
-  DBuf<double> d_ad_ck, d_ad_slots, d_ad_node_slots, d_ad_rec, d_ad_nodes, d_ad_parts;
+  DBuf<double> d_ad_ck, d_ad_slots, d_ad_node_slots, d_ad_nodes, d_ad_parts;
   DBuf<int> d_ad_ck_cell, d_ad_slot_cell, d_ad_list;
   DBuf<double> d_ad_list_buf;
 
@@ -639,6 +640,7 @@ void reset_status(mpmx_ctx* c) {
   s.nscan = 0;
   s.nplastic = 0;
   s.nplastic_t = 0;
+  s.ad_slot = 0;
   *c->h_status = s;
   c->d_status.upload(c->h_status, 1, c->stream);
 }
@@ -898,9 +900,10 @@ bool next_step_sorts(mpmx_ctx* c) {
 
 
 // Queue nsteps steps on the context's stream (one graph replay each).
-// ad_nodes: record each step's nodal quantities there (direct launches).
+// ad_nodes: record each step's nodal quantities in the slots of length
+// ad_stride there, step Status::ad_slot in its slot.
 void launch_steps(mpmx_ctx* c, int nsteps, double dt, double damping,
-                  double* ad_nodes = nullptr) {
+                  double* ad_nodes = nullptr, int64_t ad_stride = 0, int ad_nslots = 0) {
   StepArgs A;
   A.dt = dt;
   A.damping = damping;
@@ -908,6 +911,9 @@ void launch_steps(mpmx_ctx* c, int nsteps, double dt, double damping,
   for (int k = 0; k < 3; ++k) A.g[k] = c->g[k];
   A.ramp = c->ramp;
   A.ad_nodes = ad_nodes;
+  A.ad_slot = ad_nodes ? &c->d_status.p->ad_slot : nullptr;
+  A.ad_stride = ad_stride;
+  A.ad_nslots = ad_nslots;
   Launcher L{c, {}};
   const bool use_graph = !(c->flags & MPMX_NO_GRAPH) && !c->profile;
   const int rec = ad_nodes ? 2 : 0;
@@ -1883,15 +1889,14 @@ int mpmx_adjoint_backward(mpmx_ctx* c, const double* dL_dx, const double* dL_dv,
   hipStream_t st = c->stream;
   c->d_ad_slots.alloc((size_t)(K + 1) * ad_slot_doubles(c));
   c->d_ad_slot_cell.alloc((size_t)(K + 1) * np);
+  // the recording graphs hold the slots' address: captured anew for this
+  // allocation
   c->d_ad_node_slots.alloc((size_t)K * ad_node_slot_doubles(c));
-  if (c->d_ad_rec.n != (size_t)ad_node_slot_doubles(c)) {
-    c->d_ad_rec.alloc(ad_node_slot_doubles(c));
-    for (int k = 2; k < 4; ++k)
-      if (c->graph_exec[k]) {
-        (void)hipGraphExecDestroy(c->graph_exec[k]);
-        c->graph_exec[k] = nullptr;
-      }
-  }
+  for (int k = 2; k < 4; ++k)
+    if (c->graph_exec[k]) {
+      (void)hipGraphExecDestroy(c->graph_exec[k]);
+      c->graph_exec[k] = nullptr;
+    }
   c->d_ad_nodes.alloc(ad_node_doubles(c));
   c->d_ad_parts.alloc(ad_part_doubles(c));
   c->d_ad_list.alloc(1 + 2 * (size_t)np);
@@ -1928,12 +1933,10 @@ int mpmx_adjoint_backward(mpmx_ctx* c, const double* dL_dx, const double* dL_dv,
     c->step = c->ad_step0 + (int64_t)s * K;
     c->need_sort = true;
     reset_status(c);
-    for (int j = 0; j < ns; ++j) {
+    for (int j = 0; j < ns; ++j) {  // reset_status set the slot index to 0
       ad_save_slot(c, j);
-      launch_steps(c, 1, c->ad_dt, c->ad_damping, c->d_ad_rec.p);
-      HIPCHECK(hipMemcpyAsync(ad_node_slot(c, j), c->d_ad_rec.p,
-                              ad_node_slot_doubles(c) * sizeof(double), hipMemcpyDeviceToDevice,
-                              st));
+      launch_steps(c, 1, c->ad_dt, c->ad_damping, c->d_ad_node_slots.p,
+                   ad_node_slot_doubles(c), K);
     }
     ad_save_slot(c, ns);
     check_status(c);
@@ -1979,7 +1982,6 @@ int mpmx_adjoint_release(mpmx_ctx* c) {
   c->d_ad_slots.release();
   c->d_ad_slot_cell.release();
   c->d_ad_node_slots.release();
-  c->d_ad_rec.release();
   for (int k = 2; k < 4; ++k)
     if (c->graph_exec[k]) {
       (void)hipGraphExecDestroy(c->graph_exec[k]);

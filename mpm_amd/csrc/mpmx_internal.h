@@ -125,10 +125,15 @@ struct StepArgs {
   double g[3];
   double ramp;  // gravity ramping time (<= 0: none)
   // Adjoint recording: when set, the node update kernels also store the
-  // step's nodal quantities here (layout: mass [nn], then momentum, force,
-  // updated velocity, acceleration, MUSL second momentum and its velocity,
-  // [dim][nn] each). Null in ordinary steps.
+  // step's nodal quantities at ad_nodes + (*ad_slot) * ad_stride (layout:
+  // mass [nn], then momentum, force, updated velocity, acceleration, MUSL
+  // second momentum and its velocity, [dim][nn] each); *ad_slot is the
+  // device-side step index within the recorded segment (Status::ad_slot).
+  // Null in ordinary steps.
   double* ad_nodes;
+  const int* ad_slot;
+  int64_t ad_stride;
+  int ad_nslots;  // slots at ad_nodes; an index outside [0, ad_nslots) records nothing
 };
 
 // Device-side status written by kernels and read after a run of steps.
@@ -139,6 +144,7 @@ struct Status {
   int nscan;               // particles needing the full-mesh scan this step
   int nplastic;            // shear entries of the deferred plastic list (its front)
   int nplastic_t;          // tensile entries (its back)
+  int ad_slot;             // step index within an adjoint segment (recording)
 };
 
 }  // namespace mpmx

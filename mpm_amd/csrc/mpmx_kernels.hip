@@ -406,6 +406,7 @@ __global__ void k_step_end(Status* S) {
     S->nscan = 0;
     S->nplastic = 0;
     S->nplastic_t = 0;
+    S->ad_slot += 1;
   }
 }
 
@@ -927,15 +928,25 @@ __global__ __launch_bounds__(CH) void k_g2p_scatter(Mesh M, Particles P, Nodes N
 }
 
 // ------------------------------- node kernels -------------------------------
+// Where this step's nodal record goes (StepArgs::ad_nodes), or null.
+MPMX_DEV double* ad_record_base(const StepArgs& A) {
+  if (!A.ad_nodes) return nullptr;
+  const int slot = *A.ad_slot;
+  if (slot < 0 || slot >= A.ad_nslots) return nullptr;
+  return A.ad_nodes + (int64_t)slot * A.ad_stride;
+}
+
 // Adjoint recording (StepArgs::ad_nodes): the nodal quantities of this step.
 // FIRST: after the update (mass, momentum, external + internal force,
 // updated velocity, acceleration); SECOND: MUSL's second momentum and its
 // velocity.
+
 template <int D>
 MPMX_DEV void ad_record_first(const StepArgs& A, int64_t n, int64_t nn, double mass,
                               const double* mom, const double* fe, const double* fi,
                               const double* v, const double* a) {
-  double* b = A.ad_nodes;
+  double* b = ad_record_base(A);
+  if (!b) return;
   b[n] = mass;
 #pragma unroll
   for (int k = 0; k < D; ++k) {
@@ -948,7 +959,8 @@ MPMX_DEV void ad_record_first(const StepArgs& A, int64_t n, int64_t nn, double m
 template <int D>
 MPMX_DEV void ad_record_second(const StepArgs& A, int64_t n, int64_t nn, const double* mom,
                                const double* v) {
-  double* b = A.ad_nodes;
+  double* b = ad_record_base(A);
+  if (!b) return;
 #pragma unroll
   for (int k = 0; k < D; ++k) {
     b[(1 + 4 * D + k) * nn + n] = mom[k];
@@ -969,7 +981,10 @@ __global__ __launch_bounds__(kBlock) void k_node(Mesh M, Nodes Nd, StepArgs A) {
   if (n >= M.nnodes) return;
   const int64_t nn = M.nnodes;
   if (!Nd.active[n]) {
-    if ((FL & ND_ACC) != 0 && A.ad_nodes) A.ad_nodes[n] = 0.;
+    if ((FL & ND_ACC) != 0) {
+      double* b = ad_record_base(A);
+      if (b) b[n] = 0.;
+    }
     return;
   }
   const int ci = Nd.cons[n];
@@ -1080,7 +1095,10 @@ __global__ __launch_bounds__(kBlock) void k_gather(Mesh M, Particles P, Nodes Nd
   }
   if ((WHAT & SC_MV) != 0) Nd.active[n] = any ? 1 : 0;
   if (!any) {
-    if ((FL & ND_ACC) != 0 && A.ad_nodes) A.ad_nodes[n] = 0.;
+    if ((FL & ND_ACC) != 0) {
+      double* b = ad_record_base(A);
+      if (b) b[n] = 0.;
+    }
     return;
   }
   double g[3] = {0, 0, 0};
