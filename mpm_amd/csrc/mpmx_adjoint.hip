@@ -588,10 +588,20 @@ __global__ void k_ad_fill(double* a, int64_t n, double v) {
 // Pre-update nodal velocity of step t from its recorded mass and momentum
 // (node.tcc:236-252).
 template <int D>
-__global__ __launch_bounds__(kAdBlock) void k_ad_node_fwd(Mesh M, Nodes Nd, AdjNodes A) {
+// Also clears this step's nodal adjoints and the plastic list length.
+__global__ __launch_bounds__(kAdBlock) void k_ad_node_fwd(Mesh M, Nodes Nd, AdjNodes A,
+                                                          int* qcount) {
   const int64_t n = blockIdx.x * (int64_t)kAdBlock + threadIdx.x;
   const int64_t nn = M.nnodes;
+  if (n == 0) *qcount = 0;
   if (n >= nn) return;
+  A.lm[n] = 0.;
+  A.lm2[n] = 0.;
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    const int64_t i = k * nn + n;
+    A.v[i] = A.lvp[i] = A.la[i] = A.lv[i] = A.lvpp[i] = A.lf[i] = A.lP[i] = A.lP2[i] = 0.;
+  }
   const double mass = A.m[n];
   if (!(mass > 0.)) return;
   const int ci = Nd.cons[n];
@@ -957,7 +967,6 @@ __global__ __launch_bounds__(kAdBlock) void k_ad_mc_tail(Mesh M, Particles P, Sl
     for (int k = 0; k < D; ++k) ad_atomic(&lnv[k * M.nnodes + S.node[i]], acc[i][k]);
 }
 
-__global__ void k_ad_zero_int(int* a) { *a = 0; }
 
 inline dim3 ad_grid(int64_t n) { return dim3((unsigned)((n + kAdBlock - 1) / kAdBlock)); }
 
@@ -983,8 +992,7 @@ static void backward_scheme(const Mesh& M, const Particles& P, const Nodes& Nd,
     hipLaunchKernelGGL((k_ad_mc<D>), gp, b, 0, st, P, mats, s0, L, Q);
     hipLaunchKernelGGL((k_ad_mc_tail<D>), gp, b, 0, st, M, P, s0, nv, lnv, L, Q, a);
   };
-  hipLaunchKernelGGL(k_ad_zero_int, dim3(1), dim3(1), 0, st, Q.count);
-  hipLaunchKernelGGL((k_ad_node_fwd<D>), gn, b, 0, st, M, Nd, A);
+  hipLaunchKernelGGL((k_ad_node_fwd<D>), gn, b, 0, st, M, Nd, A, Q.count);
   hipLaunchKernelGGL((k_ad_b1<D, SCH>), gp, b, 0, st, M, P, mats, s0, s1, A, L, Q, a);
   if (SCH == USL) {
     plastic(A.vp, A.lvp);

@@ -1116,8 +1116,7 @@ void ad_backward_step(mpmx_ctx* c, int j, int64_t t) {
     gm = r < 1.0 ? r : 1.0;
   }
   for (int k = 0; k < 3; ++k) a.g[k] = gm * c->g[k];
-  launch_ad_fill(c->d_ad_nodes.p, ad_node_doubles(c), 0., c->stream);
-  const Slot s0 = ad_slot(c, j), s1 = ad_slot(c, j + 1);
+  const Slot s0 = ad_slot(c, j), s1 = ad_slot(c, j + 1);  // k_ad_node_fwd clears the adjoints
   AdjList Q;
   Q.cap = c->np;
   Q.count = c->d_ad_list.p;
