@@ -1,9 +1,10 @@
 """Step options beyond the benchmark decks, GPU (ordered mode) against the
 oracle bit for bit: Cundall damping, PIC and blended FLIP velocity updates,
 acceleration constraints (constant and scaled by math functions of time),
-prescribed non-zero velocities, particle velocity constraints, gravity ramping, one particle per cell, 2D Mohr-Coulomb under
-every scheme; and fast mode on the same decks within the per-step
-tolerance."""
+prescribed non-zero velocities, particle velocity constraints, gravity
+ramping, one particle per cell, 2D Mohr-Coulomb under every scheme (with
+softening, and with separated particles); and fast mode on the same decks
+within the per-step tolerance."""
 import numpy as np
 import pytest
 
@@ -148,6 +149,40 @@ def test_mohr_coulomb_2d(scheme):
     pg, po = _run(w, 60, scheme=scheme)
     _check(pg, po, exact=True)
     assert np.any(po["statevars"][:, 0] > 0), "no plastic state reached"
+
+
+def _collapse_mc(**kw):
+    w = workloads.collapse_2d(8, 8, 0.01, 1e6, 30.0, 1800.0)
+    w.materials = mat.mohr_coulomb(1e6, 0.3, 1800.0, friction=30.0, dilation=0.0, cohesion=0.0,
+                                   **kw)[None, :]
+    return w
+
+
+@pytest.mark.parametrize("scheme,ordered", [("usl", True), ("musl", True), ("usf", True),
+                                            ("usl", False)])
+def test_mohr_coulomb_softening(scheme, ordered):
+    """Linear softening of friction and cohesion with the plastic deviatoric
+    strain (mohr_coulomb.tcc:360-384): the strength state variables change
+    in the stress kernel, the deferred return mapping reads them back."""
+    w = _collapse_mc(residual_friction=22.0, residual_cohesion=0.0, peak_pdstrain=1e-6,
+                     residual_pdstrain=1e-3, softening=True)
+    pg, po = _run(w, 100, scheme=scheme, ordered=ordered)
+    _check(pg, po, exact=ordered)
+    sv = po["statevars"]
+    assert np.any(sv[:, 0] == 1) and np.any((sv[:, 8] > 1e-6) & (sv[:, 8] < 1e-3))
+    assert np.any(sv[:, 1] < np.radians(30.0) - 1e-9), "no softened friction angle"
+
+
+@pytest.mark.parametrize("scheme", ["usl", "musl"])
+def test_mohr_coulomb_separated_state(scheme):
+    """Particles whose density falls to the critical packing density are
+    separated: zero stress, yield state 3, invariants left as they were
+    (mohr_coulomb.tcc:391-396)."""
+    w = _collapse_mc(packing_fraction=0.6, packing_fraction_minimum=0.6 * (1 - 3e-4))
+    pg, po = _run(w, 60, scheme=scheme)
+    _check(pg, po, exact=True)
+    sep = po["statevars"][:, 0] == 3
+    assert np.any(sep) and np.all(po["stress"][sep] == 0.0)
 
 
 @pytest.mark.parametrize("scheme", ["usf", "usl", "musl"])
