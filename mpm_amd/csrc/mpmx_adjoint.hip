@@ -906,11 +906,14 @@ __global__ __launch_bounds__(kAdBlock) void k_ad_b5(Mesh M, Particles P, Slot s0
 // The deferred plastic Mohr-Coulomb updates: constitutive adjoint by
 // forward-mode passes (stress adjoint and parameter gradients written; the
 // strain-increment adjoint left in Q.buf).
+// One wave per workgroup: the waves' run times vary with their particles'
+// iteration counts, and a finished wave frees its slot at once.
+constexpr int kAdMcBlock = 64;
 template <int D>
-__global__ __launch_bounds__(kAdBlock) MPMX_AD_ATTR void k_ad_mc(Particles P,
-                                                                 const Material* mats, Slot s0,
-                                                                 AdjParticles L, AdjList Q) {
-  const int64_t slot = blockIdx.x * (int64_t)kAdBlock + threadIdx.x;
+__global__ __launch_bounds__(kAdMcBlock) MPMX_AD_ATTR void k_ad_mc(Particles P,
+                                                                   const Material* mats, Slot s0,
+                                                                   AdjParticles L, AdjList Q) {
+  const int64_t slot = blockIdx.x * (int64_t)kAdMcBlock + threadIdx.x;
   if (slot >= *Q.count) return;
   const int64_t np = P.np;
   const int p = Q.ids[slot];
@@ -989,7 +992,8 @@ static void backward_scheme(const Mesh& M, const Particles& P, const Nodes& Nd,
   const dim3 gp = ad_grid(P.np), gn = ad_grid(M.nnodes), b(kAdBlock);
   // the plastic Mohr-Coulomb updates queued by the stress pass
   auto plastic = [&](const double* nv, double* lnv) {
-    hipLaunchKernelGGL((k_ad_mc<D>), gp, b, 0, st, P, mats, s0, L, Q);
+    hipLaunchKernelGGL((k_ad_mc<D>), dim3((unsigned)((P.np + kAdMcBlock - 1) / kAdMcBlock)),
+                       dim3(kAdMcBlock), 0, st, P, mats, s0, L, Q);
     hipLaunchKernelGGL((k_ad_mc_tail<D>), gp, b, 0, st, M, P, s0, nv, lnv, L, Q, a);
   };
   hipLaunchKernelGGL((k_ad_node_fwd<D>), gn, b, 0, st, M, Nd, A, Q.count);

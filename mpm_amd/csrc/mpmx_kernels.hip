@@ -811,12 +811,14 @@ __global__ __launch_bounds__(kBlock) MPMX_PRED_ATTR void k_g2p(Mesh M, Particles
 // stress, state variables and strain increment, with the trial state from
 // the list record. Grid-stride loop over the shear entries (front of the
 // list), then the tensile ones (back).
-__global__ __launch_bounds__(kBlock) MPMX_PLASTIC_ATTR void k_plastic(Particles P,
-                                                                      const Material* mats,
-                                                                      const Status* S) {
+constexpr int kPlasticBlock = 64;  // one wave per workgroup (run times vary by wave)
+__global__ __launch_bounds__(kPlasticBlock) MPMX_PLASTIC_ATTR void k_plastic(Particles P,
+                                                                             const Material* mats,
+                                                                             const Status* S) {
   const int ns = S->nplastic, nt = S->nplastic_t;
   const int64_t np = P.np;
-  for (int c = blockIdx.x * kBlock + threadIdx.x; c < ns + nt; c += gridDim.x * kBlock) {
+  for (int c = blockIdx.x * kPlasticBlock + threadIdx.x; c < ns + nt;
+       c += gridDim.x * kPlasticBlock) {
     const int slot = c < ns ? c : (int)(np - 1 - (c - ns));
     const int p = P.plastic_list[slot];
     const double* r = P.plastic_rec + slot;
@@ -1301,9 +1303,9 @@ void launch_stress(bool locate, const Mesh& M, const Particles& P, const Nodes& 
 
 void launch_plastic(const Particles& P, const Material* mats, const Status* S, hipStream_t st) {
   // enough workgroups to fill the device twice over; each loops over the list
-  const int64_t blocks = std::min<int64_t>((P.np + kBlock - 1) / kBlock, 2048);
+  const int64_t blocks = std::min<int64_t>((P.np + kPlasticBlock - 1) / kPlasticBlock, 8192);
   if (blocks < 1) return;
-  hipLaunchKernelGGL(k_plastic, dim3((unsigned)blocks), dim3(kBlock), 0, st, P, mats, S);
+  hipLaunchKernelGGL(k_plastic, dim3((unsigned)blocks), dim3(kPlasticBlock), 0, st, P, mats, S);
 }
 
 template <int D>
