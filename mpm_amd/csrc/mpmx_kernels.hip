@@ -16,9 +16,11 @@
 
 namespace mpmx {
 
-// Workgroup size of the per-particle and per-node kernels: two waves (the
-// 2M-particle collapse's stress kernel runs 3.5 % faster than with four).
-static constexpr int kBlock = 128;
+static constexpr int kBlock = 256;
+// Workgroup size of the stress kernel k_stress: two waves (the 2M-particle
+// collapse's stress kernel runs 3.5 % faster than with four; the 2D G2P with
+// the stress update, k_g2p, is 15 % slower with two and keeps kBlock).
+static constexpr int kStressBlock = 128;
 
 // Occupancy floor (waves per SIMD) for the kernels that run the constitutive
 // update; 0 leaves the register budget to the compiler. Left alone, the
@@ -721,11 +723,13 @@ __global__ __launch_bounds__(CH) MPMX_MC_ATTR void k_stress_scatter(Mesh M, Part
 // Stress update in particle order (USL/MUSL postcompute, and USF in ordered
 // mode); optionally locate afterwards.
 template <int D, bool LOCATE>
-__global__ __launch_bounds__(kBlock) MPMX_PRED_ATTR void k_stress(Mesh M, Particles P, Nodes Nd,
-                                                   const Material* mats, StepArgs A, Status* S,
-                                                   int* scan_list) {
+__global__ __launch_bounds__(kStressBlock) MPMX_PRED_ATTR void k_stress(Mesh M, Particles P,
+                                                                      Nodes Nd,
+                                                                      const Material* mats,
+                                                                      StepArgs A, Status* S,
+                                                                      int* scan_list) {
   constexpr int NN = Elem<D>::NN;
-  const int64_t p = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+  const int64_t p = blockIdx.x * (int64_t)kStressBlock + threadIdx.x;
   if (p >= P.np) return;
   const int64_t np = P.np;
   const int c = P.cell[p];
@@ -1298,9 +1302,9 @@ void launch_stress(bool locate, const Mesh& M, const Particles& P, const Nodes& 
                    const Material* mats, const StepArgs& A, Status* S, int* scan_list,
                    hipStream_t st) {
   if (locate)
-    MPMX_LAUNCH((k_stress<D, true>), P.np, st, M, P, Nd, mats, A, S, scan_list);
+    MPMX_LAUNCH_CH(kStressBlock, (k_stress<D, true>), P.np, st, M, P, Nd, mats, A, S, scan_list);
   else
-    MPMX_LAUNCH((k_stress<D, false>), P.np, st, M, P, Nd, mats, A, S, scan_list);
+    MPMX_LAUNCH_CH(kStressBlock, (k_stress<D, false>), P.np, st, M, P, Nd, mats, A, S, scan_list);
 }
 
 void launch_plastic(const Particles& P, const Material* mats, const Status* S, hipStream_t st) {
