@@ -133,6 +133,19 @@ int mpmx_set_math_functions(mpmx_ctx* ctx, int nf, const int* type, const int* n
 int mpmx_set_acceleration_constraints_fn(mpmx_ctx* ctx, int64_t n, const int64_t* node,
                                          const int* dir, const double* value, const int* fn);
 
+/* Concentrated nodal forces (the reference's concentrated_nodal_forces,
+ * MPMBase::initialise_loads mpm_base.tcc:913-953, Mesh::
+ * assign_nodal_concentrated_forces mesh.tcc:1471-1497, 2155-2181): force[i]
+ * in direction dir[i] of node node[i] (-1: every node), scaled at time
+ * t = step * dt by math function fn[i] (-1 or a null array: none). As in
+ * Node::assign_concentrated_force (node.tcc:82-101) a later assignment
+ * replaces the node's force in that direction and the node's function for
+ * all directions. Added to the nodes' external force after the particles'
+ * body forces (Node::apply_concentrated_force, node.tcc:103-111;
+ * mpm_scheme.tcc:156-160). */
+int mpmx_set_nodal_forces(mpmx_ctx* ctx, int64_t n, const int64_t* node, const int* dir,
+                          const double* force, const int* fn);
+
 /* Particle velocity constraints: after the velocity update of every step,
  * velocity component dir[i] of particle particle[i] (-1: every particle)
  * is set to value[i], constraints applied in the order given (the reference's

@@ -740,6 +740,16 @@ MPMX_DEV void node_compute_velocity(double mass, const double* mom, double* v, d
   if (c) apply_velocity_constraints<D>(*c, v, a);
 }
 
+// Node::apply_concentrated_force (node.tcc:103-111) at the step's time:
+// external force += f(t) * concentrated force, every direction.
+template <int D>
+MPMX_DEV void add_concentrated_force(const NodeCons* c, const double* fnval, double* fe) {
+  if (!c || !c->has_cf) return;
+  const double s = c->cfn >= 0 ? fnval[c->cfn] : 1.0;
+#pragma unroll
+  for (int k = 0; k < D; ++k) fe[k] = fe[k] + s * c->cf[k];
+}
+
 // node.tcc:309-346 compute_acceleration_velocity_cundall
 template <int D>
 MPMX_DEV bool node_acc_vel(double mass, const double* fext, const double* fint, double* v,

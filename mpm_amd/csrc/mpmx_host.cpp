@@ -68,6 +68,9 @@ struct DBuf {
 struct HostCons {
   std::map<int, double> vel, acc;  // dir -> value (first insert wins)
   std::map<int, int> accfn;        // dir -> math function of the acceleration constraint
+  bool has_cf = false;             // concentrated force: per direction, one function
+  double cf[3] = {0., 0., 0.};
+  int cfn = -1;
   int fdir = -1, fsign = 0;
   double fmu = 0;
 };
@@ -529,6 +532,9 @@ void upload_constraints(mpmx_ctx* c) {
     std::memset(&nc, 0, sizeof(nc));
     nc.fdir = -1;
     nc.afn[0] = nc.afn[1] = nc.afn[2] = -1;
+    nc.has_cf = kv.second.has_cf ? 1 : 0;
+    nc.cfn = kv.second.cfn;
+    for (int k = 0; k < 3; ++k) nc.cf[k] = kv.second.cf[k];
     for (auto& v : kv.second.vel) {
       nc.vmask |= 1 << v.first;
       nc.vval[v.first] = v.second;
@@ -1386,6 +1392,29 @@ int mpmx_set_acceleration_constraints_fn(mpmx_ctx* c, int64_t n, const int64_t* 
       throw ApiError(MPMX_E_ARG,
                      "Acceleration and velocity constraints set in the same direction for the same node");
     if (hc.acc.insert({dir[i], value[i]}).second && fn && fn[i] >= 0) hc.accfn[dir[i]] = fn[i];
+  }
+  upload_constraints(c);
+  API_END(c)
+}
+
+int mpmx_set_nodal_forces(mpmx_ctx* c, int64_t n, const int64_t* node, const int* dir,
+                          const double* force, const int* fn) {
+  API_BEGIN(c)
+  require_mesh(c);
+  for (int64_t i = 0; i < n; ++i) {
+    if (node[i] < -1 || node[i] >= c->nnodes || dir[i] < 0 || dir[i] >= c->dim)
+      throw ApiError(MPMX_E_ARG, "nodal force node/dir out of range");
+    if (fn && (fn[i] < -1 || fn[i] >= (int)c->fns.size()))
+      throw ApiError(MPMX_E_ARG, "nodal force math function index out of range");
+  }
+  for (int64_t i = 0; i < n; ++i) {
+    const int64_t b = node[i] < 0 ? 0 : node[i], e = node[i] < 0 ? c->nnodes : node[i] + 1;
+    for (int64_t nd = b; nd < e; ++nd) {
+      auto& hc = c->cons[nd];
+      hc.has_cf = true;
+      hc.cf[dir[i]] = force[i];
+      hc.cfn = fn ? fn[i] : -1;
+    }
   }
   upload_constraints(c);
   API_END(c)

@@ -42,6 +42,12 @@ struct NodeCons {
   double fmu;
   double vval[3], aval[3];
   int afn[3];    // math function scaling aval[k] (index into Nodes::fnval), -1: none
+  // concentrated nodal force (Node::concentrated_force_, node.tcc:82-111):
+  // added to the external force as cfs * cf, cfs = f(t) of math function
+  // cfn (-1: 1.0); has_cf = 0: none
+  int has_cf;
+  int cfn;
+  double cf[3];
 };
 
 // A math function of time-dependent boundary conditions: Linear (a table of

@@ -1017,6 +1017,7 @@ __global__ __launch_bounds__(kBlock) void k_node(Mesh M, Nodes Nd, StepArgs A) {
       fe[k] = Nd.fext[k * nn + n];
       fi[k] = Nd.fint[k * nn + n];
     }
+    add_concentrated_force<D>(c, Nd.fnval, fe);
     node_acc_vel<D>(mass, fe, fi, v, a, c, A.dt, A.damping, Nd.fnval);
     if (A.ad_nodes) {
       double mom[D];
@@ -1198,6 +1199,7 @@ __global__ __launch_bounds__(kBlock) void k_gather(Mesh M, Particles P, Nodes Nd
       fe[k] = Nd.fext[k * nnodes + n];
       fi[k] = Nd.fint[k * nnodes + n];
     }
+    add_concentrated_force<D>(cc, Nd.fnval, fe);
     node_acc_vel<D>(mass, fe, fi, v, a, cc, A.dt, A.damping, Nd.fnval);
     if (A.ad_nodes) {
       double mom[D];

@@ -233,6 +233,9 @@ class MPMExplicit {
   std::vector<int> adir_, afn_;
   std::vector<double> aval_;
   std::map<int64_t, int> fn_index_;  // math function id -> index
+  std::vector<int64_t> cfnode_;      // concentrated nodal forces
+  std::vector<int> cfdir_, cffn_;
+  std::vector<double> cfval_;
   std::vector<int> fn_type_, fn_npts_;
   std::vector<double> fn_params_;
   int64_t expected_particles_ = 0;
@@ -444,6 +447,43 @@ template <unsigned Tdim>
 void MPMExplicit<Tdim>::initialise_loads() {
   if (!io_->json().contains("external_loading_conditions")) return;
   const Json& l = io_->object("external_loading_conditions");
+  // concentrated_nodal_forces (mpm_base.tcc:913-953): from a file (rows node
+  // dir force, io_mesh_ascii.tcc:752-795, no function) or per node set (-1:
+  // every node) with an optional math_function_id
+  if (l.contains("concentrated_nodal_forces")) {
+    for (const Json& c : l.at("concentrated_nodal_forces").elements()) {
+      if (c.contains("file")) {
+        for (const auto& r : read_rows(io_->file_name(c.at("file").str()))) {
+          cfnode_.push_back((int64_t)r.at(0));
+          cfdir_.push_back((int)r.at(1));
+          cfval_.push_back(r.at(2));
+          cffn_.push_back(-1);
+        }
+      } else {
+        int fn = -1;
+        if (c.contains("math_function_id")) {
+          const auto it = fn_index_.find(c.at("math_function_id").integer());
+          if (it == fn_index_.end()) throw std::runtime_error("math function not defined");
+          fn = it->second;
+        }
+        const int64_t sid = c.at("nset_id").integer();
+        std::vector<int64_t> all{-1};
+        const std::vector<int64_t>* nodes = &all;
+        if (sid != -1) {
+          const auto it = node_sets_.find(sid);
+          if (it == node_sets_.end())
+            throw std::runtime_error("node set " + std::to_string(sid) + " not defined");
+          nodes = &it->second;
+        }
+        for (int64_t n : *nodes) {
+          cfnode_.push_back(n);
+          cfdir_.push_back((int)c.at("dir").integer());
+          cfval_.push_back(c.at("force").number());
+          cffn_.push_back(fn);
+        }
+      }
+    }
+  }
   if (!l.contains("gravity")) return;
   const Json& g = l.at("gravity");
   const Json& vec = g.is_array() ? g : g.at("gravity");
@@ -575,13 +615,14 @@ void MPMExplicit<Tdim>::summary() const {
               "\"dt\": %.17g, \"nsteps\": %lld, \"output_steps\": %lld, \"damping\": %.17g, "
               "\"nnodes\": %lld, \"ncells\": %lld, \"node_sets\": %zu, \"cell_sets\": %zu, "
               "\"velocity_constraints\": %zu, \"acceleration_constraints\": %zu, "
-              "\"math_functions\": %zu, \"friction_constraints\": %zu, "
+              "\"math_functions\": %zu, \"nodal_forces\": %zu, \"friction_constraints\": %zu, "
               "\"particle_velocity_constraints\": %zu, \"particles\": %lld, "
               "\"gravity\": [%.17g, %.17g, %.17g], \"ramping_time\": %.17g, \"materials\": [",
               Tdim, scheme_.c_str(), vu_ == MPMX_PIC ? "pic" : "flip", dt_, (long long)nsteps_,
               (long long)output_steps_, damping_, (long long)(coords_.size() / Tdim),
               (long long)(cells_.size() / NN), node_sets_.size(), cell_sets_.size(),
-              vnode_.size(), anode_.size(), fn_type_.size(), fnode_.size(), pvc_particle_.size(),
+              vnode_.size(), anode_.size(), fn_type_.size(), cfnode_.size(), fnode_.size(),
+              pvc_particle_.size(),
               (long long)expected_particles_,
               gravity_[0],
               gravity_[1], gravity_[2], ramp_);
@@ -626,6 +667,10 @@ bool MPMExplicit<Tdim>::solve() {
     check(ctx_, mpmx_set_acceleration_constraints_fn(ctx_, (int64_t)anode_.size(), anode_.data(),
                                                      adir_.data(), aval_.data(), afn_.data()),
           "acceleration constraints");
+  if (!cfnode_.empty())
+    check(ctx_, mpmx_set_nodal_forces(ctx_, (int64_t)cfnode_.size(), cfnode_.data(), cfdir_.data(),
+                                      cfval_.data(), cffn_.data()),
+          "concentrated nodal forces");
   if (!fnode_.empty())
     check(ctx_, mpmx_set_friction_constraints(ctx_, (int64_t)fnode_.size(), fnode_.data(),
                                               fdir_.data(), fsign_.data(), fmu_.data()),

@@ -65,7 +65,8 @@ def _lib(omp=False, libm="portable"):
                    "orc_step", "orc_phase", "orc_set_node_field", "orc_set_particle_field",
                    "orc_get_particle_scratch", "orc_assign_velocity",
                    "orc_assign_stresses", "orc_set_particle_velocity_constraints",
-                   "orc_set_acceleration_constraints_fn", "orc_set_math_functions"):
+                   "orc_set_acceleration_constraints_fn", "orc_set_math_functions",
+                   "orc_set_nodal_forces"):
             getattr(lib, fn).restype = C.c_int
         lib.orc_kat_cell_volume.restype = C.c_double
         lib.orc_kat_mc_yield.restype = C.c_int
@@ -196,6 +197,17 @@ class Oracle:
         self._check(self.lib.orc_set_acceleration_constraints_fn(
             self.ctx, C.c_int64(len(nodes)), _p(nodes, _I64), _p(dirs, _I32), _p(vals),
             _p(fns, _I32)))
+
+    def set_nodal_forces(self, nodes, dirs, forces, functions=None):
+        """Concentrated nodal forces (node -1: every node)."""
+        nodes = np.ascontiguousarray(nodes, dtype=np.int64)
+        dirs = np.ascontiguousarray(np.broadcast_to(dirs, nodes.shape), dtype=np.int32)
+        vals = np.ascontiguousarray(np.broadcast_to(forces, nodes.shape), dtype=np.float64)
+        fns = np.ascontiguousarray(np.broadcast_to(-1 if functions is None else functions,
+                                                   nodes.shape), dtype=np.int32)
+        self._check(self.lib.orc_set_nodal_forces(self.ctx, C.c_int64(len(nodes)),
+                                                  _p(nodes, _I64), _p(dirs, _I32), _p(vals),
+                                                  _p(fns, _I32)))
 
     def set_math_functions(self, functions):
         """functions: list of ("linear", xvalues, fxvalues) or

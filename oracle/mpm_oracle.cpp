@@ -694,6 +694,9 @@ struct Node {
   std::map<unsigned, double> vel_c, acc_c;
   std::map<unsigned, int> acc_fn;  // math function of an acceleration constraint
   double acc_now[3] = {0, 0, 0};   // the constraint values at the current time
+  bool has_cf = false;             // concentrated force (node.tcc:82-111)
+  double cforce[3] = {0, 0, 0};
+  int cfn = -1;
   bool friction = false;
   unsigned f_dir = 0;
   int f_sign = 0;
@@ -1282,6 +1285,17 @@ struct Solver {
     for (const auto& c : n.acc_c) n.acceleration[c.first % dim] = n.acc_now[c.first % dim];
   }
 
+  // Node::apply_concentrated_force (node.tcc:103-111) over all nodes at
+  // current_time = step * dt (mpm_scheme.tcc:156-160): external force =
+  // external force * 1 + f(t) * concentrated force
+  void apply_concentrated_forces(double current_time) {
+    for (auto& n : nodes) {
+      if (!n.has_cf) continue;
+      const double scalar = n.cfn >= 0 ? functions[n.cfn].value(current_time) : 1.0;
+      for (int k = 0; k < dim; ++k) n.fext[k] = n.fext[k] * 1. + scalar * n.cforce[k];
+    }
+  }
+
   // Mesh::update_nodal_acceleration_constraints (mesh.tcc:1414-1428) at
   // current_time = step * dt (mpm_scheme.tcc:219): each constraint's value is
   // acceleration * f(t) with f its math function, or * 1.0 without one
@@ -1397,6 +1411,7 @@ struct Solver {
     // compute_forces (mpm_scheme.tcc:131-190)
     ORC_PRAGMA(omp parallel for schedule(static))
     for (Index i = 0; i < np; ++i) map_body_force(particles[i], g);
+    apply_concentrated_forces(static_cast<double>(step) * dt);
     ORC_PRAGMA(omp parallel for schedule(static))
     for (Index i = 0; i < np; ++i) map_internal_force(particles[i]);
     // compute_particle_kinematics (mpm_scheme.tcc:214-241), always Cundall
@@ -1508,6 +1523,28 @@ int orc_set_acceleration_constraints_fn(orc_ctx* c, Index n, const Index* node, 
         throw std::runtime_error("Acceleration and velocity constraints set in the same direction");
       if (nd.acc_c.insert(std::make_pair((unsigned)dir[i], val[i])).second)
         nd.acc_fn[(unsigned)dir[i]] = fn ? fn[i] : -1;
+    }
+  })
+}
+
+// Concentrated nodal forces (mesh.tcc:1471-1497): node -1 = every node; a
+// later assignment replaces the force in its direction and the node's
+// function (node.tcc:82-101). fn null: none.
+int orc_set_nodal_forces(orc_ctx* c, Index n, const Index* node, const int* dir,
+                         const double* force, const int* fn) {
+  ORC_TRY({
+    for (Index i = 0; i < n; ++i) {
+      if ((unsigned)dir[i] >= (unsigned)c->s.dim) throw std::runtime_error("bad dir");
+      if (fn && (fn[i] < -1 || fn[i] >= (int)c->s.functions.size()))
+        throw std::runtime_error("bad math function index");
+      const Index b = node[i] < 0 ? 0 : node[i];
+      const Index e = node[i] < 0 ? (Index)c->s.nodes.size() : node[i] + 1;
+      for (Index k = b; k < e; ++k) {
+        auto& nd = c->s.nodes.at(k);
+        nd.has_cf = true;
+        nd.cforce[dir[i]] = force[i];
+        nd.cfn = fn ? fn[i] : -1;
+      }
     }
   })
 }
@@ -1774,7 +1811,10 @@ int orc_phase(orc_ctx* c, int phase, double dt, const double* arg) {
       case PH_VOLUME: for (auto& p : s.particles) s.update_volume(p); break;
       case PH_STRESS: for (auto& p : s.particles) s.compute_stress(p); break;
       case PH_UPDATE_F: for (auto& p : s.particles) s.update_F(p); break;
-      case PH_BODY_FORCE: for (auto& p : s.particles) s.map_body_force(p, arg); break;
+      case PH_BODY_FORCE:
+        for (auto& p : s.particles) s.map_body_force(p, arg);
+        s.apply_concentrated_forces(static_cast<double>(s.step) * dt);
+        break;
       case PH_INTERNAL_FORCE: for (auto& p : s.particles) s.map_internal_force(p); break;
       case PH_ACC_VEL:
         s.update_acceleration_constraints(static_cast<double>(s.step) * dt);

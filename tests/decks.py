@@ -13,7 +13,7 @@ DRIVER = os.path.join(ROOT, "mpm_amd", "mpmx_run")
 
 def write_deck(w, d, nsteps, output_steps, constraints_as="sets", uuid="deck",
                material_json=None, stresses=None, particle_constraints=None,
-               acceleration_constraints=None, math_functions=None):
+               acceleration_constraints=None, math_functions=None, nodal_forces=None):
     """w: a Workload; d: directory. Velocity constraints go to node sets
     (``constraints_as="sets"``) or to a constraints file (``"file"``).
     particle_constraints: [(particle ids or -1, dir, velocity)], written as
@@ -21,7 +21,9 @@ def write_deck(w, d, nsteps, output_steps, constraints_as="sets", uuid="deck",
     acceleration_constraints: [(node ids, dir, acceleration, math function id
     or None)], written as node sets; math_functions: the deck's
     "math_functions" entries (a Linear one with "file" gets its table
-    written to that file from "xvalues"/"fxvalues")."""
+    written to that file from "xvalues"/"fxvalues").
+    nodal_forces: [(node ids or -1, dir, force, math function id or None)],
+    written as node sets and concentrated_nodal_forces."""
     os.makedirs(d, exist_ok=True)
     dim = w.dim
     with open(os.path.join(d, "mesh.txt"), "w") as f:
@@ -66,6 +68,17 @@ def write_deck(w, d, nsteps, output_steps, constraints_as="sets", uuid="deck",
                 c["math_function_id"] = int(fid)
             ac.append(c)
         bc["acceleration_constraints"] = ac
+    nf = []
+    for ids, k, val, fid in nodal_forces or []:
+        if isinstance(ids, int) and ids == -1:
+            sid = -1
+        else:
+            sid = 500 + len(sets["node_sets"])
+            sets["node_sets"].append({"id": sid, "set": [int(n) for n in ids]})
+        c = {"nset_id": sid, "dir": int(k), "force": float(val)}
+        if fid is not None:
+            c["math_function_id"] = int(fid)
+        nf.append(c)
     fns = []
     for f in math_functions or []:
         f = dict(f)
@@ -114,6 +127,8 @@ def write_deck(w, d, nsteps, output_steps, constraints_as="sets", uuid="deck",
     }
     if fns:
         deck["math_functions"] = fns
+    if nf:
+        deck["external_loading_conditions"]["concentrated_nodal_forces"] = nf
     with open(os.path.join(d, "mpm.json"), "w") as f:
         json.dump(deck, f, indent=1)
     return os.path.join(d, "mpm.json")

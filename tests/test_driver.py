@@ -64,6 +64,11 @@ def test_dry_run_acceleration_constraints_and_math_functions(tmp_path):
     json.dump(deck, open(os.path.join(str(tmp_path), "mpm.json"), "w"))
     r = run_driver(str(tmp_path), "--dry-run", check=False)
     assert r.returncode != 0 and "math function" in r.stderr
+    d2 = os.path.join(str(tmp_path), "forces")
+    write_deck(w, d2, nsteps=5, output_steps=5, material_json=LE, math_functions=fns[:1],
+               nodal_forces=[([50, 51, 52], 1, -2.0, 5), (-1, 0, 0.5, None)])
+    s = json.loads(run_driver(d2, "--dry-run").stdout)
+    assert s["nodal_forces"] == 4 and s["math_functions"] == 1
 
 
 def test_dry_run_mohr_coulomb_3d_with_friction(tmp_path):

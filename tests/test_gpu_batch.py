@@ -19,8 +19,14 @@ pytestmark = pytest.mark.gpu
 FIELDS = ("x", "u", "v", "stress", "strain", "F", "volume", "density", "xi", "statevars")
 
 
-def _small_batch(n=4):
-    return workloads.batch_2d(ninst=n, seed=7, nmin=200, nmax=3000)
+def _small_batch(n=4, elastic=False):
+    b = workloads.batch_2d(ninst=n, seed=7, nmin=200, nmax=3000)
+    if elastic:  # the same instances with LinearElastic (same E, nu, density)
+        for w in b.instances:
+            m = w.materials[0]
+            w.materials = mat.linear_elastic(m[2], m[3], m[1])[None, :]
+        b.materials = np.concatenate([w.materials for w in b.instances])
+    return b
 
 
 def _run_solo(w, steps, ordered):
@@ -32,8 +38,12 @@ def _run_solo(w, steps, ordered):
 
 @pytest.mark.parametrize("ordered", [True, False])
 def test_batch_equals_instances_run_alone(ordered):
+    """Ordered mode: the Mohr-Coulomb batch bit for bit. Fast mode: the
+    elastic batch to 1e-10 after 40 steps (with Mohr-Coulomb, rounding-level
+    differences of the atomic sums can move a particle across a yield-state
+    threshold within 40 steps, and the comparison would measure that)."""
     from mpm_amd.capi import Solver
-    b = _small_batch()
+    b = _small_batch(elastic=not ordered)
     g = b.setup(Solver(b.dim, b.scheme, ordered=ordered))
     steps = 40
     g.step(steps, b.dt)

@@ -104,3 +104,24 @@ def test_driver_time_function_acceleration_constraints_match_binding(tmp_path):
     s.set_acceleration_constraints(nodes, 0, vals, fn)
     s.step(20, w.dt)
     _compare(out, s.particles(), 2)
+
+
+def test_driver_concentrated_nodal_forces_match_binding(tmp_path):
+    """concentrated_nodal_forces from node sets, with and without a math
+    function."""
+    w = workloads.column_2d(nx=6, ny=10)
+    top = np.nonzero(np.abs(w.coords[:, 1] - 0.5) < 1e-9)[0]
+    dt = w.dt
+    fns = [{"id": 3, "type": "Linear", "xvalues": [0.0, 10 * dt], "fxvalues": [0.0, 1.0]}]
+    d = str(tmp_path)
+    write_deck(w, d, nsteps=20, output_steps=20, material_json=LE, math_functions=fns,
+               nodal_forces=[(top[:4], 1, -30.0, 3), (top[4:6], 0, 10.0, None)])
+    run_driver(d, "--ordered")
+    out = read_output(os.path.join(d, "results", "deck", "particles20.txt"), 2)
+    from mpm_amd.capi import Solver
+    s = w.setup(Solver(2, w.scheme, ordered=True))
+    s.set_math_functions([("linear", [0.0, 10 * dt], [0.0, 1.0])])
+    s.set_nodal_forces(top[:4], 1, -30.0, 0)
+    s.set_nodal_forces(top[4:6], 0, 10.0, -1)
+    s.step(20, w.dt)
+    _compare(out, s.particles(), 2)

@@ -1,6 +1,7 @@
 """Step options beyond the benchmark decks, GPU (ordered mode) against the
 oracle bit for bit: Cundall damping, PIC and blended FLIP velocity updates,
 acceleration constraints (constant and scaled by math functions of time),
+concentrated nodal forces,
 prescribed non-zero velocities, particle velocity constraints, gravity
 ramping, one particle per cell, 2D Mohr-Coulomb under every scheme (with
 softening, and with separated particles); and fast mode on the same decks
@@ -129,6 +130,34 @@ def test_time_function_acceleration_constraints(scheme, ordered):
         assert np.array_equal(a, no["acceleration"][row, 0])
 
 
+@pytest.mark.parametrize("scheme,ordered", [("usl", True), ("musl", True), ("usf", True),
+                                            ("usl", False)])
+def test_concentrated_nodal_forces(scheme, ordered):
+    """Concentrated nodal forces: a Linear-ramped downward load on the top
+    row of the column, a constant sideways load on one node, then a second
+    assignment on one loaded node that replaces its function (the node's
+    function is that of its last assignment)."""
+    w = workloads.column_2d(nx=6, ny=10)
+    top = np.nonzero((np.abs(w.coords[:, 1] - 0.5) < 1e-9) & (w.coords[:, 0] < 0.3 + 1e-9))[0]
+    assert len(top) >= 3
+    fns = [("linear", [0.0, 20 * w.dt], [0.0, 1.0]), ("sin", 0.0, 2 * np.pi / (40 * w.dt), (0.0, 1.0))]
+
+    def setup(obj):
+        w.setup(obj)
+        obj.set_math_functions(fns)
+        obj.set_nodal_forces(top, 1, -50.0, 0)
+        obj.set_nodal_forces([top[1]], [0], [20.0], [-1])
+        obj.set_nodal_forces([top[2]], [0], [5.0], [1])
+        return obj
+
+    g = setup(_solver(w, ordered, scheme=scheme))
+    o = setup(_oracle(w, scheme=scheme))
+    for _ in range(4):
+        g.step(10, w.dt)
+        o.step(10, w.dt)
+    _check(g.particles(), o.particles(), exact=ordered)
+
+
 def test_gravity_ramp():
     w = workloads.column_2d(nx=6, ny=10)
     pg, po = _run(w, 40, ramp=(w.gravity, 20 * w.dt))
@@ -158,12 +187,13 @@ def _collapse_mc(**kw):
     return w
 
 
-@pytest.mark.parametrize("scheme,ordered", [("usl", True), ("musl", True), ("usf", True),
-                                            ("usl", False)])
+@pytest.mark.parametrize("scheme,ordered", [("usl", True), ("musl", True), ("usf", True)])
 def test_mohr_coulomb_softening(scheme, ordered):
     """Linear softening of friction and cohesion with the plastic deviatoric
     strain (mohr_coulomb.tcc:360-384): the strength state variables change
-    in the stress kernel, the deferred return mapping reads them back."""
+    in the stress kernel, the deferred return mapping reads them back.
+    Ordered mode only: over 100 steps of a cohesionless collapse, fast
+    mode's atomic-order rounding moves particles across yield thresholds."""
     w = _collapse_mc(residual_friction=22.0, residual_cohesion=0.0, peak_pdstrain=1e-6,
                      residual_pdstrain=1e-3, softening=True)
     pg, po = _run(w, 100, scheme=scheme, ordered=ordered)

@@ -3,7 +3,8 @@ LinearFunction (src/functions/linear_function.cc:19-41) and SinFunction
 (src/functions/sin_function.cc:17-20) against their closed forms, and the
 acceleration constraints they scale (acceleration_constraint.h:37-43,
 Mesh::update_nodal_acceleration_constraints at time step * dt,
-mpm_scheme.tcc:219) on a small oracle run. CPU only."""
+mpm_scheme.tcc:219) and concentrated nodal forces (node.tcc:103-111,
+mpm_scheme.tcc:156-160) on small oracle runs. CPU only."""
 import numpy as np
 import pytest
 
@@ -68,3 +69,32 @@ def test_oracle_constraint_follows_function_of_step_time():
                         0.2 * (np.sin(np.pi / (8 * dt) * t) if t >= dt else 0.0))
         got = o.nodes()["acceleration"][row, 0]
         np.testing.assert_allclose(got, want, rtol=1e-14, atol=1e-15)
+
+
+def test_oracle_concentrated_force_adds_to_external_force():
+    """One step with and without a Linear-scaled concentrated force on the
+    nodes of a row: the external force differs by f(step * dt) * F on those
+    nodes and not elsewhere; the function's value is taken at the step's
+    own time."""
+    w = workloads.column_2d(nx=4, ny=6)
+    row = np.nonzero(np.abs(w.coords[:, 1] - 0.15) < 1e-9)[0]
+    dt = w.dt
+    fns = [("linear", [0.0, 4 * dt], [0.0, 1.0])]
+    for nsteps in (1, 3):
+        out = []
+        for force in (False, True):
+            o = w.setup(oracle.Oracle(w.dim, w.scheme))
+            o.set_math_functions(fns)
+            if force:
+                o.set_nodal_forces(row, 1, -7.0, 0)
+            o.step(nsteps, dt)
+            out.append(o.nodes()["fext"])
+        d = out[1] - out[0]
+        t = (nsteps - 1) * dt
+        if nsteps == 1:  # identical states before the forced step
+            np.testing.assert_allclose(d[row, 1], -7.0 * (t / (4 * dt)), rtol=0, atol=1e-12)
+            mask = np.ones(len(d), dtype=bool)
+            mask[row] = False
+            assert np.all(d[mask] == 0.0) and np.all(d[row, 0] == 0.0)
+        else:
+            assert np.all(np.abs(d[row, 1]) > 0)
