@@ -146,6 +146,22 @@ int mpmx_set_acceleration_constraints_fn(mpmx_ctx* ctx, int64_t n, const int64_t
 int mpmx_set_nodal_forces(mpmx_ctx* ctx, int64_t n, const int64_t* node, const int* dir,
                           const double* force, const int* fn);
 
+/* Particle surface tractions (the reference's particle_surface_traction,
+ * MPMBase::initialise_loads mpm_base.tcc:882-911, Mesh::
+ * create_particles_tractions mesh.tcc:1349-1367): load i sets, at time
+ * t = step * dt, traction value[i] * f(t) (math function fn[i]; -1 or a null
+ * array: none) in direction dir[i] of particle particle[i] (-1: every
+ * particle), the loads applied in order so a later one wins per particle
+ * and direction (Mesh::apply_traction_on_particles, mesh.tcc:1371-1386).
+ * The particle's traction vector is that value times its volume over its
+ * size pow(V0, 1/dim), V0 its volume when it was created
+ * (Particle::assign_traction, particle.tcc:962-979; assign_volume
+ * :556-568); it is mapped to the nodes' external force as N_i * traction
+ * after the body forces (Particle::map_traction_force, particle.tcc:
+ * 983-990). Replaces any earlier list; n = 0 removes them. */
+int mpmx_set_particle_tractions(mpmx_ctx* ctx, int64_t n, const int64_t* particle,
+                                const int* dir, const double* value, const int* fn);
+
 /* Particle velocity constraints: after the velocity update of every step,
  * velocity component dir[i] of particle particle[i] (-1: every particle)
  * is set to value[i], constraints applied in the order given (the reference's

@@ -26,6 +26,7 @@ EXPORTS = (
     "mpmx_set_materials",
     "mpmx_set_velocity_constraints", "mpmx_set_acceleration_constraints",
     "mpmx_set_acceleration_constraints_fn", "mpmx_set_math_functions", "mpmx_set_nodal_forces",
+    "mpmx_set_particle_tractions",
     "mpmx_set_friction_constraints", "mpmx_set_particle_velocity_constraints",
     "mpmx_set_gravity", "mpmx_generate_gauss",
     "mpmx_set_particles", "mpmx_compute_volume_mass", "mpmx_assign_velocity",
@@ -74,6 +75,7 @@ def load(path=LIB_PATH):
         "mpmx_set_acceleration_constraints_fn": [V, C.c_int64, _I64, _I32, _D, _I32],
         "mpmx_set_math_functions": [V, C.c_int, _I32, _I32, _D],
         "mpmx_set_nodal_forces": [V, C.c_int64, _I64, _I32, _D, _I32],
+        "mpmx_set_particle_tractions": [V, C.c_int64, _I64, _I32, _D, _I32],
         "mpmx_set_friction_constraints": [V, C.c_int64, _I64, _I32, _I32, _D],
         "mpmx_set_particle_velocity_constraints": [V, C.c_int64, _I64, _I32, _D],
         "mpmx_set_gravity": [V, _D, C.c_double],
@@ -222,6 +224,12 @@ class Solver:
         entry the index of its math function (-1: none)."""
         fn = -1 if functions is None else functions
         self._cons(self.lib.mpmx_set_nodal_forces, nodes, dirs, (forces, _D), (fn, _I32))
+
+    def set_particle_tractions(self, particles, dirs, values, functions=None):
+        """Particle surface tractions in order (particle -1: every particle);
+        functions: per load the index of its math function (-1: none)."""
+        fn = -1 if functions is None else functions
+        self._cons(self.lib.mpmx_set_particle_tractions, particles, dirs, (values, _D), (fn, _I32))
 
     def set_math_functions(self, functions):
         """Math functions of time-dependent constraints, indexed in list order:

@@ -740,6 +740,23 @@ MPMX_DEV void node_compute_velocity(double mass, const double* mom, double* v, d
   if (c) apply_velocity_constraints<D>(*c, v, a);
 }
 
+// Traction vector of particle p at the step's time (Traction::traction,
+// traction.h; Particle::assign_traction, particle.tcc:962-979):
+// ((value * f(t)) * volume) / size per direction, 0 where none is set.
+template <int D>
+MPMX_DEV void particle_traction(const Particles& P, const double* fnval, int64_t p, double* t) {
+  const int64_t np = P.np;
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    const int fn = P.tr_fn[k * np + p];
+    t[k] = 0.;
+    if (fn >= -1) {
+      const double s = fn >= 0 ? fnval[fn] : 1.0;
+      t[k] = ((P.tr_val[k * np + p] * s) * P.vol[p]) / P.psize[p];
+    }
+  }
+}
+
 // Node::apply_concentrated_force (node.tcc:103-111) at the step's time:
 // external force += f(t) * concentrated force, every direction.
 template <int D>

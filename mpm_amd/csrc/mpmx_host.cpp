@@ -129,6 +129,18 @@ struct mpmx_ctx {
   std::vector<Pvc> pvc;
   DBuf<unsigned char> d_pvc_mask;
   DBuf<double> d_pvc_val;
+  // particle tractions (in order) and the particles' sizes pow(V0, 1/dim)
+  struct Trac {
+    int64_t particle;
+    int dir;
+    double value;
+    int fn;
+  };
+  std::vector<Trac> trac;
+  std::vector<double> psize;
+  int ntr = 0;
+  DBuf<int> d_tr_list, d_tr_fn;
+  DBuf<double> d_tr_val, d_psize;
 
   // particles
   int64_t np = 0;
@@ -228,6 +240,11 @@ struct mpmx_ctx {
     P.pvc_val = pvc.empty() ? nullptr : d_pvc_val.p;
     P.plastic_list = any_mc ? d_plastic_list.p : nullptr;
     P.plastic_rec = any_mc ? d_plastic_rec.p : nullptr;
+    P.ntr = ntr;
+    P.tr_list = ntr ? d_tr_list.p : nullptr;
+    P.tr_val = ntr ? d_tr_val.p : nullptr;
+    P.tr_fn = ntr ? d_tr_fn.p : nullptr;
+    P.psize = ntr ? d_psize.p : nullptr;
     return P;
   }
   Nodes nodes() const {
@@ -621,6 +638,49 @@ void build_pvc(mpmx_ctx* c) {
   HIPCHECK(hipStreamSynchronize(c->stream));
 }
 
+// Per particle and direction the last traction load that covers it
+// (Particle::assign_traction overwrites per direction), the list of the
+// particles with any, and the particle sizes.
+void build_trac(mpmx_ctx* c) {
+  c->ntr = 0;
+  c->invalidate_graph();
+  if (c->trac.empty() || c->np == 0) return;
+  const int64_t np = c->np;
+  const int d = c->dim;
+  std::vector<int> fn((size_t)d * np, -2);
+  std::vector<double> val((size_t)d * np, 0.0);
+  for (const auto& t : c->trac) {
+    const int64_t b = t.particle < 0 ? 0 : t.particle;
+    const int64_t e = t.particle < 0 ? np : std::min<int64_t>(t.particle + 1, np);
+    for (int64_t p = b; p < e; ++p) {
+      fn[(size_t)t.dir * np + p] = t.fn;
+      val[(size_t)t.dir * np + p] = t.value;
+    }
+  }
+  std::vector<int> list;
+  for (int64_t p = 0; p < np; ++p) {
+    bool any = false;
+    for (int k = 0; k < d; ++k) any = any || fn[(size_t)k * np + p] >= -1;
+    if (any) list.push_back((int)p);
+  }
+  if (list.empty()) return;
+  c->d_tr_list.alloc(list.size());
+  c->d_tr_list.upload(list.data(), list.size(), c->stream);
+  c->d_tr_fn.alloc((size_t)d * np);
+  c->d_tr_fn.upload(fn.data(), (size_t)d * np, c->stream);
+  c->d_tr_val.alloc((size_t)d * np);
+  c->d_tr_val.upload(val.data(), (size_t)d * np, c->stream);
+  c->d_psize.alloc(np);
+  c->d_psize.upload(c->psize.data(), np, c->stream);
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  c->ntr = (int)list.size();
+}
+
+// Particle size of a volume (Particle::assign_volume, particle.tcc:556-568).
+double particle_size(double volume, int dim) {
+  return std::pow(volume, static_cast<double>(1. / dim));
+}
+
 void alloc_step_buffers(mpmx_ctx* c) {
   const int64_t np = c->np;
   c->d_skey.alloc(std::max<int64_t>(np, 1));
@@ -716,7 +776,10 @@ int64_t append_particles(mpmx_ctx* c, int64_t n, const double* x, const double* 
   if (c->any_mc) grow(c->d_sv, off, np, NSV, st);
   c->np = np;
   c->ad_recorded = false;
+  c->psize.resize(np);
+  for (int64_t i = 0; i < n; ++i) c->psize[off + i] = particle_size(volume ? volume[i] : 1.0, d);
   build_pvc(c);
+  build_trac(c);
   std::vector<double> zeros((size_t)n * 9, 0.0), eye((size_t)n * 9, 0.0);
   for (int64_t i = 0; i < n; ++i) eye[i * 9 + 0] = eye[i * 9 + 4] = eye[i * 9 + 8] = 1.0;
   put_soa(c->d_x, np, off, n, d, x, st);
@@ -771,7 +834,9 @@ void truncate_particles(mpmx_ctx* c, int64_t keep) {
   if (c->any_mc) shrink(c->d_sv, NSV);
   c->np = keep;
   c->ad_recorded = false;
+  c->psize.resize(keep);
   build_pvc(c);
+  build_trac(c);
   alloc_step_buffers(c);
   c->invalidate_graph();
 }
@@ -868,15 +933,18 @@ void enqueue_step(mpmx_ctx* c, const StepArgs& A, Launcher& L, bool full_sort) {
       L("p2g_mass_momentum", [&] { launch_p2g<D>(P2G_MV, c->chunk, M, P, Nd, A, S, skey, perm, st); });
       L("node_velocity", [&] { launch_node<D>(ND_INIT | ND_VEL, M, Nd, A, st); });
       L("stress_force_scatter", [&] { launch_stress_scatter<D>(c->chunk, M, P, Nd, mats, A, S, skey, perm, st); });
+      if (P.ntr > 0) L("traction_scatter", [&] { launch_traction_scatter<D>(M, P, Nd, st); });
       L("node_acceleration", [&] { launch_node<D>(ND_ACC | ND_ZERO_ALL, M, Nd, A, st); });
       L("g2p_locate", [&] { launch_g2p<D>(false, true, M, P, Nd, mats, A, c->vu, S, scan, st); });
     } else if (c->scheme == USL) {
       L("p2g_all", [&] { launch_p2g<D>(P2G_MV_F, c->chunk, M, P, Nd, A, S, skey, perm, st); });
+      if (P.ntr > 0) L("traction_scatter", [&] { launch_traction_scatter<D>(M, P, Nd, st); });
       L("node_update", [&] { launch_node<D>(ND_INIT | ND_VEL | ND_ACC | ND_ZERO_ALL, M, Nd, A, st); });
       L("g2p_stress_locate", [&] { launch_g2p<D>(true, true, M, P, Nd, mats, A, c->vu, S, scan, st); });
       plastic();
     } else {
       L("p2g_all", [&] { launch_p2g<D>(P2G_MV_F, c->chunk, M, P, Nd, A, S, skey, perm, st); });
+      if (P.ntr > 0) L("traction_scatter", [&] { launch_traction_scatter<D>(M, P, Nd, st); });
       L("node_update", [&] { launch_node<D>(ND_INIT | ND_VEL | ND_ACC | ND_ZERO_MOM, M, Nd, A, st); });
       L("g2p_momentum_scatter", [&] { launch_g2p_scatter<D>(c->chunk, M, P, Nd, A, c->vu, skey, perm, st); });
       L("node_velocity", [&] { launch_node<D>(ND_VEL | ND_ZERO_ALL, M, Nd, A, st); });
@@ -1420,6 +1488,22 @@ int mpmx_set_nodal_forces(mpmx_ctx* c, int64_t n, const int64_t* node, const int
   API_END(c)
 }
 
+int mpmx_set_particle_tractions(mpmx_ctx* c, int64_t n, const int64_t* particle, const int* dir,
+                                const double* value, const int* fn) {
+  API_BEGIN(c)
+  std::vector<mpmx_ctx::Trac> t;
+  for (int64_t i = 0; i < n; ++i) {
+    if (particle[i] < -1 || dir[i] < 0 || dir[i] >= c->dim)
+      throw ApiError(MPMX_E_ARG, "particle traction particle/dir out of range");
+    if (fn && (fn[i] < -1 || fn[i] >= (int)c->fns.size()))
+      throw ApiError(MPMX_E_ARG, "particle traction math function index out of range");
+    t.push_back({particle[i], dir[i], value[i], fn ? fn[i] : -1});
+  }
+  c->trac = t;
+  build_trac(c);
+  API_END(c)
+}
+
 int mpmx_set_acceleration_constraints(mpmx_ctx* c, int64_t n, const int64_t* node, const int* dir,
                                       const double* value) {
   return mpmx_set_acceleration_constraints_fn(c, n, node, dir, value, nullptr);
@@ -1595,6 +1679,8 @@ int mpmx_generate_gauss(mpmx_ctx* c, int nppd, int64_t nset, const int64_t* cset
   c->d_vol.upload(vol.data(), n, c->stream, off);
   c->d_mass.upload(mass.data(), n, c->stream, off);
   HIPCHECK(hipStreamSynchronize(c->stream));
+  for (int64_t i = 0; i < n; ++i) c->psize[off + i] = particle_size(vol[i], c->dim);
+  build_trac(c);
   API_END(c)
 }
 
@@ -1632,6 +1718,8 @@ int mpmx_compute_volume_mass(mpmx_ctx* c) {
   c->d_rho.upload(rho.data(), n, c->stream);
   c->d_mass.upload(mass.data(), n, c->stream);
   HIPCHECK(hipStreamSynchronize(c->stream));
+  for (int64_t i = 0; i < n; ++i) c->psize[i] = particle_size(vol[i], c->dim);
+  build_trac(c);
   API_END(c)
 }
 

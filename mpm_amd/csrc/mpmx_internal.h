@@ -103,6 +103,16 @@ struct Particles {
   // (PR_* below), by list slot. Null when no MohrCoulomb material.
   int* plastic_list;
   double* plastic_rec;  // [PR_N][np] by list slot
+  // Particle tractions (Particle::assign_traction / map_traction_force,
+  // particle.tcc:962-990): the ntr particles tr_list[] carry a traction;
+  // per direction k its value tr_val[k][p] times math function tr_fn[k][p]
+  // (-1: 1.0; -2: no traction in that direction, 0), times the volume,
+  // over the particle size psize[p]. Null / 0 when none.
+  const int* tr_list;
+  int ntr;
+  const double* tr_val;
+  const int* tr_fn;
+  const double* psize;
 };
 
 // Record of a deferred return mapping: the strain increment and what the

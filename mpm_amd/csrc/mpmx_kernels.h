@@ -38,6 +38,9 @@ template <int D>
 void launch_stress(bool locate, const Mesh& M, const Particles& P, const Nodes& Nd,
                    const Material* mats, const StepArgs& A, Status* S, int* scan_list,
                    hipStream_t st);
+// Particle tractions onto the nodes' external force (fast mode; P.ntr > 0).
+template <int D>
+void launch_traction_scatter(const Mesh& M, const Particles& P, const Nodes& Nd, hipStream_t st);
 // The deferred Mohr-Coulomb return mappings queued by launch_stress and
 // launch_g2p (stress) this step; runs after them, before anything reads the
 // stress or state variables.

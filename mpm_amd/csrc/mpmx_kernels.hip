@@ -936,6 +936,31 @@ __global__ __launch_bounds__(CH) void k_g2p_scatter(Mesh M, Particles P, Nodes N
 }
 
 // ------------------------------- node kernels -------------------------------
+// Particle tractions mapped to the nodes' external force
+// (Particle::map_traction_force, particle.tcc:983-990), one thread per
+// particle with a traction, FP64 atomics (fast mode).
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_traction_scatter(Mesh M, Particles P, Nodes Nd) {
+  constexpr int NN = Elem<D>::NN;
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= P.ntr) return;
+  const int64_t p = P.tr_list[t];
+  const int64_t np = P.np;
+  const int c = P.cell[p];
+  if (!valid_cell<D>(M, c)) return;
+  double xi[D], N[NN], tr[D];
+#pragma unroll
+  for (int k = 0; k < D; ++k) xi[k] = P.xi[k * np + p];
+  shapefn<D>(xi, N);
+  particle_traction<D>(P, Nd.fnval, p, tr);
+#pragma unroll
+  for (int i = 0; i < NN; ++i) {
+    const int n = M.cell_nodes[(int64_t)c * NN + i];
+#pragma unroll
+    for (int k = 0; k < D; ++k) atomic_add(&Nd.fext[k * M.nnodes + n], N[i] * tr[k]);
+  }
+}
+
 // Where this step's nodal record goes (StepArgs::ad_nodes), or null.
 MPMX_DEV double* ad_record_base(const StepArgs& A) {
   if (!A.ad_nodes) return nullptr;
@@ -1115,6 +1140,9 @@ __global__ __launch_bounds__(kBlock) void k_gather(Mesh M, Particles P, Nodes Nd
   double am = 0., amom[D], afe[D], afi[D];
 #pragma unroll
   for (int k = 0; k < D; ++k) amom[k] = afe[k] = afi[k] = 0.;
+  int h0[NN];
+#pragma unroll
+  for (int j = 0; j < NN; ++j) h0[j] = h[j];
   while (true) {
     int bj = -1, bp = 0x7fffffff;
 #pragma unroll
@@ -1162,6 +1190,40 @@ __global__ __launch_bounds__(kBlock) void k_gather(Mesh M, Particles P, Nodes Nd
       const double mv = -1. * P.vol[p];
 #pragma unroll
       for (int k = 0; k < D; ++k) afi[k] = afi[k] + f[k] * mv;
+    }
+  }
+  // particle tractions after every body force, in particle order
+  // (Mesh::apply_traction_on_particles, mesh.tcc:1371-1386)
+  if constexpr ((WHAT & SC_F) != 0) {
+    if (P.ntr > 0) {
+#pragma unroll
+      for (int j = 0; j < NN; ++j) h[j] = h0[j];
+      while (true) {
+        int bj = -1, bp = 0x7fffffff;
+#pragma unroll
+        for (int j = 0; j < NN; ++j)
+          if (h[j] < e[j]) {
+            const int p = perm[h[j]];
+            if (p < bp) {
+              bp = p;
+              bj = j;
+            }
+          }
+        if (bj < 0) break;
+        h[bj] += 1;
+        const int p = bp, i = loc[bj];
+        bool has = false;
+#pragma unroll
+        for (int k = 0; k < D; ++k) has = has || P.tr_fn[k * np + p] >= -1;
+        if (!has) continue;
+        double xi[D], t[D];
+#pragma unroll
+        for (int k = 0; k < D; ++k) xi[k] = P.xi[k * np + p];
+        const double N = shapefn_i<D>(xi, i);
+        particle_traction<D>(P, Nd.fnval, p, t);
+#pragma unroll
+        for (int k = 0; k < D; ++k) afe[k] = afe[k] + N * t[k];
+      }
     }
   }
   if constexpr ((WHAT & SC_MV) != 0) Nd.mass[n] = am;
@@ -1309,6 +1371,11 @@ void launch_stress(bool locate, const Mesh& M, const Particles& P, const Nodes& 
     MPMX_LAUNCH_CH(kStressBlock, (k_stress<D, false>), P.np, st, M, P, Nd, mats, A, S, scan_list);
 }
 
+template <int D>
+void launch_traction_scatter(const Mesh& M, const Particles& P, const Nodes& Nd, hipStream_t st) {
+  MPMX_LAUNCH((k_traction_scatter<D>), P.ntr, st, M, P, Nd);
+}
+
 void launch_plastic(const Particles& P, const Material* mats, const Status* S, hipStream_t st) {
   // enough workgroups to fill the device twice over; each loops over the list
   const int64_t blocks = std::min<int64_t>((P.np + kPlasticBlock - 1) / kPlasticBlock, 8192);
@@ -1407,7 +1474,9 @@ void launch_gather(int which, const Mesh& M, const Particles& P, const Nodes& Nd
   template void launch_cell_ranges<D>(int64_t, int64_t, const int*, int*, int*, hipStream_t);\
   template void launch_gather<D>(int, const Mesh&, const Particles&, const Nodes&,           \
                                  const StepArgs&, const Status*, const int*, const int*,     \
-                                 const int*, hipStream_t);
+                                 const int*, hipStream_t);                                   \
+  template void launch_traction_scatter<D>(const Mesh&, const Particles&, const Nodes&,      \
+                                           hipStream_t);
 
 MPMX_INSTANTIATE(2)
 MPMX_INSTANTIATE(3)

@@ -559,6 +559,42 @@ void MPMExplicit<Tdim>::initialise_particles() {
   }
   if (!ctx_) return;
   check(ctx_, mpmx_compute_volume_mass(ctx_), "particle volume and mass");
+  // particle_surface_traction (mpm_base.tcc:882-911): a particle set id (-1:
+  // all particles), a direction, a traction and an optional math function;
+  // applied in order
+  if (io_->json().contains("external_loading_conditions") &&
+      io_->object("external_loading_conditions").contains("particle_surface_traction")) {
+    std::vector<int64_t> tp;
+    std::vector<int> td, tf;
+    std::vector<double> tv;
+    for (const Json& c :
+         io_->object("external_loading_conditions").at("particle_surface_traction").elements()) {
+      const int64_t pset = c.at("pset_id").integer();
+      int fn = -1;
+      if (c.contains("math_function_id")) {
+        const auto it = fn_index_.find(c.at("math_function_id").integer());
+        if (it == fn_index_.end()) throw std::runtime_error("math function not defined");
+        fn = it->second;
+      }
+      std::vector<int64_t> all{-1};
+      const std::vector<int64_t>* ids = &all;
+      if (pset != -1) {
+        auto it = particle_sets_.find(pset);
+        if (it == particle_sets_.end())
+          throw std::runtime_error("No particle set found to assign traction");
+        ids = &it->second;
+      }
+      for (int64_t p : *ids) {
+        tp.push_back(p);
+        td.push_back((int)c.at("dir").integer());
+        tv.push_back(c.at("traction").number());
+        tf.push_back(fn);
+      }
+    }
+    check(ctx_, mpmx_set_particle_tractions(ctx_, (int64_t)tp.size(), tp.data(), td.data(),
+                                            tv.data(), tf.data()),
+          "particle surface tractions");
+  }
   if (!pvc_particle_.empty())
     check(ctx_, mpmx_set_particle_velocity_constraints(ctx_, (int64_t)pvc_particle_.size(),
                                                        pvc_particle_.data(), pvc_dir_.data(),

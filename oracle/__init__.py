@@ -66,7 +66,7 @@ def _lib(omp=False, libm="portable"):
                    "orc_get_particle_scratch", "orc_assign_velocity",
                    "orc_assign_stresses", "orc_set_particle_velocity_constraints",
                    "orc_set_acceleration_constraints_fn", "orc_set_math_functions",
-                   "orc_set_nodal_forces"):
+                   "orc_set_nodal_forces", "orc_set_particle_tractions"):
             getattr(lib, fn).restype = C.c_int
         lib.orc_kat_cell_volume.restype = C.c_double
         lib.orc_kat_mc_yield.restype = C.c_int
@@ -208,6 +208,17 @@ class Oracle:
         self._check(self.lib.orc_set_nodal_forces(self.ctx, C.c_int64(len(nodes)),
                                                   _p(nodes, _I64), _p(dirs, _I32), _p(vals),
                                                   _p(fns, _I32)))
+
+    def set_particle_tractions(self, particles, dirs, values, functions=None):
+        """Particle surface tractions in order (particle -1: every particle)."""
+        particles = np.ascontiguousarray(particles, dtype=np.int64)
+        dirs = np.ascontiguousarray(np.broadcast_to(dirs, particles.shape), dtype=np.int32)
+        vals = np.ascontiguousarray(np.broadcast_to(values, particles.shape), dtype=np.float64)
+        fns = np.ascontiguousarray(np.broadcast_to(-1 if functions is None else functions,
+                                                   particles.shape), dtype=np.int32)
+        self._check(self.lib.orc_set_particle_tractions(self.ctx, C.c_int64(len(particles)),
+                                                        _p(particles, _I64), _p(dirs, _I32),
+                                                        _p(vals), _p(fns, _I32)))
 
     def set_math_functions(self, functions):
         """functions: list of ("linear", xvalues, fxvalues) or

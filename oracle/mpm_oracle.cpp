@@ -726,6 +726,18 @@ struct Particle {
   double dF[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
   int mat = 0;
   double sv[kNsv] = {0};
+  double size = 0;                  // pow(volume, 1/dim) at creation (particle.tcc:563-568)
+  double traction[3] = {0, 0, 0};   // particle.tcc:962-979
+  bool has_traction = false;
+};
+
+// A particle traction (Traction, traction.h): set (-1: every particle), dir,
+// value, math function (-1: none).
+struct TractionLoad {
+  Index particle;
+  int dir;
+  double value;
+  int fn;
 };
 
 enum { SCHEME_USF = 0, SCHEME_USL = 1, SCHEME_MUSL = 2 };
@@ -744,6 +756,7 @@ struct Solver {
   double ramp = 0;
   Index step = 0;
   std::vector<MathFn> functions;
+  std::vector<TractionLoad> tractions;
   std::vector<Index> unlocatable;
   std::string error;
 
@@ -1030,6 +1043,7 @@ struct Solver {
       Particle& p = particles[i];
       const Cell& cl = cells[p.cell];
       p.volume = cl.volume / cl.nparticles;
+      p.size = std::pow(p.volume, static_cast<double>(1. / dim));
       p.density = materials[p.mat].density;
       p.mass = p.volume * p.density;
     }
@@ -1285,6 +1299,34 @@ struct Solver {
     for (const auto& c : n.acc_c) n.acceleration[c.first % dim] = n.acc_now[c.first % dim];
   }
 
+  // Mesh::apply_traction_on_particles (mesh.tcc:1371-1386) at current_time:
+  // each load in order assigns its value to its particles (last wins), then
+  // every particle with a traction maps N_i * traction to its nodes in
+  // particle order (particle.tcc:983-990)
+  void apply_tractions(double current_time) {
+    if (tractions.empty()) return;
+    const Index np = (Index)particles.size();
+    for (const auto& t : tractions) {
+      const double scalar = t.fn >= 0 ? functions[t.fn].value(current_time) : 1.0;
+      const double tv = t.value * scalar;  // Traction::traction
+      const Index b = t.particle < 0 ? 0 : t.particle;
+      const Index e = t.particle < 0 ? np : std::min<Index>(t.particle + 1, np);
+      for (Index i = b; i < e; ++i) {
+        Particle& p = particles[i];
+        p.traction[t.dir] = tv * p.volume / p.size;
+        p.has_traction = true;
+      }
+    }
+    for (auto& p : particles) {
+      if (!p.has_traction || p.cell == kNoCell) continue;
+      const Cell& cl = cells[p.cell];
+      for (int i = 0; i < nn; ++i) {
+        Node& n = nodes[cl.nodes[i]];
+        for (int k = 0; k < dim; ++k) n.fext[k] = n.fext[k] + p.N[i] * p.traction[k];
+      }
+    }
+  }
+
   // Node::apply_concentrated_force (node.tcc:103-111) over all nodes at
   // current_time = step * dt (mpm_scheme.tcc:156-160): external force =
   // external force * 1 + f(t) * concentrated force
@@ -1411,6 +1453,7 @@ struct Solver {
     // compute_forces (mpm_scheme.tcc:131-190)
     ORC_PRAGMA(omp parallel for schedule(static))
     for (Index i = 0; i < np; ++i) map_body_force(particles[i], g);
+    apply_tractions(static_cast<double>(step) * dt);
     apply_concentrated_forces(static_cast<double>(step) * dt);
     ORC_PRAGMA(omp parallel for schedule(static))
     for (Index i = 0; i < np; ++i) map_internal_force(particles[i]);
@@ -1549,6 +1592,22 @@ int orc_set_nodal_forces(orc_ctx* c, Index n, const Index* node, const int* dir,
   })
 }
 
+// Particle tractions in order (replacing any earlier list): particle -1 =
+// every particle; fn: math function index, -1 or a null array: none.
+int orc_set_particle_tractions(orc_ctx* c, Index n, const Index* particle, const int* dir,
+                               const double* value, const int* fn) {
+  ORC_TRY({
+    std::vector<orc::TractionLoad> t;
+    for (Index i = 0; i < n; ++i) {
+      if ((unsigned)dir[i] >= (unsigned)c->s.dim) throw std::runtime_error("bad dir");
+      if (fn && (fn[i] < -1 || fn[i] >= (int)c->s.functions.size()))
+        throw std::runtime_error("bad math function index");
+      t.push_back({particle[i], dir[i], value[i], fn ? fn[i] : -1});
+    }
+    c->s.tractions = t;
+  })
+}
+
 int orc_set_acceleration_constraints(orc_ctx* c, Index n, const Index* node, const int* dir,
                                      const double* val) {
   return orc_set_acceleration_constraints_fn(c, n, node, dir, val, nullptr);
@@ -1655,6 +1714,7 @@ int orc_set_particles(orc_ctx* c, Index n, const double* x, const double* u, con
           for (int b = 0; b < 3; ++b) p.F[a][b] = F[i * 9 + a * 3 + b];
       p.mass = mass[i];
       p.volume = volume[i];
+      p.size = std::pow(p.volume, static_cast<double>(1. / c->s.dim));
       p.density = density[i];
       p.mat = mat[i];
       if (p.mat < 0 || p.mat >= (int)s.materials.size()) throw std::runtime_error("bad material id");
@@ -1813,6 +1873,7 @@ int orc_phase(orc_ctx* c, int phase, double dt, const double* arg) {
       case PH_UPDATE_F: for (auto& p : s.particles) s.update_F(p); break;
       case PH_BODY_FORCE:
         for (auto& p : s.particles) s.map_body_force(p, arg);
+        s.apply_tractions(static_cast<double>(s.step) * dt);
         s.apply_concentrated_forces(static_cast<double>(s.step) * dt);
         break;
       case PH_INTERNAL_FORCE: for (auto& p : s.particles) s.map_internal_force(p); break;

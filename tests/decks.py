@@ -13,7 +13,8 @@ DRIVER = os.path.join(ROOT, "mpm_amd", "mpmx_run")
 
 def write_deck(w, d, nsteps, output_steps, constraints_as="sets", uuid="deck",
                material_json=None, stresses=None, particle_constraints=None,
-               acceleration_constraints=None, math_functions=None, nodal_forces=None):
+               acceleration_constraints=None, math_functions=None, nodal_forces=None,
+               tractions=None):
     """w: a Workload; d: directory. Velocity constraints go to node sets
     (``constraints_as="sets"``) or to a constraints file (``"file"``).
     particle_constraints: [(particle ids or -1, dir, velocity)], written as
@@ -23,7 +24,9 @@ def write_deck(w, d, nsteps, output_steps, constraints_as="sets", uuid="deck",
     "math_functions" entries (a Linear one with "file" gets its table
     written to that file from "xvalues"/"fxvalues").
     nodal_forces: [(node ids or -1, dir, force, math function id or None)],
-    written as node sets and concentrated_nodal_forces."""
+    written as node sets and concentrated_nodal_forces.
+    tractions: [(particle ids or -1, dir, traction, math function id or
+    None)], written as particle sets and particle_surface_traction."""
     os.makedirs(d, exist_ok=True)
     dim = w.dim
     with open(os.path.join(d, "mesh.txt"), "w") as f:
@@ -87,8 +90,20 @@ def write_deck(w, d, nsteps, output_steps, constraints_as="sets", uuid="deck",
                 for x, fx in zip(f.pop("xvalues"), f.pop("fxvalues")):
                     fh.write("%.17g,%.17g\n" % (x, fx))
         fns.append(f)
+    tr = []
+    for ids, k, val, fid in tractions or []:
+        sets.setdefault("particle_sets", [])
+        if isinstance(ids, int) and ids == -1:
+            sid = -1
+        else:
+            sid = 700 + len(sets["particle_sets"])
+            sets["particle_sets"].append({"id": sid, "set": [int(i) for i in ids]})
+        c = {"pset_id": sid, "dir": int(k), "traction": float(val)}
+        if fid is not None:
+            c["math_function_id"] = int(fid)
+        tr.append(c)
     if particle_constraints:
-        sets["particle_sets"] = []
+        sets.setdefault("particle_sets", [])
         pc = []
         for ids, k, val in particle_constraints:
             if isinstance(ids, int) and ids == -1:
@@ -129,6 +144,8 @@ def write_deck(w, d, nsteps, output_steps, constraints_as="sets", uuid="deck",
         deck["math_functions"] = fns
     if nf:
         deck["external_loading_conditions"]["concentrated_nodal_forces"] = nf
+    if tr:
+        deck["external_loading_conditions"]["particle_surface_traction"] = tr
     with open(os.path.join(d, "mpm.json"), "w") as f:
         json.dump(deck, f, indent=1)
     return os.path.join(d, "mpm.json")

@@ -125,3 +125,23 @@ def test_driver_concentrated_nodal_forces_match_binding(tmp_path):
     s.set_nodal_forces(top[4:6], 0, 10.0, -1)
     s.step(20, w.dt)
     _compare(out, s.particles(), 2)
+
+
+def test_driver_particle_tractions_match_binding(tmp_path):
+    """particle_surface_traction from particle sets and -1, with and without
+    a math function."""
+    w = workloads.column_2d(nx=6, ny=10)
+    dt = w.dt
+    fns = [{"id": 6, "type": "Linear", "xvalues": [0.0, 10 * dt], "fxvalues": [0.0, 1.0]}]
+    d = str(tmp_path)
+    write_deck(w, d, nsteps=20, output_steps=20, material_json=LE, math_functions=fns,
+               tractions=[([10, 11, 12, 13], 1, -200.0, 6), (-1, 0, 5.0, None)])
+    run_driver(d, "--ordered")
+    out = read_output(os.path.join(d, "results", "deck", "particles20.txt"), 2)
+    from mpm_amd.capi import Solver
+    s = w.setup(Solver(2, w.scheme, ordered=True))
+    s.set_math_functions([("linear", [0.0, 10 * dt], [0.0, 1.0])])
+    s.set_particle_tractions([10, 11, 12, 13, -1], [1, 1, 1, 1, 0],
+                             [-200.0, -200.0, -200.0, -200.0, 5.0], [0, 0, 0, 0, -1])
+    s.step(20, w.dt)
+    _compare(out, s.particles(), 2)

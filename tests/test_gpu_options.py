@@ -1,7 +1,7 @@
 """Step options beyond the benchmark decks, GPU (ordered mode) against the
 oracle bit for bit: Cundall damping, PIC and blended FLIP velocity updates,
 acceleration constraints (constant and scaled by math functions of time),
-concentrated nodal forces,
+concentrated nodal forces, particle surface tractions,
 prescribed non-zero velocities, particle velocity constraints, gravity
 ramping, one particle per cell, 2D Mohr-Coulomb under every scheme (with
 softening, and with separated particles); and fast mode on the same decks
@@ -148,6 +148,36 @@ def test_concentrated_nodal_forces(scheme, ordered):
         obj.set_nodal_forces(top, 1, -50.0, 0)
         obj.set_nodal_forces([top[1]], [0], [20.0], [-1])
         obj.set_nodal_forces([top[2]], [0], [5.0], [1])
+        return obj
+
+    g = setup(_solver(w, ordered, scheme=scheme))
+    o = setup(_oracle(w, scheme=scheme))
+    for _ in range(4):
+        g.step(10, w.dt)
+        o.step(10, w.dt)
+    _check(g.particles(), o.particles(), exact=ordered)
+
+
+@pytest.mark.parametrize("scheme,ordered", [("usl", True), ("musl", True), ("usf", True),
+                                            ("usl", False)])
+def test_particle_tractions(scheme, ordered):
+    """Particle surface tractions: a Linear-ramped downward traction on the
+    particles of the top cell row, a constant sideways one on every
+    particle that a later load replaces for three particles, and a Sin load
+    on one particle in the other direction."""
+    w = workloads.column_2d(nx=6, ny=10)
+    fns = [("linear", [0.0, 20 * w.dt], [0.0, 1.0]), ("sin", 0.0, 2 * np.pi / (40 * w.dt), (0.0, 1.0))]
+
+    def setup(obj):
+        w.setup(obj)
+        x = obj.particles(["x"])["x"]
+        top = np.nonzero(x[:, 1] > x[:, 1].max() - 0.05)[0]
+        obj.set_math_functions(fns)
+        ids = np.concatenate([top, [-1], [3, 4, 5], [7]])
+        dirs = np.concatenate([np.ones(len(top), dtype=np.int32), [0], [0, 0, 0], [1]])
+        vals = np.concatenate([np.full(len(top), -400.0), [25.0], [-60.0, -60.0, -60.0], [90.0]])
+        fn = np.concatenate([np.zeros(len(top), dtype=np.int32), [-1], [-1, -1, -1], [1]])
+        obj.set_particle_tractions(ids, dirs, vals, fn)
         return obj
 
     g = setup(_solver(w, ordered, scheme=scheme))

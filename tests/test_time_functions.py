@@ -3,8 +3,9 @@ LinearFunction (src/functions/linear_function.cc:19-41) and SinFunction
 (src/functions/sin_function.cc:17-20) against their closed forms, and the
 acceleration constraints they scale (acceleration_constraint.h:37-43,
 Mesh::update_nodal_acceleration_constraints at time step * dt,
-mpm_scheme.tcc:219) and concentrated nodal forces (node.tcc:103-111,
-mpm_scheme.tcc:156-160) on small oracle runs. CPU only."""
+mpm_scheme.tcc:219), concentrated nodal forces (node.tcc:103-111,
+mpm_scheme.tcc:156-160) and particle tractions (particle.tcc:962-990) on
+small oracle runs. CPU only."""
 import numpy as np
 import pytest
 
@@ -98,3 +99,30 @@ def test_oracle_concentrated_force_adds_to_external_force():
             assert np.all(d[mask] == 0.0) and np.all(d[row, 0] == 0.0)
         else:
             assert np.all(np.abs(d[row, 1]) > 0)
+
+
+def test_oracle_traction_total_force():
+    """The nodal shape functions of a particle sum to one, so one step with
+    and without tractions differs in total external force by the sum of the
+    particles' traction vectors: value * f(t) * V / V0^(1/dim) (V the
+    particle's volume, V0 the volume it was created with)."""
+    w = workloads.column_2d(nx=4, ny=6)
+    dt = w.dt
+    fns = [("linear", [0.0, 4 * dt], [0.0, 2.0])]
+    out = []
+    for load in (False, True):
+        o = w.setup(oracle.Oracle(w.dim, w.scheme))
+        o.set_math_functions(fns)
+        v0 = o.particles(["volume"])["volume"].copy()
+        if load:
+            o.set_particle_tractions([2, 5, 2], [1, 0, 1], [-10.0, 4.0, -20.0], [0, -1, 0])
+        o.step(2, dt)
+        out.append((o.nodes()["fext"], o.particles(["volume"])["volume"]))
+    d = out[1][0].sum(axis=0) - out[0][0].sum(axis=0)
+    vol = out[1][1]  # volumes of the second step's force computation (USL: step start)
+    t = 1 * dt       # time of the second step
+    f = 2.0 * t / (4 * dt)
+    size = np.sqrt(v0)
+    # particle 2: the later load (-20) wins in y; particle 5: constant 4 in x
+    want = np.array([4.0 * vol[5] / size[5], -20.0 * f * vol[2] / size[2]])
+    np.testing.assert_allclose(d, want, rtol=1e-12)
