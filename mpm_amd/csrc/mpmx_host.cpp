@@ -1143,8 +1143,13 @@ AdjParticles ad_parts(mpmx_ctx* c) {
 int64_t ad_part_doubles(const mpmx_ctx* c) { return (3 * c->dim + 10) * c->np; }
 
 // What the adjoint differentiates: Cartesian cells with the reference's local
-// node order, no friction constraints, no strain softening.
+// node order, no friction constraints, no strain softening, no particle
+// tractions (they depend on the particle's volume and position). Nodal
+// forces and acceleration constraints, with or without math functions, are
+// constants of the step and are supported.
 void ad_check_supported(mpmx_ctx* c) {
+  if (!c->trac.empty())
+    throw ApiError(MPMX_E_UNSUPPORTED, "adjoint: particle tractions are not supported");
   for (const auto& m : c->mats)
     if (m.type == MC && m.softening)
       throw ApiError(MPMX_E_UNSUPPORTED, "adjoint: Mohr-Coulomb strain softening is not supported");

@@ -15,7 +15,8 @@ class Problem:
     def __init__(self, dim=2, scheme="usl", material="le", ncells=(6, 10), block=(3, 6),
                  h=0.05, nsteps=40, dt=1e-4, seed=3, damping=0.0, velocity_update="flip",
                  blending=1.0, ramp=0.0, acc_constraint=False, vel_amplitude=0.05,
-                 geostatic=False, tension_cutoff=0.0, particle_constraints=False):
+                 geostatic=False, tension_cutoff=0.0, particle_constraints=False,
+                 nodal_forces=False):
         rng = np.random.default_rng(seed)
         self.dim, self.scheme, self.nsteps, self.dt = dim, scheme, nsteps, dt
         self.damping, self.vu, self.blending, self.ramp = damping, velocity_update, blending, ramp
@@ -34,6 +35,16 @@ class Problem:
         vd = [np.full(len(base), k) for k in range(dim)] + [np.zeros(len(left))]
         self.vel = (np.concatenate(vn), np.concatenate(vd).astype(np.int32),
                     np.zeros(sum(len(a) for a in vn)))
+        # concentrated forces on the nodes of a row inside the block, ramped by
+        # a Linear math function, and a Sin-scaled acceleration constraint
+        self.forces = None
+        if nodal_forces:
+            row = np.nonzero((np.abs(self.coords[:, dim - 1] - 2 * h) < 1e-12)
+                             & (self.coords[:, 0] > 1e-12) & (self.coords[:, 0] < h * block[0]))[0]
+            self.forces = (row, np.full(len(row), dim - 1, dtype=np.int32),
+                           np.full(len(row), -30.0), np.zeros(len(row), dtype=np.int32))
+            self.functions = [("linear", [0.0, 10 * dt], [0.0, 1.0]),
+                              ("sin", 0.0, np.pi / (20 * dt), (0.0, 1.0))]
         self.acc = None
         if acc_constraint:
             top = nc[dim - 1] * h
@@ -103,8 +114,14 @@ class Problem:
         obj.set_mesh(self.coords, self.cells)
         obj.set_materials(self.materials(prm["E"], prm["rho"], prm["phi"]))
         obj.set_velocity_constraints(*self.vel)
+        if self.forces is not None:
+            obj.set_math_functions(self.functions)
+            obj.set_nodal_forces(*self.forces)
         if self.acc is not None:
-            obj.set_acceleration_constraints(*self.acc)
+            if self.forces is not None:
+                obj.set_acceleration_constraints(*self.acc, np.ones(len(self.acc[0]), dtype=np.int32))
+            else:
+                obj.set_acceleration_constraints(*self.acc)
         obj.set_gravity(self.gravity, self.ramp)
         obj.set_particles(prm["x0"], mass=self.volume * prm["rho"], volume=self.volume,
                           density=prm["rho"], material_id=np.arange(self.np), v=prm["v0"],

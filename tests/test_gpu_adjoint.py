@@ -31,6 +31,8 @@ CASES = {
     "usl_le_options": dict(scheme="usl", material="le", blending=0.8, damping=0.05,
                            ramp=0.002, acc_constraint=True),
     "usl_mc": dict(scheme="usl", material="mc", geostatic=True),
+    "musl_le_timed_loads": dict(scheme="musl", material="le", nodal_forces=True,
+                                acc_constraint=True),
     "musl_le_particle_constraints": dict(scheme="musl", material="le", particle_constraints=True),
     "usf_mc_particle_constraints": dict(scheme="usf", material="mc", geostatic=True,
                                         particle_constraints=True),
@@ -120,6 +122,13 @@ def test_adjoint_unsupported_options():
     base = np.nonzero(np.abs(P.coords[:, 1]) < 1e-12)[0]
     s.set_friction_constraints(base, np.ones(len(base), dtype=np.int32),
                                -np.ones(len(base), dtype=np.int32), np.full(len(base), 0.3))
+    with pytest.raises(MpmxError) as e:
+        s.adjoint_forward(5, P.dt)
+    assert e.value.code == E_UNSUPPORTED
+    s.close()
+    s = Solver(2, "usl")
+    P.setup(s, P.params())
+    s.set_particle_tractions([0], [1], [-10.0])
     with pytest.raises(MpmxError) as e:
         s.adjoint_forward(5, P.dt)
     assert e.value.code == E_UNSUPPORTED
