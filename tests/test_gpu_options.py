@@ -188,6 +188,37 @@ def test_particle_tractions(scheme, ordered):
     _check(g.particles(), o.particles(), exact=ordered)
 
 
+@pytest.mark.parametrize("ordered", [True, False])
+def test_loads_3d(ordered):
+    """Hex8: particle tractions (Linear-ramped, on the top particles; one
+    constant on every particle) and concentrated nodal forces (Sin-scaled)
+    under MUSL."""
+    w = workloads.collapse_3d(nx=8, ny=4, nz=4, block=(4, 4, 4))
+    w.materials = mat.linear_elastic(5e6, 0.3, 1800.0)[None, :]
+    fns = [("linear", [0.0, 20 * w.dt], [0.0, 1.0]), ("sin", 0.0, 2 * np.pi / (40 * w.dt), (0.0, 1.0))]
+    top_nodes = np.nonzero((np.abs(w.coords[:, 2] - 0.04) < 1e-9) & (w.coords[:, 0] < 0.04 + 1e-9))[0]
+    assert len(top_nodes) > 0
+
+    def setup(obj):
+        w.setup(obj)
+        x = obj.particles(["x"])["x"]
+        top = np.nonzero(x[:, 2] > x[:, 2].max() - 0.005)[0]
+        obj.set_math_functions(fns)
+        obj.set_particle_tractions(np.concatenate([top, [-1]]),
+                                   np.concatenate([np.full(len(top), 2), [0]]).astype(np.int32),
+                                   np.concatenate([np.full(len(top), -300.0), [15.0]]),
+                                   np.concatenate([np.zeros(len(top)), [-1]]).astype(np.int32))
+        obj.set_nodal_forces(top_nodes, 1, 4.0, 1)
+        return obj
+
+    g = setup(_solver(w, ordered))
+    o = setup(_oracle(w))
+    for _ in range(3):
+        g.step(10, w.dt)
+        o.step(10, w.dt)
+    _check(g.particles(), o.particles(), exact=ordered)
+
+
 def test_gravity_ramp():
     w = workloads.column_2d(nx=6, ny=10)
     pg, po = _run(w, 40, ramp=(w.gravity, 20 * w.dt))
