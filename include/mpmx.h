@@ -93,7 +93,13 @@ int mpmx_set_mesh(mpmx_ctx* ctx, int64_t nnodes, const double* coords,
  * analysis. Call after mpmx_set_mesh and before adding particles. */
 int mpmx_set_instances(mpmx_ctx* ctx, int64_t n, const int64_t* cell_begin);
 
-/* Materials (params: n * MPMX_MATERIAL_PARAMS). Material id = index. */
+/* Materials (params: n * MPMX_MATERIAL_PARAMS). Material id = index.
+ * For LinearElastic, params[4] is the objective stress rate
+ * (linear_elastic.tcc:8-17, 96-124): MPMX_RATE_NONE or MPMX_RATE_JAUMANN;
+ * MPMX_RATE_GREEN_NAGHDI returns MPMX_E_UNSUPPORTED. */
+#define MPMX_RATE_NONE 0
+#define MPMX_RATE_JAUMANN 1
+#define MPMX_RATE_GREEN_NAGHDI 2
 int mpmx_set_materials(mpmx_ctx* ctx, int n, const double* params);
 
 /* Nodal constraints, Cartesian form. Velocity/acceleration: per (node, dir)

@@ -275,6 +275,91 @@ MPMX_DEV void de_mul(const Material& m, const double* x, double* y) {
   y[4] = m.G * x[4];
   y[5] = m.G * x[5];
 }
+// exp(A) of the skew matrix A with A01 = a[0], A02 = a[1], A12 = a[2], by
+// Rodrigues' formula I + sin(t)/t A + (1 - cos t)/t^2 A^2 (t = |a|); below
+// t = 1e-2 the two coefficients by their Taylor series to t^6 (truncation
+// below 1e-21). Stands for Eigen's MatrixExponential in
+// linear_elastic.tcc:106-107; both are exact to rounding for the rotations
+// of a time step.
+MPMX_DEV void skew_exp(const double* a, double (*Q)[3]) {
+  const double t2 = (a[0] * a[0] + a[1] * a[1]) + a[2] * a[2];
+  double c1, c2;
+  if (t2 < 1e-4) {
+    c1 = 1. - t2 * (1. / 6. - t2 * (1. / 120. - t2 * (1. / 5040.)));
+    c2 = 0.5 - t2 * (1. / 24. - t2 * (1. / 720. - t2 * (1. / 40320.)));
+  } else {
+    const double t = sqrt(t2);
+    double sn, cs;
+    dmath::sincos(t, &sn, &cs);
+    c1 = sn / t;
+    c2 = (1. - cs) / t2;
+  }
+  const double A[3][3] = {{0., a[0], a[1]}, {-a[0], 0., a[2]}, {-a[1], -a[2], 0.}};
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      double a2 = A[i][0] * A[0][j];
+#pragma unroll
+      for (int k = 1; k < 3; ++k) a2 = a2 + A[i][k] * A[k][j];
+      Q[i][j] = ((i == j ? 1. : 0.) + c1 * A[i][j]) + c2 * a2;
+    }
+}
+
+// Q X Q^T of a symmetric X, in Voigt order (00, 11, 22, 01, 12, 02);
+// sequential k-order in both products.
+MPMX_DEV void rotate_sym(const double (*Q)[3], const double (*X)[3], double* y) {
+  double T[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      double t = Q[i][0] * X[0][j];
+#pragma unroll
+      for (int k = 1; k < 3; ++k) t = t + Q[i][k] * X[k][j];
+      T[i][j] = t;
+    }
+  constexpr int I[6] = {0, 1, 2, 0, 1, 0}, J[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+  for (int v = 0; v < 6; ++v) {
+    double t = T[I[v]][0] * Q[J[v]][0];
+#pragma unroll
+    for (int k = 1; k < 3; ++k) t = t + T[I[v]][k] * Q[J[v]][k];
+    y[v] = t;
+  }
+}
+
+// LinearElastic stress with the Jaumann rate (linear_elastic.tcc:96-124):
+// a = the entries 01, 02, 12 of W/2, W the skew part of dF - I. The stress
+// s is rotated by exp(W) = exp(W/2)^2, the strain increment ds (tensor
+// shears) by exp(W/2), and the elastic increment of the rotated strain is
+// added. s: in/out.
+static __device__ __attribute__((noinline)) void jaumann_le(const Material& m, const double* a, const double* ds, double* s) {
+  double L[3][3], LL[3][3];
+  skew_exp(a, L);
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      double t = L[i][0] * L[0][j];
+#pragma unroll
+      for (int k = 1; k < 3; ++k) t = t + L[i][k] * L[k][j];
+      LL[i][j] = t;
+    }
+  const double S[3][3] = {{s[0], s[3], s[5]}, {s[3], s[1], s[4]}, {s[5], s[4], s[2]}};
+  const double E[3][3] = {{ds[0], 0.5 * ds[3], 0.5 * ds[5]},
+                          {0.5 * ds[3], ds[1], 0.5 * ds[4]},
+                          {0.5 * ds[5], 0.5 * ds[4], ds[2]}};
+  double rs[6], re[6], d[6];
+  rotate_sym(LL, S, rs);
+  rotate_sym(L, E, re);
+#pragma unroll
+  for (int v = 3; v < 6; ++v) re[v] = 2.0 * re[v];
+  de_mul(m, re, d);
+#pragma unroll
+  for (int v = 0; v < 6; ++v) s[v] = rs[v] + d[v];
+}
+
 // x^T D (D symmetric in structure; column sums in k order)
 MPMX_DEV void de_tmul(const Material& m, const double* x, double* y) {
   y[0] = (x[0] * m.a1 + x[1] * m.a2) + x[2] * m.a2;

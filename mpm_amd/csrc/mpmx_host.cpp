@@ -118,6 +118,7 @@ struct mpmx_ctx {
   std::vector<Material> mats;
   DBuf<Material> d_mats;
   bool any_mc = false;
+  bool any_rate = false;  // a LinearElastic material with the Jaumann rate
 
   // particle velocity constraints in the order given: (particle or -1 for
   // all, direction, value); applied per particle as a mask and values
@@ -149,6 +150,7 @@ struct mpmx_ctx {
 
   // step machinery
   DBuf<int> d_skey, d_perm, d_sort_tmp, d_scan_list, d_cstart, d_cend;
+  DBuf<double> d_spin;          // Particles::spin (any_rate)
   DBuf<int> d_plastic_list;     // deferred Mohr-Coulomb return mappings
   DBuf<double> d_plastic_rec;
   DBuf<Status> d_status;
@@ -238,6 +240,7 @@ struct mpmx_ctx {
     P.cell = d_cell.p;
     P.pvc_mask = pvc.empty() ? nullptr : d_pvc_mask.p;
     P.pvc_val = pvc.empty() ? nullptr : d_pvc_val.p;
+    P.spin = any_rate ? d_spin.p : nullptr;
     P.plastic_list = any_mc ? d_plastic_list.p : nullptr;
     P.plastic_rec = any_mc ? d_plastic_rec.p : nullptr;
     P.ntr = ntr;
@@ -473,6 +476,12 @@ Material make_material(const double* p) {
   if (!(m.density > 0) || !(E > 0) || !(nu > -1.0 && nu < 0.5))
     throw ApiError(MPMX_E_ARG, "material density, youngs_modulus or poisson_ratio out of range");
   if (m.type == LE) {
+    // objective stress rate (linear_elastic.tcc:8-17): none or Jaumann
+    m.rate = (int)p[4];
+    if (m.rate == RATE_GREEN_NAGHDI)
+      throw ApiError(MPMX_E_UNSUPPORTED, "the Green-Naghdi stress rate is not supported");
+    if (m.rate != RATE_NONE && m.rate != RATE_JAUMANN)
+      throw ApiError(MPMX_E_ARG, "unknown stress rate");
     const double K = E / (3.0 * (1. - 2. * nu));  // linear_elastic.tcc:27
     const double G = E / (2.0 * (1. + nu));       // linear_elastic.tcc:55
     m.a1 = K + (4.0 / 3.0) * G;
@@ -687,6 +696,7 @@ void alloc_step_buffers(mpmx_ctx* c) {
   c->d_perm.alloc(std::max<int64_t>(np, 1));
   c->d_sort_tmp.alloc(radix_sort_temp_ints(std::max<int64_t>(np, 1)));
   c->d_scan_list.alloc(std::max<int64_t>(np, 1));
+  if (c->any_rate) c->d_spin.alloc(3 * std::max<int64_t>(np, 1));
   if (c->any_mc) {
     c->d_plastic_list.alloc(std::max<int64_t>(np, 1));
     c->d_plastic_rec.alloc(PR_N * std::max<int64_t>(np, 1));
@@ -1150,9 +1160,12 @@ int64_t ad_part_doubles(const mpmx_ctx* c) { return (3 * c->dim + 10) * c->np; }
 void ad_check_supported(mpmx_ctx* c) {
   if (!c->trac.empty())
     throw ApiError(MPMX_E_UNSUPPORTED, "adjoint: particle tractions are not supported");
-  for (const auto& m : c->mats)
+  for (const auto& m : c->mats) {
     if (m.type == MC && m.softening)
       throw ApiError(MPMX_E_UNSUPPORTED, "adjoint: Mohr-Coulomb strain softening is not supported");
+    if (m.type == LE && m.rate != RATE_NONE)
+      throw ApiError(MPMX_E_UNSUPPORTED, "adjoint: objective stress rates are not supported");
+  }
   for (const auto& kv : c->cons)
     if (kv.second.fdir >= 0)
       throw ApiError(MPMX_E_UNSUPPORTED, "adjoint: friction constraints are not supported");
@@ -1425,9 +1438,11 @@ int mpmx_set_materials(mpmx_ctx* c, int n, const double* params) {
   if (c->np > 0) throw ApiError(MPMX_E_STATE, "materials must be set before particles");
   c->mats.clear();
   c->any_mc = false;
+  c->any_rate = false;
   for (int i = 0; i < n; ++i) {
     c->mats.push_back(make_material(params + (size_t)i * MPMX_MATERIAL_PARAMS));
     if (c->mats.back().type == MC) c->any_mc = true;
+    if (c->mats.back().type == LE && c->mats.back().rate == RATE_JAUMANN) c->any_rate = true;
   }
   c->d_mats.alloc(n);
   c->d_mats.upload(c->mats.data(), n, c->stream);

@@ -19,6 +19,7 @@ enum { FLIP = 0, PIC = 1 };
 struct Material {
   int type;
   int softening;
+  int rate;  // LinearElastic objective stress rate: RATE_NONE or RATE_JAUMANN
   double density;
   double a1, a2, G;  // elastic tensor entries: D = [a1 a2 a2; a2 a1 a2; a2 a2 a1] (+) G*I3
   double shear;      // MohrCoulomb shear_modulus_ (pdstrain of separated state)
@@ -32,6 +33,8 @@ struct Material {
   // angle, by the same functions as the per-particle evaluation
   double sphi, cphi, tphi, tpsi;
 };
+
+enum { RATE_NONE = 0, RATE_JAUMANN = 1, RATE_GREEN_NAGHDI = 2 };
 
 // Nodal constraint record, referenced by Nodes::cons[node] (-1: none).
 struct NodeCons {
@@ -101,6 +104,11 @@ struct Particles {
   // yields this step (plastic_list[0, Status::nplastic) in shear,
   // plastic_list[np - Status::nplastic_t, np) in tension) and their records
   // (PR_* below), by list slot. Null when no MohrCoulomb material.
+  // [3][np] entries 01, 02, 12 of W/2, W the skew part of the step's
+  // ΔF - I, from the kinematics to a Jaumann-rate stress update of the same
+  // thread (per-step scratch; 2D uses entry 01 only). Null without a
+  // Jaumann-rate material.
+  double* spin;
   int* plastic_list;
   double* plastic_rec;  // [PR_N][np] by list slot
   // Particle tractions (Particle::assign_traction / map_traction_force,

@@ -75,8 +75,10 @@ MPMX_DEV void gather_nodes(const Mesh& M, const Nodes& Nd, int c, double (*nv)[D
 // The kinematic half of compute_stress_strain for one particle
 // (mpm_scheme.tcc:75-103: ΔF, strain rate, strain, volume, density, F).
 // dndx: the particle's dN/dx at its xi. Writes the particle's strain, volume,
-// density and F; returns the strain increment ds and the new density.
-template <int D>
+// density and F (and with ROT P.spin: the rotation of a Jaumann-rate
+// update, linear_elastic.tcc:102-106); returns the strain increment ds and
+// the new density.
+template <int D, bool ROT = false>
 MPMX_DEV void kinematics(const Mesh& M, const Particles& P, int p, int c, const double (*dndx)[D],
                          const double (*nv)[D], double dt, double* ds, double* rho_out,
                          double* vol_out) {
@@ -180,6 +182,13 @@ MPMX_DEV void kinematics(const Mesh& M, const Particles& P, int p, int c, const 
         P.F[(a * 3 + j) * np + p] = t;
       }
   }
+  if constexpr (ROT) {  // stored rather than kept live across the stress update
+    P.spin[p] = 0.5 * (0.5 * (R[0][1] - R[1][0]));
+    if constexpr (D == 3) {
+      P.spin[np + p] = 0.5 * (0.5 * (R[0][2] - R[2][0]));
+      P.spin[2 * np + p] = 0.5 * (0.5 * (R[1][2] - R[2][1]));
+    }
+  }
 }
 
 // Mohr-Coulomb state variables written back after an update: friction,
@@ -231,13 +240,15 @@ MPMX_DEV void defer_plastic(const Particles& P, Status* S, int p, const double* 
 // DEFER: a Mohr-Coulomb particle whose trial stress yields is appended to
 // the plastic list instead (stress and state variables left as they were,
 // `s` not the new stress); k_plastic completes it with the same inputs.
-template <int D, bool DEFER = false>
+// ROT: kernels built for a Jaumann-rate material (P.spin set); without it
+// the rate is not looked at.
+template <int D, bool DEFER = false, bool ROT = false>
 MPMX_DEV void stress_strain(const Mesh& M, const Particles& P, const Material* mats, int p, int c,
                             const double (*dndx)[D], const double (*nv)[D], double dt,
                             double* s, double* vol_out, Status* S = nullptr) {
   const int64_t np = P.np;
   double ds[6], rho;
-  kinematics<D>(M, P, p, c, dndx, nv, dt, ds, &rho, vol_out);
+  kinematics<D, ROT>(M, P, p, c, dndx, nv, dt, ds, &rho, vol_out);
   // one grouped round trip after the kinematic stores
 #pragma unroll
   for (int k = 0; k < 6; ++k) s[k] = P.stress[k * np + p];
@@ -247,7 +258,16 @@ MPMX_DEV void stress_strain(const Mesh& M, const Particles& P, const Material* m
 #pragma unroll
     for (int k = 0; k < NSV; ++k) sv[k] = P.sv[k * np + p];
   }
-  if (m.type == LE) {
+  bool jaumann = false;
+  if constexpr (ROT) jaumann = m.type == LE && m.rate == RATE_JAUMANN;
+  if (jaumann) {
+    double a[3] = {P.spin[p], 0., 0.};
+    if constexpr (D == 3) {
+      a[1] = P.spin[np + p];
+      a[2] = P.spin[2 * np + p];
+    }
+    jaumann_le(m, a, ds, s);
+  } else if (m.type == LE) {
     double t[6];
     de_mul(m, ds, t);
 #pragma unroll
@@ -679,7 +699,7 @@ __global__ __launch_bounds__(CH) void k_p2g(Mesh M, Particles P, Nodes Nd, StepA
 }
 
 // USF stress update in sorted order, then the force scatter of the new stress.
-template <int D, int CH>
+template <int D, int CH, bool ROT>
 __global__ __launch_bounds__(CH) MPMX_MC_ATTR void k_stress_scatter(Mesh M, Particles P, Nodes Nd,
                                                            const Material* mats, StepArgs A,
                                                            const Status* S, const int* skey,
@@ -706,7 +726,7 @@ __global__ __launch_bounds__(CH) MPMX_MC_ATTR void k_stress_scatter(Mesh M, Part
       particle_dndx<D>(M, key, xi, dndx, Ji);
       gather_nodes<D>(M, Nd, key, nv, nullptr);
       double s[6], vol;
-      stress_strain<D>(M, P, mats, p, key, dndx, nv, A.dt, s, &vol);
+      stress_strain<D, false, ROT>(M, P, mats, p, key, dndx, nv, A.dt, s, &vol);
       st.m[tid] = P.mass[p];
       double Bm[D * D];
       force_matrix<D>(s, vol, Ji, Bm);
@@ -722,7 +742,7 @@ __global__ __launch_bounds__(CH) MPMX_MC_ATTR void k_stress_scatter(Mesh M, Part
 
 // Stress update in particle order (USL/MUSL postcompute, and USF in ordered
 // mode); optionally locate afterwards.
-template <int D, bool LOCATE>
+template <int D, bool LOCATE, bool ROT>
 __global__ __launch_bounds__(kStressBlock) MPMX_PRED_ATTR void k_stress(Mesh M, Particles P,
                                                                       Nodes Nd,
                                                                       const Material* mats,
@@ -741,7 +761,7 @@ __global__ __launch_bounds__(kStressBlock) MPMX_PRED_ATTR void k_stress(Mesh M, 
   particle_dndx<D>(M, c, xi, dndx, Ji);
   gather_nodes<D>(M, Nd, c, nv, nullptr);
   double s[6], vol;
-  stress_strain<D, true>(M, P, mats, (int)p, c, dndx, nv, A.dt, s, &vol, S);
+  stress_strain<D, true, ROT>(M, P, mats, (int)p, c, dndx, nv, A.dt, s, &vol, S);
   if constexpr (LOCATE) {
     double x[D];
 #pragma unroll
@@ -752,7 +772,7 @@ __global__ __launch_bounds__(kStressBlock) MPMX_PRED_ATTR void k_stress(Mesh M, 
 
 // G2P: compute_updated_position (particle.tcc:994-1070) in particle order,
 // optionally followed by the USL stress update and by locate.
-template <int D, bool STRESS, bool LOCATE>
+template <int D, bool STRESS, bool LOCATE, bool ROT = false>
 __global__ __launch_bounds__(kBlock) MPMX_PRED_ATTR void k_g2p(Mesh M, Particles P, Nodes Nd,
                                                 const Material* mats, StepArgs A, int vu,
                                                 Status* S, int* scan_list) {
@@ -806,7 +826,7 @@ __global__ __launch_bounds__(kBlock) MPMX_PRED_ATTR void k_g2p(Mesh M, Particles
     double dndx[NN][D], Ji[D][D];
     particle_dndx<D>(M, c, xi, dndx, Ji);
     double s[6], vol;
-    stress_strain<D, true>(M, P, mats, (int)p, c, dndx, nv, A.dt, s, &vol, S);
+    stress_strain<D, true, ROT>(M, P, mats, (int)p, c, dndx, nv, A.dt, s, &vol, S);
   }
   if constexpr (LOCATE) locate_one<D>(M, P, S, scan_list, (int)p, x, c, -1);
 }
@@ -1355,8 +1375,13 @@ template <int D>
 void launch_stress_scatter(int chunk, const Mesh& M, const Particles& P, const Nodes& Nd,
                            const Material* mats, const StepArgs& A, const Status* S,
                            const int* skey, const int* perm, hipStream_t st) {
-#define L_(CH) \
-  MPMX_LAUNCH_CH(CH, (k_stress_scatter<D, CH>), P.np, st, M, P, Nd, mats, A, S, skey, perm)
+#define L_(CH)                                                                                 \
+  if (P.spin)                                                                                  \
+    MPMX_LAUNCH_CH(CH, (k_stress_scatter<D, CH, true>), P.np, st, M, P, Nd, mats, A, S, skey, \
+                   perm);                                                                      \
+  else                                                                                         \
+    MPMX_LAUNCH_CH(CH, (k_stress_scatter<D, CH, false>), P.np, st, M, P, Nd, mats, A, S, skey, \
+                   perm)
   MPMX_CHUNK_DISPATCH(chunk, L_);
 #undef L_
 }
@@ -1365,10 +1390,19 @@ template <int D>
 void launch_stress(bool locate, const Mesh& M, const Particles& P, const Nodes& Nd,
                    const Material* mats, const StepArgs& A, Status* S, int* scan_list,
                    hipStream_t st) {
-  if (locate)
-    MPMX_LAUNCH_CH(kStressBlock, (k_stress<D, true>), P.np, st, M, P, Nd, mats, A, S, scan_list);
+  // the Jaumann-rate build only when such a material exists (P.spin set)
+  if (locate && P.spin)
+    MPMX_LAUNCH_CH(kStressBlock, (k_stress<D, true, true>), P.np, st, M, P, Nd, mats, A, S,
+                   scan_list);
+  else if (locate)
+    MPMX_LAUNCH_CH(kStressBlock, (k_stress<D, true, false>), P.np, st, M, P, Nd, mats, A, S,
+                   scan_list);
+  else if (P.spin)
+    MPMX_LAUNCH_CH(kStressBlock, (k_stress<D, false, true>), P.np, st, M, P, Nd, mats, A, S,
+                   scan_list);
   else
-    MPMX_LAUNCH_CH(kStressBlock, (k_stress<D, false>), P.np, st, M, P, Nd, mats, A, S, scan_list);
+    MPMX_LAUNCH_CH(kStressBlock, (k_stress<D, false, false>), P.np, st, M, P, Nd, mats, A, S,
+                   scan_list);
 }
 
 template <int D>
@@ -1387,7 +1421,9 @@ template <int D>
 void launch_g2p(bool stress, bool locate, const Mesh& M, const Particles& P, const Nodes& Nd,
                 const Material* mats, const StepArgs& A, int vu, Status* S, int* scan_list,
                 hipStream_t st) {
-  if (stress && locate)
+  if (stress && locate && P.spin)
+    MPMX_LAUNCH((k_g2p<D, true, true, true>), P.np, st, M, P, Nd, mats, A, vu, S, scan_list);
+  else if (stress && locate)
     MPMX_LAUNCH((k_g2p<D, true, true>), P.np, st, M, P, Nd, mats, A, vu, S, scan_list);
   else if (locate)
     MPMX_LAUNCH((k_g2p<D, false, true>), P.np, st, M, P, Nd, mats, A, vu, S, scan_list);

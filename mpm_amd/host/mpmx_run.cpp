@@ -158,16 +158,20 @@ static std::vector<double> material_params(const Json& m, int dim) {
   const std::string type = m.at("type").str();
   const std::string sfx = std::to_string(dim) + "D";
   std::vector<double> p(MPMX_MATERIAL_PARAMS, 0.);
-  if (m.contains("stress_rate") && m.at("stress_rate").is_string() &&
-      m.at("stress_rate").str() != "none")
-    throw std::runtime_error("only the default (no objective) stress rate is supported");
+  // objective stress rate (linear_elastic.tcc:8-17): "jaumann" for
+  // LinearElastic; "green_naghdi" is passed on (the library rejects it)
+  std::string rate = "none";
+  if (m.contains("stress_rate") && m.at("stress_rate").is_string()) rate = m.at("stress_rate").str();
   if (type == "LinearElastic" + sfx) {
     p[0] = 0;
     p[1] = m.at("density").number();
     p[2] = m.at("youngs_modulus").number();
     p[3] = m.at("poisson_ratio").number();
+    p[4] = rate == "jaumann" ? 1. : (rate == "green_naghdi" ? 2. : 0.);
     return p;
   }
+  if (rate == "jaumann" || rate == "green_naghdi")
+    throw std::runtime_error("objective stress rates are supported for LinearElastic only");
   if (type == "MohrCoulomb" + sfx) {
     p[0] = 1;
     p[1] = m.at("density").number();

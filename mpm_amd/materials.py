@@ -13,6 +13,10 @@ crosses the C-ABI unchanged:
     [9] residual_cohesion  [10] peak_pdstrain  [11] residual_pdstrain
     [12] tension_cutoff  [13] softening (0/1)  [14] packing_fraction_minimum
     [15] initial packing fraction (<= 0 means 1)
+
+LinearElastic uses [4] for its objective stress rate
+(linear_elastic.tcc:8-17): 0 none, 1 Jaumann, 2 Green-Naghdi (rejected by
+the library).
 """
 import numpy as np
 
@@ -28,12 +32,16 @@ MC_STATE_VARIABLES = ("yield_state", "phi", "psi", "cohesion", "tension_cutoff",
 NSTATEVARS = len(MC_STATE_VARIABLES)
 
 
-def linear_elastic(youngs_modulus, poisson_ratio, density):
+STRESS_RATES = {None: 0, "none": 0, "jaumann": 1, "green_naghdi": 2}
+
+
+def linear_elastic(youngs_modulus, poisson_ratio, density, stress_rate=None):
     p = np.zeros(NPARAMS)
     p[0] = LINEAR_ELASTIC
     p[1] = density
     p[2] = youngs_modulus
     p[3] = poisson_ratio
+    p[4] = STRESS_RATES[stress_rate]
     return p
 
 

@@ -398,6 +398,19 @@ def stress_update(params, stress, dstrain, density=1000.0, statevars=None):
     return out, sv
 
 
+def le_rate_update(params, stress, dstrain, dF):
+    """One LinearElastic update with the material's objective stress rate
+    (params[4]: 0 none, 1 Jaumann) for a deformation gradient increment dF
+    (3x3)."""
+    lib = _lib()
+    out = np.zeros(6)
+    rc = lib.orc_kat_le_rate(_p(_f64(params)), _p(_f64(stress)), _p(_f64(dstrain)),
+                             _p(_f64(np.asarray(dF, dtype=np.float64).reshape(9))), _p(out))
+    if rc != 0:
+        raise ValueError("not a LinearElastic material")
+    return out
+
+
 def invariants(stress):
     lib = _lib()
     s = _f64(stress)

@@ -95,6 +95,7 @@ static const int kNsv = 9;
 
 struct Material {
   int type = kLE;
+  int rate = 0;  // LinearElastic objective stress rate: 0 none, 1 Jaumann
   double density = 0, E = 0, nu = 0;
   // LinearElastic (linear_elastic.tcc:20-44)
   double bulk = 0;
@@ -118,7 +119,8 @@ static void material_init(Material& m, const double* p) {
   m.E = p[2];
   m.nu = p[3];
   if (m.type == kLE) {
-    // linear_elastic.tcc:26-27, 53-69
+    // linear_elastic.tcc:8-17 (stress_rate), 26-27, 53-69
+    m.rate = (int)p[4];
     m.bulk = m.E / (3.0 * (1. - 2. * m.nu));
     const double G = m.E / (2.0 * (1. + m.nu));
     const double a1 = m.bulk + (4.0 / 3.0) * G;
@@ -180,6 +182,80 @@ static inline void de_mul(const double de[6][6], const double* x, double* y) {
     for (int k = 1; k < 6; ++k) s = s + de[i][k] * x[k];
     y[i] = s;
   }
+}
+
+// exp(A) of the skew matrix A with A01 = a[0], A02 = a[1], A12 = a[2], by
+// Rodrigues' formula I + sin(t)/t A + (1 - cos t)/t^2 A^2 (t = |a|); below
+// t = 1e-2 the two coefficients by their Taylor series to t^6 (truncation
+// below 1e-21). This replaces the Pade approximant of Eigen's
+// MatrixExponential used by linear_elastic.tcc:106-107; both are exact to
+// rounding for the small rotations of a time step.
+static void skew_exp(const double* a, double Q[3][3]) {
+  const double t2 = (a[0] * a[0] + a[1] * a[1]) + a[2] * a[2];
+  double c1, c2;
+  if (t2 < 1e-4) {
+    c1 = 1. - t2 * (1. / 6. - t2 * (1. / 120. - t2 * (1. / 5040.)));
+    c2 = 0.5 - t2 * (1. / 24. - t2 * (1. / 720. - t2 * (1. / 40320.)));
+  } else {
+    const double t = std::sqrt(t2);
+    c1 = mth::sin(t) / t;
+    c2 = (1. - mth::cos(t)) / t2;
+  }
+  const double A[3][3] = {{0., a[0], a[1]}, {-a[0], 0., a[2]}, {-a[1], -a[2], 0.}};
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      double a2 = A[i][0] * A[0][j];
+      for (int k = 1; k < 3; ++k) a2 = a2 + A[i][k] * A[k][j];
+      Q[i][j] = ((i == j ? 1. : 0.) + c1 * A[i][j]) + c2 * a2;
+    }
+}
+
+// Q X Q^T of a symmetric X, entries in Voigt order (00, 11, 22, 01, 12, 02);
+// sequential k-order in both products.
+static void rotate_sym(const double Q[3][3], const double X[3][3], double* y) {
+  double T[3][3];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      double t = Q[i][0] * X[0][j];
+      for (int k = 1; k < 3; ++k) t = t + Q[i][k] * X[k][j];
+      T[i][j] = t;
+    }
+  const int I[6] = {0, 1, 2, 0, 1, 0}, J[6] = {0, 1, 2, 1, 2, 2};
+  for (int v = 0; v < 6; ++v) {
+    double t = T[I[v]][0] * Q[J[v]][0];
+    for (int k = 1; k < 3; ++k) t = t + T[I[v]][k] * Q[J[v]][k];
+    y[v] = t;
+  }
+}
+
+// LinearElastic stress with the Jaumann rate (linear_elastic.tcc:96-124):
+// W = skew(dF - I); the stress rotated by exp(W), the strain increment
+// (tensor shears) by exp(W/2), then the elastic increment of the rotated
+// strain.
+static void jaumann_stress(const double de[6][6], const double* stress, const double* dstrain,
+                           const double dF[3][3], double* out) {
+  const double a[3] = {0.5 * (0.5 * (dF[0][1] - dF[1][0])), 0.5 * (0.5 * (dF[0][2] - dF[2][0])),
+                       0.5 * (0.5 * (dF[1][2] - dF[2][1]))};
+  double L[3][3], LL[3][3];
+  skew_exp(a, L);
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      double t = L[i][0] * L[0][j];
+      for (int k = 1; k < 3; ++k) t = t + L[i][k] * L[k][j];
+      LL[i][j] = t;
+    }
+  const double S[3][3] = {{stress[0], stress[3], stress[5]},
+                          {stress[3], stress[1], stress[4]},
+                          {stress[5], stress[4], stress[2]}};
+  const double E[3][3] = {{dstrain[0], 0.5 * dstrain[3], 0.5 * dstrain[5]},
+                          {0.5 * dstrain[3], dstrain[1], 0.5 * dstrain[4]},
+                          {0.5 * dstrain[5], 0.5 * dstrain[4], dstrain[2]}};
+  double rs[6], re[6], d[6];
+  rotate_sym(LL, S, rs);
+  rotate_sym(L, E, re);
+  for (int v = 3; v < 6; ++v) re[v] = 2.0 * re[v];
+  de_mul(de, re, d);
+  for (int v = 0; v < 6; ++v) out[v] = rs[v] + d[v];
 }
 
 // x^T * D (row vector), same k-order.
@@ -1162,7 +1238,9 @@ struct Solver {
   void compute_stress(Particle& p) {
     const Material& m = materials[p.mat];
     double out[6];
-    if (m.type == kLE) {
+    if (m.type == kLE && m.rate == 1) {
+      jaumann_stress(m.de, p.stress, p.dstrain, p.dF, out);
+    } else if (m.type == kLE) {
       double ds[6];
       de_mul(m.de, p.dstrain, ds);
       for (int k = 0; k < 6; ++k) out[k] = p.stress[k] + ds[k];
@@ -2031,6 +2109,25 @@ int orc_kat_stress(const double* params, const double* stress, const double* dst
   } else {
     if (init_sv) orc::mc_initialise_state(m, sv);
     orc::mc_compute_stress(m, stress, dstrain, density, sv, out);
+  }
+  return 0;
+}
+
+// One LinearElastic update with its objective stress rate, given the
+// deformation gradient increment dF (3x3 row-major).
+int orc_kat_le_rate(const double* params, const double* stress, const double* dstrain,
+                    const double* dF, double* out) {
+  orc::Material m;
+  orc::material_init(m, params);
+  if (m.type != orc::kLE) return -1;
+  double F[3][3];
+  for (int i = 0; i < 9; ++i) F[i / 3][i % 3] = dF[i];
+  if (m.rate == 1) {
+    orc::jaumann_stress(m.de, stress, dstrain, F, out);
+  } else {
+    double ds[6];
+    orc::de_mul(m.de, dstrain, ds);
+    for (int k = 0; k < 6; ++k) out[k] = stress[k] + ds[k];
   }
   return 0;
 }

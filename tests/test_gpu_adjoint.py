@@ -134,6 +134,15 @@ def test_adjoint_unsupported_options():
     assert e.value.code == E_UNSUPPORTED
     s.close()
     s = Solver(2, "usl")
+    s.set_mesh(P.coords, P.cells)
+    s.set_materials(mat.linear_elastic(1e6, 0.3, 1800.0, stress_rate="jaumann")[None, :])
+    s.set_particles(P.x0, mass=P.volume * 1800.0, volume=P.volume,
+                    density=np.full(P.np, 1800.0), material_id=0)
+    with pytest.raises(MpmxError) as e:
+        s.adjoint_forward(5, P.dt)
+    assert e.value.code == E_UNSUPPORTED
+    s.close()
+    s = Solver(2, "usl")
     prm = P.params()
     s.set_mesh(P.coords, P.cells)
     m = mat.mohr_coulomb(1e6, 0.3, 1800.0, friction=30.0, dilation=0.0, cohesion=1000.0,

@@ -219,6 +219,36 @@ def test_loads_3d(ordered):
     _check(g.particles(), o.particles(), exact=ordered)
 
 
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("ordered", [True, False])
+def test_jaumann_stress_rate(dim, ordered):
+    """LinearElastic with the Jaumann rate (linear_elastic.tcc:96-124) on a
+    soft block under gravity with random initial velocities, so that the
+    particles spin (USF in 2D, MUSL in 3D)."""
+    if dim == 2:
+        w = workloads.column_2d(nx=6, ny=10)
+    else:
+        w = workloads.collapse_3d(nx=8, ny=4, nz=4, block=(4, 4, 4))
+    w.velocity_seed, w.velocity_amplitude = 7, 0.05
+    w.materials = mat.linear_elastic(2e5, 0.3, 1800.0, stress_rate="jaumann")[None, :]
+    scheme = "usf" if dim == 2 else "musl"
+    pg, po = _run(w, 40, ordered=ordered, scheme=scheme)
+    _check(pg, po, exact=ordered)
+    # the rotation matters: the rate-free update differs
+    w.materials = mat.linear_elastic(2e5, 0.3, 1800.0)[None, :]
+    pn, _ = _run(w, 40, ordered=True, scheme=scheme)
+    assert not np.array_equal(pn["stress"], po["stress"])
+
+
+def test_stress_rate_validation():
+    from mpm_amd.capi import MpmxError, E_UNSUPPORTED, Solver
+    s = Solver(2, "usf")
+    with pytest.raises(MpmxError) as e:
+        s.set_materials(mat.linear_elastic(1e6, 0.3, 1800.0, stress_rate="green_naghdi")[None, :])
+    assert e.value.code == E_UNSUPPORTED
+    s.close()
+
+
 def test_gravity_ramp():
     w = workloads.column_2d(nx=6, ny=10)
     pg, po = _run(w, 40, ramp=(w.gravity, 20 * w.dt))

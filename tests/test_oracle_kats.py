@@ -146,6 +146,70 @@ def test_linear_elastic(kats):
         assert_kat(s, case["stress"], c["tol"])
 
 
+def _simple_shear(p, steps, shear=10.0):
+    """Stress after each step of the reference's Jaumann simple-shear test:
+    dstrain(3) = F_inc(0,1) = shear / steps."""
+    inc = shear / steps
+    ds = np.zeros(6)
+    ds[3] = inc
+    dF = np.eye(3)
+    dF[0, 1] = inc
+    s, out = np.zeros(6), []
+    for _ in range(steps):
+        s = oracle.le_rate_update(p, s, ds, dF)
+        out.append(s.copy())
+    return np.array(out)
+
+
+def test_linear_elastic_jaumann_simple_shear(kats):
+    c = kats["linear_elastic_jaumann"]
+    m = c["material"]
+    p = materials.linear_elastic(m["youngs_modulus"], m["poisson_ratio"], m["density"],
+                                 stress_rate="jaumann")
+    for case in c["cases"]:
+        s = _simple_shear(p, case["steps"], c["shear_displacement"])
+        got = s[::case["every"], 3] / c["G"]
+        assert_kat(got, case["normalized_sigma_xy"], c["tol"])
+
+
+def test_jaumann_normal_stresses_closed_form():
+    """Simple shear under the Jaumann rate: sigma_xy = G sin(gamma),
+    sigma_xx = -sigma_yy = G (1 - cos(gamma)) (the continuous solution; the
+    incremental one approaches it as the increment shrinks). A spin of the
+    wrong sign gives sigma_xx < 0; the rate-free update gives no normal
+    stress at all."""
+    G = 1e6 / (2 * 1.3)
+    p = materials.linear_elastic(1e6, 0.3, 1000.0, stress_rate="jaumann")
+    s = _simple_shear(p, 4000, shear=2.0)
+    gamma = 2.0
+    assert abs(s[-1, 3] / G - np.sin(gamma)) < 2e-3
+    assert abs(s[-1, 0] / G - (1 - np.cos(gamma))) < 2e-3
+    assert abs(s[-1, 1] / G + (1 - np.cos(gamma))) < 2e-3
+    plain = _simple_shear(materials.linear_elastic(1e6, 0.3, 1000.0), 40, shear=2.0)
+    assert np.all(plain[:, 0] == 0.0) and abs(plain[-1, 3] / G - 2.0) < 1e-12
+
+
+def test_jaumann_rigid_rotation():
+    """A rigid spin with no strain rotates the stress: dF = I + W with W the
+    counter-clockwise spin theta about z, 200 steps to 45 degrees in total,
+    turn sigma = diag(1, 0, 0) into R(45) sigma R(45)^T, sigma_xy = +1/2
+    (a transposed rotation gives -1/2); in 3D about x, sigma_yz likewise.
+    The invariants are kept to rounding."""
+    p = materials.linear_elastic(1e6, 0.3, 1000.0, stress_rate="jaumann")
+    n = 200
+    th = (np.pi / 4) / n
+    for axis, (i, j, v) in (("z", (0, 1, 3)), ("x", (1, 2, 4))):
+        dF = np.eye(3)
+        dF[i, j], dF[j, i] = -th, th
+        s = np.zeros(6)
+        s[i] = 1.0
+        for _ in range(n):
+            s = oracle.le_rate_update(p, s, np.zeros(6), dF)
+        assert abs(s[i] - 0.5) < 1e-12 and abs(s[j] - 0.5) < 1e-12, axis
+        assert abs(s[v] - 0.5) < 1e-12, axis
+        assert abs(s[0] + s[1] + s[2] - 1.0) < 1e-13
+
+
 def _mc(m):
     return materials.mohr_coulomb(
         m["youngs_modulus"], m["poisson_ratio"], m["density"], m["friction"], m["dilation"],
