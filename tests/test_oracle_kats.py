@@ -113,28 +113,33 @@ def test_particle_chain_3d(kats):
     _replay_chain(kats["particle_chain_3d"], 3)
 
 
-def test_node_acceleration_velocity(kats):
-    c = kats["node"]["acceleration_velocity_2d"]
+@pytest.mark.parametrize("dim", [2, 3])
+def test_node_acceleration_velocity(kats, dim):
+    c = kats["node"]["acceleration_velocity_%dd" % dim]
     tol, dt = c["tol"], c["dt"]
-    st, v, a = oracle.node_update(2, c["mass"], c["fext"], c["fint"], [0, 0], [0, 0], dt)
+    zero = [0.0] * dim
+    st, v, a = oracle.node_update(dim, c["mass"], c["fext"], c["fint"], zero, zero, dt)
     assert st
     assert_kat(a, c["plain"]["acceleration"], tol)
     assert_kat(v, c["plain"]["velocity"], tol)
     vc = [np.nan if x is None else x for x in c["with_velocity_constraint"]["constraint"]]
-    st, v, a = oracle.node_update(2, c["mass"], c["fext"], c["fint"], v, a, dt, vel_constraints=vc)
+    st, v, a = oracle.node_update(dim, c["mass"], c["fext"], c["fint"], v, a, dt,
+                                  vel_constraints=vc)
     assert_kat(v, c["with_velocity_constraint"]["velocity"], tol)
     assert_kat(a, c["with_velocity_constraint"]["acceleration"], tol)
-    st, v2, a2 = oracle.node_update(2, c["mass"], c["fext"], c["fint"], v, a, dt,
+    st, v2, a2 = oracle.node_update(dim, c["mass"], c["fext"], c["fint"], v, a, dt,
                                     damping=c["cundall_damping"]["damping"], vel_constraints=vc)
     assert_kat(a2, c["cundall_damping"]["acceleration"], tol)
     # zero mass: compute_acceleration_velocity returns false and leaves the node
-    st, _, _ = oracle.node_update(2, 0.0, c["fext"], c["fint"], v, a, dt)
+    st, _, _ = oracle.node_update(dim, 0.0, c["fext"], c["fint"], v, a, dt)
     assert not st
 
 
-def test_friction_cartesian(kats):
-    c = kats["node"]["friction_cartesian_2d"]
-    a = oracle.friction(2, c["acceleration"], c["velocity"], c["dt"], c["dir"], c["sign_n"], c["mu"])
+@pytest.mark.parametrize("dim", [2, 3])
+def test_friction_cartesian(kats, dim):
+    c = kats["node"]["friction_cartesian_%dd" % dim]
+    a = oracle.friction(dim, c["acceleration"], c["velocity"], c["dt"], c["dir"], c["sign_n"],
+                        c["mu"])
     assert_kat(a, c["expected_acceleration"], c["tol"])
 
 
@@ -221,7 +226,14 @@ def _mc(m):
 SV = {n: i for i, n in enumerate(materials.MC_STATE_VARIABLES)}
 
 
-@pytest.mark.parametrize("idx", [0, 1, 2])
+def _n_mc_cases():
+    import json
+    import os
+    with open(os.path.join(os.path.dirname(__file__), "golden", "reference_kats.json")) as f:
+        return len(json.load(f)["mohr_coulomb"])
+
+
+@pytest.mark.parametrize("idx", range(_n_mc_cases()))
 def test_mohr_coulomb(kats, idx):
     c = kats["mohr_coulomb"][idx]
     tol = c["tol"]
@@ -275,10 +287,13 @@ def test_mohr_coulomb_elastic_branch_equals_linear_elastic():
     np.testing.assert_array_equal(s_mc, s_le)
 
 
-def test_cell_crossing_2d(kats):
-    c = kats["cell_crossing_2d"]
-    o = oracle.Oracle(2)
-    o.set_mesh(np.array(c["nodes"]), np.array(c["cells"]))
+@pytest.mark.parametrize("dim", [2, 3])
+def test_cell_crossing(kats, dim):
+    """particle_cell_crossing_test.cc: the particles of cell 0 whose nodal x
+    velocity is 1000 move 0.4 along x and cross into cell 1."""
+    c = kats["cell_crossing_%dd" % dim]
+    o = oracle.Oracle(dim)
+    o.set_mesh(np.array(c["nodes"], dtype=float), np.array(c["cells"]))
     m = c["material"]
     o.set_materials([materials.linear_elastic(m["youngs_modulus"], m["poisson_ratio"],
                                               m["density"])])
@@ -295,9 +310,15 @@ def test_cell_crossing_2d(kats):
     o.phase("acceleration_velocity_plain", c["dt"])
     o.phase("position", c["dt"])
     o.phase("locate")
-    cells = o.particles()["cell"]
+    p = o.particles()
+    cells = p["cell"]
     assert list(np.bincount(cells, minlength=2)) == c["cell_counts_after"]
-    assert list(cells) == [0, 1, 1, 0]
+    # the constrained nodal x velocity is 1000 x (0 on the x = 0 face, 1000 on
+    # x = 1), so G2P moves a particle by sum_i N_i v_i dt = 0.4 x: the ones at
+    # x = 0.75 end at 1.05, in cell 1
+    x0 = np.array(c["particles"])[:, 0]
+    np.testing.assert_allclose(p["x"][:, 0], 1.4 * x0, rtol=1e-13)
+    assert list(cells) == [1 if x > 0.5 else 0 for x in x0]
 
 
 def _gauss_order(dim, nppd):
@@ -346,3 +367,102 @@ def test_gauss_generator_points_and_order(dim, nppd):
         np.testing.assert_allclose(p["x"][c * per:(c + 1) * per], want, rtol=0, atol=1e-15)
         assert np.all(p["cell"][c * per:(c + 1) * per] == c)
     np.testing.assert_allclose(p["volume"], np.prod(h) / per, rtol=1e-15)
+
+
+def test_oracle_locate_particles_false_removes_in_order():
+    """Removal (mpm_scheme.tcc:258-260, mesh.tcc:658-665): the lost particle
+    is erased and the others keep their relative order; with
+    locate_particles true the same step throws instead."""
+    import lost_particle_deck as deck
+    a = deck.setup(oracle.Oracle(2, "usf"))
+    b = deck.setup(oracle.Oracle(2, "usf"))
+    b.set_locate_particles(False)
+    x0 = a.particles()["x"]
+    for s in range(200):
+        try:
+            a.step(1, deck.DT)
+        except oracle.OracleError as e:
+            assert "outside the mesh" in str(e)
+            break
+    else:
+        raise AssertionError("never left the mesh")
+    b.step(s + 1, deck.DT)
+    assert b.nparticles == len(x0) - 1
+    pa, pb = a.particles(), b.particles()
+    # the survivors of the failing step are the throwing run's particles 1..n
+    for f in ("x", "v", "stress", "cell"):
+        assert np.array_equal(pa[f][1:], pb[f]), f
+    # the velocity constraint of the last particle followed it
+    assert pb["v"][-1, 0] == 1e-3
+
+
+def _structured_3d():
+    from mpm_amd.mesh import structured_mesh
+    # dyadic coordinates: node positions, centres and lengths are exact
+    origin, h, n = np.array([0.125, -0.25, 0.5]), np.array([0.5, 1.0, 0.25]), (3, 2, 4)
+    coords, cells = structured_mesh(tuple(origin), tuple(h), n)
+    return origin, h, n, coords, cells
+
+
+def _located(coords, cells, x, hint=None):
+    o = oracle.Oracle(3)
+    o.set_mesh(coords, cells)
+    o.set_materials([materials.linear_elastic(1e6, 0.3, 1000.0)])
+    k = len(x)
+    o.set_particles(x=x, mass=[1.0] * k, volume=[1.0] * k, density=[1000.0] * k, material_id=0,
+                    cell=hint)
+    p = o.particles()
+    return p["cell"], p["xi"]
+
+
+def test_oracle_locate_3d_structured_closed_form():
+    """3D binning (mesh.tcc:1116-1153, cell.tcc:335-380, 492-606) on a mesh with
+    three different spacings: a point strictly inside a cell lands in
+    floor((x - origin) / h) with xi = 2 (x - centre) / h per axis (an axis
+    mix-up, a wrong edge length or a wrong cell numbering fails here)."""
+    origin, h, n, coords, cells = _structured_3d()
+    rng = np.random.default_rng(3)
+    idx = np.stack([rng.integers(0, n[k], 400) for k in range(3)], axis=1)
+    frac = rng.uniform(0.02, 0.98, size=(400, 3))
+    x = origin + (idx + frac) * h
+    cell, xi = _located(coords, cells, x)
+    expect = idx[:, 0] + n[0] * (idx[:, 1] + n[1] * idx[:, 2])
+    assert np.array_equal(cell, expect)
+    np.testing.assert_allclose(xi, 2.0 * frac - 1.0, rtol=0, atol=1e-13)
+
+
+def test_oracle_locate_3d_boundaries_and_hints():
+    """Points on shared faces, edges and corners: without a hint the full scan
+    takes the lowest cell id that contains the point (inclusive bounds, xi
+    clamped to 1 - DBL_EPSILON); a hint cell that contains the point is
+    kept; a hint cell that does not is followed by its node-sharing
+    neighbours in ascending id, then by the full scan."""
+    origin, h, n, coords, cells = _structured_3d()
+    cid = lambda i, j, k: i + n[0] * (j + n[1] * k)  # noqa: E731
+    eps = np.finfo(float).eps
+    # corner node (1, 1, 2): shared by the cells (0..1, 0..1, 1..2)
+    corner = origin + np.array([1, 1, 2]) * h
+    c8 = sorted(cid(i, j, k) for i in (0, 1) for j in (0, 1) for k in (1, 2))
+    cell, xi = _located(coords, cells, [corner])
+    assert cell[0] == c8[0]
+    np.testing.assert_array_equal(xi[0], [1.0 - eps] * 3)
+    # every cell touching the corner, given as hint, is kept
+    cell, xi = _located(coords, cells, [corner] * 8, hint=c8)
+    assert list(cell) == c8
+    # face point between (1,0,0) and (2,0,0): no hint -> 1; hint 2 -> kept
+    face = origin + np.array([2.0, 0.5, 0.5]) * h
+    cell, _ = _located(coords, cells, [face, face], hint=[-1, cid(2, 0, 0)])
+    assert list(cell) == [cid(1, 0, 0), cid(2, 0, 0)]
+    # a point on the face x = h between cells 0 = (0,0,0) and 1 = (1,0,0),
+    # with hint (2,0,0), which does not contain it: among the hint's
+    # neighbours (cells sharing a node with it) only cell 1 contains it, so
+    # the neighbour search gives 1 where the full scan would give 0
+    pt = origin + np.array([1.0, 0.5, 0.5]) * h
+    cell, _ = _located(coords, cells, [pt, pt], hint=[-1, cid(2, 0, 0)])
+    assert list(cell) == [cid(0, 0, 0), cid(1, 0, 0)]
+    # the same point with a hint far from both: full scan -> lowest id
+    cell, _ = _located(coords, cells, [pt], hint=[cid(2, 1, 3)])
+    assert cell[0] == cid(0, 0, 0)
+    # a point just outside the domain is not located
+    with pytest.raises(oracle.OracleError):
+        _located(coords, cells, [origin - 1e-9])
