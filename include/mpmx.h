@@ -255,6 +255,25 @@ int mpmx_step(mpmx_ctx* ctx, int nsteps, double dt, double damping_factor);
  * the reference, whose scatter works in particle order (mpm_scheme.tcc:44-71). */
 int mpmx_set_sort_interval(mpmx_ctx* ctx, int steps);
 
+/* locate_particles (the analysis key read at mpm_base.tcc:53-54, default
+ * true): with 1, a particle that no cell contains after a step stops the run
+ * at that step with MPMX_E_UNLOCATABLE (MPMScheme::locate_particles throws,
+ * mpm_scheme.tcc:250-256); the state returned is the state at the end of
+ * that step, later queued steps do nothing. With 0 such particles are
+ * removed (mpm_scheme.tcc:258-260 -> Mesh::remove_particle, mesh.tcc:658-665):
+ * they take no part in later steps and are dropped from the particle arrays,
+ * the others keeping their order; particle-indexed loads follow their
+ * particles. */
+int mpmx_set_locate_particles(mpmx_ctx* ctx, int locate_particles);
+
+/* The reference's particle ids (ParticleBase::id()) in particle order: a
+ * particle created by mpmx_generate_gauss / mpmx_set_particles gets the
+ * number of particles before it (pid = particles_.size(), mesh.tcc:531-536);
+ * mpmx_set_particle_ids replaces them (n = particle count), e.g. with the ids
+ * of a resumed checkpoint (particle.tcc:35-40). */
+int mpmx_get_particle_ids(mpmx_ctx* ctx, int64_t* ids);
+int mpmx_set_particle_ids(mpmx_ctx* ctx, int64_t n, const int64_t* ids);
+
 /* Fast mode: particles per workgroup of the LDS-staged scatter kernels (the
  * chunk of cell-sorted particles whose per-node sums are formed in LDS
  * before the FP64 atomics): 128, 256 (default) or 512. No counterpart in the

@@ -31,7 +31,8 @@ EXPORTS = (
     "mpmx_set_gravity", "mpmx_generate_gauss",
     "mpmx_set_particles", "mpmx_compute_volume_mass", "mpmx_assign_velocity",
     "mpmx_assign_stresses", "mpmx_nparticles", "mpmx_get_particles", "mpmx_get_nodes",
-    "mpmx_step", "mpmx_set_sort_interval", "mpmx_set_chunk", "mpmx_autotune", "mpmx_set_step",
+    "mpmx_step", "mpmx_set_locate_particles", "mpmx_get_particle_ids", "mpmx_set_particle_ids",
+    "mpmx_set_sort_interval", "mpmx_set_chunk", "mpmx_autotune", "mpmx_set_step",
     "mpmx_get_step",
     "mpmx_synchronize", "mpmx_profile", "mpmx_kernel_times",
     "mpmx_adjoint_forward", "mpmx_adjoint_backward", "mpmx_adjoint_set_apex_ratio",
@@ -89,6 +90,9 @@ def load(path=LIB_PATH):
         "mpmx_get_nodes": [V, _D, _D, _D, _I32],
         "mpmx_step": [V, C.c_int, C.c_double, C.c_double],
         "mpmx_set_step": [V, C.c_int64],
+        "mpmx_set_locate_particles": [V, C.c_int],
+        "mpmx_get_particle_ids": [V, _I64],
+        "mpmx_set_particle_ids": [V, C.c_int64, _I64],
         "mpmx_set_sort_interval": [V, C.c_int],
         "mpmx_set_chunk": [V, C.c_int],
         "mpmx_autotune": [V, C.c_double, C.c_double, C.c_int, _I32, _I32],
@@ -330,6 +334,20 @@ class Solver:
         s = C.c_int64(0)
         self._check(self.lib.mpmx_get_step(self.ctx, C.byref(s)))
         return s.value
+
+    def set_locate_particles(self, locate_particles=True):
+        """True: a particle that leaves the mesh fails the step (the default);
+        False: such particles are removed (the analysis key locate_particles)."""
+        self._check(self.lib.mpmx_set_locate_particles(self.ctx, 1 if locate_particles else 0))
+
+    def particle_ids(self):
+        ids = np.zeros(self.nparticles, dtype=np.int64)
+        self._check(self.lib.mpmx_get_particle_ids(self.ctx, _p(ids, _I64)))
+        return ids
+
+    def set_particle_ids(self, ids):
+        ids = np.ascontiguousarray(ids, dtype=np.int64)
+        self._check(self.lib.mpmx_set_particle_ids(self.ctx, len(ids), _p(ids, _I64)))
 
     def set_sort_interval(self, steps):
         """Fast mode: full particle sort by cell every `steps` steps."""

@@ -145,6 +145,12 @@ struct mpmx_ctx {
 
   // particles
   int64_t np = 0;
+  // the reference's particle ids (Particle::id(): particles_.size() when the
+  // particle was created, mesh.tcc:531-536, or given), in particle order
+  std::vector<int64_t> ids;
+  // locate_particles (mpm_base.tcc:53-54): false removes particles that leave
+  // the mesh instead of failing the step (mpm_scheme.tcc:245-261)
+  bool locate_all = true;
   DBuf<double> d_x, d_u, d_v, d_stress, d_strain, d_F, d_mass, d_vol, d_rho, d_xi, d_sv;
   DBuf<int> d_mat, d_cell;
 
@@ -717,6 +723,7 @@ void reset_status(mpmx_ctx* c) {
   s.nplastic = 0;
   s.nplastic_t = 0;
   s.ad_slot = 0;
+  s.nlost = 0;
   *c->h_status = s;
   c->d_status.upload(c->h_status, 1, c->stream);
 }
@@ -747,10 +754,10 @@ void locate_new(mpmx_ctx* c, int64_t off, int64_t n, const std::vector<int>& hin
   Mesh M = c->mesh();
   if (c->dim == 2) {
     launch_locate_init<2>(M, P, c->d_status.p, c->d_scan_list.p, d_hint.p, c->stream);
-    launch_locate_scan<2>(M, P, c->d_status.p, c->d_scan_list.p, d_hint.p, c->stream);
+    launch_locate_scan<2>(M, P, c->d_status.p, c->d_scan_list.p, d_hint.p, 0, c->stream);
   } else {
     launch_locate_init<3>(M, P, c->d_status.p, c->d_scan_list.p, d_hint.p, c->stream);
-    launch_locate_scan<3>(M, P, c->d_status.p, c->d_scan_list.p, d_hint.p, c->stream);
+    launch_locate_scan<3>(M, P, c->d_status.p, c->d_scan_list.p, d_hint.p, 0, c->stream);
   }
   HIPCHECK(hipGetLastError());
   read_status(c);
@@ -788,6 +795,8 @@ int64_t append_particles(mpmx_ctx* c, int64_t n, const double* x, const double* 
   c->ad_recorded = false;
   c->psize.resize(np);
   for (int64_t i = 0; i < n; ++i) c->psize[off + i] = particle_size(volume ? volume[i] : 1.0, d);
+  c->ids.resize(np);
+  for (int64_t i = 0; i < n; ++i) c->ids[off + i] = off + i;  // particles_.size() at creation
   build_pvc(c);
   build_trac(c);
   std::vector<double> zeros((size_t)n * 9, 0.0), eye((size_t)n * 9, 0.0);
@@ -845,6 +854,7 @@ void truncate_particles(mpmx_ctx* c, int64_t keep) {
   c->np = keep;
   c->ad_recorded = false;
   c->psize.resize(keep);
+  c->ids.resize(keep);
   build_pvc(c);
   build_trac(c);
   alloc_step_buffers(c);
@@ -941,28 +951,28 @@ void enqueue_step(mpmx_ctx* c, const StepArgs& A, Launcher& L, bool full_sort) {
   } else {
     if (c->scheme == USF) {
       L("p2g_mass_momentum", [&] { launch_p2g<D>(P2G_MV, c->chunk, M, P, Nd, A, S, skey, perm, st); });
-      L("node_velocity", [&] { launch_node<D>(ND_INIT | ND_VEL, M, Nd, A, st); });
+      L("node_velocity", [&] { launch_node<D>(ND_INIT | ND_VEL, M, Nd, A, S, st); });
       L("stress_force_scatter", [&] { launch_stress_scatter<D>(c->chunk, M, P, Nd, mats, A, S, skey, perm, st); });
-      if (P.ntr > 0) L("traction_scatter", [&] { launch_traction_scatter<D>(M, P, Nd, st); });
-      L("node_acceleration", [&] { launch_node<D>(ND_ACC | ND_ZERO_ALL, M, Nd, A, st); });
+      if (P.ntr > 0) L("traction_scatter", [&] { launch_traction_scatter<D>(M, P, Nd, S, st); });
+      L("node_acceleration", [&] { launch_node<D>(ND_ACC | ND_ZERO_ALL, M, Nd, A, S, st); });
       L("g2p_locate", [&] { launch_g2p<D>(false, true, M, P, Nd, mats, A, c->vu, S, scan, st); });
     } else if (c->scheme == USL) {
       L("p2g_all", [&] { launch_p2g<D>(P2G_MV_F, c->chunk, M, P, Nd, A, S, skey, perm, st); });
-      if (P.ntr > 0) L("traction_scatter", [&] { launch_traction_scatter<D>(M, P, Nd, st); });
-      L("node_update", [&] { launch_node<D>(ND_INIT | ND_VEL | ND_ACC | ND_ZERO_ALL, M, Nd, A, st); });
+      if (P.ntr > 0) L("traction_scatter", [&] { launch_traction_scatter<D>(M, P, Nd, S, st); });
+      L("node_update", [&] { launch_node<D>(ND_INIT | ND_VEL | ND_ACC | ND_ZERO_ALL, M, Nd, A, S, st); });
       L("g2p_stress_locate", [&] { launch_g2p<D>(true, true, M, P, Nd, mats, A, c->vu, S, scan, st); });
       plastic();
     } else {
       L("p2g_all", [&] { launch_p2g<D>(P2G_MV_F, c->chunk, M, P, Nd, A, S, skey, perm, st); });
-      if (P.ntr > 0) L("traction_scatter", [&] { launch_traction_scatter<D>(M, P, Nd, st); });
-      L("node_update", [&] { launch_node<D>(ND_INIT | ND_VEL | ND_ACC | ND_ZERO_MOM, M, Nd, A, st); });
-      L("g2p_momentum_scatter", [&] { launch_g2p_scatter<D>(c->chunk, M, P, Nd, A, c->vu, skey, perm, st); });
-      L("node_velocity", [&] { launch_node<D>(ND_VEL | ND_ZERO_ALL, M, Nd, A, st); });
+      if (P.ntr > 0) L("traction_scatter", [&] { launch_traction_scatter<D>(M, P, Nd, S, st); });
+      L("node_update", [&] { launch_node<D>(ND_INIT | ND_VEL | ND_ACC | ND_ZERO_MOM, M, Nd, A, S, st); });
+      L("g2p_momentum_scatter", [&] { launch_g2p_scatter<D>(c->chunk, M, P, Nd, A, c->vu, S, skey, perm, st); });
+      L("node_velocity", [&] { launch_node<D>(ND_VEL | ND_ZERO_ALL, M, Nd, A, S, st); });
       L("stress_locate", [&] { launch_stress<D>(true, M, P, Nd, mats, A, S, scan, st); });
       plastic();
     }
   }
-  L("locate_scan", [&] { launch_locate_scan<D>(M, P, S, scan, nullptr, st); });
+  L("locate_scan", [&] { launch_locate_scan<D>(M, P, S, scan, nullptr, c->locate_all ? 0 : 1, st); });
   L("step_end", [&] { launch_step_end(S, st); });
 }
 
@@ -1029,11 +1039,78 @@ void launch_steps(mpmx_ctx* c, int nsteps, double dt, double damping,
   }
 }
 
+// Remove the particles whose cell is -1 (they left the mesh in remove mode),
+// keeping the order of the others (Mesh::remove_particle erases from the
+// particle vector with std::remove, mesh.tcc:658-665, vector.tcc:23-30).
+// Particle-indexed loads follow their particles; those of removed particles
+// are dropped.
+void remove_lost_particles(mpmx_ctx* c) {
+  const int64_t np = c->np;
+  std::vector<int> cell(np);
+  c->d_cell.download(cell.data(), np, c->stream);
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  std::vector<int> keep;
+  std::vector<int64_t> remap(np, -1);
+  for (int64_t p = 0; p < np; ++p)
+    if (cell[p] >= 0) {
+      remap[p] = (int64_t)keep.size();
+      keep.push_back((int)p);
+    }
+  const int64_t nk = (int64_t)keep.size();
+  if (nk == np) return;
+  hipStream_t st = c->stream;
+  DBuf<int> d_idx;
+  d_idx.alloc(std::max<int64_t>(nk, 1));
+  d_idx.upload(keep.data(), nk, st);
+  auto take = [&](auto& b, int comps) {
+    using T = typename std::remove_reference<decltype(*b.p)>::type;
+    DBuf<T> nb;
+    nb.alloc((size_t)std::max<int64_t>(nk, 1) * comps);
+    if (nk > 0) take_rows<T>(b.p, nb.p, d_idx.p, nk, np, comps, st);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(st));
+    std::swap(b.p, nb.p);
+    std::swap(b.n, nb.n);
+  };
+  const int d = c->dim;
+  take(c->d_x, d); take(c->d_u, d); take(c->d_v, d); take(c->d_xi, d);
+  take(c->d_stress, 6); take(c->d_strain, 6); take(c->d_F, 9);
+  take(c->d_mass, 1); take(c->d_vol, 1); take(c->d_rho, 1);
+  take(c->d_mat, 1); take(c->d_cell, 1);
+  if (c->any_mc) take(c->d_sv, NSV);
+  std::vector<double> psize(nk);
+  std::vector<int64_t> ids(nk);
+  for (int64_t i = 0; i < nk; ++i) {
+    psize[i] = c->psize[keep[i]];
+    ids[i] = c->ids[keep[i]];
+  }
+  c->psize.swap(psize);
+  c->ids.swap(ids);
+  std::vector<mpmx_ctx::Pvc> pvc;
+  for (const auto& q : c->pvc)
+    if (q.particle < 0 || (q.particle < np && remap[q.particle] >= 0))
+      pvc.push_back({q.particle < 0 ? q.particle : remap[q.particle], q.dir, q.value});
+  c->pvc.swap(pvc);
+  std::vector<mpmx_ctx::Trac> trac;
+  for (const auto& t : c->trac)
+    if (t.particle < 0 || (t.particle < np && remap[t.particle] >= 0))
+      trac.push_back({t.particle < 0 ? t.particle : remap[t.particle], t.dir, t.value, t.fn});
+  c->trac.swap(trac);
+  c->np = nk;
+  c->ad_recorded = false;
+  build_pvc(c);
+  build_trac(c);
+  alloc_step_buffers(c);
+  c->invalidate_graph();
+}
+
 // Wait for the queued steps; take the step counter; fail if a particle left
-// the mesh (locate_particles, mpm_scheme.tcc:250-256).
+// the mesh (locate_particles, mpm_scheme.tcc:250-256), or, in remove mode,
+// remove the particles that did (mpm_scheme.tcc:258-260).
 void check_status(mpmx_ctx* c) {
   read_status(c);
   c->step = c->h_status->step;
+  if (c->h_status->nlost > 0) remove_lost_particles(c);
   if (c->h_status->first_bad_step != std::numeric_limits<int64_t>::max())
     throw ApiError(MPMX_E_UNLOCATABLE,
                    "Particle(s) outside the mesh domain: " + std::to_string(c->h_status->bad_particle) +
@@ -1830,6 +1907,27 @@ int mpmx_set_sort_interval(mpmx_ctx* c, int steps) {
   API_BEGIN(c)
   if (steps < 1) throw ApiError(MPMX_E_ARG, "sort interval must be >= 1");
   c->sort_every = steps;
+  API_END(c)
+}
+
+int mpmx_set_locate_particles(mpmx_ctx* c, int locate_particles) {
+  API_BEGIN(c)
+  c->locate_all = locate_particles != 0;
+  c->invalidate_graph();
+  API_END(c)
+}
+
+int mpmx_get_particle_ids(mpmx_ctx* c, int64_t* ids) {
+  API_BEGIN(c)
+  if (!ids && c->np > 0) throw ApiError(MPMX_E_ARG, "null ids");
+  std::copy(c->ids.begin(), c->ids.end(), ids);
+  API_END(c)
+}
+
+int mpmx_set_particle_ids(mpmx_ctx* c, int64_t n, const int64_t* ids) {
+  API_BEGIN(c)
+  if (n != c->np || (!ids && n > 0)) throw ApiError(MPMX_E_ARG, "one id per particle required");
+  std::copy(ids, ids + n, c->ids.begin());
   API_END(c)
 }
 

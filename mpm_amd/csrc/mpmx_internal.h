@@ -169,6 +169,7 @@ struct Status {
   int nplastic;            // shear entries of the deferred plastic list (its front)
   int nplastic_t;          // tensile entries (its back)
   int ad_slot;             // step index within an adjoint segment (recording)
+  int nlost;               // particles removed because they left the mesh (remove mode)
 };
 
 }  // namespace mpmx

@@ -17,9 +17,11 @@ enum { GK_MV_VEL = 0, GK_F_ACC = 1, GK_ALL_VEL_ACC = 2, GK_MOM_VEL = 3 };
 template <int D>
 void launch_locate_init(const Mesh& M, const Particles& P, Status* S, int* scan_list,
                         const int* hint, hipStream_t st);
+// remove_lost: a particle found in no cell is counted in Status::nlost for
+// removal instead of stopping the run (Status::first_bad_step).
 template <int D>
 void launch_locate_scan(const Mesh& M, const Particles& P, Status* S, const int* scan_list,
-                        const int* hint, hipStream_t st);
+                        const int* hint, int remove_lost, hipStream_t st);
 void launch_step_end(Status* S, hipStream_t st);
 // values of the nf (<= kMaxMathFns) math functions at the current step's time
 constexpr int kMaxMathFns = 256;
@@ -40,7 +42,8 @@ void launch_stress(bool locate, const Mesh& M, const Particles& P, const Nodes& 
                    hipStream_t st);
 // Particle tractions onto the nodes' external force (fast mode; P.ntr > 0).
 template <int D>
-void launch_traction_scatter(const Mesh& M, const Particles& P, const Nodes& Nd, hipStream_t st);
+void launch_traction_scatter(const Mesh& M, const Particles& P, const Nodes& Nd, const Status* S,
+                             hipStream_t st);
 // The deferred Mohr-Coulomb return mappings queued by launch_stress and
 // launch_g2p (stress) this step; runs after them, before anything reads the
 // stress or state variables.
@@ -51,10 +54,11 @@ void launch_g2p(bool stress, bool locate, const Mesh& M, const Particles& P, con
                 hipStream_t st);
 template <int D>
 void launch_g2p_scatter(int chunk, const Mesh& M, const Particles& P, const Nodes& Nd,
-                        const StepArgs& A, int vu, const int* skey, const int* perm,
-                        hipStream_t st);
+                        const StepArgs& A, int vu, const Status* S, const int* skey,
+                        const int* perm, hipStream_t st);
 template <int D>
-void launch_node(int fl, const Mesh& M, const Nodes& Nd, const StepArgs& A, hipStream_t st);
+void launch_node(int fl, const Mesh& M, const Nodes& Nd, const StepArgs& A, const Status* S,
+                 hipStream_t st);
 template <int D>
 void launch_cell_ranges(int64_t np, int64_t ncells, const int* skey, int* cstart, int* cend,
                         hipStream_t st);
@@ -62,6 +66,12 @@ template <int D>
 void launch_gather(int which, const Mesh& M, const Particles& P, const Nodes& Nd,
                    const StepArgs& A, const Status* S, const int* perm, const int* cstart,
                    const int* cend, hipStream_t st);
+
+// out[k * nnew + i] = in[k * nold + idx[i]] for the comps components of a
+// component-major array (T = double or int).
+template <class T>
+void take_rows(const T* in, T* out, const int* idx, int64_t nnew, int64_t nold, int comps,
+               hipStream_t st);
 
 // Stable LSD radix sort of (cell key, particle id) pairs. keys_in: cell of
 // each particle in particle order (values are the particle ids 0..n-1).

@@ -55,6 +55,12 @@ MPMX_DEV bool valid_cell(const Mesh& M, int c) {
   return c >= 0 && (int64_t)c < M.ncells;
 }
 
+// A particle could not be located in an earlier step of the queued run: the
+// reference throws at the end of that step (mpm_scheme.tcc:250-256), so the
+// kernels of every later step return at once and leave the state as that
+// step ended it. The same value for every workgroup of a launch.
+MPMX_DEV bool run_stopped(const Status* S) { return S->first_bad_step < S->step; }
+
 // ------------------------- per-particle building blocks ----------------------
 // Nodal velocity (and acceleration) of the particle's cell nodes.
 template <int D>
@@ -366,10 +372,15 @@ __global__ __launch_bounds__(kBlock) void k_locate_init(Mesh M, Particles P, Sta
 // contains the point (the single-threaded reading of mesh.tcc:1142-1151).
 // With instances, the scan covers the cells of the particle's instance (that
 // of its previous cell, or of its hint at initial placement). Persistent
-// grid; every block exits after the queue is drained.
+// grid; every block exits after the queue is drained. A particle found in no
+// cell gets cell -1 (every step kernel skips it from then on) and either
+// stops the run (the reference throws, mpm_scheme.tcc:250-256) or, with
+// remove_lost, is counted for removal (mpm_scheme.tcc:258-260).
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_locate_scan(Mesh M, Particles P, Status* S,
-                                                        const int* scan_list, const int* hint) {
+                                                        const int* scan_list, const int* hint,
+                                                        int remove_lost) {
+  if (run_stopped(S)) return;
   __shared__ int best;
   const int n = S->nscan;
   for (int q = blockIdx.x; q < n; q += gridDim.x) {
@@ -406,9 +417,13 @@ __global__ __launch_bounds__(kBlock) void k_locate_scan(Mesh M, Particles P, Sta
         for (int k = 0; k < D; ++k) P.xi[k * P.np + p] = xi[k];
       } else {
         P.cell[p] = -1;
-        const long long step = (long long)S->step;
-        atomicMin((unsigned long long*)&S->first_bad_step, (unsigned long long)step);
-        atomicExch(&S->bad_particle, p);
+        if (remove_lost) {
+          atomicAdd(&S->nlost, 1);
+        } else {
+          const long long step = (long long)S->step;
+          atomicMin((unsigned long long*)&S->first_bad_step, (unsigned long long)step);
+          atomicExch(&S->bad_particle, p);
+        }
       }
     }
     __syncthreads();
@@ -425,6 +440,7 @@ __global__ void k_fn_eval(const MathFn* fns, const double* data, int nf, const S
 }
 
 __global__ void k_step_end(Status* S) {
+  if (run_stopped(S)) return;
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     S->step += 1;
     S->nscan = 0;
@@ -553,13 +569,12 @@ MPMX_DEV void scatter_segments(Stage<D, WHAT, CH>& st, int cnt, const Mesh& M, c
   }
   __syncthreads();
   const int nseg = st.nseg;
-  // number of valid staged particles = end of last segment
-  int nvalid = 0;
-  {
-    // valid particles are a prefix of the chunk (invalid keys sort last)
-    nvalid = cnt;
-    while (nvalid > 0 && st.key[nvalid - 1] < 0) --nvalid;
-  }
+  // A segment runs from its head to the next head or to the first invalid
+  // slot before it, whichever comes first. Between full sorts the keys are
+  // an earlier permutation's with current cells, so a particle that left the
+  // mesh (key -1, nothing staged) can sit inside a run of valid keys; the
+  // valid slot after it is a head (its key differs from -1), so invalid
+  // slots only ever trail a segment.
   using CP = Comp<D, WHAT>;
   const int ntask = nseg * NN;
   for (int t0 = 0; t0 < ntask; t0 += CH) {  // same trip count for every lane
@@ -572,12 +587,12 @@ MPMX_DEV void scatter_segments(Stage<D, WHAT, CH>& st, int cnt, const Mesh& M, c
     for (int k = 0; k < CP::NC; ++k) vals[k] = 0.;
     if (task < ntask) {
     const int b = st.seg_start[sidx];
-    const int e = (sidx + 1 < nseg) ? st.seg_start[sidx + 1] : nvalid;
+    const int e = (sidx + 1 < nseg) ? st.seg_start[sidx + 1] : cnt;
     const int c = st.key[b];
     double am = 0., amom[D], afe[D], afi[D];
 #pragma unroll
     for (int k = 0; k < D; ++k) amom[k] = afe[k] = afi[k] = 0.;
-    for (int t = b; t < e; ++t) {
+    for (int t = b; t < e && st.key[t] >= 0; ++t) {
       double xi[D];
 #pragma unroll
       for (int k = 0; k < D; ++k) xi[k] = st.xi[k][t];
@@ -658,6 +673,7 @@ template <int D, int WHAT, int CH>
 __global__ __launch_bounds__(CH) void k_p2g(Mesh M, Particles P, Nodes Nd, StepArgs A,
                                                 const Status* S, const int* skey,
                                                 const int* perm) {
+  if (run_stopped(S)) return;
   __shared__ Stage<D, WHAT, CH> st;
   const int64_t base = blockIdx.x * (int64_t)CH;
   const int cnt = (int)((P.np - base) < CH ? (P.np - base) : CH);
@@ -704,6 +720,7 @@ __global__ __launch_bounds__(CH) MPMX_MC_ATTR void k_stress_scatter(Mesh M, Part
                                                            const Material* mats, StepArgs A,
                                                            const Status* S, const int* skey,
                                                            const int* perm) {
+  if (run_stopped(S)) return;
   constexpr int NN = Elem<D>::NN;
   __shared__ Stage<D, SC_F, CH> st;
   const int64_t base = blockIdx.x * (int64_t)CH;
@@ -748,6 +765,7 @@ __global__ __launch_bounds__(kStressBlock) MPMX_PRED_ATTR void k_stress(Mesh M, 
                                                                       const Material* mats,
                                                                       StepArgs A, Status* S,
                                                                       int* scan_list) {
+  if (run_stopped(S)) return;
   constexpr int NN = Elem<D>::NN;
   const int64_t p = blockIdx.x * (int64_t)kStressBlock + threadIdx.x;
   if (p >= P.np) return;
@@ -776,6 +794,7 @@ template <int D, bool STRESS, bool LOCATE, bool ROT = false>
 __global__ __launch_bounds__(kBlock) MPMX_PRED_ATTR void k_g2p(Mesh M, Particles P, Nodes Nd,
                                                 const Material* mats, StepArgs A, int vu,
                                                 Status* S, int* scan_list) {
+  if (run_stopped(S)) return;
   constexpr int NN = Elem<D>::NN;
   const int64_t p = blockIdx.x * (int64_t)kBlock + threadIdx.x;
   if (p >= P.np) return;
@@ -841,6 +860,7 @@ constexpr int kPlasticBlock = 64;  // one wave per workgroup (run times vary by 
 __global__ __launch_bounds__(kPlasticBlock) MPMX_PLASTIC_ATTR void k_plastic(Particles P,
                                                                              const Material* mats,
                                                                              const Status* S) {
+  if (run_stopped(S)) return;
   const int ns = S->nplastic, nt = S->nplastic_t;
   const int64_t np = P.np;
   for (int c = blockIdx.x * kPlasticBlock + threadIdx.x; c < ns + nt;
@@ -893,8 +913,9 @@ __global__ __launch_bounds__(kPlasticBlock) MPMX_PLASTIC_ATTR void k_plastic(Par
 // because the shape functions are not recomputed).
 template <int D, int CH>
 __global__ __launch_bounds__(CH) void k_g2p_scatter(Mesh M, Particles P, Nodes Nd, StepArgs A,
-                                                        int vu, const int* skey,
+                                                        int vu, const Status* S, const int* skey,
                                                         const int* perm) {
+  if (run_stopped(S)) return;
   constexpr int NN = Elem<D>::NN;
   __shared__ Stage<D, SC_MOM, CH> st;
   const int64_t base = blockIdx.x * (int64_t)CH;
@@ -960,7 +981,9 @@ __global__ __launch_bounds__(CH) void k_g2p_scatter(Mesh M, Particles P, Nodes N
 // (Particle::map_traction_force, particle.tcc:983-990), one thread per
 // particle with a traction, FP64 atomics (fast mode).
 template <int D>
-__global__ __launch_bounds__(kBlock) void k_traction_scatter(Mesh M, Particles P, Nodes Nd) {
+__global__ __launch_bounds__(kBlock) void k_traction_scatter(Mesh M, Particles P, Nodes Nd,
+                                                             const Status* S) {
+  if (run_stopped(S)) return;
   constexpr int NN = Elem<D>::NN;
   const int t = blockIdx.x * kBlock + threadIdx.x;
   if (t >= P.ntr) return;
@@ -1029,7 +1052,8 @@ MPMX_DEV void ad_record_second(const StepArgs& A, int64_t n, int64_t nn, const d
 // compute_velocity (ND_VEL) and/or compute_acceleration_velocity_cundall
 // (ND_ACC) for active nodes; then clears accumulators for the next pass.
 template <int D, int FL>
-__global__ __launch_bounds__(kBlock) void k_node(Mesh M, Nodes Nd, StepArgs A) {
+__global__ __launch_bounds__(kBlock) void k_node(Mesh M, Nodes Nd, StepArgs A, const Status* S) {
+  if (run_stopped(S)) return;
   const int64_t n = blockIdx.x * (int64_t)kBlock + threadIdx.x;
   if (n >= M.nnodes) return;
   const int64_t nn = M.nnodes;
@@ -1126,6 +1150,7 @@ template <int D, int WHAT, int FL>
 __global__ __launch_bounds__(kBlock) void k_gather(Mesh M, Particles P, Nodes Nd, StepArgs A,
                                                    const Status* S, const int* perm,
                                                    const int* cstart, const int* cend) {
+  if (run_stopped(S)) return;
   constexpr int NN = Elem<D>::NN;
   const int64_t n = blockIdx.x * (int64_t)kBlock + threadIdx.x;
   if (n >= M.nnodes) return;
@@ -1305,6 +1330,17 @@ __global__ __launch_bounds__(kBlock) void k_gather(Mesh M, Particles P, Nodes Nd
   }
 }
 
+// Row compaction of a component-major array: out[k * nnew + i] =
+// in[k * nold + idx[i]] for every component k (removal of particles).
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_take_rows(const T* in, T* out, const int* idx,
+                                                      int64_t nnew, int64_t nold, int comps) {
+  const int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+  if (i >= nnew) return;
+  const int64_t j = idx[i];
+  for (int k = 0; k < comps; ++k) out[k * nnew + i] = in[k * nold + j];
+}
+
 // ------------------------------- launchers ----------------------------------
 static inline dim3 grid_for(int64_t n) { return dim3((unsigned)((n + kBlock - 1) / kBlock)); }
 
@@ -1340,9 +1376,9 @@ void launch_locate_init(const Mesh& M, const Particles& P, Status* S, int* scan_
 
 template <int D>
 void launch_locate_scan(const Mesh& M, const Particles& P, Status* S, const int* scan_list,
-                        const int* hint, hipStream_t st) {
+                        const int* hint, int remove_lost, hipStream_t st) {
   hipLaunchKernelGGL((k_locate_scan<D>), dim3(256), dim3(kBlock), 0, st, M, P, S, scan_list,
-                     hint);
+                     hint, remove_lost);
 }
 
 void launch_fn_eval(const MathFn* fns, const double* data, int nf, const Status* S, double dt,
@@ -1406,8 +1442,9 @@ void launch_stress(bool locate, const Mesh& M, const Particles& P, const Nodes& 
 }
 
 template <int D>
-void launch_traction_scatter(const Mesh& M, const Particles& P, const Nodes& Nd, hipStream_t st) {
-  MPMX_LAUNCH((k_traction_scatter<D>), P.ntr, st, M, P, Nd);
+void launch_traction_scatter(const Mesh& M, const Particles& P, const Nodes& Nd, const Status* S,
+                             hipStream_t st) {
+  MPMX_LAUNCH((k_traction_scatter<D>), P.ntr, st, M, P, Nd, S);
 }
 
 void launch_plastic(const Particles& P, const Material* mats, const Status* S, hipStream_t st) {
@@ -1433,18 +1470,19 @@ void launch_g2p(bool stress, bool locate, const Mesh& M, const Particles& P, con
 
 template <int D>
 void launch_g2p_scatter(int chunk, const Mesh& M, const Particles& P, const Nodes& Nd,
-                        const StepArgs& A, int vu, const int* skey, const int* perm,
-                        hipStream_t st) {
-#define L_(CH) MPMX_LAUNCH_CH(CH, (k_g2p_scatter<D, CH>), P.np, st, M, P, Nd, A, vu, skey, perm)
+                        const StepArgs& A, int vu, const Status* S, const int* skey,
+                        const int* perm, hipStream_t st) {
+#define L_(CH) MPMX_LAUNCH_CH(CH, (k_g2p_scatter<D, CH>), P.np, st, M, P, Nd, A, vu, S, skey, perm)
   MPMX_CHUNK_DISPATCH(chunk, L_);
 #undef L_
 }
 
 template <int D>
-void launch_node(int fl, const Mesh& M, const Nodes& Nd, const StepArgs& A, hipStream_t st) {
+void launch_node(int fl, const Mesh& M, const Nodes& Nd, const StepArgs& A, const Status* S,
+                 hipStream_t st) {
   switch (fl) {
 #define MPMX_NODE_CASE(F) \
-  case (F): MPMX_LAUNCH((k_node<D, (F)>), M.nnodes, st, M, Nd, A); break;
+  case (F): MPMX_LAUNCH((k_node<D, (F)>), M.nnodes, st, M, Nd, A, S); break;
     MPMX_NODE_CASE(ND_INIT | ND_VEL)
     MPMX_NODE_CASE(ND_ACC | ND_ZERO_ALL)
     MPMX_NODE_CASE(ND_INIT | ND_VEL | ND_ACC | ND_ZERO_ALL)
@@ -1486,11 +1524,20 @@ void launch_gather(int which, const Mesh& M, const Particles& P, const Nodes& Nd
   }
 }
 
+template <class T>
+void take_rows(const T* in, T* out, const int* idx, int64_t nnew, int64_t nold, int comps,
+               hipStream_t st) {
+  MPMX_LAUNCH((k_take_rows<T>), nnew, st, in, out, idx, nnew, nold, comps);
+}
+template void take_rows<double>(const double*, double*, const int*, int64_t, int64_t, int,
+                                hipStream_t);
+template void take_rows<int>(const int*, int*, const int*, int64_t, int64_t, int, hipStream_t);
+
 #define MPMX_INSTANTIATE(D)                                                                   \
   template void launch_locate_init<D>(const Mesh&, const Particles&, Status*, int*,          \
                                       const int*, hipStream_t);                              \
   template void launch_locate_scan<D>(const Mesh&, const Particles&, Status*, const int*,    \
-                                      const int*, hipStream_t);                              \
+                                      const int*, int, hipStream_t);                         \
   template void launch_p2g<D>(int, int, const Mesh&, const Particles&, const Nodes&,         \
                               const StepArgs&, const Status*, const int*, const int*,        \
                               hipStream_t);                                                  \
@@ -1504,15 +1551,16 @@ void launch_gather(int which, const Mesh& M, const Particles& P, const Nodes& Nd
                               const Material*, const StepArgs&, int, Status*, int*,          \
                               hipStream_t);                                                  \
   template void launch_g2p_scatter<D>(int, const Mesh&, const Particles&, const Nodes&,      \
-                                      const StepArgs&, int, const int*, const int*,          \
-                                      hipStream_t);                                          \
-  template void launch_node<D>(int, const Mesh&, const Nodes&, const StepArgs&, hipStream_t);\
+                                      const StepArgs&, int, const Status*, const int*,       \
+                                      const int*, hipStream_t);                              \
+  template void launch_node<D>(int, const Mesh&, const Nodes&, const StepArgs&,             \
+                               const Status*, hipStream_t);                                  \
   template void launch_cell_ranges<D>(int64_t, int64_t, const int*, int*, int*, hipStream_t);\
   template void launch_gather<D>(int, const Mesh&, const Particles&, const Nodes&,           \
                                  const StepArgs&, const Status*, const int*, const int*,     \
                                  const int*, hipStream_t);                                   \
   template void launch_traction_scatter<D>(const Mesh&, const Particles&, const Nodes&,      \
-                                           hipStream_t);
+                                           const Status*, hipStream_t);
 
 MPMX_INSTANTIATE(2)
 MPMX_INSTANTIATE(3)

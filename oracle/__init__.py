@@ -242,6 +242,9 @@ class Oracle:
         self.lib.orc_set_velocity_update(self.ctx, VELOCITY_UPDATES[velocity_update],
                                          C.c_double(blending_ratio))
 
+    def set_locate_particles(self, locate_particles=True):
+        self.lib.orc_set_locate_particles(self.ctx, 1 if locate_particles else 0)
+
     def set_gravity(self, g, ramping_time=0.0):
         g = _f64(g)
         self._check(self.lib.orc_set_gravity(self.ctx, _p(g), C.c_double(ramping_time)))

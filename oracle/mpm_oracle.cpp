@@ -834,6 +834,7 @@ struct Solver {
   std::vector<MathFn> functions;
   std::vector<TractionLoad> tractions;
   std::vector<Index> unlocatable;
+  bool locate_all = true;  // the analysis key locate_particles (mpm_base.tcc:53-54)
   std::string error;
 
   // ---------------- mesh -----------------
@@ -1003,13 +1004,51 @@ struct Solver {
   // mpm_scheme.tcc:245-261 -> mesh.tcc:1097-1112 (sequential sweep)
   void locate_particles() {
     unlocatable.clear();
-    for (auto& p : particles)
-      if (!locate_particle(p)) unlocatable.push_back(p.id);
-    if (!unlocatable.empty()) {
+    std::vector<Index> lost;  // vector positions
+    for (Index i = 0; i < (Index)particles.size(); ++i)
+      if (!locate_particle(particles[i])) {
+        unlocatable.push_back(particles[i].id);
+        lost.push_back(i);
+      }
+    if (!unlocatable.empty() && locate_all) {
       std::string msg = "Particle(s) outside the mesh domain: ";
       for (Index id : unlocatable) msg += std::to_string(id) + " ";
       throw std::runtime_error(msg);
     }
+    if (!lost.empty()) remove_particles(lost);
+  }
+
+  // Mesh::remove_particle (mesh.tcc:658-665): the particle leaves its cell
+  // (Particle::remove_cell) and the particle vector (std::remove keeps the
+  // order of the others, vector.tcc:23-30). Loads given by vector position
+  // follow their particles; those of removed particles are dropped.
+  void remove_particles(const std::vector<Index>& lost) {
+    const Index np = (Index)particles.size();
+    std::vector<Index> remap(np, 0);
+    std::vector<char> gone(np, 0);
+    for (Index i : lost) gone[i] = 1;
+    std::vector<Particle> kept;
+    kept.reserve(np - (Index)lost.size());
+    for (Index i = 0; i < np; ++i) {
+      if (gone[i]) {
+        if (particles[i].cell != kNoCell) cells[particles[i].cell].nparticles -= 1;
+        remap[i] = -1;
+        continue;
+      }
+      remap[i] = (Index)kept.size();
+      kept.push_back(particles[i]);
+    }
+    particles.swap(kept);
+    std::vector<PVel> pv;
+    for (const auto& c : pvel)
+      if (c.particle < 0 || (c.particle < np && remap[c.particle] >= 0))
+        pv.push_back({c.particle < 0 ? c.particle : remap[c.particle], c.dir, c.value});
+    pvel.swap(pv);
+    std::vector<TractionLoad> tl;
+    for (const auto& t : tractions)
+      if (t.particle < 0 || (t.particle < np && remap[t.particle] >= 0))
+        tl.push_back({t.particle < 0 ? t.particle : remap[t.particle], t.dir, t.value, t.fn});
+    tractions.swap(tl);
   }
 
   // ---------------- particles init -----------------
@@ -1748,6 +1787,11 @@ int orc_set_particle_velocity_constraints(orc_ctx* c, Index n, const Index* part
   })
 }
 
+int orc_set_locate_particles(orc_ctx* c, int locate_particles) {
+  c->s.locate_all = locate_particles != 0;
+  return 0;
+}
+
 int orc_set_velocity_update(orc_ctx* c, int velocity_update, double blending_ratio) {
   c->s.vu = velocity_update;
   c->s.blending = blending_ratio;
@@ -1808,8 +1852,15 @@ int orc_set_particles(orc_ctx* c, Index n, const double* x, const double* u, con
         double xi[3];
         orc::Particle& q = s.particles.back();
         if (cell[i] >= (Index)s.cells.size()) throw std::runtime_error("bad cell id");
-        // particle.tcc:386-416 assign_cell: keeps the cell only if the point is in it
-        if (s.is_point_in_cell(s.cells[cell[i]], q.x, xi)) s.assign_cell_xi(q, cell[i], xi);
+        // The cell id is held without a cell (Particle::assign_cell_id,
+        // particle.tcc:470-485); locate_particle_cells first tries to assign
+        // that cell (assign_cell, particle.tcc:386-416, succeeds only if the
+        // point is in it), else searches that cell's neighbours, then every
+        // cell (mesh.tcc:1118-1151).
+        if (s.is_point_in_cell(s.cells[cell[i]], q.x, xi))
+          s.assign_cell_xi(q, cell[i], xi);
+        else
+          q.hint = cell[i];
       }
     }
     s.locate_particles();

@@ -166,17 +166,18 @@ def test_fast_2d_full_run(scheme):
     print("fast %s 100-step errors: %s" % (scheme, {k: "%.2e" % v for k, v in e.items()}))
 
 
-def test_cell_crossing_gpu(kats):
-    c = kats["cell_crossing_2d"]
+@pytest.mark.parametrize("dim", [2, 3])
+def test_cell_crossing_gpu(kats, dim):
+    c = kats["cell_crossing_%dd" % dim]
     from mpm_amd.capi import Solver
     for ordered in (True, False):
-        g = Solver(2, "usf", ordered=ordered)
-        g.set_mesh(np.array(c["nodes"]), np.array(c["cells"]))
+        g = Solver(dim, "usf", ordered=ordered)
+        g.set_mesh(np.array(c["nodes"], dtype=float), np.array(c["cells"]))
         m = c["material"]
         g.set_materials([mat.linear_elastic(m["youngs_modulus"], m["poisson_ratio"],
                                             m["density"])])
         n = len(c["particles"])
-        vol = 0.25
+        vol = 1.0 / n
         g.set_particles(x=c["particles"], mass=[vol * m["density"]] * n, volume=[vol] * n,
                         density=[m["density"]] * n, material_id=0)
         assert list(g.particles()["cell"]) == c["cells_before"]
@@ -234,3 +235,73 @@ def test_sort_interval_changes_only_summation_order():
         assert np.array_equal(pa["cell"], pb["cell"])
         for f in FIELDS:
             assert _relerr(pa[f], pb[f]) <= 1e-10, f
+
+
+def _lost_pair(ordered, sort_every=None, locate=True):
+    from mpm_amd.capi import Solver
+    import lost_particle_deck as deck
+    g = deck.setup(Solver(2, "usf", ordered=ordered))
+    o = deck.setup(oracle.Oracle(2, "usf"))
+    if sort_every:
+        g.set_sort_interval(sort_every)
+    g.set_locate_particles(locate)
+    o.set_locate_particles(locate)
+    return g, o, deck.DT
+
+
+def _oracle_failing_step(o, dt, limit=200):
+    for s in range(limit):
+        try:
+            o.step(1, dt)
+        except oracle.OracleError:
+            return s
+    raise AssertionError("particle never left the mesh")
+
+
+@pytest.mark.parametrize("ordered,sort_every", [(True, None), (False, 4), (False, 3)])
+def test_unlocatable_stops_at_the_failing_step(ordered, sort_every):
+    """A particle leaves the mesh between full sorts while the others keep
+    moving: the step raises, the state returned is that of the end of the
+    failing step (later queued steps do nothing) and every other particle
+    matches the oracle's state at its throw."""
+    from mpm_amd.capi import UnlocatableParticle
+    g, o, dt = _lost_pair(ordered, sort_every)
+    k = _oracle_failing_step(o, dt)
+    assert k > 4 and (sort_every is None or k % sort_every != 0), k
+    with pytest.raises(UnlocatableParticle) as e:
+        g.step(k + 25, dt)
+    assert "step %d" % k in str(e.value)
+    assert g.step_count == k + 1
+    pg, po = g.particles(), o.particles()
+    assert pg["cell"][0] == -1
+    assert np.array_equal(pg["cell"][1:], po["cell"][1:])
+    for f in FIELDS:
+        a, b = pg[f][1:], po[f][1:]
+        if ordered:
+            assert np.array_equal(a, b), f
+        else:
+            assert _relerr(a, b) <= 1e-10, (f, _relerr(a, b))
+
+
+@pytest.mark.parametrize("ordered", [True, False])
+def test_locate_particles_false_removes_the_lost_particle(ordered):
+    """locate_particles: false: the particle is removed at its failing step
+    and the run goes on; the others keep their order, ids and the particle
+    velocity constraint of the last particle follow the particles, and the
+    state matches the oracle's removal."""
+    g, o, dt = _lost_pair(ordered, None if ordered else 4, locate=False)
+    n0 = g.nparticles
+    ids0 = g.particle_ids()
+    assert list(ids0) == list(range(n0))
+    g.step(60, dt)
+    o.step(60, dt)
+    assert g.nparticles == n0 - 1 == o.nparticles
+    assert list(g.particle_ids()) == list(range(1, n0))
+    pg, po = g.particles(), o.particles()
+    assert np.array_equal(pg["cell"], po["cell"])
+    assert pg["v"][-1, 0] == 1e-3
+    for f in FIELDS:
+        if ordered:
+            assert np.array_equal(pg[f], po[f]), f
+        else:
+            assert _relerr(pg[f], po[f]) <= 1e-10, (f, _relerr(pg[f], po[f]))
