@@ -6,6 +6,13 @@ phi=30 psi=0 c=0, MUSL, FLIP, dt=5e-5, frictional base. A "step" is one full
 explicit step (binning, P2G, node update, G2P, stress, locate) over all
 particles, with the particle state resident in HBM.
 
+The timed window starts after a fixed pre-roll of simulated steps
+(--preroll, default 1000 on the collapse: half of its 2000-step run, with the
+Mohr-Coulomb return mappings in full swing) plus --warmup steps, whatever
+--steps and --warmup are; `config.window` names the simulated steps timed.
+The whole run (2000 steps from the initial state) is also timed, and its end
+state compared with an ordered-mode run of the same deck (`full_run`).
+
 Prints one JSON line: value = particle-updates/s = particles * steps / time.
 `roofline` is the dominant kernel's algorithmic HBM bytes per launch over its
 average launch time (HIP events on the library's stream); `step_roofline` is
@@ -14,8 +21,10 @@ the oracle's OpenMP build (the reference algorithm restated in C++) on the
 host's cores over a bounded number of steps of the same workload.
 """
 import argparse
+import ctypes
 import json
 import os
+import platform
 import sys
 import time
 
@@ -54,6 +63,86 @@ STEP_BYTES_PER_PARTICLE = {(3, "mc"): 576, (3, "le"): 528, (2, "mc"): 368, (2, "
 STEP_BYTES_PER_NODE = {3: 256, 2: 176}
 
 
+class _Roctx:
+    """roctxProfilerResume/Pause around the timed window, so that
+    `rocprofv3 --selected-regions` traces exactly the steps `value` is
+    measured on. No-ops without the ROCm profiler SDK or outside a profiler."""
+
+    def __init__(self):
+        self.lib = None
+        for name in ("librocprofiler-sdk-roctx.so.1", "librocprofiler-sdk-roctx.so"):
+            try:
+                self.lib = ctypes.CDLL(name)
+                break
+            except OSError:
+                continue
+        if self.lib is not None:
+            self.lib.roctxProfilerPause.argtypes = [ctypes.c_uint64]
+            self.lib.roctxProfilerResume.argtypes = [ctypes.c_uint64]
+
+    def pause(self):
+        if self.lib is not None:
+            self.lib.roctxProfilerPause(0)
+
+    def resume(self):
+        if self.lib is not None:
+            self.lib.roctxProfilerResume(0)
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def rel_dev(a, b):
+    scale = float(np.max(np.abs(b))) if b.size else 0.0
+    return float(np.max(np.abs(a - b))) / scale if scale > 0 else float(np.max(np.abs(a)))
+
+
+def full_run(w, args, tuned):
+    """The workload's whole run from its initial state, timed (graph
+    replays, fast mode), and its end state against an ordered-mode run of the
+    same deck (nodal sums in particle order, bitwise equal to the oracle)."""
+    from mpm_amd.capi import Solver
+    s = w.setup(Solver(w.dim, w.scheme, ordered=args.ordered))
+    if tuned:
+        s.set_chunk(tuned["chunk"])
+        s.set_sort_interval(tuned["sort_every"])
+    s.synchronize()
+    t0 = time.perf_counter()
+    s.step(w.nsteps, w.dt)
+    s.synchronize()
+    wall = time.perf_counter() - t0
+    out = {"steps": w.nsteps, "wall_s": wall, "ms_per_step": 1e3 * wall / w.nsteps,
+           "particle_updates_per_s": s.nparticles * w.nsteps / wall}
+    if args.no_accuracy or args.ordered:
+        s.close()
+        return out
+    pf = s.particles(["x", "v", "stress", "cell", "statevars"])
+    s.close()
+    o = w.setup(Solver(w.dim, w.scheme, ordered=True))
+    t0 = time.perf_counter()
+    o.step(w.nsteps, w.dt)
+    o.synchronize()
+    ordered_wall = time.perf_counter() - t0
+    po = o.particles(["x", "v", "stress", "cell", "statevars"])
+    o.close()
+    dev = {f: rel_dev(pf[f], po[f]) for f in ("x", "v", "stress")}
+    dev["cells_differing"] = int(np.sum(pf["cell"] != po["cell"]))
+    if pf.get("statevars") is not None and po.get("statevars") is not None:
+        dev["yield_states_differing"] = int(np.sum(pf["statevars"][:, 0] != po["statevars"][:, 0]))
+    out["vs_ordered"] = dict(what="end of the run, fast mode against ordered mode (particle-order "
+                                  "nodal sums), max |a - b| / max |b| per field", **dev)
+    out["ordered_ms_per_step"] = 1e3 * ordered_wall / w.nsteps
+    return out
+
+
 def active_nodes(w, cells_of_particles):
     occ = np.unique(cells_of_particles[cells_of_particles >= 0])
     return int(np.unique(w.cells[occ].ravel()).size)
@@ -71,7 +160,7 @@ def cpu_baseline(w, budget_s):
     o.step(n, w.dt)
     el = time.perf_counter() - t0
     return dict(value=o.nparticles * n / el, unit="particle-updates/s", cores=o.threads,
-                kind="port",
+                kind="port", cpu_model=cpu_model(),
                 sample="%d steps (after 1 warm-up step) of the same %d-particle workload, "
                        "oracle OpenMP build, %d threads, %.1f s" % (n, o.nparticles, o.threads, el))
 
@@ -93,6 +182,7 @@ def cpu_baseline_batch(b, budget_s):
         if el > budget_s:
             break
     return dict(value=done / el, unit="particle-updates/s", cores=threads, kind="port",
+                cpu_model=cpu_model(),
                 sample="10 steps (after 1 warm-up step) of each of the first %d instances, "
                        "one at a time, oracle OpenMP build, %d threads, %.1f s"
                        % (ninst, threads, el))
@@ -237,6 +327,13 @@ def main():
     ap.add_argument("--steps", type=int, default=None,
                     help="timed steps (default 200; adjoint_2d: its 500-step run)")
     ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--preroll", type=int, default=None,
+                    help="simulated steps run before the warm-up (default: half of the "
+                         "workload's run on the collapse, 0 otherwise)")
+    ap.add_argument("--no-full-run", action="store_true",
+                    help="skip the whole-run timing and its ordered-mode comparison")
+    ap.add_argument("--no-accuracy", action="store_true",
+                    help="time the whole run but skip the ordered-mode comparison")
     ap.add_argument("--workload", default="collapse_3d",
                     choices=["collapse_3d", "wedge_3d", "column_2d", "batch_2d", "adjoint_2d"])
     ap.add_argument("--sequential-steps", type=int, default=20,
@@ -262,7 +359,11 @@ def main():
     from mpm_amd import workloads
     from mpm_amd.capi import Solver
 
+    roctx = _Roctx()
+    roctx.pause()
     w = getattr(workloads, args.workload)()
+    if args.preroll is None:
+        args.preroll = w.nsteps // 2 if args.workload == "collapse_3d" else 0
     if args.workload == "adjoint_2d":
         print(json.dumps(bench_adjoint(w, args)))
         return
@@ -281,12 +382,14 @@ def main():
         tuned = None
     else:
         tuned = s.autotune(w.dt, trial_steps=10)
-    s.step(args.warmup, w.dt)
+    s.step(args.preroll + args.warmup, w.dt)
     s.synchronize()
+    roctx.resume()
     t0 = time.perf_counter()
     s.step(args.steps, w.dt)
     s.synchronize()
     elapsed = time.perf_counter() - t0
+    roctx.pause()
     ms_per_step = 1e3 * elapsed / args.steps
     value = npart * args.steps / elapsed
 
@@ -301,7 +404,7 @@ def main():
     if tuned:
         s.set_chunk(tuned["chunk"])
         s.set_sort_interval(tuned["sort_every"])
-    s.step(args.warmup, w.dt)
+    s.step(args.preroll + args.warmup, w.dt)
     s.profile(True)
     s.step(args.steps, w.dt)
     kt = s.kernel_times()
@@ -346,7 +449,9 @@ def main():
         "data": "synthetic",
         "config": {"workload": w.name, "particles": npart, "scheme": w.scheme.upper(),
                    "mode": "ordered" if args.ordered else "fast", **w.info,
-                   "active_nodes": nact, "dt": w.dt},
+                   "active_nodes": nact, "dt": w.dt,
+                   "window": {"first_step": args.preroll + args.warmup, "steps": args.steps,
+                              "preroll": args.preroll, "run_steps": w.nsteps}},
         "roofline": {"bound": "hbm", "kernel": "+".join(parts), "achieved": achieved,
                      "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
@@ -366,6 +471,8 @@ def main():
             "ms_per_step": 1e3 * seq, "value": npart / seq,
             "batched_speedup": seq / (ms_per_step * 1e-3),
             "steps_per_instance": args.sequential_steps}
+    if not args.no_full_run and not hasattr(w, "instances"):
+        out["full_run"] = full_run(w, args, tuned)
     if not args.no_cpu:
         out["cpu_baseline"] = (cpu_baseline_batch(w, args.cpu_budget) if hasattr(w, "instances")
                                else cpu_baseline(w, args.cpu_budget))

@@ -221,6 +221,14 @@ int mpmx_compute_volume_mass(mpmx_ctx* ctx);
  * velocities that the reference receives through its HDF5 resume path. */
 int mpmx_assign_velocity(mpmx_ctx* ctx, int64_t n, const double* v);
 
+/* Material of the listed particles (positions in the particle order) becomes
+ * material_id, their state variables that material's initial ones
+ * (Particle::assign_material, particle.tcc:499-517), as the deck's
+ * "material_sets" does for a particle set (mpm_base.tcc:449-468). Call before
+ * mpmx_compute_volume_mass, which takes the density of each particle's
+ * material. */
+int mpmx_assign_materials(mpmx_ctx* ctx, int64_t n, const int64_t* particle, int material_id);
+
 /* Assign every particle's stress (n rows of 6 Voigt components, reference
  * particle order). Replaces Mesh::assign_particles_stresses
  * (mesh.tcc:1502-1523), the "particles_stresses" input (mpm_base.tcc:1595-1632). */

@@ -69,14 +69,43 @@ def build(verbose=False, jobs=4):
     return LIB
 
 
+POD_LIB = os.path.join(HERE, "libmpmx_pod.so")
+
+
+def hdf5_dir():
+    """Prefix of the HDF5 C and high-level libraries (headers in include/,
+    libhdf5 and libhdf5_hl in lib/): $MPMX_HDF5_DIR, else the image's
+    /opt/conda."""
+    for cand in (os.environ.get("MPMX_HDF5_DIR"), "/opt/conda", "/usr"):
+        if cand and os.path.exists(os.path.join(cand, "include", "hdf5_hl.h")):
+            return cand
+    raise RuntimeError("HDF5 high-level headers not found (set MPMX_HDF5_DIR)")
+
+
+def build_pod():
+    """The particle checkpoint table library (mpm_amd/libmpmx_pod.so, host
+    only, over HDF5's high-level table API)."""
+    h5 = hdf5_dir()
+    cxx = os.environ.get("CXX", "g++")
+    _run([cxx, "-O2", "-std=c++17", "-Wall", "-shared", "-fPIC",
+          "-I", os.path.join(HERE, "..", "include"), "-I", os.path.join(h5, "include"),
+          os.path.join(HERE, "host", "mpmx_pod.cpp"), "-o", POD_LIB + ".tmp",
+          "-L", os.path.join(h5, "lib"), "-lhdf5_hl", "-lhdf5",
+          "-Wl,-rpath," + os.path.join(h5, "lib")])
+    os.replace(POD_LIB + ".tmp", POD_LIB)
+    return POD_LIB
+
+
 def build_driver():
-    """The C++ analysis driver (mpm_amd/mpmx_run) over the C ABI."""
+    """The C++ analysis driver (mpm_amd/mpmx_run) over the C ABI, writing and
+    resuming from the particle checkpoint table."""
+    build_pod()
     out = os.path.join(HERE, HOST_TOOL)
     rocm = os.path.dirname(os.path.dirname(os.path.realpath(hipcc())))
     cxx = os.environ.get("CXX", "g++")
     _run([cxx, "-O2", "-std=c++17", "-Wall", "-I", os.path.join(HERE, "..", "include"),
           os.path.join(HERE, "host", "mpmx_run.cpp"), "-o", out + ".tmp",
-          "-L", HERE, "-lmpmx", "-Wl,-rpath,$ORIGIN",
+          "-L", HERE, "-lmpmx", "-lmpmx_pod", "-Wl,-rpath,$ORIGIN",
           "-Wl,-rpath-link," + os.path.join(rocm, "lib")])
     os.replace(out + ".tmp", out)
     return out

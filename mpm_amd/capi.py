@@ -30,7 +30,7 @@ EXPORTS = (
     "mpmx_set_friction_constraints", "mpmx_set_particle_velocity_constraints",
     "mpmx_set_gravity", "mpmx_generate_gauss",
     "mpmx_set_particles", "mpmx_compute_volume_mass", "mpmx_assign_velocity",
-    "mpmx_assign_stresses", "mpmx_nparticles", "mpmx_get_particles", "mpmx_get_nodes",
+    "mpmx_assign_stresses", "mpmx_assign_materials", "mpmx_nparticles", "mpmx_get_particles", "mpmx_get_nodes",
     "mpmx_step", "mpmx_set_locate_particles", "mpmx_get_particle_ids", "mpmx_set_particle_ids",
     "mpmx_set_sort_interval", "mpmx_set_chunk", "mpmx_autotune", "mpmx_set_step",
     "mpmx_get_step",
@@ -85,6 +85,7 @@ def load(path=LIB_PATH):
         "mpmx_compute_volume_mass": [V],
         "mpmx_assign_velocity": [V, C.c_int64, _D],
         "mpmx_assign_stresses": [V, C.c_int64, _D],
+        "mpmx_assign_materials": [V, C.c_int64, _I64, C.c_int],
         "mpmx_nparticles": [V, _I64],
         "mpmx_get_particles": [V, _D, _D, _D, _D, _D, _D, _D, _D, _D, _I32, _I64, _D, _D],
         "mpmx_get_nodes": [V, _D, _D, _D, _I32],
@@ -199,6 +200,11 @@ class Solver:
     def set_materials(self, params):
         params = _f64(params).reshape(-1, 16)
         self._check(self.lib.mpmx_set_materials(self.ctx, params.shape[0], _p(params)))
+        self.material_params = params.copy()
+
+    def material_types(self):
+        """Per material id: 0 LinearElastic, 1 MohrCoulomb (parameter 0)."""
+        return self.material_params[:, 0].astype(int)
 
     def _cons(self, fn, nodes, dirs, *vals):
         nodes = np.ascontiguousarray(nodes, dtype=np.int64)
@@ -289,6 +295,13 @@ class Solver:
     def assign_velocity(self, v):
         v = _f64(v, (-1, self.dim))
         self._check(self.lib.mpmx_assign_velocity(self.ctx, v.shape[0], _p(v)))
+
+    def assign_materials(self, particles, material_id):
+        """material_sets: the listed particles take material material_id and
+        its initial state variables."""
+        idx = np.ascontiguousarray(particles, dtype=np.int64)
+        self._check(self.lib.mpmx_assign_materials(self.ctx, len(idx), _p(idx, _I64),
+                                                   int(material_id)))
 
     def assign_stresses(self, stress):
         stress = _f64(stress, (-1, 6))

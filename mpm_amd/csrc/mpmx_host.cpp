@@ -862,12 +862,45 @@ void truncate_particles(mpmx_ctx* c, int64_t keep) {
 }
 
 // ------------------------------ step sequence --------------------------------
+// The device scratch of the step kernels (what they use beyond the mesh,
+// node and particle state): the context's own buffers.
+struct StepBuffers {
+  Status* S;
+  int* skey;
+  int* perm;
+  int* sort_tmp;
+  int* scan;
+  int* cstart;
+  int* cend;
+  int* plastic_list;
+  double* plastic_rec;
+  double* spin;
+  double* fnval;  // the math functions' values at the step's time
+};
+
+StepBuffers ctx_buffers(mpmx_ctx* c) {
+  StepBuffers B;
+  B.S = c->d_status.p;
+  B.skey = c->d_skey.p;
+  B.perm = c->d_perm.p;
+  B.sort_tmp = c->d_sort_tmp.p;
+  B.scan = c->d_scan_list.p;
+  B.cstart = c->d_cstart.p;
+  B.cend = c->d_cend.p;
+  B.plastic_list = c->d_plastic_list.p;
+  B.plastic_rec = c->d_plastic_rec.p;
+  B.spin = c->d_spin.p;
+  B.fnval = c->d_fnval.p;
+  return B;
+}
+
 struct Launcher {
   mpmx_ctx* c;
+  bool timed = true;  // events around each launch while profiling is enabled
   std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> evs;
   template <class F>
   void operator()(const char* name, F&& f) {
-    if (!c->profile) {
+    if (!c->profile || !timed) {
       f();
       return;
     }
@@ -900,20 +933,27 @@ struct Launcher {
 };
 
 template <int D>
-void enqueue_step(mpmx_ctx* c, const StepArgs& A, Launcher& L, bool full_sort) {
+void enqueue_step(mpmx_ctx* c, const StepArgs& A, Launcher& L, bool full_sort,
+                  const StepBuffers& B) {
   const Mesh M = c->mesh();
-  const Particles P = c->parts();
-  const Nodes Nd = c->nodes();
+  Particles P = c->parts();
+  if (P.plastic_list) {
+    P.plastic_list = B.plastic_list;
+    P.plastic_rec = B.plastic_rec;
+  }
+  if (P.spin) P.spin = B.spin;
+  Nodes Nd = c->nodes();
+  if (Nd.fnval) Nd.fnval = B.fnval;
   hipStream_t st = c->stream;
-  Status* S = c->d_status.p;
+  Status* S = B.S;
   const Material* mats = c->d_mats.p;
-  int* skey = c->d_skey.p;
-  int* perm = c->d_perm.p;
-  int* scan = c->d_scan_list.p;
+  int* skey = B.skey;
+  int* perm = B.perm;
+  int* scan = B.scan;
   const bool ordered = (c->flags & MPMX_ORDERED) != 0;
   if (!c->fns.empty())
     L("time_functions", [&] {
-      launch_fn_eval(c->d_fns.p, c->d_fn_data.p, (int)c->fns.size(), S, A.dt, c->d_fnval.p, st);
+      launch_fn_eval(c->d_fns.p, c->d_fn_data.p, (int)c->fns.size(), S, A.dt, B.fnval, st);
     });
   // the Mohr-Coulomb return mappings deferred by the stress kernels
   auto plastic = [&] {
@@ -921,16 +961,16 @@ void enqueue_step(mpmx_ctx* c, const StepArgs& A, Launcher& L, bool full_sort) {
   };
   if (full_sort || ordered)
     L("sort_by_cell", [&] {
-      radix_sort_by_cell(P.cell, P.np, (int)M.ncells, skey, perm, c->d_sort_tmp.p, st);
+      radix_sort_by_cell(P.cell, P.np, (int)M.ncells, skey, perm, B.sort_tmp, st);
     });
   else
     L("refresh_keys", [&] { refresh_sorted_keys(P.cell, perm, P.np, (int)M.ncells, skey, st); });
   if (ordered) {
     L("cell_ranges", [&] {
-      launch_cell_ranges<D>(P.np, M.ncells, skey, c->d_cstart.p, c->d_cend.p, st);
+      launch_cell_ranges<D>(P.np, M.ncells, skey, B.cstart, B.cend, st);
     });
-    const int* cs = c->d_cstart.p;
-    const int* ce = c->d_cend.p;
+    const int* cs = B.cstart;
+    const int* ce = B.cend;
     if (c->scheme == USF) {
       L("gather_mass_momentum", [&] { launch_gather<D>(GK_MV_VEL, M, P, Nd, A, S, perm, cs, ce, st); });
       L("stress", [&] { launch_stress<D>(false, M, P, Nd, mats, A, S, scan, st); });
@@ -976,11 +1016,12 @@ void enqueue_step(mpmx_ctx* c, const StepArgs& A, Launcher& L, bool full_sort) {
   L("step_end", [&] { launch_step_end(S, st); });
 }
 
-void enqueue(mpmx_ctx* c, const StepArgs& A, Launcher& L, bool full_sort) {
+void enqueue(mpmx_ctx* c, const StepArgs& A, Launcher& L, bool full_sort,
+             const StepBuffers& B) {
   if (c->dim == 2)
-    enqueue_step<2>(c, A, L, full_sort);
+    enqueue_step<2>(c, A, L, full_sort, B);
   else
-    enqueue_step<3>(c, A, L, full_sort);
+    enqueue_step<3>(c, A, L, full_sort, B);
 }
 
 // Whether the next step sorts fully (ordered mode needs exact cell order
@@ -996,8 +1037,23 @@ bool next_step_sorts(mpmx_ctx* c) {
 // Queue nsteps steps on the context's stream (one graph replay each).
 // ad_nodes: record each step's nodal quantities in the slots of length
 // ad_stride there, step Status::ad_slot in its slot.
+StepArgs step_args(const mpmx_ctx* c, double dt, double damping) {
+  StepArgs A;
+  A.dt = dt;
+  A.damping = damping;
+  A.blending = c->blending;
+  for (int k = 0; k < 3; ++k) A.g[k] = c->g[k];
+  A.ramp = c->ramp;
+  A.ad_nodes = nullptr;
+  A.ad_slot = nullptr;
+  A.ad_stride = 0;
+  A.ad_nslots = 0;
+  return A;
+}
+
 void launch_steps(mpmx_ctx* c, int nsteps, double dt, double damping,
                   double* ad_nodes = nullptr, int64_t ad_stride = 0, int ad_nslots = 0) {
+  const StepBuffers B = ctx_buffers(c);
   StepArgs A;
   A.dt = dt;
   A.damping = damping;
@@ -1022,7 +1078,7 @@ void launch_steps(mpmx_ctx* c, int nsteps, double dt, double damping,
       if (!c->graph_exec[kind]) {
         hipGraph_t graph;
         HIPCHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        enqueue(c, A, L, (kind & 1) == 1);
+        enqueue(c, A, L, (kind & 1) == 1, B);
         HIPCHECK(hipStreamEndCapture(c->stream, &graph));
         hipError_t e = hipGraphInstantiate(&c->graph_exec[kind], graph, nullptr, nullptr, 0);
         (void)hipGraphDestroy(graph);
@@ -1032,7 +1088,7 @@ void launch_steps(mpmx_ctx* c, int nsteps, double dt, double damping,
     }
   } else {
     for (int s = 0; s < nsteps; ++s) {
-      enqueue(c, A, L, next_step_sorts(c));
+      enqueue(c, A, L, next_step_sorts(c), B);
       HIPCHECK(hipGetLastError());
       L.collect();
     }
@@ -1817,6 +1873,36 @@ int mpmx_compute_volume_mass(mpmx_ctx* c) {
   HIPCHECK(hipStreamSynchronize(c->stream));
   for (int64_t i = 0; i < n; ++i) c->psize[i] = particle_size(vol[i], c->dim);
   build_trac(c);
+  API_END(c)
+}
+
+int mpmx_assign_materials(mpmx_ctx* c, int64_t n, const int64_t* particle, int material_id) {
+  API_BEGIN(c)
+  if (n < 0 || (n > 0 && !particle)) throw ApiError(MPMX_E_ARG, "bad particle list");
+  if (material_id < 0 || material_id >= (int)c->mats.size())
+    throw ApiError(MPMX_E_ARG, "bad material id");
+  const int64_t np = c->np;
+  if (n == 0 || np == 0) return MPMX_OK;
+  std::vector<int> mid(np);
+  c->d_mat.download(mid.data(), np, c->stream);
+  std::vector<double> sv;
+  if (c->any_mc) {
+    sv.resize((size_t)NSV * np);
+    c->d_sv.download(sv.data(), (size_t)NSV * np, c->stream);
+  }
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  const Material& m = c->mats[material_id];
+  for (int64_t i = 0; i < n; ++i) {
+    const int64_t p = particle[i];
+    if (p < 0 || p >= np) throw ApiError(MPMX_E_ARG, "particle index out of range");
+    mid[p] = material_id;
+    if (c->any_mc)
+      for (int k = 0; k < NSV; ++k) sv[(size_t)k * np + p] = m.type == MC ? m.sv0[k] : 0.0;
+  }
+  c->d_mat.upload(mid.data(), np, c->stream);
+  if (c->any_mc) c->d_sv.upload(sv.data(), (size_t)NSV * np, c->stream);
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  c->ad_recorded = false;
   API_END(c)
 }
 

@@ -5,7 +5,8 @@
 // include/io/io_mesh_ascii.tcc, include/solvers/mpm_base.tcc) -- and runs the
 // analysis types MPMExplicit2D / MPMExplicit3D (mpm_explicit.tcc:24-199) with
 // the time step on the GPU through include/mpmx.h. Particle states are
-// written as text at the deck's output_steps (the reference writes HDF5/VTK).
+// written at the deck's output_steps as the reference's HDF5 particle table
+// (include/mpmx_pod.h), from which an analysis with "resume" restarts.
 //
 //   mpmx_run -f <working directory> [-i <input file, default mpm.json>]
 //            [--ordered] [--dry-run]
@@ -16,6 +17,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstring>
 #include <fstream>
 #include <functional>
 #include <iostream>
@@ -29,6 +31,7 @@
 #include <vector>
 
 #include "../../include/mpmx.h"
+#include "../../include/mpmx_pod.h"
 #include "json.h"
 
 namespace mpm {
@@ -243,13 +246,27 @@ class MPMExplicit {
   std::vector<int> fn_type_, fn_npts_;
   std::vector<double> fn_params_;
   int64_t expected_particles_ = 0;
+  // checkpoint resume (analysis "resume": {resume, uuid, step}) and
+  // locate_particles (mpm_base.tcc:53-54)
+  bool resume_ = false, locate_particles_ = true;
+  int64_t step_ = 0;
+  // natural particle size by particle id (Particle::assign_volume,
+  // particle.tcc:556-583), written to the particle table
+  std::map<int64_t, double> nsize_;
+  // particle_cells (mpm_base.tcc:1495-1520): particle id -> cell id
+  std::map<int64_t, int64_t> particle_cells_;
 
   void read_analysis();
   void initialise_materials();
   void initialise_math_functions();
   void initialise_mesh();
   void initialise_particles();
+  void resume_particles();
+  void particle_loads();
   void initialise_loads();
+  std::string output_file(const std::string& attribute, const std::string& ext,
+                          int64_t step) const;
+  std::vector<int64_t> ids_to_indices(const std::vector<int64_t>& ids) const;
   void write_outputs(int64_t step);
   void summary() const;
   int material_index(const Json& id) const {
@@ -286,9 +303,15 @@ void MPMExplicit<Tdim>::read_analysis() {
     else if (t != "None") throw std::runtime_error("damping type " + t + " is not supported");
   }
   if (a.contains("uuid")) uuid_ = a.at("uuid").str();
+  if (a.contains("locate_particles")) locate_particles_ = a.at("locate_particles").boolean();
+  // checkpoint resume (mpm_explicit.tcc:43-46, mpm_base.tcc:509-551): the
+  // particle table of step "step" of analysis "uuid"
   if (a.contains("resume") && a.at("resume").contains("resume") &&
-      a.at("resume").at("resume").is_boolean() && a.at("resume").at("resume").boolean())
-    throw std::runtime_error("resume from an HDF5 checkpoint is not supported");
+      a.at("resume").at("resume").boolean()) {
+    resume_ = true;
+    uuid_ = a.at("resume").at("uuid").str();
+    step_ = a.at("resume").at("step").integer();
+  }
   if (io_->json().contains("post_processing")) {
     const Json& pp = io_->object("post_processing");
     output_steps_ = pp.contains("output_steps") ? pp.at("output_steps").integer() : nsteps_;
@@ -451,6 +474,10 @@ template <unsigned Tdim>
 void MPMExplicit<Tdim>::initialise_loads() {
   if (!io_->json().contains("external_loading_conditions")) return;
   const Json& l = io_->object("external_loading_conditions");
+  // rotation_forces (centrifugal and Coriolis body forces, mpm_base.tcc:
+  // 845-870, Particle::map_rotation_force) are not part of this build's step
+  if (l.contains("rotation_forces"))
+    throw std::runtime_error("rotation_forces are not supported by this build");
   // concentrated_nodal_forces (mpm_base.tcc:913-953): from a file (rows node
   // dir force, io_mesh_ascii.tcc:752-795, no function) or per node set (-1:
   // every node) with an optional math_function_id
@@ -501,6 +528,19 @@ void MPMExplicit<Tdim>::initialise_loads() {
 template <unsigned Tdim>
 void MPMExplicit<Tdim>::initialise_particles() {
   const Json& mp = io_->object("mesh");
+  // particle_cells: "pid cid" rows (io_mesh_ascii.tcc:442-480); they give the
+  // cell a particle without a cell is located from first (assign_cell_id,
+  // mesh.tcc:1528-1548), i.e. particles read from a file; a generated
+  // particle already has its cell and the assignment fails
+  if (mp.contains("particle_cells")) {
+    const std::string f = io_->file_name(mp.at("particle_cells").str());
+    if (!f.empty())
+      for (const auto& r : read_rows(f)) {
+        if (r.size() < 2) throw std::runtime_error("particle_cells: two columns per row");
+        particle_cells_[(int64_t)r[0]] = (int64_t)r[1];
+      }
+  }
+  int64_t created = 0;
   for (const Json& pj : io_->object("particles").elements()) {
     const Json& g = pj.at("generator");
     const std::string type = g.at("type").str();
@@ -515,7 +555,12 @@ void MPMExplicit<Tdim>::initialise_particles() {
         cells = it->second;
       }
       const int64_t nc = cset >= 0 ? (int64_t)cells.size() : (int64_t)(cells_.size() / NN);
-      expected_particles_ += nc * (int64_t)std::pow(nppd, (int)Tdim);
+      const int64_t ng = nc * (int64_t)std::pow(nppd, (int)Tdim);
+      expected_particles_ += ng;
+      if (particle_cells_.lower_bound(created) != particle_cells_.end() &&
+          particle_cells_.lower_bound(created)->first < created + ng)
+        throw std::runtime_error("Particle cells are not properly assigned to particles");
+      created += ng;
       if (ctx_)
         check(ctx_, mpmx_generate_gauss(ctx_, nppd, cset >= 0 ? (int64_t)cells.size() : -1,
                                         cset >= 0 ? cells.data() : nullptr, mid),
@@ -524,18 +569,105 @@ void MPMExplicit<Tdim>::initialise_particles() {
       const auto x = read_particles(io_->file_name(g.at("location").str()), Tdim);
       const int64_t n = (int64_t)(x.size() / Tdim);
       expected_particles_ += n;
+      std::vector<int64_t> hint(n, -1);
+      for (int64_t i = 0; i < n; ++i) {
+        const auto it = particle_cells_.find(created + i);
+        if (it != particle_cells_.end()) hint[i] = it->second;
+      }
+      created += n;
       if (ctx_) {
         const std::vector<double> one(n, 1.0);
         const std::vector<int> mat(n, mid);
         check(ctx_, mpmx_set_particles(ctx_, n, x.data(), nullptr, nullptr, nullptr, nullptr,
                                        nullptr, one.data(), one.data(), one.data(), mat.data(),
-                                       nullptr, nullptr),
+                                       hint.data(), nullptr),
               "particles from file");
       }
     } else {
       throw std::runtime_error("particle generator " + type + " is not supported (gauss, file)");
     }
   }
+  // material_sets (mpm_base.tcc:449-468): the particles of a set take the
+  // material (and its initial state variables), before the mass is computed
+  if (ctx_ && io_->json().contains("material_sets")) {
+    for (const Json& ms : io_->object("material_sets").elements()) {
+      const int mid = material_index(ms.at("material_id"));
+      const int64_t pset = ms.at("pset_id").integer();
+      std::vector<int64_t> idx;
+      if (pset == -1) {
+        int64_t n = 0;
+        check(ctx_, mpmx_nparticles(ctx_, &n), "particle count");
+        for (int64_t i = 0; i < n; ++i) idx.push_back(i);
+      } else {
+        auto it = particle_sets_.find(pset);
+        if (it == particle_sets_.end()) throw std::runtime_error("material set: particle set not defined");
+        idx = ids_to_indices(it->second);
+      }
+      check(ctx_, mpmx_assign_materials(ctx_, (int64_t)idx.size(), idx.data(), mid),
+            "material sets");
+    }
+  }
+  if (ctx_) {
+    check(ctx_, mpmx_compute_volume_mass(ctx_), "particle volume and mass");
+    // natural size: unit element length 2 over nparticles_in_cell^(1/dim)
+    // when the volume was assigned (particle.tcc:570-580)
+    int64_t n = 0;
+    check(ctx_, mpmx_nparticles(ctx_, &n), "particle count");
+    std::vector<int64_t> cell(n), ids(n);
+    check(ctx_, mpmx_get_particles(ctx_, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                                   nullptr, nullptr, nullptr, nullptr, cell.data(), nullptr,
+                                   nullptr),
+          "particle cells");
+    check(ctx_, mpmx_get_particle_ids(ctx_, ids.data()), "particle ids");
+    std::map<int64_t, int64_t> count;
+    for (int64_t c : cell) count[c] += 1;
+    for (int64_t i = 0; i < n; ++i)
+      nsize_[ids[i]] = 2.0 / std::pow((double)count[cell[i]], 1.0 / Tdim);
+  }
+  particle_loads();
+  if (ctx_ && mp.contains("particles_stresses")) {
+    const Json& ps = mp.at("particles_stresses");
+    const std::string path = io_->file_name(ps.at("location").str());
+    const auto rows = read_rows(path);
+    const int64_t n = (int64_t)rows.at(0).at(0);
+    std::vector<double> s(n * 6);
+    for (int64_t i = 0; i < n; ++i)
+      for (int k = 0; k < 6; ++k) s[i * 6 + k] = rows.at(1 + i).at(k);
+    check(ctx_, mpmx_assign_stresses(ctx_, n, s.data()), "particles stresses");
+  }
+}
+
+// Particle sets hold particle ids; the C ABI takes positions in the particle
+// order (equal until particles are removed or read back from a checkpoint).
+// Ids of no particle are skipped, as Mesh::iterate_over_particle_set does
+// (mesh.tcc:1175-1189); -1 (every particle) stays -1.
+template <unsigned Tdim>
+std::vector<int64_t> MPMExplicit<Tdim>::ids_to_indices(const std::vector<int64_t>& ids) const {
+  int64_t n = 0;
+  check(ctx_, mpmx_nparticles(ctx_, &n), "particle count");
+  std::vector<int64_t> pid(n);
+  check(ctx_, mpmx_get_particle_ids(ctx_, pid.data()), "particle ids");
+  std::map<int64_t, int64_t> at;
+  for (int64_t i = 0; i < n; ++i) at[pid[i]] = i;
+  std::vector<int64_t> out;
+  for (int64_t id : ids) {
+    if (id < 0) {
+      out.push_back(-1);
+      continue;
+    }
+    const auto it = at.find(id);
+    if (it != at.end()) out.push_back(it->second);
+  }
+  return out;
+}
+
+// The loads and constraints given per particle (particle velocity
+// constraints, particle surface tractions), for generated or resumed
+// particles (mpm_explicit.tcc:75-91: particle_entity_sets and
+// particle_velocity_constraints also run on resume; initialise_loads after).
+template <unsigned Tdim>
+void MPMExplicit<Tdim>::particle_loads() {
+  const Json& mp = io_->object("mesh");
   // particles_velocity_constraints (mpm_base.tcc:1549-1585): a particle set
   // id (-1: all particles), a direction and a velocity; applied in order
   if (mp.contains("boundary_conditions") && mp.at("boundary_conditions").is_object() &&
@@ -561,8 +693,7 @@ void MPMExplicit<Tdim>::initialise_particles() {
       }
     }
   }
-  if (!ctx_) return;
-  check(ctx_, mpmx_compute_volume_mass(ctx_), "particle volume and mass");
+  if (!ctx_) return;  // a dry run counts the constraints only
   // particle_surface_traction (mpm_base.tcc:882-911): a particle set id (-1:
   // all particles), a direction, a traction and an optional math function;
   // applied in order
@@ -595,58 +726,171 @@ void MPMExplicit<Tdim>::initialise_particles() {
         tf.push_back(fn);
       }
     }
+    tp = ids_to_indices(tp);
     check(ctx_, mpmx_set_particle_tractions(ctx_, (int64_t)tp.size(), tp.data(), td.data(),
                                             tv.data(), tf.data()),
           "particle surface tractions");
   }
-  if (!pvc_particle_.empty())
-    check(ctx_, mpmx_set_particle_velocity_constraints(ctx_, (int64_t)pvc_particle_.size(),
-                                                       pvc_particle_.data(), pvc_dir_.data(),
-                                                       pvc_val_.data()),
+  if (!pvc_particle_.empty()) {
+    const std::vector<int64_t> idx = ids_to_indices(pvc_particle_);
+    check(ctx_, mpmx_set_particle_velocity_constraints(ctx_, (int64_t)idx.size(), idx.data(),
+                                                       pvc_dir_.data(), pvc_val_.data()),
           "particle velocity constraints");
-  if (mp.contains("particles_stresses")) {
-    const Json& ps = mp.at("particles_stresses");
-    const std::string path = io_->file_name(ps.at("location").str());
-    const auto rows = read_rows(path);
-    const int64_t n = (int64_t)rows.at(0).at(0);
-    std::vector<double> s(n * 6);
-    for (int64_t i = 0; i < n; ++i)
-      for (int k = 0; k < 6; ++k) s[i * 6 + k] = rows.at(1 + i).at(k);
-    check(ctx_, mpmx_assign_stresses(ctx_, n, s.data()), "particles stresses");
   }
 }
 
-// Particle state as text: one row per particle in the reference's order.
+// Checkpoint resume (mpm_base.tcc:509-551, mesh.tcc:1659-1721,
+// particle.tcc:35-177): the particles of the table written at step step_,
+// in its order and with its ids, located from their stored cells; the mass
+// density is mass / volume; the step counter continues from step_.
 template <unsigned Tdim>
-void MPMExplicit<Tdim>::write_outputs(int64_t step) {
+void MPMExplicit<Tdim>::resume_particles() {
+  const std::string f = output_file("particles", ".h5", step_);
   int64_t n = 0;
-  check(ctx_, mpmx_nparticles(ctx_, &n), "particle count");
-  std::vector<double> x(n * Tdim), v(n * Tdim), s(n * 6), e(n * 6), vol(n), m(n), sv(n * 9);
-  std::vector<int64_t> cell(n);
-  check(ctx_, mpmx_get_particles(ctx_, x.data(), nullptr, v.data(), s.data(), e.data(), nullptr,
-                                 m.data(), vol.data(), nullptr, nullptr, cell.data(), nullptr,
-                                 sv.data()),
-        "read particles");
+  if (mpmx_pod_count(f.c_str(), &n) != 0) throw std::runtime_error(mpmx_pod_last_error());
+  std::vector<int64_t> id(n), cell(n);
+  std::vector<double> mass(n), vol(n), pres(n), co(3 * n), di(3 * n), ns(3 * n), ve(3 * n),
+      ac(3 * n), st(6 * n), sn(6 * n), fg(9 * n), mm(9 * n), sv(MPMX_POD_NSVARS * n);
+  std::vector<unsigned char> imm(n), stat(n);
+  std::vector<uint32_t> mid(n), nsv(n);
+  mpmx_pod_columns c;
+  c.n = n;
+  c.id = id.data();
+  c.mass = mass.data();
+  c.volume = vol.data();
+  c.pressure = pres.data();
+  c.coord = co.data();
+  c.displacement = di.data();
+  c.nsize = ns.data();
+  c.velocity = ve.data();
+  c.acceleration = ac.data();
+  c.stress = st.data();
+  c.strain = sn.data();
+  c.defgrad = fg.data();
+  c.initialise_mapping_matrix = imm.data();
+  c.mapping_matrix = mm.data();
+  c.status = stat.data();
+  c.cell_id = cell.data();
+  c.material_id = mid.data();
+  c.nstate_vars = nsv.data();
+  c.svars = sv.data();
+  if (mpmx_pod_read(f.c_str(), &c) != 0) throw std::runtime_error(mpmx_pod_last_error());
+  std::vector<double> x(Tdim * n), u(Tdim * n), v(Tdim * n), rho(n), svars(MPMX_STATE_VARS * n);
+  std::vector<int> mat(n);
+  std::vector<int64_t> hint(n);
+  int64_t with_sv = 0, without_sv = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    for (unsigned k = 0; k < Tdim; ++k) {
+      x[i * Tdim + k] = co[3 * i + k];
+      u[i * Tdim + k] = di[3 * i + k];
+      v[i * Tdim + k] = ve[3 * i + k];
+    }
+    rho[i] = mass[i] / vol[i];  // particle.tcc:46-47
+    const auto it = mat_index_.find((int64_t)mid[i]);
+    if (it == mat_index_.end()) throw std::runtime_error("Material is invalid to assign to particle!");
+    mat[i] = it->second;
+    // a cell id beyond the mesh is none (max Index written as -1)
+    hint[i] = (cell[i] >= 0 && cell[i] < (int64_t)(cells_.size() / NN)) ? cell[i] : -1;
+    // the table's state variables when their count is the material's
+    // (particle.tcc:160-171), else the material's initial values
+    const bool mc = mat_params_[it->second * MPMX_MATERIAL_PARAMS] == 1.0;
+    for (int k = 0; k < MPMX_STATE_VARS; ++k)
+      svars[i * MPMX_STATE_VARS + k] = sv[MPMX_POD_NSVARS * i + k];
+    if (mc) (nsv[i] == MPMX_STATE_VARS ? with_sv : without_sv) += 1;
+    nsize_[id[i]] = ns[3 * i];
+  }
+  if (with_sv > 0 && without_sv > 0)
+    throw std::runtime_error("resumed MohrCoulomb particles with and without their state variables");
+  // F as stored; the table's stress and strain are the Voigt 6-vectors
+  check(ctx_, mpmx_set_particles(ctx_, n, x.data(), u.data(), v.data(), st.data(), sn.data(),
+                                 fg.data(), mass.data(), vol.data(), rho.data(), mat.data(),
+                                 hint.data(), without_sv > 0 ? nullptr : svars.data()),
+        "resumed particles");
+  check(ctx_, mpmx_set_particle_ids(ctx_, n, id.data()), "resumed particle ids");
+  check(ctx_, mpmx_set_step(ctx_, step_), "resume step");
+  particle_loads();
+}
+
+// <working dir><post_processing path><uuid>/<attribute><step, zero-padded
+// to the digits of nsteps><ext> (IO::output_file, src/io/io.cc:94-140);
+// creates the folders.
+template <unsigned Tdim>
+std::string MPMExplicit<Tdim>::output_file(const std::string& attribute, const std::string& ext,
+                                           int64_t step) const {
   const std::string dir = io_->working_dir() + out_path_ + uuid_ + "/";
   std::string cur;
   for (char ch : dir) {
     cur += ch;
     if (ch == '/') mkdir(cur.c_str(), 0755);
   }
-  const std::string path = dir + "particles" + std::to_string(step) + ".txt";
-  FILE* f = std::fopen(path.c_str(), "w");
-  if (!f) throw std::runtime_error("cannot write " + path);
-  std::fprintf(f, "# step %lld particles %lld\n# id cell x[%u] v[%u] stress[6] strain[6] volume mass pdstrain\n",
-               (long long)step, (long long)n, Tdim, Tdim);
+  const int digits = (int)std::log10((double)std::max<int64_t>(nsteps_, 1)) + 1;
+  std::ostringstream name;
+  name << attribute;
+  name.fill('0');
+  name.width(digits);
+  name << step;
+  name << ext;
+  return dir + name.str();
+}
+
+// The particle table at output steps (MPMBase::write_outputs ->
+// write_hdf5, mpm_base.tcc:577-595, 752-766; Particle::pod(),
+// particle.tcc:180-305).
+template <unsigned Tdim>
+void MPMExplicit<Tdim>::write_outputs(int64_t step) {
+  if (step % output_steps_ != 0) return;
+  int64_t n = 0;
+  check(ctx_, mpmx_nparticles(ctx_, &n), "particle count");
+  std::vector<double> x(n * Tdim), u(n * Tdim), v(n * Tdim), s(n * 6), e(n * 6), F(n * 9),
+      vol(n), m(n), sv(n * MPMX_STATE_VARS);
+  std::vector<int> mat(n);
+  std::vector<int64_t> cell(n), ids(n);
+  check(ctx_, mpmx_get_particles(ctx_, x.data(), u.data(), v.data(), s.data(), e.data(), F.data(),
+                                 m.data(), vol.data(), nullptr, mat.data(), cell.data(), nullptr,
+                                 sv.data()),
+        "read particles");
+  check(ctx_, mpmx_get_particle_ids(ctx_, ids.data()), "particle ids");
+  std::vector<double> co(3 * n, 0.), di(3 * n, 0.), ve(3 * n, 0.), ns(3 * n, 0.),
+      svars(MPMX_POD_NSVARS * n, 0.);
+  std::vector<uint32_t> mid(n), nsv(n, 0);
+  // material ids as the deck numbers them
+  std::vector<int64_t> deck_id(mat_index_.size());
+  for (const auto& kv : mat_index_) deck_id[kv.second] = kv.first;
   for (int64_t i = 0; i < n; ++i) {
-    std::fprintf(f, "%lld %lld", (long long)i, (long long)cell[i]);
-    for (unsigned k = 0; k < Tdim; ++k) std::fprintf(f, " %.17g", x[i * Tdim + k]);
-    for (unsigned k = 0; k < Tdim; ++k) std::fprintf(f, " %.17g", v[i * Tdim + k]);
-    for (int k = 0; k < 6; ++k) std::fprintf(f, " %.17g", s[i * 6 + k]);
-    for (int k = 0; k < 6; ++k) std::fprintf(f, " %.17g", e[i * 6 + k]);
-    std::fprintf(f, " %.17g %.17g %.17g\n", vol[i], m[i], sv[i * 9 + 8]);
+    for (unsigned k = 0; k < Tdim; ++k) {
+      co[3 * i + k] = x[i * Tdim + k];
+      di[3 * i + k] = u[i * Tdim + k];
+      ve[3 * i + k] = v[i * Tdim + k];
+      const auto it = nsize_.find(ids[i]);
+      ns[3 * i + k] = it == nsize_.end() ? 0.0 : it->second;
+    }
+    mid[i] = (uint32_t)deck_id[mat[i]];
+    if (mat_params_[mat[i] * MPMX_MATERIAL_PARAMS] == 1.0) {  // MohrCoulomb
+      nsv[i] = MPMX_STATE_VARS;
+      for (int k = 0; k < MPMX_STATE_VARS; ++k)
+        svars[MPMX_POD_NSVARS * i + k] = sv[i * MPMX_STATE_VARS + k];
+    }
   }
-  std::fclose(f);
+  mpmx_pod_columns c;
+  std::memset(&c, 0, sizeof(c));
+  c.n = n;
+  c.id = ids.data();
+  c.mass = m.data();
+  c.volume = vol.data();
+  c.coord = co.data();
+  c.displacement = di.data();
+  c.nsize = ns.data();
+  c.velocity = ve.data();
+  c.stress = s.data();
+  c.strain = e.data();
+  c.defgrad = F.data();
+  c.cell_id = cell.data();
+  c.material_id = mid.data();
+  c.nstate_vars = nsv.data();
+  c.svars = svars.data();
+  const std::string path = output_file("particles", ".h5", step);
+  if (mpmx_pod_write(path.c_str(), &c) != 0)
+    throw std::runtime_error(std::string("write ") + path + ": " + mpmx_pod_last_error());
 }
 
 template <unsigned Tdim>
@@ -716,17 +960,22 @@ bool MPMExplicit<Tdim>::solve() {
                                               fdir_.data(), fsign_.data(), fmu_.data()),
           "friction constraints");
   check(ctx_, mpmx_set_gravity(ctx_, gravity_, ramp_), "gravity");
-  initialise_particles();
-  write_outputs(0);
+  check(ctx_, mpmx_set_locate_particles(ctx_, locate_particles_ ? 1 : 0), "locate_particles");
+  if (resume_) {
+    resume_particles();
+  } else {
+    initialise_particles();
+    write_outputs(0);
+  }
   // the step loop of mpm_explicit.tcc:112-190, in runs of steps between outputs
-  int64_t step = 0;
+  int64_t step = step_;
   while (step < nsteps_) {
     const int64_t next = std::min(nsteps_, (step / output_steps_ + 1) * output_steps_);
     const int rc = mpmx_step(ctx_, (int)(next - step), dt_, damping_);
     if (rc == MPMX_E_UNLOCATABLE) throw std::runtime_error(mpmx_last_error(ctx_));
     check(ctx_, rc, "step");
     step = next;
-    if (step % output_steps_ == 0 || step == nsteps_) write_outputs(step);
+    write_outputs(step);
   }
   check(ctx_, mpmx_synchronize(ctx_), "synchronize");
   return true;

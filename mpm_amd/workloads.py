@@ -160,7 +160,11 @@ def collapse_3d(nx=128, ny=64, nz=64, h=0.01, block=(64, 64, 64), nsteps=2000, d
         geostatic=(1.0 - np.sin(np.radians(30.0)), 1800.0 * 9.81, block[2] * h),
         info=dict(mesh="hex8 %dx%dx%d @ %g m" % (nx, ny, nz, h),
                   block_cells=list(block), material="MohrCoulomb phi=30 psi=0 c=0",
-                  particles_per_cell=8))
+                  particles_per_cell=8,
+                  initial_stress="K0 geostatic: sigma_zz = -rho g (H - z), "
+                                 "sigma_xx = sigma_yy = (1 - sin phi) sigma_zz",
+                  initial_velocity="uniform in +-1e-4 m/s per component, SplitMix64 seed %d"
+                                   % seed))
 
 
 def collapse_2d(width, height, h, youngs, friction, density, dt=1e-5, nsteps=1500,

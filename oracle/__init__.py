@@ -66,7 +66,8 @@ def _lib(omp=False, libm="portable"):
                    "orc_get_particle_scratch", "orc_assign_velocity",
                    "orc_assign_stresses", "orc_set_particle_velocity_constraints",
                    "orc_set_acceleration_constraints_fn", "orc_set_math_functions",
-                   "orc_set_nodal_forces", "orc_set_particle_tractions"):
+                   "orc_set_nodal_forces", "orc_set_particle_tractions",
+                   "orc_assign_materials", "orc_compute_volume_mass"):
             getattr(lib, fn).restype = C.c_int
         lib.orc_kat_cell_volume.restype = C.c_double
         lib.orc_kat_mc_yield.restype = C.c_int
@@ -241,6 +242,15 @@ class Oracle:
     def set_velocity_update(self, velocity_update, blending_ratio=1.0):
         self.lib.orc_set_velocity_update(self.ctx, VELOCITY_UPDATES[velocity_update],
                                          C.c_double(blending_ratio))
+
+    def compute_volume_mass(self):
+        self._check(self.lib.orc_compute_volume_mass(self.ctx))
+
+    def assign_materials(self, particles, material_id):
+        idx = np.ascontiguousarray(particles, dtype=np.int64)
+        self._check(self.lib.orc_assign_materials(self.ctx, C.c_int64(len(idx)),
+                                                  idx.ctypes.data_as(C.c_void_p),
+                                                  int(material_id)))
 
     def set_locate_particles(self, locate_particles=True):
         self.lib.orc_set_locate_particles(self.ctx, 1 if locate_particles else 0)

@@ -1164,6 +1164,22 @@ struct Solver {
     }
   }
 
+  // Every particle: Particle::compute_volume (cell volume / particles in the
+  // cell, particle.tcc:589-595) then compute_mass (volume * the material's
+  // density, :618-626), as MPMBase::initialise_particles and
+  // MPMExplicit::solve run them over all particles (mpm_base.tcc:413-415,
+  // mpm_explicit.tcc:92-95).
+  void compute_volume_mass() {
+    for (auto& p : particles) {
+      if (p.cell == kNoCell) throw std::runtime_error("particle without a cell");
+      const Cell& cl = cells[p.cell];
+      p.volume = cl.volume / cl.nparticles;
+      p.size = std::pow(p.volume, static_cast<double>(1. / dim));
+      p.density = materials[p.mat].density;
+      p.mass = p.volume * p.density;
+    }
+  }
+
   // ---------------- step phases -----------------
   // particle.tcc:537-554 compute_shapefn
   void compute_shapefn(Particle& p) {
@@ -1783,6 +1799,25 @@ int orc_set_particle_velocity_constraints(orc_ctx* c, Index n, const Index* part
       if (dir[i] < 0 || dir[i] >= c->s.dim)
         throw std::runtime_error("Invalid direction of velocity constraint");
       c->s.pvel.push_back({particle[i], dir[i], value[i]});
+    }
+  })
+}
+
+int orc_compute_volume_mass(orc_ctx* c) { ORC_TRY(c->s.compute_volume_mass()) }
+
+// Particle::assign_material (particle.tcc:499-517) for the listed particles
+// (material_sets, mpm_base.tcc:449-468).
+int orc_assign_materials(orc_ctx* c, Index n, const Index* particle, int mat) {
+  ORC_TRY({
+    auto& s = c->s;
+    if (mat < 0 || mat >= (int)s.materials.size()) throw std::runtime_error("bad material id");
+    for (Index i = 0; i < n; ++i) {
+      if (particle[i] < 0 || particle[i] >= (Index)s.particles.size())
+        throw std::runtime_error("particle index out of range");
+      orc::Particle& p = s.particles[particle[i]];
+      p.mat = mat;
+      for (int k = 0; k < orc::kNsv; ++k) p.sv[k] = 0.;
+      if (s.materials[mat].type == orc::kMC) orc::mc_initialise_state(s.materials[mat], p.sv);
     }
   })
 }

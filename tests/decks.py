@@ -159,13 +159,118 @@ def run_driver(d, *extra, check=True):
     return r
 
 
+def output_path(d, step, nsteps, uuid="deck", attribute="particles", ext=".h5"):
+    """<d>/results/<uuid>/<attribute><step zero-padded to the digits of
+    nsteps><ext> (IO::output_file, src/io/io.cc:94-140)."""
+    import math
+    digits = int(math.log10(max(nsteps, 1))) + 1
+    return os.path.join(d, "results", uuid, "%s%0*d%s" % (attribute, digits, step, ext))
+
+
 def read_output(path, dim):
-    """Rows of a particles<step>.txt: id cell x v stress strain volume mass pdstrain."""
-    a = np.loadtxt(path, comments="#", ndmin=2)
-    k = 2
-    out = {"cell": a[:, 1].astype(np.int64)}
-    for name, n in (("x", dim), ("v", dim), ("stress", 6), ("strain", 6), ("volume", 1),
-                    ("mass", 1), ("pdstrain", 1)):
-        out[name] = a[:, k:k + n] if n > 1 else a[:, k]
-        k += n
-    return out
+    """A particle table written by the driver (the reference's PODParticle
+    HDF5 table) as arrays in its record order: id, cell, x, u, v, stress,
+    strain, F, volume, mass, statevars (the first 9 state variables),
+    material_id, nsize."""
+    from mpm_amd import pod
+    c = pod.read(path)
+    return {"id": c["id"], "cell": c["cell_id"], "x": c["coord"][:, :dim],
+            "u": c["displacement"][:, :dim], "v": c["velocity"][:, :dim],
+            "stress": c["stress"], "strain": c["strain"], "F": c["defgrad"],
+            "volume": c["volume"], "mass": c["mass"], "statevars": c["svars"][:, :9],
+            "pdstrain": c["svars"][:, 8], "material_id": c["material_id"],
+            "nstate_vars": c["nstate_vars"], "nsize": c["nsize"][:, 0]}
+
+
+def write_fixture_deck(d, dim, scheme="usf", nsteps=10, output_steps=5, resume=None,
+                       uuid="fixture"):
+    """The deck the reference's solver tests run (tests/io/write_mesh_particles.cc:
+    write_json, write_entity_set, write_mesh_2d/3d, write_particles_2d/3d):
+    two quad4/hex8 cells of 0.5 m, 8 (2D) or 16 (3D) particles from a file,
+    two LinearElastic materials with the second given to particle set 2 by
+    "material_sets", velocity constraints from a file, a Linear math function
+    scaling a particle traction on every particle and a concentrated force on
+    every node, Cundall damping 0.02, velocity_update true (PIC), dt 1e-3.
+    Its rotation_forces entry is left out: this build rejects them.
+    resume: None or the step to resume from."""
+    os.makedirs(d, exist_ok=True)
+    if dim == 2:
+        nodes = [(0., 0.), (0.5, 0.), (0.5, 0.5), (0., 0.5), (1.0, 0.), (1.0, 0.5)]
+        cells = [(0, 1, 2, 3), (1, 4, 5, 2)]
+        parts = [(0.125, 0.125), (0.375, 0.125), (0.375, 0.375), (0.125, 0.375),
+                 (0.625, 0.125), (0.875, 0.125), (0.875, 0.375), (0.625, 0.375)]
+        cons = [(0, 1, 0.0), (1, 1, 0.0), (4, 1, 0.0)]
+        gravity = [0.0, -9.81]
+        node_set = [4, 5]
+    else:
+        nodes = [(0., 0., 0.), (0.5, 0., 0.), (0.5, 0.5, 0.), (0., 0.5, 0.), (0., 0., 0.5),
+                 (0.5, 0., 0.5), (0.5, 0.5, 0.5), (0., 0.5, 0.5), (1.0, 0., 0.), (1.0, 0.5, 0.),
+                 (1.0, 0., 0.5), (1.0, 0.5, 0.5)]
+        cells = [(0, 1, 2, 3, 4, 5, 6, 7), (1, 8, 9, 2, 5, 10, 11, 6)]
+        parts = [(0.125, 0.125, 0.125), (0.375, 0.125, 0.125), (0.375, 0.375, 0.125),
+                 (0.125, 0.375, 0.125), (0.125, 0.125, 0.375), (0.375, 0.125, 0.375),
+                 (0.375, 0.375, 0.375), (0.125, 0.375, 0.375), (0.625, 0.125, 0.125),
+                 (0.875, 0.125, 0.125), (0.875, 0.375, 0.125), (0.625, 0.375, 0.125),
+                 (0.675, 0.125, 0.375), (0.875, 0.125, 0.375), (0.875, 0.375, 0.375),
+                 (0.675, 0.375, 0.375)]
+        cons = [(n, 2, 0.0) for n in (0, 1, 2, 3, 8, 9)]
+        gravity = [0.0, 0.0, -9.81]
+        node_set = [8, 9, 10, 11]
+    tag = "%dd" % dim
+    with open(os.path.join(d, "mesh-%s.txt" % tag), "w") as f:
+        f.write("! elementShape hexahedron\n! elementNumPoints 8\n")
+        f.write("%d\t%d\n" % (len(nodes), len(cells)))
+        for x in nodes:
+            f.write("\t".join("%g" % v for v in x) + "\t\n")
+        for c in cells:
+            f.write("\t".join(str(n) for n in c) + "\t\n")
+    with open(os.path.join(d, "velocity-constraints.txt"), "w") as f:
+        for n, k, v in cons:
+            f.write("%d\t%d\t%g\n" % (n, k, v))
+    with open(os.path.join(d, "particles-%s.txt" % tag), "w") as f:
+        f.write("%d\n" % len(parts))
+        for x in parts:
+            f.write("\t".join("%g" % v for v in x) + "\t\n")
+    with open(os.path.join(d, "entity_sets_0.json"), "w") as f:
+        json.dump({"particle_sets": [{"id": 2, "set": list(range(16))}],
+                   "node_sets": [{"id": 1, "set": node_set}]}, f, indent=2)
+    mtype = "LinearElastic%dD" % dim
+    deck = {
+        "title": "Example JSON Input for MPM",
+        "mesh": {"mesh": "mesh-%s.txt" % tag, "entity_sets": "entity_sets_0.json",
+                 "io_type": "Ascii%dD" % dim, "check_duplicates": True, "isoparametric": False,
+                 "node_type": "N%dD" % dim,
+                 "boundary_conditions": {"velocity_constraints": [
+                     {"file": "velocity-constraints.txt"}]},
+                 "cell_type": "ED2Q4" if dim == 2 else "ED3H8"},
+        "particles": [{"generator": {"type": "file", "material_id": 0, "pset_id": 0,
+                                     "io_type": "Ascii%dD" % dim,
+                                     "particle_type": "P%dD" % dim, "check_duplicates": True,
+                                     "location": "particles-%s.txt" % tag}}],
+        "materials": [{"id": 0, "type": mtype, "density": 1000., "youngs_modulus": 1.0E+8,
+                       "poisson_ratio": 0.495},
+                      {"id": 1, "type": mtype, "density": 2300., "youngs_modulus": 1.5E+6,
+                       "poisson_ratio": 0.25}],
+        "material_sets": [{"material_id": 1, "phase_id": 0, "pset_id": 2}],
+        "external_loading_conditions": {
+            "gravity": gravity,
+            "particle_surface_traction": [{"math_function_id": 0, "pset_id": -1, "dir": 1,
+                                           "traction": 10.5}],
+            "concentrated_nodal_forces": [{"math_function_id": 0, "nset_id": -1, "dir": 1,
+                                           "force": 10.5}]},
+        "math_functions": [{"id": 0, "type": "Linear", "xvalues": [0.0, 0.5, 1.0],
+                            "fxvalues": [0.0, 1.0, 1.0]}],
+        "analysis": {"type": "MPMExplicit%dD" % dim, "mpm_scheme": scheme,
+                     "locate_particles": True, "velocity_update": True, "dt": 0.001,
+                     "uuid": uuid, "nsteps": nsteps,
+                     "resume": {"resume": resume is not None, "uuid": uuid,
+                                "step": resume if resume is not None else 5},
+                     "damping": {"type": "Cundall", "damping_factor": 0.02},
+                     "newmark": {"beta": 0.25, "gamma": 0.5}},
+        "post_processing": {"path": "results/", "vtk": ["stresses", "strains", "velocities"],
+                            "vtk_statevars": [{"phase_id": 0, "statevars": ["pdstrain"]}],
+                            "output_steps": output_steps},
+    }
+    with open(os.path.join(d, "mpm.json"), "w") as f:
+        json.dump(deck, f, indent=2)
+    return deck

@@ -41,3 +41,20 @@ def test_create_without_device_fails_cleanly():
     with pytest.raises(capi.MpmxError) as e:
         capi.Solver(2)
     assert e.value.code == capi.E_HIP
+
+
+def _declared_pod():
+    src = open(os.path.join(ROOT, "include", "mpmx_pod.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(mpmx_pod_\w+)\s*\(", src, re.M)))
+
+
+def test_checkpoint_library_exports_every_declared_symbol():
+    """include/mpmx_pod.h is served by mpm_amd/libmpmx_pod.so (host only)."""
+    from mpm_amd import pod
+    names = _declared_pod()
+    assert {"mpmx_pod_write", "mpmx_pod_read", "mpmx_pod_count"} <= set(names)
+    if not os.path.exists(pod.LIB_PATH):
+        pytest.skip("libmpmx_pod.so not built")
+    lib = ctypes.CDLL(pod.LIB_PATH)
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
