@@ -1064,7 +1064,7 @@ void launch_steps(mpmx_ctx* c, int nsteps, double dt, double damping,
   A.ad_slot = ad_nodes ? &c->d_status.p->ad_slot : nullptr;
   A.ad_stride = ad_stride;
   A.ad_nslots = ad_nslots;
-  Launcher L{c, {}};
+  Launcher L{c};
   const bool use_graph = !(c->flags & MPMX_NO_GRAPH) && !c->profile;
   const int rec = ad_nodes ? 2 : 0;
   if (use_graph) {

@@ -74,6 +74,9 @@ class Workload:
     geostatic: Optional[tuple] = None
     damping: float = 0.0
     info: dict = field(default_factory=dict)
+    # Steps the deck can run before a particle leaves the mesh (None: all
+    # nsteps). The reference would stop there too (mpm_scheme.tcc:250-256).
+    runnable_steps: Optional[int] = None
 
     def setup(self, solver):
         """Apply this deck to a solver (mesh, materials, constraints, particles)."""
@@ -158,6 +161,11 @@ def collapse_3d(nx=128, ny=64, nz=64, h=0.01, block=(64, 64, 64), nsteps=2000, d
         fric_sign=np.full(len(base), -1, dtype=np.int32), fric_mu=np.full(len(base), 0.3),
         velocity_seed=seed, velocity_amplitude=1e-4,
         geostatic=(1.0 - np.sin(np.radians(30.0)), 1800.0 * 9.81, block[2] * h),
+        # Undamped FLIP/MUSL: the toe's velocities grow from ~0.25 m/s at step
+        # 200 to ~2.4 m/s at step 800 and without bound after ~900; a particle
+        # leaves the mesh near step 1000, in ordered mode (the reference's
+        # arithmetic, bitwise) as in fast mode.
+        runnable_steps=900 if nsteps >= 900 else None,
         info=dict(mesh="hex8 %dx%dx%d @ %g m" % (nx, ny, nz, h),
                   block_cells=list(block), material="MohrCoulomb phi=30 psi=0 c=0",
                   particles_per_cell=8,

@@ -216,6 +216,27 @@ def test_graph_replay_matches_direct_launches():
             assert _relerr(pa[f], pb[f]) <= 1e-12
 
 
+def test_profiled_steps_record_every_kernel_and_match():
+    """Profiling launches each kernel between HIP events (no graph): every
+    kernel of the sequence is timed, and the ordered-mode state equals an
+    unprofiled run's bit for bit."""
+    w = _collapse()
+    from mpm_amd.capi import Solver
+    a = w.setup(Solver(3, "musl", ordered=True))
+    b = w.setup(Solver(3, "musl", ordered=True))
+    a.profile(True)
+    a.step(4, w.dt)
+    kt = a.kernel_times()
+    a.profile(False)
+    b.step(4, w.dt)
+    for k in ("sort_by_cell", "gather_all", "g2p", "gather_momentum", "stress_locate",
+              "mc_plastic", "locate_scan", "step_end"):
+        assert k in kt and kt[k][1] == 4 and kt[k][0] > 0.0, (k, kt)
+    pa, pb = a.particles(), b.particles()
+    for f in FIELDS + ("statevars", "cell"):
+        assert np.array_equal(pa[f], pb[f]), f
+
+
 def test_sort_interval_changes_only_summation_order():
     """Reusing an earlier sort's order (particles that changed cell form their
     own runs) changes only the atomic summation order: a run sorted every step
