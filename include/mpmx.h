@@ -256,6 +256,22 @@ int mpmx_get_nodes(mpmx_ctx* ctx, double* mass, double* velocity, double* accele
  * mpmx_last_error() then names it and the step. */
 int mpmx_step(mpmx_ctx* ctx, int nsteps, double dt, double damping_factor);
 
+/* mpmx_step without any device allocation. No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same steps and results as mpmx_step; the kernels launch directly (no
+ * graph is instantiated) and take their scratch (sort keys and permutation,
+ * radix-sort temporaries, locate queue, deferred Mohr-Coulomb list and
+ * records, Jaumann spin, math-function values, ordered-mode cell ranges)
+ * from one 64 MiB file-scope device array per GPU, held until the process
+ * ends. A context whose step needs more (144 bytes per particle with a
+ * Mohr-Coulomb material, 32 without, plus 24 with the Jaumann rate: about
+ * 460k Mohr-Coulomb particles) gets MPMX_E_ARG; particle removal
+ * (mpmx_set_locate_particles(ctx, 0)) gets MPMX_E_UNSUPPORTED. Every call on
+ * one GPU shares that array and the first step of each call sorts fully:
+ * calls that overlap on one GPU (contexts used from several threads) race. */
+int mpmx_step_noalloc(mpmx_ctx* ctx, int nsteps, double dt, double damping_factor);
+
 /* Fast mode: a full radix sort of the particles by cell every `steps` steps
  * (default 8; 1 = every step). In between, the last sort's order is reused
  * with the current cells; particles that changed cell add scatter atomics

@@ -31,7 +31,7 @@ EXPORTS = (
     "mpmx_set_gravity", "mpmx_generate_gauss",
     "mpmx_set_particles", "mpmx_compute_volume_mass", "mpmx_assign_velocity",
     "mpmx_assign_stresses", "mpmx_assign_materials", "mpmx_nparticles", "mpmx_get_particles", "mpmx_get_nodes",
-    "mpmx_step", "mpmx_set_locate_particles", "mpmx_get_particle_ids", "mpmx_set_particle_ids",
+    "mpmx_step", "mpmx_step_noalloc", "mpmx_set_locate_particles", "mpmx_get_particle_ids", "mpmx_set_particle_ids",
     "mpmx_set_sort_interval", "mpmx_set_chunk", "mpmx_autotune", "mpmx_set_step",
     "mpmx_get_step",
     "mpmx_synchronize", "mpmx_profile", "mpmx_kernel_times",
@@ -90,6 +90,7 @@ def load(path=LIB_PATH):
         "mpmx_get_particles": [V, _D, _D, _D, _D, _D, _D, _D, _D, _D, _I32, _I64, _D, _D],
         "mpmx_get_nodes": [V, _D, _D, _D, _I32],
         "mpmx_step": [V, C.c_int, C.c_double, C.c_double],
+        "mpmx_step_noalloc": [V, C.c_int, C.c_double, C.c_double],
         "mpmx_set_step": [V, C.c_int64],
         "mpmx_set_locate_particles": [V, C.c_int],
         "mpmx_get_particle_ids": [V, _I64],
@@ -341,6 +342,12 @@ class Solver:
 
     def step(self, nsteps, dt, damping_factor=0.0):
         self._check(self.lib.mpmx_step(self.ctx, int(nsteps), float(dt), float(damping_factor)))
+
+    def step_noalloc(self, nsteps, dt, damping_factor=0.0):
+        """step() with no device allocation: scratch from the library's
+        fixed per-GPU array (mpmx_step_noalloc)."""
+        self._check(self.lib.mpmx_step_noalloc(self.ctx, int(nsteps), float(dt),
+                                               float(damping_factor)))
 
     @property
     def step_count(self):

@@ -12,6 +12,7 @@
 #include <limits>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -191,6 +192,13 @@ struct mpmx_ctx {
   DBuf<double> d_ad_ck, d_ad_slots, d_ad_node_slots, d_ad_nodes, d_ad_parts;
   DBuf<int> d_ad_ck_cell, d_ad_slot_cell, d_ad_list;
   DBuf<double> d_ad_list_buf;
+
+  // device the context was created on, and the address there of the
+  // file-scope scratch array of mpmx_step_noalloc (null: not resolved, with
+  // the reason in scratch_error)
+  int device = 0;
+  void* scratch = nullptr;
+  std::string scratch_error;
 
   // profiling
   bool profile = false;
@@ -1173,6 +1181,68 @@ void check_status(mpmx_ctx* c) {
                        " (step " + std::to_string(c->h_status->first_bad_step) + ")");
 }
 
+// ------------------------ step without allocation ----------------------------
+// mpmx_step_noalloc takes the step kernels' scratch (everything StepBuffers
+// holds but the status block) from one file-scope device array instead of
+// the context's buffers. Each GPU holds its own copy until the process ends.
+// In host code the array's name is not a device address: its address on a
+// device comes from hipGetSymbolAddress with that device current, resolved
+// once per device when a context is created there, and only that address is
+// used.
+constexpr size_t kScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_addr[kMaxDevices] = {};
+
+// Address of the scratch array on the current device `dev` (resolved once).
+hipError_t resolve_scratch(int dev, void** out) {
+  *out = nullptr;
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_addr[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_addr[dev] = ptr;
+  }
+  *out = g_scratch_addr[dev];
+  return hipSuccess;
+}
+
+// The step scratch laid out from `base` (256-byte aligned pieces); returns
+// the bytes it spans. With base = 0 only the size is computed.
+size_t carve_step_buffers(const mpmx_ctx* c, uintptr_t base, StepBuffers* B) {
+  const int64_t np = std::max<int64_t>(c->np, 1);
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    off = (off + 255) & ~size_t(255);
+    const uintptr_t p = base + off;
+    off += bytes;
+    return p;
+  };
+  B->S = c->d_status.p;
+  B->skey = (int*)take(np * sizeof(int));
+  B->perm = (int*)take(np * sizeof(int));
+  B->sort_tmp = (int*)take(radix_sort_temp_ints(np) * sizeof(int));
+  B->scan = (int*)take(np * sizeof(int));
+  B->cstart = B->cend = nullptr;
+  if (c->flags & MPMX_ORDERED) {
+    B->cstart = (int*)take(std::max<int64_t>(c->ncells, 1) * sizeof(int));
+    B->cend = (int*)take(std::max<int64_t>(c->ncells, 1) * sizeof(int));
+  }
+  B->plastic_list = nullptr;
+  B->plastic_rec = nullptr;
+  if (c->any_mc) {
+    B->plastic_list = (int*)take(np * sizeof(int));
+    B->plastic_rec = (double*)take((size_t)PR_N * np * sizeof(double));
+  }
+  B->spin = c->any_rate ? (double*)take(3 * np * sizeof(double)) : nullptr;
+  B->fnval = c->fns.empty() ? nullptr : (double*)take(c->fns.size() * sizeof(double));
+  return off;
+}
+
 // ------------------------------ adjoint --------------------------------------
 // The particle state a checkpoint holds: every per-particle array the step
 // reads or writes (mass and material are constant and not copied).
@@ -1393,6 +1463,13 @@ int mpmx_create(mpmx_ctx** out, int dim, int scheme, int velocity_update, double
     c->error = e.what();
     *out = c;
     return e.code;
+  }
+  // the no-allocation step's scratch on this device (resolved once per device)
+  hipError_t se = hipGetDevice(&c->device);
+  if (se == hipSuccess) se = resolve_scratch(c->device, &c->scratch);
+  if (se != hipSuccess) {
+    c->scratch = nullptr;
+    c->scratch_error = std::string("scratch address: ") + hipGetErrorString(se);
   }
   *out = c;
   return MPMX_OK;
@@ -1985,6 +2062,41 @@ int mpmx_step(mpmx_ctx* c, int nsteps, double dt, double damping_factor) {
   }
   reset_status(c);
   launch_steps(c, nsteps, dt, damping_factor);
+  check_status(c);
+  API_END(c)
+}
+
+int mpmx_step_noalloc(mpmx_ctx* c, int nsteps, double dt, double damping_factor) {
+  API_BEGIN(c)
+  require_mesh(c);
+  if (c->mats.empty()) throw ApiError(MPMX_E_STATE, "materials are not set");
+  if (nsteps < 0 || !(dt > 0)) throw ApiError(MPMX_E_ARG, "nsteps >= 0 and dt > 0 required");
+  if (!c->locate_all)
+    throw ApiError(MPMX_E_UNSUPPORTED,
+                   "mpmx_step_noalloc: removing particles that leave the mesh reallocates the "
+                   "particle state; use mpmx_step with locate_particles = 0");
+  if (!c->scratch) throw ApiError(MPMX_E_HIP, "mpmx_step_noalloc: " + c->scratch_error);
+  if (nsteps == 0 || c->np == 0) {
+    c->step += nsteps;
+    return MPMX_OK;
+  }
+  StepBuffers B;
+  const size_t need = carve_step_buffers(c, 0, &B);
+  if (need > kScratchBytes)
+    throw ApiError(MPMX_E_ARG, "mpmx_step_noalloc: the step needs " + std::to_string(need) +
+                                   " bytes of scratch, more than its maximum of " +
+                                   std::to_string(kScratchBytes) + " (use mpmx_step)");
+  carve_step_buffers(c, reinterpret_cast<uintptr_t>(c->scratch), &B);
+  reset_status(c);
+  const StepArgs A = step_args(c, dt, damping_factor);
+  Launcher L{c};
+  L.timed = false;
+  // the scratch holds whatever the last call on this GPU left: sort fully at
+  // the first step; the context's own permutation was not advanced either
+  c->need_sort = true;
+  for (int s = 0; s < nsteps; ++s) enqueue(c, A, L, next_step_sorts(c), B);
+  HIPCHECK(hipGetLastError());
+  c->need_sort = true;
   check_status(c);
   API_END(c)
 }
