@@ -74,8 +74,11 @@ const char* mpmx_last_error(const mpmx_ctx* ctx);
 /* Mesh: node coordinates [nnodes*dim] and cell connectivity [ncells*2^dim]
  * in the element's local node order. Replaces Mesh::create_nodes /
  * create_cells / find_cell_neighbours (mesh.tcc:17-49, 211-257, 287-307).
- * Only isoparametric = 0 (Cartesian local coordinates, cell.tcc:423-606) is
- * supported; 1 returns MPMX_E_UNSUPPORTED. */
+ * isoparametric = 0: local coordinates by the Cartesian formula
+ * (Cell::local_coordinates_point, cell.tcc:423-606); 1: by the inverse of the
+ * element map (Cell::transform_real_to_unit_cell, cell.tcc:617-828), the
+ * reference's default when the mesh's "isoparametric" key is absent
+ * (mpm_base.tcc:768-783). */
 int mpmx_set_mesh(mpmx_ctx* ctx, int64_t nnodes, const double* coords,
                   int64_t ncells, const int64_t* cell_nodes, int isoparametric);
 
@@ -113,6 +116,17 @@ int mpmx_set_acceleration_constraints(mpmx_ctx* ctx, int64_t n, const int64_t* n
                                       const int* dir, const double* value);
 int mpmx_set_friction_constraints(mpmx_ctx* ctx, int64_t n, const int64_t* node,
                                   const int* dir, const int* sign_n, const double* mu);
+
+/* General (inclined) boundaries: the velocity, acceleration and friction
+ * constraints of node[i] act in the frame rotated by the Euler angles
+ * angles[i*dim ...] (radians; alpha, beta in 2D, alpha, beta, gamma in 3D),
+ * directions given in that frame. Replaces the "nodal_euler_angles" file
+ * (MPMBase::node_euler_angles, mpm_base.tcc:1041-1063) and
+ * Mesh::compute_nodal_rotation_matrices (mesh.tcc:1320-1340) with
+ * mpm::geometry::rotation_matrix (src/geometry.cc:6-56); applied by
+ * node.tcc:385-401, 469-482, 540-583, 602-655. */
+int mpmx_set_nodal_euler_angles(mpmx_ctx* ctx, int64_t n, const int64_t* node,
+                                const double* angles);
 
 /* Math functions of time-dependent boundary conditions, indexed 0..nf-1
  * (the reference's "math_functions", MPMBase::initialise_math_functions

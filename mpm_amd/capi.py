@@ -28,6 +28,7 @@ EXPORTS = (
     "mpmx_set_acceleration_constraints_fn", "mpmx_set_math_functions", "mpmx_set_nodal_forces",
     "mpmx_set_particle_tractions",
     "mpmx_set_friction_constraints", "mpmx_set_particle_velocity_constraints",
+    "mpmx_set_nodal_euler_angles",
     "mpmx_set_gravity", "mpmx_generate_gauss",
     "mpmx_set_particles", "mpmx_compute_volume_mass", "mpmx_assign_velocity",
     "mpmx_assign_stresses", "mpmx_assign_materials", "mpmx_nparticles", "mpmx_get_particles", "mpmx_get_nodes",
@@ -78,6 +79,7 @@ def load(path=LIB_PATH):
         "mpmx_set_nodal_forces": [V, C.c_int64, _I64, _I32, _D, _I32],
         "mpmx_set_particle_tractions": [V, C.c_int64, _I64, _I32, _D, _I32],
         "mpmx_set_friction_constraints": [V, C.c_int64, _I64, _I32, _I32, _D],
+        "mpmx_set_nodal_euler_angles": [V, C.c_int64, _I64, _D],
         "mpmx_set_particle_velocity_constraints": [V, C.c_int64, _I64, _I32, _D],
         "mpmx_set_gravity": [V, _D, C.c_double],
         "mpmx_generate_gauss": [V, C.c_int, C.c_int64, _I64, C.c_int],
@@ -260,6 +262,14 @@ class Solver:
         va = np.ascontiguousarray(values, dtype=np.float64)
         self._check(self.lib.mpmx_set_particle_velocity_constraints(
             self.ctx, len(pa), _p(pa, _I64), _p(di, _I32), _p(va)))
+
+    def set_nodal_euler_angles(self, nodes, angles):
+        """General (inclined) boundaries: the constraints of these nodes act in
+        the frame rotated by the Euler angles (radians, rows of dim)."""
+        nodes = np.ascontiguousarray(nodes, dtype=np.int64)
+        ang = _f64(angles, (len(nodes), self.dim))
+        self._check(self.lib.mpmx_set_nodal_euler_angles(self.ctx, len(nodes), _p(nodes, _I64),
+                                                         _p(ang)))
 
     def set_gravity(self, g, ramping_time=0.0):
         self._check(self.lib.mpmx_set_gravity(self.ctx, _p(_f64(g)), ramping_time))

@@ -91,6 +91,9 @@ MPMX_DEV void load_cell_X(const Mesh& M, int c, double (*X)[D]) {
   }
 }
 
+template <int D>
+MPMX_DEV void mat_inverse(const double (*J)[D], double (*Ji)[D]);
+
 // J = grad^T X (sequential over nodes), J^-1 by adjugate / determinant.
 // quadrilateral_element.tcc:295-311, hexahedron_element.tcc:307-321
 template <int D>
@@ -113,6 +116,13 @@ MPMX_DEV void jacobian_inverse(const double* xi, const double (*X)[D], double (*
         for (int b = 0; b < D; ++b) J[a][b] = J[a][b] + g[a] * X[i][b];
     }
   }
+  mat_inverse<D>(J, Ji);
+}
+
+// Inverse of a DxD matrix by the adjugate over the determinant (cofactor
+// expansion along the first column).
+template <int D>
+MPMX_DEV void mat_inverse(const double (*J)[D], double (*Ji)[D]) {
   if constexpr (D == 2) {
     const double det = J[0][0] * J[1][1] - J[1][0] * J[0][1];
     const double invdet = 1.0 / det;
@@ -154,8 +164,279 @@ MPMX_DEV void dndx_i(const double* xi, int i, const double (*Ji)[D], double* out
   }
 }
 
-// --------------------------- locate (cell.tcc:324-380) ----------------------
+// ------------------ isoparametric local coordinates -------------------------
+// Cell::transform_real_to_unit_cell (cell.tcc:617-828) for quad4 / hex8, in
+// the same operation order as the oracle's restatement.
+
+// Vertex i of the unit cell (quad counter-clockwise from (-1,-1); hex the
+// same square at zeta = -1, then +1), coordinate a.
+MPMX_DEV double unit_vertex(int i, int a) {
+  const int l = i & 3;
+  if (a == 0) return (l == 0 || l == 3) ? -1. : 1.;
+  if (a == 1) return l < 2 ? -1. : 1.;
+  return i < 4 ? -1. : 1.;
+}
+
+// r = X^T N(xi) - p (sequential over nodes)
 template <int D>
+MPMX_DEV void iso_residual(const double (*X)[D], const double* xi, const double* p, double* r) {
+  constexpr int NN = Elem<D>::NN;
+  double N[NN];
+  shapefn<D>(xi, N);
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    double s = X[0][k] * N[0];
+#pragma unroll
+    for (int i = 1; i < NN; ++i) s = s + X[i][k] * N[i];
+    r[k] = s - p[k];
+  }
+}
+
+template <int D>
+MPMX_DEV double sqnorm(const double* v) {
+  double s = v[0] * v[0];
+#pragma unroll
+  for (int k = 1; k < D; ++k) s = s + v[k] * v[k];
+  return s;
+}
+
+// Closed-form inverse of the quad4 bilinear map (after Zhao et al. 1999,
+// QuadrilateralElement<2,4>::natural_coordinates_analytical,
+// quadrilateral_element.tcc:523-754): with 4x = a1 + a2 xi + a3 eta +
+// a4 xi eta (b for y) the point solves a2 xi + a3 eta + a4 xi eta = c1,
+// b2 xi + b3 eta + b4 xi eta = c2. A parallelogram (a4 = b4 = 0) is a linear
+// system; otherwise eliminating the product gives p xi + q eta = k and a
+// quadratic in xi (q != 0), or xi itself (q = 0). Returns the root with both
+// coordinates in [-1, 1] (smaller xi first), else DBL_MAX.
+MPMX_DEV void q4_natural_coordinates(const double (*X)[2], const double* pt, double* xi) {
+  const double big = 1.7976931348623157e308;
+  xi[0] = xi[1] = big;
+  const double x1 = X[0][0], y1 = X[0][1], x2 = X[1][0], y2 = X[1][1];
+  const double x3 = X[2][0], y3 = X[2][1], x4 = X[3][0], y4 = X[3][1];
+  const double a1 = x1 + x2 + x3 + x4, a2 = -x1 + x2 + x3 - x4;
+  const double a3 = -x1 - x2 + x3 + x4, a4 = x1 - x2 + x3 - x4;
+  const double b1 = y1 + y2 + y3 + y4, b2 = -y1 + y2 + y3 - y4;
+  const double b3 = -y1 - y2 + y3 + y4, b4 = y1 - y2 + y3 - y4;
+  const double c1 = 4. * pt[0] - a1, c2 = 4. * pt[1] - b1;
+  if (a4 == 0. && b4 == 0.) {
+    const double det = a2 * b3 - a3 * b2;
+    xi[0] = (c1 * b3 - a3 * c2) / det;
+    xi[1] = (a2 * c2 - b2 * c1) / det;
+    return;
+  }
+  const double pl = a2 * b4 - a4 * b2, ql = a3 * b4 - a4 * b3, kl = c1 * b4 - c2 * a4;
+  if (ql == 0.) {
+    const double u = kl / pl;
+    const double den = a3 + a4 * u;
+    const double v = den != 0. ? (c1 - a2 * u) / den : (c2 - b2 * u) / (b3 + b4 * u);
+    if (u >= -1. && u <= 1. && v >= -1. && v <= 1.) {
+      xi[0] = u;
+      xi[1] = v;
+    }
+    return;
+  }
+  const double A = -a4 * pl, B = a2 * ql + a4 * kl - a3 * pl, C = a3 * kl - c1 * ql;
+  double r[2];
+  int nr = 0;
+  if (A == 0.) {
+    r[nr++] = -C / B;
+  } else {
+    const double disc = B * B - 4. * A * C;
+    if (!(disc >= 0.)) return;
+    const double sd = sqrt(disc);
+    const double qq = -0.5 * (B + (B >= 0. ? sd : -sd));
+    const double ra = qq / A;
+    const double rb = qq != 0. ? C / qq : ra;
+    r[nr++] = ra < rb ? ra : rb;
+    r[nr++] = ra < rb ? rb : ra;
+  }
+  for (int j = 0; j < nr; ++j) {
+    const double u = r[j];
+    const double v = (kl - pl * u) / ql;
+    if (u >= -1. && u <= 1. && v >= -1. && v <= 1.) {
+      xi[0] = u;
+      xi[1] = v;
+      return;
+    }
+  }
+}
+
+// The isoparametric branch of transform_real_to_unit_cell: the closed form
+// (2D) with its early exit, the affine guess A^-1 (p - b) of the
+// least-squares affine map (A = X^T KA, b = X^T Kb with KA = unit vertex /
+// nn, Kb = 1 / nn) with its early exit, then Newton-Raphson from the better
+// of the two with a halving line search (at most 10000 iterations; the
+// iteration matrix is jacobian_local on the unit cell's own vertices,
+// cell.tcc:772). Returns false where the reference throws (NaN).
+template <int D>
+MPMX_DEV bool transform_real_to_unit_cell(const double (*X)[D], double mean_length,
+                                          const double* p, double* xi) {
+  constexpr int NN = Elem<D>::NN;
+  const double big = 1.7976931348623157e308;
+  double an_xi[D], an_res[D];
+#pragma unroll
+  for (int k = 0; k < D; ++k) an_xi[k] = an_res[k] = big;
+  const double tol2 = 1.0E-16 * mean_length * mean_length;
+  if constexpr (D == 2) {
+    q4_natural_coordinates(X, p, an_xi);
+    bool in_cell = true;
+#pragma unroll
+    for (int k = 0; k < D; ++k)
+      if (an_xi[k] < -1. || an_xi[k] > 1. || isnan(an_xi[k])) in_cell = false;
+    iso_residual<D>(X, an_xi, p, an_res);
+    if (sqnorm<D>(an_res) < tol2 && in_cell) {
+#pragma unroll
+      for (int k = 0; k < D; ++k) xi[k] = an_xi[k];
+      return true;
+    }
+  }
+  const double inv_nn = 1.0 / NN;
+  double A[D][D], Ai[D][D], b[D];
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+      double s = X[0][k] * (unit_vertex(0, a) * inv_nn);
+#pragma unroll
+      for (int i = 1; i < NN; ++i) s = s + X[i][k] * (unit_vertex(i, a) * inv_nn);
+      A[k][a] = s;
+    }
+    double s = X[0][k] * inv_nn;
+#pragma unroll
+    for (int i = 1; i < NN; ++i) s = s + X[i][k] * inv_nn;
+    b[k] = p[k] - s;
+  }
+  mat_inverse<D>(A, Ai);
+  double af_xi[D], af_res[D];
+  bool af_nan = false;
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    double s = Ai[k][0] * b[0];
+#pragma unroll
+    for (int a = 1; a < D; ++a) s = s + Ai[k][a] * b[a];
+    af_xi[k] = s;
+    af_res[k] = big;
+    if (isnan(s)) af_nan = true;
+  }
+  if (!af_nan) {
+    iso_residual<D>(X, af_xi, p, af_res);
+    if (sqnorm<D>(af_res) < tol2) {
+#pragma unroll
+      for (int k = 0; k < D; ++k) xi[k] = af_xi[k];
+      return true;
+    }
+  }
+  double g_xi[D], g_res[D];
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    g_xi[k] = 0.;
+    g_res[k] = big;
+  }
+  if (sqrt(sqnorm<D>(an_res)) < sqrt(sqnorm<D>(af_res))) {
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      g_xi[k] = an_xi[k];
+      g_res[k] = an_res[k];
+    }
+  } else if (!af_nan) {
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      g_xi[k] = af_xi[k];
+      g_res[k] = af_res[k];
+    }
+  }
+  double nr[D], nr_res[D];
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    nr[k] = g_xi[k];
+    if (nr[k] < -1. && nr[k] > -1.001)
+      nr[k] = -1. + 1.E-12;
+    else if (nr[k] > 1. && nr[k] < 1.001)
+      nr[k] = 1. - 1.E-12;
+    nr_res[k] = big;
+  }
+  for (int iter = 0; iter < 10000; ++iter) {
+    double J[D][D], Ji[D][D];
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+#pragma unroll
+      for (int c = 0; c < D; ++c) J[a][c] = 0.;
+    {
+      double g[D];
+      grad_i<D>(nr, 0, g);
+#pragma unroll
+      for (int a = 0; a < D; ++a)
+#pragma unroll
+        for (int c = 0; c < D; ++c) J[a][c] = g[a] * unit_vertex(0, c);
+#pragma unroll
+      for (int i = 1; i < NN; ++i) {
+        grad_i<D>(nr, i, g);
+#pragma unroll
+        for (int a = 0; a < D; ++a)
+#pragma unroll
+          for (int c = 0; c < D; ++c) J[a][c] = J[a][c] + g[a] * unit_vertex(i, c);
+      }
+    }
+    double det;
+    if constexpr (D == 2)
+      det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+    else
+      det = J[0][0] * (J[1][1] * J[2][2] - J[2][1] * J[1][2]) -
+            J[1][0] * (J[0][1] * J[2][2] - J[2][1] * J[0][2]) +
+            J[2][0] * (J[0][1] * J[1][2] - J[1][1] * J[0][2]);
+    if (fabs(det) < 1.0E-10) {
+#pragma unroll
+      for (int k = 0; k < D; ++k) nr[k] = 0.;
+    }
+    iso_residual<D>(X, nr, p, nr_res);
+    mat_inverse<D>(J, Ji);
+    double delta[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      double s = Ji[k][0] * nr_res[0];
+#pragma unroll
+      for (int a = 1; a < D; ++a) s = s + Ji[k][a] * nr_res[a];
+      delta[k] = s;
+    }
+    double step = 1.;
+    const double rn = sqrt(sqnorm<D>(nr_res));
+    for (int t = 0; t < 10; ++t) {
+      double trial[D], tres[D];
+#pragma unroll
+      for (int k = 0; k < D; ++k) trial[k] = nr[k] - step * delta[k];
+      iso_residual<D>(X, trial, p, tres);
+      if (sqrt(sqnorm<D>(tres)) < rn) {
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+          nr[k] = trial[k];
+          nr_res[k] = tres[k];
+        }
+        break;
+      } else if (step > 0.05) {
+        step /= 2.;
+      } else {
+        break;
+      }
+    }
+    double sdel[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) sdel[k] = step * delta[k];
+    if (sqrt(sqnorm<D>(sdel)) < 1.0E-10) break;
+    if (isnan(nr[0]) || isnan(nr[1])) {
+#pragma unroll
+      for (int k = 0; k < D; ++k) nr[k] = 0.;
+    }
+  }
+  const bool geo = sqrt(sqnorm<D>(g_res)) < sqrt(sqnorm<D>(nr_res));
+#pragma unroll
+  for (int k = 0; k < D; ++k) xi[k] = geo ? g_xi[k] : nr[k];
+  return !(isnan(xi[0]) || isnan(xi[1]));
+}
+
+// --------------------------- locate (cell.tcc:324-380) ----------------------
+// ISO: the mesh is isoparametric (local coordinates by
+// transform_real_to_unit_cell); else Cartesian (local_coordinates_point).
+template <int D, bool ISO = false>
 MPMX_DEV bool point_in_cell(const Mesh& M, int c, const double* p, double* xi) {
   {
     double s = 0.0;
@@ -167,9 +448,19 @@ MPMX_DEV bool point_in_cell(const Mesh& M, int c, const double* p, double* xi) {
     if (!(sqrt(s) < M.two_mean[c])) return false;
   }
   bool status = true;
+  if constexpr (ISO) {
+    constexpr int NN = Elem<D>::NN;
+    double X[NN][D];
+    load_cell_X<D>(M, c, X);
+    if (!transform_real_to_unit_cell<D>(X, M.two_mean[c] * 0.5, p, xi)) return false;
+  }
 #pragma unroll
   for (int k = 0; k < D; ++k) {
-    double x = 2. * (p[k] - M.centroid[k * M.ncells + c]) / M.len[k * M.ncells + c];
+    double x;
+    if constexpr (ISO)
+      x = xi[k];
+    else
+      x = 2. * (p[k] - M.centroid[k * M.ncells + c]) / M.len[k * M.ncells + c];
     if (x < -1. || x > 1. || isnan(x)) {
       status = false;
     } else {
@@ -740,8 +1031,36 @@ MPMX_DEV double math_fn_value(const MathFn& f, const double* data, double x) {
 
 // ------------------------------ node update ---------------------------------
 // node.tcc:370-404 (Cartesian branch)
+// y = A x for the leading DxD block of a 3x3 matrix (sequential over k)
+template <int D>
+MPMX_DEV void rot_apply(const double (*A)[3], const double* x, double* y) {
+#pragma unroll
+  for (int i = 0; i < D; ++i) {
+    double s = A[i][0] * x[0];
+#pragma unroll
+    for (int k = 1; k < D; ++k) s = s + A[i][k] * x[k];
+    y[i] = s;
+  }
+}
+
 template <int D>
 MPMX_DEV void apply_velocity_constraints(const NodeCons& c, double* v, double* a) {
+  if (c.rot) {
+    // general boundary: per constraint to the local frame and back
+    // (node.tcc:385-401)
+#pragma unroll
+    for (int k = 0; k < D; ++k)
+      if (c.vmask & (1 << k)) {
+        double lv[D], la[D];
+        rot_apply<D>(c.Ri, v, lv);
+        rot_apply<D>(c.Ri, a, la);
+        lv[k] = c.vval[k];
+        la[k] = 0.;
+        rot_apply<D>(c.R, lv, v);
+        rot_apply<D>(c.R, la, a);
+      }
+    return;
+  }
 #pragma unroll
   for (int k = 0; k < D; ++k)
     if (c.vmask & (1 << k)) {
@@ -759,9 +1078,19 @@ MPMX_DEV void apply_friction(const NodeCons& c, double* acc, const double* vel, 
   const double mu = c.fmu;
   if constexpr (D == 2) {
     const int dir_t = (D - 1) - dir_n;
-    const double acc_n = acc[dir_n];
-    double acc_t = acc[dir_t];
-    const double vel_t = vel[dir_t];
+    double acc_n, acc_t, vel_t;
+    if (!c.rot) {
+      acc_n = acc[dir_n];
+      acc_t = acc[dir_t];
+      vel_t = vel[dir_t];
+    } else {  // general boundary: in the local frame (node.tcc:540-556)
+      double la[D], lv[D];
+      rot_apply<D>(c.Ri, acc, la);
+      rot_apply<D>(c.Ri, vel, lv);
+      acc_n = la[dir_n];
+      acc_t = la[dir_t];
+      vel_t = lv[dir_t];
+    }
     if ((acc_n * sign_n) > 0.0) {
       if (vel_t != 0.0) {
         const double vel_net = dt * acc_t + vel_t;
@@ -776,11 +1105,35 @@ MPMX_DEV void apply_friction(const NodeCons& c, double* acc, const double* vel, 
         else
           acc_t -= sign(acc_t) * mu * fabs(acc_n);
       }
-      acc[dir_t] = acc_t;
+      if (!c.rot) {
+        acc[dir_t] = acc_t;
+      } else {
+        double la[D];
+        la[dir_t] = acc_t;
+        la[dir_n] = acc_n;
+        rot_apply<D>(c.R, la, acc);
+      }
     }
   } else {
     const int t0 = dir_n == 0 ? 1 : 0;
     const int t1 = dir_n == 2 ? 1 : 2;
+    double gacc[3], gvel[3];  // the global values on a general boundary
+    if (c.rot) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        gacc[k] = acc[k];
+        gvel[k] = vel[k];
+      }
+      double lacc[3], lvel[3];
+      rot_apply<3>(c.Ri, gacc, lacc);
+      rot_apply<3>(c.Ri, gvel, lvel);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        acc[k] = lacc[k];
+        gvel[k] = lvel[k];
+      }
+      vel = gvel;
+    }
     const double acc_n = acc[dir_n];
     double acc_t = sqrt(acc[t0] * acc[t0] + acc[t1] * acc[t1]);
     const double vel_t = sqrt(vel[t0] * vel[t0] + vel[t1] * vel[t1]);
@@ -807,6 +1160,13 @@ MPMX_DEV void apply_friction(const NodeCons& c, double* acc, const double* vel, 
           acc[t1] -= mu * fabs(acc_n) * (acc[t1] / acc_t);
         }
       }
+      if (c.rot) {
+        double la[3] = {acc[0], acc[1], acc[2]};
+        rot_apply<3>(c.R, la, acc);
+      }
+    } else if (c.rot) {  // no friction force: the global acceleration stands
+#pragma unroll
+      for (int k = 0; k < 3; ++k) acc[k] = gacc[k];
     }
   }
 }
@@ -875,8 +1235,17 @@ MPMX_DEV bool node_acc_vel(double mass, const double* fext, const double* fint, 
     if (c->fdir >= 0) apply_friction<D>(*c, a, v, dt);
 #pragma unroll
     for (int k = 0; k < D; ++k)
-      if (c->amask & (1 << k))  // acceleration_constraint.h:37-43
-        a[k] = c->afn[k] >= 0 ? c->aval[k] * fnval[c->afn[k]] : c->aval[k];
+      if (c->amask & (1 << k)) {  // acceleration_constraint.h:37-43
+        const double val = c->afn[k] >= 0 ? c->aval[k] * fnval[c->afn[k]] : c->aval[k];
+        if (!c->rot) {
+          a[k] = val;
+        } else {  // general boundary (node.tcc:469-482)
+          double la[D];
+          rot_apply<D>(c->Ri, a, la);
+          la[k] = val;
+          rot_apply<D>(c->R, la, a);
+        }
+      }
   }
 #pragma unroll
   for (int k = 0; k < D; ++k) v[k] += a[k] * dt;

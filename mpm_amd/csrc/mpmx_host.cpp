@@ -74,6 +74,8 @@ struct HostCons {
   int cfn = -1;
   int fdir = -1, fsign = 0;
   double fmu = 0;
+  bool rot = false;  // general boundary: constraints in the frame of R
+  double R[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
 };
 
 struct KernelTime {
@@ -99,6 +101,9 @@ struct mpmx_ctx {
   std::vector<double> cell_volume;
   DBuf<int> d_cell_nodes, d_nb_ptr, d_nb_idx, d_node_cells;
   DBuf<double> d_node_x, d_centroid, d_len, d_two_mean, d_dndx_c;
+  // isoparametric mesh: local coordinates by transform_real_to_unit_cell
+  // (cell.tcc:617-828), located by a kernel of its own each step
+  bool iso = false;
   // instances (batched runs): cell ranges, cell -> instance
   int64_t ninst = 0;
   DBuf<int> d_cell_inst;
@@ -234,6 +239,7 @@ struct mpmx_ctx {
     M.node_cells = d_node_cells.p;
     M.cell_inst = ninst > 0 ? d_cell_inst.p : nullptr;
     M.inst_begin = ninst > 0 ? d_inst_begin.p : nullptr;
+    M.iso = iso ? 1 : 0;
     return M;
   }
   Particles parts() const {
@@ -564,6 +570,64 @@ void require_mesh(mpmx_ctx* c) {
   if (c->nnodes == 0) throw ApiError(MPMX_E_STATE, "mesh is not set");
 }
 
+// Inverse of the leading dim x dim block by the adjugate over the
+// determinant (cofactor expansion along the first column), what
+// rotation_matrix_.inverse() computes (node.tcc:388-389).
+void matrix_inverse(int dim, const double (*J)[3], double (*Ji)[3]) {
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) Ji[i][j] = i == j ? 1. : 0.;
+  if (dim == 2) {
+    const double det = J[0][0] * J[1][1] - J[1][0] * J[0][1];
+    const double invdet = 1.0 / det;
+    Ji[0][0] = J[1][1] * invdet;
+    Ji[1][0] = -J[1][0] * invdet;
+    Ji[0][1] = -J[0][1] * invdet;
+    Ji[1][1] = J[0][0] * invdet;
+    return;
+  }
+  double cf[3][3];
+  cf[0][0] = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+  cf[1][0] = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+  cf[2][0] = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+  cf[0][1] = J[2][1] * J[0][2] - J[2][2] * J[0][1];
+  cf[1][1] = J[2][2] * J[0][0] - J[2][0] * J[0][2];
+  cf[2][1] = J[2][0] * J[0][1] - J[2][1] * J[0][0];
+  cf[0][2] = J[0][1] * J[1][2] - J[0][2] * J[1][1];
+  cf[1][2] = J[0][2] * J[1][0] - J[0][0] * J[1][2];
+  cf[2][2] = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+  const double det = (J[0][0] * cf[0][0] + J[1][0] * cf[0][1]) + J[2][0] * cf[0][2];
+  const double invdet = 1.0 / det;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) Ji[i][j] = cf[i][j] * invdet;
+}
+
+// mpm::geometry::rotation_matrix (src/geometry.cc:6-56) of Euler angles
+// (alpha, beta) in 2D, (alpha, beta, gamma) in 3D.
+void euler_rotation(int dim, const double* ang, double R[3][3]) {
+  const double ca = dmath::cos(ang[0]), sa = dmath::sin(ang[0]);
+  const double cb = dmath::cos(ang[1]), sb = dmath::sin(ang[1]);
+  const double cacb = ca * cb, casb = ca * sb, sacb = sa * cb, sasb = sa * sb;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) R[i][j] = i == j ? 1. : 0.;
+  if (dim == 2) {
+    R[0][0] = cacb - sasb;
+    R[0][1] = -casb - sacb;
+    R[1][0] = sacb + casb;
+    R[1][1] = -sasb + cacb;
+    return;
+  }
+  const double sg = dmath::sin(ang[2]), cg = dmath::cos(ang[2]);
+  R[0][0] = cacb - sasb * cg;
+  R[0][1] = -casb - sacb * cg;
+  R[0][2] = sa * sg;
+  R[1][0] = sacb + casb * cg;
+  R[1][1] = -sasb + cacb * cg;
+  R[1][2] = -(ca * sg);
+  R[2][0] = sb * sg;
+  R[2][1] = cb * sg;
+  R[2][2] = cg;
+}
+
 void upload_constraints(mpmx_ctx* c) {
   std::vector<int> idx(c->nnodes, -1);
   std::vector<NodeCons> table;
@@ -590,6 +654,10 @@ void upload_constraints(mpmx_ctx* c) {
       nc.fsign = kv.second.fsign > 0 ? 1. : -1.;
       nc.fmu = kv.second.fmu;
     }
+    nc.rot = kv.second.rot ? 1 : 0;
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) nc.R[i][j] = kv.second.R[i][j];
+    if (kv.second.rot) matrix_inverse(c->dim, kv.second.R, nc.Ri);
     idx[kv.first] = (int)table.size();
     table.push_back(nc);
   }
@@ -959,6 +1027,9 @@ void enqueue_step(mpmx_ctx* c, const StepArgs& A, Launcher& L, bool full_sort,
   int* perm = B.perm;
   int* scan = B.scan;
   const bool ordered = (c->flags & MPMX_ORDERED) != 0;
+  // Cartesian meshes are located inside the last particle kernel of the step;
+  // isoparametric ones by a kernel of their own after it
+  const bool loc = !c->iso;
   if (!c->fns.empty())
     L("time_functions", [&] {
       launch_fn_eval(c->d_fns.p, c->d_fn_data.p, (int)c->fns.size(), S, A.dt, B.fnval, st);
@@ -984,16 +1055,16 @@ void enqueue_step(mpmx_ctx* c, const StepArgs& A, Launcher& L, bool full_sort,
       L("stress", [&] { launch_stress<D>(false, M, P, Nd, mats, A, S, scan, st); });
       plastic();
       L("gather_forces", [&] { launch_gather<D>(GK_F_ACC, M, P, Nd, A, S, perm, cs, ce, st); });
-      L("g2p_locate", [&] { launch_g2p<D>(false, true, M, P, Nd, mats, A, c->vu, S, scan, st); });
+      L("g2p_locate", [&] { launch_g2p<D>(false, loc, M, P, Nd, mats, A, c->vu, S, scan, st); });
     } else if (c->scheme == USL) {
       L("gather_all", [&] { launch_gather<D>(GK_ALL_VEL_ACC, M, P, Nd, A, S, perm, cs, ce, st); });
-      L("g2p_stress_locate", [&] { launch_g2p<D>(true, true, M, P, Nd, mats, A, c->vu, S, scan, st); });
+      L("g2p_stress_locate", [&] { launch_g2p<D>(true, loc, M, P, Nd, mats, A, c->vu, S, scan, st); });
       plastic();
     } else {
       L("gather_all", [&] { launch_gather<D>(GK_ALL_VEL_ACC, M, P, Nd, A, S, perm, cs, ce, st); });
       L("g2p", [&] { launch_g2p<D>(false, false, M, P, Nd, mats, A, c->vu, S, scan, st); });
       L("gather_momentum", [&] { launch_gather<D>(GK_MOM_VEL, M, P, Nd, A, S, perm, cs, ce, st); });
-      L("stress_locate", [&] { launch_stress<D>(true, M, P, Nd, mats, A, S, scan, st); });
+      L("stress_locate", [&] { launch_stress<D>(loc, M, P, Nd, mats, A, S, scan, st); });
       plastic();
     }
   } else {
@@ -1003,12 +1074,12 @@ void enqueue_step(mpmx_ctx* c, const StepArgs& A, Launcher& L, bool full_sort,
       L("stress_force_scatter", [&] { launch_stress_scatter<D>(c->chunk, M, P, Nd, mats, A, S, skey, perm, st); });
       if (P.ntr > 0) L("traction_scatter", [&] { launch_traction_scatter<D>(M, P, Nd, S, st); });
       L("node_acceleration", [&] { launch_node<D>(ND_ACC | ND_ZERO_ALL, M, Nd, A, S, st); });
-      L("g2p_locate", [&] { launch_g2p<D>(false, true, M, P, Nd, mats, A, c->vu, S, scan, st); });
+      L("g2p_locate", [&] { launch_g2p<D>(false, loc, M, P, Nd, mats, A, c->vu, S, scan, st); });
     } else if (c->scheme == USL) {
       L("p2g_all", [&] { launch_p2g<D>(P2G_MV_F, c->chunk, M, P, Nd, A, S, skey, perm, st); });
       if (P.ntr > 0) L("traction_scatter", [&] { launch_traction_scatter<D>(M, P, Nd, S, st); });
       L("node_update", [&] { launch_node<D>(ND_INIT | ND_VEL | ND_ACC | ND_ZERO_ALL, M, Nd, A, S, st); });
-      L("g2p_stress_locate", [&] { launch_g2p<D>(true, true, M, P, Nd, mats, A, c->vu, S, scan, st); });
+      L("g2p_stress_locate", [&] { launch_g2p<D>(true, loc, M, P, Nd, mats, A, c->vu, S, scan, st); });
       plastic();
     } else {
       L("p2g_all", [&] { launch_p2g<D>(P2G_MV_F, c->chunk, M, P, Nd, A, S, skey, perm, st); });
@@ -1016,10 +1087,11 @@ void enqueue_step(mpmx_ctx* c, const StepArgs& A, Launcher& L, bool full_sort,
       L("node_update", [&] { launch_node<D>(ND_INIT | ND_VEL | ND_ACC | ND_ZERO_MOM, M, Nd, A, S, st); });
       L("g2p_momentum_scatter", [&] { launch_g2p_scatter<D>(c->chunk, M, P, Nd, A, c->vu, S, skey, perm, st); });
       L("node_velocity", [&] { launch_node<D>(ND_VEL | ND_ZERO_ALL, M, Nd, A, S, st); });
-      L("stress_locate", [&] { launch_stress<D>(true, M, P, Nd, mats, A, S, scan, st); });
+      L("stress_locate", [&] { launch_stress<D>(loc, M, P, Nd, mats, A, S, scan, st); });
       plastic();
     }
   }
+  if (!loc) L("locate", [&] { launch_locate_step<D>(M, P, S, scan, st); });
   L("locate_scan", [&] { launch_locate_scan<D>(M, P, S, scan, nullptr, c->locate_all ? 0 : 1, st); });
   L("step_end", [&] { launch_step_end(S, st); });
 }
@@ -1355,12 +1427,14 @@ AdjParticles ad_parts(mpmx_ctx* c) {
 }
 int64_t ad_part_doubles(const mpmx_ctx* c) { return (3 * c->dim + 10) * c->np; }
 
-// What the adjoint differentiates: Cartesian cells with the reference's local
-// node order, no friction constraints, no strain softening, no particle
+// What the adjoint differentiates: Cartesian cells (not isoparametric) with
+// the reference's local node order, no friction constraints, no strain softening, no particle
 // tractions (they depend on the particle's volume and position). Nodal
 // forces and acceleration constraints, with or without math functions, are
 // constants of the step and are supported.
 void ad_check_supported(mpmx_ctx* c) {
+  if (c->iso)
+    throw ApiError(MPMX_E_UNSUPPORTED, "adjoint: isoparametric meshes are not supported");
   if (!c->trac.empty())
     throw ApiError(MPMX_E_UNSUPPORTED, "adjoint: particle tractions are not supported");
   for (const auto& m : c->mats) {
@@ -1369,9 +1443,13 @@ void ad_check_supported(mpmx_ctx* c) {
     if (m.type == LE && m.rate != RATE_NONE)
       throw ApiError(MPMX_E_UNSUPPORTED, "adjoint: objective stress rates are not supported");
   }
-  for (const auto& kv : c->cons)
+  for (const auto& kv : c->cons) {
     if (kv.second.fdir >= 0)
       throw ApiError(MPMX_E_UNSUPPORTED, "adjoint: friction constraints are not supported");
+    if (kv.second.rot)
+      throw ApiError(MPMX_E_UNSUPPORTED, "adjoint: constraints on general boundaries are not "
+                                         "supported");
+  }
   const int d = c->dim, nn = c->nn;
   for (int64_t e = 0; e < c->ncells; ++e) {
     double lo[3], hi[3];
@@ -1487,9 +1565,6 @@ const char* mpmx_last_error(const mpmx_ctx* c) { return c ? c->error.c_str() : "
 int mpmx_set_mesh(mpmx_ctx* c, int64_t nnodes, const double* coords, int64_t ncells,
                   const int64_t* cell_nodes, int isoparametric) {
   API_BEGIN(c)
-  if (isoparametric)
-    throw ApiError(MPMX_E_UNSUPPORTED,
-                   "isoparametric cells (cell.tcc:617-828) are not supported; use isoparametric: false");
   if (nnodes <= 0 || ncells <= 0 || !coords || !cell_nodes) throw ApiError(MPMX_E_ARG, "empty mesh");
   if (nnodes > std::numeric_limits<int>::max() / 8 || ncells > std::numeric_limits<int>::max() / 8)
     throw ApiError(MPMX_E_ARG, "mesh too large for 32-bit ids");
@@ -1498,6 +1573,8 @@ int mpmx_set_mesh(mpmx_ctx* c, int64_t nnodes, const double* coords, int64_t nce
   c->nnodes = nnodes;
   c->ncells = ncells;
   c->ninst = 0;
+  c->iso = isoparametric != 0;
+  c->invalidate_graph();
   c->node_x.assign(coords, coords + nnodes * d);
   c->cell_nodes.resize(ncells * nn);
   for (int64_t i = 0; i < ncells * nn; ++i) {
@@ -1818,6 +1895,21 @@ int mpmx_set_particle_velocity_constraints(mpmx_ctx* c, int64_t n, const int64_t
   build_pvc(c);
   c->invalidate_graph();
   c->ad_recorded = false;
+  API_END(c)
+}
+
+int mpmx_set_nodal_euler_angles(mpmx_ctx* c, int64_t n, const int64_t* node,
+                                const double* angles) {
+  API_BEGIN(c)
+  require_mesh(c);
+  if (n < 0 || (n > 0 && (!node || !angles))) throw ApiError(MPMX_E_ARG, "null arrays");
+  for (int64_t i = 0; i < n; ++i) {
+    if (node[i] < 0 || node[i] >= c->nnodes) throw ApiError(MPMX_E_ARG, "node id out of range");
+    HostCons& h = c->cons[node[i]];
+    euler_rotation(c->dim, angles + i * c->dim, h.R);
+    h.rot = true;
+  }
+  upload_constraints(c);
   API_END(c)
 }
 

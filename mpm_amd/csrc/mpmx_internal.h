@@ -51,6 +51,12 @@ struct NodeCons {
   int has_cf;
   int cfn;
   double cf[3];
+  // general (inclined) boundary (Node::assign_rotation_matrix,
+  // node.h:281-285): the constraints act in the frame of R (row-major) and
+  // Ri is its inverse; rot = 0: Cartesian
+  int rot;
+  double R[3][3];
+  double Ri[3][3];
 };
 
 // A math function of time-dependent boundary conditions: Linear (a table of
@@ -79,6 +85,9 @@ struct Mesh {
   // Null for a single mesh.
   const int* cell_inst;
   const int64_t* inst_begin;
+  // isoparametric mesh (the reference's "isoparametric" mesh key): local
+  // coordinates by the inverse of the element map, else the Cartesian formula
+  int iso;
 };
 
 struct Particles {

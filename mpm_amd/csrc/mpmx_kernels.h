@@ -17,6 +17,11 @@ enum { GK_MV_VEL = 0, GK_F_ACC = 1, GK_ALL_VEL_ACC = 2, GK_MOM_VEL = 3 };
 template <int D>
 void launch_locate_init(const Mesh& M, const Particles& P, Status* S, int* scan_list,
                         const int* hint, hipStream_t st);
+// Per-step locate of an isoparametric mesh (current cell, neighbours, else the
+// scan queue); the step kernels locate Cartesian meshes themselves.
+template <int D>
+void launch_locate_step(const Mesh& M, const Particles& P, Status* S, int* scan_list,
+                        hipStream_t st);
 // remove_lost: a particle found in no cell is counted in Status::nlost for
 // removal instead of stopping the run (Status::first_bad_step).
 template <int D>

@@ -317,12 +317,12 @@ MPMX_DEV void particle_dndx(const Mesh& M, int c, const double* xi, double (*dnd
 
 // locate_particle_cells (mesh.tcc:1116-1153): current cell, then its
 // neighbours in ascending id; otherwise queue the particle for the full scan.
-template <int D>
+template <int D, bool ISO = false>
 MPMX_DEV void locate_one(const Mesh& M, const Particles& P, Status* S, int* scan_list, int p,
                          const double* x, int cur, int hint) {
   const int64_t np = P.np;
   double xi[D];
-  if (cur >= 0 && point_in_cell<D>(M, cur, x, xi)) {
+  if (cur >= 0 && point_in_cell<D, ISO>(M, cur, x, xi)) {
 #pragma unroll
     for (int k = 0; k < D; ++k) P.xi[k * np + p] = xi[k];
     return;
@@ -332,7 +332,7 @@ MPMX_DEV void locate_one(const Mesh& M, const Particles& P, Status* S, int* scan
     const int e = M.nb_ptr[base + 1];
     for (int j = M.nb_ptr[base]; j < e; ++j) {
       const int nb = M.nb_idx[j];
-      if (point_in_cell<D>(M, nb, x, xi)) {
+      if (point_in_cell<D, ISO>(M, nb, x, xi)) {
         P.cell[p] = nb;
 #pragma unroll
         for (int k = 0; k < D; ++k) P.xi[k * np + p] = xi[k];
@@ -358,14 +358,35 @@ __global__ __launch_bounds__(kBlock) void k_locate_init(Mesh M, Particles P, Sta
   for (int k = 0; k < D; ++k) x[k] = P.x[k * P.np + p];
   const int h = hint ? hint[p] : -1;
   double xi[D];
-  if (h >= 0 && point_in_cell<D>(M, h, x, xi)) {
+  const bool in = h >= 0 && (M.iso ? point_in_cell<D, true>(M, h, x, xi)
+                                   : point_in_cell<D, false>(M, h, x, xi));
+  if (in) {
     P.cell[p] = h;
 #pragma unroll
     for (int k = 0; k < D; ++k) P.xi[k * P.np + p] = xi[k];
     return;
   }
   P.cell[p] = -1;
-  locate_one<D>(M, P, S, scan_list, (int)p, x, -1, h);
+  if (M.iso)
+    locate_one<D, true>(M, P, S, scan_list, (int)p, x, -1, h);
+  else
+    locate_one<D, false>(M, P, S, scan_list, (int)p, x, -1, h);
+}
+
+// Per-step locate of an isoparametric mesh (the step kernels locate Cartesian
+// meshes themselves): current cell, then neighbours, else the scan queue.
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_locate_step(Mesh M, Particles P, Status* S,
+                                                        int* scan_list) {
+  if (run_stopped(S)) return;
+  const int64_t p = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+  if (p >= P.np) return;
+  const int c = P.cell[p];
+  if (!valid_cell<D>(M, c)) return;
+  double x[D];
+#pragma unroll
+  for (int k = 0; k < D; ++k) x[k] = P.x[k * P.np + p];
+  locate_one<D, true>(M, P, S, scan_list, (int)p, x, c, -1);
 }
 
 // Full-mesh scan for queued particles: first cell in ascending id order that
@@ -403,7 +424,9 @@ __global__ __launch_bounds__(kBlock) void k_locate_scan(Mesh M, Particles P, Sta
     for (int64_t c0 = lo; c0 < hi; c0 += kBlock) {
       const int64_t c = c0 + threadIdx.x;
       double xi[D];
-      if (c < hi && point_in_cell<D>(M, (int)c, x, xi)) atomicMin(&best, (int)c);
+      const bool in = c < hi && (M.iso ? point_in_cell<D, true>(M, (int)c, x, xi)
+                                       : point_in_cell<D, false>(M, (int)c, x, xi));
+      if (in) atomicMin(&best, (int)c);
       __syncthreads();
       if (best != 0x7fffffff) break;
       __syncthreads();
@@ -411,7 +434,10 @@ __global__ __launch_bounds__(kBlock) void k_locate_scan(Mesh M, Particles P, Sta
     if (threadIdx.x == 0) {
       if (best != 0x7fffffff) {
         double xi[D];
-        point_in_cell<D>(M, best, x, xi);
+        if (M.iso)
+          point_in_cell<D, true>(M, best, x, xi);
+        else
+          point_in_cell<D, false>(M, best, x, xi);
         P.cell[p] = best;
 #pragma unroll
         for (int k = 0; k < D; ++k) P.xi[k * P.np + p] = xi[k];
@@ -1375,6 +1401,12 @@ void launch_locate_init(const Mesh& M, const Particles& P, Status* S, int* scan_
 }
 
 template <int D>
+void launch_locate_step(const Mesh& M, const Particles& P, Status* S, int* scan_list,
+                        hipStream_t st) {
+  MPMX_LAUNCH((k_locate_step<D>), P.np, st, M, P, S, scan_list);
+}
+
+template <int D>
 void launch_locate_scan(const Mesh& M, const Particles& P, Status* S, const int* scan_list,
                         const int* hint, int remove_lost, hipStream_t st) {
   hipLaunchKernelGGL((k_locate_scan<D>), dim3(256), dim3(kBlock), 0, st, M, P, S, scan_list,
@@ -1462,6 +1494,10 @@ void launch_g2p(bool stress, bool locate, const Mesh& M, const Particles& P, con
     MPMX_LAUNCH((k_g2p<D, true, true, true>), P.np, st, M, P, Nd, mats, A, vu, S, scan_list);
   else if (stress && locate)
     MPMX_LAUNCH((k_g2p<D, true, true>), P.np, st, M, P, Nd, mats, A, vu, S, scan_list);
+  else if (stress && P.spin)  // isoparametric meshes locate in a kernel of their own
+    MPMX_LAUNCH((k_g2p<D, true, false, true>), P.np, st, M, P, Nd, mats, A, vu, S, scan_list);
+  else if (stress)
+    MPMX_LAUNCH((k_g2p<D, true, false>), P.np, st, M, P, Nd, mats, A, vu, S, scan_list);
   else if (locate)
     MPMX_LAUNCH((k_g2p<D, false, true>), P.np, st, M, P, Nd, mats, A, vu, S, scan_list);
   else
@@ -1538,6 +1574,8 @@ template void take_rows<int>(const int*, int*, const int*, int64_t, int64_t, int
                                       const int*, hipStream_t);                              \
   template void launch_locate_scan<D>(const Mesh&, const Particles&, Status*, const int*,    \
                                       const int*, int, hipStream_t);                         \
+  template void launch_locate_step<D>(const Mesh&, const Particles&, Status*, int*,          \
+                                      hipStream_t);                                          \
   template void launch_p2g<D>(int, int, const Mesh&, const Particles&, const Nodes&,         \
                               const StepArgs&, const Status*, const int*, const int*,        \
                               hipStream_t);                                                  \

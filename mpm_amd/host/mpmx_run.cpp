@@ -226,6 +226,7 @@ class MPMExplicit {
   double gravity_[3] = {0, 0, 0};
   std::vector<double> coords_;
   std::vector<int64_t> cells_;
+  bool isoparametric_ = true;  // the mesh key "isoparametric"
   std::map<int64_t, std::vector<int64_t>> node_sets_, cell_sets_, particle_sets_;
   std::vector<double> mat_params_;
   std::map<int64_t, int> mat_index_;
@@ -335,8 +336,9 @@ void MPMExplicit<Tdim>::initialise_materials() {
 template <unsigned Tdim>
 void MPMExplicit<Tdim>::initialise_mesh() {
   const Json& mp = io_->object("mesh");
-  if (mp.contains("isoparametric") && mp.at("isoparametric").boolean())
-    throw std::runtime_error("isoparametric cells are not supported; set \"isoparametric\": false");
+  // MPMBase::is_isoparametric (mpm_base.tcc:768-783): true unless the key says
+  // otherwise
+  isoparametric_ = mp.contains("isoparametric") ? mp.at("isoparametric").boolean() : true;
   read_mesh(io_->file_name(mp.at("mesh").str()), Tdim, coords_, cells_, NN);
   if (mp.contains("entity_sets")) {
     const std::string es = io_->file_name(mp.at("entity_sets").str());
@@ -937,7 +939,7 @@ bool MPMExplicit<Tdim>::solve() {
         "create solver");
   check(ctx_, mpmx_set_materials(ctx_, (int)mat_index_.size(), mat_params_.data()), "materials");
   check(ctx_, mpmx_set_mesh(ctx_, (int64_t)(coords_.size() / Tdim), coords_.data(),
-                            (int64_t)(cells_.size() / NN), cells_.data(), 0),
+                            (int64_t)(cells_.size() / NN), cells_.data(), isoparametric_ ? 1 : 0),
         "mesh");
   if (!vnode_.empty())
     check(ctx_, mpmx_set_velocity_constraints(ctx_, (int64_t)vnode_.size(), vnode_.data(),

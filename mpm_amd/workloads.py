@@ -77,10 +77,12 @@ class Workload:
     # Steps the deck can run before a particle leaves the mesh (None: all
     # nsteps). The reference would stop there too (mpm_scheme.tcc:250-256).
     runnable_steps: Optional[int] = None
+    # the mesh's "isoparametric" key: locate by the inverse element map
+    isoparametric: bool = False
 
     def setup(self, solver):
         """Apply this deck to a solver (mesh, materials, constraints, particles)."""
-        solver.set_mesh(self.coords, self.cells)
+        solver.set_mesh(self.coords, self.cells, isoparametric=self.isoparametric)
         solver.set_materials(self.materials)
         if len(self.vel_nodes):
             solver.set_velocity_constraints(self.vel_nodes, self.vel_dirs, self.vel_values)

@@ -67,7 +67,9 @@ def _lib(omp=False, libm="portable"):
                    "orc_assign_stresses", "orc_set_particle_velocity_constraints",
                    "orc_set_acceleration_constraints_fn", "orc_set_math_functions",
                    "orc_set_nodal_forces", "orc_set_particle_tractions",
-                   "orc_assign_materials", "orc_compute_volume_mass"):
+                   "orc_assign_materials", "orc_compute_volume_mass", "orc_set_isoparametric",
+                   "orc_kat_transform_real_to_unit_cell", "orc_kat_point_in_cell",
+                   "orc_set_nodal_euler_angles"):
             getattr(lib, fn).restype = C.c_int
         lib.orc_kat_cell_volume.restype = C.c_double
         lib.orc_kat_mc_yield.restype = C.c_int
@@ -159,10 +161,11 @@ class Oracle:
     def threads(self):
         return self.lib.orc_threads()
 
-    def set_mesh(self, coords, cells):
+    def set_mesh(self, coords, cells, isoparametric=False):
         coords = _f64(coords)
         cells = np.ascontiguousarray(cells, dtype=np.int64)
         self.nnodes, self.ncells = coords.shape[0], cells.shape[0]
+        self._check(self.lib.orc_set_isoparametric(self.ctx, int(bool(isoparametric))))
         self._check(self.lib.orc_set_mesh(self.ctx, C.c_int64(self.nnodes), _p(coords),
                                           C.c_int64(self.ncells), _p(cells, _I64)))
 
@@ -254,6 +257,14 @@ class Oracle:
 
     def set_locate_particles(self, locate_particles=True):
         self.lib.orc_set_locate_particles(self.ctx, 1 if locate_particles else 0)
+
+    def set_nodal_euler_angles(self, nodes, angles):
+        """General (inclined) boundaries: the constraints of these nodes act in
+        the frame rotated by the Euler angles (rows of dim)."""
+        nodes = np.ascontiguousarray(nodes, dtype=np.int64)
+        ang = _f64(angles).reshape(len(nodes), -1)
+        self._check(self.lib.orc_set_nodal_euler_angles(self.ctx, C.c_int64(len(nodes)),
+                                                        _p(nodes, _I64), _p(ang)))
 
     def set_gravity(self, g, ramping_time=0.0):
         g = _f64(g)
@@ -395,6 +406,23 @@ def jacobian(dim, xi, X):
     return J
 
 
+def transform_real_to_unit_cell(dim, X, point):
+    """Isoparametric local coordinates of `point` in the one cell with
+    vertices X (nn x dim); None when the reference would throw (NaN)."""
+    X, point = _f64(X), _f64(point)
+    xi = np.zeros(dim)
+    rc = _lib().orc_kat_transform_real_to_unit_cell(dim, _p(X), _p(point), _p(xi))
+    return None if rc != 0 else xi
+
+
+def point_in_cell(dim, X, point, isoparametric=True):
+    """(inside, xi) of is_point_in_cell for the one cell with vertices X."""
+    X, point = _f64(X), _f64(point)
+    xi = np.zeros(dim)
+    r = _lib().orc_kat_point_in_cell(dim, int(bool(isoparametric)), _p(X), _p(point), _p(xi))
+    return bool(r), xi
+
+
 def cell_volume(dim, X):
     return _lib().orc_kat_cell_volume(dim, _p(_f64(X)))
 
@@ -472,6 +500,32 @@ def node_update(dim, mass, fext, fint, vel, acc, dt, damping=0.0, vel_constraint
                                  _p(a), C.c_double(dt), C.c_double(damping), _p(vc), _p(ac),
                                  int(fd), int(fs), C.c_double(fm))
     return bool(st), v, a
+
+
+def node_constraints(dim, vel, acc, angles=None, velocity=(), acceleration=(), friction=None,
+                     dt=0.0):
+    """Constraints of one node (general boundary when `angles`, the Euler
+    angles, are given): velocity [(dir, value)] -> apply_velocity_constraints,
+    acceleration [(dir, value)] -> apply_acceleration_constraints, friction
+    (dir, sign_n, mu) -> apply_friction_constraints(dt). Returns (vel, acc)."""
+    v, a = _f64(vel).copy(), _f64(acc).copy()
+    ang = None if angles is None else _f64(angles)
+    vd = np.array([d for d, _ in velocity] or [0], dtype=np.int32)
+    vv = np.array([x for _, x in velocity] or [0.0])
+    ad = np.array([d for d, _ in acceleration] or [0], dtype=np.int32)
+    av = np.array([x for _, x in acceleration] or [0.0])
+    which = 0 if velocity else (1 if acceleration else 2)
+    fd, fs, fm = friction if friction is not None else (0, 0, 0.0)
+    _lib().orc_kat_node_constraints(dim, _p(ang), _p(v), _p(a), len(velocity), _p(vd, _I32),
+                                    _p(vv), len(acceleration), _p(ad, _I32), _p(av), int(fd),
+                                    int(fs), C.c_double(fm), C.c_double(dt), which)
+    return v, a
+
+
+def rotation_matrix(dim, angles):
+    R = np.zeros((dim, dim))
+    _lib().orc_kat_rotation_matrix(dim, _p(_f64(angles)), _p(R))
+    return R
 
 
 def friction(dim, acc, vel, dt, direction, sign_n, mu):

@@ -704,6 +704,40 @@ static void inverse(int dim, const double J[3][3], double Ji[3][3]) {
   }
 }
 
+// mpm::geometry::rotation_matrix (src/geometry.cc:6-56): the rotation of
+// Euler angles (alpha, beta) in 2D, (alpha, beta, gamma) in 3D.
+static void rotation_matrix(int dim, const double* ang, double R[3][3]) {
+  const double ca = mth::cos(ang[0]), sa = mth::sin(ang[0]);
+  const double cb = mth::cos(ang[1]), sb = mth::sin(ang[1]);
+  const double cos_alpha_cos_beta = ca * cb;
+  const double cos_alpha_sin_beta = ca * sb;
+  const double sin_alpha_cos_beta = sa * cb;
+  const double sin_alpha_sin_beta = sa * sb;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) R[i][j] = i == j ? 1. : 0.;
+  if (dim == 2) {
+    R[0][0] = cos_alpha_cos_beta - sin_alpha_sin_beta;
+    R[0][1] = -cos_alpha_sin_beta - sin_alpha_cos_beta;
+    R[1][0] = sin_alpha_cos_beta + cos_alpha_sin_beta;
+    R[1][1] = -sin_alpha_sin_beta + cos_alpha_cos_beta;
+    return;
+  }
+  const double sg = mth::sin(ang[2]), cg = mth::cos(ang[2]);
+  const double cos_beta_sin_gamma = cb * sg;
+  const double sin_beta_sin_gamma = sb * sg;
+  const double cos_alpha_sin_gamma = ca * sg;
+  const double sin_alpha_sin_gamma = sa * sg;
+  R[0][0] = cos_alpha_cos_beta - sin_alpha_sin_beta * cg;
+  R[0][1] = -cos_alpha_sin_beta - sin_alpha_cos_beta * cg;
+  R[0][2] = sin_alpha_sin_gamma;
+  R[1][0] = sin_alpha_cos_beta + cos_alpha_sin_beta * cg;
+  R[1][1] = -sin_alpha_sin_beta + cos_alpha_cos_beta * cg;
+  R[1][2] = -cos_alpha_sin_gamma;
+  R[2][0] = sin_beta_sin_gamma;
+  R[2][1] = cos_beta_sin_gamma;
+  R[2][2] = cg;
+}
+
 // dn_dx (quadrilateral_element.tcc:295-311, hexahedron_element.tcc:307-321):
 // J = grad^T X (sequential over nodes), dN/dx = grad * (J^-1)^T.
 static void dn_dx(int dim, const double* xi, const double X[8][3], double out[8][3]) {
@@ -731,6 +765,242 @@ static inline double norm_diff(int dim, const double* a, const double* b) {
   double s = (a[0] - b[0]) * (a[0] - b[0]);
   for (int k = 1; k < dim; ++k) s = s + (a[k] - b[k]) * (a[k] - b[k]);
   return std::sqrt(s);
+}
+
+// ------------------ isoparametric local coordinates -----------------------
+// Cell::transform_real_to_unit_cell (cell.tcc:617-828) for quad4 / hex8.
+// Vertices of the unit cell (QuadrilateralElement / HexahedronElement
+// unit_cell_coordinates): quad counter-clockwise from (-1,-1); hex the same
+// square at zeta = -1, then at zeta = +1.
+static void unit_cell(int dim, double U[8][3]) {
+  static const double sq[4][2] = {{-1., -1.}, {1., -1.}, {1., 1.}, {-1., 1.}};
+  const int nn = dim == 2 ? 4 : 8;
+  for (int i = 0; i < nn; ++i) {
+    U[i][0] = sq[i % 4][0];
+    U[i][1] = sq[i % 4][1];
+    U[i][2] = i < 4 ? -1. : 1.;
+  }
+}
+
+// r = X^T N(xi) - p (nodal_coords * shapefn - point), sequential over nodes.
+static void iso_residual(int dim, const double X[8][3], const double* xi, const double* p,
+                         double* r) {
+  const int nn = dim == 2 ? 4 : 8;
+  double N[8];
+  shapefn(dim, xi, N);
+  for (int k = 0; k < dim; ++k) {
+    double s = X[0][k] * N[0];
+    for (int i = 1; i < nn; ++i) s = s + X[i][k] * N[i];
+    r[k] = s - p[k];
+  }
+}
+
+static inline double sqnorm(int dim, const double* v) {
+  double s = v[0] * v[0];
+  for (int k = 1; k < dim; ++k) s = s + v[k] * v[k];
+  return s;
+}
+
+// The bilinear map of a quad4 inverted in closed form
+// (QuadrilateralElement<2,4>::natural_coordinates_analytical,
+// quadrilateral_element.tcc:523-754, after Zhao et al. 1999). With
+// 4x = a1 + a2 xi + a3 eta + a4 xi eta (and b for y), the point solves
+// a2 xi + a3 eta + a4 xi eta = c1, b2 xi + b3 eta + b4 xi eta = c2. A
+// parallelogram (a4 = b4 = 0) is a linear system; otherwise eliminating the
+// product gives the line p xi + q eta = k, and substituting it a quadratic
+// in xi (q != 0) or xi itself (q = 0). The root with both coordinates in
+// [-1, 1] is returned (the one with the smaller xi first); none: DBL_MAX.
+// This is a derivation of the same closed form, not the reference's Sympy
+// expression: the two round differently in the last bits.
+static void q4_natural_coordinates(const double X[8][3], const double* pt, double* xi) {
+  const double big = std::numeric_limits<double>::max();
+  xi[0] = xi[1] = big;
+  const double x1 = X[0][0], y1 = X[0][1], x2 = X[1][0], y2 = X[1][1];
+  const double x3 = X[2][0], y3 = X[2][1], x4 = X[3][0], y4 = X[3][1];
+  const double a1 = x1 + x2 + x3 + x4, a2 = -x1 + x2 + x3 - x4;
+  const double a3 = -x1 - x2 + x3 + x4, a4 = x1 - x2 + x3 - x4;
+  const double b1 = y1 + y2 + y3 + y4, b2 = -y1 + y2 + y3 - y4;
+  const double b3 = -y1 - y2 + y3 + y4, b4 = y1 - y2 + y3 - y4;
+  const double c1 = 4. * pt[0] - a1, c2 = 4. * pt[1] - b1;
+  auto inside = [](double u, double v) { return u >= -1. && u <= 1. && v >= -1. && v <= 1.; };
+  if (a4 == 0. && b4 == 0.) {
+    const double det = a2 * b3 - a3 * b2;
+    xi[0] = (c1 * b3 - a3 * c2) / det;
+    xi[1] = (a2 * c2 - b2 * c1) / det;
+    return;
+  }
+  const double pl = a2 * b4 - a4 * b2, ql = a3 * b4 - a4 * b3, kl = c1 * b4 - c2 * a4;
+  if (ql == 0.) {
+    const double u = kl / pl;
+    const double den = a3 + a4 * u;
+    const double v = den != 0. ? (c1 - a2 * u) / den : (c2 - b2 * u) / (b3 + b4 * u);
+    if (inside(u, v)) {
+      xi[0] = u;
+      xi[1] = v;
+    }
+    return;
+  }
+  // A xi^2 + B xi + C = 0
+  const double A = -a4 * pl, B = a2 * ql + a4 * kl - a3 * pl, C = a3 * kl - c1 * ql;
+  double r[2];
+  int nr = 0;
+  if (A == 0.) {
+    r[nr++] = -C / B;
+  } else {
+    const double disc = B * B - 4. * A * C;
+    if (!(disc >= 0.)) return;
+    const double sd = std::sqrt(disc);
+    const double qq = -0.5 * (B + (B >= 0. ? sd : -sd));
+    const double ra = qq / A;
+    const double rb = qq != 0. ? C / qq : ra;
+    r[nr++] = ra < rb ? ra : rb;
+    r[nr++] = ra < rb ? rb : ra;
+  }
+  for (int j = 0; j < nr; ++j) {
+    const double u = r[j];
+    const double v = (kl - pl * u) / ql;
+    if (inside(u, v)) {
+      xi[0] = u;
+      xi[1] = v;
+      return;
+    }
+  }
+}
+
+// transform_real_to_unit_cell (cell.tcc:617-828), isoparametric branch:
+// the analytical quad4 inverse (2D) with its early exit, the affine guess
+// A^-1 (p - b) of the least-squares affine map (A = X^T KA, b = X^T Kb,
+// KA = unit vertex / nn, Kb = 1 / nn for the symmetric unit cell) with its
+// early exit, then Newton-Raphson from the better of the two with a
+// halving line search. The iteration matrix is the reference's
+// jacobian_local evaluated on the unit cell's own vertices (cell.tcc:772).
+static void transform_real_to_unit_cell(int dim, const double X[8][3], double mean_length,
+                                        const double* p, double* xi) {
+  const int nn = dim == 2 ? 4 : 8;
+  const double big = std::numeric_limits<double>::max();
+  for (int k = 0; k < dim; ++k) xi[k] = big;
+  const double zero[3] = {0., 0., 0.};
+  double an_xi[3] = {big, big, big}, an_res[3] = {big, big, big};
+  const double tol2 = 1.0E-16 * mean_length * mean_length;
+  if (dim == 2) {
+    q4_natural_coordinates(X, p, an_xi);
+    bool in_cell = true;
+    for (int k = 0; k < dim; ++k)
+      if (an_xi[k] < -1. || an_xi[k] > 1. || std::isnan(an_xi[k])) in_cell = false;
+    iso_residual(dim, X, an_xi, p, an_res);
+    if (sqnorm(dim, an_res) < tol2 && in_cell) {
+      for (int k = 0; k < dim; ++k) xi[k] = an_xi[k];
+      return;
+    }
+  }
+  (void)zero;
+  // affine guess
+  double U[8][3];
+  unit_cell(dim, U);
+  const double inv_nn = 1.0 / nn;
+  double A[3][3] = {{0}}, Ai[3][3], b[3];
+  for (int k = 0; k < dim; ++k) {
+    for (int a = 0; a < dim; ++a) {
+      double s = X[0][k] * (U[0][a] * inv_nn);
+      for (int i = 1; i < nn; ++i) s = s + X[i][k] * (U[i][a] * inv_nn);
+      A[k][a] = s;
+    }
+    double s = X[0][k] * inv_nn;
+    for (int i = 1; i < nn; ++i) s = s + X[i][k] * inv_nn;
+    b[k] = p[k] - s;
+  }
+  inverse(dim, A, Ai);
+  double af_xi[3] = {0., 0., 0.}, af_res[3] = {big, big, big};
+  bool af_nan = false;
+  for (int k = 0; k < dim; ++k) {
+    double s = Ai[k][0] * b[0];
+    for (int a = 1; a < dim; ++a) s = s + Ai[k][a] * b[a];
+    af_xi[k] = s;
+    if (std::isnan(s)) af_nan = true;
+  }
+  if (!af_nan) {
+    iso_residual(dim, X, af_xi, p, af_res);
+    if (sqnorm(dim, af_res) < tol2) {
+      for (int k = 0; k < dim; ++k) xi[k] = af_xi[k];
+      return;
+    }
+  }
+  // the better of the two as the starting point
+  double g_xi[3] = {0., 0., 0.}, g_res[3] = {big, big, big};
+  if (std::sqrt(sqnorm(dim, an_res)) < std::sqrt(sqnorm(dim, af_res))) {
+    for (int k = 0; k < dim; ++k) {
+      g_xi[k] = an_xi[k];
+      g_res[k] = an_res[k];
+    }
+  } else if (!af_nan) {
+    for (int k = 0; k < dim; ++k) {
+      g_xi[k] = af_xi[k];
+      g_res[k] = af_res[k];
+    }
+  }
+  double nr[3];
+  for (int k = 0; k < dim; ++k) {
+    nr[k] = g_xi[k];
+    if (nr[k] < -1. && nr[k] > -1.001)
+      nr[k] = -1. + 1.E-12;
+    else if (nr[k] > 1. && nr[k] < 1.001)
+      nr[k] = 1. - 1.E-12;
+  }
+  double nr_res[3] = {big, big, big};
+  for (unsigned iter = 0; iter < 10000; ++iter) {
+    double g[24], J[3][3] = {{0}}, Ji[3][3];
+    grad_shapefn(dim, nr, g);
+    for (int a = 0; a < dim; ++a)
+      for (int c = 0; c < dim; ++c) {
+        double s = g[0 * 3 + a] * U[0][c];
+        for (int i = 1; i < nn; ++i) s = s + g[i * 3 + a] * U[i][c];
+        J[a][c] = s;
+      }
+    double det;
+    if (dim == 2)
+      det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+    else
+      det = J[0][0] * (J[1][1] * J[2][2] - J[2][1] * J[1][2]) -
+            J[1][0] * (J[0][1] * J[2][2] - J[2][1] * J[0][2]) +
+            J[2][0] * (J[0][1] * J[1][2] - J[1][1] * J[0][2]);
+    if (std::fabs(det) < 1.0E-10)
+      for (int k = 0; k < dim; ++k) nr[k] = 0.;
+    iso_residual(dim, X, nr, p, nr_res);
+    inverse(dim, J, Ji);
+    double delta[3];
+    for (int k = 0; k < dim; ++k) {
+      double s = Ji[k][0] * nr_res[0];
+      for (int a = 1; a < dim; ++a) s = s + Ji[k][a] * nr_res[a];
+      delta[k] = s;
+    }
+    double step = 1.;
+    const double rn = std::sqrt(sqnorm(dim, nr_res));
+    for (int t = 0; t < 10; ++t) {
+      double trial[3], tres[3];
+      for (int k = 0; k < dim; ++k) trial[k] = nr[k] - step * delta[k];
+      iso_residual(dim, X, trial, p, tres);
+      if (std::sqrt(sqnorm(dim, tres)) < rn) {
+        for (int k = 0; k < dim; ++k) {
+          nr[k] = trial[k];
+          nr_res[k] = tres[k];
+        }
+        break;
+      } else if (step > 0.05) {
+        step /= 2.;
+      } else {
+        break;
+      }
+    }
+    double sdel[3] = {0., 0., 0.};
+    for (int k = 0; k < dim; ++k) sdel[k] = step * delta[k];
+    if (std::sqrt(sqnorm(dim, sdel)) < 1.0E-10) break;
+    if (std::isnan(nr[0]) || std::isnan(nr[1]))
+      for (int k = 0; k < dim; ++k) nr[k] = 0.;
+  }
+  const bool geo = std::sqrt(sqnorm(dim, g_res)) < std::sqrt(sqnorm(dim, nr_res));
+  for (int k = 0; k < dim; ++k) xi[k] = geo ? g_xi[k] : nr[k];
+  if (std::isnan(xi[0]) || std::isnan(xi[1]))
+    throw std::runtime_error("Local coordinates of xi is NAN");
 }
 
 // ---------------------------------------------------------------------------
@@ -777,6 +1047,10 @@ struct Node {
   unsigned f_dir = 0;
   int f_sign = 0;
   double f_mu = 0;
+  // general (inclined) boundary: constraints act in the frame of R
+  // (Node::assign_rotation_matrix, node.h:281-285)
+  bool generic = false;
+  double R[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
 };
 
 struct Cell {
@@ -835,6 +1109,9 @@ struct Solver {
   std::vector<TractionLoad> tractions;
   std::vector<Index> unlocatable;
   bool locate_all = true;  // the analysis key locate_particles (mpm_base.tcc:53-54)
+  // the mesh's "isoparametric" key (mpm_base.tcc:768-783): local coordinates
+  // by transform_real_to_unit_cell instead of the Cartesian formula
+  bool iso = false;
   std::string error;
 
   // ---------------- mesh -----------------
@@ -943,6 +1220,10 @@ struct Solver {
     for (int k = 0; k < dim; ++k) xi[k] = std::numeric_limits<double>::max();
     // approx_point_in_cell
     if (!(norm_diff(dim, p, c.centroid) < (c.mean_length * 2.))) return false;
+    if (iso) {
+      transform_real_to_unit_cell(dim, c.X, c.mean_length, p, xi);
+      return clamp_unit(xi);
+    }
     // local_coordinates_point (Cartesian)
     double len[3];
     len[0] = norm_diff(dim, c.X[0], c.X[1]);
@@ -955,6 +1236,12 @@ struct Solver {
       centre[k] = s / (dim == 2 ? 4.0 : (double)nn);
     }
     for (int k = 0; k < dim; ++k) xi[k] = 2. * (p[k] - centre[k]) / len[k];
+    return clamp_unit(xi);
+  }
+
+  // The unit-cell test of is_point_in_cell (cell.tcc:361-377): inside
+  // [-1, 1] (and not NaN), then kept DBL_EPSILON inside the bounds.
+  bool clamp_unit(double* xi) const {
     bool status = true;
     const double tol = kEps;
     for (int k = 0; k < dim; ++k) {
@@ -1204,12 +1491,34 @@ struct Solver {
     }
   }
 
-  // node.tcc:370-404 (Cartesian branch)
+  // y = A x for a dim x dim matrix (sequential over k)
+  void matvec(const double A[3][3], const double* x, double* y) const {
+    for (int i = 0; i < dim; ++i) {
+      double s = A[i][0] * x[0];
+      for (int k = 1; k < dim; ++k) s = s + A[i][k] * x[k];
+      y[i] = s;
+    }
+  }
+
+  // node.tcc:370-404: per constraint in direction order; on a general
+  // boundary velocity and acceleration go to the local frame (R^-1), take
+  // the constraint there and come back (R)
   void apply_velocity_constraints(Node& n) {
     for (const auto& c : n.vel_c) {
       const unsigned direction = c.first % dim;
-      n.velocity[direction] = c.second;
-      n.acceleration[direction] = 0.;
+      if (!n.generic) {
+        n.velocity[direction] = c.second;
+        n.acceleration[direction] = 0.;
+      } else {
+        double Ri[3][3], lv[3], la[3];
+        inverse(dim, n.R, Ri);
+        matvec(Ri, n.velocity, lv);
+        matvec(Ri, n.acceleration, la);
+        lv[direction] = c.second;
+        la[direction] = 0.;
+        matvec(n.R, lv, n.velocity);
+        matvec(n.R, la, n.acceleration);
+      }
     }
   }
 
@@ -1363,18 +1672,30 @@ struct Solver {
     }
   }
 
-  // node.tcc:510-657 (Cartesian branches)
+  // node.tcc:510-657; on a general boundary in the frame of R
   void apply_friction(Node& n, double dt) {
     if (!n.friction) return;
     auto sign = [](double value) { return (value > 0.) ? 1. : -1.; };
     const unsigned dir_n = n.f_dir;
     const double sign_n = sign(n.f_sign);
     const double mu = n.f_mu;
+    double Ri[3][3];
+    if (n.generic) inverse(dim, n.R, Ri);
     if (dim == 2) {
       const unsigned dir_t = (dim - 1) - dir_n;
-      double acc_n = n.acceleration[dir_n];
-      double acc_t = n.acceleration[dir_t];
-      double vel_t = n.velocity[dir_t];
+      double acc_n, acc_t, vel_t;
+      if (!n.generic) {
+        acc_n = n.acceleration[dir_n];
+        acc_t = n.acceleration[dir_t];
+        vel_t = n.velocity[dir_t];
+      } else {
+        double la[3], lv[3];
+        matvec(Ri, n.acceleration, la);
+        matvec(Ri, n.velocity, lv);
+        acc_n = la[dir_n];
+        acc_t = la[dir_t];
+        vel_t = lv[dir_t];
+      }
       if ((acc_n * sign_n) > 0.0) {
         if (vel_t != 0.0) {
           const double vel_net = dt * acc_t + vel_t;
@@ -1389,13 +1710,25 @@ struct Solver {
           else
             acc_t -= sign(acc_t) * mu * std::abs(acc_n);
         }
-        n.acceleration[dir_t] = acc_t;
+        if (!n.generic) {
+          n.acceleration[dir_t] = acc_t;
+        } else {
+          double la[3] = {0., 0., 0.};
+          la[dir_t] = acc_t;
+          la[dir_n] = acc_n;
+          matvec(n.R, la, n.acceleration);
+        }
       }
     } else {
       static const int tdir[3][2] = {{1, 2}, {0, 2}, {0, 1}};
       const unsigned t0 = tdir[dir_n][0], t1 = tdir[dir_n][1];
       double acc[3], vel[3];
-      for (int k = 0; k < 3; ++k) { acc[k] = n.acceleration[k]; vel[k] = n.velocity[k]; }
+      if (!n.generic) {
+        for (int k = 0; k < 3; ++k) { acc[k] = n.acceleration[k]; vel[k] = n.velocity[k]; }
+      } else {
+        matvec(Ri, n.acceleration, acc);
+        matvec(Ri, n.velocity, vel);
+      }
       const double acc_n = acc[dir_n];
       double acc_t = std::sqrt(acc[t0] * acc[t0] + acc[t1] * acc[t1]);
       const double vel_t = std::sqrt(vel[t0] * vel[t0] + vel[t1] * vel[t1]);
@@ -1422,14 +1755,29 @@ struct Solver {
             acc[t1] -= mu * std::abs(acc_n) * (acc[t1] / acc_t);
           }
         }
-        for (int k = 0; k < 3; ++k) n.acceleration[k] = acc[k];
+        if (!n.generic)
+          for (int k = 0; k < 3; ++k) n.acceleration[k] = acc[k];
+        else
+          matvec(n.R, acc, n.acceleration);
       }
     }
   }
 
-  // node.tcc:456-485 (Cartesian), with the values of the current time
+  // node.tcc:456-485, with the values of the current time; on a general
+  // boundary per constraint in the frame of R
   void apply_acceleration_constraints(Node& n) {
-    for (const auto& c : n.acc_c) n.acceleration[c.first % dim] = n.acc_now[c.first % dim];
+    for (const auto& c : n.acc_c) {
+      const unsigned direction = c.first % dim;
+      if (!n.generic) {
+        n.acceleration[direction] = n.acc_now[direction];
+      } else {
+        double Ri[3][3], la[3];
+        inverse(dim, n.R, Ri);
+        matvec(Ri, n.acceleration, la);
+        la[direction] = n.acc_now[direction];
+        matvec(n.R, la, n.acceleration);
+      }
+    }
   }
 
   // Mesh::apply_traction_on_particles (mesh.tcc:1371-1386) at current_time:
@@ -1666,6 +2014,28 @@ int orc_threads() {
 
 int orc_set_mesh(orc_ctx* c, Index nnodes, const double* coords, Index ncells, const Index* cn) {
   ORC_TRY(c->s.set_mesh(nnodes, coords, ncells, cn))
+}
+
+// Mesh::compute_nodal_rotation_matrices (mesh.tcc:1320-1340): node[i] gets the
+// rotation of Euler angles angles[i * dim ...] and its constraints act on a
+// general boundary.
+int orc_set_nodal_euler_angles(orc_ctx* c, Index n, const Index* node, const double* angles) {
+  ORC_TRY({
+    auto& s = c->s;
+    for (Index i = 0; i < n; ++i) {
+      if (node[i] < 0 || node[i] >= (Index)s.nodes.size())
+        throw std::runtime_error("node id out of range");
+      orc::Node& nd = s.nodes[node[i]];
+      orc::rotation_matrix(s.dim, angles + i * s.dim, nd.R);
+      nd.generic = true;
+    }
+  })
+}
+
+// The mesh's "isoparametric" key (mpm_base.tcc:768-783).
+int orc_set_isoparametric(orc_ctx* c, int iso) {
+  c->s.iso = iso != 0;
+  return 0;
 }
 
 int orc_set_materials(orc_ctx* c, int n, const double* params) {
@@ -2150,6 +2520,38 @@ void orc_kat_mc_init_state(const double* params, double* sv) {
 }
 
 // ------------------- per-function entry points for the KATs ------------------
+// Cell of one quad4 / hex8 with vertices X[nn * dim]: transform_real_to_unit_cell
+// (isoparametric) of point p into xi; returns 0, or -1 when it throws.
+int orc_kat_transform_real_to_unit_cell(int dim, const double* Xf, const double* p, double* xi) {
+  orc::Solver s;
+  s.dim = dim;
+  s.nn = dim == 2 ? 4 : 8;
+  orc::Cell c;
+  for (int i = 0; i < s.nn; ++i)
+    for (int k = 0; k < 3; ++k) c.X[i][k] = k < dim ? Xf[i * dim + k] : 0.;
+  s.init_cell(c);
+  try {
+    orc::transform_real_to_unit_cell(dim, c.X, c.mean_length, p, xi);
+  } catch (std::exception&) {
+    return -1;
+  }
+  return 0;
+}
+
+// is_point_in_cell (cell.tcc:335-380) of the same one-cell setup; iso 0 uses
+// the Cartesian local coordinates. Returns 1 inside, 0 outside.
+int orc_kat_point_in_cell(int dim, int iso, const double* Xf, const double* p, double* xi) {
+  orc::Solver s;
+  s.dim = dim;
+  s.nn = dim == 2 ? 4 : 8;
+  s.iso = iso != 0;
+  orc::Cell c;
+  for (int i = 0; i < s.nn; ++i)
+    for (int k = 0; k < 3; ++k) c.X[i][k] = k < dim ? Xf[i * dim + k] : 0.;
+  s.init_cell(c);
+  return s.is_point_in_cell(c, p, xi) ? 1 : 0;
+}
+
 void orc_kat_shapefn(int dim, const double* xi, double* N, double* grad) {
   orc::shapefn(dim, xi, N);
   if (grad) {
@@ -2297,6 +2699,54 @@ double orc_kat_math(int which, double x) {
     case 5: return orc::mth::pow(x, -1.5);
     default: return orc::mth::pow(x, -2.5);
   }
+}
+
+// One node's constraints on a general boundary (angles: Euler angles, null:
+// Cartesian): which 0 apply_velocity_constraints (nv constraints vdir/vval),
+// 1 apply_acceleration_constraints (na, adir/aval), 2
+// apply_friction_constraints(dt) with (f_dir, f_sign, f_mu). vel and acc in
+// and out.
+void orc_kat_node_constraints(int dim, const double* angles, double* vel, double* acc, int nv,
+                              const int* vdir, const double* vval, int na, const int* adir,
+                              const double* aval, int f_dir, int f_sign, double f_mu, double dt,
+                              int which) {
+  orc::Solver s;
+  s.dim = dim;
+  orc::Node n;
+  for (int k = 0; k < dim; ++k) {
+    n.velocity[k] = vel[k];
+    n.acceleration[k] = acc[k];
+  }
+  if (angles) {
+    orc::rotation_matrix(dim, angles, n.R);
+    n.generic = true;
+  }
+  for (int i = 0; i < nv; ++i) n.vel_c.insert(std::make_pair((unsigned)vdir[i], vval[i]));
+  for (int i = 0; i < na; ++i) {
+    n.acc_c.insert(std::make_pair((unsigned)adir[i], aval[i]));
+    n.acc_now[adir[i] % dim] = aval[i];
+  }
+  if (which == 0) s.apply_velocity_constraints(n);
+  if (which == 1) s.apply_acceleration_constraints(n);
+  if (which == 2) {
+    n.friction = true;
+    n.f_dir = f_dir;
+    n.f_sign = f_sign;
+    n.f_mu = f_mu;
+    s.apply_friction(n, dt);
+  }
+  for (int k = 0; k < dim; ++k) {
+    vel[k] = n.velocity[k];
+    acc[k] = n.acceleration[k];
+  }
+}
+
+// The rotation matrix of Euler angles (row-major dim x dim).
+void orc_kat_rotation_matrix(int dim, const double* angles, double* R) {
+  double M[3][3];
+  orc::rotation_matrix(dim, angles, M);
+  for (int i = 0; i < dim; ++i)
+    for (int j = 0; j < dim; ++j) R[i * dim + j] = M[i][j];
 }
 
 void orc_kat_friction(int dim, double* acc, const double* vel, double dt, int f_dir, int f_sign,

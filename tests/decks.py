@@ -119,6 +119,8 @@ def write_deck(w, d, nsteps, output_steps, constraints_as="sets", uuid="deck",
             "isoparametric": False, "check_duplicates": True,
             "node_type": "N%dD" % dim, "cell_type": "ED2Q4" if dim == 2 else "ED3H8",
             "boundary_conditions": bc}
+    if w.isoparametric:  # the key left out: the reference's default is isoparametric
+        del mesh["isoparametric"]
     if stresses is not None:
         with open(os.path.join(d, "stresses.txt"), "w") as f:
             f.write("%d\n" % len(stresses))

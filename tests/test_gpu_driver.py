@@ -55,6 +55,25 @@ def test_driver_2d_column_matches_oracle(tmp_path):
     _compare(out, _oracle_run(w, 30), 2)
 
 
+def test_driver_isoparametric_default_matches_oracle(tmp_path):
+    """A deck without the "isoparametric" key on a mesh of skewed quads: the
+    driver takes the reference's default (isoparametric, mpm_base.tcc:768-783)
+    and matches the oracle's isoparametric run bit for bit."""
+    w = workloads.column_2d(nx=6, ny=10)
+    X = w.coords.copy()
+    inner = (X[:, 0] > 1e-9) & (X[:, 0] < X[:, 0].max() - 1e-9) & (X[:, 1] > 1e-9) & \
+        (X[:, 1] < X[:, 1].max() - 1e-9)
+    X[inner, 0] += 0.012 * np.sin(7.0 * X[inner, 1])
+    X[inner, 1] += 0.009 * np.cos(5.0 * X[inner, 0])
+    w.coords = X
+    w.isoparametric = True
+    d = str(tmp_path)
+    write_deck(w, d, nsteps=30, output_steps=30, material_json=LE)
+    run_driver(d, "--ordered")
+    out = read_output(output_path(d, 30, 30), 2)
+    _compare(out, _oracle_run(w, 30), 2)
+
+
 def test_driver_3d_mohr_coulomb_with_stresses_matches_oracle(tmp_path):
     from mpm_amd.capi import Solver
     w = workloads.collapse_3d(nx=8, ny=4, nz=4, block=(4, 4, 4))

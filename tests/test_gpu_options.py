@@ -221,17 +221,19 @@ def test_loads_3d(ordered):
 
 @pytest.mark.parametrize("dim", [2, 3])
 @pytest.mark.parametrize("ordered", [True, False])
-def test_jaumann_stress_rate(dim, ordered):
+@pytest.mark.parametrize("scheme", ["usf", "usl", "musl"])
+def test_jaumann_stress_rate(dim, ordered, scheme):
     """LinearElastic with the Jaumann rate (linear_elastic.tcc:96-124) on a
     soft block under gravity with random initial velocities, so that the
-    particles spin (USF in 2D, MUSL in 3D)."""
+    particles spin; every scheme, so that each kernel build with the
+    rotation runs (USF: the stress-then-scatter kernel in fast mode; USL:
+    G2P with the stress update; MUSL: the stress kernel with locate)."""
     if dim == 2:
         w = workloads.column_2d(nx=6, ny=10)
     else:
         w = workloads.collapse_3d(nx=8, ny=4, nz=4, block=(4, 4, 4))
     w.velocity_seed, w.velocity_amplitude = 7, 0.05
     w.materials = mat.linear_elastic(2e5, 0.3, 1800.0, stress_rate="jaumann")[None, :]
-    scheme = "usf" if dim == 2 else "musl"
     pg, po = _run(w, 40, ordered=ordered, scheme=scheme)
     _check(pg, po, exact=ordered)
     # the rotation matters: the rate-free update differs
@@ -332,3 +334,76 @@ def test_gauss_generator_counts(dim, nppd):
     pg, po = _run(w, 10, scheme=w.scheme)
     assert len(pg["x"]) == len(w.cell_set if w.cell_set is not None else w.cells) * nppd ** dim
     _check(pg, po, exact=True)
+
+
+def _inclined_base(w, angles):
+    """The deck's base nodes on a general boundary: only the normal velocity
+    (local direction dim-1) is held at 0, with Coulomb friction mu = 0.3 in
+    the local frame; the base is tilted by the Euler angles."""
+    d = w.dim
+    base = np.nonzero(np.abs(w.coords[:, d - 1]) <= 1e-9)[0].astype(np.int64)
+    keep = ~np.isin(w.vel_nodes, base)
+    w.vel_nodes = np.concatenate([w.vel_nodes[keep], base])
+    w.vel_dirs = np.concatenate([w.vel_dirs[keep],
+                                 np.full(len(base), d - 1)]).astype(np.int32)
+    w.vel_values = np.concatenate([np.asarray(w.vel_values)[keep], np.zeros(len(base))])
+    w.fric_nodes = base
+    w.fric_dirs = np.full(len(base), d - 1, dtype=np.int32)
+    w.fric_sign = np.full(len(base), -1, dtype=np.int32)
+    w.fric_mu = np.full(len(base), 0.3)
+    ang = np.tile(np.radians(angles), (len(base), 1))
+    return base, ang
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("ordered", [True, False])
+def test_general_boundary_constraints(dim, ordered):
+    """Velocity and friction constraints of the base nodes in a tilted frame
+    (node.tcc:385-401, 540-583, 602-655): the block slides down the inclined
+    base; GPU against the oracle (MUSL, Mohr-Coulomb in 3D)."""
+    if dim == 2:
+        w = workloads.column_2d(nx=6, ny=10)
+        angles = (15.0, 10.0)
+    else:
+        w = workloads.collapse_3d(nx=8, ny=4, nz=4, block=(4, 4, 4))
+        angles = (10.0, 8.0, 6.0)
+    base, ang = _inclined_base(w, angles)
+    o = w.setup(_oracle(w, scheme="musl"))
+    o.set_nodal_euler_angles(base, ang)
+    if ordered:
+        g = w.setup(_solver(w, True, scheme="musl"))
+        g.set_nodal_euler_angles(base, ang)
+        for _ in range(4):
+            g.step(10, w.dt)
+            o.step(10, w.dt)
+        pg, po = g.particles(), o.particles()
+        _check(pg, po, exact=True)
+        done = 40
+    else:
+        # fast mode, each step from the oracle's state: friction's stick/slip
+        # tests and the yield surface amplify atomic-order rounding over steps
+        for _ in range(8):
+            o.step(5, w.dt)
+            po = o.particles()
+            from mpm_amd.capi import Solver
+            g = Solver(w.dim, "musl", ordered=False)
+            g.set_mesh(w.coords, w.cells)
+            g.set_materials(w.materials)
+            g.set_velocity_constraints(w.vel_nodes, w.vel_dirs, w.vel_values)
+            g.set_friction_constraints(w.fric_nodes, w.fric_dirs, w.fric_sign, w.fric_mu)
+            g.set_nodal_euler_angles(base, ang)
+            g.set_gravity(w.gravity)
+            g.set_particles(x=po["x"], mass=po["mass"], volume=po["volume"],
+                            density=po["density"], material_id=po["material_id"], v=po["v"],
+                            u=po["u"], stress=po["stress"], strain=po["strain"], F=po["F"],
+                            cell=po["cell"], statevars=po["statevars"])
+            g.step(1, w.dt)
+            o.step(1, w.dt)
+            _check(g.particles(), o.particles(), exact=False, tol=1e-12)
+            g.close()
+        po = o.particles()
+        done = 48
+    # the tilt matters: the Cartesian base gives other velocities
+    c = w.setup(_solver(w, True, scheme="musl"))
+    c.step(done, w.dt)
+    assert not np.array_equal(c.particles()["v"], po["v"])

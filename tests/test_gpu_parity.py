@@ -166,6 +166,25 @@ def test_fast_2d_full_run(scheme):
     print("fast %s 100-step errors: %s" % (scheme, {k: "%.2e" % v for k, v in e.items()}))
 
 
+def test_elastic_column_full_config():
+    """The 2D elastic column as specified: quad4 32x64 cells of 0.05 m, 8192
+    particles, USF, dt = 1e-4, all 1000 steps. Ordered mode is bitwise equal
+    to the oracle at the end; fast mode is within 1e-9 (normwise relative,
+    per field) and has the same cell ids."""
+    w = workloads.column_2d()
+    o = w.setup(oracle.Oracle(2, "usf"))
+    g = w.setup(_solver(2, "usf", True))
+    f = w.setup(_solver(2, "usf", False))
+    for _ in range(4):
+        o.step(w.nsteps // 4, w.dt)
+        g.step(w.nsteps // 4, w.dt)
+        f.step(w.nsteps // 4, w.dt)
+    assert o.nparticles == 8192 and g.step_count == 1000
+    _compare(g, o, exact=True, what="ordered column 1000 steps")
+    e = _compare(f, o, exact=False, tol=1e-9, what="fast column 1000 steps")
+    print("fast column 1000-step errors: %s" % {k: "%.2e" % v for k, v in e.items()})
+
+
 @pytest.mark.parametrize("dim", [2, 3])
 def test_cell_crossing_gpu(kats, dim):
     c = kats["cell_crossing_%dd" % dim]

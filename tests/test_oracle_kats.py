@@ -466,3 +466,132 @@ def test_oracle_locate_3d_boundaries_and_hints():
     # a point just outside the domain is not located
     with pytest.raises(oracle.OracleError):
         _located(coords, cells, [origin - 1e-9])
+
+
+# ------------------------------ isoparametric cells -------------------------
+def test_isoparametric_cell_kats(kats):
+    """is_point_in_cell and transform_real_to_unit_cell on the reference's
+    isoparametric test cells (squares, a skewed quad, a trapezoid, cubes, a
+    skewed hex, two sheared hexes sharing a face)."""
+    for c in kats["cell_isoparametric"]["cases"]:
+        what = c["cite"]
+        if "inside" in c:
+            inside, xi = oracle.point_in_cell(c["dim"], c["X"], c["point"], isoparametric=True)
+            assert inside == c["inside"], what
+            if "xi" in c and not c.get("transform"):
+                assert_kat(xi, c["xi"], c["tol"], what)
+        if c.get("transform"):
+            xi = oracle.transform_real_to_unit_cell(c["dim"], c["X"], c["point"])
+            assert_kat(xi, c["xi"], c["tol"], what)
+
+
+@pytest.mark.parametrize("case", [0, 1])
+def test_isoparametric_generate_points_round_trip(kats, case):
+    """The quadrature points generated in a skewed cell (x = N(xi_g) X) map
+    back to xi_g (cell_test.cc "Generate points in cell"), 1 to 4 points per
+    direction."""
+    c = kats["cell_isoparametric"]["generate_points"][case]
+    dim, X = c["dim"], np.array(c["X"], dtype=float)
+    for n in c["nppd"]:
+        pts = _gauss_order(dim, n)
+        for xi_g in pts:
+            N, _ = oracle.shapefn(dim, xi_g)
+            x = N @ X
+            xi = oracle.transform_real_to_unit_cell(dim, X, x)
+            assert_kat(xi, xi_g, c["tol"], "%s, %d per direction" % (c["cite"], n))
+
+
+def _cells_for_paths():
+    """A skewed quad (the analytical inverse answers), a parallelogram and a
+    parallelepiped (the affine guess answers exactly), a trapezoidal hex (only
+    the iteration answers)."""
+    quad = np.array([[2., 1.], [4., 2.], [2., 4.], [1., 3.]])
+    para2 = np.array([[0., 0.], [2., 0.5], [2.5, 2.5], [0.5, 2.]])
+    base = np.array([[0., 0., 0.], [2., 0.3, 0.], [2.4, 2.2, 0.1], [0.4, 1.9, 0.1]])
+    para3 = np.concatenate([base, base + np.array([0.3, 0.2, 1.7])])
+    trap = np.array([[0., 0., 0.], [2., 0., 0.], [2., 2., 0.], [0., 2., 0.],
+                     [0.4, 0.3, 1.6], [1.7, 0.2, 1.5], [1.5, 1.6, 1.8], [0.3, 1.7, 1.4]])
+    return [(2, quad, 1e-13), (2, para2, 1e-13), (3, para3, 1e-13), (3, trap, 1e-9)]
+
+
+def test_isoparametric_inverse_reproduces_local_coordinates():
+    """For random xi in the unit cell, x = N(xi) X maps back to xi: exact to
+    rounding where the closed form or the affine guess answers, to the
+    iteration's 1e-10 step tolerance (in coordinates of cells ~2 long)
+    where only Newton-Raphson does."""
+    rng = np.random.default_rng(3)
+    for dim, X, tol in _cells_for_paths():
+        for _ in range(200):
+            xi_true = rng.uniform(-0.98, 0.98, dim)
+            N, _ = oracle.shapefn(dim, xi_true)
+            x = N @ X
+            xi = oracle.transform_real_to_unit_cell(dim, X, x)
+            assert np.max(np.abs(xi - xi_true)) <= tol, (dim, X[1], xi, xi_true)
+            inside, xi2 = oracle.point_in_cell(dim, X, x, isoparametric=True)
+            assert inside and np.array_equal(xi, xi2)
+        # a point beyond a face is outside
+        N, _ = oracle.shapefn(dim, np.array([1.2] + [0.0] * (dim - 1)))
+        assert not oracle.point_in_cell(dim, X, N @ X, isoparametric=True)[0]
+
+
+def test_isoparametric_equals_cartesian_on_rectangles():
+    """On axis-aligned cells both local-coordinate formulas give xi to
+    rounding (cell.tcc:423-606 against :617-828)."""
+    rng = np.random.default_rng(4)
+    for dim in (2, 3):
+        h = np.array([0.3, 0.7, 0.45][:dim])
+        lo = np.array([1.1, -0.4, 2.0][:dim])
+        U = np.array([[-1, -1, -1], [1, -1, -1], [1, 1, -1], [-1, 1, -1],
+                      [-1, -1, 1], [1, -1, 1], [1, 1, 1], [-1, 1, 1]], dtype=float)
+        X = lo + (U[:2 ** dim, :dim] + 1) * 0.5 * h
+        for _ in range(100):
+            x = lo + rng.uniform(0.01, 0.99, dim) * h
+            a, xa = oracle.point_in_cell(dim, X, x, isoparametric=True)
+            b, xb = oracle.point_in_cell(dim, X, x, isoparametric=False)
+            assert a and b
+            assert np.max(np.abs(xa - xb)) <= 1e-14
+
+
+# --------------------------- general boundaries -----------------------------
+def test_general_boundary_constraints(kats):
+    """Velocity, acceleration and friction constraints on inclined boundaries
+    (node.tcc:370-404, 456-485, 510-657 with a rotation matrix): the global
+    results and, where the reference checks them, the values in the rotated
+    frame."""
+    g = kats["node"]["general_constraints"]
+    tol = g["tol"]
+    for c in g["cases"]:
+        dim = c["dim"]
+        ang = np.radians(c["angles_deg"])
+        v, a = oracle.node_constraints(
+            dim, [0.1] * dim, [5.0] * dim, angles=ang,
+            velocity=[tuple(x) for x in c.get("velocity_constraints", [])],
+            acceleration=[tuple(x) for x in c.get("acceleration_constraints", [])],
+            friction=tuple(c["friction"]) if "friction" in c else None, dt=g["dt"])
+        if "velocity" in c:
+            assert_kat(v, c["velocity"], tol, c["cite"])
+        assert_kat(a, c["acceleration"], tol, c["cite"])
+        Ri = np.linalg.inv(oracle.rotation_matrix(dim, ang))
+        for k, val in c.get("local_velocity", {}).items():
+            assert_kat((Ri @ v)[int(k)], val, tol, c["cite"])
+        for k, val in c.get("local_acceleration", {}).items():
+            assert_kat((Ri @ a)[int(k)], val, tol, c["cite"])
+
+
+def test_rotation_matrix_is_orthonormal_and_reduces_to_cartesian():
+    """R of any Euler angles is a rotation (R R^T = I, det 1); zero angles give
+    the identity, and constraints on such a 'general' boundary equal the
+    Cartesian ones."""
+    rng = np.random.default_rng(5)
+    for dim in (2, 3):
+        for _ in range(20):
+            R = oracle.rotation_matrix(dim, rng.uniform(-np.pi, np.pi, dim))
+            assert np.allclose(R @ R.T, np.eye(dim), atol=1e-14)
+            assert abs(np.linalg.det(R) - 1.0) < 1e-14
+        assert np.array_equal(oracle.rotation_matrix(dim, np.zeros(dim)), np.eye(dim))
+        for kw in (dict(velocity=[(0, 2.5)]), dict(acceleration=[(dim - 1, -1.0)]),
+                   dict(friction=(dim - 1, -1, 0.4), dt=0.01)):
+            vel, acc = [0.3, -0.2, 0.1][:dim], [1.0, -2.0, -4.0][:dim]
+            a = oracle.node_constraints(dim, vel, acc, angles=np.zeros(dim), **kw)
+            b = oracle.node_constraints(dim, vel, acc, **kw)
+            assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
