@@ -45,9 +45,14 @@ typedef struct mpmx_ctx mpmx_ctx;
 #define MPMX_USL 1
 #define MPMX_MUSL 2
 
-/* velocity update (mpm::VelocityUpdate, data_types.h:50; particle.tcc:994-1070) */
+/* velocity update (mpm::VelocityUpdate, data_types.h:50; particle.tcc:994-1140;
+ * the affine ones map momentum with map_mass_momentum_to_nodes_affine/_taylor,
+ * particle.tcc:661-724) */
 #define MPMX_FLIP 0
 #define MPMX_PIC 1
+#define MPMX_ASFLIP 2
+#define MPMX_TPIC 3
+#define MPMX_APIC 4
 
 /* creation flags */
 #define MPMX_ORDERED 0x1u /* nodal sums in particle order: bitwise equal to a
@@ -58,7 +63,7 @@ typedef struct mpmx_ctx mpmx_ctx;
 /* Number of doubles in one material parameter vector (layout documented in
  * mpm_amd/materials.py; replaces Factory<Material<Tdim>, unsigned, const Json&>
  * construction from the JSON "materials" array, src/material.cc:31-53). */
-#define MPMX_MATERIAL_PARAMS 16
+#define MPMX_MATERIAL_PARAMS 17
 /* MohrCoulomb state variables per particle (mohr_coulomb.tcc:85-111) */
 #define MPMX_STATE_VARS 9
 
@@ -261,6 +266,13 @@ int mpmx_get_particles(mpmx_ctx* ctx, double* x, double* u, double* v, double* s
  * acceleration [n*dim], active flag. NULL skips a field. */
 int mpmx_get_nodes(mpmx_ctx* ctx, double* mass, double* velocity, double* acceleration,
                    int* active);
+
+/* Mapping matrices of the affine velocity updates (Particle::mapping_matrix_,
+ * particle.h; the PODParticle mapping_matrix_ij fields, pod_particle.h):
+ * [n][dim][dim] row-major in particle order; zero at creation, set by every
+ * step with MPMX_APIC, MPMX_ASFLIP or MPMX_TPIC. Set on resume. */
+int mpmx_get_mapping_matrix(mpmx_ctx* ctx, double* B);
+int mpmx_set_mapping_matrix(mpmx_ctx* ctx, int64_t n, const double* B);
 
 /* Advance nsteps explicit steps (one iteration of the loop in
  * mpm_explicit.tcc:112-190 each: MPMScheme::initialise,

@@ -15,8 +15,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("MPMX_LIB") or os.path.join(HERE, "libmpmx.so")
 
 OK, E_ARG, E_STATE, E_HIP, E_UNLOCATABLE, E_UNSUPPORTED = 0, -1, -2, -3, -4, -5
+MATERIAL_PARAMS = 17  # MPMX_MATERIAL_PARAMS
 SCHEMES = {"usf": 0, "usl": 1, "musl": 2}
-VELOCITY_UPDATES = {"flip": 0, "pic": 1}
+VELOCITY_UPDATES = {"flip": 0, "pic": 1, "asflip": 2, "tpic": 3, "apic": 4}
 ORDERED = 0x1
 NO_GRAPH = 0x2
 
@@ -32,6 +33,7 @@ EXPORTS = (
     "mpmx_set_gravity", "mpmx_generate_gauss",
     "mpmx_set_particles", "mpmx_compute_volume_mass", "mpmx_assign_velocity",
     "mpmx_assign_stresses", "mpmx_assign_materials", "mpmx_nparticles", "mpmx_get_particles", "mpmx_get_nodes",
+    "mpmx_get_mapping_matrix", "mpmx_set_mapping_matrix",
     "mpmx_step", "mpmx_step_noalloc", "mpmx_set_locate_particles", "mpmx_get_particle_ids", "mpmx_set_particle_ids",
     "mpmx_set_sort_interval", "mpmx_set_chunk", "mpmx_autotune", "mpmx_set_step",
     "mpmx_get_step",
@@ -93,6 +95,8 @@ def load(path=LIB_PATH):
         "mpmx_get_nodes": [V, _D, _D, _D, _I32],
         "mpmx_step": [V, C.c_int, C.c_double, C.c_double],
         "mpmx_step_noalloc": [V, C.c_int, C.c_double, C.c_double],
+        "mpmx_get_mapping_matrix": [V, _D],
+        "mpmx_set_mapping_matrix": [V, C.c_int64, _D],
         "mpmx_set_step": [V, C.c_int64],
         "mpmx_set_locate_particles": [V, C.c_int],
         "mpmx_get_particle_ids": [V, _I64],
@@ -201,7 +205,7 @@ class Solver:
         self._check(self.lib.mpmx_set_instances(self.ctx, len(cb) - 1, _p(cb, _I64)))
 
     def set_materials(self, params):
-        params = _f64(params).reshape(-1, 16)
+        params = _f64(params).reshape(-1, MATERIAL_PARAMS)
         self._check(self.lib.mpmx_set_materials(self.ctx, params.shape[0], _p(params)))
         self.material_params = params.copy()
 
@@ -349,6 +353,16 @@ class Solver:
         self._check(self.lib.mpmx_get_nodes(self.ctx, _p(out["mass"]), _p(out["velocity"]),
                                             _p(out["acceleration"]), _p(out["active"], _I32)))
         return out
+
+    def mapping_matrix(self):
+        """The affine velocity updates' mapping matrices, (np, dim, dim)."""
+        B = np.zeros((self.nparticles, self.dim, self.dim))
+        self._check(self.lib.mpmx_get_mapping_matrix(self.ctx, _p(B)))
+        return B
+
+    def set_mapping_matrix(self, B):
+        B = _f64(B)
+        self._check(self.lib.mpmx_set_mapping_matrix(self.ctx, B.shape[0], _p(B)))
 
     def step(self, nsteps, dt, damping_factor=0.0):
         self._check(self.lib.mpmx_step(self.ctx, int(nsteps), float(dt), float(damping_factor)))

@@ -620,23 +620,89 @@ MPMX_DEV void rotate_sym(const double (*Q)[3], const double (*X)[3], double* y) 
   }
 }
 
-// LinearElastic stress with the Jaumann rate (linear_elastic.tcc:96-124):
-// a = the entries 01, 02, 12 of W/2, W the skew part of dF - I. The stress
-// s is rotated by exp(W) = exp(W/2)^2, the strain increment ds (tensor
-// shears) by exp(W/2), and the elastic increment of the rotated strain is
-// added. s: in/out.
-static __device__ __attribute__((noinline)) void jaumann_le(const Material& m, const double* a, const double* ds, double* s) {
-  double L[3][3], LL[3][3];
-  skew_exp(a, L);
+// C = A B and C = A B^T of 3x3 matrices (sequential k)
+MPMX_DEV void mat3_mul(const double (*A)[3], const double (*B)[3], double (*C)[3]) {
 #pragma unroll
   for (int i = 0; i < 3; ++i)
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
-      double t = L[i][0] * L[0][j];
+      double t = A[i][0] * B[0][j];
 #pragma unroll
-      for (int k = 1; k < 3; ++k) t = t + L[i][k] * L[k][j];
-      LL[i][j] = t;
+      for (int k = 1; k < 3; ++k) t = t + A[i][k] * B[k][j];
+      C[i][j] = t;
     }
+}
+MPMX_DEV void mat3_mul_t(const double (*A)[3], const double (*B)[3], double (*C)[3]) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      double t = A[i][0] * B[j][0];
+#pragma unroll
+      for (int k = 1; k < 3; ++k) t = t + A[i][k] * B[j][k];
+      C[i][j] = t;
+    }
+}
+
+// The rotation R of the polar decomposition F = R U (det F > 0), which
+// mpm::materials::compute_rotation_tensor (material_utility.tcc) takes from
+// the eigen-decomposition of F^T F: by Newton's iteration
+// X <- (X + X^-T) / 2 from X = F (Higham 1986) until an iteration changes no
+// entry by more than 1e-15 of the largest (at most 100).
+MPMX_DEV void polar_rotation(const double (*F)[3], double (*R)[3]) {
+  double X[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) X[i][j] = F[i][j];
+  for (int it = 0; it < 100; ++it) {
+    double Xi[3][3];
+    mat_inverse<3>(X, Xi);
+    double diff = 0., big = 0.;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const double n = 0.5 * (X[i][j] + Xi[j][i]);
+        const double d = fabs(n - X[i][j]), a = fabs(n);
+        if (d > diff) diff = d;
+        if (a > big) big = a;
+        X[i][j] = n;
+      }
+    if (diff <= 1.0E-15 * big) break;
+  }
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) R[i][j] = X[i][j];
+}
+
+// The two rotations of the Green-Naghdi rate (linear_elastic.tcc:134-150):
+// Lambda_delta = R_new R_half^T (strain) and Lambda_Delta = R_new R_current^T
+// (stress), R the rotations of F, (F + F_new) / 2 and F_new = dF F.
+static __device__ __attribute__((noinline)) void green_naghdi_rotations(
+    const double (*F)[3], const double (*dF)[3], double (*L)[3], double (*LL)[3]) {
+  double Fn[3][3], Fh[3][3], Rc[3][3], Rh[3][3], Rn[3][3];
+  mat3_mul(dF, F, Fn);
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) Fh[i][j] = 0.5 * F[i][j] + 0.5 * Fn[i][j];
+  polar_rotation(F, Rc);
+  polar_rotation(Fh, Rh);
+  polar_rotation(Fn, Rn);
+  mat3_mul_t(Rn, Rh, L);
+  mat3_mul_t(Rn, Rc, LL);
+}
+
+// The trial stress of an objective rate (linear_elastic.tcc:109-124,
+// 152-164; infinitesimal_elasto_plastic.tcc:1-25): the stress s rotated by
+// LL, the strain increment ds (tensor shears) by L, plus the elastic
+// increment of the rotated strain. s: in/out.
+static __device__ __attribute__((noinline)) void rate_trial(const Material& m,
+                                                           const double (*L)[3],
+                                                           const double (*LL)[3],
+                                                           const double* ds, double* s) {
   const double S[3][3] = {{s[0], s[3], s[5]}, {s[3], s[1], s[4]}, {s[5], s[4], s[2]}};
   const double E[3][3] = {{ds[0], 0.5 * ds[3], 0.5 * ds[5]},
                           {0.5 * ds[3], ds[1], 0.5 * ds[4]},
@@ -649,6 +715,13 @@ static __device__ __attribute__((noinline)) void jaumann_le(const Material& m, c
   de_mul(m, re, d);
 #pragma unroll
   for (int v = 0; v < 6; ++v) s[v] = rs[v] + d[v];
+}
+
+// The Jaumann rotations (linear_elastic.tcc:102-107): L = exp(W/2) from the
+// entries a = 01, 02, 12 of W/2 (W the skew part of dF - I) and LL = L L.
+MPMX_DEV void jaumann_rotations(const double* a, double (*L)[3], double (*LL)[3]) {
+  skew_exp(a, L);
+  mat3_mul(L, L, LL);
 }
 
 // x^T D (D symmetric in structure; column sums in k order)
@@ -865,8 +938,11 @@ struct McIter {
 // complete (elastic or separated; stress and sv final). Otherwise `trial` and
 // T describe the plastic trial state, and sv holds the updated strength
 // parameters and yield_state.
+// trial_in: the trial stress of an objective rate
+// (InfinitesimalElastoPlastic::compute_trial_stress); null: stress + D dstrain.
 MPMX_DEV bool mc_begin(const Material& m, double* stress, const double* dstrain, double density,
-                       double* sv, double* trial, McTrial& T) {
+                       double* sv, double* trial, McTrial& T,
+                       const double* trial_in = nullptr) {
   const double pdstrain = sv[SV_PDS];
   if (m.softening && pdstrain > m.pd_peak) {
     if (pdstrain < m.pd_res) {
@@ -882,7 +958,9 @@ MPMX_DEV bool mc_begin(const Material& m, double* stress, const double* dstrain,
     if (sv[SV_TCUT] > apex) sv[SV_TCUT] = (apex > 1.0e-15 ? apex : 1.0e-15);
   }
   sv[SV_YIELD] = 0;
-  {
+  if (trial_in) {
+    for (int i = 0; i < 6; ++i) trial[i] = trial_in[i];
+  } else {
     double ds[6];
     de_mul(m, dstrain, ds);
     for (int i = 0; i < 6; ++i) trial[i] = stress[i] + ds[i];
@@ -1000,10 +1078,10 @@ MPMX_DEV bool mc_iterate(const Material& m, double* stress, double* sv, McIter& 
 
 // The whole update in one call. Updates `stress` in place.
 MPMX_DEV void mc_stress(const Material& m, double* stress, const double* dstrain, double density,
-                        double* sv) {
+                        double* sv, const double* trial_in = nullptr) {
   double trial[6];
   McTrial T;
-  if (mc_begin(m, stress, dstrain, density, sv, trial, T)) return;
+  if (mc_begin(m, stress, dstrain, density, sv, trial, T, trial_in)) return;
   McIter R;
   mc_first(m, stress, trial, dstrain, sv, T, R);
   mc_iterate(m, stress, sv, R, 100);

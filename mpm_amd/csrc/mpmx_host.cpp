@@ -124,7 +124,9 @@ struct mpmx_ctx {
   std::vector<Material> mats;
   DBuf<Material> d_mats;
   bool any_mc = false;
-  bool any_rate = false;  // a LinearElastic material with the Jaumann rate
+  bool any_rate = false;     // a material with an objective stress rate
+  bool any_gn = false;       // ... with the Green-Naghdi rate
+  bool any_mc_rate = false;  // a MohrCoulomb material with a stress rate
 
   // particle velocity constraints in the order given: (particle or -1 for
   // all, direction, value); applied per particle as a mask and values
@@ -158,11 +160,16 @@ struct mpmx_ctx {
   // the mesh instead of failing the step (mpm_scheme.tcc:245-261)
   bool locate_all = true;
   DBuf<double> d_x, d_u, d_v, d_stress, d_strain, d_F, d_mass, d_vol, d_rho, d_xi, d_sv;
+  // mapping matrices of the affine velocity updates (APIC, ASFLIP, TPIC)
+  DBuf<double> d_Bmat;
   DBuf<int> d_mat, d_cell;
 
   // step machinery
   DBuf<int> d_skey, d_perm, d_sort_tmp, d_scan_list, d_cstart, d_cend;
   DBuf<double> d_spin;          // Particles::spin (any_rate)
+  DBuf<double> d_rotF;          // Particles::rotF (any_gn)
+  DBuf<double> d_plastic_trial; // Particles::plastic_trial (any_mc_rate)
+  DBuf<double> d_Cmat, d_beta;  // affine transfer matrices, ASFLIP beta (affine())
   DBuf<int> d_plastic_list;     // deferred Mohr-Coulomb return mappings
   DBuf<double> d_plastic_rec;
   DBuf<Status> d_status;
@@ -217,6 +224,9 @@ struct mpmx_ctx {
     if (stream) (void)hipStreamDestroy(stream);
   }
 
+  // an affine velocity update: APIC, ASFLIP or TPIC
+  bool affine() const { return vu == ASFLIP || vu == TPIC || vu == APIC; }
+
   void invalidate_graph() {
     for (auto& g : graph_exec) {
       if (g) (void)hipGraphExecDestroy(g);
@@ -261,6 +271,11 @@ struct mpmx_ctx {
     P.pvc_mask = pvc.empty() ? nullptr : d_pvc_mask.p;
     P.pvc_val = pvc.empty() ? nullptr : d_pvc_val.p;
     P.spin = any_rate ? d_spin.p : nullptr;
+    P.rotF = any_gn ? d_rotF.p : nullptr;
+    P.plastic_trial = any_mc_rate ? d_plastic_trial.p : nullptr;
+    P.Bmat = affine() ? d_Bmat.p : nullptr;
+    P.Cmat = affine() ? d_Cmat.p : nullptr;
+    P.beta = affine() ? d_beta.p : nullptr;
     P.plastic_list = any_mc ? d_plastic_list.p : nullptr;
     P.plastic_rec = any_mc ? d_plastic_rec.p : nullptr;
     P.ntr = ntr;
@@ -495,13 +510,13 @@ Material make_material(const double* p) {
   m.E = E;
   if (!(m.density > 0) || !(E > 0) || !(nu > -1.0 && nu < 0.5))
     throw ApiError(MPMX_E_ARG, "material density, youngs_modulus or poisson_ratio out of range");
+  // objective stress rate (linear_elastic.tcc:8-17; MohrCoulomb inherits it,
+  // infinitesimal_elasto_plastic.tcc:1-25): parameter 4 of LinearElastic,
+  // 16 of MohrCoulomb
+  m.rate = (int)p[m.type == LE ? 4 : 16];
+  if (m.rate != RATE_NONE && m.rate != RATE_JAUMANN && m.rate != RATE_GREEN_NAGHDI)
+    throw ApiError(MPMX_E_ARG, "unknown stress rate");
   if (m.type == LE) {
-    // objective stress rate (linear_elastic.tcc:8-17): none or Jaumann
-    m.rate = (int)p[4];
-    if (m.rate == RATE_GREEN_NAGHDI)
-      throw ApiError(MPMX_E_UNSUPPORTED, "the Green-Naghdi stress rate is not supported");
-    if (m.rate != RATE_NONE && m.rate != RATE_JAUMANN)
-      throw ApiError(MPMX_E_ARG, "unknown stress rate");
     const double K = E / (3.0 * (1. - 2. * nu));  // linear_elastic.tcc:27
     const double G = E / (2.0 * (1. + nu));       // linear_elastic.tcc:55
     m.a1 = K + (4.0 / 3.0) * G;
@@ -779,6 +794,12 @@ void alloc_step_buffers(mpmx_ctx* c) {
   c->d_sort_tmp.alloc(radix_sort_temp_ints(std::max<int64_t>(np, 1)));
   c->d_scan_list.alloc(std::max<int64_t>(np, 1));
   if (c->any_rate) c->d_spin.alloc(3 * std::max<int64_t>(np, 1));
+  if (c->any_gn) c->d_rotF.alloc((size_t)(c->dim * 3 + c->dim * c->dim) * std::max<int64_t>(np, 1));
+  if (c->any_mc_rate) c->d_plastic_trial.alloc(6 * std::max<int64_t>(np, 1));
+  if (c->affine()) {
+    c->d_Cmat.alloc((size_t)c->dim * c->dim * std::max<int64_t>(np, 1));
+    c->d_beta.alloc(std::max<int64_t>(np, 1));
+  }
   if (c->any_mc) {
     c->d_plastic_list.alloc(std::max<int64_t>(np, 1));
     c->d_plastic_rec.alloc(PR_N * std::max<int64_t>(np, 1));
@@ -867,6 +888,7 @@ int64_t append_particles(mpmx_ctx* c, int64_t n, const double* x, const double* 
   grow(c->d_mat, off, np, 1, st);
   grow(c->d_cell, off, np, 1, st);
   if (c->any_mc) grow(c->d_sv, off, np, NSV, st);
+  grow(c->d_Bmat, off, np, d * d, st);
   c->np = np;
   c->ad_recorded = false;
   c->psize.resize(np);
@@ -881,6 +903,7 @@ int64_t append_particles(mpmx_ctx* c, int64_t n, const double* x, const double* 
   put_soa(c->d_u, np, off, n, d, u ? u : zeros.data(), st);
   put_soa(c->d_v, np, off, n, d, v ? v : zeros.data(), st);
   put_soa(c->d_xi, np, off, n, d, zeros.data(), st);
+  put_soa(c->d_Bmat, np, off, n, d * d, zeros.data(), st);  // mapping_matrix_ starts at 0
   put_soa(c->d_stress, np, off, n, 6, stress ? stress : zeros.data(), st);
   put_soa(c->d_strain, np, off, n, 6, strain ? strain : zeros.data(), st);
   put_soa(c->d_F, np, off, n, 9, F ? F : eye.data(), st);
@@ -927,6 +950,7 @@ void truncate_particles(mpmx_ctx* c, int64_t keep) {
   shrink(c->d_mass, 1); shrink(c->d_vol, 1); shrink(c->d_rho, 1);
   shrink(c->d_mat, 1); shrink(c->d_cell, 1);
   if (c->any_mc) shrink(c->d_sv, NSV);
+  shrink(c->d_Bmat, d * d);
   c->np = keep;
   c->ad_recorded = false;
   c->psize.resize(keep);
@@ -952,6 +976,10 @@ struct StepBuffers {
   double* plastic_rec;
   double* spin;
   double* fnval;  // the math functions' values at the step's time
+  double* Cmat;   // affine velocity updates: the transfer matrices, beta
+  double* beta;
+  double* rotF;   // Green-Naghdi scratch; rotated trial stresses of the deferred list
+  double* plastic_trial;
 };
 
 StepBuffers ctx_buffers(mpmx_ctx* c) {
@@ -967,6 +995,10 @@ StepBuffers ctx_buffers(mpmx_ctx* c) {
   B.plastic_rec = c->d_plastic_rec.p;
   B.spin = c->d_spin.p;
   B.fnval = c->d_fnval.p;
+  B.Cmat = c->d_Cmat.p;
+  B.beta = c->d_beta.p;
+  B.rotF = c->d_rotF.p;
+  B.plastic_trial = c->d_plastic_trial.p;
   return B;
 }
 
@@ -1018,6 +1050,12 @@ void enqueue_step(mpmx_ctx* c, const StepArgs& A, Launcher& L, bool full_sort,
     P.plastic_rec = B.plastic_rec;
   }
   if (P.spin) P.spin = B.spin;
+  if (P.Cmat) {
+    P.Cmat = B.Cmat;
+    P.beta = B.beta;
+  }
+  if (P.rotF) P.rotF = B.rotF;
+  if (P.plastic_trial) P.plastic_trial = B.plastic_trial;
   Nodes Nd = c->nodes();
   if (Nd.fnval) Nd.fnval = B.fnval;
   hipStream_t st = c->stream;
@@ -1038,6 +1076,15 @@ void enqueue_step(mpmx_ctx* c, const StepArgs& A, Launcher& L, bool full_sort,
   auto plastic = [&] {
     if (c->any_mc) L("mc_plastic", [&] { launch_plastic(P, mats, S, st); });
   };
+  // affine velocity updates (APIC, ASFLIP, TPIC): the transfer matrix before
+  // each particle-to-grid map, the mapping matrix before grid-to-particle
+  const bool aff = c->affine();
+  auto affine = [&] {
+    if (aff) L("affine_matrix", [&] { launch_affine_matrix<D>(M, P, c->vu, S, st); });
+  };
+  auto mapping = [&] {
+    if (aff) L("mapping_matrix", [&] { launch_mapping_matrix<D>(M, P, Nd, A, c->vu, S, st); });
+  };
   if (full_sort || ordered)
     L("sort_by_cell", [&] {
       radix_sort_by_cell(P.cell, P.np, (int)M.ncells, skey, perm, B.sort_tmp, st);
@@ -1050,42 +1097,57 @@ void enqueue_step(mpmx_ctx* c, const StepArgs& A, Launcher& L, bool full_sort,
     });
     const int* cs = B.cstart;
     const int* ce = B.cend;
+    affine();
     if (c->scheme == USF) {
       L("gather_mass_momentum", [&] { launch_gather<D>(GK_MV_VEL, M, P, Nd, A, S, perm, cs, ce, st); });
       L("stress", [&] { launch_stress<D>(false, M, P, Nd, mats, A, S, scan, st); });
       plastic();
       L("gather_forces", [&] { launch_gather<D>(GK_F_ACC, M, P, Nd, A, S, perm, cs, ce, st); });
+      mapping();
       L("g2p_locate", [&] { launch_g2p<D>(false, loc, M, P, Nd, mats, A, c->vu, S, scan, st); });
     } else if (c->scheme == USL) {
       L("gather_all", [&] { launch_gather<D>(GK_ALL_VEL_ACC, M, P, Nd, A, S, perm, cs, ce, st); });
+      mapping();
       L("g2p_stress_locate", [&] { launch_g2p<D>(true, loc, M, P, Nd, mats, A, c->vu, S, scan, st); });
       plastic();
     } else {
       L("gather_all", [&] { launch_gather<D>(GK_ALL_VEL_ACC, M, P, Nd, A, S, perm, cs, ce, st); });
+      mapping();
       L("g2p", [&] { launch_g2p<D>(false, false, M, P, Nd, mats, A, c->vu, S, scan, st); });
+      affine();
       L("gather_momentum", [&] { launch_gather<D>(GK_MOM_VEL, M, P, Nd, A, S, perm, cs, ce, st); });
       L("stress_locate", [&] { launch_stress<D>(loc, M, P, Nd, mats, A, S, scan, st); });
       plastic();
     }
   } else {
+    affine();
     if (c->scheme == USF) {
       L("p2g_mass_momentum", [&] { launch_p2g<D>(P2G_MV, c->chunk, M, P, Nd, A, S, skey, perm, st); });
       L("node_velocity", [&] { launch_node<D>(ND_INIT | ND_VEL, M, Nd, A, S, st); });
       L("stress_force_scatter", [&] { launch_stress_scatter<D>(c->chunk, M, P, Nd, mats, A, S, skey, perm, st); });
       if (P.ntr > 0) L("traction_scatter", [&] { launch_traction_scatter<D>(M, P, Nd, S, st); });
       L("node_acceleration", [&] { launch_node<D>(ND_ACC | ND_ZERO_ALL, M, Nd, A, S, st); });
+      mapping();
       L("g2p_locate", [&] { launch_g2p<D>(false, loc, M, P, Nd, mats, A, c->vu, S, scan, st); });
     } else if (c->scheme == USL) {
       L("p2g_all", [&] { launch_p2g<D>(P2G_MV_F, c->chunk, M, P, Nd, A, S, skey, perm, st); });
       if (P.ntr > 0) L("traction_scatter", [&] { launch_traction_scatter<D>(M, P, Nd, S, st); });
       L("node_update", [&] { launch_node<D>(ND_INIT | ND_VEL | ND_ACC | ND_ZERO_ALL, M, Nd, A, S, st); });
+      mapping();
       L("g2p_stress_locate", [&] { launch_g2p<D>(true, loc, M, P, Nd, mats, A, c->vu, S, scan, st); });
       plastic();
     } else {
       L("p2g_all", [&] { launch_p2g<D>(P2G_MV_F, c->chunk, M, P, Nd, A, S, skey, perm, st); });
       if (P.ntr > 0) L("traction_scatter", [&] { launch_traction_scatter<D>(M, P, Nd, S, st); });
       L("node_update", [&] { launch_node<D>(ND_INIT | ND_VEL | ND_ACC | ND_ZERO_MOM, M, Nd, A, S, st); });
-      L("g2p_momentum_scatter", [&] { launch_g2p_scatter<D>(c->chunk, M, P, Nd, A, c->vu, S, skey, perm, st); });
+      if (aff) {  // G2P, then the affine momentum map with the new positions
+        mapping();
+        L("g2p", [&] { launch_g2p<D>(false, false, M, P, Nd, mats, A, c->vu, S, scan, st); });
+        affine();
+        L("p2g_momentum", [&] { launch_p2g<D>(P2G_MOM, c->chunk, M, P, Nd, A, S, skey, perm, st); });
+      } else {
+        L("g2p_momentum_scatter", [&] { launch_g2p_scatter<D>(c->chunk, M, P, Nd, A, c->vu, S, skey, perm, st); });
+      }
       L("node_velocity", [&] { launch_node<D>(ND_VEL | ND_ZERO_ALL, M, Nd, A, S, st); });
       L("stress_locate", [&] { launch_stress<D>(loc, M, P, Nd, mats, A, S, scan, st); });
       plastic();
@@ -1214,6 +1276,7 @@ void remove_lost_particles(mpmx_ctx* c) {
   take(c->d_mass, 1); take(c->d_vol, 1); take(c->d_rho, 1);
   take(c->d_mat, 1); take(c->d_cell, 1);
   if (c->any_mc) take(c->d_sv, NSV);
+  take(c->d_Bmat, d * d);
   std::vector<double> psize(nk);
   std::vector<int64_t> ids(nk);
   for (int64_t i = 0; i < nk; ++i) {
@@ -1312,6 +1375,14 @@ size_t carve_step_buffers(const mpmx_ctx* c, uintptr_t base, StepBuffers* B) {
   }
   B->spin = c->any_rate ? (double*)take(3 * np * sizeof(double)) : nullptr;
   B->fnval = c->fns.empty() ? nullptr : (double*)take(c->fns.size() * sizeof(double));
+  B->rotF = c->any_gn ? (double*)take((size_t)(c->dim * 3 + c->dim * c->dim) * np * sizeof(double))
+                      : nullptr;
+  B->plastic_trial = c->any_mc_rate ? (double*)take(6 * np * sizeof(double)) : nullptr;
+  B->Cmat = B->beta = nullptr;
+  if (c->affine()) {
+    B->Cmat = (double*)take((size_t)c->dim * c->dim * np * sizeof(double));
+    B->beta = (double*)take(np * sizeof(double));
+  }
   return off;
 }
 
@@ -1328,6 +1399,7 @@ std::vector<StateArray> ad_state(mpmx_ctx* c) {
                                {c->d_stress.p, 6}, {c->d_strain.p, 6}, {c->d_F.p, 9},
                                {c->d_vol.p, 1},    {c->d_rho.p, 1},    {c->d_xi.p, d}};
   if (c->any_mc) v.push_back({c->d_sv.p, NSV});
+  if (c->affine()) v.push_back({c->d_Bmat.p, (int64_t)d * d});
   return v;
 }
 
@@ -1433,6 +1505,8 @@ int64_t ad_part_doubles(const mpmx_ctx* c) { return (3 * c->dim + 10) * c->np; }
 // forces and acceleration constraints, with or without math functions, are
 // constants of the step and are supported.
 void ad_check_supported(mpmx_ctx* c) {
+  if (c->affine())
+    throw ApiError(MPMX_E_UNSUPPORTED, "adjoint: APIC, ASFLIP and TPIC are not supported");
   if (c->iso)
     throw ApiError(MPMX_E_UNSUPPORTED, "adjoint: isoparametric meshes are not supported");
   if (!c->trac.empty())
@@ -1440,7 +1514,7 @@ void ad_check_supported(mpmx_ctx* c) {
   for (const auto& m : c->mats) {
     if (m.type == MC && m.softening)
       throw ApiError(MPMX_E_UNSUPPORTED, "adjoint: Mohr-Coulomb strain softening is not supported");
-    if (m.type == LE && m.rate != RATE_NONE)
+    if (m.rate != RATE_NONE)
       throw ApiError(MPMX_E_UNSUPPORTED, "adjoint: objective stress rates are not supported");
   }
   for (const auto& kv : c->cons) {
@@ -1514,7 +1588,7 @@ int mpmx_create(mpmx_ctx** out, int dim, int scheme, int velocity_update, double
   if (!out) return MPMX_E_ARG;
   *out = nullptr;
   if ((dim != 2 && dim != 3) || scheme < 0 || scheme > 2 || velocity_update < 0 ||
-      velocity_update > 1)
+      velocity_update > 4)
     return MPMX_E_ARG;
   auto* c = new mpmx_ctx();
   c->dim = dim;
@@ -1725,11 +1799,13 @@ int mpmx_set_materials(mpmx_ctx* c, int n, const double* params) {
   if (c->np > 0) throw ApiError(MPMX_E_STATE, "materials must be set before particles");
   c->mats.clear();
   c->any_mc = false;
-  c->any_rate = false;
+  c->any_rate = c->any_gn = c->any_mc_rate = false;
   for (int i = 0; i < n; ++i) {
     c->mats.push_back(make_material(params + (size_t)i * MPMX_MATERIAL_PARAMS));
     if (c->mats.back().type == MC) c->any_mc = true;
-    if (c->mats.back().type == LE && c->mats.back().rate == RATE_JAUMANN) c->any_rate = true;
+    if (c->mats.back().rate != RATE_NONE) c->any_rate = true;
+    if (c->mats.back().rate == RATE_GREEN_NAGHDI) c->any_gn = true;
+    if (c->mats.back().type == MC && c->mats.back().rate != RATE_NONE) c->any_mc_rate = true;
   }
   c->d_mats.alloc(n);
   c->d_mats.upload(c->mats.data(), n, c->stream);
@@ -2140,6 +2216,33 @@ int mpmx_get_nodes(mpmx_ctx* c, double* mass, double* velocity, double* accelera
   }
   if (velocity) get_soa(c->d_nvel, nn, d, velocity, st);
   if (acceleration) get_soa(c->d_nacc, nn, d, acceleration, st);
+  API_END(c)
+}
+
+int mpmx_get_mapping_matrix(mpmx_ctx* c, double* B) {
+  API_BEGIN(c)
+  if (!B) throw ApiError(MPMX_E_ARG, "null array");
+  const int64_t np = c->np;
+  const int dd = c->dim * c->dim;
+  if (np == 0) return MPMX_OK;
+  std::vector<double> soa((size_t)dd * np);
+  c->d_Bmat.download(soa.data(), soa.size(), c->stream);
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  for (int64_t p = 0; p < np; ++p)
+    for (int k = 0; k < dd; ++k) B[p * dd + k] = soa[(size_t)k * np + p];
+  API_END(c)
+}
+
+int mpmx_set_mapping_matrix(mpmx_ctx* c, int64_t n, const double* B) {
+  API_BEGIN(c)
+  if (n != c->np || (n > 0 && !B)) throw ApiError(MPMX_E_ARG, "one matrix per particle required");
+  const int dd = c->dim * c->dim;
+  if (n == 0) return MPMX_OK;
+  std::vector<double> soa((size_t)dd * n);
+  for (int64_t p = 0; p < n; ++p)
+    for (int k = 0; k < dd; ++k) soa[(size_t)k * n + p] = B[p * dd + k];
+  c->d_Bmat.upload(soa.data(), soa.size(), c->stream);
+  HIPCHECK(hipStreamSynchronize(c->stream));
   API_END(c)
 }
 

@@ -12,14 +12,15 @@ namespace mpmx {
 enum { LE = 0, MC = 1 };
 enum { SV_YIELD = 0, SV_PHI, SV_PSI, SV_COH, SV_TCUT, SV_EPS, SV_RHO, SV_THETA, SV_PDS, NSV };
 enum { USF = 0, USL = 1, MUSL = 2 };
-enum { FLIP = 0, PIC = 1 };
+// velocity updates, the order of mpm::VelocityUpdate (data_types.h:50)
+enum { FLIP = 0, PIC = 1, ASFLIP = 2, TPIC = 3, APIC = 4 };
 
 // Material constants, derived once on the host with the reference's
 // expressions (linear_elastic.tcc:20-69, mohr_coulomb.tcc:9-77, 517-533).
 struct Material {
   int type;
   int softening;
-  int rate;  // LinearElastic objective stress rate: RATE_NONE or RATE_JAUMANN
+  int rate;  // objective stress rate: RATE_NONE, RATE_JAUMANN or RATE_GREEN_NAGHDI
   double density;
   double a1, a2, G;  // elastic tensor entries: D = [a1 a2 a2; a2 a1 a2; a2 a2 a1] (+) G*I3
   double shear;      // MohrCoulomb shear_modulus_ (pdstrain of separated state)
@@ -118,6 +119,13 @@ struct Particles {
   // thread (per-step scratch; 2D uses entry 01 only). Null without a
   // Jaumann-rate material.
   double* spin;
+  // Green-Naghdi rate: [D*3 + D*D][np] the rows 0..D-1 of the step-start F
+  // and the block of dF, stored by the kinematics for the constitutive
+  // update of the same thread (per-step scratch; null without such a
+  // material). Mohr-Coulomb with a stress rate: [6][np] the rotated trial
+  // stress of each deferred return mapping, by list slot (null without).
+  double* rotF;
+  double* plastic_trial;
   int* plastic_list;
   double* plastic_rec;  // [PR_N][np] by list slot
   // Particle tractions (Particle::assign_traction / map_traction_force,
@@ -130,6 +138,14 @@ struct Particles {
   const double* tr_val;
   const int* tr_fn;
   const double* psize;
+  // Affine velocity updates (APIC, ASFLIP, TPIC): the mapping matrix B
+  // (mapping_matrix_, [D*D][np], row-major per particle) kept between steps;
+  // the matrix C of the next particle-to-grid transfer ([D*D][np], momentum
+  // m N_i (v + C (x_i - x_p))); ASFLIP's beta of the step ([np]). Null for
+  // FLIP and PIC.
+  double* Bmat;
+  double* Cmat;
+  double* beta;
 };
 
 // Record of a deferred return mapping: the strain increment and what the

@@ -10,7 +10,7 @@ namespace mpmx {
 // node-kernel flags
 enum { ND_VEL = 1, ND_ACC = 2, ND_ZERO_MOM = 4, ND_ZERO_ALL = 8, ND_INIT = 16 };
 // scatter contents
-enum { P2G_MV = 1, P2G_MV_F = 3 };
+enum { P2G_MV = 1, P2G_MV_F = 3, P2G_MOM = 4 };  // P2G_MOM: affine velocity updates only
 // ordered-mode gathers
 enum { GK_MV_VEL = 0, GK_F_ACC = 1, GK_ALL_VEL_ACC = 2, GK_MOM_VEL = 3 };
 
@@ -22,6 +22,15 @@ void launch_locate_init(const Mesh& M, const Particles& P, Status* S, int* scan_
 template <int D>
 void launch_locate_step(const Mesh& M, const Particles& P, Status* S, int* scan_list,
                         hipStream_t st);
+// Affine velocity updates (APIC, ASFLIP, TPIC): the mapping matrices (and
+// ASFLIP's beta) before G2P; the matrix of the next particle-to-grid
+// transfer before each P2G.
+template <int D>
+void launch_mapping_matrix(const Mesh& M, const Particles& P, const Nodes& Nd,
+                           const StepArgs& A, int vu, const Status* S, hipStream_t st);
+template <int D>
+void launch_affine_matrix(const Mesh& M, const Particles& P, int vu, const Status* S,
+                          hipStream_t st);
 // remove_lost: a particle found in no cell is counted in Status::nlost for
 // removal instead of stopping the run (Status::first_bad_step).
 template <int D>

@@ -189,6 +189,16 @@ MPMX_DEV void kinematics(const Mesh& M, const Particles& P, int p, int c, const 
       }
   }
   if constexpr (ROT) {  // stored rather than kept live across the stress update
+    if (P.rotF) {  // Green-Naghdi: the step-start F rows and the dF block
+#pragma unroll
+      for (int a = 0; a < D; ++a)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) P.rotF[(a * 3 + j) * np + p] = Fr[a][j];
+#pragma unroll
+      for (int a = 0; a < D; ++a)
+#pragma unroll
+        for (int b = 0; b < D; ++b) P.rotF[(D * 3 + a * D + b) * np + p] = R[a][b];
+    }
     P.spin[p] = 0.5 * (0.5 * (R[0][1] - R[1][0]));
     if constexpr (D == 3) {
       P.spin[np + p] = 0.5 * (0.5 * (R[0][2] - R[2][0]));
@@ -214,7 +224,7 @@ MPMX_DEV void store_sv(const Particles& P, const Material& m, int64_t p, const d
 // tensile ones from the back, so that k_plastic's waves mostly run one
 // branch of the flow rule (one atomic per wave and kind on the counts).
 MPMX_DEV void defer_plastic(const Particles& P, Status* S, int p, const double* ds,
-                            const McTrial& T) {
+                            const McTrial& T, const double* trial = nullptr) {
   const bool tensile = T.yt == FS_TENSILE;
   const unsigned long long mask = __ballot(tensile);
   const unsigned long long all = __ballot(1);
@@ -239,6 +249,10 @@ MPMX_DEV void defer_plastic(const Particles& P, Status* S, int p, const double* 
   r[PR_S3 * np] = T.I.s3;
   r[PR_C3 * np] = T.I.c3;
   r[PR_YTRIAL * np] = T.yield_trial;
+  if (P.plastic_trial) {  // a stress rate: the (rotated) trial stress itself
+#pragma unroll
+    for (int c = 0; c < 6; ++c) P.plastic_trial[c * np + slot] = trial[c];
+  }
 }
 
 // compute_stress_strain for one particle: kinematics, then the constitutive
@@ -264,15 +278,37 @@ MPMX_DEV void stress_strain(const Mesh& M, const Particles& P, const Material* m
 #pragma unroll
     for (int k = 0; k < NSV; ++k) sv[k] = P.sv[k * np + p];
   }
-  bool jaumann = false;
-  if constexpr (ROT) jaumann = m.type == LE && m.rate == RATE_JAUMANN;
-  if (jaumann) {
-    double a[3] = {P.spin[p], 0., 0.};
-    if constexpr (D == 3) {
-      a[1] = P.spin[np + p];
-      a[2] = P.spin[2 * np + p];
+  // an objective stress rate: the rotated trial stress (ROT builds only)
+  int rate = RATE_NONE;
+  if constexpr (ROT) rate = m.rate;
+  double rtrial[6];
+  if (rate != RATE_NONE) {
+    double L[3][3], LL[3][3];
+    if (rate == RATE_JAUMANN) {
+      double a[3] = {P.spin[p], 0., 0.};
+      if constexpr (D == 3) {
+        a[1] = P.spin[np + p];
+        a[2] = P.spin[2 * np + p];
+      }
+      jaumann_rotations(a, L, LL);
+    } else {  // Green-Naghdi: F (step start) and dF as 3x3
+      double F0[3][3], G[3][3];
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          F0[a][j] = a < D ? P.rotF[(a * 3 + j) * np + p] : P.F[(a * 3 + j) * np + p];
+          G[a][j] = (a < D && j < D) ? P.rotF[(D * 3 + a * D + j) * np + p] : (a == j ? 1. : 0.);
+        }
+      green_naghdi_rotations(F0, G, L, LL);
     }
-    jaumann_le(m, a, ds, s);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) rtrial[k] = s[k];
+    rate_trial(m, L, LL, ds, rtrial);
+  }
+  if (m.type == LE && rate != RATE_NONE) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) s[k] = rtrial[k];
   } else if (m.type == LE) {
     double t[6];
     de_mul(m, ds, t);
@@ -281,7 +317,7 @@ MPMX_DEV void stress_strain(const Mesh& M, const Particles& P, const Material* m
   } else if constexpr (DEFER) {
     double trial[6];
     McTrial T;
-    if (!mc_begin(m, s, ds, rho, sv, trial, T)) {
+    if (!mc_begin(m, s, ds, rho, sv, trial, T, rate != RATE_NONE ? rtrial : nullptr)) {
       // the yield state (and softened strength parameters) now, the rest of
       // the state variables and the stress in k_plastic
       P.sv[SV_YIELD * np + p] = sv[SV_YIELD];
@@ -291,12 +327,12 @@ MPMX_DEV void stress_strain(const Mesh& M, const Particles& P, const Material* m
         P.sv[SV_COH * np + p] = sv[SV_COH];
         P.sv[SV_TCUT * np + p] = sv[SV_TCUT];
       }
-      defer_plastic(P, S, p, ds, T);
+      defer_plastic(P, S, p, ds, T, trial);
       return;
     }
     store_sv(P, m, p, sv);
   } else {
-    mc_stress(m, s, ds, rho, sv);
+    mc_stress(m, s, ds, rho, sv, rate != RATE_NONE ? rtrial : nullptr);
     store_sv(P, m, p, sv);
   }
 #pragma unroll
@@ -477,8 +513,9 @@ __global__ void k_step_end(Status* S) {
 }
 
 // ---------------- fast scatter: LDS-staged segmented reduction --------------
-// What the staging carries per particle.
-enum { SC_MV = 1, SC_F = 2, SC_MOM = 4 };
+// What the staging carries per particle. SC_AFF: the momentum is mapped with
+// the affine velocity v + C (x_i - x_p) (APIC, ASFLIP, TPIC).
+enum { SC_MV = 1, SC_F = 2, SC_MOM = 4, SC_AFF = 8 };
 
 // CH: particles per workgroup (one per thread), a multiple of 64.
 template <int D, int WHAT, int CH>
@@ -489,10 +526,30 @@ struct Stage {
   double m[CH];
   double v[D][CH];
   double B[(WHAT & SC_F) ? D * D : 1][CH];  // force matrix, see force_matrix()
+  double C[(WHAT & SC_AFF) ? D * D : 1][CH];  // affine velocity matrix
+  double xp[(WHAT & SC_AFF) ? D : 1][CH];     // particle position
   int seg_start[CH + 1];
   int nseg;
   int wave_heads[CH / 64];
 };
+
+// The affine velocity of a particle at node position xn: v + C (xn - xp)
+// (map_mass_momentum_to_nodes_affine / _taylor, particle.tcc:661-724), with
+// C = B D^-1 (APIC, ASFLIP) or B (TPIC) precomputed per transfer.
+template <int D>
+MPMX_DEV void affine_velocity(const double* v, const double* C, const double* xp,
+                              const double* xn, double* mv) {
+  double r[D];
+#pragma unroll
+  for (int k = 0; k < D; ++k) r[k] = xn[k] - xp[k];
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    double t = C[a * D] * r[0];
+#pragma unroll
+    for (int k = 1; k < D; ++k) t = t + C[a * D + k] * r[k];
+    mv[a] = v[a] + t;
+  }
+}
 
 // Internal force of a particle on node i (particle.tcc:914-950) is
 // f_i = -V (sigma . dN_i/dx) with dN_i/dx = J^-1 grad_i(xi), i.e.
@@ -618,22 +675,34 @@ MPMX_DEV void scatter_segments(Stage<D, WHAT, CH>& st, int cnt, const Mesh& M, c
     double am = 0., amom[D], afe[D], afi[D];
 #pragma unroll
     for (int k = 0; k < D; ++k) amom[k] = afe[k] = afi[k] = 0.;
+    double xn[D];  // the node's position (affine transfers)
+    if constexpr ((WHAT & SC_AFF) != 0) {
+      const int nid = M.cell_nodes[(int64_t)st.key[b] * NN + i];
+#pragma unroll
+      for (int k = 0; k < D; ++k) xn[k] = M.node_x[k * M.nnodes + nid];
+    }
     for (int t = b; t < e && st.key[t] >= 0; ++t) {
       double xi[D];
 #pragma unroll
       for (int k = 0; k < D; ++k) xi[k] = st.xi[k][t];
       const double N = shapefn_i<D>(xi, i);
       const double m = st.m[t];
-      if constexpr ((WHAT & SC_MV) != 0) {
+      if constexpr ((WHAT & (SC_MV | SC_MOM)) != 0) {
         const double mN = m * N;
-        am = am + mN;
+        if constexpr ((WHAT & SC_MV) != 0) am = am + mN;
+        double vel[D];
 #pragma unroll
-        for (int k = 0; k < D; ++k) amom[k] = amom[k] + mN * st.v[k][t];
-      }
-      if constexpr ((WHAT & SC_MOM) != 0) {
-        const double mN = m * N;
+        for (int k = 0; k < D; ++k) vel[k] = st.v[k][t];
+        if constexpr ((WHAT & SC_AFF) != 0) {
+          double Cp[D * D], xp[D];
 #pragma unroll
-        for (int k = 0; k < D; ++k) amom[k] = amom[k] + mN * st.v[k][t];
+          for (int k = 0; k < D * D; ++k) Cp[k] = st.C[k][t];
+#pragma unroll
+          for (int k = 0; k < D; ++k) xp[k] = st.xp[k][t];
+          affine_velocity<D>(vel, Cp, xp, xn, vel);
+        }
+#pragma unroll
+        for (int k = 0; k < D; ++k) amom[k] = amom[k] + mN * vel[k];
       }
       if constexpr ((WHAT & SC_F) != 0) {
 #pragma unroll
@@ -719,6 +788,12 @@ __global__ __launch_bounds__(CH) void k_p2g(Mesh M, Particles P, Nodes Nd, StepA
         st.v[k][tid] = P.v[k * np + p];
       }
       st.m[tid] = P.mass[p];
+      if constexpr ((WHAT & SC_AFF) != 0) {
+#pragma unroll
+        for (int k = 0; k < D * D; ++k) st.C[k][tid] = P.Cmat[k * np + p];
+#pragma unroll
+        for (int k = 0; k < D; ++k) st.xp[k][tid] = P.x[k * np + p];
+      }
       if constexpr ((WHAT & SC_F) != 0) {
         double sg[6];
 #pragma unroll
@@ -852,20 +927,26 @@ __global__ __launch_bounds__(kBlock) MPMX_PRED_ATTR void k_g2p(Mesh M, Particles
     u0[k] = P.u[k * np + p];
   }
   const int pvc = P.pvc_mask ? P.pvc_mask[p] : 0;
+  const double beta = vu == ASFLIP ? P.beta[p] : 0.;
 #pragma unroll
   for (int k = 0; k < D; ++k) {
     double v = v0[k];
+    double dx = sv[k];  // position increment velocity
     if (vu == FLIP) {
       v += sa[k] * A.dt;
       v = A.blending * v + (1.0 - A.blending) * sv[k];
+    } else if (vu == ASFLIP) {  // particle.tcc:1097-1111
+      const double flip = v + sa[k] * A.dt;
+      v = A.blending * flip + (1.0 - A.blending) * sv[k];
+      dx = beta * A.blending * flip + (1.0 - beta * A.blending) * sv[k];
     } else {
       v = sv[k];
     }
     if (pvc & (1 << k)) v = P.pvc_val[k * np + p];  // particle.tcc:1190-1195
     P.v[k * np + p] = v;
-    x[k] = x[k] + sv[k] * A.dt;
+    x[k] = x[k] + dx * A.dt;
     P.x[k * np + p] = x[k];
-    P.u[k * np + p] = u0[k] + sv[k] * A.dt;
+    P.u[k * np + p] = u0[k] + dx * A.dt;
   }
   if constexpr (STRESS) {
     double dndx[NN][D], Ji[D][D];
@@ -874,6 +955,160 @@ __global__ __launch_bounds__(kBlock) MPMX_PRED_ATTR void k_g2p(Mesh M, Particles
     stress_strain<D, true, ROT>(M, P, mats, (int)p, c, dndx, nv, A.dt, s, &vol, S);
   }
   if constexpr (LOCATE) locate_one<D>(M, P, S, scan_list, (int)p, x, c, -1);
+}
+
+// The mapping matrices of the affine velocity updates at the start of G2P
+// (before the positions move), |B_ij| < 1e-15 -> 0:
+// compute_affine_mapping_matrix (particle.tcc:1687-1708),
+// B = sum_i N_i v_i (x_i - x_p)^T for APIC and ASFLIP, and
+// compute_velocity_gradient (:1662-1684), B = sum_i v_i dN_i/dx^T, for TPIC.
+// ASFLIP also gets its beta (compute_asflip_beta, :1711-1730): 0 when a node
+// of the cell has a velocity constraint or the deformation gradient
+// increment of the current nodal velocities compresses (det < 1), else 1.
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_mapping_matrix(Mesh M, Particles P, Nodes Nd,
+                                                           StepArgs A, int vu,
+                                                           const Status* S) {
+  if (run_stopped(S)) return;
+  constexpr int NN = Elem<D>::NN;
+  const int64_t p = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+  if (p >= P.np) return;
+  const int64_t np = P.np;
+  const int c = P.cell[p];
+  if (!valid_cell<D>(M, c)) return;
+  double xi[D], xp[D];
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    xi[k] = P.xi[k * np + p];
+    xp[k] = P.x[k * np + p];
+  }
+  double nv[NN][D];
+  gather_nodes<D>(M, Nd, c, nv, nullptr);
+  double B[D][D];
+#pragma unroll
+  for (int a = 0; a < D; ++a)
+#pragma unroll
+    for (int b = 0; b < D; ++b) B[a][b] = 0.;
+  double dndx[NN][D];
+  if (vu == TPIC || vu == ASFLIP) {
+    double Ji[D][D];
+    particle_dndx<D>(M, c, xi, dndx, Ji);
+  }
+  if (vu == TPIC) {
+#pragma unroll
+    for (int i = 0; i < NN; ++i)
+#pragma unroll
+      for (int a = 0; a < D; ++a)
+#pragma unroll
+        for (int b = 0; b < D; ++b) B[a][b] = B[a][b] + nv[i][a] * dndx[i][b];
+  } else {
+    double N[NN], X[NN][D];
+    shapefn<D>(xi, N);
+    load_cell_X<D>(M, c, X);
+#pragma unroll
+    for (int i = 0; i < NN; ++i)
+#pragma unroll
+      for (int a = 0; a < D; ++a)
+#pragma unroll
+        for (int b = 0; b < D; ++b) B[a][b] = B[a][b] + (N[i] * nv[i][a]) * (X[i][b] - xp[b]);
+  }
+#pragma unroll
+  for (int a = 0; a < D; ++a)
+#pragma unroll
+    for (int b = 0; b < D; ++b) P.Bmat[(a * D + b) * np + p] = fabs(B[a][b]) < 1.E-15 ? 0. : B[a][b];
+  if (vu != ASFLIP) return;
+  double beta = 1.0;
+#pragma unroll
+  for (int i = 0; i < NN; ++i) {
+    const int n = M.cell_nodes[(int64_t)c * NN + i];
+    const int ci = Nd.cons ? Nd.cons[n] : -1;
+    if (ci >= 0 && Nd.table[ci].vmask != 0) beta = 0.0;
+  }
+  if (beta != 0.0) {
+    double R[3][3] = {{1., 0., 0.}, {0., 1., 0.}, {0., 0., 1.}};
+#pragma unroll
+    for (int i = 0; i < NN; ++i)
+#pragma unroll
+      for (int a = 0; a < D; ++a)
+#pragma unroll
+        for (int b = 0; b < D; ++b) R[a][b] = R[a][b] + (nv[i][a] * dndx[i][b]) * A.dt;
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+#pragma unroll
+      for (int b = 0; b < D; ++b) {
+        if (a != b && fabs(R[a][b]) < 1.E-15) R[a][b] = 0.;
+        if (a == b && fabs(R[a][b] - 1.) < 1.E-15) R[a][b] = 1.;
+      }
+    const double det = R[0][0] * (R[1][1] * R[2][2] - R[2][1] * R[1][2]) -
+                       R[1][0] * (R[0][1] * R[2][2] - R[2][1] * R[0][2]) +
+                       R[2][0] * (R[0][1] * R[1][2] - R[1][1] * R[0][2]);
+    if (det < 1.0) beta = 0.0;
+  }
+  P.beta[p] = beta;
+}
+
+// The matrix C of the next affine particle-to-grid transfer, momentum
+// m N_i (v + C (x_i - x_p)): C = B D^-1 with the shape tensor
+// D = sum_i N_i (x_i - x_p)(x_i - x_p)^T for APIC and ASFLIP
+// (map_mass_momentum_to_nodes_affine, particle.tcc:661-696), C = B for TPIC
+// (_taylor, :699-724). N at the particle's local coordinates, x_p its
+// current position.
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_affine_matrix(Mesh M, Particles P, int vu,
+                                                          const Status* S) {
+  if (run_stopped(S)) return;
+  constexpr int NN = Elem<D>::NN;
+  const int64_t p = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+  if (p >= P.np) return;
+  const int64_t np = P.np;
+  const int c = P.cell[p];
+  if (!valid_cell<D>(M, c)) return;
+  double B[D][D];
+#pragma unroll
+  for (int a = 0; a < D; ++a)
+#pragma unroll
+    for (int b = 0; b < D; ++b) B[a][b] = P.Bmat[(a * D + b) * np + p];
+  if (vu == TPIC) {
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+#pragma unroll
+      for (int b = 0; b < D; ++b) P.Cmat[(a * D + b) * np + p] = B[a][b];
+    return;
+  }
+  double xi[D], xp[D];
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    xi[k] = P.xi[k * np + p];
+    xp[k] = P.x[k * np + p];
+  }
+  double N[NN], X[NN][D];
+  shapefn<D>(xi, N);
+  load_cell_X<D>(M, c, X);
+  double Dm[D][D], Di[D][D];
+#pragma unroll
+  for (int a = 0; a < D; ++a)
+#pragma unroll
+    for (int b = 0; b < D; ++b) Dm[a][b] = 0.;
+#pragma unroll
+  for (int i = 0; i < NN; ++i) {
+    double r[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) r[k] = X[i][k] - xp[k];
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+#pragma unroll
+      for (int b = 0; b < D; ++b) Dm[a][b] = Dm[a][b] + (N[i] * r[a]) * r[b];
+  }
+  mat_inverse<D>(Dm, Di);
+#pragma unroll
+  for (int a = 0; a < D; ++a)
+#pragma unroll
+    for (int b = 0; b < D; ++b) {
+      double t = B[a][0] * Di[0][b];
+#pragma unroll
+      for (int k = 1; k < D; ++k) t = t + B[a][k] * Di[k][b];
+      P.Cmat[(a * D + b) * np + p] = t;
+    }
 }
 
 // The deferred Mohr-Coulomb return mappings of this step: the rest of
@@ -916,7 +1151,10 @@ __global__ __launch_bounds__(kPlasticBlock) MPMX_PLASTIC_ATTR void k_plastic(Par
     sv[SV_PDS] = P.sv[SV_PDS * np + p];
     const Material& m = mats[P.mat[p]];
     double trial[6];
-    {
+    if (P.plastic_trial) {  // stress rates: the trial stress the predictor used
+#pragma unroll
+      for (int k = 0; k < 6; ++k) trial[k] = P.plastic_trial[k * np + slot];
+    } else {
       double t[6];
       de_mul(m, ds, t);
 #pragma unroll
@@ -1236,8 +1474,22 @@ __global__ __launch_bounds__(kBlock) void k_gather(Mesh M, Particles P, Nodes Nd
     if constexpr ((WHAT & (SC_MV | SC_MOM)) != 0) {
       const double mN = m * N;
       if constexpr ((WHAT & SC_MV) != 0) am = am + mN;
+      double vel[D];
 #pragma unroll
-      for (int k = 0; k < D; ++k) amom[k] = amom[k] + mN * P.v[k * np + p];
+      for (int k = 0; k < D; ++k) vel[k] = P.v[k * np + p];
+      if (P.Cmat) {  // affine velocity updates
+        double Cp[D * D], xp[D], xn[D];
+#pragma unroll
+        for (int k = 0; k < D * D; ++k) Cp[k] = P.Cmat[k * np + p];
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+          xp[k] = P.x[k * np + p];
+          xn[k] = M.node_x[k * M.nnodes + n];
+        }
+        affine_velocity<D>(vel, Cp, xp, xn, vel);
+      }
+#pragma unroll
+      for (int k = 0; k < D; ++k) amom[k] = amom[k] + mN * vel[k];
     }
     if constexpr ((WHAT & SC_F) != 0) {
 #pragma unroll
@@ -1401,6 +1653,18 @@ void launch_locate_init(const Mesh& M, const Particles& P, Status* S, int* scan_
 }
 
 template <int D>
+void launch_mapping_matrix(const Mesh& M, const Particles& P, const Nodes& Nd,
+                           const StepArgs& A, int vu, const Status* S, hipStream_t st) {
+  MPMX_LAUNCH((k_mapping_matrix<D>), P.np, st, M, P, Nd, A, vu, S);
+}
+
+template <int D>
+void launch_affine_matrix(const Mesh& M, const Particles& P, int vu, const Status* S,
+                          hipStream_t st) {
+  MPMX_LAUNCH((k_affine_matrix<D>), P.np, st, M, P, vu, S);
+}
+
+template <int D>
 void launch_locate_step(const Mesh& M, const Particles& P, Status* S, int* scan_list,
                         hipStream_t st) {
   MPMX_LAUNCH((k_locate_step<D>), P.np, st, M, P, S, scan_list);
@@ -1427,6 +1691,16 @@ template <int D>
 void launch_p2g(int what, int chunk, const Mesh& M, const Particles& P, const Nodes& Nd,
                 const StepArgs& A, const Status* S, const int* skey, const int* perm,
                 hipStream_t st) {
+  if (P.Cmat) {  // affine velocity updates: 128-particle chunks
+    if (what == SC_MV)
+      MPMX_LAUNCH_CH(128, (k_p2g<D, SC_MV | SC_AFF, 128>), P.np, st, M, P, Nd, A, S, skey, perm);
+    else if (what == SC_MOM)
+      MPMX_LAUNCH_CH(128, (k_p2g<D, SC_MOM | SC_AFF, 128>), P.np, st, M, P, Nd, A, S, skey, perm);
+    else
+      MPMX_LAUNCH_CH(128, (k_p2g<D, SC_MV | SC_F | SC_AFF, 128>), P.np, st, M, P, Nd, A, S, skey,
+                     perm);
+    return;
+  }
   if (what == SC_MV) {
 #define L_(CH) MPMX_LAUNCH_CH(CH, (k_p2g<D, SC_MV, CH>), P.np, st, M, P, Nd, A, S, skey, perm)
     MPMX_CHUNK_DISPATCH(chunk, L_);
@@ -1576,6 +1850,10 @@ template void take_rows<int>(const int*, int*, const int*, int64_t, int64_t, int
                                       const int*, int, hipStream_t);                         \
   template void launch_locate_step<D>(const Mesh&, const Particles&, Status*, int*,          \
                                       hipStream_t);                                          \
+  template void launch_mapping_matrix<D>(const Mesh&, const Particles&, const Nodes&,        \
+                                         const StepArgs&, int, const Status*, hipStream_t);  \
+  template void launch_affine_matrix<D>(const Mesh&, const Particles&, int, const Status*,   \
+                                        hipStream_t);                                        \
   template void launch_p2g<D>(int, int, const Mesh&, const Particles&, const Nodes&,         \
                               const StepArgs&, const Status*, const int*, const int*,        \
                               hipStream_t);                                                  \

@@ -161,20 +161,19 @@ static std::vector<double> material_params(const Json& m, int dim) {
   const std::string type = m.at("type").str();
   const std::string sfx = std::to_string(dim) + "D";
   std::vector<double> p(MPMX_MATERIAL_PARAMS, 0.);
-  // objective stress rate (linear_elastic.tcc:8-17): "jaumann" for
-  // LinearElastic; "green_naghdi" is passed on (the library rejects it)
+  // objective stress rate (linear_elastic.tcc:8-17, inherited by MohrCoulomb
+  // through InfinitesimalElastoPlastic): "jaumann", "green_naghdi", else none
   std::string rate = "none";
   if (m.contains("stress_rate") && m.at("stress_rate").is_string()) rate = m.at("stress_rate").str();
+  const double rate_id = rate == "jaumann" ? 1. : (rate == "green_naghdi" ? 2. : 0.);
   if (type == "LinearElastic" + sfx) {
     p[0] = 0;
     p[1] = m.at("density").number();
     p[2] = m.at("youngs_modulus").number();
     p[3] = m.at("poisson_ratio").number();
-    p[4] = rate == "jaumann" ? 1. : (rate == "green_naghdi" ? 2. : 0.);
+    p[4] = rate_id;
     return p;
   }
-  if (rate == "jaumann" || rate == "green_naghdi")
-    throw std::runtime_error("objective stress rates are supported for LinearElastic only");
   if (type == "MohrCoulomb" + sfx) {
     p[0] = 1;
     p[1] = m.at("density").number();
@@ -193,6 +192,7 @@ static std::vector<double> material_params(const Json& m, int dim) {
     p[14] = m.get("packing_fraction_minimum", 0.);
     if (m.contains("packing_fraction")) p[15] = m.at("packing_fraction").number();
     else if (m.contains("porosity")) p[15] = 1. - m.at("porosity").number();
+    p[16] = rate_id;
     return p;
   }
   throw std::runtime_error("material type " + type + " is not supported in " + sfx);
@@ -288,9 +288,13 @@ void MPMExplicit<Tdim>::read_analysis() {
   if (a.contains("velocity_update")) {
     const Json& v = a.at("velocity_update");
     const std::string t = v.is_boolean() ? (v.boolean() ? "pic" : "flip") : v.str();
+    // VelocityUpdateType (src/mpm.cc:19-24)
     if (t == "pic") vu_ = MPMX_PIC;
     else if (t == "flip") vu_ = MPMX_FLIP;
-    else throw std::runtime_error("velocity update " + t + " is not supported (flip, pic)");
+    else if (t == "asflip") vu_ = MPMX_ASFLIP;
+    else if (t == "tpic") vu_ = MPMX_TPIC;
+    else if (t == "apic") vu_ = MPMX_APIC;
+    else throw std::runtime_error("velocity update " + t + " is not supported");
   }
   if (a.contains("velocity_update_settings") &&
       a.at("velocity_update_settings").contains("blending_ratio")) {
@@ -809,6 +813,17 @@ void MPMExplicit<Tdim>::resume_particles() {
                                  hint.data(), without_sv > 0 ? nullptr : svars.data()),
         "resumed particles");
   check(ctx_, mpmx_set_particle_ids(ctx_, n, id.data()), "resumed particle ids");
+  // mapping matrices of the affine updates where the table flags them
+  // (particle.tcc:111-125)
+  std::vector<double> B(Tdim * Tdim * n, 0.);
+  bool any_mm = false;
+  for (int64_t i = 0; i < n; ++i)
+    if (imm[i]) {
+      any_mm = true;
+      for (unsigned a = 0; a < Tdim; ++a)
+        for (unsigned b = 0; b < Tdim; ++b) B[(i * Tdim + a) * Tdim + b] = mm[9 * i + 3 * a + b];
+    }
+  if (any_mm) check(ctx_, mpmx_set_mapping_matrix(ctx_, n, B.data()), "resumed mapping matrices");
   check(ctx_, mpmx_set_step(ctx_, step_), "resume step");
   particle_loads();
 }
@@ -890,6 +905,21 @@ void MPMExplicit<Tdim>::write_outputs(int64_t step) {
   c.material_id = mid.data();
   c.nstate_vars = nsv.data();
   c.svars = svars.data();
+  // the affine updates' mapping matrices, flagged once they exist (after the
+  // first step; Particle::pod, particle.tcc:212-218)
+  std::vector<double> mm(9 * n, 0.), B(Tdim * Tdim * n);
+  std::vector<unsigned char> imm(n, 0);
+  const bool affine = vu_ == MPMX_APIC || vu_ == MPMX_ASFLIP || vu_ == MPMX_TPIC;
+  if (affine && step > 0) {
+    check(ctx_, mpmx_get_mapping_matrix(ctx_, B.data()), "mapping matrices");
+    for (int64_t i = 0; i < n; ++i) {
+      imm[i] = 1;
+      for (unsigned a = 0; a < Tdim; ++a)
+        for (unsigned b = 0; b < Tdim; ++b) mm[9 * i + 3 * a + b] = B[(i * Tdim + a) * Tdim + b];
+    }
+  }
+  c.initialise_mapping_matrix = imm.data();
+  c.mapping_matrix = mm.data();
   const std::string path = output_file("particles", ".h5", step);
   if (mpmx_pod_write(path.c_str(), &c) != 0)
     throw std::runtime_error(std::string("write ") + path + ": " + mpmx_pod_last_error());
@@ -904,7 +934,9 @@ void MPMExplicit<Tdim>::summary() const {
               "\"math_functions\": %zu, \"nodal_forces\": %zu, \"friction_constraints\": %zu, "
               "\"particle_velocity_constraints\": %zu, \"particles\": %lld, "
               "\"gravity\": [%.17g, %.17g, %.17g], \"ramping_time\": %.17g, \"materials\": [",
-              Tdim, scheme_.c_str(), vu_ == MPMX_PIC ? "pic" : "flip", dt_, (long long)nsteps_,
+              Tdim, scheme_.c_str(),
+              vu_ == MPMX_PIC ? "pic" : vu_ == MPMX_ASFLIP ? "asflip" : vu_ == MPMX_TPIC ? "tpic"
+              : vu_ == MPMX_APIC ? "apic" : "flip", dt_, (long long)nsteps_,
               (long long)output_steps_, damping_, (long long)(coords_.size() / Tdim),
               (long long)(cells_.size() / NN), node_sets_.size(), cell_sets_.size(),
               vnode_.size(), anode_.size(), fn_type_.size(), cfnode_.size(), fnode_.size(),

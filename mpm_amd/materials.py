@@ -5,22 +5,22 @@ The reference registers ``LinearElastic2D/3D`` and ``MohrCoulomb2D/3D`` in a
 reads their properties from the JSON ``materials`` array
 (include/materials/infinitesimal_strain/linear_elastic.tcc:3-49,
 mohr_coulomb.tcc:3-81). Here a material is a 16-double parameter vector that
-crosses the C-ABI unchanged:
+crosses the C-ABI unchanged (17 doubles):
 
     [0] type (0 LinearElastic, 1 MohrCoulomb)  [1] density  [2] youngs_modulus
     [3] poisson_ratio  [4] friction (deg)  [5] dilation (deg)  [6] cohesion
     [7] residual_friction (deg)  [8] residual_dilation (deg)
     [9] residual_cohesion  [10] peak_pdstrain  [11] residual_pdstrain
     [12] tension_cutoff  [13] softening (0/1)  [14] packing_fraction_minimum
-    [15] initial packing fraction (<= 0 means 1)
+    [15] initial packing fraction (<= 0 means 1)  [16] stress rate
 
-LinearElastic uses [4] for its objective stress rate
-(linear_elastic.tcc:8-17): 0 none, 1 Jaumann, 2 Green-Naghdi (rejected by
-the library).
+The objective stress rate (linear_elastic.tcc:8-17; MohrCoulomb inherits it,
+infinitesimal_elasto_plastic.tcc:1-25): 0 none, 1 Jaumann, 2 Green-Naghdi,
+in [4] for LinearElastic and in [16] for MohrCoulomb.
 """
 import numpy as np
 
-NPARAMS = 16
+NPARAMS = 17
 LINEAR_ELASTIC = 0
 MOHR_COULOMB = 1
 
@@ -50,7 +50,7 @@ def mohr_coulomb(youngs_modulus, poisson_ratio, density, friction, dilation,
                  residual_cohesion=None, peak_pdstrain=0.0,
                  residual_pdstrain=0.0, tension_cutoff=0.0, softening=False,
                  packing_fraction=None, porosity=None,
-                 packing_fraction_minimum=0.0):
+                 packing_fraction_minimum=0.0, stress_rate=None):
     """MohrCoulomb parameters; residual values default to the peak ones.
 
     ``packing_fraction`` / ``porosity`` set the initial packing fraction as in
@@ -76,6 +76,7 @@ def mohr_coulomb(youngs_modulus, poisson_ratio, density, friction, dilation,
         p[15] = packing_fraction
     elif porosity is not None:
         p[15] = 1.0 - porosity
+    p[16] = STRESS_RATES[stress_rate]
     return p
 
 
@@ -104,11 +105,11 @@ def from_json(props, dim):
     missing = [k for k in _REQUIRED[base] if k not in props]
     if missing:
         raise KeyError("Material parameter not set: %s" % ", ".join(missing))
-    if props.get("stress_rate", "none") not in ("none", None):
-        raise ValueError("only the default (no objective) stress rate is supported")
+    rate = props.get("stress_rate")
+    rate = rate if rate in STRESS_RATES else None  # other names: no rate (linear_elastic.tcc:17)
     if base == "LinearElastic":
         return linear_elastic(props["youngs_modulus"], props["poisson_ratio"],
-                              props["density"])
+                              props["density"], stress_rate=rate)
     return mohr_coulomb(
         props["youngs_modulus"], props["poisson_ratio"], props["density"],
         props["friction"], props["dilation"], props["cohesion"],
@@ -116,4 +117,4 @@ def from_json(props, dim):
         props["residual_cohesion"], props["peak_pdstrain"],
         props["residual_pdstrain"], props["tension_cutoff"],
         bool(props["softening"]), props.get("packing_fraction"),
-        props.get("porosity"), props.get("packing_fraction_minimum", 0.0))
+        props.get("porosity"), props.get("packing_fraction_minimum", 0.0), stress_rate=rate)

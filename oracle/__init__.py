@@ -23,7 +23,7 @@ _I64 = C.POINTER(C.c_int64)
 _I32 = C.POINTER(C.c_int)
 
 SCHEMES = {"usf": 0, "usl": 1, "musl": 2}
-VELOCITY_UPDATES = {"flip": 0, "pic": 1}
+VELOCITY_UPDATES = {"flip": 0, "pic": 1, "asflip": 2, "tpic": 3, "apic": 4}
 
 # Phase ids of orc_phase (each is one reference sweep)
 PH = dict(reset_nodes=0, shapefn=1, map_mass_momentum=2, compute_velocity=3,
@@ -69,7 +69,8 @@ def _lib(omp=False, libm="portable"):
                    "orc_set_nodal_forces", "orc_set_particle_tractions",
                    "orc_assign_materials", "orc_compute_volume_mass", "orc_set_isoparametric",
                    "orc_kat_transform_real_to_unit_cell", "orc_kat_point_in_cell",
-                   "orc_set_nodal_euler_angles"):
+                   "orc_set_nodal_euler_angles", "orc_get_mapping_matrix", "orc_kat_mc_rate",
+                   "orc_set_mapping_matrix"):
             getattr(lib, fn).restype = C.c_int
         lib.orc_kat_cell_volume.restype = C.c_double
         lib.orc_kat_mc_yield.restype = C.c_int
@@ -170,7 +171,7 @@ class Oracle:
                                           C.c_int64(self.ncells), _p(cells, _I64)))
 
     def set_materials(self, params):
-        params = _f64(params).reshape(-1, 16)
+        params = _f64(params).reshape(-1, 17)
         self._check(self.lib.orc_set_materials(self.ctx, params.shape[0], _p(params)))
 
     def _constraints(self, fn, nodes, dirs, vals, ints=False):
@@ -360,6 +361,16 @@ class Oracle:
         self._check(self.lib.orc_set_particle_field(self.ctx, C.c_int64(pid),
                                                     PARTICLE_FIELDS[field], _p(v)))
 
+    def mapping_matrix(self):
+        """The affine velocity updates' mapping matrices, (np, dim, dim)."""
+        B = np.zeros((self.nparticles, self.dim, self.dim))
+        self._check(self.lib.orc_get_mapping_matrix(self.ctx, _p(B)))
+        return B
+
+    def set_mapping_matrix(self, B):
+        B = _f64(B)
+        self._check(self.lib.orc_set_mapping_matrix(self.ctx, C.c_int64(B.shape[0]), _p(B)))
+
     def particle_scratch(self, pid):
         nn, d = self.nn, self.dim
         out = dict(N=np.zeros(nn), dndx=np.zeros((nn, d)), dstrain=np.zeros(6),
@@ -439,17 +450,42 @@ def stress_update(params, stress, dstrain, density=1000.0, statevars=None):
     return out, sv
 
 
-def le_rate_update(params, stress, dstrain, dF):
+def le_rate_update(params, stress, dstrain, dF, F=None):
     """One LinearElastic update with the material's objective stress rate
-    (params[4]: 0 none, 1 Jaumann) for a deformation gradient increment dF
-    (3x3)."""
+    (params[4]: 0 none, 1 Jaumann, 2 Green-Naghdi) for a deformation gradient
+    increment dF (3x3) at the step-start deformation gradient F (3x3,
+    default identity)."""
     lib = _lib()
     out = np.zeros(6)
+    Fp = None if F is None else _f64(np.asarray(F, dtype=np.float64).reshape(9))
     rc = lib.orc_kat_le_rate(_p(_f64(params)), _p(_f64(stress)), _p(_f64(dstrain)),
-                             _p(_f64(np.asarray(dF, dtype=np.float64).reshape(9))), _p(out))
+                             _p(_f64(np.asarray(dF, dtype=np.float64).reshape(9))), _p(Fp),
+                             _p(out))
     if rc != 0:
         raise ValueError("not a LinearElastic material")
     return out
+
+
+def mc_rate_update(params, stress, dstrain, dF, F=None, density=1000.0, statevars=None):
+    """One MohrCoulomb update with the material's stress rate (params[16]);
+    returns (stress, state variables)."""
+    lib = _lib()
+    out = np.zeros(6)
+    init = statevars is None
+    sv = np.zeros(9) if init else _f64(statevars).copy()
+    Fp = None if F is None else _f64(np.asarray(F, dtype=np.float64).reshape(9))
+    rc = lib.orc_kat_mc_rate(_p(_f64(params)), _p(_f64(stress)), _p(_f64(dstrain)),
+                             _p(_f64(np.asarray(dF, dtype=np.float64).reshape(9))), _p(Fp),
+                             C.c_double(density), _p(sv), 1 if init else 0, _p(out))
+    if rc != 0:
+        raise ValueError("not a MohrCoulomb material")
+    return out, sv
+
+
+def polar_rotation(F):
+    R = np.zeros(9)
+    _lib().orc_kat_polar_rotation(_p(_f64(np.asarray(F, dtype=np.float64).reshape(9))), _p(R))
+    return R.reshape(3, 3)
 
 
 def invariants(stress):

@@ -86,16 +86,18 @@ static const Index kNoCell = -1;
 // [7] residual_friction (deg) [8] residual_dilation (deg) [9] residual_cohesion
 // [10] peak_pdstrain [11] residual_pdstrain [12] tension_cutoff [13] softening
 // [14] packing_fraction_minimum [15] initial packing fraction (<=0 -> 1)
+// [16] MohrCoulomb stress_rate (0 none, 1 Jaumann, 2 Green-Naghdi); LinearElastic
+// has its stress_rate in [4]
 // ---------------------------------------------------------------------------
 enum { kLE = 0, kMC = 1 };
-static const int kMatParams = 16;
+static const int kMatParams = 17;
 // MC state variables, reference order (mohr_coulomb.tcc:116-118)
 enum { SV_YIELD = 0, SV_PHI, SV_PSI, SV_COH, SV_TCUT, SV_EPS, SV_RHO, SV_THETA, SV_PDS };
 static const int kNsv = 9;
 
 struct Material {
   int type = kLE;
-  int rate = 0;  // LinearElastic objective stress rate: 0 none, 1 Jaumann
+  int rate = 0;  // objective stress rate: 0 none, 1 Jaumann, 2 Green-Naghdi
   double density = 0, E = 0, nu = 0;
   // LinearElastic (linear_elastic.tcc:20-44)
   double bulk = 0;
@@ -145,6 +147,7 @@ static void material_init(Material& m, const double* p) {
     m.pd_peak = p[10];
     m.pd_res = p[11];
     m.tcut = p[12];
+    m.rate = (int)p[16];  // stress_rate, read by the LinearElastic base (linear_elastic.tcc:8-17)
     m.bulk = m.E / (3.0 * (1. - 2. * m.nu));
     m.shear = m.E / (2.0 * (1 + m.nu));
     // compute_elastic_tensor (mohr_coulomb.tcc:517-533)
@@ -232,18 +235,85 @@ static void rotate_sym(const double Q[3][3], const double X[3][3], double* y) {
 // W = skew(dF - I); the stress rotated by exp(W), the strain increment
 // (tensor shears) by exp(W/2), then the elastic increment of the rotated
 // strain.
-static void jaumann_stress(const double de[6][6], const double* stress, const double* dstrain,
-                           const double dF[3][3], double* out) {
-  const double a[3] = {0.5 * (0.5 * (dF[0][1] - dF[1][0])), 0.5 * (0.5 * (dF[0][2] - dF[2][0])),
-                       0.5 * (0.5 * (dF[1][2] - dF[2][1]))};
-  double L[3][3], LL[3][3];
-  skew_exp(a, L);
+// C = A B and C = A B^T of 3x3 matrices (sequential k)
+static void mat_mul(const double A[3][3], const double B[3][3], double C[3][3]) {
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j) {
-      double t = L[i][0] * L[0][j];
-      for (int k = 1; k < 3; ++k) t = t + L[i][k] * L[k][j];
-      LL[i][j] = t;
+      double t = A[i][0] * B[0][j];
+      for (int k = 1; k < 3; ++k) t = t + A[i][k] * B[k][j];
+      C[i][j] = t;
     }
+}
+static void mat_mul_t(const double A[3][3], const double B[3][3], double C[3][3]) {
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      double t = A[i][0] * B[j][0];
+      for (int k = 1; k < 3; ++k) t = t + A[i][k] * B[j][k];
+      C[i][j] = t;
+    }
+}
+
+static void inverse(int dim, const double J[3][3], double Ji[3][3]);
+
+// The rotation R of the polar decomposition F = R U (det F > 0), what
+// mpm::materials::compute_rotation_tensor (material_utility.tcc) obtains
+// from the eigen-decomposition of F^T F: here by Newton's iteration
+// X <- (X + X^-T) / 2 from X = F (Higham 1986), until an iteration changes no
+// entry by more than 1e-15 of the largest (at most 100 iterations).
+static void polar_rotation(const double F[3][3], double R[3][3]) {
+  double X[3][3];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) X[i][j] = F[i][j];
+  for (int it = 0; it < 100; ++it) {
+    double Xi[3][3], N[3][3];
+    inverse(3, X, Xi);
+    double diff = 0., big = 0.;
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        N[i][j] = 0.5 * (X[i][j] + Xi[j][i]);
+        const double d = std::fabs(N[i][j] - X[i][j]), a = std::fabs(N[i][j]);
+        if (d > diff) diff = d;
+        if (a > big) big = a;
+      }
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) X[i][j] = N[i][j];
+    if (diff <= 1.0E-15 * big) break;
+  }
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) R[i][j] = X[i][j];
+}
+
+// The two rotations of an objective stress rate: Lambda_delta (strain) and
+// Lambda_Delta (stress). Jaumann (linear_elastic.tcc:102-107): exp(W/2) and
+// its square, W the skew part of dF - I. Green-Naghdi (:134-150):
+// R_new R_half^T and R_new R_current^T, R the rotations of F, (F + F_new)/2
+// and F_new = dF F.
+static void rate_rotations(int rate, const double dF[3][3], const double F[3][3], double L[3][3],
+                           double LL[3][3]) {
+  if (rate == 1) {
+    const double a[3] = {0.5 * (0.5 * (dF[0][1] - dF[1][0])),
+                         0.5 * (0.5 * (dF[0][2] - dF[2][0])),
+                         0.5 * (0.5 * (dF[1][2] - dF[2][1]))};
+    skew_exp(a, L);
+    mat_mul(L, L, LL);
+    return;
+  }
+  double Fn[3][3], Fh[3][3], Rc[3][3], Rh[3][3], Rn[3][3];
+  mat_mul(dF, F, Fn);
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) Fh[i][j] = 0.5 * F[i][j] + 0.5 * Fn[i][j];
+  polar_rotation(F, Rc);
+  polar_rotation(Fh, Rh);
+  polar_rotation(Fn, Rn);
+  mat_mul_t(Rn, Rh, L);
+  mat_mul_t(Rn, Rc, LL);
+}
+
+// The trial stress of an objective rate: the stress rotated by Lambda_Delta,
+// the strain increment (tensor shears) by Lambda_delta, then the elastic
+// increment of the rotated strain (linear_elastic.tcc:109-124, 152-164).
+static void rate_stress(const double de[6][6], const double* stress, const double* dstrain,
+                        const double L[3][3], const double LL[3][3], double* out) {
   const double S[3][3] = {{stress[0], stress[3], stress[5]},
                           {stress[3], stress[1], stress[4]},
                           {stress[5], stress[4], stress[2]}};
@@ -256,6 +326,14 @@ static void jaumann_stress(const double de[6][6], const double* stress, const do
   for (int v = 3; v < 6; ++v) re[v] = 2.0 * re[v];
   de_mul(de, re, d);
   for (int v = 0; v < 6; ++v) out[v] = rs[v] + d[v];
+}
+
+// LinearElastic with the Jaumann rate (linear_elastic.tcc:96-124)
+static void jaumann_stress(const double de[6][6], const double* stress, const double* dstrain,
+                           const double dF[3][3], double* out) {
+  double L[3][3], LL[3][3];
+  rate_rotations(1, dF, dF, L, LL);
+  rate_stress(de, stress, dstrain, L, LL, out);
 }
 
 // x^T * D (row vector), same k-order.
@@ -507,8 +585,12 @@ static void mc_df_dp(const Material& m, int yield_type, const double* sv, const 
 }
 
 // mohr_coulomb.tcc:348-513. `density` is the particle's current mass density.
+// trial_in: the trial stress of an objective stress rate
+// (InfinitesimalElastoPlastic::compute_trial_stress,
+// infinitesimal_elasto_plastic.tcc:1-25); null: stress + D dstrain.
 static void mc_compute_stress(const Material& m, const double* stress, const double* dstrain,
-                              double density, double* sv, double* out) {
+                              double density, double* sv, double* out,
+                              const double* trial_in = nullptr) {
   const double current_packing_density = density;
   const double critical_density = m.min_packing * m.grain_density;
   const double pdstrain = sv[SV_PDS];
@@ -530,7 +612,9 @@ static void mc_compute_stress(const Material& m, const double* stress, const dou
   }
   sv[SV_YIELD] = 0;
   double trial[6];
-  {
+  if (trial_in) {
+    for (int i = 0; i < 6; ++i) trial[i] = trial_in[i];
+  } else {
     double ds[6];
     de_mul(m.de, dstrain, ds);
     for (int i = 0; i < 6; ++i) trial[i] = stress[i] + ds[i];
@@ -1077,6 +1161,7 @@ struct Particle {
   int mat = 0;
   double sv[kNsv] = {0};
   double size = 0;                  // pow(volume, 1/dim) at creation (particle.tcc:563-568)
+  double B[3][3] = {{0}};           // mapping_matrix_ of the affine velocity updates
   double traction[3] = {0, 0, 0};   // particle.tcc:962-979
   bool has_traction = false;
 };
@@ -1091,7 +1176,8 @@ struct TractionLoad {
 };
 
 enum { SCHEME_USF = 0, SCHEME_USL = 1, SCHEME_MUSL = 2 };
-enum { VU_FLIP = 0, VU_PIC = 1 };
+// mpm::VelocityUpdate (data_structures/data_types.h:50), same order
+enum { VU_FLIP = 0, VU_PIC = 1, VU_ASFLIP = 2, VU_TPIC = 3, VU_APIC = 4 };
 
 struct Solver {
   int dim = 2, nn = 4;
@@ -1482,13 +1568,94 @@ struct Solver {
   }
 
   // particle.tcc:630-658
+  // particle.tcc:630-658; APIC and ASFLIP map with the affine velocity
+  // v + B D^-1 (x_i - x_p), D = sum_i N_i (x_i - x_p)(x_i - x_p)^T
+  // (map_mass_momentum_to_nodes_affine, :661-696), TPIC with v + B (x_i - x_p)
+  // (map_mass_momentum_to_nodes_taylor, :699-724)
   void map_mass_momentum(const Particle& p) {
     const Cell& c = cells[p.cell];
+    const bool affine = vu == VU_APIC || vu == VU_ASFLIP;
+    double C[3][3] = {{0}};
+    if (affine) {
+      double D[3][3] = {{0}}, Di[3][3];
+      for (int i = 0; i < nn; ++i) {
+        const Node& n = nodes[c.nodes[i]];
+        double r[3];
+        for (int k = 0; k < dim; ++k) r[k] = n.x[k] - p.x[k];
+        for (int a = 0; a < dim; ++a)
+          for (int b = 0; b < dim; ++b) D[a][b] = D[a][b] + (p.N[i] * r[a]) * r[b];
+      }
+      inverse(dim, D, Di);
+      for (int a = 0; a < dim; ++a)
+        for (int b = 0; b < dim; ++b) {
+          double t = p.B[a][0] * Di[0][b];
+          for (int k = 1; k < dim; ++k) t = t + p.B[a][k] * Di[k][b];
+          C[a][b] = t;
+        }
+    } else if (vu == VU_TPIC) {
+      for (int a = 0; a < dim; ++a)
+        for (int b = 0; b < dim; ++b) C[a][b] = p.B[a][b];
+    }
     for (int i = 0; i < nn; ++i) {
       Node& n = nodes[c.nodes[i]];
       add(n.mass, p.mass * p.N[i]);
-      for (int k = 0; k < dim; ++k) add(n.momentum[k], p.mass * p.N[i] * p.v[k]);
+      if (!affine && vu != VU_TPIC) {
+        for (int k = 0; k < dim; ++k) add(n.momentum[k], p.mass * p.N[i] * p.v[k]);
+        continue;
+      }
+      double r[3], mv[3];
+      for (int k = 0; k < dim; ++k) r[k] = n.x[k] - p.x[k];
+      for (int a = 0; a < dim; ++a) {
+        double t = C[a][0] * r[0];
+        for (int k = 1; k < dim; ++k) t = t + C[a][k] * r[k];
+        mv[a] = p.v[a] + t;
+      }
+      for (int k = 0; k < dim; ++k) add(n.momentum[k], p.mass * p.N[i] * mv[k]);
     }
+  }
+
+  // The mapping matrices of the affine updates, |B_ij| < 1e-15 -> 0:
+  // compute_affine_mapping_matrix (particle.tcc:1687-1708),
+  // B = sum_i N_i v_i (x_i - x_p)^T, and compute_velocity_gradient
+  // (:1662-1684), B = sum_i v_i dN_i/dx^T.
+  void mapping_matrix(Particle& p, bool gradient) {
+    const Cell& c = cells[p.cell];
+    double B[3][3] = {{0}};
+    for (int i = 0; i < nn; ++i) {
+      const Node& n = nodes[c.nodes[i]];
+      for (int a = 0; a < dim; ++a)
+        for (int b = 0; b < dim; ++b) {
+          const double t = gradient ? n.velocity[a] * p.dndx[i][b]
+                                    : (p.N[i] * n.velocity[a]) * (n.x[b] - p.x[b]);
+          B[a][b] = B[a][b] + t;
+        }
+    }
+    for (int a = 0; a < dim; ++a)
+      for (int b = 0; b < dim; ++b) p.B[a][b] = std::fabs(B[a][b]) < 1.E-15 ? 0. : B[a][b];
+  }
+
+  // compute_asflip_beta (particle.tcc:1711-1730): 0 next to a node with a
+  // velocity constraint or where the deformation gradient increment of the
+  // current nodal velocities compresses (det < 1), else 1
+  double asflip_beta(const Particle& p, double dt) {
+    const Cell& c = cells[p.cell];
+    for (int i = 0; i < nn; ++i)
+      if (!nodes[c.nodes[i]].vel_c.empty()) return 0.0;
+    double R[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    for (int i = 0; i < nn; ++i) {
+      const Node& n = nodes[c.nodes[i]];
+      for (int a = 0; a < dim; ++a)
+        for (int b = 0; b < dim; ++b) R[a][b] = R[a][b] + (n.velocity[a] * p.dndx[i][b]) * dt;
+    }
+    for (int a = 0; a < dim; ++a)
+      for (int b = 0; b < dim; ++b) {
+        if (a != b && std::fabs(R[a][b]) < 1.E-15) R[a][b] = 0.;
+        if (a == b && std::fabs(R[a][b] - 1.) < 1.E-15) R[a][b] = 1.;
+      }
+    const double det = R[0][0] * (R[1][1] * R[2][2] - R[2][1] * R[1][2]) -
+                       R[1][0] * (R[0][1] * R[2][2] - R[2][1] * R[0][2]) +
+                       R[2][0] * (R[0][1] * R[1][2] - R[1][1] * R[0][2]);
+    return det < 1.0 ? 0.0 : 1.0;
   }
 
   // y = A x for a dim x dim matrix (sequential over k)
@@ -1602,14 +1769,21 @@ struct Solver {
   void compute_stress(Particle& p) {
     const Material& m = materials[p.mat];
     double out[6];
-    if (m.type == kLE && m.rate == 1) {
-      jaumann_stress(m.de, p.stress, p.dstrain, p.dF, out);
+    double rtrial[6];
+    if (m.rate != 0) {  // objective stress rate (F is the step-start one)
+      double L[3][3], LL[3][3];
+      rate_rotations(m.rate, p.dF, p.F, L, LL);
+      rate_stress(m.de, p.stress, p.dstrain, L, LL, rtrial);
+    }
+    if (m.type == kLE && m.rate != 0) {
+      for (int k = 0; k < 6; ++k) out[k] = rtrial[k];
     } else if (m.type == kLE) {
       double ds[6];
       de_mul(m.de, p.dstrain, ds);
       for (int k = 0; k < 6; ++k) out[k] = p.stress[k] + ds[k];
     } else {
-      mc_compute_stress(m, p.stress, p.dstrain, p.density, p.sv, out);
+      mc_compute_stress(m, p.stress, p.dstrain, p.density, p.sv, out,
+                        m.rate != 0 ? rtrial : nullptr);
     }
     std::memcpy(p.stress, out, sizeof(out));
   }
@@ -1858,9 +2032,13 @@ struct Solver {
     return true;
   }
 
-  // particle.tcc:994-1047 (FLIP), :1050-1070 (PIC)
+  // particle.tcc:994-1047 (FLIP), :1050-1070 (PIC), :1073-1112 (ASFLIP),
+  // :1115-1126 (APIC: mapping matrix, then PIC), :1129-1140 (TPIC: velocity
+  // gradient, then PIC)
   void compute_updated_position(Particle& p, double dt) {
     const Cell& c = cells[p.cell];
+    if (vu == VU_APIC || vu == VU_ASFLIP) mapping_matrix(p, false);
+    if (vu == VU_TPIC) mapping_matrix(p, true);
     double nv[3] = {0, 0, 0}, na[3] = {0, 0, 0};
     for (int i = 0; i < nn; ++i) {
       const Node& n = nodes[c.nodes[i]];
@@ -1868,6 +2046,17 @@ struct Solver {
         nv[k] += p.N[i] * n.velocity[k];
         na[k] += p.N[i] * n.acceleration[k];
       }
+    }
+    if (vu == VU_ASFLIP) {
+      const double beta = asflip_beta(p, dt);
+      double flip[3], sep[3];
+      for (int k = 0; k < dim; ++k) flip[k] = p.v[k] + na[k] * dt;
+      for (int k = 0; k < dim; ++k) p.v[k] = blending * flip[k] + (1.0 - blending) * nv[k];
+      for (int k = 0; k < dim; ++k)
+        sep[k] = beta * blending * flip[k] + (1.0 - beta * blending) * nv[k];
+      for (int k = 0; k < dim; ++k) p.x[k] += sep[k] * dt;
+      for (int k = 0; k < dim; ++k) p.u[k] += sep[k] * dt;
+      return;
     }
     if (vu == VU_FLIP) {
       for (int k = 0; k < dim; ++k) p.v[k] += na[k] * dt;
@@ -2471,6 +2660,30 @@ int orc_set_particle_field(orc_ctx* c, Index pid, int field, const double* val) 
 }
 
 // Per-particle scratch fields: N (nn), dndx (nn*d), dstrain (6), dF (9), dvol.
+// The particles' mapping matrices of the affine velocity updates
+// (mapping_matrix_), [np][dim][dim] row-major, in particle order.
+int orc_get_mapping_matrix(orc_ctx* c, double* B) {
+  ORC_TRY({
+    const int d = c->s.dim;
+    Index q = 0;
+    for (const auto& p : c->s.particles) {
+      for (int a = 0; a < d; ++a)
+        for (int b = 0; b < d; ++b) B[(q * d + a) * d + b] = p.B[a][b];
+      ++q;
+    }
+  })
+}
+
+int orc_set_mapping_matrix(orc_ctx* c, Index n, const double* B) {
+  ORC_TRY({
+    const int d = c->s.dim;
+    if (n != (Index)c->s.particles.size()) throw std::runtime_error("particle count differs");
+    for (Index q = 0; q < n; ++q)
+      for (int a = 0; a < d; ++a)
+        for (int b = 0; b < d; ++b) c->s.particles[q].B[a][b] = B[(q * d + a) * d + b];
+  })
+}
+
 int orc_get_particle_scratch(orc_ctx* c, Index pid, double* N, double* dndx, double* dstrain,
                              double* dF, double* dvol) {
   ORC_TRY({
@@ -2603,15 +2816,23 @@ int orc_kat_stress(const double* params, const double* stress, const double* dst
 
 // One LinearElastic update with its objective stress rate, given the
 // deformation gradient increment dF (3x3 row-major).
+// One LinearElastic update with the material's stress rate for the
+// increment dF (3x3) at the step-start deformation gradient F (3x3, null:
+// identity; the Green-Naghdi rate needs it).
 int orc_kat_le_rate(const double* params, const double* stress, const double* dstrain,
-                    const double* dF, double* out) {
+                    const double* dF, const double* Fin, double* out) {
   orc::Material m;
   orc::material_init(m, params);
   if (m.type != orc::kLE) return -1;
-  double F[3][3];
-  for (int i = 0; i < 9; ++i) F[i / 3][i % 3] = dF[i];
-  if (m.rate == 1) {
-    orc::jaumann_stress(m.de, stress, dstrain, F, out);
+  double G[3][3], F[3][3];
+  for (int i = 0; i < 9; ++i) {
+    G[i / 3][i % 3] = dF[i];
+    F[i / 3][i % 3] = Fin ? Fin[i] : (i % 4 == 0 ? 1. : 0.);
+  }
+  if (m.rate != 0) {
+    double L[3][3], LL[3][3];
+    orc::rate_rotations(m.rate, G, F, L, LL);
+    orc::rate_stress(m.de, stress, dstrain, L, LL, out);
   } else {
     double ds[6];
     orc::de_mul(m.de, dstrain, ds);
@@ -2739,6 +2960,41 @@ void orc_kat_node_constraints(int dim, const double* angles, double* vel, double
     vel[k] = n.velocity[k];
     acc[k] = n.acceleration[k];
   }
+}
+
+// The rotation of the polar decomposition of F (3x3 row-major).
+void orc_kat_polar_rotation(const double* F, double* R) {
+  double A[3][3], B[3][3];
+  for (int i = 0; i < 9; ++i) A[i / 3][i % 3] = F[i];
+  orc::polar_rotation(A, B);
+  for (int i = 0; i < 9; ++i) R[i] = B[i / 3][i % 3];
+}
+
+// One MohrCoulomb update with the material's stress rate (increment dF at the
+// step-start F): new stress and state variables (sv in/out; init: start from
+// the material's initial state).
+int orc_kat_mc_rate(const double* params, const double* stress, const double* dstrain,
+                    const double* dF, const double* Fin, double density, double* sv, int init,
+                    double* out) {
+  orc::Material m;
+  orc::material_init(m, params);
+  if (m.type == orc::kLE) return -1;
+  if (init) orc::mc_initialise_state(m, sv);
+  double G[3][3], F[3][3];
+  for (int i = 0; i < 9; ++i) {
+    G[i / 3][i % 3] = dF[i];
+    F[i / 3][i % 3] = Fin ? Fin[i] : (i % 4 == 0 ? 1. : 0.);
+  }
+  double trial[6];
+  const double* tp = nullptr;
+  if (m.rate != 0) {
+    double L[3][3], LL[3][3];
+    orc::rate_rotations(m.rate, G, F, L, LL);
+    orc::rate_stress(m.de, stress, dstrain, L, LL, trial);
+    tp = trial;
+  }
+  orc::mc_compute_stress(m, stress, dstrain, density, sv, out, tp);
+  return 0;
 }
 
 // The rotation matrix of Euler angles (row-major dim x dim).

@@ -14,7 +14,7 @@ DRIVER = os.path.join(ROOT, "mpm_amd", "mpmx_run")
 def write_deck(w, d, nsteps, output_steps, constraints_as="sets", uuid="deck",
                material_json=None, stresses=None, particle_constraints=None,
                acceleration_constraints=None, math_functions=None, nodal_forces=None,
-               tractions=None):
+               tractions=None, velocity_update=False):
     """w: a Workload; d: directory. Velocity constraints go to node sets
     (``constraints_as="sets"``) or to a constraints file (``"file"``).
     particle_constraints: [(particle ids or -1, dir, velocity)], written as
@@ -136,7 +136,7 @@ def write_deck(w, d, nsteps, output_steps, constraints_as="sets", uuid="deck",
         "materials": [material_json],
         "external_loading_conditions": {"gravity": [float(g) for g in w.gravity]},
         "analysis": {"type": "MPMExplicit%dD" % dim, "mpm_scheme": w.scheme,
-                     "velocity_update": False, "locate_particles": True, "dt": w.dt,
+                     "velocity_update": velocity_update, "locate_particles": True, "dt": w.dt,
                      "nsteps": nsteps, "uuid": uuid,
                      "damping": {"type": "Cundall", "damping_factor": w.damping},
                      "resume": {"resume": False}},
@@ -181,7 +181,9 @@ def read_output(path, dim):
             "stress": c["stress"], "strain": c["strain"], "F": c["defgrad"],
             "volume": c["volume"], "mass": c["mass"], "statevars": c["svars"][:, :9],
             "pdstrain": c["svars"][:, 8], "material_id": c["material_id"],
-            "nstate_vars": c["nstate_vars"], "nsize": c["nsize"][:, 0]}
+            "nstate_vars": c["nstate_vars"], "nsize": c["nsize"][:, 0],
+            "mapping_matrix": c["mapping_matrix"].reshape(-1, 3, 3)[:, :dim, :dim],
+            "initialise_mapping_matrix": c["initialise_mapping_matrix"]}
 
 
 def write_fixture_deck(d, dim, scheme="usf", nsteps=10, output_steps=5, resume=None,

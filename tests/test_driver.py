@@ -95,17 +95,20 @@ def test_deck_errors_are_reported(tmp_path):
 
 
 def test_dry_run_stress_rate(tmp_path):
-    """LinearElastic reads "stress_rate" (linear_elastic.tcc:8-17); a
-    Jaumann MohrCoulomb is reported as unsupported."""
+    """"stress_rate" (linear_elastic.tcc:8-17, inherited by MohrCoulomb):
+    "jaumann" and "green_naghdi" map to 1 and 2, in parameter [4] for
+    LinearElastic and [16] for MohrCoulomb; any other name is no rate."""
+    import json
     w = workloads.column_2d(nx=2, ny=2)
-    write_deck(w, str(tmp_path), nsteps=1, output_steps=1,
-               material_json=dict(LE, stress_rate="jaumann"))
-    r = run_driver(str(tmp_path), "--dry-run", check=False)
-    assert r.returncode == 0, r.stderr
     mc = dict(LE, type="MohrCoulomb2D", friction=30.0, dilation=0.0, cohesion=1000.0,
               residual_friction=30.0, residual_dilation=0.0, residual_cohesion=1000.0,
               peak_pdstrain=0.0, residual_pdstrain=0.0, tension_cutoff=0.0,
-              softening=False, stress_rate="jaumann")
-    write_deck(w, str(tmp_path), nsteps=1, output_steps=1, material_json=mc)
-    r = run_driver(str(tmp_path), "--dry-run", check=False)
-    assert r.returncode == 1 and "LinearElastic only" in r.stderr
+              softening=False)
+    for m, idx in ((LE, 4), (mc, 16)):
+        for rate, want in (("jaumann", 1.0), ("green_naghdi", 2.0), ("other", 0.0)):
+            write_deck(w, str(tmp_path), nsteps=1, output_steps=1,
+                       material_json=dict(m, stress_rate=rate))
+            r = run_driver(str(tmp_path), "--dry-run", check=False)
+            assert r.returncode == 0, r.stderr
+            params = json.loads(r.stdout.strip().splitlines()[-1])["materials"][0]
+            assert len(params) == 17 and params[idx] == want, (m["type"], rate, params)

@@ -234,6 +234,39 @@ def test_driver_reference_fixture_deck_matches_oracle(tmp_path, dim, scheme):
     _compare(out, p, dim)
 
 
+def test_driver_apic_deck_matches_oracle_and_resumes(tmp_path):
+    """"velocity_update": "apic" (VelocityUpdateType, src/mpm.cc:19-24): the
+    driver's tables match the oracle's APIC run bit for bit, mapping matrices
+    included (written from the first step on, flagged as the reference's
+    Particle::pod does); resuming from the table of step 10 carries the
+    mapping matrices and continues bit for bit."""
+    import json
+    w = workloads.column_2d(nx=6, ny=10)
+    w.scheme = "musl"
+    d = str(tmp_path)
+    write_deck(w, d, nsteps=30, output_steps=10, material_json=LE, velocity_update="apic")
+    run_driver(d, "--ordered")
+    o = w.setup(oracle.Oracle(2, "musl", velocity_update="apic"))
+    o.step(30, w.dt)
+    out = read_output(output_path(d, 30, 30), 2)
+    _compare(out, o.particles(), 2)
+    assert np.all(out["initialise_mapping_matrix"] == 1)
+    assert np.array_equal(out["mapping_matrix"], o.mapping_matrix())
+    assert np.all(read_output(output_path(d, 0, 30), 2)["initialise_mapping_matrix"] == 0)
+    first = read_output(output_path(d, 30, 30), 2)
+    with open(os.path.join(d, "mpm.json")) as f:
+        deck = json.load(f)
+    deck["analysis"]["resume"] = {"resume": True, "uuid": "deck", "step": 10}
+    with open(os.path.join(d, "mpm.json"), "w") as f:
+        json.dump(deck, f)
+    os.remove(output_path(d, 20, 30))
+    os.remove(output_path(d, 30, 30))
+    run_driver(d, "--ordered")
+    again = read_output(output_path(d, 30, 30), 2)
+    for k in first:
+        assert np.array_equal(first[k], again[k]), k
+
+
 def test_driver_resume_continues_bitwise(tmp_path):
     """A Mohr-Coulomb collapse run for 30 steps with tables every 10, then
     resumed from the table of step 10 ("resume": {"resume": true, "uuid",

@@ -243,12 +243,67 @@ def test_jaumann_stress_rate(dim, ordered, scheme):
 
 
 def test_stress_rate_validation():
-    from mpm_amd.capi import MpmxError, E_UNSUPPORTED, Solver
+    from mpm_amd.capi import MpmxError, E_ARG, Solver
     s = Solver(2, "usf")
+    bad = mat.linear_elastic(1e6, 0.3, 1800.0)
+    bad[4] = 3
     with pytest.raises(MpmxError) as e:
-        s.set_materials(mat.linear_elastic(1e6, 0.3, 1800.0, stress_rate="green_naghdi")[None, :])
-    assert e.value.code == E_UNSUPPORTED
+        s.set_materials(bad[None, :])
+    assert e.value.code == E_ARG
     s.close()
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("ordered", [True, False])
+@pytest.mark.parametrize("scheme", ["usf", "usl", "musl"])
+def test_green_naghdi_stress_rate(dim, ordered, scheme):
+    """LinearElastic with the Green-Naghdi rate (linear_elastic.tcc:129-164):
+    a spinning block, every scheme, against the oracle."""
+    if dim == 2:
+        w = workloads.column_2d(nx=6, ny=10)
+    else:
+        w = workloads.collapse_3d(nx=8, ny=4, nz=4, block=(4, 4, 4))
+    w.velocity_seed, w.velocity_amplitude = 7, 0.05
+    w.materials = mat.linear_elastic(2e5, 0.3, 1800.0, stress_rate="green_naghdi")[None, :]
+    pg, po = _run(w, 40, ordered=ordered, scheme=scheme)
+    _check(pg, po, exact=ordered)
+    w.materials = mat.linear_elastic(2e5, 0.3, 1800.0, stress_rate="jaumann")[None, :]
+    pj, _ = _run(w, 40, ordered=True, scheme=scheme)
+    assert not np.array_equal(pj["stress"], po["stress"])
+
+
+@pytest.mark.parametrize("rate", ["jaumann", "green_naghdi"])
+@pytest.mark.parametrize("scheme", ["usf", "usl", "musl"])
+def test_mohr_coulomb_stress_rates(rate, scheme):
+    """MohrCoulomb with an objective stress rate (compute_trial_stress,
+    infinitesimal_elasto_plastic.tcc:1-25): the rotated trial stress feeds the
+    return mapping, also when it is deferred; ordered mode bitwise against the
+    oracle on the collapse deck, with a rate-free MohrCoulomb and a
+    LinearElastic material beside it."""
+    w = workloads.collapse_3d(nx=8, ny=4, nz=4, block=(4, 4, 4))
+    w.velocity_seed, w.velocity_amplitude = 7, 0.05
+    mc = mat.mohr_coulomb(5e6, 0.3, 1800.0, friction=30.0, dilation=0.0, cohesion=0.0,
+                          stress_rate=rate)
+    w.materials = np.stack([mc, mat.mohr_coulomb(5e6, 0.3, 1800.0, friction=30.0, dilation=0.0,
+                                                 cohesion=0.0),
+                            mat.linear_elastic(5e6, 0.3, 1800.0, stress_rate=rate)])
+    n = w.nparticles
+    ids = np.arange(n)
+
+    def assign(o):
+        o.assign_materials(ids[ids % 3 == 1], 1)
+        o.assign_materials(ids[ids % 3 == 2], 2)
+        return o
+
+    g = assign(w.setup(_solver(w, True, scheme=scheme)))
+    o = assign(w.setup(_oracle(w, scheme=scheme)))
+    for _ in range(4):
+        g.step(10, w.dt)
+        o.step(10, w.dt)
+    pg, po = g.particles(), o.particles()
+    _check(pg, po, exact=True)
+    y = po["statevars"][ids % 3 == 0, 0]
+    assert np.sum((y == 1) | (y == 2)) > 0, "no rate particle yields"
 
 
 def test_gravity_ramp():
@@ -407,3 +462,35 @@ def test_general_boundary_constraints(dim, ordered):
     c = w.setup(_solver(w, True, scheme="musl"))
     c.step(done, w.dt)
     assert not np.array_equal(c.particles()["v"], po["v"])
+
+
+@pytest.mark.parametrize("vu", ["apic", "asflip", "tpic"])
+@pytest.mark.parametrize("scheme", ["usf", "usl", "musl"])
+@pytest.mark.parametrize("dim", [2, 3])
+def test_affine_velocity_updates(vu, scheme, dim):
+    """APIC, ASFLIP (blend 0.9) and TPIC (particle.tcc:661-724, 1073-1140):
+    ordered mode bit for bit against the oracle in every particle field and
+    the mapping matrices, under every scheme; fast mode to rounding."""
+    if dim == 2:
+        w = workloads.column_2d(nx=6, ny=10)
+    else:  # elastic, so that fast mode's rounding is not amplified by yielding
+        w = workloads.collapse_3d(nx=8, ny=4, nz=4, block=(4, 4, 4))
+        w.materials = mat.linear_elastic(5e6, 0.3, 1800.0)[None, :]
+    w.velocity_seed, w.velocity_amplitude = 3, 0.05
+    blend = 0.9 if vu == "asflip" else 1.0
+    kw = dict(scheme=scheme, vu=vu, blend=blend)
+    g = w.setup(_solver(w, True, **kw))
+    f = w.setup(_solver(w, False, **kw))
+    o = w.setup(_oracle(w, **kw))
+    for _ in range(3):
+        g.step(10, w.dt)
+        f.step(10, w.dt)
+        o.step(10, w.dt)
+    po = o.particles()
+    _check(g.particles(), po, exact=True)
+    Bo = o.mapping_matrix()
+    assert np.array_equal(g.mapping_matrix(), Bo)
+    assert np.max(np.abs(Bo)) > 0.0
+    _check(f.particles(), po, exact=False, tol=1e-9)
+    scale = float(np.max(np.abs(Bo)))
+    assert float(np.max(np.abs(f.mapping_matrix() - Bo))) <= 1e-9 * scale

@@ -595,3 +595,98 @@ def test_rotation_matrix_is_orthonormal_and_reduces_to_cartesian():
             a = oracle.node_constraints(dim, vel, acc, angles=np.zeros(dim), **kw)
             b = oracle.node_constraints(dim, vel, acc, **kw)
             assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+# ------------------------------ Green-Naghdi ---------------------------------
+def _simple_shear_gn(p, steps, shear=10.0):
+    """The reference's Green-Naghdi simple-shear test: the deformation gradient
+    is updated by F_inc after each step (particle.update_deformation_gradient)."""
+    inc = shear / steps
+    ds = np.zeros(6)
+    ds[3] = inc
+    dF = np.eye(3)
+    dF[0, 1] = inc
+    F = np.eye(3)
+    s, out = np.zeros(6), []
+    for _ in range(steps):
+        s = oracle.le_rate_update(p, s, ds, dF, F)
+        out.append(s.copy())
+        F = dF @ F
+    return np.array(out)
+
+
+def test_linear_elastic_green_naghdi_simple_shear(kats):
+    c = kats["linear_elastic_green_naghdi"]
+    m = c["material"]
+    p = materials.linear_elastic(m["youngs_modulus"], m["poisson_ratio"], m["density"],
+                                 stress_rate="green_naghdi")
+    for case in c["cases"]:
+        s = _simple_shear_gn(p, case["steps"], c["shear_displacement"])
+        got = s[::case["every"], 3] / c["G"]
+        assert_kat(got, case["normalized_sigma_xy"], c["tol"])
+
+
+def test_polar_rotation():
+    """F = R U: R orthonormal with det 1, R^T F symmetric positive definite; a
+    rotation times a known stretch gives back the rotation."""
+    rng = np.random.default_rng(11)
+    for _ in range(50):
+        A = rng.normal(size=(3, 3))
+        U = A @ A.T + 0.2 * np.eye(3)          # symmetric positive definite
+        Q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+        if np.linalg.det(Q) < 0:
+            Q[:, 0] = -Q[:, 0]
+        R = oracle.polar_rotation(Q @ U)
+        assert np.allclose(R, Q, atol=1e-12)
+        assert np.allclose(R @ R.T, np.eye(3), atol=1e-14)
+        S = R.T @ (Q @ U)
+        assert np.allclose(S, S.T, atol=1e-12) and np.all(np.linalg.eigvalsh(S) > 0)
+    assert np.array_equal(oracle.polar_rotation(np.eye(3)), np.eye(3))
+
+
+def test_green_naghdi_rigid_rotation():
+    """A rigid rotation about z by angle a per step with no strain: the
+    Green-Naghdi update rotates the stress with the body (sigma' = Q sigma
+    Q^T) and adds nothing."""
+    p = materials.linear_elastic(1e6, 0.3, 1000.0, stress_rate="green_naghdi")
+    a = 0.1
+    Q = np.array([[np.cos(a), -np.sin(a), 0], [np.sin(a), np.cos(a), 0], [0, 0, 1.]])
+    s0 = np.array([3.0, -1.0, 0.5, 0.7, 0.0, 0.0])
+    S0 = np.array([[3.0, 0.7, 0], [0.7, -1.0, 0], [0, 0, 0.5]])
+    F = np.eye(3)
+    s = s0
+    for k in range(1, 4):
+        s = oracle.le_rate_update(p, s, np.zeros(6), Q, F)
+        F = Q @ F
+        Qk = np.linalg.matrix_power(Q, k)
+        want = Qk @ S0 @ Qk.T
+        got = np.array([[s[0], s[3], s[5]], [s[3], s[1], s[4]], [s[5], s[4], s[2]]])
+        assert np.allclose(got, want, atol=1e-12)
+
+
+@pytest.mark.parametrize("rate", ["jaumann", "green_naghdi"])
+def test_mohr_coulomb_rate_elastic_branch_equals_linear_elastic(rate):
+    """Below the yield surface MohrCoulomb's rotated trial stress is its update
+    (compute_trial_stress, infinitesimal_elasto_plastic.tcc:1-25), the same
+    as LinearElastic's with the same rate; above it the rotated trial is what
+    the return mapping corrects, which differs from the unrotated one."""
+    rng = np.random.default_rng(12)
+    le = materials.linear_elastic(1e7, 0.3, 1800.0, stress_rate=rate)
+    mc = materials.mohr_coulomb(1e7, 0.3, 1800.0, friction=35.0, dilation=5.0,
+                                cohesion=5e4, stress_rate=rate)
+    mc0 = materials.mohr_coulomb(1e7, 0.3, 1800.0, friction=35.0, dilation=5.0,
+                                 cohesion=5e4)
+    s = np.array([-2e4, -3e4, -2.5e4, 3e3, -1e3, 2e3])
+    for _ in range(20):
+        dF = np.eye(3) + rng.normal(scale=2e-3, size=(3, 3))
+        F = np.eye(3) + rng.normal(scale=5e-2, size=(3, 3))
+        ds = rng.normal(scale=1e-5, size=6)
+        a, sv = oracle.mc_rate_update(mc, s, ds, dF, F)
+        assert sv[0] == 0  # elastic
+        assert np.array_equal(a, oracle.le_rate_update(le, s, ds, dF, F))
+    ds = np.array([0, 0, 0, 2e-2, 0, 0])
+    dF = np.eye(3)
+    dF[0, 1] = 2e-2
+    a, sv = oracle.mc_rate_update(mc, s, ds, dF)
+    b, sv0 = oracle.mc_rate_update(mc0, s, ds, dF)
+    assert sv[0] in (1, 2) and sv0[0] in (1, 2) and not np.allclose(a, b, rtol=1e-9)
