@@ -122,6 +122,9 @@ struct mpmx_ctx {
 
   // materials
   std::vector<Material> mats;
+  // per material: some particle was given strength state variables other
+  // than the material's initial ones (Material::uniform then stays off)
+  std::vector<char> sv_custom;
   DBuf<Material> d_mats;
   bool any_mc = false;
   bool any_rate = false;     // a material with an objective stress rate
@@ -863,6 +866,21 @@ void locate_new(mpmx_ctx* c, int64_t off, int64_t n, const std::vector<int>& hin
                    "Particle outside the mesh domain: " + std::to_string(c->h_status->bad_particle));
 }
 
+// Material::uniform from the materials and the strength state variables the
+// particles were given; uploads the table when it changed (or `force`).
+void refresh_uniform(mpmx_ctx* c, bool force = false) {
+  bool changed = force;
+  for (size_t i = 0; i < c->mats.size(); ++i) {
+    Material& m = c->mats[i];
+    const int u = (m.type == MC && !m.softening && !c->sv_custom[i]) ? 1 : 0;
+    if (u != m.uniform) changed = true;
+    m.uniform = u;
+  }
+  if (!changed) return;
+  c->d_mats.upload(c->mats.data(), c->mats.size(), c->stream);
+  HIPCHECK(hipStreamSynchronize(c->stream));
+}
+
 // Append n particles (rows given on the host); returns the offset.
 int64_t append_particles(mpmx_ctx* c, int64_t n, const double* x, const double* u, const double* v,
                          const double* stress, const double* strain, const double* F,
@@ -919,8 +937,12 @@ int64_t append_particles(mpmx_ctx* c, int64_t n, const double* x, const double* 
       const Material& m = c->mats[mat[i]];
       for (int k = 0; k < NSV; ++k)
         s[i * NSV + k] = (sv && m.type == MC) ? sv[i * NSV + k] : m.sv0[k];
+      if (m.type == MC)
+        for (const int k : {SV_PHI, SV_PSI, SV_COH, SV_TCUT})
+          if (s[i * NSV + k] != m.sv0[k]) c->sv_custom[mat[i]] = 1;
     }
     put_soa(c->d_sv, np, off, n, NSV, s.data(), st);
+    refresh_uniform(c);
   }
   alloc_step_buffers(c);
   c->invalidate_graph();
@@ -1798,6 +1820,7 @@ int mpmx_set_materials(mpmx_ctx* c, int n, const double* params) {
   if (n <= 0 || !params) throw ApiError(MPMX_E_ARG, "no materials");
   if (c->np > 0) throw ApiError(MPMX_E_STATE, "materials must be set before particles");
   c->mats.clear();
+  c->sv_custom.assign(n, 0);
   c->any_mc = false;
   c->any_rate = c->any_gn = c->any_mc_rate = false;
   for (int i = 0; i < n; ++i) {
@@ -1808,8 +1831,7 @@ int mpmx_set_materials(mpmx_ctx* c, int n, const double* params) {
     if (c->mats.back().type == MC && c->mats.back().rate != RATE_NONE) c->any_mc_rate = true;
   }
   c->d_mats.alloc(n);
-  c->d_mats.upload(c->mats.data(), n, c->stream);
-  HIPCHECK(hipStreamSynchronize(c->stream));
+  refresh_uniform(c, true);
   c->invalidate_graph();
   API_END(c)
 }

@@ -33,6 +33,10 @@ struct Material {
   // sin, cos, tan of the peak friction angle and tan of the peak dilation
   // angle, by the same functions as the per-particle evaluation
   double sphi, cphi, tphi, tpsi;
+  // MohrCoulomb without softening whose particles all hold the initial
+  // friction, dilation, cohesion and tension cutoff (state variables 1-4,
+  // which nothing changes then): the kernels take them from sv0
+  int uniform;
 };
 
 enum { RATE_NONE = 0, RATE_JAUMANN = 1, RATE_GREEN_NAGHDI = 2 };

@@ -207,16 +207,47 @@ MPMX_DEV void kinematics(const Mesh& M, const Particles& P, int p, int c, const 
   }
 }
 
-// Mohr-Coulomb state variables written back after an update: friction,
-// dilation, cohesion and tension cutoff change only with softening
-// (mohr_coulomb.tcc:360-384); the others are rewritten.
+// Mohr-Coulomb state variables written back after an update
+// (mohr_coulomb.tcc:348-513): the yield state always; epsilon, rho and theta
+// unless the particle separated (the update returns before the invariants);
+// pdstrain when it changed (separated or plastic); friction, dilation,
+// cohesion and tension cutoff only with softening (:360-384).
 MPMX_DEV void store_sv(const Particles& P, const Material& m, int64_t p, const double* sv) {
   const int64_t np = P.np;
-#pragma unroll
-  for (int k = 0; k < NSV; ++k) {
-    const bool strength = k == SV_PHI || k == SV_PSI || k == SV_COH || k == SV_TCUT;
-    if (!strength || m.softening) P.sv[k * np + p] = sv[k];
+  const double y = sv[SV_YIELD];
+  P.sv[SV_YIELD * np + p] = y;
+  if (y != 3.) {
+    P.sv[SV_EPS * np + p] = sv[SV_EPS];
+    P.sv[SV_RHO * np + p] = sv[SV_RHO];
+    P.sv[SV_THETA * np + p] = sv[SV_THETA];
   }
+  if (y != 0.) P.sv[SV_PDS * np + p] = sv[SV_PDS];
+  if (m.softening) {
+    P.sv[SV_PHI * np + p] = sv[SV_PHI];
+    P.sv[SV_PSI * np + p] = sv[SV_PSI];
+    P.sv[SV_COH * np + p] = sv[SV_COH];
+    P.sv[SV_TCUT * np + p] = sv[SV_TCUT];
+  }
+}
+
+// The state variables an update reads: friction, dilation, cohesion, tension
+// cutoff (from the material when uniform) and pdstrain; the yield state,
+// epsilon, rho and theta are outputs only.
+MPMX_DEV void load_sv(const Particles& P, const Material& m, int64_t p, double* sv) {
+  const int64_t np = P.np;
+  if (m.uniform) {
+    sv[SV_PHI] = m.sv0[SV_PHI];
+    sv[SV_PSI] = m.sv0[SV_PSI];
+    sv[SV_COH] = m.sv0[SV_COH];
+    sv[SV_TCUT] = m.sv0[SV_TCUT];
+  } else {
+    sv[SV_PHI] = P.sv[SV_PHI * np + p];
+    sv[SV_PSI] = P.sv[SV_PSI * np + p];
+    sv[SV_COH] = P.sv[SV_COH * np + p];
+    sv[SV_TCUT] = P.sv[SV_TCUT * np + p];
+  }
+  sv[SV_PDS] = P.sv[SV_PDS * np + p];
+  sv[SV_YIELD] = sv[SV_EPS] = sv[SV_RHO] = sv[SV_THETA] = 0.;
 }
 
 // Appends particle p to the deferred plastic list with its record: strain
@@ -274,10 +305,7 @@ MPMX_DEV void stress_strain(const Mesh& M, const Particles& P, const Material* m
   for (int k = 0; k < 6; ++k) s[k] = P.stress[k * np + p];
   const Material& m = mats[P.mat[p]];
   double sv[NSV];
-  if (m.type != LE) {
-#pragma unroll
-    for (int k = 0; k < NSV; ++k) sv[k] = P.sv[k * np + p];
-  }
+  if (m.type != LE) load_sv(P, m, p, sv);
   // an objective stress rate: the rotated trial stress (ROT builds only)
   int rate = RATE_NONE;
   if constexpr (ROT) rate = m.rate;
@@ -1144,12 +1172,8 @@ __global__ __launch_bounds__(kPlasticBlock) MPMX_PLASTIC_ATTR void k_plastic(Par
     T.yt = c < ns ? FS_SHEAR : FS_TENSILE;
 #pragma unroll
     for (int k = 0; k < 6; ++k) s[k] = P.stress[k * np + p];
-    sv[SV_PHI] = P.sv[SV_PHI * np + p];
-    sv[SV_PSI] = P.sv[SV_PSI * np + p];
-    sv[SV_COH] = P.sv[SV_COH * np + p];
-    sv[SV_TCUT] = P.sv[SV_TCUT * np + p];
-    sv[SV_PDS] = P.sv[SV_PDS * np + p];
     const Material& m = mats[P.mat[p]];
+    load_sv(P, m, p, sv);
     double trial[6];
     if (P.plastic_trial) {  // stress rates: the trial stress the predictor used
 #pragma unroll
