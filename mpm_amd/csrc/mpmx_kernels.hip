@@ -1430,39 +1430,155 @@ __global__ void k_fill_int(int* a, int64_t n, int v) {
   if (i < n) a[i] = v;
 }
 
-// Per node: merge the sorted particle lists of its cells by particle id and
-// sum contributions in that order (the single-threaded reference order).
-// WHAT: SC_MV (mass+momentum), SC_F (body+internal force), SC_MOM (momentum
-// only, mass kept). Then the node update flags FL as in k_node.
-template <int D, int WHAT, int FL>
-__global__ __launch_bounds__(kBlock) void k_gather(Mesh M, Particles P, Nodes Nd, StepArgs A,
-                                                   const Status* S, const int* perm,
-                                                   const int* cstart, const int* cend) {
-  if (run_stopped(S)) return;
+// Per node: the particles of its cells summed in particle-id order (the
+// single-threaded reference order). WHAT: SC_MV (mass+momentum), SC_F
+// (body+internal force), SC_MOM (momentum only, mass kept). Then the node
+// update flags FL as in k_node.
+//
+// One wave per node. Each lane takes slots of the node's concatenated cell
+// lists (each sorted by particle id, the lists disjoint), ranks its
+// particles by id among all of the node's particles, computes their
+// contributions and stores them at their rank in LDS; then one lane per
+// sum folds it in rank order, so every sum adds the same terms in the same
+// order as one thread walking the particles by id. A node with more than
+// kGatherMax particles is summed by one lane merging the cell lists.
+constexpr int kGatherMax = 128;
+
+// Slots of a node's sums: mass, momentum, external force, internal force.
+template <int D>
+struct GatherSlots {
+  static constexpr int M = 0, MOM = 1, FE = 1 + D, FI = 1 + 2 * D, N = 1 + 3 * D;
+  template <int WHAT>
+  static constexpr MPMX_DEV bool used(int v) {
+    return v == M ? (WHAT & SC_MV) != 0
+                  : (v < FE ? (WHAT & (SC_MV | SC_MOM)) != 0 : (WHAT & SC_F) != 0);
+  }
+  // the first slot WHAT uses (the used slots are consecutive) and their count
+  template <int WHAT>
+  static constexpr int first() {
+    return (WHAT & SC_MV) != 0 ? M : ((WHAT & SC_MOM) != 0 ? MOM : FE);
+  }
+  template <int WHAT>
+  static constexpr int count() {
+    return ((WHAT & SC_MV) != 0 ? 1 + D : ((WHAT & SC_MOM) != 0 ? D : 0)) +
+           ((WHAT & SC_F) != 0 ? 2 * D : 0);
+  }
+};
+
+// One particle's terms of node n's sums (n is local node i of the
+// particle's cell, whose node coordinates are X), each computed as the sum
+// it enters takes it: m N, m N v_k, g_k m N, and -(dN/dx . sigma)_k V.
+template <int D, int WHAT>
+MPMX_DEV void gather_terms(const Mesh& M, const Particles& P, int64_t n, int p,
+                           const double (*X)[D], int i, const double* g, double* out) {
+  using GS = GatherSlots<D>;
   constexpr int NN = Elem<D>::NN;
-  const int64_t n = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+  const int64_t np = P.np;
+  double xi[D];
+#pragma unroll
+  for (int k = 0; k < D; ++k) xi[k] = P.xi[k * np + p];
+  const double N = shapefn_i<D>(xi, i);
+  const double m = P.mass[p];
+  if constexpr ((WHAT & (SC_MV | SC_MOM)) != 0) {
+    const double mN = m * N;
+    out[GS::M] = mN;
+    double vel[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) vel[k] = P.v[k * np + p];
+    if (P.Cmat) {  // affine velocity updates
+      double Cp[D * D], xp[D], xn[D];
+#pragma unroll
+      for (int k = 0; k < D * D; ++k) Cp[k] = P.Cmat[k * np + p];
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+        xp[k] = P.x[k * np + p];
+        xn[k] = M.node_x[k * M.nnodes + n];
+      }
+      affine_velocity<D>(vel, Cp, xp, xn, vel);
+    }
+#pragma unroll
+    for (int k = 0; k < D; ++k) out[GS::MOM + k] = mN * vel[k];
+  }
+  if constexpr ((WHAT & SC_F) != 0) {
+#pragma unroll
+    for (int k = 0; k < D; ++k) out[GS::FE + k] = g[k] * m * N;
+    double Ji[D][D], d[D];
+    jacobian_inverse<D>(xi, X, Ji);
+    dndx_i<D>(xi, i, Ji, d);
+    double s[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) s[k] = P.stress[k * np + p];
+    double f[D];
+    if constexpr (D == 2) {
+      f[0] = d[0] * s[0] + d[1] * s[3];
+      f[1] = d[1] * s[1] + d[0] * s[3];
+    } else {
+      f[0] = d[0] * s[0] + d[1] * s[3] + d[2] * s[5];
+      f[1] = d[1] * s[1] + d[0] * s[3] + d[2] * s[4];
+      f[2] = d[2] * s[2] + d[1] * s[4] + d[0] * s[5];
+    }
+    const double mv = -1. * P.vol[p];
+#pragma unroll
+    for (int k = 0; k < D; ++k) out[GS::FI + k] = f[k] * mv;
+  }
+}
+
+template <int D, int WHAT, int FL>
+__global__ __launch_bounds__(64) void k_gather(Mesh M, Particles P, Nodes Nd, StepArgs A,
+                                               const Status* S, const int* perm,
+                                               const int* cstart, const int* cend) {
+  if (run_stopped(S)) return;
+  using GS = GatherSlots<D>;
+  constexpr int NN = Elem<D>::NN;
+  constexpr int NV = GS::N;
+  constexpr int V0 = GS::template first<WHAT>(), NU = GS::template count<WHAT>();
+  __shared__ double terms[NU][kGatherMax];  // the used slots V0 .. V0 + NU - 1
+  __shared__ double cellX[(WHAT & SC_F) != 0 ? NN : 1][NN][D];  // the cells' node coordinates
+  const int lane = threadIdx.x;
+  const int64_t n = blockIdx.x;
   if (n >= M.nnodes) return;
   const int64_t nnodes = M.nnodes, np = P.np;
-  int cells[NN], loc[NN], h[NN], e[NN];
-  bool any = false;
-#pragma unroll
-  for (int j = 0; j < NN; ++j) {
-    const int c = M.node_cells[n * NN + j];
-    cells[j] = c;
-    h[j] = e[j] = 0;
-    loc[j] = 0;
-    if (c >= 0) {
-      h[j] = cstart[c];
-      e[j] = cend[c];
-      if (h[j] < e[j]) any = true;
+  // lane j < NN: cell j of the node, its range in the sorted order and the
+  // node's local index in it
+  int cj = -1, hj = 0, ej = 0, lj = 0;
+  if (lane < NN) {
+    cj = M.node_cells[n * NN + lane];
+    if (cj >= 0) {
+      hj = cstart[cj];
+      ej = cend[cj];
 #pragma unroll
       for (int i = 0; i < NN; ++i)
-        if (M.cell_nodes[(int64_t)c * NN + i] == (int)n) loc[j] = i;
+        if (M.cell_nodes[(int64_t)cj * NN + i] == (int)n) lj = i;
     }
   }
-  if ((WHAT & SC_MV) != 0) Nd.active[n] = any ? 1 : 0;
+  int cells[NN], loc[NN], h[NN], e[NN], off[NN];
+  int T = 0;
+#pragma unroll
+  for (int j = 0; j < NN; ++j) {
+    cells[j] = __shfl(cj, j, 64);
+    loc[j] = __shfl(lj, j, 64);
+    h[j] = __shfl(hj, j, 64);
+    e[j] = __shfl(ej, j, 64);
+    if (e[j] < h[j]) e[j] = h[j];
+    off[j] = T;
+    T += e[j] - h[j];
+  }
+  const bool any = T > 0;
+  if constexpr ((WHAT & SC_F) != 0) {
+    if (any)
+      for (int q = lane; q < NN * NN; q += 64) {
+        const int j = q / NN, i = q % NN;
+        if (cells[j] >= 0) {
+          const int nd = M.cell_nodes[(int64_t)cells[j] * NN + i];
+#pragma unroll
+          for (int k = 0; k < D; ++k) cellX[j][i][k] = M.node_x[k * M.nnodes + nd];
+        }
+      }
+    __syncthreads();
+  }
+  if ((WHAT & SC_MV) != 0 && lane == 0) Nd.active[n] = any ? 1 : 0;
   if (!any) {
-    if ((FL & ND_ACC) != 0) {
+    if ((FL & ND_ACC) != 0 && lane == 0) {
       double* b = ad_record_base(A);
       if (b) b[n] = 0.;
     }
@@ -1470,75 +1586,97 @@ __global__ __launch_bounds__(kBlock) void k_gather(Mesh M, Particles P, Nodes Nd
   }
   double g[3] = {0, 0, 0};
   if constexpr ((WHAT & SC_F) != 0) gravity_now(A, S, D, g);
-  double am = 0., amom[D], afe[D], afi[D];
+  double sums[NV];
 #pragma unroll
-  for (int k = 0; k < D; ++k) amom[k] = afe[k] = afi[k] = 0.;
+  for (int v = 0; v < NV; ++v) sums[v] = 0.;
+  if (T <= kGatherMax) {
+    // this lane's slots lane and lane + 64: particle, cell slot, local node
+    int ps[2], cs[2] = {0, 0}, is[2] = {0, 0};
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int sl = lane + 64 * q;
+      ps[q] = 0x7fffffff;
+      if (sl < T) {  // the cell slot j holding sl (selects, no indexed register arrays)
+        int j = 0, hs = h[0], os = off[0], ls = loc[0];
+#pragma unroll
+        for (int jj = 1; jj < NN; ++jj)
+          if (sl >= off[jj]) {
+            j = jj;
+            hs = h[jj];
+            os = off[jj];
+            ls = loc[jj];
+          }
+        ps[q] = perm[hs + (sl - os)];
+        cs[q] = j;
+        is[q] = ls;
+      }
+    }
+    // rank among the node's particles (ids are distinct)
+    int rk[2] = {0, 0};
+    for (int t = 0; t < T; ++t) {
+      const int pt = t < 64 ? __builtin_amdgcn_readlane(ps[0], t)
+                            : __builtin_amdgcn_readlane(ps[1], t - 64);
+      rk[0] += pt < ps[0] ? 1 : 0;
+      rk[1] += pt < ps[1] ? 1 : 0;
+    }
+#pragma unroll 1
+    for (int q = 0; q < 2; ++q) {  // one slot at a time (register pressure)
+      if (lane + 64 * q < T) {
+        const int pq = q ? ps[1] : ps[0], cq = q ? cs[1] : cs[0], iq = q ? is[1] : is[0];
+        const int rq = q ? rk[1] : rk[0];
+        double out[NV];
+        gather_terms<D, WHAT>(M, P, n, pq, cellX[(WHAT & SC_F) != 0 ? cq : 0], iq, g, out);
+#pragma unroll
+        for (int v = 0; v < NU; ++v) terms[v][rq] = out[V0 + v];
+      }
+    }
+    __syncthreads();
+    double acc = 0.;
+    if (lane < NU) {
+#pragma unroll 8
+      for (int r = 0; r < T; ++r) acc = acc + terms[lane][r];
+    }
+#pragma unroll
+    for (int v = 0; v < NU; ++v) sums[V0 + v] = __shfl(acc, v, 64);
+  } else if (lane == 0) {
+    // too many particles for the LDS slots: merge the cell lists by id
+    int hh[NN];
+#pragma unroll
+    for (int j = 0; j < NN; ++j) hh[j] = h[j];
+    while (true) {
+      int bj = -1, bp = 0x7fffffff, bl = 0;
+#pragma unroll
+      for (int j = 0; j < NN; ++j)
+        if (hh[j] < e[j]) {
+          const int p = perm[hh[j]];
+          if (p < bp) {
+            bp = p;
+            bj = j;
+            bl = loc[j];
+          }
+        }
+      if (bj < 0) break;
+#pragma unroll
+      for (int j = 0; j < NN; ++j)
+        if (j == bj) hh[j] += 1;
+      double out[NV];
+      gather_terms<D, WHAT>(M, P, n, bp, cellX[(WHAT & SC_F) != 0 ? bj : 0], bl, g, out);
+#pragma unroll
+      for (int v = 0; v < NV; ++v)
+        if (GS::template used<WHAT>(v)) sums[v] = sums[v] + out[v];
+    }
+  }
+  if (lane != 0) return;
+  double am = sums[GS::M], amom[D], afe[D], afi[D];
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    amom[k] = sums[GS::MOM + k];
+    afe[k] = sums[GS::FE + k];
+    afi[k] = sums[GS::FI + k];
+  }
   int h0[NN];
 #pragma unroll
   for (int j = 0; j < NN; ++j) h0[j] = h[j];
-  while (true) {
-    int bj = -1, bp = 0x7fffffff;
-#pragma unroll
-    for (int j = 0; j < NN; ++j)
-      if (h[j] < e[j]) {
-        const int p = perm[h[j]];
-        if (p < bp) {
-          bp = p;
-          bj = j;
-        }
-      }
-    if (bj < 0) break;
-    h[bj] += 1;
-    const int p = bp, c = cells[bj], i = loc[bj];
-    double xi[D];
-#pragma unroll
-    for (int k = 0; k < D; ++k) xi[k] = P.xi[k * np + p];
-    const double N = shapefn_i<D>(xi, i);
-    const double m = P.mass[p];
-    if constexpr ((WHAT & (SC_MV | SC_MOM)) != 0) {
-      const double mN = m * N;
-      if constexpr ((WHAT & SC_MV) != 0) am = am + mN;
-      double vel[D];
-#pragma unroll
-      for (int k = 0; k < D; ++k) vel[k] = P.v[k * np + p];
-      if (P.Cmat) {  // affine velocity updates
-        double Cp[D * D], xp[D], xn[D];
-#pragma unroll
-        for (int k = 0; k < D * D; ++k) Cp[k] = P.Cmat[k * np + p];
-#pragma unroll
-        for (int k = 0; k < D; ++k) {
-          xp[k] = P.x[k * np + p];
-          xn[k] = M.node_x[k * M.nnodes + n];
-        }
-        affine_velocity<D>(vel, Cp, xp, xn, vel);
-      }
-#pragma unroll
-      for (int k = 0; k < D; ++k) amom[k] = amom[k] + mN * vel[k];
-    }
-    if constexpr ((WHAT & SC_F) != 0) {
-#pragma unroll
-      for (int k = 0; k < D; ++k) afe[k] = afe[k] + g[k] * m * N;
-      double X[NN][D], Ji[D][D], d[D];
-      load_cell_X<D>(M, c, X);
-      jacobian_inverse<D>(xi, X, Ji);
-      dndx_i<D>(xi, i, Ji, d);
-      double s[6];
-#pragma unroll
-      for (int k = 0; k < 6; ++k) s[k] = P.stress[k * np + p];
-      double f[D];
-      if constexpr (D == 2) {
-        f[0] = d[0] * s[0] + d[1] * s[3];
-        f[1] = d[1] * s[1] + d[0] * s[3];
-      } else {
-        f[0] = d[0] * s[0] + d[1] * s[3] + d[2] * s[5];
-        f[1] = d[1] * s[1] + d[0] * s[3] + d[2] * s[4];
-        f[2] = d[2] * s[2] + d[1] * s[4] + d[0] * s[5];
-      }
-      const double mv = -1. * P.vol[p];
-#pragma unroll
-      for (int k = 0; k < D; ++k) afi[k] = afi[k] + f[k] * mv;
-    }
-  }
   // particle tractions after every body force, in particle order
   // (Mesh::apply_traction_on_particles, mesh.tcc:1371-1386)
   if constexpr ((WHAT & SC_F) != 0) {
@@ -1546,7 +1684,7 @@ __global__ __launch_bounds__(kBlock) void k_gather(Mesh M, Particles P, Nodes Nd
 #pragma unroll
       for (int j = 0; j < NN; ++j) h[j] = h0[j];
       while (true) {
-        int bj = -1, bp = 0x7fffffff;
+        int bj = -1, bp = 0x7fffffff, bl = 0;
 #pragma unroll
         for (int j = 0; j < NN; ++j)
           if (h[j] < e[j]) {
@@ -1554,11 +1692,14 @@ __global__ __launch_bounds__(kBlock) void k_gather(Mesh M, Particles P, Nodes Nd
             if (p < bp) {
               bp = p;
               bj = j;
+              bl = loc[j];
             }
           }
         if (bj < 0) break;
-        h[bj] += 1;
-        const int p = bp, i = loc[bj];
+#pragma unroll
+        for (int j = 0; j < NN; ++j)
+          if (j == bj) h[j] += 1;
+        const int p = bp, i = bl;
         bool has = false;
 #pragma unroll
         for (int k = 0; k < D; ++k) has = has || P.tr_fn[k * np + p] >= -1;
@@ -1652,6 +1793,12 @@ static inline dim3 grid_for(int64_t n) { return dim3((unsigned)((n + kBlock - 1)
     if ((n) > 0)                                                                      \
       hipLaunchKernelGGL(kernel, dim3((unsigned)(((n) + (CH)-1) / (CH))), dim3(CH), 0, \
                          stream, __VA_ARGS__);                                        \
+  } while (0)
+// One 64-lane workgroup per item (the ordered-mode node gathers).
+#define MPMX_LAUNCH_WAVES(kernel, n, stream, ...)                                  \
+  do {                                                                             \
+    if ((n) > 0)                                                                   \
+      hipLaunchKernelGGL(kernel, dim3((unsigned)(n)), dim3(64), 0, stream, __VA_ARGS__); \
   } while (0)
 #define MPMX_CHUNK_DISPATCH(chunk, LAUNCH) \
   do {                                     \
@@ -1843,17 +1990,20 @@ void launch_gather(int which, const Mesh& M, const Particles& P, const Nodes& Nd
                    const int* cend, hipStream_t st) {
   switch (which) {
     case GK_MV_VEL:  // USF first pass
-      MPMX_LAUNCH((k_gather<D, SC_MV, ND_INIT | ND_VEL>), M.nnodes, st, M, P, Nd, A, S, perm, cstart, cend);
+      MPMX_LAUNCH_WAVES((k_gather<D, SC_MV, ND_INIT | ND_VEL>), M.nnodes, st, M, P, Nd, A, S,
+                     perm, cstart, cend);
       break;
     case GK_F_ACC:  // USF force pass
-      MPMX_LAUNCH((k_gather<D, SC_F, ND_ACC>), M.nnodes, st, M, P, Nd, A, S, perm, cstart, cend);
+      MPMX_LAUNCH_WAVES((k_gather<D, SC_F, ND_ACC>), M.nnodes, st, M, P, Nd, A, S, perm, cstart,
+                     cend);
       break;
     case GK_ALL_VEL_ACC:  // USL / MUSL first pass
-      MPMX_LAUNCH((k_gather<D, SC_MV | SC_F, ND_INIT | ND_VEL | ND_ACC>), M.nnodes, st, M, P, Nd, A, S,
-                  perm, cstart, cend);
+      MPMX_LAUNCH_WAVES((k_gather<D, SC_MV | SC_F, ND_INIT | ND_VEL | ND_ACC>), M.nnodes, st, M,
+                     P, Nd, A, S, perm, cstart, cend);
       break;
     case GK_MOM_VEL:  // MUSL second momentum pass
-      MPMX_LAUNCH((k_gather<D, SC_MOM, ND_VEL>), M.nnodes, st, M, P, Nd, A, S, perm, cstart, cend);
+      MPMX_LAUNCH_WAVES((k_gather<D, SC_MOM, ND_VEL>), M.nnodes, st, M, P, Nd, A, S, perm, cstart,
+                     cend);
       break;
   }
 }
