@@ -288,11 +288,12 @@ int mpmx_step(mpmx_ctx* ctx, int nsteps, double dt, double damping_factor);
  * Same steps and results as mpmx_step; the kernels launch directly (no
  * graph is instantiated) and take their scratch (sort keys and permutation,
  * radix-sort temporaries, locate queue, deferred Mohr-Coulomb list and
- * records, Jaumann spin, math-function values, ordered-mode cell ranges)
- * from one 64 MiB file-scope device array per GPU, held until the process
- * ends. A context whose step needs more (144 bytes per particle with a
- * Mohr-Coulomb material, 32 without, plus 24 with the Jaumann rate: about
- * 460k Mohr-Coulomb particles) gets MPMX_E_ARG; particle removal
+ * records, Jaumann spin, math-function values, ordered-mode cell ranges
+ * and gather terms) from one 64 MiB file-scope device array per GPU, held
+ * until the process ends. A context whose step needs more (144 bytes per
+ * particle with a Mohr-Coulomb material, 32 without, plus 24 with the
+ * Jaumann rate, plus 256 in 3D and 96 in 2D in ordered mode: about 460k
+ * Mohr-Coulomb particles in fast mode) gets MPMX_E_ARG; particle removal
  * (mpmx_set_locate_particles(ctx, 0)) gets MPMX_E_UNSUPPORTED. Every call on
  * one GPU shares that array and the first step of each call sorts fully:
  * calls that overlap on one GPU (contexts used from several threads) race. */

@@ -173,6 +173,7 @@ struct mpmx_ctx {
   DBuf<double> d_rotF;          // Particles::rotF (any_gn)
   DBuf<double> d_plastic_trial; // Particles::plastic_trial (any_mc_rate)
   DBuf<double> d_Cmat, d_beta;  // affine transfer matrices, ASFLIP beta (affine())
+  DBuf<double> d_gterm;         // Particles::gterm (ordered mode)
   DBuf<int> d_plastic_list;     // deferred Mohr-Coulomb return mappings
   DBuf<double> d_plastic_rec;
   DBuf<Status> d_status;
@@ -276,6 +277,7 @@ struct mpmx_ctx {
     P.spin = any_rate ? d_spin.p : nullptr;
     P.rotF = any_gn ? d_rotF.p : nullptr;
     P.plastic_trial = any_mc_rate ? d_plastic_trial.p : nullptr;
+    P.gterm = (flags & MPMX_ORDERED) ? d_gterm.p : nullptr;
     P.Bmat = affine() ? d_Bmat.p : nullptr;
     P.Cmat = affine() ? d_Cmat.p : nullptr;
     P.beta = affine() ? d_beta.p : nullptr;
@@ -811,6 +813,7 @@ void alloc_step_buffers(mpmx_ctx* c) {
   if (c->flags & MPMX_ORDERED) {
     c->d_cstart.alloc(std::max<int64_t>(c->ncells, 1));
     c->d_cend.alloc(std::max<int64_t>(c->ncells, 1));
+    c->d_gterm.alloc((size_t)c->nn * (1 + c->dim) * std::max<int64_t>(np, 1));
   }
 }
 
@@ -1002,6 +1005,7 @@ struct StepBuffers {
   double* beta;
   double* rotF;   // Green-Naghdi scratch; rotated trial stresses of the deferred list
   double* plastic_trial;
+  double* gterm;  // ordered mode: per-particle gather terms
 };
 
 StepBuffers ctx_buffers(mpmx_ctx* c) {
@@ -1021,6 +1025,7 @@ StepBuffers ctx_buffers(mpmx_ctx* c) {
   B.beta = c->d_beta.p;
   B.rotF = c->d_rotF.p;
   B.plastic_trial = c->d_plastic_trial.p;
+  B.gterm = c->d_gterm.p;
   return B;
 }
 
@@ -1078,6 +1083,7 @@ void enqueue_step(mpmx_ctx* c, const StepArgs& A, Launcher& L, bool full_sort,
   }
   if (P.rotF) P.rotF = B.rotF;
   if (P.plastic_trial) P.plastic_trial = B.plastic_trial;
+  if (P.gterm) P.gterm = B.gterm;
   Nodes Nd = c->nodes();
   if (Nd.fnval) Nd.fnval = B.fnval;
   hipStream_t st = c->stream;
@@ -1385,9 +1391,11 @@ size_t carve_step_buffers(const mpmx_ctx* c, uintptr_t base, StepBuffers* B) {
   B->sort_tmp = (int*)take(radix_sort_temp_ints(np) * sizeof(int));
   B->scan = (int*)take(np * sizeof(int));
   B->cstart = B->cend = nullptr;
+  B->gterm = nullptr;
   if (c->flags & MPMX_ORDERED) {
     B->cstart = (int*)take(std::max<int64_t>(c->ncells, 1) * sizeof(int));
     B->cend = (int*)take(std::max<int64_t>(c->ncells, 1) * sizeof(int));
+    B->gterm = (double*)take((size_t)c->nn * (1 + c->dim) * np * sizeof(double));
   }
   B->plastic_list = nullptr;
   B->plastic_rec = nullptr;

@@ -130,6 +130,11 @@ struct Particles {
   // stress of each deferred return mapping, by list slot (null without).
   double* rotF;
   double* plastic_trial;
+  // Ordered mode: each particle's shape function and internal-force term
+  // for each node of its cell, [NN][1 + dim][np] (N_i, then
+  // -(dN_i/dx . sigma)_k V), written before a force gather (null in fast
+  // mode).
+  double* gterm;
   int* plastic_list;
   double* plastic_rec;  // [PR_N][np] by list slot
   // Particle tractions (Particle::assign_traction / map_traction_force,

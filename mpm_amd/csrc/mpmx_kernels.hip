@@ -1465,19 +1465,66 @@ struct GatherSlots {
   }
 };
 
-// One particle's terms of node n's sums (n is local node i of the
-// particle's cell, whose node coordinates are X), each computed as the sum
-// it enters takes it: m N, m N v_k, g_k m N, and -(dN/dx . sigma)_k V.
-template <int D, int WHAT>
-MPMX_DEV void gather_terms(const Mesh& M, const Particles& P, int64_t n, int p,
-                           const double (*X)[D], int i, const double* g, double* out) {
-  using GS = GatherSlots<D>;
+// Before a force gather: each particle's shape function and internal-force
+// term for every node of its cell (Particles::gterm), so that the node
+// gathers read them instead of recomputing the cell's Jacobian per node.
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_gather_prep(Mesh M, Particles P, const Status* S) {
+  if (run_stopped(S)) return;
   constexpr int NN = Elem<D>::NN;
+  const int64_t p = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+  if (p >= P.np) return;
   const int64_t np = P.np;
-  double xi[D];
+  const int c = P.cell[p];
+  if (!valid_cell<D>(M, c)) return;
+  double xi[D], X[NN][D], Ji[D][D], s[6];
 #pragma unroll
   for (int k = 0; k < D; ++k) xi[k] = P.xi[k * np + p];
-  const double N = shapefn_i<D>(xi, i);
+#pragma unroll
+  for (int k = 0; k < 6; ++k) s[k] = P.stress[k * np + p];
+  const double mv = -1. * P.vol[p];
+  load_cell_X<D>(M, c, X);
+  jacobian_inverse<D>(xi, X, Ji);
+#pragma unroll
+  for (int i = 0; i < NN; ++i) {
+    double d[D], f[D];
+    dndx_i<D>(xi, i, Ji, d);
+    if constexpr (D == 2) {
+      f[0] = d[0] * s[0] + d[1] * s[3];
+      f[1] = d[1] * s[1] + d[0] * s[3];
+    } else {
+      f[0] = d[0] * s[0] + d[1] * s[3] + d[2] * s[5];
+      f[1] = d[1] * s[1] + d[0] * s[3] + d[2] * s[4];
+      f[2] = d[2] * s[2] + d[1] * s[4] + d[0] * s[5];
+    }
+    double* t = P.gterm + (int64_t)i * (1 + D) * np + p;
+    t[0] = shapefn_i<D>(xi, i);
+#pragma unroll
+    for (int k = 0; k < D; ++k) t[(1 + k) * np] = f[k] * mv;
+  }
+}
+
+// One particle's terms of node n's sums (n is local node i of the
+// particle's cell), each computed as the sum it enters takes it: m N,
+// m N v_k, g_k m N, and -(dN/dx . sigma)_k V. N and the force terms come
+// from Particles::gterm in the force gathers and in the MUSL momentum
+// gather (which follows the first gather of the same step at unchanged
+// local coordinates).
+template <int D, int WHAT>
+MPMX_DEV void gather_terms(const Mesh& M, const Particles& P, int64_t n, int p, int i,
+                           const double* g, double* out) {
+  using GS = GatherSlots<D>;
+  const int64_t np = P.np;
+  double N;
+  const double* t = P.gterm + (int64_t)i * (1 + D) * np + p;
+  if constexpr ((WHAT & (SC_F | SC_MOM)) != 0) {
+    N = t[0];
+  } else {
+    double xi[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) xi[k] = P.xi[k * np + p];
+    N = shapefn_i<D>(xi, i);
+  }
   const double m = P.mass[p];
   if constexpr ((WHAT & (SC_MV | SC_MOM)) != 0) {
     const double mN = m * N;
@@ -1501,25 +1548,10 @@ MPMX_DEV void gather_terms(const Mesh& M, const Particles& P, int64_t n, int p,
   }
   if constexpr ((WHAT & SC_F) != 0) {
 #pragma unroll
-    for (int k = 0; k < D; ++k) out[GS::FE + k] = g[k] * m * N;
-    double Ji[D][D], d[D];
-    jacobian_inverse<D>(xi, X, Ji);
-    dndx_i<D>(xi, i, Ji, d);
-    double s[6];
-#pragma unroll
-    for (int k = 0; k < 6; ++k) s[k] = P.stress[k * np + p];
-    double f[D];
-    if constexpr (D == 2) {
-      f[0] = d[0] * s[0] + d[1] * s[3];
-      f[1] = d[1] * s[1] + d[0] * s[3];
-    } else {
-      f[0] = d[0] * s[0] + d[1] * s[3] + d[2] * s[5];
-      f[1] = d[1] * s[1] + d[0] * s[3] + d[2] * s[4];
-      f[2] = d[2] * s[2] + d[1] * s[4] + d[0] * s[5];
+    for (int k = 0; k < D; ++k) {
+      out[GS::FE + k] = g[k] * m * N;
+      out[GS::FI + k] = t[(1 + k) * np];
     }
-    const double mv = -1. * P.vol[p];
-#pragma unroll
-    for (int k = 0; k < D; ++k) out[GS::FI + k] = f[k] * mv;
   }
 }
 
@@ -1533,7 +1565,6 @@ __global__ __launch_bounds__(64) void k_gather(Mesh M, Particles P, Nodes Nd, St
   constexpr int NV = GS::N;
   constexpr int V0 = GS::template first<WHAT>(), NU = GS::template count<WHAT>();
   __shared__ double terms[NU][kGatherMax];  // the used slots V0 .. V0 + NU - 1
-  __shared__ double cellX[(WHAT & SC_F) != 0 ? NN : 1][NN][D];  // the cells' node coordinates
   const int lane = threadIdx.x;
   const int64_t n = blockIdx.x;
   if (n >= M.nnodes) return;
@@ -1564,18 +1595,6 @@ __global__ __launch_bounds__(64) void k_gather(Mesh M, Particles P, Nodes Nd, St
     T += e[j] - h[j];
   }
   const bool any = T > 0;
-  if constexpr ((WHAT & SC_F) != 0) {
-    if (any)
-      for (int q = lane; q < NN * NN; q += 64) {
-        const int j = q / NN, i = q % NN;
-        if (cells[j] >= 0) {
-          const int nd = M.cell_nodes[(int64_t)cells[j] * NN + i];
-#pragma unroll
-          for (int k = 0; k < D; ++k) cellX[j][i][k] = M.node_x[k * M.nnodes + nd];
-        }
-      }
-    __syncthreads();
-  }
   if ((WHAT & SC_MV) != 0 && lane == 0) Nd.active[n] = any ? 1 : 0;
   if (!any) {
     if ((FL & ND_ACC) != 0 && lane == 0) {
@@ -1590,24 +1609,22 @@ __global__ __launch_bounds__(64) void k_gather(Mesh M, Particles P, Nodes Nd, St
 #pragma unroll
   for (int v = 0; v < NV; ++v) sums[v] = 0.;
   if (T <= kGatherMax) {
-    // this lane's slots lane and lane + 64: particle, cell slot, local node
-    int ps[2], cs[2] = {0, 0}, is[2] = {0, 0};
+    // this lane's slots lane and lane + 64: particle and local node
+    int ps[2], is[2] = {0, 0};
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
       const int sl = lane + 64 * q;
       ps[q] = 0x7fffffff;
-      if (sl < T) {  // the cell slot j holding sl (selects, no indexed register arrays)
-        int j = 0, hs = h[0], os = off[0], ls = loc[0];
+      if (sl < T) {  // the cell holding sl (selects, no indexed register arrays)
+        int hs = h[0], os = off[0], ls = loc[0];
 #pragma unroll
         for (int jj = 1; jj < NN; ++jj)
           if (sl >= off[jj]) {
-            j = jj;
             hs = h[jj];
             os = off[jj];
             ls = loc[jj];
           }
         ps[q] = perm[hs + (sl - os)];
-        cs[q] = j;
         is[q] = ls;
       }
     }
@@ -1622,10 +1639,10 @@ __global__ __launch_bounds__(64) void k_gather(Mesh M, Particles P, Nodes Nd, St
 #pragma unroll 1
     for (int q = 0; q < 2; ++q) {  // one slot at a time (register pressure)
       if (lane + 64 * q < T) {
-        const int pq = q ? ps[1] : ps[0], cq = q ? cs[1] : cs[0], iq = q ? is[1] : is[0];
+        const int pq = q ? ps[1] : ps[0], iq = q ? is[1] : is[0];
         const int rq = q ? rk[1] : rk[0];
         double out[NV];
-        gather_terms<D, WHAT>(M, P, n, pq, cellX[(WHAT & SC_F) != 0 ? cq : 0], iq, g, out);
+        gather_terms<D, WHAT>(M, P, n, pq, iq, g, out);
 #pragma unroll
         for (int v = 0; v < NU; ++v) terms[v][rq] = out[V0 + v];
       }
@@ -1660,7 +1677,7 @@ __global__ __launch_bounds__(64) void k_gather(Mesh M, Particles P, Nodes Nd, St
       for (int j = 0; j < NN; ++j)
         if (j == bj) hh[j] += 1;
       double out[NV];
-      gather_terms<D, WHAT>(M, P, n, bp, cellX[(WHAT & SC_F) != 0 ? bj : 0], bl, g, out);
+      gather_terms<D, WHAT>(M, P, n, bp, bl, g, out);
 #pragma unroll
       for (int v = 0; v < NV; ++v)
         if (GS::template used<WHAT>(v)) sums[v] = sums[v] + out[v];
@@ -1994,10 +2011,12 @@ void launch_gather(int which, const Mesh& M, const Particles& P, const Nodes& Nd
                      perm, cstart, cend);
       break;
     case GK_F_ACC:  // USF force pass
+      MPMX_LAUNCH((k_gather_prep<D>), P.np, st, M, P, S);
       MPMX_LAUNCH_WAVES((k_gather<D, SC_F, ND_ACC>), M.nnodes, st, M, P, Nd, A, S, perm, cstart,
                      cend);
       break;
     case GK_ALL_VEL_ACC:  // USL / MUSL first pass
+      MPMX_LAUNCH((k_gather_prep<D>), P.np, st, M, P, S);
       MPMX_LAUNCH_WAVES((k_gather<D, SC_MV | SC_F, ND_INIT | ND_VEL | ND_ACC>), M.nnodes, st, M,
                      P, Nd, A, S, perm, cstart, cend);
       break;
