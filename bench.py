@@ -24,7 +24,6 @@ the oracle's OpenMP build (the reference algorithm restated in C++) on the
 host's cores over a bounded number of steps of the same workload.
 """
 import argparse
-import ctypes
 import json
 import os
 import platform
@@ -66,30 +65,52 @@ STEP_BYTES_PER_PARTICLE = {(3, "mc"): 576, (3, "le"): 528, (2, "mc"): 368, (2, "
 STEP_BYTES_PER_NODE = {3: 256, 2: 176}
 
 
-class _Roctx:
-    """roctxProfilerResume/Pause around the timed window, so that
-    `rocprofv3 --selected-regions` traces exactly the steps `value` is
-    measured on. No-ops without the ROCm profiler SDK or outside a profiler."""
-
-    def __init__(self):
-        self.lib = None
-        for name in ("librocprofiler-sdk-roctx.so.1", "librocprofiler-sdk-roctx.so"):
-            try:
-                self.lib = ctypes.CDLL(name)
-                break
-            except OSError:
-                continue
-        if self.lib is not None:
-            self.lib.roctxProfilerPause.argtypes = [ctypes.c_uint64]
-            self.lib.roctxProfilerResume.argtypes = [ctypes.c_uint64]
-
-    def pause(self):
-        if self.lib is not None:
-            self.lib.roctxProfilerPause(0)
-
-    def resume(self):
-        if self.lib is not None:
-            self.lib.roctxProfilerResume(0)
+def summarize_trace(path, first, count, scale=None):
+    """Per-kernel statistics of the dispatches in a rocprofv3 CSV (kernel
+    trace, or one counter of a --pmc pass) that belong to simulated steps
+    first+1 .. first+count of the run: those after the first-th k_step_end
+    dispatch up to the (first+count)-th. Steps are counted from the first
+    solver the command creates, so run it without the autotuner (--chunk and
+    --sort-every fixed), whose trial steps would shift the count. Returns
+    markdown rows."""
+    import csv
+    rows = list(csv.DictReader(open(path)))
+    rows.sort(key=lambda r: int(r["Start_Timestamp"]))
+    ends = 0
+    sel = []
+    for r in rows:
+        name = r["Kernel_Name"]
+        if first <= ends < first + count:
+            sel.append(r)
+        if name.startswith("mpmx::k_step_end"):
+            ends += 1
+    if ends < first + count:
+        raise SystemExit("only %d steps in %s" % (ends, path))
+    per = {}
+    for r in sel:
+        name = r["Kernel_Name"].split("(")[0].replace("void ", "")
+        if scale is None:
+            v = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3
+        else:
+            v = float(r["Counter_Value"]) * 1024.0 * scale
+        per.setdefault(name, []).append(v)
+    out = []
+    if scale is None:
+        total = sum(sum(v) for v in per.values())
+        out.append("| kernel | calls | avg us | min us | max us | % |")
+        out.append("|---|---|---|---|---|---|")
+        for k, v in sorted(per.items(), key=lambda kv: -sum(kv[1])):
+            out.append("| `%s` | %d | %.1f | %.1f | %.1f | %.1f |" % (
+                k, len(v), sum(v) / len(v), min(v), max(v), 100 * sum(v) / total))
+        out.append("")
+        out.append("kernel time per step: %.4f ms" % (total / count / 1e3))
+    else:
+        out.append("| kernel | dispatches | median bytes | median MB |")
+        out.append("|---|---|---|---|")
+        for k, v in sorted(per.items(), key=lambda kv: -sorted(kv[1])[len(kv[1]) // 2]):
+            med = sorted(v)[len(v) // 2]
+            out.append("| `%s` | %d | %.0f | %.1f |" % (k, len(v), med, med / 1e6))
+    return "\n".join(out)
 
 
 def cpu_model():
@@ -386,7 +407,17 @@ def main():
                     help="steps between cell sorts instead of autotuning")
     ap.add_argument("--no-check", action="store_true",
                     help="adjoint_2d: skip the finite-difference checks")
+    ap.add_argument("--summarize", metavar="CSV", default=None,
+                    help="print per-kernel statistics of the timed window's dispatches from a "
+                         "rocprofv3 kernel-trace CSV (or a --pmc counter CSV with --scale) and "
+                         "exit; the window is --preroll + --warmup steps in, --steps long")
+    ap.add_argument("--scale", type=float, default=None,
+                    help="with --summarize of a counter CSV: bytes per KiB counted / 1024")
     args = ap.parse_args()
+    if args.summarize:
+        pre = 300 if args.preroll is None and args.workload == "collapse_3d" else (args.preroll or 0)
+        print(summarize_trace(args.summarize, pre + args.warmup, args.steps or 200, args.scale))
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: the explicit step is built and measured on a "
               "single MI355X (multi-GPU sharding is a design, not part of this build)",
@@ -396,8 +427,6 @@ def main():
     from mpm_amd import workloads
     from mpm_amd.capi import Solver
 
-    roctx = _Roctx()
-    roctx.pause()
     w = getattr(workloads, args.workload)()
     if args.preroll is None:
         args.preroll = 300 if args.workload == "collapse_3d" else 0
@@ -421,12 +450,10 @@ def main():
         tuned = s.autotune(w.dt, trial_steps=10)
     s.step(args.preroll + args.warmup, w.dt)
     s.synchronize()
-    roctx.resume()
     t0 = time.perf_counter()
     s.step(args.steps, w.dt)
     s.synchronize()
     elapsed = time.perf_counter() - t0
-    roctx.pause()
     ms_per_step = 1e3 * elapsed / args.steps
     value = npart * args.steps / elapsed
 
