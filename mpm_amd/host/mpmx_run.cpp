@@ -236,6 +236,8 @@ class MPMExplicit {
   std::vector<double> pvc_val_;
   std::vector<int> vdir_, fdir_, fsign_;
   std::vector<double> vval_, fmu_;
+  std::vector<int64_t> enode_;  // nodal Euler angles: nodes, dim angles each
+  std::vector<double> eang_;
   // acceleration constraints and the math functions they may refer to
   std::vector<int64_t> anode_;
   std::vector<int> adir_, afn_;
@@ -357,6 +359,20 @@ void MPMExplicit<Tdim>::initialise_mesh() {
     if (it == node_sets_.end()) throw std::runtime_error("node set " + std::to_string(id) + " not defined");
     return it->second;
   };
+  // nodal_euler_angles (MPMBase::node_euler_angles, mpm_base.tcc:1041-1063):
+  // a file of "node angle_1 .. angle_dim" rows in radians
+  // (IOMeshAscii::read_euler_angles, io_mesh_ascii.tcc:351-391; a node's
+  // first row counts)
+  if (bc.contains("nodal_euler_angles")) {
+    std::map<int64_t, std::vector<double>> ang;
+    for (const auto& r : read_rows(io_->file_name(bc.at("nodal_euler_angles").str())))
+      for (size_t q = 0; q + Tdim < r.size(); q += 1 + Tdim)
+        ang.emplace((int64_t)r[q], std::vector<double>(r.begin() + q + 1, r.begin() + q + 1 + Tdim));
+    for (const auto& kv : ang) {
+      enode_.push_back(kv.first);
+      eang_.insert(eang_.end(), kv.second.begin(), kv.second.end());
+    }
+  }
   if (bc.contains("velocity_constraints")) {
     for (const Json& c : bc.at("velocity_constraints").elements()) {
       if (c.contains("file")) {
@@ -932,6 +948,7 @@ void MPMExplicit<Tdim>::summary() const {
               "\"nnodes\": %lld, \"ncells\": %lld, \"node_sets\": %zu, \"cell_sets\": %zu, "
               "\"velocity_constraints\": %zu, \"acceleration_constraints\": %zu, "
               "\"math_functions\": %zu, \"nodal_forces\": %zu, \"friction_constraints\": %zu, "
+              "\"nodal_euler_angles\": %zu, "
               "\"particle_velocity_constraints\": %zu, \"particles\": %lld, "
               "\"gravity\": [%.17g, %.17g, %.17g], \"ramping_time\": %.17g, \"materials\": [",
               Tdim, scheme_.c_str(),
@@ -940,6 +957,7 @@ void MPMExplicit<Tdim>::summary() const {
               (long long)output_steps_, damping_, (long long)(coords_.size() / Tdim),
               (long long)(cells_.size() / NN), node_sets_.size(), cell_sets_.size(),
               vnode_.size(), anode_.size(), fn_type_.size(), cfnode_.size(), fnode_.size(),
+              enode_.size(),
               pvc_particle_.size(),
               (long long)expected_particles_,
               gravity_[0],
@@ -973,6 +991,10 @@ bool MPMExplicit<Tdim>::solve() {
   check(ctx_, mpmx_set_mesh(ctx_, (int64_t)(coords_.size() / Tdim), coords_.data(),
                             (int64_t)(cells_.size() / NN), cells_.data(), isoparametric_ ? 1 : 0),
         "mesh");
+  if (!enode_.empty())
+    check(ctx_, mpmx_set_nodal_euler_angles(ctx_, (int64_t)enode_.size(), enode_.data(),
+                                            eang_.data()),
+          "Euler angles are not properly assigned/computed");
   if (!vnode_.empty())
     check(ctx_, mpmx_set_velocity_constraints(ctx_, (int64_t)vnode_.size(), vnode_.data(),
                                               vdir_.data(), vval_.data()),

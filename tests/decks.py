@@ -14,7 +14,7 @@ DRIVER = os.path.join(ROOT, "mpm_amd", "mpmx_run")
 def write_deck(w, d, nsteps, output_steps, constraints_as="sets", uuid="deck",
                material_json=None, stresses=None, particle_constraints=None,
                acceleration_constraints=None, math_functions=None, nodal_forces=None,
-               tractions=None, velocity_update=False):
+               tractions=None, velocity_update=False, euler_angles=None):
     """w: a Workload; d: directory. Velocity constraints go to node sets
     (``constraints_as="sets"``) or to a constraints file (``"file"``).
     particle_constraints: [(particle ids or -1, dir, velocity)], written as
@@ -26,7 +26,9 @@ def write_deck(w, d, nsteps, output_steps, constraints_as="sets", uuid="deck",
     nodal_forces: [(node ids or -1, dir, force, math function id or None)],
     written as node sets and concentrated_nodal_forces.
     tractions: [(particle ids or -1, dir, traction, math function id or
-    None)], written as particle sets and particle_surface_traction."""
+    None)], written as particle sets and particle_surface_traction.
+    euler_angles: (node ids, angles[n, dim] in radians), written to the
+    "nodal_euler_angles" file of the mesh's boundary conditions."""
     os.makedirs(d, exist_ok=True)
     dim = w.dim
     with open(os.path.join(d, "mesh.txt"), "w") as f:
@@ -54,6 +56,12 @@ def write_deck(w, d, nsteps, output_steps, constraints_as="sets", uuid="deck",
                     sets["node_sets"].append({"id": sid, "set": [int(n) for n in w.vel_nodes[sel]]})
                     vc.append({"nset_id": sid, "dir": k, "velocity": val})
             bc["velocity_constraints"] = vc
+    if euler_angles is not None:
+        with open(os.path.join(d, "euler-angles.txt"), "w") as f:
+            f.write("! node alpha beta (gamma)\n")
+            for n, a in zip(*euler_angles):
+                f.write("%d %s\n" % (n, " ".join("%.17g" % v for v in a)))
+        bc["nodal_euler_angles"] = "euler-angles.txt"
     if len(w.fric_nodes):
         fc = []
         sid = 100

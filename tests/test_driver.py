@@ -81,6 +81,20 @@ def test_dry_run_mohr_coulomb_3d_with_friction(tmp_path):
     np.testing.assert_array_equal(s["materials"][0], mat.from_json(MC3, 3))
 
 
+def test_dry_run_nodal_euler_angles(tmp_path):
+    """The mesh's "nodal_euler_angles" file (mpm_base.tcc:1041-1063): one
+    row per node, a node's first row counting (io_mesh_ascii.tcc:351-391)."""
+    w = workloads.column_2d(nx=4, ny=4)
+    base = np.nonzero(np.abs(w.coords[:, 1]) <= 1e-9)[0]
+    ang = np.tile(np.radians([15.0, 10.0]), (len(base), 1))
+    write_deck(w, str(tmp_path), nsteps=1, output_steps=1, material_json=LE,
+               euler_angles=(base, ang))
+    with open(os.path.join(str(tmp_path), "euler-angles.txt"), "a") as f:
+        f.write("%d 0.5 0.5\n" % base[0])  # a second row for a node: ignored
+    s = json.loads(run_driver(str(tmp_path), "--dry-run").stdout)
+    assert s["nodal_euler_angles"] == len(base)
+
+
 def test_deck_errors_are_reported(tmp_path):
     w = workloads.column_2d(nx=2, ny=2)
     bad = dict(LE)

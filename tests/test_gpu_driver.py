@@ -55,6 +55,31 @@ def test_driver_2d_column_matches_oracle(tmp_path):
     _compare(out, _oracle_run(w, 30), 2)
 
 
+def test_driver_nodal_euler_angles_match_oracle(tmp_path):
+    """A deck whose base nodes lie on a general boundary: the mesh's
+    "nodal_euler_angles" file tilts their frame, the velocity constraint
+    holds the local normal velocity and friction acts in that frame
+    (node.tcc:385-401, 540-583); the block slides down the tilted base."""
+    w = workloads.column_2d(nx=6, ny=10)
+    base = np.nonzero(np.abs(w.coords[:, 1]) <= 1e-9)[0].astype(np.int64)
+    keep = ~np.isin(w.vel_nodes, base)
+    w.vel_nodes = np.concatenate([w.vel_nodes[keep], base])
+    w.vel_dirs = np.concatenate([w.vel_dirs[keep], np.ones(len(base))]).astype(np.int32)
+    w.vel_values = np.concatenate([np.asarray(w.vel_values)[keep], np.zeros(len(base))])
+    w.fric_nodes = base
+    w.fric_dirs = np.ones(len(base), dtype=np.int32)
+    w.fric_sign = np.full(len(base), -1, dtype=np.int32)
+    w.fric_mu = np.full(len(base), 0.3)
+    ang = np.tile(np.radians([15.0, 10.0]), (len(base), 1))
+    d = str(tmp_path)
+    write_deck(w, d, nsteps=30, output_steps=30, material_json=LE, euler_angles=(base, ang))
+    run_driver(d, "--ordered")
+    out = read_output(output_path(d, 30, 30), 2)
+    p = _oracle_run(w, 30, extra=lambda o: o.set_nodal_euler_angles(base, ang))
+    _compare(out, p, 2)
+    assert np.max(np.abs(out["x"][:, 0] - w.setup(oracle.Oracle(2, w.scheme)).particles()["x"][:, 0])) > 0
+
+
 def test_driver_isoparametric_default_matches_oracle(tmp_path):
     """A deck without the "isoparametric" key on a mesh of skewed quads: the
     driver takes the reference's default (isoparametric, mpm_base.tcc:768-783)
