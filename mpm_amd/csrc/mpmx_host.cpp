@@ -200,6 +200,9 @@ struct mpmx_ctx {
   // ad_every steps and at its end), the per-step records of one segment, node
   // and particle adjoint buffers
   bool ad_recorded = false;
+  // the step records of the last segment were written by the recorded run
+  // itself (the backward pass need not replay it)
+  bool ad_last_recorded = false;
   int ad_every = 0, ad_nsteps = 0;
   double ad_dt = 0, ad_damping = 0;
   double ad_apex_ratio = 0.3;
@@ -1484,6 +1487,19 @@ void ad_save_slot(mpmx_ctx* c, int j) {
 // layout): mass, then momentum, force, updated velocity, acceleration, MUSL
 // second momentum and its velocity.
 int64_t ad_node_slot_doubles(const mpmx_ctx* c) { return (1 + 6 * c->dim) * c->nnodes; }
+// The step records of one segment: K + 1 particle slots and K nodal slots.
+// The recording graphs hold their addresses, so they are captured anew.
+void ad_alloc_slots(mpmx_ctx* c, int K) {
+  c->d_ad_slots.alloc((size_t)(K + 1) * ad_slot_doubles(c));
+  c->d_ad_slot_cell.alloc((size_t)(K + 1) * c->np);
+  c->d_ad_node_slots.alloc((size_t)K * ad_node_slot_doubles(c));
+  for (int k = 2; k < 4; ++k)
+    if (c->graph_exec[k]) {
+      (void)hipGraphExecDestroy(c->graph_exec[k]);
+      c->graph_exec[k] = nullptr;
+    }
+}
+
 double* ad_node_slot(mpmx_ctx* c, int j) {
   return c->d_ad_node_slots.p + (int64_t)j * ad_node_slot_doubles(c);
 }
@@ -2518,13 +2534,29 @@ int mpmx_adjoint_forward(mpmx_ctx* c, int nsteps, double dt, double damping_fact
   c->d_ad_ck.alloc((size_t)(nseg + 1) * c->ad_ck_stride);
   c->d_ad_ck_cell.alloc((size_t)(nseg + 1) * c->np);
   c->ad_step0 = c->step;
+  c->ad_last_recorded = false;
+  ad_alloc_slots(c, K);
   reset_status(c);
   for (int s = 0; s < nseg; ++s) {
+    const int ns = std::min(K, nsteps - s * K);
     ad_checkpoint(c, s, true);
-    launch_steps(c, std::min(K, nsteps - s * K), dt, damping_factor);
+    if (s < nseg - 1) {
+      launch_steps(c, ns, dt, damping_factor);
+      continue;
+    }
+    // the last segment runs recording its step records, as the backward
+    // pass's replay of a segment does (slot index from 0)
+    check_status(c);
+    reset_status(c);
+    for (int j = 0; j < ns; ++j) {
+      ad_save_slot(c, j);
+      launch_steps(c, 1, dt, damping_factor, c->d_ad_node_slots.p, ad_node_slot_doubles(c), K);
+    }
+    ad_save_slot(c, ns);
   }
   ad_checkpoint(c, nseg, true);
   check_status(c);
+  c->ad_last_recorded = true;
   c->ad_every = K;
   c->ad_nsteps = nsteps;
   c->ad_dt = dt;
@@ -2544,16 +2576,7 @@ int mpmx_adjoint_backward(mpmx_ctx* c, const double* dL_dx, const double* dL_dv,
   const int64_t np = c->np;
   const int d = c->dim;
   hipStream_t st = c->stream;
-  c->d_ad_slots.alloc((size_t)(K + 1) * ad_slot_doubles(c));
-  c->d_ad_slot_cell.alloc((size_t)(K + 1) * np);
-  // the recording graphs hold the slots' address: captured anew for this
-  // allocation
-  c->d_ad_node_slots.alloc((size_t)K * ad_node_slot_doubles(c));
-  for (int k = 2; k < 4; ++k)
-    if (c->graph_exec[k]) {
-      (void)hipGraphExecDestroy(c->graph_exec[k]);
-      c->graph_exec[k] = nullptr;
-    }
+  if (!c->ad_last_recorded) ad_alloc_slots(c, K);
   c->d_ad_nodes.alloc(ad_node_doubles(c));
   c->d_ad_parts.alloc(ad_part_doubles(c));
   c->d_ad_list.alloc(1 + 2 * (size_t)np);
@@ -2586,17 +2609,21 @@ int mpmx_adjoint_backward(mpmx_ctx* c, const double* dL_dx, const double* dL_dv,
   // then transpose the steps in reverse
   for (int s = nseg - 1; s >= 0; --s) {
     const int ns = std::min(K, c->ad_nsteps - s * K);
-    ad_checkpoint(c, s, false);
-    c->step = c->ad_step0 + (int64_t)s * K;
-    c->need_sort = true;
-    reset_status(c);
-    for (int j = 0; j < ns; ++j) {  // reset_status set the slot index to 0
-      ad_save_slot(c, j);
-      launch_steps(c, 1, c->ad_dt, c->ad_damping, c->d_ad_node_slots.p,
-                   ad_node_slot_doubles(c), K);
+    if (s == nseg - 1 && c->ad_last_recorded) {
+      c->ad_last_recorded = false;  // the replays below overwrite the records
+    } else {
+      ad_checkpoint(c, s, false);
+      c->step = c->ad_step0 + (int64_t)s * K;
+      c->need_sort = true;
+      reset_status(c);
+      for (int j = 0; j < ns; ++j) {  // reset_status set the slot index to 0
+        ad_save_slot(c, j);
+        launch_steps(c, 1, c->ad_dt, c->ad_damping, c->d_ad_node_slots.p,
+                     ad_node_slot_doubles(c), K);
+      }
+      ad_save_slot(c, ns);
+      check_status(c);
     }
-    ad_save_slot(c, ns);
-    check_status(c);
     for (int j = ns - 1; j >= 0; --j) ad_backward_step(c, j, c->ad_step0 + (int64_t)s * K + j);
   }
   HIPCHECK(hipStreamSynchronize(st));
@@ -2634,6 +2661,7 @@ int mpmx_adjoint_set_apex_ratio(mpmx_ctx* c, double ratio) {
 int mpmx_adjoint_release(mpmx_ctx* c) {
   API_BEGIN(c)
   c->ad_recorded = false;
+  c->ad_last_recorded = false;
   c->d_ad_ck.release();
   c->d_ad_ck_cell.release();
   c->d_ad_slots.release();
