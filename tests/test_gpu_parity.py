@@ -123,6 +123,21 @@ def test_fast_3d_mohr_coulomb_per_step():
     print("fast MC worst per-step field rel err over 40 steps: %.3e" % worst)
 
 
+def test_fast_3d_odd_grid_per_step():
+    """Fast mode on a 7 x 5 x 6 grid (no power-of-two dimension: cell and
+    node strides that are not shifts), each step from the oracle's state, to
+    1e-12."""
+    from mpm_amd import workloads
+    w = workloads.collapse_3d(nx=7, ny=5, nz=6, block=(3, 3, 3))
+    o = w.setup(oracle.Oracle(3, w.scheme))
+    for s in range(10):
+        g = _gpu_from(o, w, w.scheme, False)
+        g.step(1, w.dt)
+        o.step(1, w.dt)
+        _compare(g, o, exact=False, what="odd grid step %d" % s)
+        g.close()
+
+
 def _gpu_from(o, w, scheme, ordered=False):
     """A fresh GPU solver holding exactly the oracle's particle state."""
     from mpm_amd.capi import Solver
