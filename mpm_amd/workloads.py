@@ -212,6 +212,7 @@ class Batch:
         self.instances = instances
         w0 = instances[0]
         self.dim, self.scheme, self.dt = w0.dim, w0.scheme, w0.dt
+        self.nsteps, self.runnable_steps = w0.nsteps, None
         self.gravity = w0.gravity
         for w in instances:
             assert (w.dim, w.scheme, w.dt) == (self.dim, self.scheme, self.dt)
