@@ -387,6 +387,17 @@ int mpmx_adjoint_forward(mpmx_ctx* ctx, int nsteps, double dt, double damping_fa
 int mpmx_adjoint_backward(mpmx_ctx* ctx, const double* dL_dx, const double* dL_dv,
                           double* dL_dE, double* dL_drho, double* dL_dphi, double* dL_dx0,
                           double* dL_dv0);
+/* mpmx_adjoint_backward with every array in the memory of the context's
+ * device (e.g. the storage of torch CUDA tensors), same layouts and results:
+ * no host staging. Returns when the gradients are written. No counterpart
+ * in the reference (which has no derivatives). */
+int mpmx_adjoint_backward_device(mpmx_ctx* ctx, const double* dL_dx, const double* dL_dv,
+                                 double* dL_dE, double* dL_drho, double* dL_dphi,
+                                 double* dL_dx0, double* dL_dv0);
+/* Particle positions and velocities (rows of dim, the reference's particle
+ * order) copied into arrays in the memory of the context's device; either
+ * may be null. The device-side counterpart of mpmx_get_particles' x and v. */
+int mpmx_get_particles_device(mpmx_ctx* ctx, double* x, double* v);
 /* Mohr-Coulomb returns near the apex of the yield surface (cohesionless
  * material at low stress, e.g. at a free surface): the directions of the
  * yield and plastic-potential gradients there depend on a Lode angle that is

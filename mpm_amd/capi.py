@@ -8,6 +8,7 @@ is missing; there is no CPU fallback.
 """
 import ctypes as C
 import os
+import sys
 
 import numpy as np
 
@@ -39,7 +40,7 @@ EXPORTS = (
     "mpmx_get_step",
     "mpmx_synchronize", "mpmx_profile", "mpmx_kernel_times",
     "mpmx_adjoint_forward", "mpmx_adjoint_backward", "mpmx_adjoint_set_apex_ratio",
-    "mpmx_adjoint_release",
+    "mpmx_adjoint_release", "mpmx_adjoint_backward_device", "mpmx_get_particles_device",
 )
 
 _D = C.POINTER(C.c_double)
@@ -65,6 +66,13 @@ def load(path=LIB_PATH):
         return _lib
     if not os.path.exists(path):
         raise RuntimeError("mpmx HIP library not built: %s (run __graft_entry__.build())" % path)
+    # torch carries its own HIP runtime: when it is in the process, its
+    # device must be initialized before the library's, or torch finds no
+    # device afterwards (the library then shares torch's runtime, and torch
+    # tensors' device memory can be passed to the *_device entry points)
+    torch = sys.modules.get("torch")
+    if torch is not None and torch.cuda.is_available():
+        torch.cuda.init()
     lib = C.CDLL(path)
     V = C.c_void_p
     sig = {
@@ -110,6 +118,8 @@ def load(path=LIB_PATH):
         "mpmx_kernel_times": [V, C.c_int, C.c_char_p, _D, _I64, _I32],
         "mpmx_adjoint_forward": [V, C.c_int, C.c_double, C.c_double, C.c_int],
         "mpmx_adjoint_backward": [V, _D, _D, _D, _D, _D, _D, _D],
+        "mpmx_adjoint_backward_device": [V, V, V, V, V, V, V, V],
+        "mpmx_get_particles_device": [V, V, V],
         "mpmx_adjoint_set_apex_ratio": [V, C.c_double],
         "mpmx_adjoint_release": [V],
     }
@@ -453,6 +463,22 @@ class Solver:
             self.ctx, _p(gx), _p(gv), _p(out["E"]), _p(out["rho"]), _p(out["phi"]),
             _p(out["x0"]), _p(out["v0"])))
         return out
+
+    def adjoint_backward_device(self, dL_dx=None, dL_dv=None, outputs=None):
+        """adjoint_backward with device arrays: dL_dx, dL_dv and the entries
+        of `outputs` (keys E, rho, phi, x0, v0) are device addresses (ints,
+        e.g. torch CUDA tensors' data_ptr(), float64, rows of dim), any of
+        them None. The library's stream is synchronized on return; callers
+        on another stream synchronize theirs before the call."""
+        o = outputs or {}
+        self._check(self.lib.mpmx_adjoint_backward_device(
+            self.ctx, dL_dx, dL_dv, o.get("E"), o.get("rho"), o.get("phi"), o.get("x0"),
+            o.get("v0")))
+
+    def particles_device(self, x=None, v=None):
+        """Positions and velocities (rows of dim) into device arrays given by
+        address (ints), either None."""
+        self._check(self.lib.mpmx_get_particles_device(self.ctx, x, v))
 
     def adjoint_set_apex_ratio(self, ratio):
         """Mohr-Coulomb returns near the apex take constant flow directions in

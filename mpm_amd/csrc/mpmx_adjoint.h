@@ -87,6 +87,14 @@ struct AdjArgs {
 template <int D>
 void launch_ad_save(const Particles& P, const SlotW& s, hipStream_t st);
 void launch_ad_fill(double* a, int64_t n, double v, hipStream_t st);
+// Row-major n x comps <-> component-major [comps][n] copies on the device;
+// to rows, each value times `scale` and, with per_mass, times the
+// particle's mass over its material's density (the density gradient
+// through m = V rho).
+void launch_rows_to_soa(const double* rows, double* soa, int64_t n, int comps, hipStream_t st);
+void launch_soa_to_rows(const double* soa, double* rows, int64_t n, int comps, double scale,
+                        const double* mass, const int* mat, const Material* mats,
+                        hipStream_t st);
 // One backward step: transpose step t from the recorded particle states s0
 // (step t) and s1 (step t+1) and the step's recorded nodal quantities.
 template <int D>

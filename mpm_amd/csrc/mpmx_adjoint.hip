@@ -984,6 +984,34 @@ void launch_ad_fill(double* a, int64_t n, double v, hipStream_t st) {
   if (n > 0) hipLaunchKernelGGL(k_ad_fill, ad_grid(n), dim3(kAdBlock), 0, st, a, n, v);
 }
 
+namespace {
+__global__ void k_rows_to_soa(const double* rows, double* soa, int64_t n, int comps) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int k = 0; k < comps; ++k) soa[k * n + i] = rows[i * comps + k];
+}
+__global__ void k_soa_to_rows(const double* soa, double* rows, int64_t n, int comps,
+                              double scale, const double* mass, const int* mat,
+                              const Material* mats) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double per = mass ? mass[i] / mats[mat[i]].density : 1.0;
+  for (int k = 0; k < comps; ++k) rows[i * comps + k] = soa[k * n + i] * scale * per;
+}
+}  // namespace
+
+void launch_rows_to_soa(const double* rows, double* soa, int64_t n, int comps, hipStream_t st) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_rows_to_soa, ad_grid(n), dim3(kAdBlock), 0, st, rows, soa, n, comps);
+}
+void launch_soa_to_rows(const double* soa, double* rows, int64_t n, int comps, double scale,
+                        const double* mass, const int* mat, const Material* mats,
+                        hipStream_t st) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_soa_to_rows, ad_grid(n), dim3(kAdBlock), 0, st, soa, rows, n, comps,
+                       scale, mass, mat, mats);
+}
+
 template <int D, int SCH>
 static void backward_scheme(const Mesh& M, const Particles& P, const Nodes& Nd,
                             const Material* mats, const Slot& s0, const Slot& s1,

@@ -2566,9 +2566,14 @@ int mpmx_adjoint_forward(mpmx_ctx* c, int nsteps, double dt, double damping_fact
   API_END(c)
 }
 
-int mpmx_adjoint_backward(mpmx_ctx* c, const double* dL_dx, const double* dL_dv, double* dL_dE,
-                          double* dL_drho, double* dL_dphi, double* dL_dx0, double* dL_dv0) {
-  API_BEGIN(c)
+}  // extern "C"
+
+namespace {
+// The backward pass; `dev`: the loss gradients come in and the parameter
+// gradients go out as arrays on the context's device, else on the host.
+void adjoint_backward(mpmx_ctx* c, bool dev, const double* dL_dx, const double* dL_dv,
+                      double* dL_dE, double* dL_drho, double* dL_dphi, double* dL_dx0,
+                      double* dL_dv0) {
   if (!c->ad_recorded || c->ad_np != c->np)
     throw ApiError(MPMX_E_STATE, "no recorded run (mpmx_adjoint_forward) for these particles");
   const int K = c->ad_every;
@@ -2583,20 +2588,31 @@ int mpmx_adjoint_backward(mpmx_ctx* c, const double* dL_dx, const double* dL_dv,
   c->d_ad_list_buf.alloc(6 * (size_t)np);
   launch_ad_fill(c->d_ad_parts.p, ad_part_doubles(c), 0., st);
   const AdjParticles L = ad_parts(c);
-  auto put_rows = [&](double* dev, int comps, const double* rows) {
+  auto put_rows = [&](double* soa, int comps, const double* rows) {
+    if (dev) {
+      launch_rows_to_soa(rows, soa, np, comps, st);
+      HIPCHECK(hipGetLastError());
+      return;
+    }
     std::vector<double> col(np);
     for (int k = 0; k < comps; ++k) {
       for (int64_t i = 0; i < np; ++i) col[i] = rows[i * comps + k];
-      HIPCHECK(hipMemcpyAsync(dev + k * np, col.data(), np * sizeof(double),
+      HIPCHECK(hipMemcpyAsync(soa + k * np, col.data(), np * sizeof(double),
                               hipMemcpyHostToDevice, st));
       HIPCHECK(hipStreamSynchronize(st));
     }
   };
-  auto get_rows = [&](const double* dev, int comps, double* rows, double scale,
+  auto get_rows = [&](const double* soa, int comps, double* rows, double scale,
                       const std::vector<double>* per) {
+    if (dev) {
+      launch_soa_to_rows(soa, rows, np, comps, scale, per ? c->d_mass.p : nullptr,
+                         per ? c->d_mat.p : nullptr, c->d_mats.p, st);
+      HIPCHECK(hipGetLastError());
+      return;
+    }
     std::vector<double> col(np);
     for (int k = 0; k < comps; ++k) {
-      HIPCHECK(hipMemcpyAsync(col.data(), dev + k * np, np * sizeof(double),
+      HIPCHECK(hipMemcpyAsync(col.data(), soa + k * np, np * sizeof(double),
                               hipMemcpyDeviceToHost, st));
       HIPCHECK(hipStreamSynchronize(st));
       for (int64_t i = 0; i < np; ++i)
@@ -2629,8 +2645,8 @@ int mpmx_adjoint_backward(mpmx_ctx* c, const double* dL_dx, const double* dL_dv,
   HIPCHECK(hipStreamSynchronize(st));
   // parameter gradients: E per particle; phi in degrees (the parameter
   // vector's unit); density through the mass m = V rho of the material
-  std::vector<double> m_over_rho(np);
-  {
+  std::vector<double> m_over_rho(dev ? 0 : np);  // device: taken from mass and material
+  if (!dev) {
     std::vector<double> mass(np);
     std::vector<int> mat(np);
     HIPCHECK(hipMemcpyAsync(mass.data(), c->d_mass.p, np * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -2648,8 +2664,36 @@ int mpmx_adjoint_backward(mpmx_ctx* c, const double* dL_dx, const double* dL_dv,
   c->step = c->ad_step0 + c->ad_nsteps;
   c->need_sort = true;
   HIPCHECK(hipStreamSynchronize(st));
+}
+}  // namespace
+
+extern "C" {
+
+int mpmx_adjoint_backward(mpmx_ctx* c, const double* dL_dx, const double* dL_dv, double* dL_dE,
+                          double* dL_drho, double* dL_dphi, double* dL_dx0, double* dL_dv0) {
+  API_BEGIN(c)
+  adjoint_backward(c, false, dL_dx, dL_dv, dL_dE, dL_drho, dL_dphi, dL_dx0, dL_dv0);
   API_END(c)
 }
+
+int mpmx_adjoint_backward_device(mpmx_ctx* c, const double* dL_dx, const double* dL_dv,
+                                 double* dL_dE, double* dL_drho, double* dL_dphi,
+                                 double* dL_dx0, double* dL_dv0) {
+  API_BEGIN(c)
+  adjoint_backward(c, true, dL_dx, dL_dv, dL_dE, dL_drho, dL_dphi, dL_dx0, dL_dv0);
+  API_END(c)
+}
+
+int mpmx_get_particles_device(mpmx_ctx* c, double* x, double* v) {
+  API_BEGIN(c)
+  if (c->np == 0) return MPMX_OK;
+  if (x) launch_soa_to_rows(c->d_x.p, x, c->np, c->dim, 1.0, nullptr, nullptr, nullptr, c->stream);
+  if (v) launch_soa_to_rows(c->d_v.p, v, c->np, c->dim, 1.0, nullptr, nullptr, nullptr, c->stream);
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  API_END(c)
+}
+
 
 int mpmx_adjoint_set_apex_ratio(mpmx_ctx* c, double ratio) {
   API_BEGIN(c)

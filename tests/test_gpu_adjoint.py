@@ -13,6 +13,11 @@ return-mapping iteration-count or cell changes between the two runs) and
 above rounding noise. The bar is 1e-6 relative."""
 import numpy as np
 import pytest
+# torch in the process before the first solver: mpm_amd.capi.load then
+# initializes torch's device ahead of the library's, which the torch
+# binding's CUDA-tensor test needs (test modules are imported before any
+# test runs)
+import torch  # noqa: F401
 
 from adjoint_problem import Problem
 
@@ -160,7 +165,6 @@ def test_adjoint_unsupported_options():
 def test_torch_autograd_function():
     """mpm_amd.adjoint.trajectory: torch.autograd through the C-ABI adjoint
     gives the gradients of mpmx_adjoint_backward for the same loss."""
-    import torch
     from mpm_amd.adjoint import trajectory
     P = Problem(scheme="usl", material="mix", geostatic=True)
 
@@ -179,6 +183,24 @@ def test_torch_autograd_function():
     for name, t in (("E", E), ("rho", rho), ("phi", phi)):
         ref = g[name]
         assert np.max(np.abs(t.grad.numpy() - ref)) <= 1e-9 * np.max(np.abs(ref)), name
+    # CUDA tensors: final state and gradients stay on the device
+    # (mpmx_get_particles_device, mpmx_adjoint_backward_device); another
+    # fast-mode run, so equal to the atomic-order rounding
+    dev = torch.device("cuda:0")
+    Ed, rd, pd = (torch.tensor(a, device=dev, requires_grad=True) for a in (P.E, P.rho, P.phi))
+    xd, vd = trajectory(setup, Ed, rd, pd, P.nsteps, P.dt, checkpoint_every=7, dim=2,
+                        scheme="usl")
+    assert xd.device.type == "cuda" and vd.device.type == "cuda"
+    for a, b in ((xd, x), (vd, v)):
+        b = b.detach().numpy()
+        assert np.max(np.abs(a.detach().cpu().numpy() - b)) <= 1e-12 * np.max(np.abs(b))
+    lossd = ((torch.from_numpy(P.wx).to(dev) * xd).sum() +
+             (torch.from_numpy(P.wv).to(dev) * vd).sum())
+    lossd.backward()
+    for td, t in ((Ed, E), (rd, rho), (pd, phi)):
+        assert td.grad.device.type == "cuda"
+        ref = t.grad.numpy()
+        assert np.max(np.abs(td.grad.cpu().numpy() - ref)) <= 1e-9 * np.max(np.abs(ref))
 
 
 def test_apex_ratio_bounds_long_collapse_gradients():
