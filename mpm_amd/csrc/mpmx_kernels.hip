@@ -1628,13 +1628,32 @@ __global__ __launch_bounds__(64) void k_gather(Mesh M, Particles P, Nodes Nd, St
         is[q] = ls;
       }
     }
-    // rank among the node's particles (ids are distinct)
+    // rank among the node's particles: each cell list holds ascending ids
+    // (stable sort of id-ordered input) and the lists are disjoint, so a
+    // particle's rank is the sum over the lists of how many of their ids are
+    // below its own (its own list: its position there), each count by a
+    // binary search in LDS with the list's (wave-uniform) power-of-two steps
+    __shared__ int ids[kGatherMax];
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+      if (lane + 64 * q < T) ids[lane + 64 * q] = ps[q];
+    __syncthreads();
+    const int nq = T > 64 ? 2 : 1;
     int rk[2] = {0, 0};
-    for (int t = 0; t < T; ++t) {
-      const int pt = t < 64 ? __builtin_amdgcn_readlane(ps[0], t)
-                            : __builtin_amdgcn_readlane(ps[1], t - 64);
-      rk[0] += pt < ps[0] ? 1 : 0;
-      rk[1] += pt < ps[1] ? 1 : 0;
+#pragma unroll
+    for (int j = 0; j < NN; ++j) {
+      const int len = (j + 1 < NN ? off[j + 1] : T) - off[j];
+      if (len == 0) continue;
+      const int top = 1 << (31 - __builtin_clz(len));  // largest power of two <= len
+#pragma unroll 1
+      for (int q = 0; q < nq; ++q) {
+        const int a = q ? ps[1] : ps[0];
+        int below = 0;
+        for (int step = top; step > 0; step >>= 1)
+          if (below + step <= len && ids[off[j] + below + step - 1] < a) below += step;
+        if (q) rk[1] += below;
+        else rk[0] += below;
+      }
     }
 #pragma unroll 1
     for (int q = 0; q < 2; ++q) {  // one slot at a time (register pressure)
