@@ -399,6 +399,9 @@ def main():
     ap.add_argument("--ordered", action="store_true", help="particle-order nodal sums")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of timed CPU work")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch the timed steps' kernels one by one instead of replaying "
+                         "the captured step (kernel traces without graph launches)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default scatter chunk and sort interval")
     ap.add_argument("--chunk", type=int, default=None,
@@ -436,7 +439,7 @@ def main():
     if args.steps is None:
         args.steps = 200
     t0 = time.perf_counter()
-    s = w.setup(Solver(w.dim, w.scheme, ordered=args.ordered))
+    s = w.setup(Solver(w.dim, w.scheme, ordered=args.ordered, graph=not args.no_graph))
     setup_s = time.perf_counter() - t0
     npart = s.nparticles
     # scatter chunk and sort interval from timed trial steps (state restored)

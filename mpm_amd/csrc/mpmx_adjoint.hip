@@ -80,14 +80,14 @@ struct Shape {
 template <int D>
 MPMX_DEV bool load_shape(const Mesh& M, const Slot& s, int64_t np, int64_t p, Shape<D>& S) {
   constexpr int NN = Elem<D>::NN;
-  const int c = s.cell[p];
+  const int c = stream_ld(&s.cell[p]);
   if (c < 0 || (int64_t)c >= M.ncells) return false;
   S.c = c;
 #pragma unroll
   for (int i = 0; i < NN; ++i) S.node[i] = M.cell_nodes[(int64_t)c * NN + i];
 #pragma unroll
   for (int k = 0; k < D; ++k) {
-    S.xi[k] = s.xi[k * np + p];
+    S.xi[k] = stream_ld(&s.xi[k * np + p]);
     const bool clamped = S.xi[k] <= -1. + kDblEps || S.xi[k] >= 1. - kDblEps;
     S.dxi_dx[k] = clamped ? 0. : 2. / M.len[k * M.ncells + c];
   }
@@ -553,11 +553,11 @@ MPMX_DEV void gather_vjp(const Mesh& M, const Shape<D>& S, const double* lmass,
 
 template <int D>
 MPMX_DEV void load_vec(const double* a, int64_t np, int64_t p, int n, double* out) {
-  for (int k = 0; k < n; ++k) out[k] = a[k * np + p];
+  for (int k = 0; k < n; ++k) out[k] = stream_ld(&a[k * np + p]);
 }
 template <int D>
 MPMX_DEV void store_vec(double* a, int64_t np, int64_t p, int n, const double* in) {
-  for (int k = 0; k < n; ++k) a[k * np + p] = in[k];
+  for (int k = 0; k < n; ++k) stream_st(&a[k * np + p], in[k]);
 }
 
 // ------------------------------- kernels -------------------------------------
@@ -569,15 +569,15 @@ __global__ __launch_bounds__(kAdBlock) void k_ad_save(Particles P, SlotW s) {
   if (p >= np) return;
 #pragma unroll
   for (int k = 0; k < D; ++k) {
-    s.v[k * np + p] = P.v[k * np + p];
-    s.xi[k * np + p] = P.xi[k * np + p];
+    stream_st(&s.v[k * np + p], stream_ld(&P.v[k * np + p]));
+    stream_st(&s.xi[k * np + p], P.xi[k * np + p]);
   }
 #pragma unroll
-  for (int k = 0; k < 6; ++k) s.stress[k * np + p] = P.stress[k * np + p];
-  s.vol[p] = P.vol[p];
-  s.rho[p] = P.rho[p];
-  if (s.yield) s.yield[p] = P.sv[SV_YIELD * np + p];
-  s.cell[p] = P.cell[p];
+  for (int k = 0; k < 6; ++k) stream_st(&s.stress[k * np + p], stream_ld(&P.stress[k * np + p]));
+  stream_st(&s.vol[p], stream_ld(&P.vol[p]));
+  stream_st(&s.rho[p], stream_ld(&P.rho[p]));
+  if (s.yield) stream_st(&s.yield[p], stream_ld(&P.sv[SV_YIELD * np + p]));
+  stream_st(&s.cell[p], P.cell[p]);
 }
 
 __global__ void k_ad_fill(double* a, int64_t n, double v) {
@@ -755,19 +755,19 @@ __global__ __launch_bounds__(kAdBlock) MPMX_AD_ATTR void k_ad_b1(Mesh M, Particl
 #pragma unroll
     for (int k = 0; k < D; ++k) lxi[k] = 0.;
     if (SCH == USL || SCH == MUSL) {
-      const Material mat = mats[P.mat[p]];
+      const Material mat = mats[stream_ld(&P.mat[p])];
       double s_in[6], s_out[6], ls_out[6], ls_in[6];
       load_vec<D>(s0.stress, np, p, 6, s_in);
       load_vec<D>(s1.stress, np, p, 6, s_out);
       load_vec<D>(L.ls, np, p, 6, ls_out);
-      double lV_in = 0., gE = L.gE[p];
-      const double ynew = s1.yield ? s1.yield[p] : 0.;
+      double lV_in = 0., gE = stream_ld(&L.gE[p]);
+      const double ynew = s1.yield ? stream_ld(&s1.yield[p]) : 0.;
       const bool deferred = stress_vjp<D>(M, mat, p, S, SCH == USL ? A.vp : A.vpp, a.dt,
                                           a.apex_ratio, s_in, s0.vol[p], ynew, s_out, ls_out,
-                                          L.lvol[p], ls_in, lV_in, gE, acc0, lxi, Q);
+                                          stream_ld(&L.lvol[p]), ls_in, lV_in, gE, acc0, lxi, Q);
       if (!deferred) store_vec<D>(L.ls, np, p, 6, ls_in);
-      L.lvol[p] = lV_in;
-      L.gE[p] = gE;
+      stream_st(&L.lvol[p], lV_in);
+      stream_st(&L.gE[p], gE);
     }
     if (SCH == USL || SCH == USF) {
       double lx[D], lv[D];
@@ -815,30 +815,30 @@ __global__ __launch_bounds__(kAdBlock) MPMX_AD_ATTR void k_ad_b3(Mesh M, Particl
       load_vec<D>(s0.v, np, p, D, v0);
       load_vec<D>(s0.stress, np, p, 6, st);
       load_vec<D>(L.ls, np, p, 6, ls);
-      double lV = L.lvol[p];
+      double lV = stream_ld(&L.lvol[p]);
       gather_vjp<D>(M, S, A.lm, A.lP, A.lf, mp, v0, a.g, st, s0.vol[p], lm_p, lv, ls, lV, lxi);
       store_vec<D>(L.ls, np, p, 6, ls);
-      L.lvol[p] = lV;
+      stream_st(&L.lvol[p], lV);
     } else if (SCH == USF) {
       // forces of the updated stress and volume
       double st1[6], ls1[6];
       load_vec<D>(s1.stress, np, p, 6, st1);
       load_vec<D>(L.ls, np, p, 6, ls1);
-      double lV1 = L.lvol[p];
+      double lV1 = stream_ld(&L.lvol[p]);
       gather_vjp<D>(M, S, nullptr, nullptr, A.lf, mp, nullptr, a.g, st1, s1.vol[p], lm_p, lv,
                     ls1, lV1, lxi);
       // stress update from the pre-update nodal velocity
-      const Material mat = mats[P.mat[p]];
+      const Material mat = mats[stream_ld(&P.mat[p])];
       double s_in[6], ls_in[6];
       load_vec<D>(s0.stress, np, p, 6, s_in);
-      double lV_in = 0., gE = L.gE[p];
-      const double ynew = s1.yield ? s1.yield[p] : 0.;
+      double lV_in = 0., gE = stream_ld(&L.gE[p]);
+      const double ynew = s1.yield ? stream_ld(&s1.yield[p]) : 0.;
       const bool deferred = stress_vjp<D>(M, mat, p, S, A.v, a.dt, a.apex_ratio, s_in,
                                           s0.vol[p], ynew, st1, ls1, lV1, ls_in, lV_in, gE, acc0,
                                           lxi, Q);
       store_vec<D>(L.ls, np, p, 6, deferred ? ls1 : ls_in);
-      L.lvol[p] = lV_in;
-      L.gE[p] = gE;
+      stream_st(&L.lvol[p], lV_in);
+      stream_st(&L.gE[p], gE);
     } else {
       // second P2G with the updated particle velocity, then G2P
       double v1[D], dummy_s[6] = {0, 0, 0, 0, 0, 0}, dummy_ls[6] = {0, 0, 0, 0, 0, 0};
@@ -856,8 +856,8 @@ __global__ __launch_bounds__(kAdBlock) MPMX_AD_ATTR void k_ad_b3(Mesh M, Particl
     if (SCH == USL) {
 #pragma unroll
       for (int k = 0; k < D; ++k) {
-        L.lx[k * np + p] += L.lxq[k * np + p];
-        L.lxq[k * np + p] = 0.;
+        L.lx[k * np + p] += stream_ld(&L.lxq[k * np + p]);
+        stream_st(&L.lxq[k * np + p], 0.);
       }
     }
   }
@@ -888,18 +888,18 @@ __global__ __launch_bounds__(kAdBlock) void k_ad_b5(Mesh M, Particles P, Slot s0
   load_vec<D>(s0.v, np, p, D, v0);
   load_vec<D>(s0.stress, np, p, 6, st);
   load_vec<D>(L.ls, np, p, 6, ls);
-  double lV = L.lvol[p];
+  double lV = stream_ld(&L.lvol[p]);
   gather_vjp<D>(M, S, A.lm, A.lP, SCH == MUSL ? A.lf : nullptr, mp, v0, a.g, st, s0.vol[p],
                 lm_p, lv, ls, lV, lxi);
   store_vec<D>(L.ls, np, p, 6, ls);
-  L.lvol[p] = lV;
+  stream_st(&L.lvol[p], lV);
   store_vec<D>(L.lv, np, p, D, lv);
   L.gm[p] += lm_p;
   add_lxq<D>(L, np, p, S, lxi);
 #pragma unroll
   for (int k = 0; k < D; ++k) {
-    L.lx[k * np + p] += L.lxq[k * np + p];
-    L.lxq[k * np + p] = 0.;
+    L.lx[k * np + p] += stream_ld(&L.lxq[k * np + p]);
+    stream_st(&L.lxq[k * np + p], 0.);
   }
 }
 
@@ -917,23 +917,23 @@ __global__ __launch_bounds__(kAdMcBlock) MPMX_AD_ATTR void k_ad_mc(Particles P,
   if (slot >= *Q.count) return;
   const int64_t np = P.np;
   const int p = Q.ids[slot];
-  const Material m = mats[P.mat[p]];
+  const Material m = mats[stream_ld(&P.mat[p])];
   double s_in[6], de[6], ls_out[6], ls_in[6], lde[6];
 #pragma unroll
   for (int k = 0; k < 6; ++k) {
-    s_in[k] = s0.stress[k * np + p];
+    s_in[k] = stream_ld(&s0.stress[k * np + p]);
     de[k] = Q.buf[k * Q.cap + slot];
-    ls_out[k] = L.ls[k * np + p];
+    ls_out[k] = stream_ld(&L.ls[k * np + p]);
     ls_in[k] = lde[k] = 0.;
   }
-  const double phi = P.sv[SV_PHI * np + p], psi = P.sv[SV_PSI * np + p];
-  const double coh = P.sv[SV_COH * np + p], tcut = P.sv[SV_TCUT * np + p];
+  const double phi = stream_ld(&P.sv[SV_PHI * np + p]), psi = stream_ld(&P.sv[SV_PSI * np + p]);
+  const double coh = stream_ld(&P.sv[SV_COH * np + p]), tcut = stream_ld(&P.sv[SV_TCUT * np + p]);
   double gE = 0., gphi = 0.;
   mc_vjp<D, MPMX_AD_TANGENTS>(m, phi, psi, coh, tcut, s_in, de, ls_out, ls_in, lde, gE, gphi,
                               Q.apex[slot] != 0);
 #pragma unroll
   for (int k = 0; k < 6; ++k) {
-    L.ls[k * np + p] = ls_in[k];
+    stream_st(&L.ls[k * np + p], ls_in[k]);
     Q.buf[k * Q.cap + slot] = lde[k];
   }
   L.gE[p] += gE;

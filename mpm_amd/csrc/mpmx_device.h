@@ -16,6 +16,24 @@ namespace mpmx {
 
 static constexpr double kDblEps = 2.220446049250313e-16;  // DBL_EPSILON
 
+// Particle state is streamed: a kernel reads and writes each particle's
+// position, displacement, velocity, stress, strain, F, volume, density,
+// state variables and material once, so these accesses carry the
+// nontemporal hint and leave the L2 to the node and cell data that
+// neighbouring particles gather again (collapse step -2 %). Local
+// coordinates, cell ids and masses keep the default caching, which
+// measured faster for the scatter kernels that read them through the
+// cell-sorted permutation; so do the ordered-mode gathers, which read each
+// particle once per node of its cell.
+template <class T>
+MPMX_DEV T stream_ld(const T* a) {
+  return __builtin_nontemporal_load(a);
+}
+template <class T>
+MPMX_DEV void stream_st(T* a, T v) {
+  __builtin_nontemporal_store(v, a);
+}
+
 template <int D> struct Elem;
 template <> struct Elem<2> { static constexpr int NN = 4; };
 template <> struct Elem<3> { static constexpr int NN = 8; };

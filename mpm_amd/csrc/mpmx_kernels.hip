@@ -95,14 +95,15 @@ MPMX_DEV void kinematics(const Mesh& M, const Particles& P, int p, int c, const 
   // placed after a store waits for the round trip of the loads before it.
   double eps[6];
 #pragma unroll
-  for (int k = 0; k < 6; ++k) eps[k] = (D == 2 && (k == 2 || k > 3)) ? 0. : P.strain[k * np + p];
-  double vol = P.vol[p];
-  const double rho0 = P.rho[p];
+  for (int k = 0; k < 6; ++k)
+    eps[k] = (D == 2 && (k == 2 || k > 3)) ? 0. : stream_ld(&P.strain[k * np + p]);
+  double vol = stream_ld(&P.vol[p]);
+  const double rho0 = stream_ld(&P.rho[p]);
   double Fr[D][3];
 #pragma unroll
   for (int a = 0; a < D; ++a)
 #pragma unroll
-    for (int j = 0; j < 3; ++j) Fr[a][j] = P.F[(a * 3 + j) * np + p];
+    for (int j = 0; j < 3; ++j) Fr[a][j] = stream_ld(&P.F[(a * 3 + j) * np + p]);
   // update_deformation_gradient_increment (particle.tcc:1578-1651)
   double R[D][D];
 #pragma unroll
@@ -157,12 +158,12 @@ MPMX_DEV void kinematics(const Mesh& M, const Particles& P, int p, int c, const 
 #pragma unroll
   for (int k = 0; k < 6; ++k) ds[k] = sr[k] * dt;
   if constexpr (D == 2) {
-    P.strain[0 * np + p] = eps[0] + ds[0];
-    P.strain[1 * np + p] = eps[1] + ds[1];
-    P.strain[3 * np + p] = eps[3] + ds[3];
+    stream_st(&P.strain[0 * np + p], eps[0] + ds[0]);
+    stream_st(&P.strain[1 * np + p], eps[1] + ds[1]);
+    stream_st(&P.strain[3 * np + p], eps[3] + ds[3]);
   } else {
 #pragma unroll
-    for (int k = 0; k < 6; ++k) P.strain[k * np + p] = eps[k] + ds[k];
+    for (int k = 0; k < 6; ++k) stream_st(&P.strain[k * np + p], eps[k] + ds[k]);
   }
   double svol = sc[0];
 #pragma unroll
@@ -171,8 +172,8 @@ MPMX_DEV void kinematics(const Mesh& M, const Particles& P, int p, int c, const 
   // update_volume (particle.tcc:598-606)
   vol *= (1. + dvol);
   const double rho = rho0 / (1. + dvol);
-  P.vol[p] = vol;
-  P.rho[p] = rho;
+  stream_st(&P.vol[p], vol);
+  stream_st(&P.rho[p], rho);
   *vol_out = vol;
   *rho_out = rho;
   // update_deformation_gradient (particle.tcc:1654-1659): rows 0..D-1 of
@@ -185,7 +186,7 @@ MPMX_DEV void kinematics(const Mesh& M, const Particles& P, int p, int c, const 
         double t = R[a][0] * Fr[0][j];
 #pragma unroll
         for (int b = 1; b < D; ++b) t = t + R[a][b] * Fr[b][j];
-        P.F[(a * 3 + j) * np + p] = t;
+        stream_st(&P.F[(a * 3 + j) * np + p], t);
       }
   }
   if constexpr (ROT) {  // stored rather than kept live across the stress update
@@ -215,18 +216,18 @@ MPMX_DEV void kinematics(const Mesh& M, const Particles& P, int p, int c, const 
 MPMX_DEV void store_sv(const Particles& P, const Material& m, int64_t p, const double* sv) {
   const int64_t np = P.np;
   const double y = sv[SV_YIELD];
-  P.sv[SV_YIELD * np + p] = y;
+  stream_st(&P.sv[SV_YIELD * np + p], y);
   if (y != 3.) {
-    P.sv[SV_EPS * np + p] = sv[SV_EPS];
-    P.sv[SV_RHO * np + p] = sv[SV_RHO];
-    P.sv[SV_THETA * np + p] = sv[SV_THETA];
+    stream_st(&P.sv[SV_EPS * np + p], sv[SV_EPS]);
+    stream_st(&P.sv[SV_RHO * np + p], sv[SV_RHO]);
+    stream_st(&P.sv[SV_THETA * np + p], sv[SV_THETA]);
   }
-  if (y != 0.) P.sv[SV_PDS * np + p] = sv[SV_PDS];
+  if (y != 0.) stream_st(&P.sv[SV_PDS * np + p], sv[SV_PDS]);
   if (m.softening) {
-    P.sv[SV_PHI * np + p] = sv[SV_PHI];
-    P.sv[SV_PSI * np + p] = sv[SV_PSI];
-    P.sv[SV_COH * np + p] = sv[SV_COH];
-    P.sv[SV_TCUT * np + p] = sv[SV_TCUT];
+    stream_st(&P.sv[SV_PHI * np + p], sv[SV_PHI]);
+    stream_st(&P.sv[SV_PSI * np + p], sv[SV_PSI]);
+    stream_st(&P.sv[SV_COH * np + p], sv[SV_COH]);
+    stream_st(&P.sv[SV_TCUT * np + p], sv[SV_TCUT]);
   }
 }
 
@@ -241,12 +242,12 @@ MPMX_DEV void load_sv(const Particles& P, const Material& m, int64_t p, double* 
     sv[SV_COH] = m.sv0[SV_COH];
     sv[SV_TCUT] = m.sv0[SV_TCUT];
   } else {
-    sv[SV_PHI] = P.sv[SV_PHI * np + p];
-    sv[SV_PSI] = P.sv[SV_PSI * np + p];
-    sv[SV_COH] = P.sv[SV_COH * np + p];
-    sv[SV_TCUT] = P.sv[SV_TCUT * np + p];
+    sv[SV_PHI] = stream_ld(&P.sv[SV_PHI * np + p]);
+    sv[SV_PSI] = stream_ld(&P.sv[SV_PSI * np + p]);
+    sv[SV_COH] = stream_ld(&P.sv[SV_COH * np + p]);
+    sv[SV_TCUT] = stream_ld(&P.sv[SV_TCUT * np + p]);
   }
-  sv[SV_PDS] = P.sv[SV_PDS * np + p];
+  sv[SV_PDS] = stream_ld(&P.sv[SV_PDS * np + p]);
   sv[SV_YIELD] = sv[SV_EPS] = sv[SV_RHO] = sv[SV_THETA] = 0.;
 }
 
@@ -302,8 +303,8 @@ MPMX_DEV void stress_strain(const Mesh& M, const Particles& P, const Material* m
   kinematics<D, ROT>(M, P, p, c, dndx, nv, dt, ds, &rho, vol_out);
   // one grouped round trip after the kinematic stores
 #pragma unroll
-  for (int k = 0; k < 6; ++k) s[k] = P.stress[k * np + p];
-  const Material& m = mats[P.mat[p]];
+  for (int k = 0; k < 6; ++k) s[k] = stream_ld(&P.stress[k * np + p]);
+  const Material& m = mats[stream_ld(&P.mat[p])];
   double sv[NSV];
   if (m.type != LE) load_sv(P, m, p, sv);
   // an objective stress rate: the rotated trial stress (ROT builds only)
@@ -325,7 +326,7 @@ MPMX_DEV void stress_strain(const Mesh& M, const Particles& P, const Material* m
       for (int a = 0; a < 3; ++a)
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
-          F0[a][j] = a < D ? P.rotF[(a * 3 + j) * np + p] : P.F[(a * 3 + j) * np + p];
+          F0[a][j] = a < D ? P.rotF[(a * 3 + j) * np + p] : stream_ld(&P.F[(a * 3 + j) * np + p]);
           G[a][j] = (a < D && j < D) ? P.rotF[(D * 3 + a * D + j) * np + p] : (a == j ? 1. : 0.);
         }
       green_naghdi_rotations(F0, G, L, LL);
@@ -348,12 +349,12 @@ MPMX_DEV void stress_strain(const Mesh& M, const Particles& P, const Material* m
     if (!mc_begin(m, s, ds, rho, sv, trial, T, rate != RATE_NONE ? rtrial : nullptr)) {
       // the yield state (and softened strength parameters) now, the rest of
       // the state variables and the stress in k_plastic
-      P.sv[SV_YIELD * np + p] = sv[SV_YIELD];
+      stream_st(&P.sv[SV_YIELD * np + p], sv[SV_YIELD]);
       if (m.softening) {
-        P.sv[SV_PHI * np + p] = sv[SV_PHI];
-        P.sv[SV_PSI * np + p] = sv[SV_PSI];
-        P.sv[SV_COH * np + p] = sv[SV_COH];
-        P.sv[SV_TCUT * np + p] = sv[SV_TCUT];
+        stream_st(&P.sv[SV_PHI * np + p], sv[SV_PHI]);
+        stream_st(&P.sv[SV_PSI * np + p], sv[SV_PSI]);
+        stream_st(&P.sv[SV_COH * np + p], sv[SV_COH]);
+        stream_st(&P.sv[SV_TCUT * np + p], sv[SV_TCUT]);
       }
       defer_plastic(P, S, p, ds, T, trial);
       return;
@@ -364,7 +365,7 @@ MPMX_DEV void stress_strain(const Mesh& M, const Particles& P, const Material* m
     store_sv(P, m, p, sv);
   }
 #pragma unroll
-  for (int k = 0; k < 6; ++k) P.stress[k * np + p] = s[k];
+  for (int k = 0; k < 6; ++k) stream_st(&P.stress[k * np + p], s[k]);
 }
 
 // Particle-level dN/dx at its xi.
@@ -419,7 +420,7 @@ __global__ __launch_bounds__(kBlock) void k_locate_init(Mesh M, Particles P, Sta
   if (p >= P.np) return;
   double x[D];
 #pragma unroll
-  for (int k = 0; k < D; ++k) x[k] = P.x[k * P.np + p];
+  for (int k = 0; k < D; ++k) x[k] = stream_ld(&P.x[k * P.np + p]);
   const int h = hint ? hint[p] : -1;
   double xi[D];
   const bool in = h >= 0 && (M.iso ? point_in_cell<D, true>(M, h, x, xi)
@@ -449,7 +450,7 @@ __global__ __launch_bounds__(kBlock) void k_locate_step(Mesh M, Particles P, Sta
   if (!valid_cell<D>(M, c)) return;
   double x[D];
 #pragma unroll
-  for (int k = 0; k < D; ++k) x[k] = P.x[k * P.np + p];
+  for (int k = 0; k < D; ++k) x[k] = stream_ld(&P.x[k * P.np + p]);
   locate_one<D, true>(M, P, S, scan_list, (int)p, x, c, -1);
 }
 
@@ -472,7 +473,7 @@ __global__ __launch_bounds__(kBlock) void k_locate_scan(Mesh M, Particles P, Sta
     const int p = scan_list[q];
     double x[D];
 #pragma unroll
-    for (int k = 0; k < D; ++k) x[k] = P.x[k * P.np + p];
+    for (int k = 0; k < D; ++k) x[k] = stream_ld(&P.x[k * P.np + p]);
     int64_t lo = 0, hi = M.ncells;
     if (M.cell_inst) {
       int prev = P.cell[p];
@@ -813,20 +814,20 @@ __global__ __launch_bounds__(CH) void k_p2g(Mesh M, Particles P, Nodes Nd, StepA
       for (int k = 0; k < D; ++k) {
         xi[k] = P.xi[k * np + p];
         st.xi[k][tid] = xi[k];
-        st.v[k][tid] = P.v[k * np + p];
+        st.v[k][tid] = stream_ld(&P.v[k * np + p]);
       }
       st.m[tid] = P.mass[p];
       if constexpr ((WHAT & SC_AFF) != 0) {
 #pragma unroll
         for (int k = 0; k < D * D; ++k) st.C[k][tid] = P.Cmat[k * np + p];
 #pragma unroll
-        for (int k = 0; k < D; ++k) st.xp[k][tid] = P.x[k * np + p];
+        for (int k = 0; k < D; ++k) st.xp[k][tid] = stream_ld(&P.x[k * np + p]);
       }
       if constexpr ((WHAT & SC_F) != 0) {
         double sg[6];
 #pragma unroll
-        for (int k = 0; k < 6; ++k) sg[k] = P.stress[k * np + p];
-        const double vol = P.vol[p];
+        for (int k = 0; k < 6; ++k) sg[k] = stream_ld(&P.stress[k * np + p]);
+        const double vol = stream_ld(&P.vol[p]);
         constexpr int NN = Elem<D>::NN;
         double X[NN][D], Ji[D][D], Bm[D * D];
         load_cell_X<D>(M, key, X);
@@ -912,7 +913,7 @@ __global__ __launch_bounds__(kStressBlock) MPMX_PRED_ATTR void k_stress(Mesh M, 
   if constexpr (LOCATE) {
     double x[D];
 #pragma unroll
-    for (int k = 0; k < D; ++k) x[k] = P.x[k * np + p];
+    for (int k = 0; k < D; ++k) x[k] = stream_ld(&P.x[k * np + p]);
     locate_one<D>(M, P, S, scan_list, (int)p, x, c, -1);
   }
 }
@@ -950,9 +951,9 @@ __global__ __launch_bounds__(kBlock) MPMX_PRED_ATTR void k_g2p(Mesh M, Particles
   double x[D], v0[D], u0[D];
 #pragma unroll
   for (int k = 0; k < D; ++k) {  // loads before stores (see kinematics)
-    v0[k] = P.v[k * np + p];
-    x[k] = P.x[k * np + p];
-    u0[k] = P.u[k * np + p];
+    v0[k] = stream_ld(&P.v[k * np + p]);
+    x[k] = stream_ld(&P.x[k * np + p]);
+    u0[k] = stream_ld(&P.u[k * np + p]);
   }
   const int pvc = P.pvc_mask ? P.pvc_mask[p] : 0;
   const double beta = vu == ASFLIP ? P.beta[p] : 0.;
@@ -971,10 +972,10 @@ __global__ __launch_bounds__(kBlock) MPMX_PRED_ATTR void k_g2p(Mesh M, Particles
       v = sv[k];
     }
     if (pvc & (1 << k)) v = P.pvc_val[k * np + p];  // particle.tcc:1190-1195
-    P.v[k * np + p] = v;
+    stream_st(&P.v[k * np + p], v);
     x[k] = x[k] + dx * A.dt;
-    P.x[k * np + p] = x[k];
-    P.u[k * np + p] = u0[k] + dx * A.dt;
+    stream_st(&P.x[k * np + p], x[k]);
+    stream_st(&P.u[k * np + p], u0[k] + dx * A.dt);
   }
   if constexpr (STRESS) {
     double dndx[NN][D], Ji[D][D];
@@ -1008,7 +1009,7 @@ __global__ __launch_bounds__(kBlock) void k_mapping_matrix(Mesh M, Particles P, 
 #pragma unroll
   for (int k = 0; k < D; ++k) {
     xi[k] = P.xi[k * np + p];
-    xp[k] = P.x[k * np + p];
+    xp[k] = stream_ld(&P.x[k * np + p]);
   }
   double nv[NN][D];
   gather_nodes<D>(M, Nd, c, nv, nullptr);
@@ -1107,7 +1108,7 @@ __global__ __launch_bounds__(kBlock) void k_affine_matrix(Mesh M, Particles P, i
 #pragma unroll
   for (int k = 0; k < D; ++k) {
     xi[k] = P.xi[k * np + p];
-    xp[k] = P.x[k * np + p];
+    xp[k] = stream_ld(&P.x[k * np + p]);
   }
   double N[NN], X[NN][D];
   shapefn<D>(xi, N);
@@ -1171,8 +1172,8 @@ __global__ __launch_bounds__(kPlasticBlock) MPMX_PLASTIC_ATTR void k_plastic(Par
     T.yield_trial = r[PR_YTRIAL * np];
     T.yt = c < ns ? FS_SHEAR : FS_TENSILE;
 #pragma unroll
-    for (int k = 0; k < 6; ++k) s[k] = P.stress[k * np + p];
-    const Material& m = mats[P.mat[p]];
+    for (int k = 0; k < 6; ++k) s[k] = stream_ld(&P.stress[k * np + p]);
+    const Material& m = mats[stream_ld(&P.mat[p])];
     load_sv(P, m, p, sv);
     double trial[6];
     if (P.plastic_trial) {  // stress rates: the trial stress the predictor used
@@ -1187,12 +1188,12 @@ __global__ __launch_bounds__(kPlasticBlock) MPMX_PLASTIC_ATTR void k_plastic(Par
     McIter R;
     mc_first(m, s, trial, ds, sv, T, R);
     mc_iterate(m, s, sv, R, 100);
-    P.sv[SV_EPS * np + p] = sv[SV_EPS];
-    P.sv[SV_THETA * np + p] = sv[SV_THETA];
-    P.sv[SV_RHO * np + p] = sv[SV_RHO];
-    P.sv[SV_PDS * np + p] = sv[SV_PDS];
+    stream_st(&P.sv[SV_EPS * np + p], sv[SV_EPS]);
+    stream_st(&P.sv[SV_THETA * np + p], sv[SV_THETA]);
+    stream_st(&P.sv[SV_RHO * np + p], sv[SV_RHO]);
+    stream_st(&P.sv[SV_PDS * np + p], sv[SV_PDS]);
 #pragma unroll
-    for (int k = 0; k < 6; ++k) P.stress[k * np + p] = s[k];
+    for (int k = 0; k < 6; ++k) stream_st(&P.stress[k * np + p], s[k]);
   }
 }
 
@@ -1236,9 +1237,9 @@ __global__ __launch_bounds__(CH) void k_g2p_scatter(Mesh M, Particles P, Nodes N
       double v0[D], x0[D], u0[D];
 #pragma unroll
       for (int k = 0; k < D; ++k) {  // loads before stores (see kinematics)
-        v0[k] = P.v[k * np + p];
-        x0[k] = P.x[k * np + p];
-        u0[k] = P.u[k * np + p];
+        v0[k] = stream_ld(&P.v[k * np + p]);
+        x0[k] = stream_ld(&P.x[k * np + p]);
+        u0[k] = stream_ld(&P.u[k * np + p]);
       }
       st.m[tid] = P.mass[p];
       const int pvc = P.pvc_mask ? P.pvc_mask[p] : 0;
@@ -1252,9 +1253,9 @@ __global__ __launch_bounds__(CH) void k_g2p_scatter(Mesh M, Particles P, Nodes N
           v = sv[k];
         }
         if (pvc & (1 << k)) v = P.pvc_val[k * np + p];  // particle.tcc:1190-1195
-        P.v[k * np + p] = v;
-        P.x[k * np + p] = x0[k] + sv[k] * A.dt;
-        P.u[k * np + p] = u0[k] + sv[k] * A.dt;
+        stream_st(&P.v[k * np + p], v);
+        stream_st(&P.x[k * np + p], x0[k] + sv[k] * A.dt);
+        stream_st(&P.u[k * np + p], u0[k] + sv[k] * A.dt);
         st.v[k][tid] = v;
         st.xi[k][tid] = xi[k];
       }
@@ -1481,8 +1482,8 @@ __global__ __launch_bounds__(kBlock) void k_gather_prep(Mesh M, Particles P, con
 #pragma unroll
   for (int k = 0; k < D; ++k) xi[k] = P.xi[k * np + p];
 #pragma unroll
-  for (int k = 0; k < 6; ++k) s[k] = P.stress[k * np + p];
-  const double mv = -1. * P.vol[p];
+  for (int k = 0; k < 6; ++k) s[k] = stream_ld(&P.stress[k * np + p]);
+  const double mv = -1. * stream_ld(&P.vol[p]);
   load_cell_X<D>(M, c, X);
   jacobian_inverse<D>(xi, X, Ji);
 #pragma unroll
