@@ -66,9 +66,10 @@ struct AdjParticles {
 };
 
 // Plastic Mohr-Coulomb updates of a backward step, whose constitutive
-// adjoint runs in a separate pass over this compacted list: count at
-// count[0]; per entry the particle id, the apex flag and (SoA, stride cap)
-// the strain increment, replaced by its adjoint.
+// adjoint runs in a separate pass over this compacted list: shear returns
+// from the front (count[0] entries), tensile ones from the back (count[1]
+// entries, slot cap - 1 down); per entry the particle id, the apex flag and
+// (SoA, stride cap) the strain increment, replaced by its adjoint.
 struct AdjList {
   int* count;
   int* ids;

@@ -251,7 +251,19 @@ MPMX_DEV bool material_vjp(const Material& m, const double* s_in, const double* 
 #pragma unroll
       for (int k = 0; k < 6; ++k) corr = fmax(corr, fabs((s_in[k] + dde[k]) - s_out[k]));
       const bool apex = apex_ratio > 0. && corr > apex_ratio * sqrt(2. * ad::mu_j2(s_out));
-      const int slot = atomicAdd(Q.count, 1);
+      // tensile returns from the back of the list, shear ones from the
+      // front, so that a wave of k_ad_mc mostly runs one branch; one atomic
+      // per wave and kind on the counts
+      const bool tensile = yield_new == FS_TENSILE;
+      const unsigned long long tmask = __ballot(tensile);
+      const unsigned long long mine = tensile ? tmask : (__ballot(1) & ~tmask);
+      const int lane = threadIdx.x & 63;
+      const int leader = __ffsll((long long)mine) - 1;
+      int base = 0;
+      if (lane == leader) base = atomicAdd(Q.count + (tensile ? 1 : 0), __popcll(mine));
+      base = __shfl(base, leader, 64);
+      const int k = base + __popcll(mine & ((1ull << lane) - 1ull));
+      const int64_t slot = tensile ? Q.cap - 1 - k : k;
       Q.ids[slot] = (int)p;
       Q.apex[slot] = apex ? 1 : 0;
 #pragma unroll
@@ -593,7 +605,7 @@ __global__ __launch_bounds__(kAdBlock) void k_ad_node_fwd(Mesh M, Nodes Nd, AdjN
                                                           int* qcount) {
   const int64_t n = blockIdx.x * (int64_t)kAdBlock + threadIdx.x;
   const int64_t nn = M.nnodes;
-  if (n == 0) *qcount = 0;
+  if (n == 0) qcount[0] = qcount[1] = 0;
   if (n >= nn) return;
   A.lm[n] = 0.;
   A.lm2[n] = 0.;
@@ -913,8 +925,10 @@ template <int D>
 __global__ __launch_bounds__(kAdMcBlock) MPMX_AD_ATTR void k_ad_mc(Particles P,
                                                                    const Material* mats, Slot s0,
                                                                    AdjParticles L, AdjList Q) {
-  const int64_t slot = blockIdx.x * (int64_t)kAdMcBlock + threadIdx.x;
-  if (slot >= *Q.count) return;
+  const int64_t e = blockIdx.x * (int64_t)kAdMcBlock + threadIdx.x;
+  const int ns = Q.count[0];
+  if (e >= ns + Q.count[1]) return;
+  const int64_t slot = e < ns ? e : Q.cap - 1 - (e - ns);
   const int64_t np = P.np;
   const int p = Q.ids[slot];
   const Material m = mats[stream_ld(&P.mat[p])];
@@ -948,8 +962,10 @@ __global__ __launch_bounds__(kAdBlock) void k_ad_mc_tail(Mesh M, Particles P, Sl
                                                          AdjParticles L, AdjList Q,
                                                          AdjArgs a) {
   constexpr int NN = Elem<D>::NN;
-  const int64_t slot = blockIdx.x * (int64_t)kAdBlock + threadIdx.x;
-  if (slot >= *Q.count) return;
+  const int64_t e = blockIdx.x * (int64_t)kAdBlock + threadIdx.x;
+  const int ns = Q.count[0];
+  if (e >= ns + Q.count[1]) return;
+  const int64_t slot = e < ns ? e : Q.cap - 1 - (e - ns);
   const int64_t np = P.np;
   const int p = Q.ids[slot];
   Shape<D> S;

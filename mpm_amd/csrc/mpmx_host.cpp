@@ -1613,8 +1613,8 @@ void ad_backward_step(mpmx_ctx* c, int j, int64_t t) {
   AdjList Q;
   Q.cap = c->np;
   Q.count = c->d_ad_list.p;
-  Q.ids = c->d_ad_list.p + 1;
-  Q.apex = c->d_ad_list.p + 1 + c->np;
+  Q.ids = c->d_ad_list.p + 2;
+  Q.apex = c->d_ad_list.p + 2 + c->np;
   Q.buf = c->d_ad_list_buf.p;
   if (c->dim == 2)
     launch_ad_backward_step<2>(c->scheme, c->mesh(), c->parts(), c->nodes(), c->d_mats.p, s0, s1,
@@ -2584,7 +2584,7 @@ void adjoint_backward(mpmx_ctx* c, bool dev, const double* dL_dx, const double* 
   if (!c->ad_last_recorded) ad_alloc_slots(c, K);
   c->d_ad_nodes.alloc(ad_node_doubles(c));
   c->d_ad_parts.alloc(ad_part_doubles(c));
-  c->d_ad_list.alloc(1 + 2 * (size_t)np);
+  c->d_ad_list.alloc(2 + 2 * (size_t)np);
   c->d_ad_list_buf.alloc(6 * (size_t)np);
   launch_ad_fill(c->d_ad_parts.p, ad_part_doubles(c), 0., st);
   const AdjParticles L = ad_parts(c);
