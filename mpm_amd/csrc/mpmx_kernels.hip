@@ -1469,6 +1469,8 @@ struct GatherSlots {
 // Before a force gather: each particle's shape function and internal-force
 // term for every node of its cell (Particles::gterm), so that the node
 // gathers read them instead of recomputing the cell's Jacobian per node.
+// Each term is read once, by the gather of its node: streamed like the
+// particle state.
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_gather_prep(Mesh M, Particles P, const Status* S) {
   if (run_stopped(S)) return;
@@ -1499,9 +1501,9 @@ __global__ __launch_bounds__(kBlock) void k_gather_prep(Mesh M, Particles P, con
       f[2] = d[2] * s[2] + d[1] * s[4] + d[0] * s[5];
     }
     double* t = P.gterm + (int64_t)i * (1 + D) * np + p;
-    t[0] = shapefn_i<D>(xi, i);
+    stream_st(&t[0], shapefn_i<D>(xi, i));
 #pragma unroll
-    for (int k = 0; k < D; ++k) t[(1 + k) * np] = f[k] * mv;
+    for (int k = 0; k < D; ++k) stream_st(&t[(1 + k) * np], f[k] * mv);
   }
 }
 
@@ -1519,7 +1521,7 @@ MPMX_DEV void gather_terms(const Mesh& M, const Particles& P, int64_t n, int p, 
   double N;
   const double* t = P.gterm + (int64_t)i * (1 + D) * np + p;
   if constexpr ((WHAT & (SC_F | SC_MOM)) != 0) {
-    N = t[0];
+    N = stream_ld(&t[0]);
   } else {
     double xi[D];
 #pragma unroll
@@ -1551,7 +1553,7 @@ MPMX_DEV void gather_terms(const Mesh& M, const Particles& P, int64_t n, int p, 
 #pragma unroll
     for (int k = 0; k < D; ++k) {
       out[GS::FE + k] = g[k] * m * N;
-      out[GS::FI + k] = t[(1 + k) * np];
+      out[GS::FI + k] = stream_ld(&t[(1 + k) * np]);
     }
   }
 }
